@@ -1,0 +1,155 @@
+/*
+ * pypty_amd — C ABI of the MI355X (gfx950) iterative-ptychography hot path.
+ *
+ * Every pointer is a device pointer owned by the caller unless stated
+ * otherwise; `stream` is a hipStream_t passed as void*.  No function
+ * allocates device memory: scratch space is the caller's `workspace`, whose
+ * size pty_workspace_bytes() reports.  Functions return PTY_OK (0) or a
+ * negative status; pty_status_string() names it.  Complex data are complex64
+ * (interleaved float re, im).  Windows are n x n with n in
+ * {16, 32, 48, 64, 96, 128} for the LDS-resident kernels.
+ *
+ * Data layouts
+ *   object, planar     (S, K, Ny, Nx)   plane = s*K + k
+ *   object, reference  (Ny, Nx, S, K)   the layout objective.py keeps
+ *   probe              (M, n, n)        mode-major
+ *   positions          (P, 2) int32     integer corner (y, x) of each window
+ *   corrections        (P, 2) float32   sub-pixel (y, x) correction
+ *   measured           (P, n, n) float32, DC at (n/2, n/2) (fftshifted)
+ */
+#ifndef PYPTY_AMD_H
+#define PYPTY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTY_OK 0
+#define PTY_ERR_ARG -1        /* bad argument / shape */
+#define PTY_ERR_SIZE -2       /* window size not supported */
+#define PTY_ERR_HIP -3        /* HIP runtime error (launch, event) */
+#define PTY_ERR_BOUNDS -4     /* a window leaves the object */
+#define PTY_ERR_WORKSPACE -5  /* workspace missing or too small */
+
+/* fft modes */
+#define PTY_FFT_FWD 0        /* fft2                        fft.py:114 */
+#define PTY_FFT_INV 1        /* ifft2 (1/n^2)               fft.py:136 */
+#define PTY_FFT_FWD_SHIFT 2  /* fftshift(fft2(x))           fft.py:91  */
+#define PTY_FFT_INV_ISHIFT 3 /* ifft2(ifftshift(x))         fft.py:158 */
+
+/* loss families (objective.py:527-587) */
+#define PTY_LOSS_LSQ_SQRT 0
+#define PTY_LOSS_ML 1
+
+/* object layouts */
+#define PTY_LAYOUT_REFERENCE 0
+#define PTY_LAYOUT_PLANAR 1
+
+const char* pty_status_string(int status);
+int pty_version(void);
+/* 1 if the LDS-resident kernels support window size n, else 0 */
+int pty_supported_window(int n);
+
+/* Batched in-place 2D FFT of `batch` contiguous n x n complex64 tiles.
+ * Replaces fft.fft2 / ifft2 / shift_fft2 / ifft2_ishift (pypty/fft.py:91-178)
+ * for windows; the Python mirror moves the transformed axes innermost. */
+int pty_fft2_c64(void* data, int64_t batch, int n, int mode, void* stream);
+
+/* x <- ifft2(mask * fft2(x)) per tile, mask: n x n float32, natural
+ * (unshifted) frequency order.  The window band-limit of
+ * objective.py:439-441 and multislice_core.py:188-190. */
+int pty_bandlimit_c64(void* data, int64_t batch, int n, const float* mask, void* stream);
+
+/* full[p, y0+i, x0+j] += patches[b, p, i, j] for every window b, plane p
+ * (overlapping windows accumulate).  full: planar (planes, ny, nx) or
+ * reference (ny, nx, planes) per `layout`; patches: (batch, planes, n, n);
+ * corners (batch, 2).
+ * Replaces multislice_core.scatteradd (pypty/multislice_core.py:206-230). */
+int pty_scatteradd_c64(void* full, int ny, int nx, int planes, int layout, const void* patches,
+                       int batch, int n, const int32_t* corners, void* stream);
+
+/* Geometry, physics and switches of one loss/gradient evaluation: the
+ * arguments of objective.loss_and_direction (pypty/objective.py:47) that the
+ * far-field multislice path reads. */
+typedef struct pty_problem {
+  int n;             /* window (probe) size */
+  int n_slices;      /* S */
+  int n_obj_modes;   /* K */
+  int n_probe_modes; /* M */
+  int obj_ny, obj_nx;
+  int n_positions;   /* P: rows of positions / corrections / measured */
+  int compute_batch; /* windows per launch */
+  int algorithm;     /* PTY_LOSS_* */
+  int obj_layout;    /* PTY_LAYOUT_* of obj and obj_grad */
+  int step_obj, step_probe, step_pos; /* which gradients to produce */
+  int shift_active;  /* sub-pixel probe shift (objective.py:352) */
+  float loss_weight;
+  float slice_distance;  /* dz, Angstrom (single distance) */
+  float wavelength;      /* Angstrom */
+  float pixel_size_x, pixel_size_y; /* Angstrom */
+  float tan_x, tan_y;    /* in-specimen tilt of the propagator */
+  float damping_cutoff;  /* band-mask cutoff (utils.py:153) */
+  float edge_multiplier; /* first-slice propagator power (>1 active) */
+} pty_problem;
+
+typedef struct pty_buffers {
+  const void* obj;           /* object, layout per obj_layout */
+  const void* probe;         /* (M, n, n) */
+  const int32_t* positions;  /* (P, 2) */
+  const float* corrections;  /* (P, 2) */
+  const float* measured;     /* (P, n, n) */
+  const int32_t* sequence;   /* (n_seq) position indices, each at most once */
+  int n_seq;
+  double* loss_sse;          /* out [2]: loss (x loss_weight), sse */
+  void* obj_grad;            /* out, like obj (written if step_obj) */
+  void* probe_grad;          /* out (M, n, n) (written if step_probe) */
+  float* pos_grad;           /* out (P, 2): rows of `sequence` assigned (if step_pos) */
+  float* sim_patterns;       /* optional out (P, n, n) modelled intensity, or NULL */
+  void* workspace;
+  size_t workspace_bytes;
+} pty_buffers;
+
+int pty_workspace_bytes(const pty_problem* p, size_t* bytes);
+
+/* One full forward + loss + adjoint pass over `sequence`:
+ * replaces objective.loss_and_direction (pypty/objective.py:47-967) for
+ * far-field multislice, lsq_sqrt / ml losses, object / probe / position
+ * gradients, single slice distance, no tilts. */
+int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* stream);
+
+/* Multislice forward on pre-cut windows (pypty/multislice_core.py:9-82).
+ * probe_b (B, M, n, n); obj_b (B, S, K, n, n) band-limited windows;
+ * propagator (n, n) natural frequency order, band mask included (the
+ * reference's master_propagator_phase_space); raised to edge_multiplier for
+ * the first slice when that is > 1.  Uses n, n_slices, n_obj_modes,
+ * n_probe_modes, edge_multiplier of *p.
+ * stash out (B, M, K, S, n, n) incident waves (not conjugated);
+ * exit_wave out (B, M, K, n, n).  tables: scratch of pty_tables_bytes(n). */
+int pty_tables_bytes(int n, size_t* bytes);
+int pty_ms_forward(const pty_problem* p, int batch, const void* probe_b, const void* obj_b,
+                   const void* propagator, void* stash, void* exit_wave, void* tables, void* stream);
+
+/* Multislice adjoint (pypty/multislice_core.py:85-200).  g (B, M, K, n, n)
+ * holds dL/d(exit)* on entry and dL/d(incident probe)* per (m, k) on exit;
+ * mask (n, n) natural order (mask_clean); grad_b out (B, S, K, n, n):
+ * band-limited sum over probe modes of the per-slice object gradient
+ * (before the scatter-add). */
+int pty_ms_adjoint(const pty_problem* p, int batch, void* g, const void* stash, const void* obj_b,
+                   const void* propagator, const float* mask, void* grad_b, void* tables,
+                   void* stream);
+
+/* Kernel timing (HIP events around every launch of the library). */
+void pty_timing_enable(int on);
+/* Writes up to `cap` entries: names (static strings), total ms, launches;
+ * returns the number of kernel kinds; synchronises pending events. */
+int pty_timing_read(const char** names, double* total_ms, int64_t* launches, int cap);
+void pty_timing_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYPTY_AMD_H */

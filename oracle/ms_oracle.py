@@ -1,0 +1,299 @@
+"""numpy restatement of PyPty's far-field multislice path (test oracle only).
+
+Arrays use the reference's own layouts:
+  probe (N, N, M), object (Ny, Nx, S, K), positions / corrections (P, 2) as (y, x),
+  measured (P, N, N) with the zero frequency at (N/2, N/2),
+  per-window probe (B, N, N, M), windows (B, N, N, S, K),
+  wavefield stash (B, N, N, S, M, K), exit wave (B, N, N, M, K).
+``cdtype`` selects complex64 (the reference's "single" path, same op order)
+or complex128 (the FP64 oracle).  Supported subset: far field, single slice
+distance, no tilts, losses lsq_sqrt / ml, object / probe / position
+gradients.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.fft as sf
+
+LSQ_SQRT, ML = "lsq_sqrt", "ml"
+
+
+def _fdt(cdtype):
+    return np.float32 if np.dtype(cdtype) == np.complex64 else np.float64
+
+
+# ----------------------------------------------------------------- FFTs
+def fft2(a, axes=(0, 1)):
+    """pypty/fft.py:114-134: unnormalised forward 2D DFT."""
+    return sf.fft2(a, axes=axes)
+
+
+def ifft2(a, axes=(0, 1)):
+    """pypty/fft.py:136-156: inverse 2D DFT with 1/N^2."""
+    return sf.ifft2(a, axes=axes)
+
+
+def shift_fft2(a, axes=(0, 1)):
+    """pypty/fft.py:91-112: fftshift(fft2(a))."""
+    return sf.fftshift(sf.fft2(a, axes=axes), axes=axes)
+
+
+def ifft2_ishift(a, axes=(0, 1)):
+    """pypty/fft.py:158-178: ifft2(ifftshift(a))."""
+    return sf.ifft2(sf.ifftshift(a, axes=axes), axes=axes)
+
+
+def dft2_bruteforce(a):
+    """Direct O(N^4) evaluation of the 2D DFT over axes (0, 1), in float64/complex128."""
+    a = np.asarray(a, dtype=np.complex128)
+    ny, nx = a.shape[:2]
+    wy = np.exp(-2j * np.pi * np.outer(np.arange(ny), np.arange(ny)) / ny)
+    wx = np.exp(-2j * np.pi * np.outer(np.arange(nx), np.arange(nx)) / nx)
+    return np.einsum("ky,lx,yx...->kl...", wy, wx, a)
+
+
+# ---------------------------------------------------------- grids, masks
+def create_spatial_frequencies(px, py, n, cutoff, rolloff=0.0, fdtype=np.float32):
+    """pypty/utils.py:153-199 (rolloff forced to 0 on the loss path, iterative.py:483)."""
+    qx, qy = np.meshgrid(np.fft.fftfreq(n, px), np.fft.fftfreq(n, py), indexing="xy")
+    qx, qy = qx.astype(fdtype), qy.astype(fdtype)
+    q2 = qx ** 2 + qy ** 2
+    if cutoff is None:
+        cutoff = 2
+    if cutoff < 0:
+        cutoff = 2 / 3
+    max_x = np.max(qx)
+    r_max = (max_x * fdtype(cutoff)) ** 2
+    mask = (q2 <= r_max).astype(fdtype)
+    if rolloff != 0:
+        r0 = (max_x * (cutoff - rolloff)) ** 2
+        sel = q2 > r0
+        mask[sel] *= (0.5 * (1 + np.cos(3.141592654 * (q2 - r0) / (r_max - r0))))[sel]
+    mask = mask.astype(fdtype)
+    return q2, qx, qy, sf.fftshift(mask), mask
+
+
+def propagator(dz, wavelength, q2, qx, qy, mask, tx=0.0, ty=0.0, cdtype=np.complex64):
+    """Fresnel propagator of one slice, objective.py:337-345 (and multislice_core.py:70-72)."""
+    fd = _fdt(cdtype)
+    dz, wavelength, tx, ty = fd(dz), fd(wavelength), fd(tx), fd(ty)
+    h = np.exp(-3.141592654j * dz * (wavelength * q2 + 2 * (qx * tx + qy * ty))) * mask
+    return h.astype(cdtype)
+
+
+def shift_masks(n, mask, cdtype=np.complex64):
+    """Sub-pixel shift kernels Ky, Kx = -2*pi*i*fftfreq(n)*mask, objective.py:308-313."""
+    sx, sy = np.meshgrid(np.fft.fftfreq(n), np.fft.fftfreq(n), indexing="xy")
+    sx = (-6.2831855j * sx).astype(cdtype) * mask
+    sy = (-6.2831855j * sy).astype(cdtype) * mask
+    return sy.astype(cdtype), sx.astype(cdtype)
+
+
+def fourier_clean(array, cutoff=0.66, rolloff=0.0, fdtype=np.float32):
+    """pypty/utils.py:102-150: band-limit over axes (0, 1) in fftshifted normalised frequency."""
+    shape = array.shape
+    x = sf.fftshift(np.fft.fftfreq(shape[1]))
+    y = sf.fftshift(np.fft.fftfreq(shape[0]))
+    x, y = np.meshgrid(x, y, indexing="xy")
+    r = x ** 2 + y ** 2
+    max_r = 0.5 * cutoff
+    mask = (r <= max_r ** 2).astype(fdtype)
+    if rolloff != 0:
+        r0 = (0.5 * (cutoff - rolloff)) ** 2
+        sel = r > r0
+        mask[sel] *= (0.5 * (1 + np.cos(3.141592654 * (r - r0) / (max_r ** 2 - r0))))[sel]
+    ff = shift_fft2(array, axes=(0, 1))
+    ff = ff * mask.reshape(mask.shape + (1,) * (len(shape) - 2))
+    return ifft2_ishift(ff, axes=(0, 1))
+
+
+# ------------------------------------------------------------ multislice
+def multislice(probe_b, obj_b, H, edge=1.0):
+    """Split-step forward, pypty/multislice_core.py:9-82 (single distance, no tilts).
+
+    probe_b (B, N, N, M); obj_b (B, N, N, S, K); H (N, N) propagator.
+    Returns the conjugated stash (B, N, N, S, M, K) and the exit wave (B, N, N, M, K).
+    """
+    B, N, _, M = probe_b.shape
+    S, K = obj_b.shape[3], obj_b.shape[4]
+    Hb = H[None, :, :, None, None]
+    waves = np.zeros((B, N, N, S, M, K), dtype=probe_b.dtype)
+    wave = np.repeat(probe_b[:, :, :, :, None], K, axis=-1)
+    for s in range(S):
+        waves[:, :, :, s] = wave
+        wave = wave * obj_b[:, :, :, s:s + 1, :]
+        if s < S - 1:
+            prop = Hb ** edge if (edge > 1 and s == 0) else Hb
+            wave = ifft2(fft2(wave, axes=(1, 2)) * prop, axes=(1, 2))
+    return np.conj(waves), wave
+
+
+def multislice_grads(g, waves, obj_b, H, mask, edge=1.0, step_obj=True):
+    """Reverse-mode adjoint, pypty/multislice_core.py:85-200 (no tilts).
+
+    g (B, N, N, M, K) = dL/d(exit)*; waves = conjugated stash from multislice().
+    Returns (band-limited window gradients (B, N, N, S, K) or None,
+             probe-side gradient summed over object modes (B, N, N, M)).
+    """
+    waves = waves.copy()
+    objc = np.conj(obj_b)
+    S = waves.shape[3]
+    Hc = np.conj(H)[None, :, :, None, None]
+    for s in range(S - 1, -1, -1):
+        if step_obj:
+            waves[:, :, :, s] *= g
+        g = g * objc[:, :, :, s:s + 1, :]
+        if s > 0:
+            prop = Hc ** edge if (edge > 1 and s == 1) else Hc
+            g = ifft2(fft2(g, axes=(1, 2)) * prop, axes=(1, 2))
+    grad = None
+    if step_obj:
+        G = waves[:, :, :, :, 0, :] if waves.shape[4] == 1 else np.sum(waves, axis=4)
+        grad = ifft2(fft2(G, axes=(1, 2)) * mask[None, :, :, None, None], axes=(1, 2))
+    interm = g[..., 0] if g.shape[-1] == 1 else np.sum(g, axis=-1)
+    return grad, interm
+
+
+def scatteradd(full, my, mx, chop):
+    """pypty/multislice_core.py:206-230: full[my, mx] += chop with repeated indices accumulating."""
+    np.add.at(full, (my, mx), chop)
+
+
+# ------------------------------------------------------ loss + gradients
+def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *, dz, wavelength,
+                       pixel_size_x, pixel_size_y, algorithm=LSQ_SQRT, loss_weight=1.0, cutoff=2 / 3,
+                       edge=1.0, step_obj=True, step_probe=True, step_pos=False, compute_batch=64,
+                       cdtype=np.complex64, return_patterns=False):
+    """objective.loss_and_direction restricted to the far-field multislice path.
+
+    Follows pypty/objective.py:218-361 (grids, propagator, shift kernels),
+    :393-703 (chunk loop: window cut and band-limit, probe shift, multislice,
+    far field, loss, adjoint, probe shift adjoint, position and probe
+    gradients), :742 (loss x loss_weight), :925 (NaN check) and :927-933
+    (probe-gradient band-limit).
+    Returns loss, sse, obj_grad (Ny, Nx, S, K), probe_grad (N, N, M), pos_grad (P, 2)
+    [, modelled intensity (P, N, N) if return_patterns].
+    """
+    fd = _fdt(cdtype)
+    obj = np.asarray(obj, dtype=cdtype)
+    probe = np.asarray(probe, dtype=cdtype)
+    corrections = np.asarray(corrections, dtype=fd)
+    measured = np.asarray(measured, dtype=fd)
+    N, M = probe.shape[0], probe.shape[2]
+    S, K = obj.shape[2], obj.shape[3]
+    q2, qx, qy, mask_s, mask = create_spatial_frequencies(pixel_size_x, pixel_size_y, N, cutoff, 0.0, fd)
+    H = propagator(dz, wavelength, q2, qx, qy, mask, cdtype=cdtype)
+    shy, shx = shift_masks(N, mask, cdtype)
+    shift_flag = bool(np.sum(corrections.astype(np.float64) ** 2) != 0.0) or bool(step_pos)
+    runx, runy = np.meshgrid(np.arange(N), np.arange(N), indexing="xy")
+    coherent = (M == 1 and K == 1)
+    obj_grad = np.zeros_like(obj)
+    probe_grad = np.zeros((N, N, M), dtype=cdtype)
+    pos_grad = np.zeros_like(corrections)
+    patterns = np.zeros(measured.shape, dtype=fd) if return_patterns else None
+    FP = fft2(probe[None], axes=(1, 2))
+    loss, sse = fd(0), fd(0)
+    seq = np.asarray(sequence)
+    for i in range(0, len(seq), compute_batch):
+        tcs = seq[i:i + compute_batch]
+        meas = measured[tcs]
+        my = positions[tcs, 0][:, None, None] + runy[None]
+        mx = positions[tcs, 1][:, None, None] + runx[None]
+        chop = obj[my, mx, :, :]
+        chop = ifft2(fft2(chop, axes=(1, 2)) * mask[None, :, :, None, None], axes=(1, 2))
+        this_probe = probe[None]
+        E = None
+        if shift_flag:
+            corr = corrections[tcs]
+            E = np.exp(shx[None] * corr[:, 1, None, None] + shy[None] * corr[:, 0, None, None]) * mask[None]
+            this_probe = ifft2(FP * E[..., None], axes=(1, 2))
+        if this_probe.shape[0] == 1 and len(tcs) > 1:
+            this_probe = np.repeat(this_probe, len(tcs), axis=0)
+        waves, exitw = multislice(this_probe, chop, H, edge)
+        Psi = shift_fft2(exitw, axes=(1, 2)) * mask_s[None, :, :, None, None]
+        I = np.abs(Psi) ** 2
+        I = I[:, :, :, 0, 0] if coherent else np.mean(I, axis=(-1, -2))
+        if return_patterns:
+            patterns[tcs] = I
+        if algorithm == LSQ_SQRT:
+            sse = sse + np.sum((meas - I) ** 2)
+            Is, Ds = np.sqrt(I), np.sqrt(meas)
+            dI = Is - Ds
+            loss = loss + np.sum(dI ** 2)
+            Is[Is == 0] = 1
+            dI = dI / Is
+        elif algorithm == ML:
+            dI = I - meas
+            sse = sse + np.sum(dI ** 2)
+            zero = I == 0
+            loss = loss + np.sum(I - meas * np.log(I + zero))
+            I = I.copy()
+            I[zero] = 1
+            dI = dI / I
+        else:
+            raise ValueError(f"unsupported loss {algorithm}")
+        if not coherent:
+            dI = dI / (M * K)
+        dWave = dI[:, :, :, None, None] * Psi * (loss_weight * N * N)
+        g = ifft2_ishift(dWave, axes=(1, 2))
+        G, interm = multislice_grads(g, waves, chop, H, mask, edge, step_obj)
+        if step_obj:
+            scatteradd(obj_grad, my, mx, G)
+        if shift_flag:
+            Fg = fft2(interm, axes=(1, 2)) * np.conj(E)[..., None]
+            interm = ifft2(Fg, axes=(1, 2))
+            if step_pos:
+                smg = np.conj(FP) * Fg
+                sh = -2 / (N * N)
+                yx = np.stack((shy, shx), axis=0)  # (2, N, N)
+                pos_grad[tcs] = sh * np.sum(np.real(yx[None, :, :, :, None] * smg[:, None]), axis=(2, 3, 4))
+        if step_probe:
+            probe_grad += np.sum(interm, axis=0)
+    loss = loss * loss_weight
+    if loss != loss:
+        raise ValueError("loss is NaN")
+    if step_probe:
+        probe_grad = ifft2(fft2(probe_grad, axes=(0, 1)) * mask[:, :, None], axes=(0, 1))
+    out = (float(loss), float(sse), obj_grad, probe_grad, pos_grad)
+    if return_patterns:
+        out = out + (patterns,)
+    return out
+
+
+# --------------------------------------------------------------- L-BFGS
+def wolfe_1(value, new_value, d_value, step, wolfe_c1=0.5):
+    """pypty/utils.py:326-348: Armijo condition."""
+    return new_value <= value + wolfe_c1 * step * d_value
+
+
+def wolfe_2(d_value, new_d_value, wolfe_c2=0.9):
+    """pypty/utils.py:350-368: curvature condition."""
+    return -1 * new_d_value <= -1 * d_value * wolfe_c2
+
+
+def cdot(a, b):
+    """Real inner product used by bfgs_update: 2*Re<a, b> for complex blocks, <a, b> for real ones
+    (iterative.py:597, 652, 683-690)."""
+    if np.iscomplexobj(a) or np.iscomplexobj(b):
+        return 2.0 * float(np.sum(np.real(a * np.conj(b))))
+    return float(np.sum(a * b))
+
+
+def two_loop(grads, hist_s, hist_y, rhos, gammas):
+    """L-BFGS two-loop recursion over a list of variable blocks, iterative.py:573-681.
+
+    grads: list of gradient blocks; hist_s / hist_y: lists (oldest first) of lists of blocks;
+    gammas: per-block initial scaling.  Returns the search direction blocks (already negated).
+    """
+    q = [np.array(g, copy=True) for g in grads]
+    alphas = []
+    for i in range(len(rhos) - 1, -1, -1):
+        a = rhos[i] * sum(cdot(s, u) for s, u in zip(hist_s[i], q))
+        alphas.append(a)
+        q = [u - a * y for u, y in zip(q, hist_y[i])]
+    alphas = alphas[::-1]
+    q = [u * g if (g != 0 and g == g) else u for u, g in zip(q, gammas)]
+    for i in range(len(rhos)):
+        beta = alphas[i] - rhos[i] * sum(cdot(y, u) for y, u in zip(hist_y[i], q))
+        q = [u + beta * s for u, s in zip(q, hist_s[i])]
+    return [-u for u in q]

@@ -1,0 +1,1238 @@
+// MI355X (gfx950) kernels of the far-field multislice ptychography hot path
+// and the C ABI that launches them.
+//
+// One workgroup owns one n x n window held in LDS (see pty_fft.hpp).  The
+// per-batch pipeline of pty_loss_and_direction is
+//
+//   k_patch      (b, s, k)  cut object window, band-limit           2 FFT
+//   k_probe_shift(b, m)     sub-pixel shift of the probe             1 FFT
+//   k_forward    (b, m, k)  S slices: stash, transmit, propagate,    2(S-1)+1 FFT
+//                           far field (spectrum kept in position order)
+//   k_loss       (b)        mode-mean intensity, loss/sse (FP64), dL/dI
+//   k_adjoint    (b, m, k)  far-field adjoint, S slices backwards,   1+2(S-1) FFT
+//                           per-slice object gradient summed over modes
+//   k_objgrad    (b, s, k)  band-limit the window gradient,          2 FFT
+//                           scatter-add into the object gradient
+//   k_probegrad  (g, m)     probe-shift adjoint, position gradient   1 FFT
+//                           (FP64 sums), probe gradient in the spectrum
+//
+// Workgroups of the same window b but different modes are placed on the same
+// XCD (blockIdx % 8 equal) so that they share the window's object patch and
+// gradient accumulator in that XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "pty_fft.hpp"
+#include "pypty_amd.h"
+
+using namespace pty;
+
+// ------------------------------------------------------------ twiddles
+template <int N>
+struct TwIL {
+  float v[2 * N];
+};
+template <int N>
+constexpr TwIL<N> make_twil() {
+  TwIL<N> t{};
+  for (int k = 0; k < N; ++k) {
+    int kk = (2 * k > N) ? k - N : k;
+    double a = -2.0 * kPi * double(kk) / double(N);
+    t.v[2 * k] = float(cx_cos(a));
+    t.v[2 * k + 1] = float(cx_sin(a));
+  }
+  return t;
+}
+template <int N>
+__device__ __align__(16) TwIL<N> g_tw = make_twil<N>();
+
+template <int N>
+__device__ __forceinline__ const float2* twiddles() {
+  return reinterpret_cast<const float2*>(g_tw<N>.v);
+}
+
+// ------------------------------------------------------------- helpers
+// numpy.fft.fftfreq(n, d)[f] rounded to float32 (computed in double as numpy does)
+__device__ __forceinline__ float fftfreq_f(int f, int n, float d) {
+  int k = (2 * f < n) ? f : f - n;
+  double val = 1.0 / (double(n) * double(d));
+  return float(double(k) * val);
+}
+
+__device__ __forceinline__ void xcd_decode(int bid, int mk_count, int& b, int& mk) {
+  const int xcd = bid & 7, slot = bid >> 3;
+  b = (slot / mk_count) * 8 + xcd;
+  mk = slot % mk_count;
+}
+static inline int xcd_grid(int B, int mk_count) { return ((B + 7) / 8) * 8 * mk_count; }
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+  // wave64 reduction with shuffles, then across waves through LDS
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) s += red[i];
+  }
+  return s;  // valid in thread 0
+}
+
+__device__ __forceinline__ void atomic_add_c(float2* p, float2 v) {
+  unsafeAtomicAdd(&p->x, v.x);
+  unsafeAtomicAdd(&p->y, v.y);
+}
+
+// ------------------------------------------------------- generic tiles
+enum { MODE_FWD = 0, MODE_INV = 1, MODE_FWD_SHIFT = 2, MODE_INV_ISHIFT = 3, MODE_BAND = 4 };
+
+template <int N, int MODE>
+__global__ __launch_bounds__(Tile<N>::NT) void k_fft2_tiles(float2* __restrict__ data,
+                                                            const float* __restrict__ mask_nat) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const float2* tw = twiddles<N>();
+  float2* g = data + size_t(blockIdx.x) * N * N;
+  const int tid = threadIdx.x;
+  if (MODE == MODE_INV || MODE == MODE_INV_ISHIFT) {
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT, y = e / N, x = e % N;
+      const int fy = (MODE == MODE_INV_ISHIFT) ? (y + N / 2) % N : y;
+      const int fx = (MODE == MODE_INV_ISHIFT) ? (x + N / 2) % N : x;
+      t[T::lds(T::freq2pos(fy), T::freq2pos(fx))] = g[e];
+    }
+    __syncthreads();
+    tile_fft_inv<N>(t, tw);
+    const float s = 1.0f / float(N * N);
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      g[e] = cscale(t[T::lds(e / N, e % N)], s);
+    }
+    return;
+  }
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    t[T::lds(e / N, e % N)] = g[e];
+  }
+  __syncthreads();
+  tile_fft_fwd<N>(t, tw);
+  if (MODE == MODE_BAND) {
+    const float s = 1.0f / float(N * N);
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT, py = e / N, px = e % N;
+      const float m = mask_nat[T::pos2freq(py) * N + T::pos2freq(px)];
+      t[T::lds(py, px)] = cscale(t[T::lds(py, px)], m * s);
+    }
+    __syncthreads();
+    tile_fft_inv<N>(t, tw);
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      g[e] = t[T::lds(e / N, e % N)];
+    }
+    return;
+  }
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT, y = e / N, x = e % N;
+    const int fy = (MODE == MODE_FWD_SHIFT) ? (y + N / 2) % N : y;
+    const int fx = (MODE == MODE_FWD_SHIFT) ? (x + N / 2) % N : x;
+    g[e] = t[T::lds(T::freq2pos(fy), T::freq2pos(fx))];
+  }
+}
+
+// --------------------------------------------------------------- tables
+// Propagator H = exp(-i*pi*dz*(lambda*q^2 + 2*(qx*tx + qy*ty))) * mask
+// (objective.py:337-345), its first-slice power H^e (multislice_core.py:73)
+// and the band mask q^2 <= (max(qx)*cutoff)^2 (utils.py:153-199), all in
+// tile position order.  H and H^e carry the 1/n^2 of the inverse FFT.
+struct TableArgs {
+  float2* H;
+  float2* H0;
+  float* mask;
+  float dz, lambda, px, py, tx, ty, cutoff, edge;
+};
+// float32 products/sums without contraction, in numpy's order
+__device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
+
+template <int N>
+__global__ void k_tables(TableArgs a) {
+  using T = Tile<N>;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * N) return;
+  const int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
+  const float qx = fftfreq_f(fx, N, a.px), qy = fftfreq_f(fy, N, a.py);
+  const float q2 = fadd(fmul(qx, qx), fmul(qy, qy));
+  const float max_x = fftfreq_f(N / 2 - 1, N, a.px);
+  float c = a.cutoff;
+  if (c < 0.f) c = float(2.0 / 3.0);
+  const float rm = fmul(max_x, c);
+  const float r_max = fmul(rm, rm);
+  const float m = (q2 <= r_max) ? 1.f : 0.f;
+  const float k = fmul(-3.141592654f, a.dz);
+  const float r = fadd(fmul(a.lambda, q2), fmul(2.f, fadd(fmul(qx, a.tx), fmul(qy, a.ty))));
+  const float ang = fmul(k, r);
+  float sn, cs;
+  sincosf(ang, &sn, &cs);
+  const float s = 1.0f / float(N * N);
+  a.H[e] = make_float2(cs * m * s, sn * m * s);
+  if (a.edge > 1.f) {
+    // complex power through the principal branch, as numpy's z**e
+    const float th = atan2f(sn * m, cs * m);
+    const float mag = (m > 0.f) ? 1.f : 0.f;
+    float s0, c0;
+    sincosf(a.edge * th, &s0, &c0);
+    a.H0[e] = make_float2(c0 * mag * m * s, s0 * mag * m * s);
+  } else {
+    a.H0[e] = a.H[e];
+  }
+  a.mask[e] = m;
+}
+
+// Position-order tables from a caller-supplied propagator (natural order,
+// mask included, as multislice() receives master_propagator_phase_space) and
+// band mask; H^e through the principal branch when edge > 1.
+template <int N>
+__global__ void k_tables_natural(const float2* __restrict__ Hn, const float* __restrict__ mn, float edge,
+                                 float2* __restrict__ H, float2* __restrict__ H0, float* __restrict__ mask) {
+  using T = Tile<N>;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * N) return;
+  const int f = T::pos2freq(e / N) * N + T::pos2freq(e % N);
+  const float s = 1.0f / float(N * N);
+  const float2 h = Hn[f];
+  H[e] = cscale(h, s);
+  if (edge > 1.f) {
+    const float mag = sqrtf(cabs2(h));
+    const float th = atan2f(h.y, h.x);
+    float s0, c0;
+    sincosf(edge * th, &s0, &c0);
+    const float me = (mag > 0.f) ? powf(mag, edge) : 0.f;
+    H0[e] = make_float2(c0 * me * s, s0 * me * s);
+  } else {
+    H0[e] = H[e];
+  }
+  mask[e] = mn ? mn[f] : 1.f;
+}
+
+// ------------------------------------------------------------- layouts
+// reference (Ny, Nx, P) <-> planar (P, Ny, Nx), P = S*K
+__global__ void k_ref_to_planar(const float2* __restrict__ src, float2* __restrict__ dst, int ny, int nx,
+                                int planes) {
+  const size_t total = size_t(ny) * nx * planes;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+    const size_t pix = i / planes;
+    const int p = int(i % planes);
+    dst[size_t(p) * ny * nx + pix] = src[i];
+  }
+}
+__global__ void k_planar_to_ref(const float2* __restrict__ src, float2* __restrict__ dst, int ny, int nx,
+                                int planes) {
+  const size_t total = size_t(ny) * nx * planes;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+    const size_t pix = i / planes;
+    const int p = int(i % planes);
+    dst[i] = src[size_t(p) * ny * nx + pix];
+  }
+}
+__global__ void k_zero(float* __restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.f;
+}
+
+// ------------------------------------------------------- batch pipeline
+// A window whose corner or sequence index is invalid is skipped (reads as
+// zero, contributes nothing): the kernels never touch memory outside the
+// object, the data or the position arrays whatever the inputs hold.
+__device__ __forceinline__ bool window_inside(int y0, int x0, int n, int ny, int nx) {
+  return y0 >= 0 && x0 >= 0 && y0 + n <= ny && x0 + n <= nx;
+}
+__device__ __forceinline__ int seq_index(const int* seq, int slot, int P) {
+  const int idx = seq[slot];
+  return (idx >= 0 && idx < P) ? idx : -1;
+}
+
+struct BatchArgs {
+  // geometry
+  int B, S, K, M, ny, nx, b0, P;
+  int algorithm, step_obj, step_probe, step_pos, shift, far_field, write_stash, scatter;
+  float loss_weight;
+  // inputs
+  const float2* obj;        // planar (S*K, ny, nx)
+  const float2* probe;      // (M, N, N) natural
+  const float2* fprobe;     // (M, N, N) spectrum, position order, unnormalised
+  const int* pos;           // (P, 2)
+  const float* corr;        // (P, 2)
+  const float* meas;        // (P, N, N) shifted natural
+  const int* seq;           // sequence
+  const float2* H;          // position order, /n^2
+  const float2* H0;
+  const float* mask;        // position order
+  // per-batch buffers
+  float2* Ob;               // (B, S, K, N, N) band-limited windows
+  float2* Pp;               // (B, M, N, N) shifted probes
+  float2* stash;            // (B, M, K, S, N, N)
+  float2* Psi;              // (B, M, K, N, N) far field, position order (or exit wave)
+  float* coef;              // (B, N, N) position order
+  float2* Gbuf;             // (B, S, K, N, N) window gradients
+  float2* gP;               // (B, M, K, N, N)
+  float2* probe_part;       // (G, M, N, N)
+  double* pos_part;         // (B, M, 2)
+  double* loss_part;        // (n_seq, 2)
+  float* sim;               // (P, N, N) or null
+  // outputs
+  float2* obj_grad;         // planar
+  float* pos_grad;          // (P, 2)
+  int G;                    // probe-gradient groups
+};
+
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_patch(BatchArgs a) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const int planes = a.S * a.K;
+  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
+  const bool inside = window_inside(y0, x0, N, a.ny, a.nx);
+  const float2* src = a.obj + size_t(plane) * a.ny * a.nx;
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT, y = e / N, x = e % N;
+    t[T::lds(y, x)] = inside ? src[size_t(y0 + y) * a.nx + (x0 + x)] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const float2* tw = twiddles<N>();
+  tile_fft_fwd<N>(t, tw);
+  const float s = 1.0f / float(N * N);
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT, py = e / N, px = e % N;
+    t[T::lds(py, px)] = cscale(t[T::lds(py, px)], a.mask[e] * s);
+  }
+  __syncthreads();
+  tile_fft_inv<N>(t, tw);
+  float2* dst = a.Ob + (size_t(b) * planes + plane) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    dst[e] = t[T::lds(e / N, e % N)];
+  }
+}
+
+// spectrum of each probe mode, position order, unnormalised
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_fprobe(const float2* __restrict__ probe, float2* __restrict__ fprobe) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const float2* src = probe + size_t(m) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    t[T::lds(e / N, e % N)] = src[e];
+  }
+  __syncthreads();
+  tile_fft_fwd<N>(t, twiddles<N>());
+  float2* dst = fprobe + size_t(m) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    dst[e] = t[T::lds(e / N, e % N)];
+  }
+}
+
+// shift kernel E = exp(Kx*dx + Ky*dy) * mask with K = -2*pi*i*fftfreq * mask
+// (objective.py:309-313, 473): returns cos/sin of the phase, times the mask
+template <int N>
+__device__ __forceinline__ float2 shift_kernel(int py, int px, float dy, float dx, float m) {
+  using T = Tile<N>;
+  const float kx = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(px), N, 1.f)), m);
+  const float ky = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(py), N, 1.f)), m);
+  const float ph = fadd(fmul(kx, dx), fmul(ky, dy));
+  float sn, cs;
+  sincosf(ph, &sn, &cs);
+  return make_float2(cs * m, sn * m);
+}
+
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_probe_shift(BatchArgs a) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+  const float2* src = a.fprobe + size_t(m) * N * N;
+  const float s = 1.0f / float(N * N);
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT, py = e / N, px = e % N;
+    const float2 E = shift_kernel<N>(py, px, dy, dx, a.mask[e]);
+    t[T::lds(py, px)] = cscale(cmul(src[e], E), s);
+  }
+  __syncthreads();
+  tile_fft_inv<N>(t, twiddles<N>());
+  float2* dst = a.Pp + (size_t(b) * a.M + m) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    dst[e] = t[T::lds(e / N, e % N)];
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_forward(BatchArgs a) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int m = mk / a.K, k = mk % a.K;
+  const int tid = threadIdx.x;
+  const float2* tw = twiddles<N>();
+  const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    t[T::lds(e / N, e % N)] = src[e];
+  }
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  for (int s = 0; s < a.S; ++s) {
+    __syncthreads();
+    const float2* o = a.Ob + ((size_t(b) * a.S + s) * a.K + k) * N * N;
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      const int l = T::lds(e / N, e % N);
+      const float2 u = t[l];
+      if (a.write_stash) st[size_t(s) * N * N + e] = u;
+      t[l] = cmul(u, o[e]);
+    }
+    __syncthreads();
+    if (s < a.S - 1) {
+      tile_fft_fwd<N>(t, tw);
+      const float2* H = (s == 0) ? a.H0 : a.H;
+#pragma unroll 4
+      for (int i = 0; i < T::PER_THREAD; ++i) {
+        const int e = tid + i * T::NT;
+        const int l = T::lds(e / N, e % N);
+        t[l] = cmul(t[l], H[e]);
+      }
+      __syncthreads();
+      tile_fft_inv<N>(t, tw);
+    }
+  }
+  float2* out = a.Psi + (size_t(b) * MK + mk) * N * N;
+  if (!a.far_field) {
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      out[e] = t[T::lds(e / N, e % N)];
+    }
+    return;
+  }
+  tile_fft_fwd<N>(t, tw);
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    out[e] = cscale(t[T::lds(e / N, e % N)], a.mask[e]);
+  }
+}
+
+// loss, sse and dL/dI (objective.py:527-587), one workgroup per window
+template <int N>
+__global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
+  using T = Tile<N>;
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const int MK = a.M * a.K;
+  if (idx < 0) {  // invalid sequence entry: no loss, zero gradient
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) a.coef[size_t(b) * N * N + e] = 0.f;
+    if (threadIdx.x == 0) {
+      a.loss_part[2 * size_t(a.b0 + b)] = 0.0;
+      a.loss_part[2 * size_t(a.b0 + b) + 1] = 0.0;
+    }
+    return;
+  }
+  const float* D = a.meas + size_t(idx) * N * N;
+  const float2* psi = a.Psi + size_t(b) * MK * N * N;
+  const float scale = a.loss_weight * float(N) * float(N);
+  double lsum = 0.0, ssum = 0.0;
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+    float I = 0.f;
+    for (int j = 0; j < MK; ++j) I += cabs2(psi[size_t(j) * N * N + e]);
+    if (MK > 1) I = I / float(MK);
+    const int ys = (T::pos2freq(e / N) + N / 2) % N, xs = (T::pos2freq(e % N) + N / 2) % N;
+    const float d = D[ys * N + xs];
+    float l;
+    if (a.algorithm == PTY_LOSS_ML) {
+      const float diff = I - d;
+      ssum += double(diff) * double(diff);
+      lsum += double(I) - double(d) * double(logf(I + (I == 0.f ? 1.f : 0.f)));
+      l = diff / (I == 0.f ? 1.f : I);
+    } else {
+      const float r = d - I;
+      ssum += double(r) * double(r);
+      float si = sqrtf(I);
+      const float dd = si - sqrtf(d);
+      lsum += double(dd) * double(dd);
+      if (si == 0.f) si = 1.f;
+      l = dd / si;
+    }
+    if (MK > 1) l = l / float(MK);
+    a.coef[size_t(b) * N * N + e] = l * scale;
+    if (a.sim) a.sim[size_t(idx) * N * N + ys * N + xs] = I;
+  }
+  const double L = block_sum(lsum, red);
+  const double Ssum = block_sum(ssum, red);
+  if (threadIdx.x == 0) {
+    a.loss_part[2 * size_t(a.b0 + b)] = L;
+    a.loss_part[2 * size_t(a.b0 + b) + 1] = Ssum;
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_adjoint(BatchArgs a) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int k = mk % a.K;
+  const int tid = threadIdx.x;
+  const float2* tw = twiddles<N>();
+  const float2* psi = a.Psi + (size_t(b) * MK + mk) * N * N;
+  if (a.far_field) {
+    const float* c = a.coef + size_t(b) * N * N;
+    const float s = 1.0f / float(N * N);
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      t[T::lds(e / N, e % N)] = cscale(psi[e], c[e] * s);
+    }
+    __syncthreads();
+    tile_fft_inv<N>(t, tw);
+  } else {
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      t[T::lds(e / N, e % N)] = psi[e];
+    }
+  }
+  const float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  for (int s = a.S - 1; s >= 0; --s) {
+    __syncthreads();
+    const size_t win = ((size_t(b) * a.S + s) * a.K + k) * N * N;
+    const float2* o = a.Ob + win;
+    float2* G = a.Gbuf + win;
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      const int l = T::lds(e / N, e % N);
+      const float2 g = t[l];
+      if (a.step_obj) {
+        const float2 gr = cmulc(g, st[size_t(s) * N * N + e]);  // conj(u) * g
+        if (a.M > 1)
+          atomic_add_c(&G[e], gr);
+        else
+          G[e] = gr;
+      }
+      t[l] = cmulc(g, o[e]);
+    }
+    __syncthreads();
+    if (s > 0) {
+      tile_fft_fwd<N>(t, tw);
+      const float2* H = (s == 1) ? a.H0 : a.H;
+#pragma unroll 4
+      for (int i = 0; i < T::PER_THREAD; ++i) {
+        const int e = tid + i * T::NT;
+        const int l = T::lds(e / N, e % N);
+        t[l] = cmulc(t[l], H[e]);
+      }
+      __syncthreads();
+      tile_fft_inv<N>(t, tw);
+    }
+  }
+  float2* out = a.gP + (size_t(b) * MK + mk) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    out[e] = t[T::lds(e / N, e % N)];
+  }
+}
+
+// band-limit each window gradient and scatter-add it (multislice_core.py:184-192)
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const int planes = a.S * a.K;
+  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
+  const int tid = threadIdx.x;
+  float2* G = a.Gbuf + (size_t(b) * planes + plane) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    t[T::lds(e / N, e % N)] = G[e];
+    if (a.scatter) G[e] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const float2* tw = twiddles<N>();
+  tile_fft_fwd<N>(t, tw);
+  const float s = 1.0f / float(N * N);
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    const int l = T::lds(e / N, e % N);
+    t[l] = cscale(t[l], a.mask[e] * s);
+  }
+  __syncthreads();
+  tile_fft_inv<N>(t, tw);
+  if (!a.scatter) {
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      G[e] = t[T::lds(e / N, e % N)];
+    }
+    return;
+  }
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
+  if (!window_inside(y0, x0, N, a.ny, a.nx)) return;
+  float2* dst = a.obj_grad + size_t(plane) * a.ny * a.nx;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT, y = e / N, x = e % N;
+    atomic_add_c(&dst[size_t(y0 + y) * a.nx + (x0 + x)], t[T::lds(y, x)]);
+  }
+}
+
+// Probe-side adjoint (objective.py:641-658, 697-703): per window and mode
+// the gradient summed over object modes is shifted back, the position
+// gradient -2/n^2 * sum Re(K * conj(FP) * Q) is accumulated in FP64, and the
+// spectrum Q (or the gradient itself when no shift is active) is summed over
+// the group's windows in registers.
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_probegrad(BatchArgs a) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  __shared__ double red[16];
+  const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const int tid = threadIdx.x;
+  const float2* tw = twiddles<N>();
+  const float2* fp = a.fprobe + size_t(m) * N * N;
+  // this workgroup is the only writer of its partial-sum slot
+  float2* acc = a.probe_part + (size_t(g) * a.M + m) * N * N;
+  for (int b = g; b < a.B; b += a.G) {
+    const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    // sum over object modes
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      float2 v = make_float2(0.f, 0.f);
+      for (int k = 0; k < a.K; ++k) v = cadd(v, a.gP[((size_t(b) * a.M + m) * a.K + k) * N * N + e]);
+      t[T::lds(e / N, e % N)] = v;
+    }
+    __syncthreads();
+    if (a.shift) {
+      tile_fft_fwd<N>(t, tw);
+      const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+      double sy = 0.0, sx = 0.0;
+#pragma unroll 4
+      for (int i = 0; i < T::PER_THREAD; ++i) {
+        const int e = tid + i * T::NT, py = e / N, px = e % N;
+        const float mk = a.mask[e];
+        const float2 E = shift_kernel<N>(py, px, dy, dx, mk);
+        const float2 Q = cmulc(t[T::lds(py, px)], E);
+        acc[e] = cadd(acc[e], Q);
+        if (a.step_pos) {
+          const float2 Z = cmulc(Q, fp[e]);  // conj(FP) * Q
+          const float kx = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(px), N, 1.f)), mk);
+          const float ky = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(py), N, 1.f)), mk);
+          // Re(i*k * Z) = -k * Im(Z)
+          sx += -double(kx) * double(Z.y);
+          sy += -double(ky) * double(Z.y);
+        }
+      }
+      if (a.step_pos) {
+        const double SY = block_sum(sy, red);
+        const double SX = block_sum(sx, red);
+        if (tid == 0) {
+          const double sh = -2.0 / (double(N) * double(N));
+          a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
+          a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
+        }
+      }
+    } else {
+#pragma unroll 4
+      for (int i = 0; i < T::PER_THREAD; ++i) {
+        const int e = tid + i * T::NT;
+        acc[e] = cadd(acc[e], t[T::lds(e / N, e % N)]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// pos_grad[idx] = sum over probe modes (assigned per window, objective.py:658)
+__global__ void k_pos_final(BatchArgs a) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.B) return;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  if (idx < 0) return;
+  double y = 0.0, x = 0.0;
+  for (int m = 0; m < a.M; ++m) {
+    y += a.pos_part[(size_t(b) * a.M + m) * 2];
+    x += a.pos_part[(size_t(b) * a.M + m) * 2 + 1];
+  }
+  a.pos_grad[2 * idx] = float(y);
+  a.pos_grad[2 * idx + 1] = float(x);
+}
+
+// probe_grad = ifft2(mask * fft2(sum_b interm)) (objective.py:927-933)
+template <int N>
+__global__ __launch_bounds__(Tile<N>::NT) void k_probe_final(const float2* __restrict__ part, int G, int M,
+                                                             int spectral, const float* __restrict__ mask,
+                                                             float2* __restrict__ probe_grad) {
+  using T = Tile<N>;
+  __shared__ float2 t[T::ELEMS];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const float2* tw = twiddles<N>();
+  const float s = 1.0f / float(N * N);
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    float2 v = make_float2(0.f, 0.f);
+    for (int g = 0; g < G; ++g) v = cadd(v, part[(size_t(g) * M + m) * N * N + e]);
+    t[T::lds(e / N, e % N)] = spectral ? cscale(v, mask[e] * s) : v;
+  }
+  __syncthreads();
+  if (!spectral) {
+    tile_fft_fwd<N>(t, tw);
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      const int l = T::lds(e / N, e % N);
+      t[l] = cscale(t[l], mask[e] * s);
+    }
+    __syncthreads();
+  }
+  tile_fft_inv<N>(t, tw);
+  float2* dst = probe_grad + size_t(m) * N * N;
+#pragma unroll 4
+  for (int i = 0; i < T::PER_THREAD; ++i) {
+    const int e = tid + i * T::NT;
+    dst[e] = t[T::lds(e / N, e % N)];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_loss_final(const double* __restrict__ part, int n, double w,
+                                                   double* __restrict__ out) {
+  __shared__ double red[8];
+  double l = 0.0, s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    l += part[2 * size_t(i)];
+    s += part[2 * size_t(i) + 1];
+  }
+  const double L = block_sum(l, red);
+  const double S = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    out[0] = L * w;
+    out[1] = S;
+  }
+}
+
+__global__ void k_scatteradd(float2* __restrict__ full, int ny, int nx, int planes, const float2* __restrict__ patches,
+                             int n, const int* __restrict__ corners, int layout) {
+  // grid: (batch*planes), block 256
+  const int b = blockIdx.x / planes, p = blockIdx.x % planes;
+  const int y0 = corners[2 * b], x0 = corners[2 * b + 1];
+  if (!window_inside(y0, x0, n, ny, nx)) return;
+  const float2* src = patches + (size_t(b) * planes + p) * n * n;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int y = e / n, x = e % n;
+    const size_t pix = size_t(y0 + y) * nx + (x0 + x);
+    const size_t at = (layout == PTY_LAYOUT_PLANAR) ? size_t(p) * ny * nx + pix : pix * planes + p;
+    atomic_add_c(&full[at], src[e]);
+  }
+}
+
+// =================================================================== host
+namespace {
+
+// ------------------------------------------------------------- timing
+struct TimedLaunch {
+  int kind;
+  hipEvent_t a, b;
+};
+enum {
+  KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
+  KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_COUNT
+};
+const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
+                                    "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
+                                    "loss_final", "zero", "scatteradd"};
+std::mutex g_tmu;
+bool g_timing = false;
+std::vector<TimedLaunch> g_pending;
+double g_total_ms[KT_COUNT];
+int64_t g_launches[KT_COUNT];
+
+struct TimeScope {
+  int kind;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  bool on;
+  TimeScope(int k, hipStream_t st) : kind(k), s(st), on(g_timing) {
+    if (on) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, s);
+    }
+  }
+  ~TimeScope() {
+    if (on) {
+      (void)hipEventRecord(b, s);
+      std::lock_guard<std::mutex> lk(g_tmu);
+      g_pending.push_back({kind, a, b});
+    }
+  }
+};
+
+inline int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "pypty_amd: HIP launch error: %s\n", hipGetErrorString(e));
+    return PTY_ERR_HIP;
+  }
+  return PTY_OK;
+}
+
+#define PTY_DISPATCH_N(n, FN, ...)          \
+  switch (n) {                              \
+    case 16: return FN<16>(__VA_ARGS__);    \
+    case 32: return FN<32>(__VA_ARGS__);    \
+    case 48: return FN<48>(__VA_ARGS__);    \
+    case 64: return FN<64>(__VA_ARGS__);    \
+    case 96: return FN<96>(__VA_ARGS__);    \
+    case 128: return FN<128>(__VA_ARGS__);  \
+    default: return PTY_ERR_SIZE;           \
+  }
+
+template <int N>
+int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream_t s) {
+  using T = Tile<N>;
+  TimeScope ts(KT_FFT2, s);
+  const int64_t kMaxGrid = 1LL << 30;
+  for (int64_t off = 0; off < batch; off += kMaxGrid) {
+    const int g = int(std::min<int64_t>(kMaxGrid, batch - off));
+    float2* p = d + off * N * N;
+    switch (mode) {
+      case PTY_FFT_FWD: k_fft2_tiles<N, MODE_FWD><<<g, T::NT, 0, s>>>(p, mask); break;
+      case PTY_FFT_INV: k_fft2_tiles<N, MODE_INV><<<g, T::NT, 0, s>>>(p, mask); break;
+      case PTY_FFT_FWD_SHIFT: k_fft2_tiles<N, MODE_FWD_SHIFT><<<g, T::NT, 0, s>>>(p, mask); break;
+      case PTY_FFT_INV_ISHIFT: k_fft2_tiles<N, MODE_INV_ISHIFT><<<g, T::NT, 0, s>>>(p, mask); break;
+      case MODE_BAND: k_fft2_tiles<N, MODE_BAND><<<g, T::NT, 0, s>>>(p, mask); break;
+      default: return PTY_ERR_ARG;
+    }
+  }
+  return check_launch();
+}
+
+// ---------------------------------------------------------- workspace
+struct Layout {
+  size_t H, H0, mask, fprobe, objp, gradp, Ob, Pp, stash, Psi, coef, Gbuf, gP, probe_part, pos_part, loss_part,
+      total;
+  int B, G;
+};
+inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+Layout plan_layout(const pty_problem* p) {
+  Layout L{};
+  const size_t n2 = size_t(p->n) * p->n;
+  const int B = p->compute_batch;
+  const int S = p->n_slices, K = p->n_obj_modes, M = p->n_probe_modes;
+  const int G = std::min(B, 64);
+  const size_t objsz = size_t(p->obj_ny) * p->obj_nx * S * K * 8;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t r = o; o += al(bytes); return r; };
+  L.H = take(n2 * 8);
+  L.H0 = take(n2 * 8);
+  L.mask = take(n2 * 4);
+  L.fprobe = take(size_t(M) * n2 * 8);
+  L.objp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
+  L.gradp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
+  L.Ob = take(size_t(B) * S * K * n2 * 8);
+  L.Pp = take(size_t(B) * M * n2 * 8);
+  L.stash = take(size_t(B) * M * K * S * n2 * 8);
+  L.Psi = take(size_t(B) * M * K * n2 * 8);
+  L.coef = take(size_t(B) * n2 * 4);
+  L.Gbuf = take(size_t(B) * S * K * n2 * 8);
+  L.gP = take(size_t(B) * M * K * n2 * 8);
+  L.probe_part = take(size_t(G) * M * n2 * 8);
+  L.pos_part = take(size_t(B) * M * 2 * 8);
+  L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8);
+  L.total = o;
+  L.B = B;
+  L.G = G;
+  return L;
+}
+
+int validate(const pty_problem* p) {
+  if (!p) return PTY_ERR_ARG;
+  if (!pty_supported_window(p->n)) return PTY_ERR_SIZE;
+  if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1 || p->compute_batch < 1) return PTY_ERR_ARG;
+  if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
+  if (p->algorithm != PTY_LOSS_LSQ_SQRT && p->algorithm != PTY_LOSS_ML) return PTY_ERR_ARG;
+  return PTY_OK;
+}
+
+template <int N>
+int launch_tables(const pty_problem* p, float2* H, float2* H0, float* mask, hipStream_t s) {
+  TimeScope ts(KT_TABLES, s);
+  TableArgs a{H, H0, mask, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x, p->tan_y,
+              p->damping_cutoff, p->edge_multiplier};
+  k_tables<N><<<(N * N + 255) / 256, 256, 0, s>>>(a);
+  return check_launch();
+}
+
+int zero_fill(void* ptr, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return PTY_OK;
+  TimeScope ts(KT_ZERO, s);
+  const size_t nf = bytes / 4;
+  const int grid = int(std::min<size_t>((nf + 255) / 256, 65536));
+  k_zero<<<grid, 256, 0, s>>>(static_cast<float*>(ptr), nf);
+  return check_launch();
+}
+
+template <int N>
+int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
+  using T = Tile<N>;
+  const int planes = a.S * a.K;
+  const int MK = a.M * a.K;
+  {
+    TimeScope ts(KT_PATCH, s);
+    k_patch<N><<<a.B * planes, T::NT, 0, s>>>(a);
+  }
+  if (a.shift) {
+    TimeScope ts(KT_PROBE_SHIFT, s);
+    k_probe_shift<N><<<a.B * a.M, T::NT, 0, s>>>(a);
+  }
+  {
+    TimeScope ts(KT_FORWARD, s);
+    k_forward<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
+  }
+  {
+    TimeScope ts(KT_LOSS, s);
+    k_loss<N><<<a.B, 256, 0, s>>>(a);
+  }
+  if (p->step_obj || p->step_probe || p->step_pos) {
+    TimeScope ts(KT_ADJOINT, s);
+    k_adjoint<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
+  }
+  if (p->step_obj) {
+    TimeScope ts(KT_OBJGRAD, s);
+    k_objgrad<N><<<a.B * planes, T::NT, 0, s>>>(a);
+  }
+  if (p->step_probe || (p->step_pos && a.shift)) {
+    TimeScope ts(KT_PROBEGRAD, s);
+    k_probegrad<N><<<a.G * a.M, T::NT, 0, s>>>(a);
+  }
+  if (p->step_pos && a.shift) {
+    TimeScope ts(KT_POS_FINAL, s);
+    k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a);
+  }
+  return check_launch();
+}
+
+template <int N>
+int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_t s) {
+  using T = Tile<N>;
+  const Layout L = plan_layout(p);
+  if (!bf->workspace || bf->workspace_bytes < L.total) return PTY_ERR_WORKSPACE;
+  char* ws = static_cast<char*>(bf->workspace);
+  const int S = p->n_slices, K = p->n_obj_modes, M = p->n_probe_modes, planes = S * K;
+  const size_t n2 = size_t(N) * N;
+  const size_t objelems = size_t(p->obj_ny) * p->obj_nx * planes;
+  int rc;
+  float2* H = reinterpret_cast<float2*>(ws + L.H);
+  float2* H0 = reinterpret_cast<float2*>(ws + L.H0);
+  float* mask = reinterpret_cast<float*>(ws + L.mask);
+  if ((rc = launch_tables<N>(p, H, H0, mask, s))) return rc;
+  float2* fprobe = reinterpret_cast<float2*>(ws + L.fprobe);
+  {
+    TimeScope ts(KT_FPROBE, s);
+    k_fprobe<N><<<M, T::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe);
+  }
+  const float2* objp = static_cast<const float2*>(bf->obj);
+  float2* gradp = static_cast<float2*>(bf->obj_grad);
+  const int lgrid = int(std::min<size_t>((objelems + 255) / 256, 65536));
+  if (p->obj_layout == PTY_LAYOUT_REFERENCE) {
+    float2* tmp = reinterpret_cast<float2*>(ws + L.objp);
+    TimeScope ts(KT_LAYOUT, s);
+    k_ref_to_planar<<<lgrid, 256, 0, s>>>(objp, tmp, p->obj_ny, p->obj_nx, planes);
+    objp = tmp;
+    gradp = reinterpret_cast<float2*>(ws + L.gradp);
+  }
+  if (p->step_obj) {
+    if (!bf->obj_grad) return PTY_ERR_ARG;
+    if ((rc = zero_fill(gradp, objelems * 8, s))) return rc;
+    if ((rc = zero_fill(ws + L.Gbuf, size_t(L.B) * planes * n2 * 8, s))) return rc;
+  }
+  if (p->step_probe || p->step_pos) {
+    if ((rc = zero_fill(ws + L.probe_part, size_t(L.G) * M * n2 * 8, s))) return rc;
+  }
+  BatchArgs a{};
+  a.S = S; a.K = K; a.M = M; a.ny = p->obj_ny; a.nx = p->obj_nx; a.P = p->n_positions;
+  a.algorithm = p->algorithm; a.step_obj = p->step_obj; a.step_probe = p->step_probe; a.step_pos = p->step_pos;
+  a.shift = p->shift_active; a.far_field = 1; a.write_stash = 1; a.scatter = 1;
+  a.loss_weight = p->loss_weight;
+  a.obj = objp; a.probe = static_cast<const float2*>(bf->probe); a.fprobe = fprobe;
+  a.pos = bf->positions; a.corr = bf->corrections; a.meas = bf->measured; a.seq = bf->sequence;
+  a.H = H; a.H0 = H0; a.mask = mask;
+  a.Ob = reinterpret_cast<float2*>(ws + L.Ob); a.Pp = reinterpret_cast<float2*>(ws + L.Pp);
+  a.stash = reinterpret_cast<float2*>(ws + L.stash); a.Psi = reinterpret_cast<float2*>(ws + L.Psi);
+  a.coef = reinterpret_cast<float*>(ws + L.coef); a.Gbuf = reinterpret_cast<float2*>(ws + L.Gbuf);
+  a.gP = reinterpret_cast<float2*>(ws + L.gP); a.probe_part = reinterpret_cast<float2*>(ws + L.probe_part);
+  a.pos_part = reinterpret_cast<double*>(ws + L.pos_part); a.loss_part = reinterpret_cast<double*>(ws + L.loss_part);
+  a.sim = bf->sim_patterns; a.obj_grad = gradp; a.pos_grad = bf->pos_grad; a.G = L.G;
+  for (int b0 = 0; b0 < bf->n_seq; b0 += L.B) {
+    a.b0 = b0;
+    a.B = std::min(L.B, bf->n_seq - b0);
+    a.G = std::min(L.G, a.B);
+    if ((rc = run_batch<N>(a, p, s))) return rc;
+  }
+  {
+    TimeScope ts(KT_LOSS_FINAL, s);
+    k_loss_final<<<1, 256, 0, s>>>(a.loss_part, bf->n_seq, double(p->loss_weight), bf->loss_sse);
+  }
+  if (p->step_probe) {
+    if (!bf->probe_grad) return PTY_ERR_ARG;
+    TimeScope ts(KT_PROBE_FINAL, s);
+    k_probe_final<N><<<M, T::NT, 0, s>>>(a.probe_part, L.G, M, p->shift_active, mask,
+                                          static_cast<float2*>(bf->probe_grad));
+  }
+  if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE) {
+    TimeScope ts(KT_LAYOUT, s);
+    k_planar_to_ref<<<lgrid, 256, 0, s>>>(gradp, static_cast<float2*>(bf->obj_grad), p->obj_ny, p->obj_nx,
+                                           planes);
+  }
+  return check_launch();
+}
+
+template <int N>
+int launch_tables_natural(const void* prop, const float* mask_nat, float edge, void* tables, hipStream_t s) {
+  const size_t n2 = size_t(N) * N;
+  float2* H = static_cast<float2*>(tables);
+  float2* H0 = H + n2;
+  float* mask = reinterpret_cast<float*>(H0 + n2);
+  TimeScope ts(KT_TABLES, s);
+  k_tables_natural<N><<<(N * N + 255) / 256, 256, 0, s>>>(static_cast<const float2*>(prop), mask_nat, edge, H, H0,
+                                                           mask);
+  return check_launch();
+}
+
+template <int N>
+int ms_forward_n(const pty_problem* p, int batch, const void* probe_b, const void* obj_b, const void* prop,
+                 void* stash, void* exit_wave, void* tables, hipStream_t s) {
+  using T = Tile<N>;
+  const size_t n2 = size_t(N) * N;
+  float2* H = static_cast<float2*>(tables);
+  float2* H0 = H + n2;
+  float* mask = reinterpret_cast<float*>(H0 + n2);
+  int rc;
+  if ((rc = launch_tables_natural<N>(prop, nullptr, p->edge_multiplier, tables, s))) return rc;
+  BatchArgs a{};
+  a.B = batch; a.S = p->n_slices; a.K = p->n_obj_modes; a.M = p->n_probe_modes;
+  a.shift = 1; a.far_field = 0; a.write_stash = 1;
+  a.Pp = const_cast<float2*>(static_cast<const float2*>(probe_b));
+  a.Ob = const_cast<float2*>(static_cast<const float2*>(obj_b));
+  a.stash = static_cast<float2*>(stash);
+  a.Psi = static_cast<float2*>(exit_wave);
+  a.H = H; a.H0 = H0; a.mask = mask;
+  TimeScope ts(KT_FORWARD, s);
+  k_forward<N><<<xcd_grid(batch, a.M * a.K), T::NT, 0, s>>>(a);
+  return check_launch();
+}
+
+template <int N>
+int ms_adjoint_n(const pty_problem* p, int batch, void* g, const void* stash, const void* obj_b, const void* prop,
+                 const float* mask_nat, void* grad_b, void* tables, hipStream_t s) {
+  using T = Tile<N>;
+  const size_t n2 = size_t(N) * N;
+  float2* H = static_cast<float2*>(tables);
+  float2* H0 = H + n2;
+  float* mask = reinterpret_cast<float*>(H0 + n2);
+  int rc;
+  if ((rc = launch_tables_natural<N>(prop, mask_nat, p->edge_multiplier, tables, s))) return rc;
+  BatchArgs a{};
+  a.B = batch; a.S = p->n_slices; a.K = p->n_obj_modes; a.M = p->n_probe_modes;
+  a.far_field = 0; a.step_obj = 1; a.scatter = 0;
+  a.Psi = static_cast<float2*>(g);
+  a.gP = static_cast<float2*>(g);  // each workgroup reads its own tile before overwriting it
+  a.stash = const_cast<float2*>(static_cast<const float2*>(stash));
+  a.Ob = const_cast<float2*>(static_cast<const float2*>(obj_b));
+  a.Gbuf = static_cast<float2*>(grad_b);
+  a.H = H; a.H0 = H0; a.mask = mask;
+  if (a.M > 1 && (rc = zero_fill(grad_b, size_t(batch) * a.S * a.K * n2 * 8, s))) return rc;
+  {
+    TimeScope ts(KT_ADJOINT, s);
+    k_adjoint<N><<<xcd_grid(batch, a.M * a.K), T::NT, 0, s>>>(a);
+  }
+  {
+    TimeScope ts(KT_OBJGRAD, s);
+    k_objgrad<N><<<batch * a.S * a.K, T::NT, 0, s>>>(a);
+  }
+  return check_launch();
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* pty_status_string(int status) {
+  switch (status) {
+    case PTY_OK: return "ok";
+    case PTY_ERR_ARG: return "invalid argument";
+    case PTY_ERR_SIZE: return "unsupported window size";
+    case PTY_ERR_HIP: return "HIP runtime error";
+    case PTY_ERR_BOUNDS: return "window outside the object";
+    case PTY_ERR_WORKSPACE: return "workspace missing or too small";
+    default: return "unknown status";
+  }
+}
+
+int pty_version(void) { return 100; }
+
+int pty_supported_window(int n) {
+  return (n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128) ? 1 : 0;
+}
+
+int pty_fft2_c64(void* data, int64_t batch, int n, int mode, void* stream) {
+  if (batch < 0 || (batch > 0 && !data)) return PTY_ERR_ARG;
+  if (mode < PTY_FFT_FWD || mode > PTY_FFT_INV_ISHIFT) return PTY_ERR_ARG;
+  if (batch == 0) return pty_supported_window(n) ? PTY_OK : PTY_ERR_SIZE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PTY_DISPATCH_N(n, launch_fft2, static_cast<float2*>(data), batch, mode, nullptr, s);
+}
+
+int pty_bandlimit_c64(void* data, int64_t batch, int n, const float* mask, void* stream) {
+  if (batch < 0 || (batch > 0 && (!data || !mask))) return PTY_ERR_ARG;
+  if (batch == 0) return pty_supported_window(n) ? PTY_OK : PTY_ERR_SIZE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PTY_DISPATCH_N(n, launch_fft2, static_cast<float2*>(data), batch, (int)MODE_BAND, mask, s);
+}
+
+int pty_scatteradd_c64(void* full, int ny, int nx, int planes, int layout, const void* patches, int batch, int n,
+                       const int32_t* corners, void* stream) {
+  if (!full || !patches || !corners || ny < 1 || nx < 1 || planes < 1 || n < 1 || batch < 0) return PTY_ERR_ARG;
+  if (layout != PTY_LAYOUT_PLANAR && layout != PTY_LAYOUT_REFERENCE) return PTY_ERR_ARG;
+  if (batch == 0) return PTY_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TimeScope ts(KT_SCATTER, s);
+  k_scatteradd<<<batch * planes, 256, 0, s>>>(static_cast<float2*>(full), ny, nx, planes,
+                                               static_cast<const float2*>(patches), n, corners, layout);
+  return check_launch();
+}
+
+int pty_workspace_bytes(const pty_problem* p, size_t* bytes) {
+  int rc = validate(p);
+  if (rc) return rc;
+  if (!bytes) return PTY_ERR_ARG;
+  *bytes = plan_layout(p).total;
+  return PTY_OK;
+}
+
+int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* stream) {
+  int rc = validate(p);
+  if (rc) return rc;
+  if (!b || !b->obj || !b->probe || !b->positions || !b->measured || !b->sequence || !b->loss_sse) return PTY_ERR_ARG;
+  if (b->n_seq < 1 || b->n_seq > p->n_positions) return PTY_ERR_ARG;
+  if ((p->shift_active || p->step_pos) && !b->corrections) return PTY_ERR_ARG;
+  if (p->step_pos && !b->pos_grad) return PTY_ERR_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PTY_DISPATCH_N(p->n, loss_and_direction_n, p, b, s);
+}
+
+int pty_tables_bytes(int n, size_t* bytes) {
+  if (!bytes || !pty_supported_window(n)) return PTY_ERR_SIZE;
+  *bytes = size_t(n) * n * (8 + 8 + 4);
+  return PTY_OK;
+}
+
+int pty_ms_forward(const pty_problem* p, int batch, const void* probe_b, const void* obj_b, const void* propagator,
+                   void* stash, void* exit_wave, void* tables, void* stream) {
+  if (!p || !pty_supported_window(p->n)) return PTY_ERR_SIZE;
+  if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1) return PTY_ERR_ARG;
+  if (batch < 1 || !probe_b || !obj_b || !propagator || !stash || !exit_wave || !tables) return PTY_ERR_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PTY_DISPATCH_N(p->n, ms_forward_n, p, batch, probe_b, obj_b, propagator, stash, exit_wave, tables, s);
+}
+
+int pty_ms_adjoint(const pty_problem* p, int batch, void* g, const void* stash, const void* obj_b,
+                   const void* propagator, const float* mask, void* grad_b, void* tables, void* stream) {
+  if (!p || !pty_supported_window(p->n)) return PTY_ERR_SIZE;
+  if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1) return PTY_ERR_ARG;
+  if (batch < 1 || !g || !stash || !obj_b || !propagator || !mask || !grad_b || !tables) return PTY_ERR_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PTY_DISPATCH_N(p->n, ms_adjoint_n, p, batch, g, stash, obj_b, propagator, mask, grad_b, tables, s);
+}
+
+void pty_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_timing = on != 0;
+}
+
+int pty_timing_read(const char** names, double* total_ms, int64_t* launches, int cap) {
+  std::vector<TimedLaunch> pend;
+  {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    pend.swap(g_pending);
+  }
+  for (auto& t : pend) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(t.b);
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      g_total_ms[t.kind] += ms;
+      g_launches[t.kind] += 1;
+    }
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  const int n = std::min<int>(cap, KT_COUNT);
+  for (int i = 0; i < n; ++i) {
+    if (names) names[i] = kKindNames[i];
+    if (total_ms) total_ms[i] = g_total_ms[i];
+    if (launches) launches[i] = g_launches[i];
+  }
+  return KT_COUNT;
+}
+
+void pty_timing_reset(void) {
+  pty_timing_read(nullptr, nullptr, nullptr, 0);
+  for (int i = 0; i < KT_COUNT; ++i) {
+    g_total_ms[i] = 0.0;
+    g_launches[i] = 0;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+"""Complex 2D FFTs of probe-sized windows (mirror of pypty/fft.py:91-178).
+
+Same call signatures as the reference: ``fft2(arr, axes=(0,1),
+overwrite_x=False, xp=None)``.  ``arr`` is a complex64 torch tensor on a HIP
+device; the two transformed axes are moved innermost, the batch of n x n
+tiles is transformed by the LDS-resident gfx950 kernel (pty_fft2_c64) and the
+axes are moved back.  numpy convention: forward unnormalised, inverse 1/n^2.
+``xp`` is accepted for signature compatibility and ignored.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+
+def _run(arr: torch.Tensor, axes, mode: int, overwrite_x: bool) -> torch.Tensor:
+    if not isinstance(arr, torch.Tensor) or arr.device.type != "cuda":
+        raise TypeError("pypty_amd.fft: input must be a torch tensor on a HIP device")
+    if arr.dtype != torch.complex64:
+        raise TypeError(f"pypty_amd.fft: complex64 required, got {arr.dtype}")
+    nd = arr.dim()
+    a0, a1 = (ax % nd for ax in axes)
+    if a0 == a1:
+        raise ValueError("pypty_amd.fft: axes must differ")
+    n0, n1 = arr.shape[a0], arr.shape[a1]
+    if n0 != n1:
+        raise ValueError(f"pypty_amd.fft: square windows required, got {n0}x{n1}")
+    lib = _lib.load()
+    if not lib.pty_supported_window(n0):
+        raise ValueError(f"pypty_amd.fft: window size {n0} not supported")
+    moved = torch.movedim(arr, (a0, a1), (nd - 2, nd - 1))
+    work = moved if (overwrite_x and moved.is_contiguous()) else moved.contiguous()
+    if work.data_ptr() == arr.data_ptr() and not overwrite_x:
+        work = work.clone()
+    batch = work.numel() // (n0 * n1)
+    _lib.check(lib.pty_fft2_c64(work.data_ptr(), batch, n0, mode, _lib.stream_ptr(arr.device)), "fft2")
+    return torch.movedim(work, (nd - 2, nd - 1), (a0, a1))
+
+
+def fft2(arr, axes=(0, 1), overwrite_x=False, xp=None):
+    """Unnormalised 2D DFT over ``axes`` (pypty/fft.py:114-134)."""
+    return _run(arr, axes, _lib.PTY_FFT_FWD, overwrite_x)
+
+
+def ifft2(arr, axes=(0, 1), overwrite_x=False, xp=None):
+    """Inverse 2D DFT with 1/n^2 (pypty/fft.py:136-156)."""
+    return _run(arr, axes, _lib.PTY_FFT_INV, overwrite_x)
+
+
+def shift_fft2(arr, axes=(0, 1), overwrite_x=False, xp=None):
+    """fftshift(fft2(arr)) over ``axes`` (pypty/fft.py:91-112)."""
+    return _run(arr, axes, _lib.PTY_FFT_FWD_SHIFT, overwrite_x)
+
+
+def ifft2_ishift(arr, axes=(0, 1), overwrite_x=False, xp=None):
+    """ifft2(ifftshift(arr)) over ``axes`` (pypty/fft.py:158-178)."""
+    return _run(arr, axes, _lib.PTY_FFT_INV_ISHIFT, overwrite_x)
+
+
+def fftfreq(length, sampling=1, xp=None):
+    """numpy.fft.fftfreq as a float64 CPU tensor (pypty/fft.py:15-31)."""
+    n = int(length)
+    k = torch.cat([torch.arange(0, (n - 1) // 2 + 1), torch.arange(-(n // 2), 0)]).to(torch.float64)
+    return k * (1.0 / (n * float(sampling)))
+
+
+def fftshift(array, xp=None):
+    """Shift over every axis, like scipy.fft.fftshift (pypty/fft.py:52-68)."""
+    return torch.roll(array, shifts=[s // 2 for s in array.shape], dims=list(range(array.dim())))
+
+
+def ifftshift(array, xp=None):
+    """Inverse of fftshift over every axis (pypty/fft.py:70-89)."""
+    return torch.roll(array, shifts=[-(s // 2) for s in array.shape], dims=list(range(array.dim())))

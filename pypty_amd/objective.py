@@ -1,0 +1,251 @@
+"""Loss and gradient over all scan positions (mirror of pypty/objective.py:47-967).
+
+``loss_and_direction`` keeps the reference's 77-argument signature and
+10-tuple result.  The far-field multislice path with the lsq_sqrt / ml
+losses and object, probe and position gradients runs entirely in the gfx950
+kernels behind ``pty_loss_and_direction``; arguments that select features
+outside that path (near field, tilts, compressed losses, static background,
+aberrations, beam current, phase plates, regularisers, data padding /
+binning / upsampling) raise ``NotImplementedError`` instead of silently
+computing something else.
+
+``PtychoPass`` is the allocation-owning engine under it: it caches the
+device workspace per problem shape so repeated passes (line-search trials,
+epochs, benchmark steps) allocate nothing.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+LOSSES = {"lsq_sqrt": _lib.PTY_LOSS_LSQ_SQRT, "ml": _lib.PTY_LOSS_ML}
+
+
+@dataclass(frozen=True)
+class PassConfig:
+    """Static description of one loss/gradient evaluation (the pty_problem struct)."""
+
+    n: int
+    n_slices: int
+    n_obj_modes: int
+    n_probe_modes: int
+    obj_ny: int
+    obj_nx: int
+    n_positions: int
+    compute_batch: int
+    algorithm: str = "lsq_sqrt"
+    planar: bool = False
+    step_obj: bool = True
+    step_probe: bool = True
+    step_pos: bool = False
+    shift_active: bool = False
+    loss_weight: float = 1.0
+    slice_distance: float = 1.0
+    wavelength: float = 0.0196868
+    pixel_size_x: float = 1.0
+    pixel_size_y: float = 1.0
+    tan_x: float = 0.0
+    tan_y: float = 0.0
+    damping_cutoff: float = 2.0 / 3.0
+    edge_multiplier: float = 1.0
+
+    def to_c(self) -> _lib.PtyProblem:
+        if self.algorithm not in LOSSES:
+            raise NotImplementedError(f"loss '{self.algorithm}' is not on the accelerated path")
+        return _lib.PtyProblem(
+            self.n, self.n_slices, self.n_obj_modes, self.n_probe_modes, self.obj_ny, self.obj_nx,
+            self.n_positions, self.compute_batch, LOSSES[self.algorithm],
+            _lib.PTY_LAYOUT_PLANAR if self.planar else _lib.PTY_LAYOUT_REFERENCE,
+            int(bool(self.step_obj)), int(bool(self.step_probe)), int(bool(self.step_pos)),
+            int(bool(self.shift_active)), float(self.loss_weight), float(self.slice_distance),
+            float(self.wavelength), float(self.pixel_size_x), float(self.pixel_size_y), float(self.tan_x),
+            float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier))
+
+
+class PtychoPass:
+    """One forward + loss + adjoint pass over a sequence of positions, on one HIP device."""
+
+    _workspaces: dict = {}
+
+    def __init__(self, cfg: PassConfig, device=None):
+        self.cfg = cfg
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.lib = _lib.load()
+        self.c_problem = cfg.to_c()
+        nbytes = ctypes.c_size_t(0)
+        _lib.check(self.lib.pty_workspace_bytes(ctypes.byref(self.c_problem), ctypes.byref(nbytes)), "workspace")
+        self.workspace_bytes = int(nbytes.value)
+        key = (self.device, self.workspace_bytes)
+        ws = PtychoPass._workspaces.get(key)
+        if ws is None:
+            PtychoPass._workspaces.clear()  # keep a single cached workspace per process
+            ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
+            PtychoPass._workspaces[key] = ws
+        self.workspace = ws
+        self.loss_sse = torch.zeros(2, dtype=torch.float64, device=self.device)
+
+    def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
+            pos_grad=None, sim_patterns=None, sync=True):
+        """Launch the pass.  probe_mnn is mode-major (M, n, n).  Returns (loss, sse) as floats
+        when ``sync`` else the device tensor [loss, sse]."""
+        c = self.cfg
+        for t, name in ((obj, "obj"), (probe_mnn, "probe"), (positions, "positions"), (measured, "measured"),
+                        (sequence, "sequence")):
+            _lib.require_device_tensor(t, name)
+        if obj.dtype != torch.complex64 or probe_mnn.dtype != torch.complex64:
+            raise TypeError("pypty_amd: object and probe must be complex64")
+        if positions.dtype != torch.int32 or sequence.dtype != torch.int32:
+            raise TypeError("pypty_amd: positions and sequence must be int32")
+        if measured.dtype != torch.float32:
+            raise TypeError("pypty_amd: measured data must be float32")
+        if c.step_obj and obj_grad is None:
+            raise ValueError("obj_grad buffer required")
+        if c.step_probe and probe_grad is None:
+            raise ValueError("probe_grad buffer required")
+        if (c.step_pos or c.shift_active) and corrections is None:
+            raise ValueError("corrections required")
+        if c.step_pos and pos_grad is None:
+            raise ValueError("pos_grad buffer required")
+        ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+        bufs = _lib.PtyBuffers(
+            ptr(obj), ptr(probe_mnn), ptr(positions), ptr(corrections), ptr(measured), ptr(sequence),
+            int(sequence.numel()), ptr(self.loss_sse), ptr(obj_grad), ptr(probe_grad), ptr(pos_grad),
+            ptr(sim_patterns), ptr(self.workspace), self.workspace_bytes)
+        rc = self.lib.pty_loss_and_direction(ctypes.byref(self.c_problem), ctypes.byref(bufs),
+                                              _lib.stream_ptr(self.device))
+        _lib.check(rc, "loss_and_direction")
+        if not sync:
+            return self.loss_sse
+        loss, sse = self.loss_sse.tolist()
+        return loss, sse
+
+
+def check_windows_inside(positions: torch.Tensor, sequence: torch.Tensor, n: int, ny: int, nx: int) -> None:
+    """Raise if a window [y, y+n) x [x, x+n) of the sequence leaves the object.
+
+    The reference would index out of range (or wrap for negative positions);
+    the kernels read the object without bounds checks, so this is enforced here.
+    """
+    p = positions.index_select(0, sequence.long())
+    mn = p.amin(dim=0).tolist() if p.numel() else [0, 0]
+    mx = p.amax(dim=0).tolist() if p.numel() else [0, 0]
+    if mn[0] < 0 or mn[1] < 0 or mx[0] + n > ny or mx[1] + n > nx:
+        raise ValueError(f"pypty_amd: scan windows leave the object: corners in y [{mn[0]}, {mx[0]}], "
+                         f"x [{mn[1]}, {mx[1]}], window {n}, object {ny}x{nx}")
+
+
+def _as_device(x, dtype, device):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device).contiguous()
+
+
+def _unsupported(name, cond):
+    if cond:
+        raise NotImplementedError(f"pypty_amd.loss_and_direction: '{name}' is outside the accelerated path")
+
+
+def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction, this_tilt_array,
+                       this_tilts_correction, this_distances, edge_slice_thickness_multiplier, measured_array,
+                       algorithm_type, this_wavelength, this_step_probe, this_step_obj, this_step_pos_correction,
+                       this_step_tilts, masks, pixel_size_x_A, pixel_size_y_A, recon_type, Cs, defocus_array,
+                       alpha_near_field, damping_cutoff_multislice, smooth_rolloff, propmethod,
+                       this_chopped_sequence, load_one_by_one, data_multiplier, data_pad, phase_plate_in_h5,
+                       this_loss_weight, data_bin, data_shift_vector, upsample_pattern, static_background,
+                       this_step_static_background, tilt_mode, aberration_marker, probe_marker, aberrations_array,
+                       compute_batch, phase_only_obj, beam_current, this_beam_current_step,
+                       this_step_aberrations_array, default_float, default_complex, xp, is_first_epoch, scan_size,
+                       scan_step_A, fast_axis_reg_weight_positions, slow_axis_reg_weight_positions,
+                       slow_axis_reg_weight_tilts, current_deformation_reg_weight_positions,
+                       current_deformation_reg_weight_tilts, fast_axis_reg_weight_tilts, aperture_mask,
+                       probe_reg_weight, current_window_weight, current_window, phase_norm_weight, abs_norm_weight,
+                       atv_weight, atv_q, atv_p, mixed_variance_weight, mixed_variance_sigma, wedge_mu, beta_wedge,
+                       probe_mod_reg_weight, probe_mod_tol, smart_memory, print_flag, data_simulation_flag,
+                       data_simulation_exitwave, use_padded_intensity):
+    """Drop-in for pypty.objective.loss_and_direction (pypty/objective.py:47).
+
+    this_obj (Ny, Nx, S, K) complex64 and full_probe (N, N, M) complex64 are
+    torch tensors on the HIP device; positions (P, 2) int, corrections (P, 2)
+    float32 and measured_array (P, N, N) float32 may be torch tensors or numpy
+    arrays.  Returns (loss, sse, object_grad, probe_grad, pos_grad, tilts_grad,
+    static_background_grad, aberrations_array_grad, beam_current_grad,
+    constraint_contributions) like the reference; unused gradients are 0 /
+    zero tensors as there.
+    """
+    _unsupported("recon_type", recon_type != "far_field")
+    _unsupported("propmethod", propmethod != "multislice")
+    _unsupported("algorithm_type", algorithm_type not in LOSSES)
+    _unsupported("this_step_tilts", bool(this_step_tilts))
+    _unsupported("tilt_mode", bool(tilt_mode))
+    _unsupported("static_background", static_background is not None and not np.isscalar(static_background))
+    _unsupported("this_step_static_background", bool(this_step_static_background))
+    _unsupported("aberration_marker", aberration_marker is not None)
+    _unsupported("probe_marker", probe_marker is not None)
+    _unsupported("beam_current", beam_current is not None)
+    _unsupported("phase_plate_in_h5", phase_plate_in_h5 is not None)
+    _unsupported("phase_only_obj", bool(phase_only_obj))
+    _unsupported("data_pad", int(data_pad) != 0)
+    _unsupported("data_bin", int(data_bin) != 1)
+    _unsupported("upsample_pattern", upsample_pattern != 1)
+    _unsupported("data_multiplier", float(data_multiplier) != 1.0)
+    _unsupported("data_shift_vector", any(int(v) != 0 for v in data_shift_vector))
+    _unsupported("data_simulation_flag", bool(data_simulation_flag))
+    _unsupported("smooth_rolloff", float(smooth_rolloff) != 0.0)
+    for name, w in (("fast_axis_reg_weight_positions", fast_axis_reg_weight_positions),
+                    ("slow_axis_reg_weight_positions", slow_axis_reg_weight_positions),
+                    ("probe_reg_weight", probe_reg_weight), ("current_window_weight", current_window_weight),
+                    ("phase_norm_weight", phase_norm_weight), ("abs_norm_weight", abs_norm_weight),
+                    ("atv_weight", atv_weight), ("mixed_variance_weight", mixed_variance_weight),
+                    ("wedge_mu", wedge_mu), ("probe_mod_reg_weight", probe_mod_reg_weight)):
+        _unsupported(name, bool(w))
+    dist = np.asarray(this_distances.cpu() if isinstance(this_distances, torch.Tensor) else this_distances,
+                      dtype=np.float64).ravel()
+    _unsupported("per-slice distances", dist.shape[0] != 1)
+    tilts = np.asarray(this_tilt_array.cpu() if isinstance(this_tilt_array, torch.Tensor) else this_tilt_array,
+                       dtype=np.float64)
+    tcorr = np.asarray(this_tilts_correction.cpu() if isinstance(this_tilts_correction, torch.Tensor)
+                       else this_tilts_correction, dtype=np.float64)
+    tt = tilts + tcorr
+    _unsupported("per-position tilts", tt.shape[0] != 1)
+    _unsupported("tilts before/after the specimen", bool(np.any(tt[0, [0, 1, 4, 5]] != 0)))
+
+    if not isinstance(this_obj, torch.Tensor) or this_obj.device.type != "cuda":
+        raise TypeError("pypty_amd: this_obj must be a torch tensor on a HIP device")
+    device = this_obj.device
+    obj = this_obj.contiguous()
+    ny, nx, S, K = obj.shape
+    n, _, M = full_probe.shape
+    probe_mnn = full_probe.to(device).permute(2, 0, 1).contiguous()
+    positions = _as_device(this_pos_array, torch.int32, device)
+    corrections = _as_device(this_pos_correction, torch.float32, device)
+    measured = _as_device(measured_array, torch.float32, device)
+    seq = _as_device(this_chopped_sequence, torch.int32, device)
+    check_windows_inside(positions, seq, n, ny, nx)
+    shift_active = bool(torch.any(corrections != 0).item()) or bool(this_step_pos_correction)
+    cutoff = float(damping_cutoff_multislice)
+    cfg = PassConfig(
+        n=n, n_slices=S, n_obj_modes=K, n_probe_modes=M, obj_ny=ny, obj_nx=nx, n_positions=positions.shape[0],
+        compute_batch=int(min(int(compute_batch), seq.numel())), algorithm=algorithm_type, planar=False,
+        step_obj=bool(this_step_obj), step_probe=bool(this_step_probe),
+        step_pos=bool(this_step_pos_correction), shift_active=shift_active,
+        loss_weight=float(this_loss_weight), slice_distance=float(dist[0]), wavelength=float(this_wavelength),
+        pixel_size_x=float(pixel_size_x_A), pixel_size_y=float(pixel_size_y_A),
+        tan_x=float(tt[0, 3]), tan_y=float(tt[0, 2]), damping_cutoff=cutoff,
+        edge_multiplier=float(edge_slice_thickness_multiplier))
+    engine = PtychoPass(cfg, device)
+    object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
+    probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None
+    pos_grad = torch.zeros_like(corrections)
+    loss, sse = engine.run(obj, probe_mnn, positions, corrections, measured, seq, obj_grad=object_grad,
+                           probe_grad=probe_grad_mnn, pos_grad=pos_grad)
+    if loss != loss:
+        raise ValueError("A very specific bad thing. Loss is Nan.")
+    probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe
+                  else torch.zeros_like(full_probe))
+    tilts_grad = torch.zeros(tt.shape, dtype=torch.float32, device=device)
+    return loss, sse, object_grad, probe_grad, pos_grad, tilts_grad, 0, 0, 0, np.array([0])

@@ -1,0 +1,79 @@
+"""Seeded small ptychography problems shared by the CPU and GPU tests."""
+from __future__ import annotations
+
+import inspect
+
+import numpy as np
+
+from oracle import ms_oracle as orc
+
+
+def make_problem(n=32, S=3, K=1, M=2, ny=80, nx=80, P=12, shift=True, seed=0, algorithm="lsq_sqrt",
+                 dz=2.0, wavelength=0.0196868, px=0.2, step_pos=None, noise=0.05):
+    """Weakly scattering object, aperture-limited probe modes, random scan, measured data
+    simulated by the FP64 oracle from a perturbed object plus multiplicative noise."""
+    rng = np.random.default_rng(seed)
+    phase = rng.uniform(0.0, 0.6, (ny, nx, S, K))
+    amp = 1.0 - 0.05 * rng.uniform(size=(ny, nx, S, K))
+    obj = (amp * np.exp(1j * phase)).astype(np.complex64)
+    f = np.fft.fftfreq(n)
+    fx, fy = np.meshgrid(f, f, indexing="xy")
+    ap = (fx ** 2 + fy ** 2) <= 0.22 ** 2
+    probe = np.zeros((n, n, M), np.complex64)
+    for m in range(M):
+        spec = ap * np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n))) * (1.0 / (1 + m))
+        probe[:, :, m] = np.fft.ifft2(spec) * n
+    pos = np.stack([rng.integers(0, ny - n + 1, P), rng.integers(0, nx - n + 1, P)], axis=1).astype(np.int32)
+    corr = (rng.uniform(-0.5, 0.5, (P, 2)) if shift else np.zeros((P, 2))).astype(np.float32)
+    seq = np.arange(P)
+    true_obj = (obj * np.exp(1j * 0.1 * rng.standard_normal(obj.shape))).astype(np.complex128)
+    _, _, _, _, _, I = orc.loss_and_direction(
+        true_obj, probe.astype(np.complex128), pos, corr, np.zeros((P, n, n)), seq, dz=dz,
+        wavelength=wavelength, pixel_size_x=px, pixel_size_y=px, algorithm=algorithm, step_obj=False,
+        step_probe=False, step_pos=False, cdtype=np.complex128, return_patterns=True)
+    meas = np.maximum(I * (1 + noise * rng.standard_normal(I.shape)), 0).astype(np.float32)
+    return dict(obj=obj, probe=probe, pos=pos, corr=corr, meas=meas, seq=seq, dz=dz, wavelength=wavelength, px=px,
+                algorithm=algorithm, step_pos=bool(shift) if step_pos is None else step_pos)
+
+
+def oracle_pass(pr, cdtype=np.complex64, step_obj=True, step_probe=True, compute_batch=5, loss_weight=1.0):
+    return orc.loss_and_direction(
+        pr["obj"], pr["probe"], pr["pos"], pr["corr"], pr["meas"], pr["seq"], dz=pr["dz"],
+        wavelength=pr["wavelength"], pixel_size_x=pr["px"], pixel_size_y=pr["px"], algorithm=pr["algorithm"],
+        step_obj=step_obj, step_probe=step_probe, step_pos=pr["step_pos"], cdtype=cdtype,
+        compute_batch=compute_batch, loss_weight=loss_weight)
+
+
+def reference_kwargs(**over):
+    """Arguments of loss_and_direction with the reference's defaults for the inactive features."""
+    kw = dict(
+        this_tilt_array=np.zeros((1, 6), np.float32), this_tilts_correction=np.zeros((1, 6), np.float32),
+        edge_slice_thickness_multiplier=1.0, masks=None, recon_type="far_field", Cs=0.0,
+        defocus_array=np.array([0.0]), alpha_near_field=0.0, damping_cutoff_multislice=2 / 3, smooth_rolloff=0.0,
+        propmethod="multislice", load_one_by_one=False, data_multiplier=1, data_pad=0, phase_plate_in_h5=None,
+        this_loss_weight=1.0, data_bin=1, data_shift_vector=[0, 0], upsample_pattern=1, static_background=None,
+        this_step_static_background=0, tilt_mode=0, aberration_marker=None, probe_marker=None,
+        aberrations_array=np.zeros((1, 1)), compute_batch=5, phase_only_obj=False, beam_current=None,
+        this_beam_current_step=0, this_step_aberrations_array=0, default_float=np.float32,
+        default_complex=np.complex64, xp=np, is_first_epoch=True, scan_size=None, scan_step_A=1.0,
+        fast_axis_reg_weight_positions=0, slow_axis_reg_weight_positions=0, slow_axis_reg_weight_tilts=0,
+        current_deformation_reg_weight_positions=0, current_deformation_reg_weight_tilts=0,
+        fast_axis_reg_weight_tilts=0, aperture_mask=None, probe_reg_weight=0, current_window_weight=0,
+        current_window=None, phase_norm_weight=0, abs_norm_weight=0, atv_weight=0, atv_q=1, atv_p=2,
+        mixed_variance_weight=0, mixed_variance_sigma=0.5, wedge_mu=0, beta_wedge=0, probe_mod_reg_weight=0,
+        probe_mod_tol=0.05, smart_memory=False, print_flag=0, data_simulation_flag=False,
+        data_simulation_exitwave=False, use_padded_intensity=True)
+    kw.update(over)
+    return kw
+
+
+def call_loss_and_direction(fn, **kw):
+    """Call a loss_and_direction with the reference's positional argument order."""
+    names = list(inspect.signature(fn).parameters)
+    return fn(*[kw[k] for k in names])
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
