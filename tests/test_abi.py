@@ -1,0 +1,79 @@
+"""The C-ABI library loads without a GPU and exports every symbol include/pypty_amd.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pypty_amd.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pty_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from pypty_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        import __graft_entry__
+
+        __graft_entry__.build()
+    return _lib.load()
+
+
+def test_header_declares_the_expected_entry_points():
+    from pypty_amd import _lib
+
+    assert set(_declared()) == set(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [s for s in _declared() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_host_only_entry_points(lib):
+    assert lib.pty_version() >= 100
+    for n in (16, 32, 48, 64, 96, 128):
+        assert lib.pty_supported_window(n) == 1
+    for n in (0, 7, 100, 256):
+        assert lib.pty_supported_window(n) == 0
+    assert lib.pty_status_string(0) == b"ok"
+    assert lib.pty_status_string(-4) == b"window outside the object"
+
+
+def test_argument_errors_need_no_device(lib):
+    from pypty_amd import _lib
+
+    # null data / unsupported sizes are rejected before any HIP call
+    assert lib.pty_fft2_c64(None, 4, 64, 0, None) == -1
+    assert lib.pty_fft2_c64(None, 0, 100, 0, None) == -2
+    assert lib.pty_fft2_c64(None, 0, 64, 9, None) == -1
+    p = _lib.PtyProblem()
+    p.n, p.n_slices, p.n_obj_modes, p.n_probe_modes = 100, 1, 1, 1
+    size = ctypes.c_size_t(0)
+    assert lib.pty_workspace_bytes(ctypes.byref(p), ctypes.byref(size)) == -2
+    p.n, p.obj_ny, p.obj_nx, p.compute_batch = 64, 128, 128, 32
+    assert lib.pty_workspace_bytes(ctypes.byref(p), ctypes.byref(size)) == 0
+    assert size.value > 0
+    b = _lib.PtyBuffers()
+    assert lib.pty_loss_and_direction(ctypes.byref(p), ctypes.byref(b), None) == -1
+
+
+def test_workspace_size_formula(lib):
+    from pypty_amd import _lib
+
+    p = _lib.PtyProblem()
+    p.n, p.n_slices, p.n_obj_modes, p.n_probe_modes = 128, 10, 1, 5
+    p.obj_ny = p.obj_nx = 512
+    p.compute_batch, p.n_positions, p.obj_layout = 256, 4096, 1
+    size = ctypes.c_size_t(0)
+    assert lib.pty_workspace_bytes(ctypes.byref(p), ctypes.byref(size)) == 0
+    n2 = 128 * 128 * 8
+    stash = 256 * 5 * 10 * n2
+    assert stash < size.value < 3 * stash
