@@ -141,6 +141,35 @@ int pty_ms_adjoint(const pty_problem* p, int batch, void* g, const void* stash, 
                    const void* propagator, const float* mask, void* grad_b, void* tables,
                    void* stream);
 
+/* Fourier band-limit of whole planes, in place: pypty/utils.py:102-150
+ * (fourier_clean) applied over axes (0, 1) of every plane.  data planar
+ * (planes, ny, nx); ny and nx products of 2 and 3 with 8*ny, 8*nx <= 64 KiB;
+ * mask: normalised frequency radius <= cutoff/2 with a cosine taper of
+ * width rolloff/2 when rolloff != 0. */
+int pty_fourier_clean_c64(void* data, int64_t planes, int ny, int nx, double cutoff, double rolloff,
+                          void* stream);
+
+/* 2D FFT of whole planes (fft.py:91-178 for any ny, nx = 2^a 3^b up to
+ * 8192), modes as pty_fft2_c64; scratch: device buffer of the data's size. */
+int pty_plane_fft2_c64(void* data, void* scratch, int64_t planes, int ny, int nx, int mode,
+                       void* stream);
+
+/* L-BFGS inner products (pypty/iterative.py:593-681, 683-690, 873-916):
+ * out[p] = sum_i Re(a[p][i] * conj(b[p][i])) accumulated in FP64, for
+ * npairs <= 16 pairs of n-element complex64 vectors, or of n-element float32
+ * vectors when is_real (a, b: host arrays of device pointers).
+ * partial: device scratch of npairs * 512 doubles; out: device, npairs doubles. */
+int pty_multidot_c64(const void* const* a, const void* const* b, int npairs, int64_t n, int is_real,
+                     double* out, double* partial, void* stream);
+
+/* y <- alpha * y + beta * x over n complex64 (x NULL: y <- alpha * y):
+ * the axpy steps of the two-loop recursion and of the trial point. */
+int pty_axpby_c64(void* y, const void* x, int64_t n, float alpha, float beta, void* stream);
+
+/* out = 2 Re(-i conj(z) g): Wirtinger gradient -> phase gradient
+ * (pypty/utils.py:260-276, complex_grad_to_phase_grad); out float32, n elements. */
+int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream);
+
 /* Kernel timing (HIP events around every launch of the library). */
 void pty_timing_enable(int on);
 /* Writes up to `cap` entries: names (static strings), total ms, launches;

@@ -22,6 +22,7 @@ EXPORTED = (
     "pty_status_string", "pty_version", "pty_supported_window", "pty_fft2_c64", "pty_bandlimit_c64",
     "pty_scatteradd_c64", "pty_workspace_bytes", "pty_loss_and_direction", "pty_tables_bytes",
     "pty_ms_forward", "pty_ms_adjoint", "pty_timing_enable", "pty_timing_read", "pty_timing_reset",
+    "pty_fourier_clean_c64", "pty_plane_fft2_c64", "pty_multidot_c64", "pty_axpby_c64", "pty_phase_grad_c64",
 )
 
 
@@ -89,6 +90,16 @@ def load(path: str | None = None) -> ctypes.CDLL:
     lib.pty_ms_forward.argtypes = [P(PtyProblem), i, vp, vp, vp, vp, vp, vp, vp]
     lib.pty_ms_adjoint.restype = i
     lib.pty_ms_adjoint.argtypes = [P(PtyProblem), i, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.pty_fourier_clean_c64.restype = i
+    lib.pty_fourier_clean_c64.argtypes = [vp, i64, i, i, ctypes.c_double, ctypes.c_double, vp]
+    lib.pty_plane_fft2_c64.restype = i
+    lib.pty_plane_fft2_c64.argtypes = [vp, vp, i64, i, i, i, vp]
+    lib.pty_multidot_c64.restype = i
+    lib.pty_multidot_c64.argtypes = [P(vp), P(vp), i, i64, i, vp, vp, vp]
+    lib.pty_axpby_c64.restype = i
+    lib.pty_axpby_c64.argtypes = [vp, vp, i64, ctypes.c_float, ctypes.c_float, vp]
+    lib.pty_phase_grad_c64.restype = i
+    lib.pty_phase_grad_c64.argtypes = [vp, vp, vp, i64, vp]
     lib.pty_timing_enable.restype = None
     lib.pty_timing_enable.argtypes = [i]
     lib.pty_timing_read.restype = i

@@ -1,0 +1,404 @@
+// Large 2D FFTs of whole object planes (e.g. 512x512, 1536x1536) and the
+// Fourier band-limit of pypty/utils.py:102-150 (fourier_clean), plus the
+// vector kernels of the L-BFGS update (pypty/iterative.py:573-932).
+//
+// A plane is transformed by two kinds of workgroups, each keeping its lines
+// in LDS:
+//   rows pass : a workgroup owns `rows` full rows of one plane
+//   cols pass : a workgroup owns a block of `cols` adjacent columns of one
+//               plane (all ny samples of each)
+// A line of length L = prod(r_t), r_t in {2, 3, 4, 8}, is transformed in
+// place by mixed-radix decimation in frequency: stage t splits blocks of size
+// B into r_t interleaved sub-blocks (butterfly radix r_t, stride B/r_t,
+// twiddle W_B^(j k)); the spectrum comes out in mixed-radix digit-reversed
+// order.  The inverse runs the stages backwards (conjugate twiddle, then
+// butterfly) and consumes that order.  The band-limit is pointwise in the
+// spectrum, so it is applied in the column pass between the forward and the
+// inverse column transforms, with each position mapped to its frequency.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "pty_fft.hpp"
+#include "pypty_amd.h"
+
+using namespace pty;
+
+namespace {
+
+constexpr int kMaxStages = 16;
+
+struct LinePlan {
+  int L;
+  int nst;
+  int r[kMaxStages];
+};
+
+// factor L into radices 8/4/2/3 (largest power-of-two radices first)
+bool make_plan(int L, LinePlan& p) {
+  p.L = L;
+  p.nst = 0;
+  int m = L;
+  while (m % 3 == 0) {
+    p.r[p.nst++] = 3;
+    m /= 3;
+  }
+  while (m % 8 == 0) {
+    p.r[p.nst++] = 8;
+    m /= 8;
+  }
+  while (m % 4 == 0) {
+    p.r[p.nst++] = 4;
+    m /= 4;
+  }
+  while (m % 2 == 0) {
+    p.r[p.nst++] = 2;
+    m /= 2;
+  }
+  return m == 1 && p.nst <= kMaxStages && L >= 2;
+}
+
+}  // namespace
+
+// frequency held by position p after the forward transform
+__device__ __forceinline__ int plan_freq(const LinePlan& pl, int p) {
+  int f = 0, mult = 1, B = pl.L;
+  for (int t = 0; t < pl.nst; ++t) {
+    const int r = pl.r[t], s = B / r;
+    f += mult * (p / s);
+    p %= s;
+    mult *= r;
+    B = s;
+  }
+  return f;
+}
+
+__device__ __forceinline__ float2 twiddle_rt(int num, int den, bool inv) {
+  // exp(-+2 pi i num/den) with an accurate sincospi
+  float s, c;
+  sincospif(-2.0f * float(num) / float(den), &s, &c);
+  return make_float2(c, inv ? -s : s);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void stage_bfly(float2* x, int es, int base, int s, int j, int B) {
+  float2 v[R];
+  if constexpr (!INV) {
+#pragma unroll
+    for (int n = 0; n < R; ++n) v[n] = x[(base + s * n) * es];
+    dft<R, false>(v);
+#pragma unroll
+    for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle_rt(j * k, B, false));
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[(base + s * k) * es] = v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = x[(base + s * k) * es];
+#pragma unroll
+    for (int k = 1; k < R; ++k) v[k] = cmul(v[k], twiddle_rt(j * k, B, true));
+    dft<R, true>(v);
+#pragma unroll
+    for (int n = 0; n < R; ++n) x[(base + s * n) * es] = v[n];
+  }
+}
+
+// Transform `nlines` lines in LDS; line l starts at buf + l*lstride and has
+// element stride es.  All threads of the block take part.
+template <bool INV>
+__device__ void lines_fft(float2* buf, int nlines, int lstride, int es, const LinePlan& pl) {
+  const int L = pl.L;
+  for (int tt = 0; tt < pl.nst; ++tt) {
+    const int t = INV ? pl.nst - 1 - tt : tt;
+    int B = L;
+    for (int u = 0; u < t; ++u) B /= pl.r[u];
+    const int r = pl.r[t], s = B / r;
+    const int nb = L / r;  // butterflies per line
+    const int work = nb * nlines;
+    for (int w = threadIdx.x; w < work; w += blockDim.x) {
+      const int line = w / nb, q = w % nb;
+      const int blk = q / s, j = q % s;
+      float2* x = buf + line * lstride;
+      const int base = blk * B + j;
+      switch (r) {
+        case 2: stage_bfly<2, INV>(x, es, base, s, j, B); break;
+        case 3: stage_bfly<3, INV>(x, es, base, s, j, B); break;
+        case 4: stage_bfly<4, INV>(x, es, base, s, j, B); break;
+        default: stage_bfly<8, INV>(x, es, base, s, j, B); break;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// rows pass: grid (planes * ceil(ny/rows)); in place on planar (planes, ny, nx)
+template <bool INV>
+__global__ __launch_bounds__(256) void k_plane_rows(float2* __restrict__ data, int ny, int nx, int rows, LinePlan pl,
+                                                    float scale) {
+  extern __shared__ float2 lds[];
+  const int nblk = (ny + rows - 1) / rows;
+  const int plane = blockIdx.x / nblk, y0 = (blockIdx.x % nblk) * rows;
+  const int nr = min(rows, ny - y0);
+  float2* g = data + (size_t(plane) * ny + y0) * nx;
+  for (int i = threadIdx.x; i < nr * nx; i += blockDim.x) lds[i] = g[i];
+  __syncthreads();
+  lines_fft<INV>(lds, nr, nx, 1, pl);
+  for (int i = threadIdx.x; i < nr * nx; i += blockDim.x) g[i] = cscale(lds[i], scale);
+}
+
+// cols pass: forward column transform, band mask, inverse column transform.
+// The mask is the circle nu_x^2 + nu_y^2 <= (cutoff/2)^2 of
+// pypty/utils.py:126-131 (normalised frequency, cosine taper of width
+// rolloff/2 when rolloff != 0); positions are mapped to frequencies through
+// both axes' digit-reversed orders.
+__global__ __launch_bounds__(256) void k_plane_cols_band(float2* __restrict__ data, int ny, int nx, int cols,
+                                                         LinePlan py, LinePlan px, double cutoff, double rolloff) {
+  extern __shared__ float2 lds[];
+  const int nblk = (nx + cols - 1) / cols;
+  const int plane = blockIdx.x / nblk, x0 = (blockIdx.x % nblk) * cols;
+  const int nc = min(cols, nx - x0);
+  float2* g = data + size_t(plane) * ny * nx + x0;
+  for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
+    const int y = i / nc, c = i % nc;
+    lds[y * cols + c] = g[size_t(y) * nx + c];
+  }
+  __syncthreads();
+  lines_fft<false>(lds, nc, 1, cols, py);
+  const double max_r = 0.5 * cutoff;
+  const double r0 = (0.5 * (cutoff - rolloff)) * (0.5 * (cutoff - rolloff));
+  for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
+    const int y = i / nc, c = i % nc;
+    const int fy = plan_freq(py, y), fx = plan_freq(px, x0 + c);
+    const double vy = double((2 * fy < ny) ? fy : fy - ny) / double(ny);
+    const double vx = double((2 * fx < nx) ? fx : fx - nx) / double(nx);
+    const double r = vx * vx + vy * vy;
+    float m = (r <= max_r * max_r) ? 1.f : 0.f;
+    if (rolloff != 0.0 && r > r0 && m > 0.f)
+      m *= float(0.5 * (1.0 + cos(3.141592654 * (r - r0) / (max_r * max_r - r0))));
+    lds[y * cols + c] = cscale(lds[y * cols + c], m);
+  }
+  __syncthreads();
+  lines_fft<true>(lds, nc, 1, cols, py);
+  for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
+    const int y = i / nc, c = i % nc;
+    g[size_t(y) * nx + c] = lds[y * cols + c];
+  }
+}
+
+// cols pass of a plain transform, writing natural order (scatter of the
+// digit-reversed positions of both axes; shift folds in fftshift)
+template <bool INV>
+__global__ __launch_bounds__(256) void k_plane_cols(const float2* __restrict__ src, float2* __restrict__ dst, int ny,
+                                                    int nx, int cols, LinePlan py, LinePlan px, int shift,
+                                                    float scale) {
+  extern __shared__ float2 lds[];
+  const int nblk = (nx + cols - 1) / cols;
+  const int plane = blockIdx.x / nblk, x0 = (blockIdx.x % nblk) * cols;
+  const int nc = min(cols, nx - x0);
+  const float2* g = src + size_t(plane) * ny * nx;
+  float2* o = dst + size_t(plane) * ny * nx;
+  if (!INV) {
+    for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
+      const int y = i / nc, c = i % nc;
+      lds[y * cols + c] = g[size_t(y) * nx + x0 + c];
+    }
+    __syncthreads();
+    lines_fft<false>(lds, nc, 1, cols, py);
+    for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
+      const int y = i / nc, c = i % nc;
+      int fy = plan_freq(py, y), fx = plan_freq(px, x0 + c);
+      if (shift) {
+        fy = (fy + ny / 2) % ny;
+        fx = (fx + nx / 2) % nx;
+      }
+      o[size_t(fy) * nx + fx] = cscale(lds[y * cols + c], scale);
+    }
+  } else {
+    // natural (optionally shifted) spectrum in; gather each position's frequency
+    for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
+      const int y = i / nc, c = i % nc;
+      int fy = plan_freq(py, y), fx = plan_freq(px, x0 + c);
+      if (shift) {
+        fy = (fy + ny / 2) % ny;
+        fx = (fx + nx / 2) % nx;
+      }
+      lds[y * cols + c] = g[size_t(fy) * nx + fx];
+    }
+    __syncthreads();
+    lines_fft<true>(lds, nc, 1, cols, py);
+    for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
+      const int y = i / nc, c = i % nc;
+      o[size_t(y) * nx + x0 + c] = cscale(lds[y * cols + c], scale);
+    }
+  }
+}
+
+// ----------------------------------------------------------- L-BFGS vectors
+// out[p] = sum Re(a_p * conj(b_p)) over n complex elements, FP64 accumulation.
+// Pairs are given as arrays of pointers; one block-row of the grid per pair.
+struct DotPairs {
+  const float2* a[16];
+  const float2* b[16];
+};
+
+__global__ __launch_bounds__(256) void k_multidot(DotPairs pr, size_t n, int is_real, double* __restrict__ partial) {
+  __shared__ double red[4];
+  const int p = blockIdx.y;
+  double s = 0.0;
+  if (is_real) {
+    const float* a = reinterpret_cast<const float*>(pr.a[p]);
+    const float* b = reinterpret_cast<const float*>(pr.b[p]);
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+      s += double(a[i]) * double(b[i]);
+  } else {
+    const float2* a = pr.a[p];
+    const float2* b = pr.b[p];
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+      const float2 u = a[i], v = b[i];
+      s += double(u.x) * double(v.x) + double(u.y) * double(v.y);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[size_t(p) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_rows(const double* __restrict__ partial, int cols,
+                                                     double* __restrict__ out) {
+  __shared__ double red[4];
+  const int p = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < cols; i += blockDim.x) s += partial[size_t(p) * cols + i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[p] = red[0] + red[1] + red[2] + red[3];
+}
+
+// y = alpha * y + beta * x  (float32 complex viewed as 2n floats); alpha == 0
+// does not read y, so y may start uninitialised
+__global__ __launch_bounds__(256) void k_axpby(float* __restrict__ y, const float* __restrict__ x, size_t n, float alpha,
+                                               float beta) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    y[i] = (alpha == 0.f ? 0.f : alpha * y[i]) + (x ? beta * x[i] : 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_phase_grad(const float2* __restrict__ g, const float2* __restrict__ z,
+                                                    float* __restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const float2 a = z[i], b = g[i];
+    out[i] = 2.f * (a.x * b.y - a.y * b.x);
+  }
+}
+
+// ================================================================ host
+namespace {
+
+inline int plane_launch_check() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "pypty_amd: HIP launch error: %s\n", hipGetErrorString(e));
+    return PTY_ERR_HIP;
+  }
+  return PTY_OK;
+}
+
+constexpr size_t kPlaneLds = 64 * 1024;
+
+int rows_per_block(int nx) { return int(std::max<size_t>(1, kPlaneLds / (size_t(nx) * 8))); }
+int cols_per_block(int ny) {
+  int c = int(kPlaneLds / (size_t(ny) * 8));
+  if (c >= 16) return 16;
+  if (c >= 8) return 8;
+  return std::max(c, 1);
+}
+
+int plane_checks(void* data, int64_t planes, int ny, int nx, LinePlan& py, LinePlan& px) {
+  if (!data || planes < 1 || ny < 2 || nx < 2) return PTY_ERR_ARG;
+  if (!make_plan(ny, py) || !make_plan(nx, px)) return PTY_ERR_SIZE;
+  if (size_t(nx) * 8 > kPlaneLds || size_t(ny) * 8 > kPlaneLds) return PTY_ERR_SIZE;
+  if (planes * int64_t((ny + 1)) > (int64_t(1) << 31)) return PTY_ERR_ARG;
+  return PTY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pty_fourier_clean_c64(void* data, int64_t planes, int ny, int nx, double cutoff, double rolloff, void* stream) {
+  LinePlan py{}, px{};
+  int rc = plane_checks(data, planes, ny, nx, py, px);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float2* d = static_cast<float2*>(data);
+  const int rows = rows_per_block(nx), cols = cols_per_block(ny);
+  const int gr = int(planes) * ((ny + rows - 1) / rows), gc = int(planes) * ((nx + cols - 1) / cols);
+  k_plane_rows<false><<<gr, 256, size_t(rows) * nx * 8, s>>>(d, ny, nx, rows, px, 1.0f);
+  k_plane_cols_band<<<gc, 256, size_t(cols) * ny * 8, s>>>(d, ny, nx, cols, py, px, cutoff, rolloff);
+  k_plane_rows<true><<<gr, 256, size_t(rows) * nx * 8, s>>>(d, ny, nx, rows, px, 1.0f / (float(ny) * float(nx)));
+  return plane_launch_check();
+}
+
+int pty_plane_fft2_c64(void* data, void* scratch, int64_t planes, int ny, int nx, int mode, void* stream) {
+  LinePlan py{}, px{};
+  int rc = plane_checks(data, planes, ny, nx, py, px);
+  if (rc) return rc;
+  if (!scratch) return PTY_ERR_WORKSPACE;
+  if (mode < PTY_FFT_FWD || mode > PTY_FFT_INV_ISHIFT) return PTY_ERR_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float2* d = static_cast<float2*>(data);
+  float2* w = static_cast<float2*>(scratch);
+  const int rows = rows_per_block(nx), cols = cols_per_block(ny);
+  const int gr = int(planes) * ((ny + rows - 1) / rows), gc = int(planes) * ((nx + cols - 1) / cols);
+  const bool inv = (mode == PTY_FFT_INV || mode == PTY_FFT_INV_ISHIFT);
+  const int shift = (mode == PTY_FFT_FWD_SHIFT || mode == PTY_FFT_INV_ISHIFT) ? 1 : 0;
+  if (!inv) {
+    // rows (digit-reversed x) in place, then columns scattered to natural order into scratch, copy back
+    k_plane_rows<false><<<gr, 256, size_t(rows) * nx * 8, s>>>(d, ny, nx, rows, px, 1.0f);
+    k_plane_cols<false><<<gc, 256, size_t(cols) * ny * 8, s>>>(d, w, ny, nx, cols, py, px, shift, 1.0f);
+  } else {
+    // gather natural spectrum into digit-reversed columns (scratch), then inverse rows
+    k_plane_cols<true><<<gc, 256, size_t(cols) * ny * 8, s>>>(d, w, ny, nx, cols, py, px, shift, 1.0f);
+    k_plane_rows<true><<<gr, 256, size_t(rows) * nx * 8, s>>>(w, ny, nx, rows, px, 1.0f / (float(ny) * float(nx)));
+  }
+  const hipError_t e = hipMemcpyAsync(d, w, size_t(planes) * ny * nx * 8, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return PTY_ERR_HIP;
+  return plane_launch_check();
+}
+
+int pty_multidot_c64(const void* const* a, const void* const* b, int npairs, int64_t n, int is_real, double* out,
+                     double* partial, void* stream) {
+  if (npairs < 1 || npairs > 16 || n < 0 || !a || !b || !out || !partial) return PTY_ERR_ARG;
+  DotPairs pr{};
+  for (int i = 0; i < npairs; ++i) {
+    if (!a[i] || !b[i]) return PTY_ERR_ARG;
+    pr.a[i] = static_cast<const float2*>(a[i]);
+    pr.b[i] = static_cast<const float2*>(b[i]);
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int cols = 512;
+  k_multidot<<<dim3(cols, npairs), 256, 0, s>>>(pr, size_t(n), is_real ? 1 : 0, partial);
+  k_reduce_rows<<<npairs, 256, 0, s>>>(partial, cols, out);
+  return plane_launch_check();
+}
+
+int pty_axpby_c64(void* y, const void* x, int64_t n, float alpha, float beta, void* stream) {
+  if (!y || n < 0) return PTY_ERR_ARG;
+  if (n == 0) return PTY_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t nf = size_t(n) * 2;
+  const int grid = int(std::min<size_t>((nf + 255) / 256, 8192));
+  k_axpby<<<grid, 256, 0, s>>>(static_cast<float*>(y), static_cast<const float*>(x), nf, alpha, beta);
+  return plane_launch_check();
+}
+
+int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream) {
+  if (!g || !z || !out || n < 0) return PTY_ERR_ARG;
+  if (n == 0) return PTY_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int grid = int(std::min<size_t>((size_t(n) + 255) / 256, 8192));
+  k_phase_grad<<<grid, 256, 0, s>>>(static_cast<const float2*>(g), static_cast<const float2*>(z), out, size_t(n));
+  return plane_launch_check();
+}
+
+}  // extern "C"

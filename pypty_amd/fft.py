@@ -24,17 +24,21 @@ def _run(arr: torch.Tensor, axes, mode: int, overwrite_x: bool) -> torch.Tensor:
     if a0 == a1:
         raise ValueError("pypty_amd.fft: axes must differ")
     n0, n1 = arr.shape[a0], arr.shape[a1]
-    if n0 != n1:
-        raise ValueError(f"pypty_amd.fft: square windows required, got {n0}x{n1}")
     lib = _lib.load()
-    if not lib.pty_supported_window(n0):
-        raise ValueError(f"pypty_amd.fft: window size {n0} not supported")
     moved = torch.movedim(arr, (a0, a1), (nd - 2, nd - 1))
     work = moved if (overwrite_x and moved.is_contiguous()) else moved.contiguous()
     if work.data_ptr() == arr.data_ptr() and not overwrite_x:
         work = work.clone()
     batch = work.numel() // (n0 * n1)
-    _lib.check(lib.pty_fft2_c64(work.data_ptr(), batch, n0, mode, _lib.stream_ptr(arr.device)), "fft2")
+    stream = _lib.stream_ptr(arr.device)
+    if n0 == n1 and lib.pty_supported_window(n0):
+        # LDS-resident window kernel
+        _lib.check(lib.pty_fft2_c64(work.data_ptr(), batch, n0, mode, stream), "fft2")
+    else:
+        # whole-plane kernels (rows pass + columns pass through HBM)
+        scratch = torch.empty_like(work)
+        _lib.check(lib.pty_plane_fft2_c64(work.data_ptr(), scratch.data_ptr(), batch, n0, n1, mode, stream),
+                   "plane fft2")
     return torch.movedim(work, (nd - 2, nd - 1), (a0, a1))
 
 

@@ -1,0 +1,107 @@
+"""Spectral helpers and line-search tests (mirror of the hot-path parts of pypty/utils.py).
+
+``fourier_clean`` (utils.py:102-150) band-limits whole object planes with the
+plane FFT kernels (``pty_fourier_clean_c64``); ``create_spatial_frequencies``
+(utils.py:153-199) builds the q grids and band mask on the host (setup work
+done once per run; the kernels evaluate the same grids themselves);
+``wolfe_1`` / ``wolfe_2`` (utils.py:326-368) compare host scalars;
+``complex_grad_to_phase_grad`` (utils.py:260-276) runs ``pty_phase_grad_c64``.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _planar(array: torch.Tensor):
+    """(Ny, Nx, ...) -> contiguous (planes, Ny, Nx) and the inverse mapping."""
+    nd = array.dim()
+    moved = torch.movedim(array, (0, 1), (nd - 2, nd - 1))
+    shape = moved.shape
+    flat = moved.reshape(-1, shape[-2], shape[-1]).contiguous()
+
+    def back(t):
+        return torch.movedim(t.reshape(shape), (nd - 2, nd - 1), (0, 1))
+
+    return flat, back
+
+
+def fourier_clean_planar(planes: torch.Tensor, cutoff: float, rolloff: float = 0.0) -> torch.Tensor:
+    """In-place band-limit of contiguous (planes, Ny, Nx) complex64 data."""
+    _lib.require_device_tensor(planes, "planes")
+    if planes.dtype != torch.complex64:
+        raise TypeError("pypty_amd.fourier_clean: complex64 required")
+    P, ny, nx = planes.shape
+    rc = _lib.load().pty_fourier_clean_c64(planes.data_ptr(), P, ny, nx, float(cutoff), float(rolloff),
+                                           _lib.stream_ptr(planes.device))
+    _lib.check(rc, "fourier_clean")
+    return planes
+
+
+def fourier_clean(array, cutoff=0.66, mask=None, rolloff=0, default_float=None, xp=None):
+    """Band-limit over axes (0, 1) (pypty/utils.py:102-150): ifft2_ishift(mask * shift_fft2(array)).
+
+    ``array`` is a complex64 torch tensor (Ny, Nx[, ...]) on a HIP device;
+    ``mask`` (an explicit mask instead of the cutoff circle) is not on the
+    accelerated path.  Returns a new tensor like the reference.
+    """
+    if cutoff is None and mask is None:
+        return array
+    if mask is not None:
+        raise NotImplementedError("pypty_amd.fourier_clean: explicit masks are not on the accelerated path")
+    flat, back = _planar(array)
+    if flat.data_ptr() == array.data_ptr():
+        flat = flat.clone()
+    fourier_clean_planar(flat, float(cutoff), float(rolloff))
+    return back(flat).contiguous()
+
+
+def create_spatial_frequencies(px, py, shape, damping_cutoff_multislice, smooth_rolloff, default_float=np.float32,
+                               xp=None):
+    """q2, qx, qy, exclude_mask (fftshifted), exclude_mask_ishift as float tensors on the current
+    HIP device (pypty/utils.py:153-199)."""
+    fd = np.dtype(default_float).type if default_float is not None else np.float32
+    qx, qy = np.meshgrid(np.fft.fftfreq(shape, px), np.fft.fftfreq(shape, py), indexing="xy")
+    qx, qy = qx.astype(fd), qy.astype(fd)
+    q2 = qx ** 2 + qy ** 2
+    c = damping_cutoff_multislice
+    if c is None:
+        c = 2
+    if c < 0:
+        c = 2 / 3
+    max_x = np.max(qx)
+    r_max = (max_x * fd(c)) ** 2
+    m = (q2 <= r_max).astype(fd)
+    if smooth_rolloff != 0:
+        r0 = (max_x * (c - smooth_rolloff)) ** 2
+        sel = q2 > r0
+        m[sel] *= (0.5 * (1 + np.cos(3.141592654 * (q2 - r0) / (r_max - r0))))[sel]
+    m = m.astype(fd)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
+    return t(q2), t(qx), t(qy), t(np.fft.fftshift(m)), t(m)
+
+
+def complex_grad_to_phase_grad(grad, array, xp=None):
+    """2 Re(-i conj(array) grad) (pypty/utils.py:260-276)."""
+    for t, name in ((grad, "grad"), (array, "array")):
+        _lib.require_device_tensor(t, name)
+    if grad.shape != array.shape or grad.dtype != torch.complex64 or array.dtype != torch.complex64:
+        raise TypeError("pypty_amd.complex_grad_to_phase_grad: matching complex64 tensors required")
+    out = torch.empty(grad.shape, dtype=torch.float32, device=grad.device)
+    rc = _lib.load().pty_phase_grad_c64(grad.data_ptr(), array.data_ptr(), out.data_ptr(), grad.numel(),
+                                        _lib.stream_ptr(grad.device))
+    _lib.check(rc, "phase_grad")
+    return out
+
+
+def wolfe_1(value, new_value, d_value, step, wolfe_c1=0.5):
+    """Armijo condition (pypty/utils.py:326-348)."""
+    return new_value <= value + wolfe_c1 * step * d_value
+
+
+def wolfe_2(d_value, new_d_value, wolfe_c2=0.9):
+    """Curvature condition (pypty/utils.py:350-368)."""
+    return -1 * new_d_value <= -1 * d_value * wolfe_c2

@@ -77,3 +77,34 @@ def rel(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def bfgs_kwargs(pr, **over):
+    """Arguments of bfgs_update (reference order via call_loss_and_direction-style lookup)."""
+    kw = dict(
+        algorithm_type=pr["algorithm"], this_slice_distances=np.array([pr["dz"]], np.float32),
+        edge_slice_thickness_multiplier=1.0, this_step_probe=1, this_step_obj=1,
+        this_step_pos_correction=int(pr["step_pos"]), this_step_tilts=0, measured_array=pr["meas"],
+        this_wavelength=pr["wavelength"], masks=None, pixel_size_x_A=pr["px"], pixel_size_y_A=pr["px"],
+        phase_norm_weight=0, abs_norm_weight=0, stepsize_threshold_low=1e-20, probe_reg_weight=0, aperture_mask=None,
+        recon_type="far_field", defocus_array=np.array([0.0]), Cs=0.0, alpha_near_field=0.0,
+        damping_cutoff_multislice=np.float32(2 / 3), smooth_rolloff=0.0, update_extra_cut=np.float32(0.005),
+        keep_probe_states_orthogonal=False, do_charge_flip=False, cf_delta_phase=0.1, cf_delta_abs=0.01,
+        cf_beta_phase=-0.95, cf_beta_abs=-0.95, phase_only_obj=False, wedge_mu=0, beta_wedge=0,
+        wolfe_c1_constant=0.1, wolfe_c2_constant=0.9, atv_weight=0, atv_q=1, atv_p=2, tune_only_probe_phase=False,
+        propmethod="multislice", this_chopped_sequence=pr["seq"], load_one_by_one=False, data_multiplier=1,
+        data_pad=0, phase_plate_in_h5=None, print_flag=0, this_loss_weight=1.0, max_count=20, reduce_factor=0.1,
+        optimism=3, mixed_variance_weight=0, mixed_variance_sigma=0.5, data_bin=1, data_shift_vector=[0, 0],
+        smart_memory=False, default_float=np.float32, default_complex=np.complex64, default_int=np.int32,
+        upsample_pattern=1, this_step_static_background=0, tilt_mode=0, fancy_sigma=None, tune_only_probe_abs=False,
+        aberration_marker=None, this_step_aberrations_array=0, probe_marker=None, compute_batch=5,
+        current_window=None, current_window_weight=0, dynamically_resize_yx_object=False, lazy_clean=False,
+        current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=0,
+        xp=np, remove_fft_cache=False, is_first_epoch=True, hist_length=10, actual_step=1.0,
+        fast_axis_reg_weight_positions=0, fast_axis_reg_weight_tilts=0, slow_axis_reg_weight_positions=0,
+        slow_axis_reg_weight_tilts=0, scan_size=None, scan_step_A=1.0, current_deformation_reg_weight_positions=0,
+        current_deformation_reg_weight_tilts=0, warnings="", data_simulation_flag=False,
+        data_simulation_exitwave=False, pos_puzzling=False, probe_mod_reg_weight=0, probe_mod_tol=0.05,
+        use_padded_intensity=True, counter_failed_epochs=0)
+    kw.update(over)
+    return kw

@@ -71,3 +71,62 @@ def test_loss_and_direction_matches_oracle(case):
     print("  vs FP64 oracle:     ", {k: f"{v:.2e}" for k, v in errs64.items()})
     for k, v in errs.items():
         assert v < TOL, (k, v)
+
+
+@pytest.mark.parametrize("shape", [(192, 192), (256, 256), (96, 128), (512, 512), (384, 1536)])
+@pytest.mark.parametrize("mode", ["fft2", "ifft2", "shift_fft2", "ifft2_ishift"])
+def test_plane_fft_modes_match_numpy(shape, mode):
+    torch = _torch()
+    from pypty_amd import fft as pfft
+    from oracle import ms_oracle as orc
+
+    print(f"[plane fft] shape={shape} mode={mode}", flush=True)
+    rng = np.random.default_rng(shape[0] + shape[1])
+    x = (rng.standard_normal((2,) + shape) + 1j * rng.standard_normal((2,) + shape)).astype(np.complex64)
+    ref = getattr(orc, mode)(x.astype(np.complex128), axes=(1, 2))
+    got = getattr(pfft, mode)(torch.from_numpy(x).cuda(), axes=(1, 2)).cpu().numpy()
+    assert rel(got, ref) < 3e-6
+
+
+@pytest.mark.parametrize("shape,cutoff,rolloff", [((512, 512, 3, 1), 2 / 3 - 0.005, 0.0),
+                                                  ((256, 384, 2, 2), 0.66, 0.0),
+                                                  ((1536, 1536, 1, 1), 2 / 3 - 0.005, 0.0),
+                                                  ((192, 192, 1, 1), 0.5, 0.1)])
+def test_fourier_clean_matches_oracle(shape, cutoff, rolloff):
+    torch = _torch()
+    from pypty_amd import utils as putils
+    from oracle import ms_oracle as orc
+
+    print(f"[fourier_clean] shape={shape} cutoff={cutoff} rolloff={rolloff}", flush=True)
+    rng = np.random.default_rng(shape[0])
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    ref = orc.fourier_clean(x.astype(np.complex128), cutoff=cutoff, rolloff=rolloff, fdtype=np.float64)
+    got = putils.fourier_clean(torch.from_numpy(x).cuda(), cutoff=cutoff, rolloff=rolloff).cpu().numpy()
+    assert rel(got, ref) < 3e-6
+    # idempotent
+    again = putils.fourier_clean(torch.from_numpy(got).cuda(), cutoff=cutoff, rolloff=0.0).cpu().numpy()
+    if rolloff == 0.0:
+        assert rel(again, got) < 3e-6
+
+
+def test_lbfgs_vector_kernels():
+    torch = _torch()
+    from pypty_amd import lbfgs
+
+    print("[lbfgs vectors]", flush=True)
+    rng = np.random.default_rng(5)
+    a = [(rng.standard_normal(100003) + 1j * rng.standard_normal(100003)).astype(np.complex64) for _ in range(3)]
+    b = [(rng.standard_normal(100003) + 1j * rng.standard_normal(100003)).astype(np.complex64) for _ in range(3)]
+    ta = [torch.from_numpy(v).cuda() for v in a]
+    tb = [torch.from_numpy(v).cuda() for v in b]
+    got = lbfgs.dots(list(zip(ta, tb)))
+    ref = [float(np.sum(np.real(x.astype(np.complex128) * np.conj(y.astype(np.complex128))))) for x, y in zip(a, b)]
+    assert np.allclose(got, ref, rtol=1e-9, atol=1e-6)
+    y = ta[0].clone()
+    lbfgs.axpby(y, tb[0], 0.5, -2.0)
+    assert rel(y.cpu().numpy(), 0.5 * a[0] - 2.0 * b[0]) < 1e-6
+    # real float32 vectors
+    r1 = rng.standard_normal(1001).astype(np.float32)
+    r2 = rng.standard_normal(1001).astype(np.float32)
+    got = lbfgs.dots([(torch.from_numpy(r1).cuda(), torch.from_numpy(r2).cuda())])
+    assert np.isclose(got[0], float(np.dot(r1.astype(np.float64), r2.astype(np.float64))), rtol=1e-9)

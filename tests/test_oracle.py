@@ -261,3 +261,26 @@ def test_wolfe_conditions():
     assert not orc.wolfe_1(10.0, 9.99, -5.0, 1.0, 0.1)
     assert orc.wolfe_2(-5.0, -1.0, 0.9)
     assert not orc.wolfe_2(-5.0, -4.9, 0.9)
+
+
+def test_bfgs_oracle_epochs_decrease_loss():
+    from oracle import bfgs_oracle
+
+    pr = make_problem(n=16, S=2, K=1, M=1, ny=40, nx=40, P=8, seed=31, shift=True, noise=0.0)
+    state = {"obj": np.ones_like(pr["obj"], dtype=np.complex128), "probe": pr["probe"].astype(np.complex128),
+             "positions": pr["pos"], "corr": pr["corr"].astype(np.float64)}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "compute_batch": 4}
+    np.random.seed(0)
+    losses, accepted, step = [], [], 1.0
+    for _ in range(6):
+        out = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=True, actual_step=step, cdtype=np.complex128)
+        step = out[2]
+        losses.append(out[0])
+        accepted.append(out[6])
+    # a rejected epoch keeps the iterate (and resets the history); accepted ones satisfy Armijo
+    assert all(b <= a * (1 + 1e-12) for a, b in zip(losses, losses[1:]))
+    assert losses[-1] < 0.7 * losses[0]
+    assert sum(accepted) >= 4
+    assert hist["rho_hist"] and all(r > 0 for r in hist["rho_hist"])
