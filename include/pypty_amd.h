@@ -134,10 +134,12 @@ int pty_ms_forward(const pty_problem* p, int batch, const void* probe_b, const v
 
 /* Multislice adjoint (pypty/multislice_core.py:85-200).  g (B, M, K, n, n)
  * holds dL/d(exit)* on entry and dL/d(incident probe)* per (m, k) on exit;
+ * stash (from pty_ms_forward) is overwritten with the per-mode window
+ * gradients conj(u_s) g, as the reference overwrites waves_multislice;
  * mask (n, n) natural order (mask_clean); grad_b out (B, S, K, n, n):
  * band-limited sum over probe modes of the per-slice object gradient
  * (before the scatter-add). */
-int pty_ms_adjoint(const pty_problem* p, int batch, void* g, const void* stash, const void* obj_b,
+int pty_ms_adjoint(const pty_problem* p, int batch, void* g, void* stash, const void* obj_b,
                    const void* propagator, const float* mask, void* grad_b, void* tables,
                    void* stream);
 

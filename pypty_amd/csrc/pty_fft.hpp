@@ -160,6 +160,12 @@ __device__ __forceinline__ void dft(float2* v) {
 }
 
 // --------------------------------------------------------- tile geometry
+// Upper bound on threads per tile workgroup.  With 1024 threads a 128 x 128
+// tile gives each thread 16 samples but caps it at 128 VGPRs; 512 threads
+// double the samples per thread and the register budget.
+#ifndef PTY_MAX_NT
+#define PTY_MAX_NT 1024
+#endif
 // Split N = R1 * R2 with R2 | R1; R2 threads cooperate on one line.
 template <int N>
 struct Plan;
@@ -177,7 +183,7 @@ struct Tile {
   static_assert(R1 % R2 == 0, "R2 must divide R1");
   static constexpr int LS = N + 1;              // padded LDS row stride (elements)
   static constexpr int ELEMS = N * LS;          // LDS elements per tile
-  static constexpr int NT = (N * R2 > 1024) ? 1024 : N * R2;   // threads per tile
+  static constexpr int NT = (N * R2 > PTY_MAX_NT) ? PTY_MAX_NT : N * R2;   // threads per tile
   static constexpr int PER_THREAD = N * N / NT; // pointwise elements per thread
   static_assert((N * N) % NT == 0, "tile/threads");
   __device__ __forceinline__ static int lds(int y, int x) { return y * LS + x; }
@@ -186,6 +192,17 @@ struct Tile {
   // frequency index -> position index along one axis
   __host__ __device__ __forceinline__ static int freq2pos(int f) { return f / R1 + R2 * (f % R1); }
 };
+
+// w / N is the same for the 64 lanes of a wavefront when N is a multiple of 64
+// (lanes take consecutive lines): make that explicit so the twiddle indices
+// live in scalar registers and the table reads become scalar-cache loads.
+template <int N>
+__device__ __forceinline__ int wave_uniform(int v) {
+  if constexpr (N % 64 == 0)
+    return __builtin_amdgcn_readfirstlane(v);
+  else
+    return v;
+}
 
 // One line pass over the whole tile.  ALONG_X: lines are rows (transform over
 // x); otherwise lines are columns.  tw[m] = W_N^m, m in [0, N).
@@ -198,7 +215,7 @@ __device__ __forceinline__ void line_pass(float2* __restrict__ t, const float2* 
   auto at = [](int line, int pos) { return ALONG_X ? line * LS + pos : pos * LS + line; };
   if constexpr (!INV) {
     for (int w = tid; w < WORK; w += NT) {
-      const int line = w % N, j = w / N;
+      const int line = w % N, j = wave_uniform<N>(w / N);
       float2 v[R1];
 #pragma unroll
       for (int n = 0; n < R1; ++n) v[n] = t[at(line, j + R2 * n)];
@@ -225,7 +242,7 @@ __device__ __forceinline__ void line_pass(float2* __restrict__ t, const float2* 
     __syncthreads();
   } else {
     for (int w = tid; w < WORK; w += NT) {
-      const int line = w % N, g = w / N;
+      const int line = w % N, g = wave_uniform<N>(w / N);
 #pragma unroll
       for (int c = 0; c < R1 / R2; ++c) {
         const int b = g * (R1 / R2) + c;

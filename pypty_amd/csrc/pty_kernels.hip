@@ -49,7 +49,7 @@ constexpr TwIL<N> make_twil() {
   return t;
 }
 template <int N>
-__device__ __align__(16) TwIL<N> g_tw = make_twil<N>();
+__constant__ __align__(16) TwIL<N> g_tw = make_twil<N>();
 
 template <int N>
 __device__ __forceinline__ const float2* twiddles() {
@@ -455,19 +455,30 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_forward(BatchArgs a) {
   }
 }
 
-// loss, sse and dL/dI (objective.py:527-587), one workgroup per window
+// loss, sse and dL/dI (objective.py:527-587); a window is split into
+// LOSS_CHUNKS pixel ranges, one 256-thread workgroup each, with FP64 partial
+// sums per (window, chunk) reduced deterministically by k_loss_final
+template <int N>
+struct LossSplit {
+  static constexpr int CHUNKS = (N * N >= 4096) ? N * N / 1024 : 1;
+  static constexpr int PIX = N * N / CHUNKS;
+};
+
 template <int N>
 __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
   using T = Tile<N>;
+  constexpr int CH = LossSplit<N>::CHUNKS, PIX = LossSplit<N>::PIX;
   __shared__ double red[8];
-  const int b = blockIdx.x;
+  const int b = blockIdx.x / CH, ch = blockIdx.x % CH;
   const int idx = seq_index(a.seq, a.b0 + b, a.P);
   const int MK = a.M * a.K;
+  const size_t slot = (size_t(a.b0 + b) * CH + ch) * 2;
+  const int e0 = ch * PIX;
   if (idx < 0) {  // invalid sequence entry: no loss, zero gradient
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) a.coef[size_t(b) * N * N + e] = 0.f;
+    for (int e = e0 + threadIdx.x; e < e0 + PIX; e += blockDim.x) a.coef[size_t(b) * N * N + e] = 0.f;
     if (threadIdx.x == 0) {
-      a.loss_part[2 * size_t(a.b0 + b)] = 0.0;
-      a.loss_part[2 * size_t(a.b0 + b) + 1] = 0.0;
+      a.loss_part[slot] = 0.0;
+      a.loss_part[slot + 1] = 0.0;
     }
     return;
   }
@@ -475,7 +486,7 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
   const float2* psi = a.Psi + size_t(b) * MK * N * N;
   const float scale = a.loss_weight * float(N) * float(N);
   double lsum = 0.0, ssum = 0.0;
-  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+  for (int e = e0 + threadIdx.x; e < e0 + PIX; e += blockDim.x) {
     float I = 0.f;
     for (int j = 0; j < MK; ++j) I += cabs2(psi[size_t(j) * N * N + e]);
     if (MK > 1) I = I / float(MK);
@@ -503,8 +514,8 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
   const double L = block_sum(lsum, red);
   const double Ssum = block_sum(ssum, red);
   if (threadIdx.x == 0) {
-    a.loss_part[2 * size_t(a.b0 + b)] = L;
-    a.loss_part[2 * size_t(a.b0 + b) + 1] = Ssum;
+    a.loss_part[slot] = L;
+    a.loss_part[slot + 1] = Ssum;
   }
 }
 
@@ -537,24 +548,21 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_adjoint(BatchArgs a) {
       t[T::lds(e / N, e % N)] = psi[e];
     }
   }
-  const float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  // the stash slot of slice s is overwritten by this mode's window gradient
+  // conj(u_s) * g (the reference's waves_multislice *= dLoss); k_objgrad sums
+  // the modes, so no two workgroups write the same element
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
   for (int s = a.S - 1; s >= 0; --s) {
     __syncthreads();
     const size_t win = ((size_t(b) * a.S + s) * a.K + k) * N * N;
     const float2* o = a.Ob + win;
-    float2* G = a.Gbuf + win;
+    float2* u = st + size_t(s) * N * N;
 #pragma unroll 4
     for (int i = 0; i < T::PER_THREAD; ++i) {
       const int e = tid + i * T::NT;
       const int l = T::lds(e / N, e % N);
       const float2 g = t[l];
-      if (a.step_obj) {
-        const float2 gr = cmulc(g, st[size_t(s) * N * N + e]);  // conj(u) * g
-        if (a.M > 1)
-          atomic_add_c(&G[e], gr);
-        else
-          G[e] = gr;
-      }
+      if (a.step_obj) u[e] = cmulc(g, u[e]);  // conj(u) * g
       t[l] = cmulc(g, o[e]);
     }
     __syncthreads();
@@ -586,13 +594,17 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
   __shared__ float2 t[T::ELEMS];
   const int planes = a.S * a.K;
   const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
+  const int sl = plane / a.K, k = plane % a.K;
   const int tid = threadIdx.x;
-  float2* G = a.Gbuf + (size_t(b) * planes + plane) * N * N;
+  // sum over probe modes of the per-mode window gradients left in the stash
+  const size_t mstride = size_t(a.K) * a.S * N * N;
+  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.S + sl) * N * N;
 #pragma unroll 4
   for (int i = 0; i < T::PER_THREAD; ++i) {
     const int e = tid + i * T::NT;
-    t[T::lds(e / N, e % N)] = G[e];
-    if (a.scatter) G[e] = make_float2(0.f, 0.f);
+    float2 v = src[e];
+    for (int m = 1; m < a.M; ++m) v = cadd(v, src[m * mstride + e]);
+    t[T::lds(e / N, e % N)] = v;
   }
   __syncthreads();
   const float2* tw = twiddles<N>();
@@ -607,6 +619,7 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
   __syncthreads();
   tile_fft_inv<N>(t, tw);
   if (!a.scatter) {
+    float2* G = a.Gbuf + (size_t(b) * planes + plane) * N * N;
 #pragma unroll 4
     for (int i = 0; i < T::PER_THREAD; ++i) {
       const int e = tid + i * T::NT;
@@ -865,6 +878,18 @@ struct Layout {
 };
 inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
+int loss_chunks(int n) {
+  switch (n) {
+    case 16: return LossSplit<16>::CHUNKS;
+    case 32: return LossSplit<32>::CHUNKS;
+    case 48: return LossSplit<48>::CHUNKS;
+    case 64: return LossSplit<64>::CHUNKS;
+    case 96: return LossSplit<96>::CHUNKS;
+    case 128: return LossSplit<128>::CHUNKS;
+    default: return 1;
+  }
+}
+
 Layout plan_layout(const pty_problem* p) {
   Layout L{};
   const size_t n2 = size_t(p->n) * p->n;
@@ -885,11 +910,11 @@ Layout plan_layout(const pty_problem* p) {
   L.stash = take(size_t(B) * M * K * S * n2 * 8);
   L.Psi = take(size_t(B) * M * K * n2 * 8);
   L.coef = take(size_t(B) * n2 * 4);
-  L.Gbuf = take(size_t(B) * S * K * n2 * 8);
+  L.Gbuf = 0;
   L.gP = take(size_t(B) * M * K * n2 * 8);
   L.probe_part = take(size_t(G) * M * n2 * 8);
   L.pos_part = take(size_t(B) * M * 2 * 8);
-  L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8);
+  L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
   L.total = o;
   L.B = B;
   L.G = G;
@@ -942,7 +967,7 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   }
   {
     TimeScope ts(KT_LOSS, s);
-    k_loss<N><<<a.B, 256, 0, s>>>(a);
+    k_loss<N><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
   }
   if (p->step_obj || p->step_probe || p->step_pos) {
     TimeScope ts(KT_ADJOINT, s);
@@ -995,7 +1020,6 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   if (p->step_obj) {
     if (!bf->obj_grad) return PTY_ERR_ARG;
     if ((rc = zero_fill(gradp, objelems * 8, s))) return rc;
-    if ((rc = zero_fill(ws + L.Gbuf, size_t(L.B) * planes * n2 * 8, s))) return rc;
   }
   if (p->step_probe || p->step_pos) {
     if ((rc = zero_fill(ws + L.probe_part, size_t(L.G) * M * n2 * 8, s))) return rc;
@@ -1010,7 +1034,7 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.H = H; a.H0 = H0; a.mask = mask;
   a.Ob = reinterpret_cast<float2*>(ws + L.Ob); a.Pp = reinterpret_cast<float2*>(ws + L.Pp);
   a.stash = reinterpret_cast<float2*>(ws + L.stash); a.Psi = reinterpret_cast<float2*>(ws + L.Psi);
-  a.coef = reinterpret_cast<float*>(ws + L.coef); a.Gbuf = reinterpret_cast<float2*>(ws + L.Gbuf);
+  a.coef = reinterpret_cast<float*>(ws + L.coef); a.Gbuf = nullptr;
   a.gP = reinterpret_cast<float2*>(ws + L.gP); a.probe_part = reinterpret_cast<float2*>(ws + L.probe_part);
   a.pos_part = reinterpret_cast<double*>(ws + L.pos_part); a.loss_part = reinterpret_cast<double*>(ws + L.loss_part);
   a.sim = bf->sim_patterns; a.obj_grad = gradp; a.pos_grad = bf->pos_grad; a.G = L.G;
@@ -1022,7 +1046,8 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   }
   {
     TimeScope ts(KT_LOSS_FINAL, s);
-    k_loss_final<<<1, 256, 0, s>>>(a.loss_part, bf->n_seq, double(p->loss_weight), bf->loss_sse);
+    k_loss_final<<<1, 256, 0, s>>>(a.loss_part, bf->n_seq * LossSplit<N>::CHUNKS, double(p->loss_weight),
+                                   bf->loss_sse);
   }
   if (p->step_probe) {
     if (!bf->probe_grad) return PTY_ERR_ARG;
@@ -1074,7 +1099,7 @@ int ms_forward_n(const pty_problem* p, int batch, const void* probe_b, const voi
 }
 
 template <int N>
-int ms_adjoint_n(const pty_problem* p, int batch, void* g, const void* stash, const void* obj_b, const void* prop,
+int ms_adjoint_n(const pty_problem* p, int batch, void* g, void* stash, const void* obj_b, const void* prop,
                  const float* mask_nat, void* grad_b, void* tables, hipStream_t s) {
   using T = Tile<N>;
   const size_t n2 = size_t(N) * N;
@@ -1088,11 +1113,10 @@ int ms_adjoint_n(const pty_problem* p, int batch, void* g, const void* stash, co
   a.far_field = 0; a.step_obj = 1; a.scatter = 0;
   a.Psi = static_cast<float2*>(g);
   a.gP = static_cast<float2*>(g);  // each workgroup reads its own tile before overwriting it
-  a.stash = const_cast<float2*>(static_cast<const float2*>(stash));
+  a.stash = static_cast<float2*>(stash);
   a.Ob = const_cast<float2*>(static_cast<const float2*>(obj_b));
   a.Gbuf = static_cast<float2*>(grad_b);
   a.H = H; a.H0 = H0; a.mask = mask;
-  if (a.M > 1 && (rc = zero_fill(grad_b, size_t(batch) * a.S * a.K * n2 * 8, s))) return rc;
   {
     TimeScope ts(KT_ADJOINT, s);
     k_adjoint<N><<<xcd_grid(batch, a.M * a.K), T::NT, 0, s>>>(a);
@@ -1188,7 +1212,7 @@ int pty_ms_forward(const pty_problem* p, int batch, const void* probe_b, const v
   PTY_DISPATCH_N(p->n, ms_forward_n, p, batch, probe_b, obj_b, propagator, stash, exit_wave, tables, s);
 }
 
-int pty_ms_adjoint(const pty_problem* p, int batch, void* g, const void* stash, const void* obj_b,
+int pty_ms_adjoint(const pty_problem* p, int batch, void* g, void* stash, const void* obj_b,
                    const void* propagator, const float* mask, void* grad_b, void* tables, void* stream) {
   if (!p || !pty_supported_window(p->n)) return PTY_ERR_SIZE;
   if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1) return PTY_ERR_ARG;

@@ -7,7 +7,7 @@ oracle is printed beside the complex64 one.
 import numpy as np
 import pytest
 
-from tests._problems import call_loss_and_direction, make_problem, oracle_pass, reference_kwargs, rel
+from tests._problems import call_loss_and_direction, make_problem, oracle_pass, reference_kwargs, rel  # noqa: F401
 
 TOL = 1e-4
 
@@ -130,3 +130,82 @@ def test_lbfgs_vector_kernels():
     r2 = rng.standard_normal(1001).astype(np.float32)
     got = lbfgs.dots([(torch.from_numpy(r1).cuda(), torch.from_numpy(r2).cuda())])
     assert np.isclose(got[0], float(np.dot(r1.astype(np.float64), r2.astype(np.float64))), rtol=1e-9)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 2), (64, 64, 2, 1), (16, 16), (48, 48, 3)])
+def test_fourier_clean_small_planes(shape):
+    torch = _torch()
+    from pypty_amd import utils as putils
+    from oracle import ms_oracle as orc
+
+    print(f"[fourier_clean small] shape={shape}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    cut = float(np.float32(2 / 3) - np.float32(0.005))
+    ref = orc.fourier_clean(x.astype(np.complex128), cutoff=cut, fdtype=np.float64)
+    got = putils.fourier_clean(torch.from_numpy(x).cuda(), cutoff=cut).cpu().numpy()
+    print(f"  rel err {rel(got, ref):.2e}", flush=True)
+    assert got.shape == x.shape
+    assert rel(got, ref) < 3e-6
+
+
+def test_axpby_special_cases():
+    torch = _torch()
+    from pypty_amd import lbfgs
+
+    print("[axpby special cases]", flush=True)
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal((64, 64, 2, 1)) + 1j * rng.standard_normal((64, 64, 2, 1))).astype(np.complex64)
+    tx = torch.from_numpy(x).cuda()
+    y = torch.full_like(tx, float("nan"))
+    lbfgs.axpby(y, tx, 0.0, -1.5)                      # alpha = 0 never reads y
+    assert rel(y.cpu().numpy(), -1.5 * x) < 1e-7
+    lbfgs.axpby(y, None, 2.0, 0.0)                     # x = NULL
+    assert rel(y.cpu().numpy(), -3.0 * x) < 1e-7
+    z = lbfgs.lincomb(tx, 0.25)
+    assert rel(z.cpu().numpy(), 0.25 * x) < 1e-7
+    r = rng.standard_normal((10, 2)).astype(np.float32)
+    tr = torch.from_numpy(r).cuda()
+    w = lbfgs.lincomb(tr, -2.0)
+    assert rel(w.cpu().numpy(), -2.0 * r) < 1e-7
+    (d,) = lbfgs.dots([(tx, tx)])
+    assert np.isclose(d, float(np.sum(np.abs(x.astype(np.complex128)) ** 2)), rtol=1e-9)
+
+
+def test_first_bfgs_direction_matches_oracle():
+    """Cleaned gradients and the first search direction of an L-BFGS epoch (iterative.py:497-572)."""
+    torch = _torch()
+    from pypty_amd import objective, utils as putils, lbfgs
+    from oracle import ms_oracle as orc
+
+    print("[bfgs first direction]", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=True, seed=21, step_pos=False, noise=0.02)
+    cut = float(np.float32(2 / 3) - np.float32(0.005))
+    obj = orc.fourier_clean(pr["obj"], cutoff=cut).astype(np.complex64)
+    probe = orc.fourier_clean(pr["probe"], cutoff=cut).astype(np.complex64)
+    q = dict(pr, obj=obj, probe=probe)
+    ref = oracle_pass(q, np.complex64)
+    ref64 = oracle_pass(q, np.complex128)
+    assert rel(ref[2], ref64[2]) < 1e-5  # well conditioned (a flat start object is not)
+    og_ref = orc.fourier_clean(ref[2], cutoff=cut).astype(np.complex64)
+    pg_ref = orc.fourier_clean(ref[3], cutoff=cut).astype(np.complex64)
+    dev = torch.device("cuda", 0)
+    tobj = putils.fourier_clean(torch.from_numpy(pr["obj"]).to(dev), cutoff=cut)
+    tprobe = putils.fourier_clean(torch.from_numpy(pr["probe"]).to(dev), cutoff=cut)
+    print(f"  cleaned obj {rel(tobj.cpu().numpy(), obj):.2e} probe {rel(tprobe.cpu().numpy(), probe):.2e}")
+    kw = reference_kwargs(
+        this_obj=tobj, full_probe=tprobe, this_pos_array=pr["pos"], this_pos_correction=pr["corr"],
+        this_distances=np.array([pr["dz"]], np.float32), measured_array=pr["meas"], algorithm_type=pr["algorithm"],
+        this_wavelength=pr["wavelength"], this_step_probe=True, this_step_obj=True, this_step_pos_correction=False,
+        this_step_tilts=0, pixel_size_x_A=pr["px"], pixel_size_y_A=pr["px"], this_chopped_sequence=pr["seq"])
+    out = call_loss_and_direction(objective.loss_and_direction, **kw)
+    og = putils.fourier_clean(out[2], cutoff=cut)
+    pg = putils.fourier_clean(out[3], cutoff=cut)
+    e = dict(loss=abs(out[0] - ref[0]) / ref[0], og=rel(og.cpu().numpy(), og_ref), pg=rel(pg.cpu().numpy(), pg_ref))
+    u_obj = lbfgs.lincomb(og, -1.0)
+    d_val = 2 * lbfgs.dots([(u_obj, u_obj)])[0]
+    d_ref = 2 * float(np.sum(np.abs(og_ref.astype(np.complex128)) ** 2))
+    e["d_val_obj"] = abs(d_val - d_ref) / d_ref
+    print("  ", {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    for k, v in e.items():
+        assert v < 1e-4, (k, v)
