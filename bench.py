@@ -156,10 +156,14 @@ def main(argv=None):
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
                     "avg_launch_ms": round(avg_s * 1e3, 4), "windows_per_launch": args.batch}
-        tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_multislice_512x512x10_n128_m5.json")
-        if os.path.exists(tf):
+        # HBM bytes per launch of this kernel at this workload, measured with rocprofv3
+        # FETCH_SIZE / WRITE_SIZE (scaled by a plain-copy calibration) and committed under profiles/
+        tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_multislice_512x512x10_n128_m5.txt")
+        if os.path.exists(tf) and args.batch == 256 and args.scan == 64:
             with open(tf) as fh:
-                roofline["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+                for line in fh:
+                    if line.startswith("hbm_bytes_per_launch:"):
+                        roofline["traffic"] = float(line.split(":", 1)[1])
 
     if args.copy_calibration_mib:
         a = torch.empty(args.copy_calibration_mib << 20, dtype=torch.uint8, device=dev)
