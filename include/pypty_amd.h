@@ -40,9 +40,11 @@ extern "C" {
 #define PTY_FFT_FWD_SHIFT 2  /* fftshift(fft2(x))           fft.py:91  */
 #define PTY_FFT_INV_ISHIFT 3 /* ifft2(ifftshift(x))         fft.py:158 */
 
-/* loss families (objective.py:527-587) */
-#define PTY_LOSS_LSQ_SQRT 0
-#define PTY_LOSS_ML 1
+/* loss families (objective.py:527-567) */
+#define PTY_LOSS_LSQ_SQRT 0   /* (sqrt I - sqrt D)^2, the reference default */
+#define PTY_LOSS_ML 1         /* Poisson: I - D log I */
+#define PTY_LOSS_LSQ 2        /* (I - D)^2 */
+#define PTY_LOSS_LSQ_SQRT_2 3 /* 0.5 (D - I)^2 / D for D > 0.5, I elsewhere */
 
 /* object layouts */
 #define PTY_LAYOUT_REFERENCE 0
@@ -116,7 +118,7 @@ int pty_workspace_bytes(const pty_problem* p, size_t* bytes);
 
 /* One full forward + loss + adjoint pass over `sequence`:
  * replaces objective.loss_and_direction (pypty/objective.py:47-967) for
- * far-field multislice, lsq_sqrt / ml losses, object / probe / position
+ * far-field multislice, lsq_sqrt / ml / lsq / lsq_sqrt_2 losses, object / probe / position
  * gradients, single slice distance, no tilts. */
 int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* stream);
 

@@ -7,15 +7,15 @@ Arrays use the reference's own layouts:
   wavefield stash (B, N, N, S, M, K), exit wave (B, N, N, M, K).
 ``cdtype`` selects complex64 (the reference's "single" path, same op order)
 or complex128 (the FP64 oracle).  Supported subset: far field, single slice
-distance, no tilts, losses lsq_sqrt / ml, object / probe / position
-gradients.
+distance, no tilts, losses lsq_sqrt / ml / lsq / lsq_sqrt_2, object / probe /
+position gradients.
 """
 from __future__ import annotations
 
 import numpy as np
 import scipy.fft as sf
 
-LSQ_SQRT, ML = "lsq_sqrt", "ml"
+LSQ_SQRT, ML, LSQ, LSQ_SQRT_2 = "lsq_sqrt", "ml", "lsq", "lsq_sqrt_2"
 
 
 def _fdt(cdtype):
@@ -230,6 +230,21 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
             I = I.copy()
             I[zero] = 1
             dI = dI / I
+        elif algorithm == LSQ:
+            # objective.py:547-551
+            tterm = np.sum((I - meas) ** 2)
+            loss = loss + tterm
+            sse = sse + tterm
+            dI = 2 * (I - meas)
+        elif algorithm == LSQ_SQRT_2:
+            # objective.py:552-559
+            term = (meas - I) ** 2
+            sse = sse + np.sum(term)
+            nz = meas > 0.5
+            loss = loss + 0.5 * np.sum(term[nz] / meas[nz])
+            loss = loss + np.sum(I[~nz])
+            dI = np.ones_like(I)
+            dI[nz] = I[nz] / meas[nz] - 1
         else:
             raise ValueError(f"unsupported loss {algorithm}")
         if not coherent:

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libpypty_amd.so")
 
 PTY_OK = 0
 PTY_FFT_FWD, PTY_FFT_INV, PTY_FFT_FWD_SHIFT, PTY_FFT_INV_ISHIFT = 0, 1, 2, 3
-PTY_LOSS_LSQ_SQRT, PTY_LOSS_ML = 0, 1
+PTY_LOSS_LSQ_SQRT, PTY_LOSS_ML, PTY_LOSS_LSQ, PTY_LOSS_LSQ_SQRT_2 = 0, 1, 2, 3
 PTY_LAYOUT_REFERENCE, PTY_LAYOUT_PLANAR = 0, 1
 
 EXPORTED = (

@@ -498,6 +498,22 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
       ssum += double(diff) * double(diff);
       lsum += double(I) - double(d) * double(logf(I + (I == 0.f ? 1.f : 0.f)));
       l = diff / (I == 0.f ? 1.f : I);
+    } else if (a.algorithm == PTY_LOSS_LSQ) {
+      const float diff = I - d;
+      const double t2 = double(diff) * double(diff);
+      ssum += t2;
+      lsum += t2;
+      l = 2.f * diff;
+    } else if (a.algorithm == PTY_LOSS_LSQ_SQRT_2) {
+      const float r = d - I;
+      ssum += double(r) * double(r);
+      if (d > 0.5f) {
+        lsum += 0.5 * double(r) * double(r) / double(d);
+        l = I / d - 1.f;
+      } else {
+        lsum += double(I);
+        l = 1.f;
+      }
     } else {
       const float r = d - I;
       ssum += double(r) * double(r);
@@ -926,7 +942,7 @@ int validate(const pty_problem* p) {
   if (!pty_supported_window(p->n)) return PTY_ERR_SIZE;
   if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1 || p->compute_batch < 1) return PTY_ERR_ARG;
   if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
-  if (p->algorithm != PTY_LOSS_LSQ_SQRT && p->algorithm != PTY_LOSS_ML) return PTY_ERR_ARG;
+  if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_LSQ_SQRT_2) return PTY_ERR_ARG;
   return PTY_OK;
 }
 

@@ -1,8 +1,8 @@
 """Loss and gradient over all scan positions (mirror of pypty/objective.py:47-967).
 
 ``loss_and_direction`` keeps the reference's 77-argument signature and
-10-tuple result.  The far-field multislice path with the lsq_sqrt / ml
-losses and object, probe and position gradients runs entirely in the gfx950
+10-tuple result.  The far-field multislice path with the lsq_sqrt / ml / lsq /
+lsq_sqrt_2 losses and object, probe and position gradients runs entirely in the gfx950
 kernels behind ``pty_loss_and_direction``; arguments that select features
 outside that path (near field, tilts, compressed losses, static background,
 aberrations, beam current, phase plates, regularisers, data padding /
@@ -23,7 +23,8 @@ import torch
 
 from . import _lib
 
-LOSSES = {"lsq_sqrt": _lib.PTY_LOSS_LSQ_SQRT, "ml": _lib.PTY_LOSS_ML}
+LOSSES = {"lsq_sqrt": _lib.PTY_LOSS_LSQ_SQRT, "ml": _lib.PTY_LOSS_ML, "lsq": _lib.PTY_LOSS_LSQ,
+          "lsq_sqrt_2": _lib.PTY_LOSS_LSQ_SQRT_2}
 
 
 @dataclass(frozen=True)

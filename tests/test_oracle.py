@@ -122,7 +122,7 @@ def test_scatteradd_bruteforce_and_conservation():
 
 
 # ------------------------------------------------------------------ loss
-@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "ml"])
+@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "ml", "lsq", "lsq_sqrt_2"])
 def test_loss_closed_form_flat_probe(algorithm):
     """Single slice, O = 1, flat probe modes c_m: I is (1/M) sum |c_m N^2|^2 at DC, 0 elsewhere."""
     n, M = 16, 2
@@ -139,8 +139,12 @@ def test_loss_closed_form_flat_probe(algorithm):
     assert np.allclose(np.delete(I.ravel(), (n // 2) * n + n // 2), 0.0, atol=1e-9 * I0)
     if algorithm == "lsq_sqrt":
         L = (np.sqrt(I0) - 1) ** 2 + (n * n - 1)
-    else:
+    elif algorithm == "ml":
         L = I0 - np.log(I0)
+    elif algorithm == "lsq":
+        L = (I0 - 1) ** 2 + (n * n - 1)
+    else:  # lsq_sqrt_2 with D = 1 > 0.5 everywhere
+        L = 0.5 * (1 - I0) ** 2 + 0.5 * (n * n - 1)
     assert np.isclose(out[0], L, rtol=1e-10)
     assert np.isclose(out[1], (1 - I0) ** 2 + (n * n - 1), rtol=1e-10)
 
@@ -156,7 +160,7 @@ def _fd_loss(pr, what, idx, delta, eps=1e-4):
     return (L(eps * delta) - L(-eps * delta)) / (2 * eps)
 
 
-@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "ml"])
+@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "ml", "lsq", "lsq_sqrt_2"])
 def test_object_gradient_central_fd(algorithm):
     pr = make_problem(n=16, S=2, K=2, M=2, ny=40, nx=40, P=6, seed=11, algorithm=algorithm)
     pr["obj"] = pr["obj"].astype(np.complex128)
