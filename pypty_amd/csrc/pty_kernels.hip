@@ -11,8 +11,10 @@
 //   k_loss       (b)        mode-mean intensity, loss/sse (FP64), dL/dI
 //   k_adjoint    (b, m, k)  far-field adjoint, S slices backwards,   1+2(S-1) FFT
 //                           per-slice object gradient summed over modes
-//   k_objgrad    (b, s, k)  band-limit the window gradient,          2 FFT
-//                           scatter-add into the object gradient
+//   k_objgrad    (b, s, k)  sum the per-mode window gradients, band-  2 FFT
+//                           limit them
+//   k_gather_grad (tile)    add every overlapping window gradient to
+//                           each object tile (deterministic scatter-add)
 //   k_probegrad  (g, m)     probe-shift adjoint, position gradient   1 FFT
 //                           (FP64 sums), probe gradient in the spectrum
 //
@@ -634,23 +636,91 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
   }
   __syncthreads();
   tile_fft_inv<N>(t, tw);
-  if (!a.scatter) {
-    float2* G = a.Gbuf + (size_t(b) * planes + plane) * N * N;
-#pragma unroll 4
-    for (int i = 0; i < T::PER_THREAD; ++i) {
-      const int e = tid + i * T::NT;
-      G[e] = t[T::lds(e / N, e % N)];
-    }
-    return;
-  }
-  const int idx = seq_index(a.seq, a.b0 + b, a.P);
-  const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
-  if (!window_inside(y0, x0, N, a.ny, a.nx)) return;
-  float2* dst = a.obj_grad + size_t(plane) * a.ny * a.nx;
+  // band-limited window gradient: to Gbuf (pty_ms_adjoint) or, in a full pass,
+  // into the window buffer Ob (free after the adjoint), from where
+  // k_gather_grad accumulates the object gradient
+  float2* G = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
 #pragma unroll 4
   for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT, y = e / N, x = e % N;
-    atomic_add_c(&dst[size_t(y0 + y) * a.nx + (x0 + x)], t[T::lds(y, x)]);
+    const int e = tid + i * T::NT;
+    G[e] = t[T::lds(e / N, e % N)];
+  }
+}
+
+// Deterministic scatter-add of a batch's window gradients (multislice_core.py:192,
+// 206-230): one workgroup per GT x GT object tile of one plane sums, in window
+// order, every window of the batch that overlaps the tile, then adds the sum
+// to the object gradient.  Tiles are disjoint, so no atomics are needed and
+// the result does not depend on scheduling.
+constexpr int GT = 32;
+
+template <int N>
+__global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
+  __shared__ int wy0[256], wx0[256], wb[256];
+  const int tiles_x = (a.nx + GT - 1) / GT, tiles_y = (a.ny + GT - 1) / GT;
+  const int planes = a.S * a.K;
+  const int plane = blockIdx.x / (tiles_x * tiles_y);
+  const int tile = blockIdx.x % (tiles_x * tiles_y);
+  const int ty0 = (tile / tiles_x) * GT, tx0 = (tile % tiles_x) * GT;
+  const int cx = threadIdx.x % GT, ry = threadIdx.x / GT;  // 8 row groups of 4 rows
+  const int x = tx0 + cx;
+  float2 acc[GT / 8];
+#pragma unroll
+  for (int r = 0; r < GT / 8; ++r) acc[r] = make_float2(0.f, 0.f);
+  bool any = false;
+  // windows in chunks of 256: each thread tests one window, overlapping ones are
+  // compacted into LDS in window order, then all threads sum them
+  for (int c0 = 0; c0 < a.B; c0 += 256) {
+    const int b = c0 + threadIdx.x;
+    bool hit = false;
+    int y0 = 0, x0 = 0;
+    if (b < a.B) {
+      const int idx = seq_index(a.seq, a.b0 + b, a.P);
+      if (idx >= 0) {
+        y0 = a.pos[2 * idx];
+        x0 = a.pos[2 * idx + 1];
+        hit = window_inside(y0, x0, N, a.ny, a.nx) && !(y0 >= ty0 + GT || y0 + N <= ty0 || x0 >= tx0 + GT ||
+                                                        x0 + N <= tx0);
+      }
+    }
+    // order-preserving compaction: ballot per wavefront, prefix over the 4 wavefronts
+    __shared__ int wcount[4];
+    const unsigned long long m = __ballot(hit);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[wid] = __popcll(m);
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wid; ++w) off += wcount[w];
+    const int n = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    if (hit) {
+      wy0[off + before] = y0;
+      wx0[off + before] = x0;
+      wb[off + before] = b;
+    }
+    __syncthreads();
+    any = any || n > 0;
+    for (int i = 0; i < n; ++i) {
+      const int y0s = wy0[i], x0s = wx0[i];
+      const float2* G = a.Ob + (size_t(wb[i]) * planes + plane) * N * N;
+      const int wx = x - x0s;
+#pragma unroll
+      for (int r = 0; r < GT / 8; ++r) {
+        const int wy = ty0 + ry + 8 * r - y0s;
+        if (wx >= 0 && wx < N && wy >= 0 && wy < N) acc[r] = cadd(acc[r], G[wy * N + wx]);
+      }
+    }
+    __syncthreads();
+  }
+  if (!any || x >= a.nx) return;
+  float2* dst = a.obj_grad + size_t(plane) * a.ny * a.nx;
+#pragma unroll
+  for (int r = 0; r < GT / 8; ++r) {
+    const int y = ty0 + ry + 8 * r;
+    if (y < a.ny) {
+      float2* d = &dst[size_t(y) * a.nx + x];
+      *d = cadd(*d, acc[r]);
+    }
   }
 }
 
@@ -814,11 +884,11 @@ struct TimedLaunch {
 };
 enum {
   KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
-  KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_COUNT
+  KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER, KT_COUNT
 };
 const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
                                     "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
-                                    "loss_final", "zero", "scatteradd"};
+                                    "loss_final", "zero", "scatteradd", "gather_grad"};
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
@@ -990,8 +1060,13 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
     k_adjoint<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
   }
   if (p->step_obj) {
-    TimeScope ts(KT_OBJGRAD, s);
-    k_objgrad<N><<<a.B * planes, T::NT, 0, s>>>(a);
+    {
+      TimeScope ts(KT_OBJGRAD, s);
+      k_objgrad<N><<<a.B * planes, T::NT, 0, s>>>(a);
+    }
+    TimeScope ts(KT_GATHER, s);
+    const int tiles = ((a.ny + GT - 1) / GT) * ((a.nx + GT - 1) / GT);
+    k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
   }
   if (p->step_probe || (p->step_pos && a.shift)) {
     TimeScope ts(KT_PROBEGRAD, s);
