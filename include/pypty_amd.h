@@ -7,7 +7,9 @@
  * size pty_workspace_bytes() reports.  Functions return PTY_OK (0) or a
  * negative status; pty_status_string() names it.  Complex data are complex64
  * (interleaved float re, im).  Windows are n x n with n in
- * {16, 32, 48, 64, 96, 128} for the LDS-resident kernels.
+ * {16, 32, 48, 64, 96, 128} for the LDS-resident kernels; the full pass
+ * (pty_loss_and_direction) also takes n = 192 and 256, whose tiles live in
+ * workspace memory.
  *
  * Data layouts
  *   object, planar     (S, K, Ny, Nx)   plane = s*K + k

@@ -175,15 +175,24 @@ template <> struct Plan<48>  { static constexpr int R2 = 4; };
 template <> struct Plan<64>  { static constexpr int R2 = 8; };
 template <> struct Plan<96>  { static constexpr int R2 = 4; };
 template <> struct Plan<128> { static constexpr int R2 = 8; };
+template <> struct Plan<192> { static constexpr int R2 = 8; };
+template <> struct Plan<256> { static constexpr int R2 = 16; };
 
+// Windows up to 128 px live in LDS (128 x 129 x 8 B = 129 KB of the 160 KB).
+// Larger windows (192, 256) do not fit: their tile lives in a per-workgroup
+// slab of global memory (L2 / MALL resident), with the same line passes.
 template <int N>
 struct Tile {
   static constexpr int R2 = Plan<N>::R2;
   static constexpr int R1 = N / R2;
   static_assert(R1 % R2 == 0, "R2 must divide R1");
-  static constexpr int LS = N + 1;              // padded LDS row stride (elements)
+  static constexpr bool GLOBAL = N > 128;
+  static constexpr int LS = GLOBAL ? N : N + 1; // row stride (elements), padded in LDS
   static constexpr int ELEMS = N * LS;          // LDS elements per tile
-  static constexpr int NT = (N * R2 > PTY_MAX_NT) ? PTY_MAX_NT : N * R2;   // threads per tile
+  // threads per tile; global-memory tiles use 512 so each thread may hold
+  // 256 VGPRs (their radix-24 / radix-16 stages do not fit in 128)
+  static constexpr int NT_CAP = GLOBAL ? 512 : PTY_MAX_NT;
+  static constexpr int NT = (N * R2 > NT_CAP) ? NT_CAP : N * R2;
   static constexpr int PER_THREAD = N * N / NT; // pointwise elements per thread
   static_assert((N * N) % NT == 0, "tile/threads");
   __device__ __forceinline__ static int lds(int y, int x) { return y * LS + x; }
@@ -211,11 +220,17 @@ __device__ __forceinline__ void line_pass(float2* __restrict__ t, const float2* 
   using T = Tile<N>;
   constexpr int R1 = T::R1, R2 = T::R2, LS = T::LS, NT = T::NT;
   constexpr int WORK = N * R2;
+  // lanes walk across lines (conflict-free LDS, coalesced columns) except for
+  // rows of a global-memory tile, where the R2 lanes of a line are adjacent so
+  // that a wavefront reads contiguous row segments
+  constexpr bool JFAST = T::GLOBAL && ALONG_X;
   const int tid = threadIdx.x;
   auto at = [](int line, int pos) { return ALONG_X ? line * LS + pos : pos * LS + line; };
+  auto line_of = [](int w) { return JFAST ? w / R2 : w % N; };
+  auto part_of = [](int w) { return JFAST ? w % R2 : wave_uniform<N>(w / N); };
   if constexpr (!INV) {
     for (int w = tid; w < WORK; w += NT) {
-      const int line = w % N, j = wave_uniform<N>(w / N);
+      const int line = line_of(w), j = part_of(w);
       float2 v[R1];
 #pragma unroll
       for (int n = 0; n < R1; ++n) v[n] = t[at(line, j + R2 * n)];
@@ -227,7 +242,7 @@ __device__ __forceinline__ void line_pass(float2* __restrict__ t, const float2* 
     }
     __syncthreads();
     for (int w = tid; w < WORK; w += NT) {
-      const int line = w % N, g = w / N;
+      const int line = line_of(w), g = part_of(w);
 #pragma unroll
       for (int c = 0; c < R1 / R2; ++c) {
         const int k1 = g * (R1 / R2) + c;
@@ -242,7 +257,7 @@ __device__ __forceinline__ void line_pass(float2* __restrict__ t, const float2* 
     __syncthreads();
   } else {
     for (int w = tid; w < WORK; w += NT) {
-      const int line = w % N, g = wave_uniform<N>(w / N);
+      const int line = line_of(w), g = part_of(w);
 #pragma unroll
       for (int c = 0; c < R1 / R2; ++c) {
         const int b = g * (R1 / R2) + c;
@@ -258,7 +273,7 @@ __device__ __forceinline__ void line_pass(float2* __restrict__ t, const float2* 
     }
     __syncthreads();
     for (int w = tid; w < WORK; w += NT) {
-      const int line = w % N, j = w / N;
+      const int line = line_of(w), j = part_of(w);
       float2 v[R1];
 #pragma unroll
       for (int n = 0; n < R1; ++n) v[n] = t[at(line, j + R2 * n)];

@@ -23,6 +23,7 @@
 // gradient accumulator in that XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -59,6 +60,11 @@ __device__ __forceinline__ const float2* twiddles() {
 }
 
 // ------------------------------------------------------------- helpers
+// The window tile of a workgroup: LDS up to 128 px, else its own slab of the
+// global scratch buffer (one tile per workgroup of the launch).
+#define PTY_TILE(T, t, scratch)                                                 \
+  __shared__ float2 t##_lds[T::GLOBAL ? 1 : T::ELEMS];                          \
+  float2* t = T::GLOBAL ? (scratch) + size_t(blockIdx.x) * T::ELEMS : t##_lds
 // numpy.fft.fftfreq(n, d)[f] rounded to float32 (computed in double as numpy does)
 __device__ __forceinline__ float fftfreq_f(int f, int n, float d) {
   int k = (2 * f < n) ? f : f - n;
@@ -295,6 +301,7 @@ struct BatchArgs {
   double* pos_part;         // (B, M, 2)
   double* loss_part;        // (n_seq, 2)
   float* sim;               // (P, N, N) or null
+  float2* scratch;          // tiles of windows larger than 128 px (one per workgroup)
   // outputs
   float2* obj_grad;         // planar
   float* pos_grad;          // (P, 2)
@@ -304,7 +311,7 @@ struct BatchArgs {
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_patch(BatchArgs a) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, a.scratch);
   const int planes = a.S * a.K;
   const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
   const int idx = seq_index(a.seq, a.b0 + b, a.P);
@@ -338,9 +345,10 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_patch(BatchArgs a) {
 
 // spectrum of each probe mode, position order, unnormalised
 template <int N>
-__global__ __launch_bounds__(Tile<N>::NT) void k_fprobe(const float2* __restrict__ probe, float2* __restrict__ fprobe) {
+__global__ __launch_bounds__(Tile<N>::NT) void k_fprobe(const float2* __restrict__ probe, float2* __restrict__ fprobe,
+                                                        float2* __restrict__ scratch) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, scratch);
   const int m = blockIdx.x, tid = threadIdx.x;
   const float2* src = probe + size_t(m) * N * N;
 #pragma unroll 4
@@ -374,7 +382,7 @@ __device__ __forceinline__ float2 shift_kernel(int py, int px, float dy, float d
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_probe_shift(BatchArgs a) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, a.scratch);
   const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
   const int idx = seq_index(a.seq, a.b0 + b, a.P);
   const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
@@ -400,7 +408,7 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_probe_shift(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_forward(BatchArgs a) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, a.scratch);
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
@@ -540,7 +548,7 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_adjoint(BatchArgs a) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, a.scratch);
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
@@ -609,7 +617,7 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_adjoint(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, a.scratch);
   const int planes = a.S * a.K;
   const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
   const int sl = plane / a.K, k = plane % a.K;
@@ -732,7 +740,7 @@ __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_probegrad(BatchArgs a) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, a.scratch);
   __shared__ double red[16];
   const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
   const int tid = threadIdx.x;
@@ -810,9 +818,10 @@ __global__ void k_pos_final(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_probe_final(const float2* __restrict__ part, int G, int M,
                                                              int spectral, const float* __restrict__ mask,
-                                                             float2* __restrict__ probe_grad) {
+                                                             float2* __restrict__ probe_grad,
+                                                             float2* __restrict__ scratch) {
   using T = Tile<N>;
-  __shared__ float2 t[T::ELEMS];
+  PTY_TILE(T, t, scratch);
   const int m = blockIdx.x, tid = threadIdx.x;
   const float2* tw = twiddles<N>();
   const float s = 1.0f / float(N * N);
@@ -925,6 +934,19 @@ inline int check_launch() {
   return PTY_OK;
 }
 
+#define PTY_DISPATCH_FUSED(n, FN, ...)      \
+  switch (n) {                              \
+    case 16: return FN<16>(__VA_ARGS__);    \
+    case 32: return FN<32>(__VA_ARGS__);    \
+    case 48: return FN<48>(__VA_ARGS__);    \
+    case 64: return FN<64>(__VA_ARGS__);    \
+    case 96: return FN<96>(__VA_ARGS__);    \
+    case 128: return FN<128>(__VA_ARGS__);  \
+    case 192: return FN<192>(__VA_ARGS__);  \
+    case 256: return FN<256>(__VA_ARGS__);  \
+    default: return PTY_ERR_SIZE;           \
+  }
+
 #define PTY_DISPATCH_N(n, FN, ...)          \
   switch (n) {                              \
     case 16: return FN<16>(__VA_ARGS__);    \
@@ -959,7 +981,7 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 // ---------------------------------------------------------- workspace
 struct Layout {
   size_t H, H0, mask, fprobe, objp, gradp, Ob, Pp, stash, Psi, coef, Gbuf, gP, probe_part, pos_part, loss_part,
-      total;
+      scratch, total;
   int B, G;
 };
 inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
@@ -972,14 +994,29 @@ int loss_chunks(int n) {
     case 64: return LossSplit<64>::CHUNKS;
     case 96: return LossSplit<96>::CHUNKS;
     case 128: return LossSplit<128>::CHUNKS;
+    case 192: return LossSplit<192>::CHUNKS;
+    case 256: return LossSplit<256>::CHUNKS;
     default: return 1;
   }
+}
+
+// windows handled by one launch; windows larger than 128 px keep one tile per
+// workgroup in global scratch, so their batch is capped to bound that buffer
+int batch_windows(const pty_problem* p) {
+  int B = p->compute_batch;
+  if (p->n > 128) {
+    const size_t tile = size_t(p->n) * p->n * 8;
+    const size_t per_window = size_t(std::max(p->n_slices * p->n_obj_modes, p->n_probe_modes * p->n_obj_modes)) * tile;
+    const int cap = int(std::max<size_t>(8, (size_t(2) << 30) / per_window));
+    B = std::min(B, cap);
+  }
+  return B;
 }
 
 Layout plan_layout(const pty_problem* p) {
   Layout L{};
   const size_t n2 = size_t(p->n) * p->n;
-  const int B = p->compute_batch;
+  const int B = batch_windows(p);
   const int S = p->n_slices, K = p->n_obj_modes, M = p->n_probe_modes;
   const int G = std::min(B, 64);
   const size_t objsz = size_t(p->obj_ny) * p->obj_nx * S * K * 8;
@@ -1001,6 +1038,12 @@ Layout plan_layout(const pty_problem* p) {
   L.probe_part = take(size_t(G) * M * n2 * 8);
   L.pos_part = take(size_t(B) * M * 2 * 8);
   L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
+  if (p->n > 128) {
+    // one tile per workgroup of the largest launch
+    const size_t grid = std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K, size_t(G) * M,
+                                          size_t(M)});
+    L.scratch = take(grid * n2 * 8);
+  }
   L.total = o;
   L.B = B;
   L.G = G;
@@ -1009,7 +1052,7 @@ Layout plan_layout(const pty_problem* p) {
 
 int validate(const pty_problem* p) {
   if (!p) return PTY_ERR_ARG;
-  if (!pty_supported_window(p->n)) return PTY_ERR_SIZE;
+  if (!pty_supported_window(p->n) && p->n != 192 && p->n != 256) return PTY_ERR_SIZE;
   if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1 || p->compute_batch < 1) return PTY_ERR_ARG;
   if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
   if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_LSQ_SQRT_2) return PTY_ERR_ARG;
@@ -1096,7 +1139,8 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   float2* fprobe = reinterpret_cast<float2*>(ws + L.fprobe);
   {
     TimeScope ts(KT_FPROBE, s);
-    k_fprobe<N><<<M, T::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe);
+    k_fprobe<N><<<M, T::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
+                                     reinterpret_cast<float2*>(ws + L.scratch));
   }
   const float2* objp = static_cast<const float2*>(bf->obj);
   float2* gradp = static_cast<float2*>(bf->obj_grad);
@@ -1129,6 +1173,7 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.gP = reinterpret_cast<float2*>(ws + L.gP); a.probe_part = reinterpret_cast<float2*>(ws + L.probe_part);
   a.pos_part = reinterpret_cast<double*>(ws + L.pos_part); a.loss_part = reinterpret_cast<double*>(ws + L.loss_part);
   a.sim = bf->sim_patterns; a.obj_grad = gradp; a.pos_grad = bf->pos_grad; a.G = L.G;
+  a.scratch = L.scratch ? reinterpret_cast<float2*>(ws + L.scratch) : nullptr;
   for (int b0 = 0; b0 < bf->n_seq; b0 += L.B) {
     a.b0 = b0;
     a.B = std::min(L.B, bf->n_seq - b0);
@@ -1144,7 +1189,7 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     if (!bf->probe_grad) return PTY_ERR_ARG;
     TimeScope ts(KT_PROBE_FINAL, s);
     k_probe_final<N><<<M, T::NT, 0, s>>>(a.probe_part, L.G, M, p->shift_active, mask,
-                                          static_cast<float2*>(bf->probe_grad));
+                                          static_cast<float2*>(bf->probe_grad), a.scratch);
   }
   if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE) {
     TimeScope ts(KT_LAYOUT, s);
@@ -1285,7 +1330,7 @@ int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* str
   if ((p->shift_active || p->step_pos) && !b->corrections) return PTY_ERR_ARG;
   if (p->step_pos && !b->pos_grad) return PTY_ERR_ARG;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  PTY_DISPATCH_N(p->n, loss_and_direction_n, p, b, s);
+  PTY_DISPATCH_FUSED(p->n, loss_and_direction_n, p, b, s);
 }
 
 int pty_tables_bytes(int n, size_t* bytes) {

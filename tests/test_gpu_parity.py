@@ -45,6 +45,8 @@ def test_fft_modes_match_numpy(n, mode):
     dict(n=128, S=3, K=1, M=2, shift=True, algorithm="lsq_sqrt", ny=192, nx=192, P=6),
     dict(n=64, S=2, K=1, M=3, shift=True, algorithm="lsq", ny=96, nx=96, P=7),
     dict(n=32, S=3, K=2, M=1, shift=True, algorithm="lsq_sqrt_2", P=9),
+    dict(n=192, S=3, K=1, M=2, shift=True, algorithm="lsq_sqrt", ny=256, nx=256, P=5),
+    dict(n=256, S=2, K=1, M=2, shift=True, algorithm="ml", ny=320, nx=320, P=4),
 ])
 def test_loss_and_direction_matches_oracle(case):
     torch = _torch()
