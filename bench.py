@@ -1,11 +1,13 @@
 """Throughput of the multislice forward + loss + adjoint pass on one MI355X.
 
 One step = one full loss-and-gradient pass (object, probe and position
-gradients, probe sub-pixel shift active) over every scan position of the
-multislice workload: object 512x512x10 (dz 2 A, 0.2 A pixels), 128 px probe
-with 5 incoherent modes, 64x64 raster = 4096 positions, 256 windows per
-launch, Poisson data at 1e4 e-.  The loss/sse readback of every step is
-inside the timed region.
+gradients, probe sub-pixel shift active) over every scan position.  The
+default workload is the multislice configuration: object 512x512x10 (dz 2 A,
+0.2 A pixels), 128 px probe with 5 incoherent modes, 64x64 raster = 4096
+positions, 256 windows per launch, Poisson data at 1e4 e-.  --config ragged
+runs eight mixed-shape datasets (windows 96-192 px) one after the other per
+step; --config thick the 1536x1536x40 object with a 256 px, 10-mode probe.
+The loss/sse readback of every pass is inside the timed region.
 
 Prints one JSON line: patterns/s (= positions x steps / seconds), the
 roofline of the dominant kernel (FP32 flops per launch / HIP-event average
@@ -61,22 +63,72 @@ def pass_bytes(n, S, K):
     return 3 * S * K * n * n * 8 + 4 * n * n
 
 
-def cpu_baseline(ds, n_sample, workers):
-    """Time the oracle (PyPty's numpy op sequence, complex64, scipy.fft) on n_sample positions."""
+def cpu_baseline(datasets, n_sample, workers):
+    """Time the oracle (PyPty's numpy op sequence, complex64, scipy.fft) on the first n_sample
+    positions of each dataset; returns (patterns/s, seconds, positions timed)."""
     import numpy as np
     import scipy.fft
 
     from oracle import ms_oracle as orc
 
-    seq = np.arange(min(n_sample, ds.positions.shape[0]))
+    total_t, total_p = 0.0, 0
     with scipy.fft.set_workers(workers):
-        t0 = time.perf_counter()
-        orc.loss_and_direction(ds.obj, ds.probe, ds.positions, ds.corrections, ds.measured, seq, dz=ds.dz_A,
-                               wavelength=ds.wavelength_A, pixel_size_x=ds.pixel_A, pixel_size_y=ds.pixel_A,
-                               algorithm="lsq_sqrt", step_obj=True, step_probe=True, step_pos=True,
-                               compute_batch=32, cdtype=np.complex64)
-        dt = time.perf_counter() - t0
-    return len(seq) / dt, dt
+        for ds in datasets:
+            seq = np.arange(min(n_sample, ds.positions.shape[0]))
+            t0 = time.perf_counter()
+            orc.loss_and_direction(ds.obj, ds.probe, ds.positions, ds.corrections, ds.measured, seq, dz=ds.dz_A,
+                                   wavelength=ds.wavelength_A, pixel_size_x=ds.pixel_A, pixel_size_y=ds.pixel_A,
+                                   algorithm="lsq_sqrt", step_obj=True, step_probe=True, step_pos=True,
+                                   compute_batch=16, cdtype=np.complex64)
+            total_t += time.perf_counter() - t0
+            total_p += len(seq)
+    return total_p / total_t, total_t, total_p
+
+
+WORKLOADS = {
+    "multislice": "multislice object 512x512x10 dz2A, probe 128px x5 modes, 64x64 scan (4096 positions), "
+                  "batch 256, lsq_sqrt, object+probe+position gradients",
+    "ragged": "8 seeded datasets, windows {96,128,192} px, 4-16 slices, 1-5 probe modes, scans 32x32-96x96, "
+              "dose 1e3-1e5, one pass over every dataset per step",
+    "thick": "thick object 1536x1536x40 dz1A, probe 256px x10 modes, 64x64 scan (4096 positions), "
+             "object+probe+position gradients",
+}
+
+
+class Job:
+    """One dataset resident on the device with its engine and gradient buffers."""
+
+    def __init__(self, ds, batch, dev):
+        import numpy as np
+        import torch
+
+        from pypty_amd.objective import PassConfig, PtychoPass
+
+        ny, nx, S, K = ds.obj.shape
+        n, _, M = ds.probe.shape
+        self.P = ds.positions.shape[0]
+        self.shape = (n, S, M, K)
+        # reconstruction starts from a flat object; data come from the true one
+        self.obj = torch.ones((ny, nx, S, K), dtype=torch.complex64, device=dev)
+        self.probe = torch.from_numpy(np.ascontiguousarray(ds.probe.transpose(2, 0, 1))).to(dev)
+        self.pos = torch.from_numpy(ds.positions).to(dev)
+        self.corr = torch.from_numpy(ds.corrections).to(dev)
+        self.meas = torch.from_numpy(ds.measured).to(dev)
+        self.seq = torch.arange(self.P, dtype=torch.int32, device=dev)
+        self.cfg = PassConfig(n=n, n_slices=S, n_obj_modes=K, n_probe_modes=M, obj_ny=ny, obj_nx=nx,
+                              n_positions=self.P, compute_batch=batch, algorithm="lsq_sqrt", planar=False,
+                              step_obj=True, step_probe=True, step_pos=True, shift_active=True,
+                              slice_distance=ds.dz_A, wavelength=ds.wavelength_A, pixel_size_x=ds.pixel_A,
+                              pixel_size_y=ds.pixel_A)
+        self.dev = dev
+        self.engine = PtychoPass(self.cfg, dev)
+        self.og = torch.empty_like(self.obj)
+        self.pg = torch.empty_like(self.probe)
+        self.posg = torch.zeros_like(self.corr)
+
+    def step(self):
+        return self.engine.run(self.obj, self.probe, self.pos, self.corr, self.meas, self.seq, obj_grad=self.og,
+                               probe_grad=self.pg, pos_grad=self.posg)
 
 
 def main(argv=None):
@@ -85,8 +137,10 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--scan", type=int, default=64, help="scan grid side (64 -> 4096 positions)")
-    ap.add_argument("--cpu-sample", type=int, default=160, help="positions timed on the CPU (0: skip)")
+    ap.add_argument("--config", choices=sorted(WORKLOADS), default="multislice")
+    ap.add_argument("--scan", type=int, default=64, help="scan grid side of the single-dataset workloads")
+    ap.add_argument("--cpu-sample", type=int, default=160,
+                    help="positions timed on the CPU (per dataset for the ragged workload; 0: skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for rocprofv3 counter scaling)")
@@ -95,37 +149,27 @@ def main(argv=None):
         print("bench.py: this build measures one GPU only (one-GPU boxes); run with --gpus 1", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
 
     from pypty_amd import _lib, synth
-    from pypty_amd.objective import PassConfig, PtychoPass
 
     dev = torch.device("cuda", 0)
     t_gen = time.perf_counter()
-    ds = synth.config1(dev, n_scan=args.scan)
+    if args.config == "multislice":
+        datasets = [synth.config1(dev, n_scan=args.scan)]
+    elif args.config == "ragged":
+        datasets = synth.ragged_datasets(dev)
+    else:
+        datasets = [synth.config4(dev, n_scan=args.scan)]
     t_gen = time.perf_counter() - t_gen
-    ny, nx, S, K = ds.obj.shape
-    n, _, M = ds.probe.shape
-    P = ds.positions.shape[0]
-    # reconstruction starts from a flat object; data come from the true one
-    obj = torch.ones((ny, nx, S, K), dtype=torch.complex64, device=dev)
-    probe = torch.from_numpy(np.ascontiguousarray(ds.probe.transpose(2, 0, 1))).to(dev)
-    pos = torch.from_numpy(ds.positions).to(dev)
-    corr = torch.from_numpy(ds.corrections).to(dev)
-    meas = torch.from_numpy(ds.measured).to(dev)
-    seq = torch.arange(P, dtype=torch.int32, device=dev)
-    cfg = PassConfig(n=n, n_slices=S, n_obj_modes=K, n_probe_modes=M, obj_ny=ny, obj_nx=nx, n_positions=P,
-                     compute_batch=args.batch, algorithm="lsq_sqrt", planar=False, step_obj=True, step_probe=True,
-                     step_pos=True, shift_active=True, slice_distance=ds.dz_A, wavelength=ds.wavelength_A,
-                     pixel_size_x=ds.pixel_A, pixel_size_y=ds.pixel_A)
-    eng = PtychoPass(cfg, dev)
-    og = torch.empty_like(obj)
-    pg = torch.empty_like(probe)
-    posg = torch.zeros_like(corr)
+    jobs = [Job(ds, args.batch, dev) for ds in datasets]
+    P_total = sum(j.P for j in jobs)
 
     def step():
-        return eng.run(obj, probe, pos, corr, meas, seq, obj_grad=og, probe_grad=pg, pos_grad=posg)
+        out = None
+        for j in jobs:
+            out = j.step()
+        return out
 
     for _ in range(args.warmup):
         step()
@@ -135,7 +179,7 @@ def main(argv=None):
         loss, sse = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    value = P * args.steps / elapsed
+    value = P_total * args.steps / elapsed
 
     kernels = {}
     roofline = None
@@ -151,15 +195,17 @@ def main(argv=None):
         dom = max(kernels, key=lambda k: kernels[k][0])
         tot_ms, launches = kernels[dom]
         avg_s = tot_ms / launches / 1e3
-        per_launch = kernel_flops(dom, n, S, M, K) * args.batch
+        # algorithmic flops of all launches of the dominant kernel in one step, per launch
+        flops_step = sum(kernel_flops(dom, *j.shape) * j.P for j in jobs)
+        per_launch = flops_step * args.steps / launches
         achieved = per_launch / avg_s / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
-                    "avg_launch_ms": round(avg_s * 1e3, 4), "windows_per_launch": args.batch}
+                    "avg_launch_ms": round(avg_s * 1e3, 4), "launches_per_step": launches // args.steps}
         # HBM bytes per launch of this kernel at this workload, measured with rocprofv3
         # FETCH_SIZE / WRITE_SIZE (scaled by a plain-copy calibration) and committed under profiles/
         tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_multislice_512x512x10_n128_m5.txt")
-        if os.path.exists(tf) and args.batch == 256 and args.scan == 64:
+        if args.config == "multislice" and os.path.exists(tf) and args.batch == 256 and args.scan == 64:
             with open(tf) as fh:
                 for line in fh:
                     if line.startswith("hbm_bytes_per_launch:"):
@@ -173,24 +219,29 @@ def main(argv=None):
 
     cpu = None
     if args.cpu_sample > 0:
-        v1, dt1 = cpu_baseline(ds, args.cpu_sample, 1)
+        n_cpu = args.cpu_sample if args.config != "ragged" else max(1, args.cpu_sample // 10)
+        if args.config == "thick":
+            n_cpu = max(1, args.cpu_sample // 20)
+        v1, dt1, npos = cpu_baseline(datasets, n_cpu, 1)
         cpu = {"value": round(v1, 3), "unit": "patterns/s", "cores": 1, "kind": "port",
-               "sample": f"oracle complex64 numpy/scipy.fft path (PyPty op order), first {min(args.cpu_sample, P)} "
-                         f"positions of the same workload, scipy workers=1, {dt1:.1f} s"}
+               "sample": f"oracle complex64 numpy/scipy.fft path (PyPty op order), first {n_cpu} positions of "
+                         f"each of the {len(datasets)} dataset(s) ({npos} patterns), scipy workers=1, {dt1:.1f} s"}
 
-    flops_pat = pass_flops(n, S, M, K, True)
+    flops_step = sum(pass_flops(*j.shape, True) * j.P for j in jobs)
+    bytes_step = sum(pass_bytes(j.shape[0], j.shape[1], j.shape[3]) * j.P for j in jobs)
+    cfg_info = {"workload": WORKLOADS[args.config], "positions": P_total, "compute_batch": args.batch,
+                "datasets": [{"n": j.shape[0], "slices": j.shape[1], "probe_modes": j.shape[2],
+                              "obj_modes": j.shape[3], "positions": j.P} for j in jobs]}
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "patterns/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "complex64", "data": "synthetic",
-        "config": {"workload": "multislice object 512x512x10 dz2A, probe 128px x5 modes, 64x64 scan "
-                               "(4096 positions), batch 256, lsq_sqrt, object+probe+position gradients",
-                   "n": n, "slices": S, "probe_modes": M, "obj_modes": K, "positions": P,
-                   "compute_batch": args.batch},
-        "roofline": roofline, "cpu_baseline": cpu,
-        "pass": {"loss": loss, "sse": sse, "alg_gflop_per_pattern": round(flops_pat / 1e9, 4),
-                 "alg_tflops": round(flops_pat * value / 1e12, 3),
-                 "hbm_frac_alg": round(pass_bytes(n, S, K) * value / (HBM_PEAK_GBS * 1e9), 4),
+        "config": cfg_info, "roofline": roofline, "cpu_baseline": cpu,
+        "pass": {"loss_last_dataset": loss, "sse_last_dataset": sse,
+                 "alg_gflop_per_pattern": round(flops_step / P_total / 1e9, 4),
+                 "alg_tflops": round(flops_step * args.steps / elapsed / 1e12, 3),
+                 "hbm_frac_alg": round(bytes_step * args.steps / elapsed / (HBM_PEAK_GBS * 1e9), 4),
+                 "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
                  "data_gen_s": round(t_gen, 1)},
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kernels.items()},
     }

@@ -69,9 +69,18 @@ class PassConfig:
 
 
 class PtychoPass:
-    """One forward + loss + adjoint pass over a sequence of positions, on one HIP device."""
+    """One forward + loss + adjoint pass over a sequence of positions, on one HIP device.
+
+    Workspaces are cached per (device, size) for the life of the process, so
+    repeated passes of the same problem shapes allocate nothing; passes on one
+    stream may share a workspace.  ``release_workspaces()`` drops the cache.
+    """
 
     _workspaces: dict = {}
+
+    @classmethod
+    def release_workspaces(cls):
+        cls._workspaces.clear()
 
     def __init__(self, cfg: PassConfig, device=None):
         self.cfg = cfg
@@ -84,7 +93,6 @@ class PtychoPass:
         key = (self.device, self.workspace_bytes)
         ws = PtychoPass._workspaces.get(key)
         if ws is None:
-            PtychoPass._workspaces.clear()  # keep a single cached workspace per process
             ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
             PtychoPass._workspaces[key] = ws
         self.workspace = ws

@@ -3,8 +3,8 @@
 Objects, probes and scan grids are built on the host with numpy (setup, not
 per-pattern work); diffraction patterns are simulated on the GPU by the same
 forward kernels the reconstruction uses (``sim_patterns`` output of
-``pty_loss_and_direction``) and Poisson noise is drawn from a seeded numpy
-generator.  Conventions follow PyPty: electron wavelength from
+``pty_loss_and_direction``) and Poisson counts are drawn on the device from a
+seeded generator.  Conventions follow PyPty: electron wavelength from
 pypty/utils.py:1721, defocus phase exp(-i*pi*df*lambda*q^2) from
 pypty/utils.py:1525, positions split into an integer corner and a sub-pixel
 correction as in pypty/utils.py:1742-1746, intensity = mean over modes.
@@ -157,10 +157,12 @@ def simulate(obj, probe, positions, corrections, pixel_A, dz_A, wavelength_A, do
             torch.from_numpy(positions).to(dev), torch.from_numpy(corrections).to(dev),
             torch.zeros((P, n, n), dtype=torch.float32, device=dev), torch.arange(P, dtype=torch.int32, device=dev),
             sim_patterns=sim)
-    I = sim.cpu().numpy().astype(np.float64)
-    I *= dose / np.maximum(I.sum(axis=(1, 2), keepdims=True), 1e-30)
-    rng = np.random.default_rng(seed)
-    return rng.poisson(I).astype(np.float32)
+    # scale to the dose and draw Poisson counts on the device (seeded generator)
+    I = sim.double()
+    I *= dose / torch.clamp(I.sum(dim=(1, 2), keepdim=True), min=1e-30)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    return torch.poisson(I, generator=gen).float().cpu().numpy()
 
 
 def config1(device=None, n_scan=64, seed=1001):
@@ -182,6 +184,43 @@ def config0(device=None, seed=1000):
     obj = hex_lattice_object(256, 256, px, 2.46, 0.3, 0.5, seed)
     probe = make_probe(64, px, 300.0, 20.0, 100.0, 1, 0.0, seed + 100)
     pos = raster_scan(16, 16, 2.0, origin_px=(96.0, 96.0), seed=seed + 200)
+    ints, corr = split_positions(pos)
+    lam = electron_wavelength(300.0)
+    meas = simulate(obj, probe, ints, corr, px, dz, lam, 1e4, seed + 300, device)
+    return Dataset(obj, probe, ints, corr, meas, px, dz, lam, 1e4)
+
+
+def ragged_datasets(device=None, seed=2000, n_datasets=8):
+    """Eight independent seeded datasets of mixed shapes: windows {96, 128, 192} px, 4-16 slices,
+    1-5 probe modes, raster scans 32x32 .. 96x96 (step 6 px), dose log-uniform in 1e3 .. 1e5 e-."""
+    out = []
+    px, dz = 0.2, 2.0
+    lam = electron_wavelength(300.0)
+    for i in range(n_datasets):
+        rng = np.random.default_rng(seed + i)
+        n = int(rng.choice([96, 128, 192]))
+        S = int(rng.integers(4, 17))
+        M = int(rng.integers(1, 6))
+        scan = int(rng.integers(32, 97))
+        dose = float(10 ** rng.uniform(3, 5))
+        step = 6.0
+        side = int(np.ceil(((scan - 1) * step + n + 4) / 32.0) * 32)
+        obj = random_atom_object(side, side, S, px, dz, 0.1, 0.3, 0.5, seed + 100 + i)
+        probe = make_probe(n, px, 300.0, 20.0, 100.0, M, 20.0, seed + 200 + i)
+        pos = raster_scan(scan, scan, step, origin_px=(1.0, 1.0), jitter_px=0.5, seed=seed + 300 + i)
+        ints, corr = split_positions(pos)
+        meas = simulate(obj, probe, ints, corr, px, dz, lam, dose, seed + 400 + i, device)
+        out.append(Dataset(obj, probe, ints, corr, meas, px, dz, lam, dose))
+    return out
+
+
+def config4(device=None, seed=4000, n_scan=128):
+    """Thick-sample config: 1536x1536x40 object (1 A slices), 256 px probe with 10 modes,
+    128x128 raster (step 10 px), 1e4 e- per pattern."""
+    px, dz = 0.2, 1.0
+    obj = random_atom_object(1536, 1536, 40, px, dz, 0.1, 0.3, 0.5, seed)
+    probe = make_probe(256, px, 300.0, 20.0, 100.0, 10, 20.0, seed + 100)
+    pos = raster_scan(n_scan, n_scan, 10.0, origin_px=(1.0, 1.0), jitter_px=0.5, seed=seed + 200)
     ints, corr = split_positions(pos)
     lam = electron_wavelength(300.0)
     meas = simulate(obj, probe, ints, corr, px, dz, lam, 1e4, seed + 300, device)
