@@ -298,3 +298,207 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     if update_pos:
         positions_correction = new_pc
     return total_loss, this_sse, cc, actual_step, count, d_value, new_d_value, warnings, counter_failed_epochs
+
+
+# ----------------------------------------------------------------- driver
+def _per_epoch(v, epoch):
+    """PyPty's "lambda-type" parameters (pypty/utils.py:2260-2285): callables of the epoch.  Strings are
+    not evaluated (the reference eval()s them); pass a callable instead."""
+    if isinstance(v, str):
+        raise TypeError("pypty_amd.run: string-valued per-epoch parameters are not evaluated; pass a callable")
+    return v(epoch) if callable(v) else v
+
+
+def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos):
+    """co.npy / cp.npy / cg.npy as pypty/utils.py:2085-2112 writes them."""
+    import os
+
+    if not folder:
+        return
+    os.makedirs(folder, exist_ok=True)
+    if save_obj:
+        np.save(os.path.join(folder, "co.npy"), o.cpu().numpy())
+    if save_probe:
+        np.save(os.path.join(folder, "cp.npy"), p.cpu().numpy())
+    if save_pos:
+        np.save(os.path.join(folder, "cg.npy"), (pos.double() + corr.double()).cpu().numpy())
+
+
+LOSS_CSV_FIELDS = ["epoch", "time / s", "loss", "sse", "initial step", "matching step", "N linesearch iterations",
+                   "dir. derivative", "new dir. derivative", "Constraints contribution", "Free GiB", "Total GiB",
+                   "Warnings"]
+
+
+def _log_loss(folder, row):
+    """Append one row of loss.csv in the reference's 13-column layout (pypty/utils.py:1877-1940)."""
+    import csv
+    import os
+
+    if not folder:
+        return
+    os.makedirs(folder, exist_ok=True)
+    with open(os.path.join(folder, "loss.csv"), mode="a", newline="") as fh:
+        csv.DictWriter(fh, fieldnames=LOSS_CSV_FIELDS).writerow(row)
+
+
+def run(pypty_params):
+    """Iterative reconstruction (mirror of pypty.iterative.run, pypty/iterative.py:27-436).
+
+    Reads the keys of the far-field multislice path with the reference's
+    defaults: dataset (array, or data_path to a .npy), probe (N, N, M),
+    positions (P, 2) in pixels, obj, acc_voltage, pixel_size_x_A/y_A,
+    slice_distances, num_slices, algorithm, epoch_max, epoch_prev, update_obj,
+    update_probe, update_probe_pos (values or callables of the epoch),
+    compute_batch, damping_cutoff_multislice, update_extra_cut, smooth_rolloff,
+    wolfe_c1_constant, wolfe_c2_constant, max_count, reduce_factor, optimism,
+    min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
+    extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
+    save_loss_log, print_flag.  Keys selecting features outside the path raise
+    NotImplementedError through bfgs_update / loss_and_direction.  Writes
+    co.npy, cp.npy, cg.npy and loss.csv like the reference; returns the final
+    object, probe, positions and per-epoch losses (the reference returns None).
+    """
+    import os
+    import time
+
+    global obj, probe, positions, positions_correction, tilts, tilts_correction, history_bfgs
+    get = pypty_params.get
+    dev = torch.device("cuda", torch.cuda.current_device())
+    dataset = get("dataset", None)
+    if dataset is None:
+        path = get("data_path", "")
+        if not path.endswith(".npy"):
+            raise NotImplementedError("pypty_amd.run: data_path must be a .npy file (HDF5 is not supported)")
+        dataset = np.load(path)
+    dataset = np.asarray(dataset, dtype=np.float32)
+    if dataset.ndim == 4:
+        dataset = dataset.reshape(dataset.shape[0] * dataset.shape[1], dataset.shape[2], dataset.shape[3])
+    if get("probe", None) is None:
+        raise NotImplementedError("pypty_amd.run: an initial probe is required (probe synthesis is setup code)")
+    pr = np.asarray(get("probe"), dtype=np.complex64)
+    if pr.ndim == 2:
+        pr = pr[:, :, None]
+    n = pr.shape[0]
+    num_slices = get("num_slices", 1)
+    o = np.asarray(get("obj", np.ones((1, 1, num_slices, 1))), dtype=np.complex64)
+    acc = np.float32(get("acc_voltage", 60))
+    wavelength = np.float32(12.398 / np.sqrt((2 * 511.0 + acc) * acc))  # pypty/utils.py:1721
+    px, py = np.float32(get("pixel_size_x_A", 1)), np.float32(get("pixel_size_y_A", 1))
+    slice_distances = np.asarray(get("slice_distances", np.array([10])), dtype=np.float32)
+    pos = np.asarray(get("positions", np.array([[0.0, 0.0]])), dtype=np.float32).astype(np.float64)
+    seq_param = get("sequence", None)
+    seq0 = np.arange(dataset.shape[0]) if seq_param is None else np.asarray(_per_epoch(seq_param, 0))
+    # positions: shift to the origin, integer corner + sub-pixel correction, pad the object
+    # (pypty/utils.py:1722-1764)
+    extra = int(get("extra_space_on_side_px", 0))
+    pos[:, 0] -= pos[:, 0].min()
+    pos[:, 1] -= pos[:, 1].min()
+    pos += extra
+    if extra > 0:
+        o = np.pad(o, [[extra, 0], [extra, 0], [0, 0], [0, 0]], constant_values=np.mean(o))
+    corr = (pos - np.round(pos)) if get("allow_subPixel_shift", True) else np.zeros_like(pos)
+    ipos = np.round(pos).astype(np.int32)
+    diff_x = int(ipos[:, 1].max() + extra + n - o.shape[1])
+    if diff_x > 0:
+        o = np.concatenate([o, np.max(np.abs(o)) * np.ones((o.shape[0], diff_x) + o.shape[2:], np.complex64)], 1)
+    diff_y = int(ipos[:, 0].max() + extra + n - o.shape[0])
+    if diff_y > 0:
+        o = np.concatenate([o, np.max(np.abs(o)) * np.ones((diff_y,) + o.shape[1:], np.complex64)], 0)
+    cutoff = np.float32(get("damping_cutoff_multislice", 2 / 3))
+    rolloff = np.float32(get("smooth_rolloff", 0))
+    obj = utils.fourier_clean(torch.from_numpy(np.ascontiguousarray(o)).to(dev), cutoff=cutoff, rolloff=rolloff)
+    probe = utils.fourier_clean(torch.from_numpy(pr).to(dev), cutoff=cutoff, rolloff=rolloff)
+    positions = torch.from_numpy(ipos).to(dev)
+    positions_correction = torch.from_numpy(corr.astype(np.float32)).to(dev)
+    tilts = np.zeros((1, 6), np.float32)
+    tilts_correction = np.zeros((1, 6), np.float32)
+    measured = torch.from_numpy(dataset).to(dev)
+    folder = get("output_folder", "")
+    print_flag = get("print_flag", 3)
+    save_inter = get("save_inter_checkpoints", True)
+    save_log = get("save_loss_log", 1)
+    if save_log == 2:
+        raise NotImplementedError("pypty_amd.run: save_loss_log=2 (regulariser columns) is not supported")
+    _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
+    reset_bfgs_history()
+    t0 = time.time()
+    counter_failed = 0
+    fail_pos, fail_probe = np.inf, np.inf
+    prev_steps = 0
+    losses = []
+    is_first = True
+    actual_step_default = get("update_step_bfgs", 1)
+    for epoch in range(int(get("epoch_prev", 0)), int(get("epoch_max", 200))):
+        warnings = ""
+        s_obj = _per_epoch(get("update_obj", 1), epoch)
+        s_probe = _per_epoch(get("update_probe", 1), epoch)
+        s_pos = _per_epoch(get("update_probe_pos", 0), epoch)
+        # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, probe after 6
+        if counter_failed >= 2:
+            fail_pos = epoch
+            warnings += "\nWARNING: Scan grid refinement was manually disabled for the next 10 epochs!"
+            if counter_failed >= 6:
+                fail_probe = epoch
+                warnings += "\nWARNING: Probe refinement was manually disabled for the next 10 epochs!"
+        if abs(fail_pos - epoch) < 10:
+            s_pos = 0
+        if abs(fail_probe - epoch) < 10:
+            s_probe = 0
+        steps = sum(int(x != 0) for x in (s_obj, s_probe, s_pos))
+        if epoch != int(get("epoch_prev", 0)) and steps != prev_steps:
+            reset_bfgs_history()
+            warnings += "\nResetting the history!"
+        prev_steps = steps
+        seq = np.sort(np.asarray(_per_epoch(seq_param, epoch))) if seq_param is not None else seq0
+        lw = _per_epoch(get("loss_weight", 1), epoch)
+        if lw == "mean":
+            lw = 1 / dataset.shape[0]
+        out = bfgs_update(
+            algorithm_type=get("algorithm", "lsq_sqrt"), this_slice_distances=slice_distances,
+            edge_slice_thickness_multiplier=1.0, this_step_probe=s_probe, this_step_obj=s_obj,
+            this_step_pos_correction=s_pos, this_step_tilts=0, measured_array=measured, this_wavelength=wavelength,
+            masks=None, pixel_size_x_A=px, pixel_size_y_A=py, phase_norm_weight=0, abs_norm_weight=0,
+            stepsize_threshold_low=get("min_step", 1e-20), probe_reg_weight=0, aperture_mask=None,
+            recon_type="far_field", defocus_array=np.array([0.0]), Cs=0.0, alpha_near_field=0.0,
+            damping_cutoff_multislice=cutoff, smooth_rolloff=rolloff,
+            update_extra_cut=np.float32(get("update_extra_cut", 0.005)), keep_probe_states_orthogonal=False,
+            do_charge_flip=False, cf_delta_phase=0.1, cf_delta_abs=0.01, cf_beta_phase=-0.95, cf_beta_abs=-0.95,
+            phase_only_obj=False, wedge_mu=0, beta_wedge=0,
+            wolfe_c1_constant=_per_epoch(get("wolfe_c1_constant", 0.1), epoch),
+            wolfe_c2_constant=_per_epoch(get("wolfe_c2_constant", 0.9), epoch), atv_weight=0, atv_q=1, atv_p=2,
+            tune_only_probe_phase=False, propmethod="multislice", this_chopped_sequence=seq, load_one_by_one=False,
+            data_multiplier=1, data_pad=0, phase_plate_in_h5=None, print_flag=print_flag, this_loss_weight=lw,
+            max_count=get("max_count", 20), reduce_factor=np.float32(get("reduce_factor", 0.1)),
+            optimism=get("optimism", 3), mixed_variance_weight=0, mixed_variance_sigma=0.5, data_bin=1,
+            data_shift_vector=[0, 0], smart_memory=False, default_float=np.float32, default_complex=np.complex64,
+            default_int=np.int32, upsample_pattern=1, this_step_static_background=0, tilt_mode=0,
+            fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=None, this_step_aberrations_array=0,
+            probe_marker=None, compute_batch=get("compute_batch", 256), current_window=None,
+            current_window_weight=0, dynamically_resize_yx_object=False, lazy_clean=False,
+            current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=0,
+            xp=None, remove_fft_cache=False, is_first_epoch=is_first,
+            hist_length=_per_epoch(get("hist_length", 10), epoch), actual_step=actual_step_default,
+            fast_axis_reg_weight_positions=0, fast_axis_reg_weight_tilts=0, slow_axis_reg_weight_positions=0,
+            slow_axis_reg_weight_tilts=0, scan_size=None, scan_step_A=1.0, current_deformation_reg_weight_positions=0,
+            current_deformation_reg_weight_tilts=0, warnings=warnings, data_simulation_flag=False,
+            data_simulation_exitwave=False, pos_puzzling=False, probe_mod_reg_weight=0, probe_mod_tol=0.05,
+            use_padded_intensity=True, counter_failed_epochs=counter_failed)
+        total_loss, sse, cc, actual_step, count, d_value, new_d_value, warnings, counter_failed = out
+        is_first = False
+        losses.append(total_loss)
+        if print_flag:
+            print(f"--> epoch {epoch}: loss {total_loss:.6e} sse {sse:.6e} step {actual_step:.3e} "
+                  f"linesearch {count + 1} time {time.time() - t0:.2f} s", flush=True)
+        if save_inter and epoch % int(save_inter) == 0:
+            _save_checkpoint(folder, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos)
+        if save_log:
+            free, total = torch.cuda.mem_get_info(dev)
+            _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,
+                               "initial step": actual_step_default, "matching step": actual_step,
+                               "N linesearch iterations": count + 1, "dir. derivative": d_value,
+                               "new dir. derivative": new_d_value, "Constraints contribution": 0.0,
+                               "Free GiB": free / 1024 ** 3, "Total GiB": total / 1024 ** 3, "Warnings": warnings})
+    _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
+    result = {"obj": obj, "probe": probe, "positions": positions.double() + positions_correction.double(),
+              "losses": losses, "time_s": time.time() - t0}
+    return result

@@ -1,0 +1,50 @@
+"""End-to-end reconstruction through the run() mirror: checkpoints, loss log, convergence."""
+import csv
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_run_reconstructs_and_writes_reference_files(tmp_path):
+    import torch
+
+    from pypty_amd import iterative, metrics, synth
+
+    print("[run] 8 epochs, 64 px probe, 12x12 scan", flush=True)
+    dev = torch.device("cuda", 0)
+    px = 0.2
+    true_obj = synth.hex_lattice_object(160, 160, px, 2.46, 0.3, 0.5, seed=5)
+    probe = synth.make_probe(64, px, 300.0, 20.0, 100.0, 1, 0.0, seed=6)
+    pos = synth.raster_scan(12, 12, 6.0, origin_px=(10.0, 10.0), jitter_px=0.3, seed=7)
+    ints, corr = synth.split_positions(pos)
+    lam = synth.electron_wavelength(300.0)
+    meas = synth.simulate(true_obj, probe, ints, corr, px, 2.0, lam, 1e5, seed=8, device=dev)
+    out = str(tmp_path) + os.sep
+    params = dict(dataset=meas, probe=probe, positions=pos, obj=np.ones((160, 160, 1, 1), np.complex64),
+                  acc_voltage=300.0, pixel_size_x_A=px, pixel_size_y_A=px, slice_distances=np.array([2.0]),
+                  num_slices=1, algorithm="lsq_sqrt", epoch_max=8, update_obj=1, update_probe=0,
+                  update_probe_pos=0, compute_batch=64, output_folder=out, save_inter_checkpoints=1, print_flag=1)
+    res = iterative.run(params)
+    losses = res["losses"]
+    assert all(b <= a * (1 + 1e-9) for a, b in zip(losses, losses[1:])), losses
+    assert losses[-1] < 0.5 * losses[0]
+    # positions were shifted to the origin by the driver (pypty/utils.py:1730): compare on the scanned FOV
+    shift = np.round(pos.min(axis=0)).astype(int)
+    rec = metrics.projected_phase(res["obj"])
+    truth = metrics.projected_phase(true_obj)[shift[0]:shift[0] + rec.shape[0], shift[1]:shift[1] + rec.shape[1]]
+    rec = rec[:truth.shape[0], :truth.shape[1]]
+    mask = metrics.fov_mask(truth.shape, pos - pos.min(axis=0), 64)
+    e = metrics.nmse(rec, truth, mask)
+    e0 = metrics.nmse(np.zeros_like(truth), truth, mask)
+    print(f"  loss {losses[0]:.4e} -> {losses[-1]:.4e}; NMSE {e0:.3f} -> {e:.3f}", flush=True)
+    assert e < 0.7 * e0
+    for f in ("co.npy", "cp.npy", "cg.npy", "loss.csv"):
+        assert os.path.exists(out + f), f
+    co = np.load(out + "co.npy")
+    assert co.shape[2:] == (1, 1) and co.dtype == np.complex64
+    assert np.load(out + "cg.npy").shape == (144, 2)
+    rows = list(csv.reader(open(out + "loss.csv")))
+    assert len(rows) == 8 and len(rows[0]) == 13
