@@ -149,14 +149,15 @@ int pty_ms_adjoint(const pty_problem* p, int batch, void* g, void* stash, const 
 
 /* Fourier band-limit of whole planes, in place: pypty/utils.py:102-150
  * (fourier_clean) applied over axes (0, 1) of every plane.  data planar
- * (planes, ny, nx); ny and nx products of 2 and 3 with 8*ny, 8*nx <= 64 KiB;
+ * (planes, ny, nx); ny and nx products of primes <= 31 with 8*ny, 8*nx <= 64 KiB;
  * mask: normalised frequency radius <= cutoff/2 with a cosine taper of
  * width rolloff/2 when rolloff != 0. */
 int pty_fourier_clean_c64(void* data, int64_t planes, int ny, int nx, double cutoff, double rolloff,
                           void* stream);
 
-/* 2D FFT of whole planes (fft.py:91-178 for any ny, nx = 2^a 3^b up to
- * 8192), modes as pty_fft2_c64; scratch: device buffer of the data's size. */
+/* 2D FFT of whole planes (fft.py:91-178 for ny, nx up to 8192 whose prime
+ * factors are <= 31), modes as pty_fft2_c64; scratch: device buffer of the
+ * data's size. */
 int pty_plane_fft2_c64(void* data, void* scratch, int64_t planes, int ny, int nx, int mode,
                        void* stream);
 

@@ -103,7 +103,23 @@ __device__ __forceinline__ float2 twiddle_const(float2 v, int e) {
 // ------------------------------------------------------ in-register DFTs
 template <int R>
 constexpr int first_radix() {
-  return (R % 4 == 0 && R != 8) ? 4 : (R % 2 == 0 ? 2 : (R % 3 == 0 ? 3 : R));
+  return (R % 4 == 0 && R != 8) ? 4
+                                : (R % 2 == 0 ? 2 : (R % 3 == 0 ? 3 : (R % 5 == 0 ? 5 : (R % 7 == 0 ? 7 : R))));
+}
+
+// direct O(R^2) DFT for small odd primes (5, 7): X[k] = sum_n x[n] W_R^(nk)
+template <int R, bool INV>
+__device__ __forceinline__ void dft_direct(float2* v) {
+  float2 out[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    float2 acc = v[0];
+#pragma unroll
+    for (int n = 1; n < R; ++n) acc = cadd(acc, twiddle_const<R, INV>(v[n], (n * k) % R));
+    out[k] = acc;
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = out[k];
 }
 
 // Unnormalised DFT of v[0..R) in natural order (sign -1 forward, +1 inverse).
@@ -135,6 +151,8 @@ __device__ __forceinline__ void dft(float2* v) {
     v[2] = csub(s0, s1);
     v[1] = cadd(d0, d1);
     v[3] = csub(d0, d1);
+  } else if constexpr (R == 5 || R == 7) {
+    dft_direct<R, INV>(v);
   } else {
     // R = P*Q, n = Q*n1 + n2, k = k1 + P*k2
     constexpr int P = first_radix<R>();

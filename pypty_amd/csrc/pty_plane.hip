@@ -7,7 +7,8 @@
 //   rows pass : a workgroup owns `rows` full rows of one plane
 //   cols pass : a workgroup owns a block of `cols` adjacent columns of one
 //               plane (all ny samples of each)
-// A line of length L = prod(r_t), r_t in {2, 3, 4, 8}, is transformed in
+// A line of length L = prod(r_t), r_t in {2, 3, 4, 5, 7, 8} (other primes up to
+// 31 through a direct-DFT stage), is transformed in
 // place by mixed-radix decimation in frequency: stage t splits blocks of size
 // B into r_t interleaved sub-blocks (butterfly radix r_t, stride B/r_t,
 // twiddle W_B^(j k)); the spectrum comes out in mixed-radix digit-reversed
@@ -34,11 +35,28 @@ struct LinePlan {
   int r[kMaxStages];
 };
 
-// factor L into radices 8/4/2/3 (largest power-of-two radices first)
+// factor L into radices 8/4/2/3/5/7 and, for any other prime factor up to
+// kMaxGenericRadix, a direct-DFT stage
+constexpr int kMaxGenericRadix = 31;
+
 bool make_plan(int L, LinePlan& p) {
   p.L = L;
   p.nst = 0;
   int m = L;
+  for (int q = 11; q <= kMaxGenericRadix && m > 1; q += 2) {
+    while (m % q == 0 && p.nst < kMaxStages) {
+      p.r[p.nst++] = q;
+      m /= q;
+    }
+  }
+  while (m % 7 == 0) {
+    p.r[p.nst++] = 7;
+    m /= 7;
+  }
+  while (m % 5 == 0) {
+    p.r[p.nst++] = 5;
+    m /= 5;
+  }
   while (m % 3 == 0) {
     p.r[p.nst++] = 3;
     m /= 3;
@@ -76,7 +94,7 @@ __device__ __forceinline__ int plan_freq(const LinePlan& pl, int p) {
 __device__ __forceinline__ float2 twiddle_rt(int num, int den, bool inv) {
   // exp(-+2 pi i num/den) with an accurate sincospi
   float s, c;
-  sincospif(-2.0f * float(num) / float(den), &s, &c);
+  sincospif(-2.0f * float(num % den) / float(den), &s, &c);
   return make_float2(c, inv ? -s : s);
 }
 
@@ -102,6 +120,23 @@ __device__ __forceinline__ void stage_bfly(float2* x, int es, int base, int s, i
   }
 }
 
+// butterfly of a prime radix r in (7, 31]: direct DFT with run-time twiddles
+template <bool INV>
+__device__ __noinline__ void stage_bfly_generic(float2* x, int es, int base, int s, int j, int B, int r) {
+  float2 v[kMaxGenericRadix], out[kMaxGenericRadix];
+  for (int n = 0; n < r; ++n) v[n] = x[(base + s * n) * es];
+  if (INV)
+    for (int k = 1; k < r; ++k) v[k] = cmul(v[k], twiddle_rt(j * k, B, true));
+  for (int k = 0; k < r; ++k) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int n = 0; n < r; ++n) acc = cadd(acc, cmul(v[n], twiddle_rt((n * k) % r, r, INV)));
+    out[k] = acc;
+  }
+  if (!INV)
+    for (int k = 1; k < r; ++k) out[k] = cmul(out[k], twiddle_rt(j * k, B, false));
+  for (int k = 0; k < r; ++k) x[(base + s * k) * es] = out[k];
+}
+
 // Transform `nlines` lines in LDS; line l starts at buf + l*lstride and has
 // element stride es.  All threads of the block take part.
 template <bool INV>
@@ -123,7 +158,10 @@ __device__ void lines_fft(float2* buf, int nlines, int lstride, int es, const Li
         case 2: stage_bfly<2, INV>(x, es, base, s, j, B); break;
         case 3: stage_bfly<3, INV>(x, es, base, s, j, B); break;
         case 4: stage_bfly<4, INV>(x, es, base, s, j, B); break;
-        default: stage_bfly<8, INV>(x, es, base, s, j, B); break;
+        case 5: stage_bfly<5, INV>(x, es, base, s, j, B); break;
+        case 7: stage_bfly<7, INV>(x, es, base, s, j, B); break;
+        case 8: stage_bfly<8, INV>(x, es, base, s, j, B); break;
+        default: stage_bfly_generic<INV>(x, es, base, s, j, B, r); break;
       }
     }
     __syncthreads();

@@ -77,7 +77,8 @@ def test_loss_and_direction_matches_oracle(case):
         assert v < TOL, (k, v)
 
 
-@pytest.mark.parametrize("shape", [(192, 192), (256, 256), (96, 128), (512, 512), (384, 1536)])
+@pytest.mark.parametrize("shape", [(192, 192), (256, 256), (96, 128), (512, 512), (384, 1536),
+                                   (160, 160), (350, 210), (143, 242), (2000, 35)])
 @pytest.mark.parametrize("mode", ["fft2", "ifft2", "shift_fft2", "ifft2_ishift"])
 def test_plane_fft_modes_match_numpy(shape, mode):
     torch = _torch()
@@ -95,7 +96,9 @@ def test_plane_fft_modes_match_numpy(shape, mode):
 @pytest.mark.parametrize("shape,cutoff,rolloff", [((512, 512, 3, 1), 2 / 3 - 0.005, 0.0),
                                                   ((256, 384, 2, 2), 0.66, 0.0),
                                                   ((1536, 1536, 1, 1), 2 / 3 - 0.005, 0.0),
-                                                  ((192, 192, 1, 1), 0.5, 0.1)])
+                                                  ((192, 192, 1, 1), 0.5, 0.1),
+                                                  ((160, 160, 1, 1), 2 / 3 - 0.005, 0.0),
+                                                  ((350, 210, 2, 1), 0.6, 0.05)])
 def test_fourier_clean_matches_oracle(shape, cutoff, rolloff):
     torch = _torch()
     from pypty_amd import utils as putils
