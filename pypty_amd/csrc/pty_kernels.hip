@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "pty_fft.hpp"
+#include "pty_rt.hpp"
 #include "pypty_amd.h"
 
 using namespace pty;
@@ -165,12 +166,16 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_fft2_tiles(float2* __restrict__
 // --------------------------------------------------------------- tables
 // Propagator H = exp(-i*pi*dz*(lambda*q^2 + 2*(qx*tx + qy*ty))) * mask
 // (objective.py:337-345), its first-slice power H^e (multislice_core.py:73)
-// and the band mask q^2 <= (max(qx)*cutoff)^2 (utils.py:153-199), all in
-// tile position order.  H and H^e carry the 1/n^2 of the inverse FFT.
+// and the band mask q^2 <= (max(qx)*cutoff)^2 (utils.py:153-199).  The mask
+// is stored in tile position order; H, H^e and maskc in the order of the
+// fused forward/adjoint kernels: position order for the LDS tile, C-order
+// for the register tile (N = 64, 128).  H and H^e carry the 1/n^2 of the
+// inverse FFT.
 struct TableArgs {
   float2* H;
   float2* H0;
   float* mask;
+  float* maskc;
   float dz, lambda, px, py, tx, ty, cutoff, edge;
 };
 // float32 products/sums without contraction, in numpy's order
@@ -178,11 +183,7 @@ __device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b)
 __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
 
 template <int N>
-__global__ void k_tables(TableArgs a) {
-  using T = Tile<N>;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N * N) return;
-  const int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
+__device__ __forceinline__ float band_mask(const TableArgs& a, int fy, int fx) {
   const float qx = fftfreq_f(fx, N, a.px), qy = fftfreq_f(fy, N, a.py);
   const float q2 = fadd(fmul(qx, qx), fmul(qy, qy));
   const float max_x = fftfreq_f(N / 2 - 1, N, a.px);
@@ -190,7 +191,20 @@ __global__ void k_tables(TableArgs a) {
   if (c < 0.f) c = float(2.0 / 3.0);
   const float rm = fmul(max_x, c);
   const float r_max = fmul(rm, rm);
-  const float m = (q2 <= r_max) ? 1.f : 0.f;
+  return (q2 <= r_max) ? 1.f : 0.f;
+}
+
+template <int N, bool RTL>
+__global__ void k_tables(TableArgs a) {
+  using T = Tile<N>;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * N) return;
+  a.mask[e] = band_mask<N>(a, T::pos2freq(e / N), T::pos2freq(e % N));
+  int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
+  if constexpr (RTL) RT<N>::cspec(e, fy, fx);
+  const float qx = fftfreq_f(fx, N, a.px), qy = fftfreq_f(fy, N, a.py);
+  const float q2 = fadd(fmul(qx, qx), fmul(qy, qy));
+  const float m = band_mask<N>(a, fy, fx);
   const float k = fmul(-3.141592654f, a.dz);
   const float r = fadd(fmul(a.lambda, q2), fmul(2.f, fadd(fmul(qx, a.tx), fmul(qy, a.ty))));
   const float ang = fmul(k, r);
@@ -208,7 +222,7 @@ __global__ void k_tables(TableArgs a) {
   } else {
     a.H0[e] = a.H[e];
   }
-  a.mask[e] = m;
+  a.maskc[e] = m;
 }
 
 // Position-order tables from a caller-supplied propagator (natural order,
@@ -289,6 +303,7 @@ struct BatchArgs {
   const float2* H;          // position order, /n^2
   const float2* H0;
   const float* mask;        // position order
+  const float* maskc;       // order of H (C-order for the register tile)
   // per-batch buffers
   float2* Ob;               // (B, S, K, N, N) band-limited windows
   float2* Pp;               // (B, M, N, N) shifted probes
@@ -474,7 +489,7 @@ struct LossSplit {
   static constexpr int PIX = N * N / CHUNKS;
 };
 
-template <int N>
+template <int N, bool RTL>
 __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
   using T = Tile<N>;
   constexpr int CH = LossSplit<N>::CHUNKS, PIX = LossSplit<N>::PIX;
@@ -500,7 +515,9 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
     float I = 0.f;
     for (int j = 0; j < MK; ++j) I += cabs2(psi[size_t(j) * N * N + e]);
     if (MK > 1) I = I / float(MK);
-    const int ys = (T::pos2freq(e / N) + N / 2) % N, xs = (T::pos2freq(e % N) + N / 2) % N;
+    int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
+    if constexpr (RTL) RT<N>::cspec(e, fy, fx);
+    const int ys = (fy + N / 2) % N, xs = (fx + N / 2) % N;
     const float d = D[ys * N + xs];
     float l;
     if (a.algorithm == PTY_LOSS_ML) {
@@ -653,6 +670,163 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
     const int e = tid + i * T::NT;
     G[e] = t[T::lds(e / N, e % N)];
   }
+}
+
+// --------------------------------------------- register-tile pipeline
+// The same per-batch steps for N = 64 and 128 over the register tile of
+// pty_rt.hpp.  Buffer orders: Ob and stash R-order; Psi, coef, H, H0, maskc
+// C-order; Pp, gP and the band-limited window gradient natural order.
+//
+// Register i of an R/C-order array is base[i*NT + tid]: (base + i*NT) is
+// wavefront-uniform and goes to scalar registers, the lane offset tid is one
+// vector register for all i (an immediate offset cannot hold i*NT*8 bytes).
+template <int N, typename P>
+__device__ __forceinline__ P* rt_reg(P* base, int i) {
+  return base + size_t(i) * RT<N>::NT;
+}
+
+// k_patch over the register tile (objective.py:436-441)
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  const int planes = a.S * a.K;
+  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
+  const bool inside = window_inside(y0, x0, N, a.ny, a.nx);
+  const int tid = threadIdx.x;
+  const RTIds<N> id;
+  const float2* tw = twiddles<N>();
+  float2 v[R::R1];
+  const float2* src = a.obj + size_t(plane) * a.ny * a.nx + (inside ? size_t(y0 + id.row) * a.nx + x0 + id.rj : 0);
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) v[n] = inside ? src[8 * n] : make_float2(0.f, 0.f);
+  rt_fft2_fwd<N>(v, T, id, tw);
+  const float sc = 1.0f / float(N * N);
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], rt_reg<N>(a.maskc, i)[tid] * sc);
+  rt_fft2_inv<N>(v, T, id, tw);
+  float2* dst = a.Ob + (size_t(b) * planes + plane) * N * N;
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) rt_reg<N>(dst, n)[tid] = v[n];
+}
+
+// k_forward over the register tile (multislice_core.py:9-82, objective.py:501-512)
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int m = mk / a.K, k = mk % a.K;
+  const int tid = threadIdx.x;
+  const RTIds<N> id;
+  const float2* tw = twiddles<N>();
+  const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
+  float2 v[R::R1];
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + 8 * n];
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  for (int s = 0; s < a.S; ++s) {
+    const float2* o = a.Ob + ((size_t(b) * a.S + s) * a.K + k) * N * N;
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) {
+      if (a.write_stash) rt_reg<N>(st + size_t(s) * N * N, n)[tid] = v[n];
+      v[n] = cmul(v[n], rt_reg<N>(o, n)[tid]);
+    }
+    if (s < a.S - 1) {
+      rt_fft2_fwd<N>(v, T, id, tw);
+      const float2* H = (s == 0) ? a.H0 : a.H;
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) v[i] = cmul(v[i], rt_reg<N>(H, i)[tid]);
+      rt_fft2_inv<N>(v, T, id, tw);
+    }
+  }
+  rt_fft2_fwd<N>(v, T, id, tw);
+  float2* out = a.Psi + (size_t(b) * MK + mk) * N * N;
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) rt_reg<N>(out, i)[tid] = cscale(v[i], rt_reg<N>(a.maskc, i)[tid]);
+}
+
+// k_adjoint over the register tile (objective.py:595-597, multislice_core.py:85-200)
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int k = mk % a.K;
+  const int tid = threadIdx.x;
+  const RTIds<N> id;
+  const float2* tw = twiddles<N>();
+  const float2* psi = a.Psi + (size_t(b) * MK + mk) * N * N;
+  const float* c = a.coef + size_t(b) * N * N;
+  const float sc = 1.0f / float(N * N);
+  float2 v[R::R1];
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) {
+    v[i] = cscale(rt_reg<N>(psi, i)[tid], rt_reg<N>(c, i)[tid] * sc);
+  }
+  rt_fft2_inv<N>(v, T, id, tw);
+  // the stash slot of slice s is overwritten by this mode's window gradient
+  // conj(u_s) * g; k_objgrad_rt sums the modes
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  for (int s = a.S - 1; s >= 0; --s) {
+    const float2* o = a.Ob + ((size_t(b) * a.S + s) * a.K + k) * N * N;
+    float2* u = st + size_t(s) * N * N;
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) {
+      if (a.step_obj) rt_reg<N>(u, n)[tid] = cmulc(v[n], rt_reg<N>(u, n)[tid]);
+      v[n] = cmulc(v[n], rt_reg<N>(o, n)[tid]);
+    }
+    if (s > 0) {
+      rt_fft2_fwd<N>(v, T, id, tw);
+      const float2* H = (s == 1) ? a.H0 : a.H;
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) v[i] = cmulc(v[i], rt_reg<N>(H, i)[tid]);
+      rt_fft2_inv<N>(v, T, id, tw);
+    }
+  }
+  float2* out = a.gP + (size_t(b) * MK + mk) * N * N;
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) out[id.row * N + id.rj + 8 * n] = v[n];
+}
+
+// k_objgrad over the register tile: mode sum from the R-order stash, band
+// limit, natural-order window gradient into Ob for k_gather_grad
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  const int planes = a.S * a.K;
+  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
+  const int sl = plane / a.K, k = plane % a.K;
+  const int tid = threadIdx.x;
+  const RTIds<N> id;
+  const float2* tw = twiddles<N>();
+  const size_t mstride = size_t(a.K) * a.S * N * N;
+  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.S + sl) * N * N;
+  float2 v[R::R1];
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) {
+    const float2* sn = rt_reg<N>(src, n);
+    float2 x = sn[tid];
+    for (int m = 1; m < a.M; ++m) x = cadd(x, sn[m * mstride + tid]);
+    v[n] = x;
+  }
+  rt_fft2_fwd<N>(v, T, id, tw);
+  const float sc = 1.0f / float(N * N);
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], rt_reg<N>(a.maskc, i)[tid] * sc);
+  rt_fft2_inv<N>(v, T, id, tw);
+  float2* G = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + 8 * n] = v[n];
 }
 
 // Deterministic scatter-add of a batch's window gradients (multislice_core.py:192,
@@ -980,7 +1154,7 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 
 // ---------------------------------------------------------- workspace
 struct Layout {
-  size_t H, H0, mask, fprobe, objp, gradp, Ob, Pp, stash, Psi, coef, Gbuf, gP, probe_part, pos_part, loss_part,
+  size_t H, H0, mask, maskc, fprobe, objp, gradp, Ob, Pp, stash, Psi, coef, Gbuf, gP, probe_part, pos_part, loss_part,
       scratch, total;
   int B, G;
 };
@@ -1025,6 +1199,7 @@ Layout plan_layout(const pty_problem* p) {
   L.H = take(n2 * 8);
   L.H0 = take(n2 * 8);
   L.mask = take(n2 * 4);
+  L.maskc = take(n2 * 4);
   L.fprobe = take(size_t(M) * n2 * 8);
   L.objp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
   L.gradp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
@@ -1059,12 +1234,16 @@ int validate(const pty_problem* p) {
   return PTY_OK;
 }
 
+// the fused kernels of windows of 64 and 128 px use the register tile
 template <int N>
-int launch_tables(const pty_problem* p, float2* H, float2* H0, float* mask, hipStream_t s) {
+constexpr bool kRegTile = (N == 64 || N == 128);
+
+template <int N>
+int launch_tables(const pty_problem* p, float2* H, float2* H0, float* mask, float* maskc, hipStream_t s) {
   TimeScope ts(KT_TABLES, s);
-  TableArgs a{H, H0, mask, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x, p->tan_y,
-              p->damping_cutoff, p->edge_multiplier};
-  k_tables<N><<<(N * N + 255) / 256, 256, 0, s>>>(a);
+  TableArgs a{H, H0, mask, maskc, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x,
+              p->tan_y, p->damping_cutoff, p->edge_multiplier};
+  k_tables<N, kRegTile<N>><<<(N * N + 255) / 256, 256, 0, s>>>(a);
   return check_launch();
 }
 
@@ -1080,11 +1259,15 @@ int zero_fill(void* ptr, size_t bytes, hipStream_t s) {
 template <int N>
 int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   using T = Tile<N>;
+  constexpr bool RTL = kRegTile<N>;
   const int planes = a.S * a.K;
   const int MK = a.M * a.K;
   {
     TimeScope ts(KT_PATCH, s);
-    k_patch<N><<<a.B * planes, T::NT, 0, s>>>(a);
+    if constexpr (RTL)
+      k_patch_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
+    else
+      k_patch<N><<<a.B * planes, T::NT, 0, s>>>(a);
   }
   if (a.shift) {
     TimeScope ts(KT_PROBE_SHIFT, s);
@@ -1092,20 +1275,29 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   }
   {
     TimeScope ts(KT_FORWARD, s);
-    k_forward<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
+    if constexpr (RTL)
+      k_forward_rt<N><<<xcd_grid(a.B, MK), RT<N>::NT, 0, s>>>(a);
+    else
+      k_forward<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
   }
   {
     TimeScope ts(KT_LOSS, s);
-    k_loss<N><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
+    k_loss<N, RTL><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
   }
   if (p->step_obj || p->step_probe || p->step_pos) {
     TimeScope ts(KT_ADJOINT, s);
-    k_adjoint<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
+    if constexpr (RTL)
+      k_adjoint_rt<N><<<xcd_grid(a.B, MK), RT<N>::NT, 0, s>>>(a);
+    else
+      k_adjoint<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
   }
   if (p->step_obj) {
     {
       TimeScope ts(KT_OBJGRAD, s);
-      k_objgrad<N><<<a.B * planes, T::NT, 0, s>>>(a);
+      if constexpr (RTL)
+        k_objgrad_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
+      else
+        k_objgrad<N><<<a.B * planes, T::NT, 0, s>>>(a);
     }
     TimeScope ts(KT_GATHER, s);
     const int tiles = ((a.ny + GT - 1) / GT) * ((a.nx + GT - 1) / GT);
@@ -1135,7 +1327,8 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   float2* H = reinterpret_cast<float2*>(ws + L.H);
   float2* H0 = reinterpret_cast<float2*>(ws + L.H0);
   float* mask = reinterpret_cast<float*>(ws + L.mask);
-  if ((rc = launch_tables<N>(p, H, H0, mask, s))) return rc;
+  float* maskc = reinterpret_cast<float*>(ws + L.maskc);
+  if ((rc = launch_tables<N>(p, H, H0, mask, maskc, s))) return rc;
   float2* fprobe = reinterpret_cast<float2*>(ws + L.fprobe);
   {
     TimeScope ts(KT_FPROBE, s);
@@ -1166,7 +1359,7 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.loss_weight = p->loss_weight;
   a.obj = objp; a.probe = static_cast<const float2*>(bf->probe); a.fprobe = fprobe;
   a.pos = bf->positions; a.corr = bf->corrections; a.meas = bf->measured; a.seq = bf->sequence;
-  a.H = H; a.H0 = H0; a.mask = mask;
+  a.H = H; a.H0 = H0; a.mask = mask; a.maskc = maskc;
   a.Ob = reinterpret_cast<float2*>(ws + L.Ob); a.Pp = reinterpret_cast<float2*>(ws + L.Pp);
   a.stash = reinterpret_cast<float2*>(ws + L.stash); a.Psi = reinterpret_cast<float2*>(ws + L.Psi);
   a.coef = reinterpret_cast<float*>(ws + L.coef); a.Gbuf = nullptr;

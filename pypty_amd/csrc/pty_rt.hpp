@@ -1,0 +1,241 @@
+// Register-resident window tile for N = 64 and 128 (gfx950).
+//
+// Each thread keeps R1 = N/8 samples of one line in registers; the 8 threads
+// of a line are 8 lanes of the same wavefront.  A 1D transform of a line is
+// the split N = R1 * 8:
+//
+//   forward: DFT-R1 over the thread's samples x[j + 8n] (n = 0..R1-1),
+//            twiddle W_N^(j*k1), exchange within the line's 8 lanes through
+//            LDS, then R1/8 DFT-8s over j -> X[k1 + R1*k2]
+//   inverse: the same steps backwards
+//
+// The exchange only involves lanes of one wavefront, and LDS instructions of
+// a wavefront execute in issue order, so it needs no workgroup barrier.
+//
+// The window is processed in two phases that alternate:
+//   row phase    : wavefront w owns rows 8w..8w+7 (lane = j + 8*row_in_wave)
+//   column phase : wavefront w owns columns 8w..8w+7 (lane = col_in_wave + 8*j)
+// Every step of the multislice recursion that is local to a row (inverse row
+// transform, stash write, object product, forward row transform) runs in the
+// row phase, every step local to a column (forward column transform,
+// propagator, inverse column transform) in the column phase.  Switching phase
+// is one transpose through an LDS tile (row stride LS = N + 8) between two
+// workgroup barriers.  Every LDS access pattern (exchange slots, transposes)
+// puts the 32 lanes of a half-wavefront on 32 distinct 8-byte bank pairs.
+//
+// Global layouts that go with it (e = i * NT + tid, fully coalesced):
+//   R-order: register i of thread tid in the row phase = natural pixel
+//            (y, x) = (8w + rr, j + 8i)
+//   C-order: register i of thread tid in the column phase = frequency
+//            (fy, fx) after a forward 2D transform (cspec below)
+#pragma once
+#include "pty_fft.hpp"
+
+namespace pty {
+
+template <int N>
+struct RT {
+  static_assert(N == 64 || N == 128, "register tile: N = 64 or 128");
+  static constexpr int R1 = N / 8;   // samples per thread and line
+  static constexpr int CPT = R1 / 8; // DFT-8 groups per thread in stage 2
+  static constexpr int NT = N * 8;   // threads per window
+  static constexpr int LS = N + 8;   // LDS row stride (float2), = 8 mod 32
+  static constexpr int ELEMS = N * LS;
+
+  // exchange slot of (thread j, stage-1 output k1) within a line
+  __host__ __device__ __forceinline__ static int xslot(int j, int k1) { return k1 * 8 + (j ^ (k1 / CPT)); }
+  // column position of frequency f after a row transform, and back
+  __host__ __device__ __forceinline__ static int col_freq(int col) {
+    const int g = col >> 3;
+    return CPT * (col & 7) + (g % CPT) + R1 * (g / CPT);
+  }
+  // R-order slot of natural pixel (y, x)
+  __host__ __device__ __forceinline__ static int rslot(int y, int x) {
+    return (x >> 3) * NT + (y >> 3) * 64 + (y & 7) * 8 + (x & 7);
+  }
+  // natural pixel of R-order slot e
+  __host__ __device__ __forceinline__ static void rpos(int e, int& y, int& x) {
+    const int i = e / NT, t = e % NT;
+    y = (t >> 6) * 8 + ((t & 63) >> 3);
+    x = (t & 7) + 8 * i;
+  }
+  // frequency (fy, fx) of C-order slot e
+  __host__ __device__ __forceinline__ static void cspec(int e, int& fy, int& fx) {
+    const int i = e / NT, t = e % NT, lane = t & 63;
+    fx = col_freq((t >> 6) * 8 + (lane & 7));
+    fy = CPT * (lane >> 3) + (i >> 3) + R1 * (i & 7);
+  }
+};
+
+// Orders the LDS accesses of one wavefront around an exchange: nothing is
+// moved across it by the compiler, and the wavefront's own writes are
+// complete before its next reads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// v[k1] *= W_N^(+-j*k1) for k1 = 1..R1-1, built from tw[r*j] (r = 1..3) and
+// tw[4*q*j] (q = 1..R1/4-1): one table read per factor, one product per k1
+template <int N, bool INV>
+__device__ __forceinline__ void rt_twiddle(float2* v, unsigned j, const float2* __restrict__ tw) {
+  constexpr int R1 = RT<N>::R1;
+  float2 a[4], b[R1 / 4];
+  a[0] = make_float2(1.f, 0.f);
+  b[0] = a[0];
+#pragma unroll
+  for (unsigned r = 1; r < 4; ++r) a[r] = tw[r * j];
+#pragma unroll
+  for (unsigned q = 1; q < R1 / 4; ++q) b[q] = tw[4 * q * j];
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) {
+    const int r = k1 & 3, q = k1 >> 2;
+    const float2 w = (q == 0) ? a[r] : (r == 0 ? b[q] : cmul(a[r], b[q]));
+    v[k1] = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
+  }
+}
+
+// Exchange addressing of one line.  Slot s = 8*k1 + (j ^ q), q = k1 / CPT,
+// lies at line*LS + s (rows) or s*LS + line (columns).  With k1 = CPT*q + c:
+//   write of (j, k1) : wbase(q) + c*8*step,  wbase(q) = ((CPT*q)*8 + (j ^ q))*step + line0
+//   read  of (jj, k1 = CPT*j + c) : rbase(jj) + c*8*step,
+//                                   rbase(jj) = ((CPT*j)*8 + (jj ^ j))*step + line0
+// so one exchange needs 8 lane addresses and compile-time offsets.  The line
+// origin and j pass through an empty asm so that these addresses (and the
+// twiddle reads) are recomputed at every exchange instead of being kept live
+// across the kernel.
+template <int N, bool ROW>
+struct XAddr {
+  static constexpr int STEP = ROW ? 1 : RT<N>::LS;
+  int line0;
+  __device__ __forceinline__ XAddr(int line) {
+    line0 = ROW ? line * RT<N>::LS : line;
+    asm volatile("" : "+v"(line0));
+  }
+  __device__ __forceinline__ static int opaque(int j) {
+    asm volatile("" : "+v"(j));
+    return j;
+  }
+  __device__ __forceinline__ int w(int j, int q) const { return ((RT<N>::CPT * q) * 8 + (j ^ q)) * STEP + line0; }
+  __device__ __forceinline__ int r(int j, int jj) const { return ((RT<N>::CPT * j) * 8 + (jj ^ j)) * STEP + line0; }
+};
+
+// forward line transform: in v[n] = x[j + 8n]; out v[c*8 + k2] = X[CPT*j + c + R1*k2]
+template <int N, bool ROW>
+__device__ __forceinline__ void rt_line_fwd(float2* v, float2* T, int line, int j, const float2* __restrict__ tw) {
+  using R = RT<N>;
+  constexpr int CS = 8 * XAddr<N, ROW>::STEP;  // offset of c
+  j = XAddr<N, ROW>::opaque(j);
+  dft<R::R1, false>(v);
+  rt_twiddle<N, false>(v, unsigned(j), tw);
+  const XAddr<N, ROW> X(line);
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    float2* p = T + X.w(j, q);
+#pragma unroll
+    for (int c = 0; c < R::CPT; ++c) p[c * CS] = v[R::CPT * q + c];
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const float2* p = T + X.r(j, jj);
+#pragma unroll
+    for (int c = 0; c < R::CPT; ++c) v[c * 8 + jj] = p[c * CS];
+  }
+#pragma unroll
+  for (int c = 0; c < R::CPT; ++c) dft<8, false>(&v[c * 8]);
+}
+
+// inverse line transform (unnormalised): exact reverse of rt_line_fwd
+template <int N, bool ROW>
+__device__ __forceinline__ void rt_line_inv(float2* v, float2* T, int line, int j, const float2* __restrict__ tw) {
+  using R = RT<N>;
+  constexpr int CS = 8 * XAddr<N, ROW>::STEP;
+  j = XAddr<N, ROW>::opaque(j);
+#pragma unroll
+  for (int c = 0; c < R::CPT; ++c) dft<8, true>(&v[c * 8]);
+  const XAddr<N, ROW> X(line);
+  wave_lds_sync();
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    float2* p = T + X.r(j, jj);
+#pragma unroll
+    for (int c = 0; c < R::CPT; ++c) p[c * CS] = v[c * 8 + jj];
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const float2* p = T + X.w(j, q);
+#pragma unroll
+    for (int c = 0; c < R::CPT; ++c) v[R::CPT * q + c] = p[c * CS];
+  }
+  rt_twiddle<N, true>(v, unsigned(j), tw);
+  dft<R::R1, true>(v);
+}
+
+// Thread coordinates of both phases.
+template <int N>
+struct RTIds {
+  int row, rj;  // row phase: line and position within the line
+  int col, cj;  // column phase
+  __device__ __forceinline__ RTIds() {
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    row = 8 * w + (lane >> 3);
+    rj = lane & 7;
+    col = 8 * w + (lane & 7);
+    cj = lane >> 3;
+  }
+};
+
+// row phase -> column phase: spectrum of the thread's row goes to column
+// position 8*(CPT*k2 + c) + rj of that row; barrier; read the thread's column
+template <int N>
+__device__ __forceinline__ void rt_rows_to_cols(float2* v, float2* T, const RTIds<N>& id) {
+  using R = RT<N>;
+  int rb = id.row * R::LS + id.rj, cb = id.cj * R::LS + id.col;
+  asm volatile("" : "+v"(rb), "+v"(cb));
+#pragma unroll
+  for (int c = 0; c < R::CPT; ++c) {
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) T[rb + 8 * (R::CPT * k2 + c)] = v[c * 8 + k2];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) v[n] = T[cb + 8 * n * R::LS];
+}
+
+// column phase -> row phase: the exact reverse
+template <int N>
+__device__ __forceinline__ void rt_cols_to_rows(float2* v, float2* T, const RTIds<N>& id) {
+  using R = RT<N>;
+  int rb = id.row * R::LS + id.rj, cb = id.cj * R::LS + id.col;
+  asm volatile("" : "+v"(rb), "+v"(cb));
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) T[cb + 8 * n * R::LS] = v[n];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < R::CPT; ++c) {
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) v[c * 8 + k2] = T[rb + 8 * (R::CPT * k2 + c)];
+  }
+}
+
+// Forward 2D transform: row phase natural samples -> column phase spectrum.
+template <int N>
+__device__ __forceinline__ void rt_fft2_fwd(float2* v, float2* T, const RTIds<N>& id, const float2* tw) {
+  rt_line_fwd<N, true>(v, T, id.row, id.rj, tw);
+  rt_rows_to_cols<N>(v, T, id);
+  rt_line_fwd<N, false>(v, T, id.col, id.cj, tw);
+}
+// Inverse 2D transform (unnormalised): column phase spectrum -> row phase samples.
+template <int N>
+__device__ __forceinline__ void rt_fft2_inv(float2* v, float2* T, const RTIds<N>& id, const float2* tw) {
+  rt_line_inv<N, false>(v, T, id.col, id.cj, tw);
+  rt_cols_to_rows<N>(v, T, id);
+  rt_line_inv<N, true>(v, T, id.row, id.rj, tw);
+}
+
+}  // namespace pty

@@ -47,6 +47,8 @@ def test_fft_modes_match_numpy(n, mode):
     dict(n=32, S=3, K=2, M=1, shift=True, algorithm="lsq_sqrt_2", P=9),
     dict(n=192, S=3, K=1, M=2, shift=True, algorithm="lsq_sqrt", ny=256, nx=256, P=5),
     dict(n=256, S=2, K=1, M=2, shift=True, algorithm="ml", ny=320, nx=320, P=4),
+    dict(n=128, S=2, K=2, M=1, shift=True, algorithm="ml", ny=160, nx=160, P=5),
+    dict(n=64, S=3, K=2, M=2, shift=True, algorithm="lsq_sqrt_2", ny=128, nx=128, P=6),
 ])
 def test_loss_and_direction_matches_oracle(case):
     torch = _torch()
@@ -56,6 +58,10 @@ def test_loss_and_direction_matches_oracle(case):
     pr = make_problem(**case)
     ref = oracle_pass(pr, np.complex64)
     ref64 = oracle_pass(pr, np.complex128)
+    # the case must be well conditioned: FP32 and FP64 arithmetic of the same
+    # algorithm agree far inside the tolerance (with a single slice the ml
+    # loss's 1/I makes the gradients depend on round-off by 6e-4)
+    assert max(rel(ref[2], ref64[2]), rel(ref[3], ref64[3])) < TOL / 3
     dev = torch.device("cuda", 0)
     kw = reference_kwargs(
         this_obj=torch.from_numpy(pr["obj"]).to(dev), full_probe=torch.from_numpy(pr["probe"]).to(dev),
