@@ -677,13 +677,37 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
 // pty_rt.hpp.  Buffer orders: Ob and stash R-order; Psi, coef, H, H0, maskc
 // C-order; Pp, gP and the band-limited window gradient natural order.
 //
-// Register i of an R/C-order array is base[i*NT + tid]: (base + i*NT) is
-// wavefront-uniform and goes to scalar registers, the lane offset tid is one
-// vector register for all i (an immediate offset cannot hold i*NT*8 bytes).
-template <int N, typename P>
-__device__ __forceinline__ P* rt_reg(P* base, int i) {
-  return base + size_t(i) * RT<N>::NT;
-}
+// Register i of an R/C-order array is element i*NT + tid of one window's
+// N*N block.  It is accessed through a buffer resource on the block: the
+// lane offset tid*4*W (one vector register for all i) plus the scalar offset
+// i*NT*4*W, so no per-register address arithmetic runs on the vector ALU
+// (a global load's immediate offset cannot hold i*NT*8 bytes).  The resource
+// covers exactly the block, so an access can never leave it.
+template <int N, typename E>
+struct RView {
+  static_assert(sizeof(E) == 8 || sizeof(E) == 4, "float2 or float");
+  __amdgpu_buffer_rsrc_t r;
+  unsigned voff;
+  __device__ __forceinline__ RView(const E* block, unsigned tid)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<E*>(block), 0, N * N * int(sizeof(E)), 0x00020000)),
+        voff(tid * unsigned(sizeof(E))) {}
+  __device__ __forceinline__ E operator[](int i) const {
+    const int so = i * RT<N>::NT * int(sizeof(E));
+    if constexpr (sizeof(E) == 8)
+      return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, voff, so, 0));
+    else
+      return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, voff, so, 0));
+  }
+  __device__ __forceinline__ void store(int i, E v) const {
+    const int so = i * RT<N>::NT * int(sizeof(E));
+    if constexpr (sizeof(E) == 8) {
+      using B = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B, v), r, voff, so, 0);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, so, 0);
+    }
+  }
+};
 
 // k_patch over the register tile (objective.py:436-441)
 template <int N>
@@ -695,21 +719,21 @@ __global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
   const int idx = seq_index(a.seq, a.b0 + b, a.P);
   const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
   const bool inside = window_inside(y0, x0, N, a.ny, a.nx);
-  const int tid = threadIdx.x;
+  const unsigned tid = threadIdx.x;
   const RTIds<N> id;
-  const float2* tw = twiddles<N>();
   float2 v[R::R1];
   const float2* src = a.obj + size_t(plane) * a.ny * a.nx + (inside ? size_t(y0 + id.row) * a.nx + x0 + id.rj : 0);
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) v[n] = inside ? src[8 * n] : make_float2(0.f, 0.f);
-  rt_fft2_fwd<N>(v, T, id, tw);
+  rt_fft2_fwd<N>(v, T, id);
   const float sc = 1.0f / float(N * N);
+  const RView<N, float> mask(a.maskc, tid);
 #pragma unroll
-  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], rt_reg<N>(a.maskc, i)[tid] * sc);
-  rt_fft2_inv<N>(v, T, id, tw);
-  float2* dst = a.Ob + (size_t(b) * planes + plane) * N * N;
+  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], mask[i] * sc);
+  rt_fft2_inv<N>(v, T, id);
+  const RView<N, float2> dst(a.Ob + (size_t(b) * planes + plane) * N * N, tid);
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) rt_reg<N>(dst, n)[tid] = v[n];
+  for (int n = 0; n < R::R1; ++n) dst.store(n, v[n]);
 }
 
 // k_forward over the register tile (multislice_core.py:9-82, objective.py:501-512)
@@ -722,33 +746,54 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
   xcd_decode(blockIdx.x, MK, b, mk);
   if (b >= a.B) return;
   const int m = mk / a.K, k = mk % a.K;
-  const int tid = threadIdx.x;
+  const unsigned tid = threadIdx.x;
   const RTIds<N> id;
-  const float2* tw = twiddles<N>();
   const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
-  float2 v[R::R1];
+  // o holds the object window of the current slice; the next slice's is read
+  // during the column phase so that its latency hides behind the transforms,
+  // and the propagator is read before the column transform that precedes its use
+  float2 v[R::R1], o[R::R1];
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + 8 * n];
+  const size_t ostride = size_t(a.K) * N * N;
+  const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  {
+    const RView<N, float2> O(ob, tid);
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) o[n] = O[n];
+  }
   float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
   for (int s = 0; s < a.S; ++s) {
-    const float2* o = a.Ob + ((size_t(b) * a.S + s) * a.K + k) * N * N;
+    if (a.write_stash) {
+      const RView<N, float2> U(st + size_t(s) * N * N, tid);
 #pragma unroll
-    for (int n = 0; n < R::R1; ++n) {
-      if (a.write_stash) rt_reg<N>(st + size_t(s) * N * N, n)[tid] = v[n];
-      v[n] = cmul(v[n], rt_reg<N>(o, n)[tid]);
+      for (int n = 0; n < R::R1; ++n) U.store(n, v[n]);
     }
-    if (s < a.S - 1) {
-      rt_fft2_fwd<N>(v, T, id, tw);
-      const float2* H = (s == 0) ? a.H0 : a.H;
 #pragma unroll
-      for (int i = 0; i < R::R1; ++i) v[i] = cmul(v[i], rt_reg<N>(H, i)[tid]);
-      rt_fft2_inv<N>(v, T, id, tw);
+    for (int n = 0; n < R::R1; ++n) v[n] = cmul(v[n], o[n]);
+    if (s < a.S - 1) {
+      rt_line_fwd<N, true>(v, T, id.row, id.rj);
+      rt_rows_to_cols<N>(v, T, id);
+      const RView<N, float2> H((s == 0) ? a.H0 : a.H, tid);
+      float2 h[R::R1];
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) h[i] = H[i];
+      rt_line_fwd<N, false>(v, T, id.col, id.cj);
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) v[i] = cmul(v[i], h[i]);
+      const RView<N, float2> O(ob + size_t(s + 1) * ostride, tid);
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) o[n] = O[n];
+      rt_line_inv<N, false>(v, T, id.col, id.cj);
+      rt_cols_to_rows<N>(v, T, id);
+      rt_line_inv<N, true>(v, T, id.row, id.rj);
     }
   }
-  rt_fft2_fwd<N>(v, T, id, tw);
-  float2* out = a.Psi + (size_t(b) * MK + mk) * N * N;
+  rt_fft2_fwd<N>(v, T, id);
+  const RView<N, float2> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
+  const RView<N, float> mask(a.maskc, tid);
 #pragma unroll
-  for (int i = 0; i < R::R1; ++i) rt_reg<N>(out, i)[tid] = cscale(v[i], rt_reg<N>(a.maskc, i)[tid]);
+  for (int i = 0; i < R::R1; ++i) out.store(i, cscale(v[i], mask[i]));
 }
 
 // k_adjoint over the register tile (objective.py:595-597, multislice_core.py:85-200)
@@ -761,35 +806,56 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
   xcd_decode(blockIdx.x, MK, b, mk);
   if (b >= a.B) return;
   const int k = mk % a.K;
-  const int tid = threadIdx.x;
+  const unsigned tid = threadIdx.x;
   const RTIds<N> id;
-  const float2* tw = twiddles<N>();
-  const float2* psi = a.Psi + (size_t(b) * MK + mk) * N * N;
-  const float* c = a.coef + size_t(b) * N * N;
   const float sc = 1.0f / float(N * N);
   float2 v[R::R1];
+  {
+    const RView<N, float2> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
+    const RView<N, float> c(a.coef + size_t(b) * N * N, tid);
 #pragma unroll
-  for (int i = 0; i < R::R1; ++i) {
-    v[i] = cscale(rt_reg<N>(psi, i)[tid], rt_reg<N>(c, i)[tid] * sc);
+    for (int i = 0; i < R::R1; ++i) v[i] = cscale(psi[i], c[i] * sc);
   }
-  rt_fft2_inv<N>(v, T, id, tw);
+  rt_fft2_inv<N>(v, T, id);
   // the stash slot of slice s is overwritten by this mode's window gradient
-  // conj(u_s) * g; k_objgrad_rt sums the modes
+  // conj(u_s) * g (k_objgrad_rt sums the modes); pu holds slot s, read one
+  // column phase ahead like the forward's object window
   float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  const size_t ostride = size_t(a.K) * N * N;
+  const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  float2 pu[R::R1];
+  if (a.step_obj) {
+    const RView<N, float2> U(st + size_t(a.S - 1) * N * N, tid);
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) pu[n] = U[n];
+  }
   for (int s = a.S - 1; s >= 0; --s) {
-    const float2* o = a.Ob + ((size_t(b) * a.S + s) * a.K + k) * N * N;
-    float2* u = st + size_t(s) * N * N;
+    const RView<N, float2> O(ob + size_t(s) * ostride, tid);
+    if (a.step_obj) {
+      const RView<N, float2> U(st + size_t(s) * N * N, tid);
 #pragma unroll
-    for (int n = 0; n < R::R1; ++n) {
-      if (a.step_obj) rt_reg<N>(u, n)[tid] = cmulc(v[n], rt_reg<N>(u, n)[tid]);
-      v[n] = cmulc(v[n], rt_reg<N>(o, n)[tid]);
+      for (int n = 0; n < R::R1; ++n) U.store(n, cmulc(v[n], pu[n]));
     }
-    if (s > 0) {
-      rt_fft2_fwd<N>(v, T, id, tw);
-      const float2* H = (s == 1) ? a.H0 : a.H;
 #pragma unroll
-      for (int i = 0; i < R::R1; ++i) v[i] = cmulc(v[i], rt_reg<N>(H, i)[tid]);
-      rt_fft2_inv<N>(v, T, id, tw);
+    for (int n = 0; n < R::R1; ++n) v[n] = cmulc(v[n], O[n]);
+    if (s > 0) {
+      rt_line_fwd<N, true>(v, T, id.row, id.rj);
+      rt_rows_to_cols<N>(v, T, id);
+      const RView<N, float2> H((s == 1) ? a.H0 : a.H, tid);
+      float2 h[R::R1];
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) h[i] = H[i];
+      rt_line_fwd<N, false>(v, T, id.col, id.cj);
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) v[i] = cmulc(v[i], h[i]);
+      if (a.step_obj) {
+        const RView<N, float2> U(st + size_t(s - 1) * N * N, tid);
+#pragma unroll
+        for (int n = 0; n < R::R1; ++n) pu[n] = U[n];
+      }
+      rt_line_inv<N, false>(v, T, id.col, id.cj);
+      rt_cols_to_rows<N>(v, T, id);
+      rt_line_inv<N, true>(v, T, id.row, id.rj);
     }
   }
   float2* out = a.gP + (size_t(b) * MK + mk) * N * N;
@@ -806,24 +872,27 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
   const int planes = a.S * a.K;
   const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
   const int sl = plane / a.K, k = plane % a.K;
-  const int tid = threadIdx.x;
+  const unsigned tid = threadIdx.x;
   const RTIds<N> id;
-  const float2* tw = twiddles<N>();
   const size_t mstride = size_t(a.K) * a.S * N * N;
   const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.S + sl) * N * N;
   float2 v[R::R1];
+  {
+    const RView<N, float2> G0(src, tid);
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) {
-    const float2* sn = rt_reg<N>(src, n);
-    float2 x = sn[tid];
-    for (int m = 1; m < a.M; ++m) x = cadd(x, sn[m * mstride + tid]);
-    v[n] = x;
+    for (int n = 0; n < R::R1; ++n) v[n] = G0[n];
   }
-  rt_fft2_fwd<N>(v, T, id, tw);
-  const float sc = 1.0f / float(N * N);
+  for (int m = 1; m < a.M; ++m) {
+    const RView<N, float2> Gm(src + m * mstride, tid);
 #pragma unroll
-  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], rt_reg<N>(a.maskc, i)[tid] * sc);
-  rt_fft2_inv<N>(v, T, id, tw);
+    for (int n = 0; n < R::R1; ++n) v[n] = cadd(v[n], Gm[n]);
+  }
+  rt_fft2_fwd<N>(v, T, id);
+  const float sc = 1.0f / float(N * N);
+  const RView<N, float> mask(a.maskc, tid);
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], mask[i] * sc);
+  rt_fft2_inv<N>(v, T, id);
   float2* G = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + 8 * n] = v[n];

@@ -77,18 +77,51 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// v[k1] *= W_N^(+-j*k1) for k1 = 1..R1-1, built from tw[r*j] (r = 1..3) and
-// tw[4*q*j] (q = 1..R1/4-1): one table read per factor, one product per k1
+// Twiddle factors of the line split, indexed [row][j] (j = 0..7) so that a
+// lane reads row r at byte offset j*8 + r*64 through a buffer resource: one
+// lane offset per pass and compile-time row offsets, no vector address
+// arithmetic.  Rows 0..3 hold W_N^(r*j), rows 3+q (q = 1..R1/4-1) W_N^(4*q*j);
+// W_N^(j*k1) = W^(j*(k1 & 3)) * W^(4*j*(k1 >> 2)) costs one product.
+template <int N>
+struct RTTw {
+  static constexpr int ROWS = 3 + N / 32;
+  float v[ROWS * 16];
+};
+template <int N>
+constexpr RTTw<N> make_rt_tw() {
+  RTTw<N> t{};
+  for (int row = 0; row < RTTw<N>::ROWS; ++row) {
+    const int mult = row < 4 ? row : 4 * (row - 3);
+    for (int j = 0; j < 8; ++j) {
+      int e = (mult * j) % N;
+      if (2 * e > N) e -= N;
+      const double a = -2.0 * kPi * double(e) / double(N);
+      t.v[(row * 8 + j) * 2] = float(cx_cos(a));
+      t.v[(row * 8 + j) * 2 + 1] = float(cx_sin(a));
+    }
+  }
+  return t;
+}
+template <int N>
+__constant__ __align__(16) RTTw<N> g_rt_tw = make_rt_tw<N>();
+
+// v[k1] *= W_N^(+-j*k1) for k1 = 1..R1-1
 template <int N, bool INV>
-__device__ __forceinline__ void rt_twiddle(float2* v, unsigned j, const float2* __restrict__ tw) {
+__device__ __forceinline__ void rt_twiddle(float2* v, unsigned j) {
   constexpr int R1 = RT<N>::R1;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<RTTw<N>*>(&g_rt_tw<N>), 0, int(sizeof(RTTw<N>)), 0x00020000);
+  const unsigned voff = j * 8u;
+  auto row = [&](int r) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff + r * 64, 0, 0));
+  };
   float2 a[4], b[R1 / 4];
   a[0] = make_float2(1.f, 0.f);
   b[0] = a[0];
 #pragma unroll
-  for (unsigned r = 1; r < 4; ++r) a[r] = tw[r * j];
+  for (int r = 1; r < 4; ++r) a[r] = row(r);
 #pragma unroll
-  for (unsigned q = 1; q < R1 / 4; ++q) b[q] = tw[4 * q * j];
+  for (int q = 1; q < R1 / 4; ++q) b[q] = row(3 + q);
 #pragma unroll
   for (int k1 = 1; k1 < R1; ++k1) {
     const int r = k1 & 3, q = k1 >> 2;
@@ -104,7 +137,7 @@ __device__ __forceinline__ void rt_twiddle(float2* v, unsigned j, const float2* 
 //                                   rbase(jj) = ((CPT*j)*8 + (jj ^ j))*step + line0
 // so one exchange needs 8 lane addresses and compile-time offsets.  The line
 // origin and j pass through an empty asm so that these addresses (and the
-// twiddle reads) are recomputed at every exchange instead of being kept live
+// twiddle offsets) are recomputed at every exchange instead of being kept live
 // across the kernel.
 template <int N, bool ROW>
 struct XAddr {
@@ -124,12 +157,12 @@ struct XAddr {
 
 // forward line transform: in v[n] = x[j + 8n]; out v[c*8 + k2] = X[CPT*j + c + R1*k2]
 template <int N, bool ROW>
-__device__ __forceinline__ void rt_line_fwd(float2* v, float2* T, int line, int j, const float2* __restrict__ tw) {
+__device__ __forceinline__ void rt_line_fwd(float2* v, float2* T, int line, int j) {
   using R = RT<N>;
   constexpr int CS = 8 * XAddr<N, ROW>::STEP;  // offset of c
   j = XAddr<N, ROW>::opaque(j);
   dft<R::R1, false>(v);
-  rt_twiddle<N, false>(v, unsigned(j), tw);
+  rt_twiddle<N, false>(v, unsigned(j));
   const XAddr<N, ROW> X(line);
   wave_lds_sync();
 #pragma unroll
@@ -151,7 +184,7 @@ __device__ __forceinline__ void rt_line_fwd(float2* v, float2* T, int line, int 
 
 // inverse line transform (unnormalised): exact reverse of rt_line_fwd
 template <int N, bool ROW>
-__device__ __forceinline__ void rt_line_inv(float2* v, float2* T, int line, int j, const float2* __restrict__ tw) {
+__device__ __forceinline__ void rt_line_inv(float2* v, float2* T, int line, int j) {
   using R = RT<N>;
   constexpr int CS = 8 * XAddr<N, ROW>::STEP;
   j = XAddr<N, ROW>::opaque(j);
@@ -172,7 +205,7 @@ __device__ __forceinline__ void rt_line_inv(float2* v, float2* T, int line, int 
 #pragma unroll
     for (int c = 0; c < R::CPT; ++c) v[R::CPT * q + c] = p[c * CS];
   }
-  rt_twiddle<N, true>(v, unsigned(j), tw);
+  rt_twiddle<N, true>(v, unsigned(j));
   dft<R::R1, true>(v);
 }
 
@@ -225,17 +258,17 @@ __device__ __forceinline__ void rt_cols_to_rows(float2* v, float2* T, const RTId
 
 // Forward 2D transform: row phase natural samples -> column phase spectrum.
 template <int N>
-__device__ __forceinline__ void rt_fft2_fwd(float2* v, float2* T, const RTIds<N>& id, const float2* tw) {
-  rt_line_fwd<N, true>(v, T, id.row, id.rj, tw);
+__device__ __forceinline__ void rt_fft2_fwd(float2* v, float2* T, const RTIds<N>& id) {
+  rt_line_fwd<N, true>(v, T, id.row, id.rj);
   rt_rows_to_cols<N>(v, T, id);
-  rt_line_fwd<N, false>(v, T, id.col, id.cj, tw);
+  rt_line_fwd<N, false>(v, T, id.col, id.cj);
 }
 // Inverse 2D transform (unnormalised): column phase spectrum -> row phase samples.
 template <int N>
-__device__ __forceinline__ void rt_fft2_inv(float2* v, float2* T, const RTIds<N>& id, const float2* tw) {
-  rt_line_inv<N, false>(v, T, id.col, id.cj, tw);
+__device__ __forceinline__ void rt_fft2_inv(float2* v, float2* T, const RTIds<N>& id) {
+  rt_line_inv<N, false>(v, T, id.col, id.cj);
   rt_cols_to_rows<N>(v, T, id);
-  rt_line_inv<N, true>(v, T, id.row, id.rj, tw);
+  rt_line_inv<N, true>(v, T, id.row, id.rj);
 }
 
 }  // namespace pty
