@@ -381,17 +381,27 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_fprobe(const float2* __restrict
   }
 }
 
-// shift kernel E = exp(Kx*dx + Ky*dy) * mask with K = -2*pi*i*fftfreq * mask
-// (objective.py:309-313, 473): returns cos/sin of the phase, times the mask
+// Shift kernel E = exp(Kx*dx + Ky*dy) * mask with K = -2*pi*i*fftfreq * mask
+// (objective.py:309-313, 473).  Where the mask is 1 the exponent is a sum of
+// an x and a y term, so E = ex[px] * ey[py] * mask with the two axis factors
+// exp(-2*pi*i*fftfreq*d) tabulated once per window in LDS (2N sincos instead
+// of N^2); where the mask is 0, E is 0 either way.
 template <int N>
-__device__ __forceinline__ float2 shift_kernel(int py, int px, float dy, float dx, float m) {
+__device__ __forceinline__ float2 shift_axis(int p, float d) {
   using T = Tile<N>;
-  const float kx = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(px), N, 1.f)), m);
-  const float ky = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(py), N, 1.f)), m);
-  const float ph = fadd(fmul(kx, dx), fmul(ky, dy));
+  const float k = fmul(-6.2831855f, fftfreq_f(T::pos2freq(p), N, 1.f));
   float sn, cs;
-  sincosf(ph, &sn, &cs);
-  return make_float2(cs * m, sn * m);
+  sincosf(fmul(k, d), &sn, &cs);
+  return make_float2(cs, sn);
+}
+// exy[0..N) = x factors for dx, exy[N..2N) = y factors for dy; the caller
+// synchronises before use
+template <int N>
+__device__ __forceinline__ void shift_tables(float2* exy, float dy, float dx) {
+  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) exy[i] = i < N ? shift_axis<N>(i, dx) : shift_axis<N>(i - N, dy);
+}
+__device__ __forceinline__ float2 shift_at(const float2* exy, int n, int py, int px, float m) {
+  return cscale(cmul(exy[px], exy[n + py]), m);
 }
 
 template <int N>
@@ -404,10 +414,13 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_probe_shift(BatchArgs a) {
   const float2* src = a.fprobe + size_t(m) * N * N;
   const float s = 1.0f / float(N * N);
   const int tid = threadIdx.x;
+  __shared__ float2 exy[2 * N];
+  shift_tables<N>(exy, dy, dx);
+  __syncthreads();
 #pragma unroll 4
   for (int i = 0; i < T::PER_THREAD; ++i) {
     const int e = tid + i * T::NT, py = e / N, px = e % N;
-    const float2 E = shift_kernel<N>(py, px, dy, dx, a.mask[e]);
+    const float2 E = shift_at(exy, N, py, px, a.mask[e]);
     t[T::lds(py, px)] = cscale(cmul(src[e], E), s);
   }
   __syncthreads();
@@ -951,6 +964,9 @@ __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
     }
     __syncthreads();
     any = any || n > 0;
+    // unrolled so that the loads of several windows are in flight together;
+    // the sum is still taken in window order
+#pragma unroll 4
     for (int i = 0; i < n; ++i) {
       const int y0s = wy0[i], x0s = wx0[i];
       const float2* G = a.Ob + (size_t(wb[i]) * planes + plane) * N * N;
@@ -985,6 +1001,7 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_probegrad(BatchArgs a) {
   using T = Tile<N>;
   PTY_TILE(T, t, a.scratch);
   __shared__ double red[16];
+  __shared__ float2 exy[2 * N];
   const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
   const int tid = threadIdx.x;
   const float2* tw = twiddles<N>();
@@ -993,6 +1010,8 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_probegrad(BatchArgs a) {
   float2* acc = a.probe_part + (size_t(g) * a.M + m) * N * N;
   for (int b = g; b < a.B; b += a.G) {
     const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+    if (a.shift) shift_tables<N>(exy, dy, dx);
     // sum over object modes
 #pragma unroll 4
     for (int i = 0; i < T::PER_THREAD; ++i) {
@@ -1004,13 +1023,12 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_probegrad(BatchArgs a) {
     __syncthreads();
     if (a.shift) {
       tile_fft_fwd<N>(t, tw);
-      const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
       double sy = 0.0, sx = 0.0;
 #pragma unroll 4
       for (int i = 0; i < T::PER_THREAD; ++i) {
         const int e = tid + i * T::NT, py = e / N, px = e % N;
         const float mk = a.mask[e];
-        const float2 E = shift_kernel<N>(py, px, dy, dx, mk);
+        const float2 E = shift_at(exy, N, py, px, mk);
         const float2 Q = cmulc(t[T::lds(py, px)], E);
         acc[e] = cadd(acc[e], Q);
         if (a.step_pos) {
@@ -1055,6 +1073,16 @@ __global__ void k_pos_final(BatchArgs a) {
   }
   a.pos_grad[2 * idx] = float(y);
   a.pos_grad[2 * idx + 1] = float(x);
+}
+
+// part[0] = sum over the G group partials of the probe gradient, one thread
+// per element (in group order, so the sum is deterministic)
+__global__ __launch_bounds__(256) void k_probe_reduce(float2* __restrict__ part, int G, size_t per_group) {
+  const size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (e >= per_group) return;
+  float2 v = part[e];
+  for (int g = 1; g < G; ++g) v = cadd(v, part[size_t(g) * per_group + e]);
+  part[e] = v;
 }
 
 // probe_grad = ifft2(mask * fft2(sum_b interm)) (objective.py:927-933)
@@ -1261,7 +1289,8 @@ Layout plan_layout(const pty_problem* p) {
   const size_t n2 = size_t(p->n) * p->n;
   const int B = batch_windows(p);
   const int S = p->n_slices, K = p->n_obj_modes, M = p->n_probe_modes;
-  const int G = std::min(B, 64);
+  // probe-gradient groups: enough workgroups (G * M) to fill the chip
+  const int G = std::min(B, std::max(64, (1024 + M - 1) / M));
   const size_t objsz = size_t(p->obj_ny) * p->obj_nx * S * K * 8;
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t r = o; o += al(bytes); return r; };
@@ -1450,7 +1479,9 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   if (p->step_probe) {
     if (!bf->probe_grad) return PTY_ERR_ARG;
     TimeScope ts(KT_PROBE_FINAL, s);
-    k_probe_final<N><<<M, T::NT, 0, s>>>(a.probe_part, L.G, M, p->shift_active, mask,
+    const size_t per_group = size_t(M) * n2;
+    if (L.G > 1) k_probe_reduce<<<int((per_group + 255) / 256), 256, 0, s>>>(a.probe_part, L.G, per_group);
+    k_probe_final<N><<<M, T::NT, 0, s>>>(a.probe_part, 1, M, p->shift_active, mask,
                                           static_cast<float2*>(bf->probe_grad), a.scratch);
   }
   if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE) {
