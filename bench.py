@@ -11,8 +11,13 @@ The loss/sse readback of every pass is inside the timed region.
 
 Prints one JSON line: patterns/s (= positions x steps / seconds), the
 roofline of the dominant kernel (FP32 flops per launch / HIP-event average
-duration) and the CPU baseline (the oracle's complex64 numpy path on a
-bounded sample of positions, timed on this host).
+duration), the CPU baseline (the oracle's complex64 numpy path on a bounded
+sample of positions, timed on this host, with scipy.fft on one worker as PyPty
+calls it and on all host cores), and "recon": the wall-clock of a plain
+L-BFGS reconstruction of the single-slice workload (256x256 hexagonal-lattice
+phase object, 64 px probe, 16x16 scan, 20 epochs, object updates) through the
+run() driver on the GPU against the oracle's L-BFGS on one CPU core, with the
+NMSE each reaches and the CPU time to reach the GPU's final NMSE.
 """
 from __future__ import annotations
 
@@ -85,6 +90,71 @@ def cpu_baseline(datasets, n_sample, workers):
     return total_p / total_t, total_t, total_p
 
 
+def recon_benchmark(dev, epochs, cpu_epochs):
+    """Plain reconstruction of the single-slice workload: run() on the GPU, the oracle's bfgs_epoch on the CPU.
+
+    Both start from a flat object and the true probe and update the object only.  The NMSE is measured over
+    the scanned field of view (metrics.nmse: constant phase offset removed).  run() shifts positions to the
+    origin (pypty/utils.py:1730), so its object frame is offset by the scan origin."""
+    import numpy as np
+    import torch
+
+    from oracle import bfgs_oracle
+    from oracle import ms_oracle as orc
+    from pypty_amd import iterative, metrics, synth
+
+    ds = synth.config0(dev)
+    n = ds.probe.shape[0]
+    ny, nx = ds.obj.shape[:2]
+    P = ds.positions.shape[0]
+    pos = ds.positions.astype(np.float64) + ds.corrections
+    truth = metrics.projected_phase(ds.obj)
+    params = dict(dataset=ds.measured, probe=ds.probe, positions=pos, obj=np.ones((ny, nx, 1, 1), np.complex64),
+                  acc_voltage=300.0, pixel_size_x_A=ds.pixel_A, pixel_size_y_A=ds.pixel_A,
+                  slice_distances=np.array([ds.dz_A]), num_slices=1, algorithm="lsq_sqrt", epoch_max=epochs,
+                  update_obj=1, update_probe=0, update_probe_pos=0, compute_batch=256, output_folder="",
+                  print_flag=0)
+    np.random.seed(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = iterative.run(params)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    sh = np.round(pos.min(axis=0)).astype(int)
+    rec = metrics.projected_phase(res["obj"])
+    tr = truth[sh[0]:sh[0] + rec.shape[0], sh[1]:sh[1] + rec.shape[1]]
+    rec = rec[:tr.shape[0], :tr.shape[1]]
+    mask_g = metrics.fov_mask(tr.shape, pos - pos.min(axis=0), n)
+    nmse_gpu = metrics.nmse(rec, tr, mask_g)
+    mask_c = metrics.fov_mask(truth.shape, pos, n)
+    nmse0 = metrics.nmse(np.zeros_like(truth), truth, mask_c)
+    out = {"workload": "single slice: object 256x256 (hexagonal lattice, pure phase), probe 64 px x1 mode, "
+                       "16x16 raster (256 positions), dose 1e4, lsq_sqrt, object updates, flat start",
+           "epochs": epochs, "gpu_s": round(t_gpu, 3), "gpu_nmse": round(nmse_gpu, 5), "start_nmse": round(nmse0, 5),
+           "gpu_losses": [float(x) for x in res["losses"][:1] + res["losses"][-1:]]}
+    if cpu_epochs > 0:
+        state = {"obj": np.ones((ny, nx, 1, 1), np.complex64), "probe": orc.fourier_clean(ds.probe, cutoff=2 / 3)
+                 .astype(np.complex64), "positions": ds.positions, "corr": ds.corrections}
+        hist = bfgs_oracle.new_history()
+        data = {"measured": ds.measured, "sequence": np.arange(P), "dz": ds.dz_A, "wavelength": ds.wavelength_A,
+                "px": ds.pixel_A, "algorithm": "lsq_sqrt", "compute_batch": 256}
+        step, t_cpu, t_target, nmse_cpu = 1.0, 0.0, None, None
+        for e in range(cpu_epochs):
+            np.random.seed(e)
+            t1 = time.perf_counter()
+            r = bfgs_oracle.bfgs_epoch(state, hist, data, update_obj=True, update_probe=False, update_pos=False,
+                                       actual_step=step)
+            t_cpu += time.perf_counter() - t1
+            step = r[2]
+            nmse_cpu = metrics.nmse(metrics.projected_phase(state["obj"]), truth, mask_c)
+            if t_target is None and nmse_cpu <= nmse_gpu:
+                t_target = t_cpu
+        out.update({"cpu_epochs": cpu_epochs, "cpu_s": round(t_cpu, 3), "cpu_nmse": round(nmse_cpu, 5),
+                    "cpu_s_to_gpu_nmse": None if t_target is None else round(t_target, 3), "cpu_cores": 1,
+                    "speedup_to_target": None if t_target is None else round(t_target / t_gpu, 1)})
+    return out
+
+
 WORKLOADS = {
     "multislice": "multislice object 512x512x10 dz2A, probe 128px x5 modes, 64x64 scan (4096 positions), "
                   "batch 256, lsq_sqrt, object+probe+position gradients",
@@ -144,6 +214,8 @@ def main(argv=None):
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for rocprofv3 counter scaling)")
+    ap.add_argument("--recon-epochs", type=int, default=20,
+                    help="epochs of the reconstruction wall-clock measurement (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (one-GPU boxes); run with --gpus 1", file=sys.stderr)
@@ -226,6 +298,13 @@ def main(argv=None):
         cpu = {"value": round(v1, 3), "unit": "patterns/s", "cores": 1, "kind": "port",
                "sample": f"oracle complex64 numpy/scipy.fft path (PyPty op order), first {n_cpu} positions of "
                          f"each of the {len(datasets)} dataset(s) ({npos} patterns), scipy workers=1, {dt1:.1f} s"}
+        # the same sample with scipy.fft on all host cores (numpy's elementwise work stays on one core)
+        ncores = max(1, min(os.cpu_count() or 1, 16))
+        va, dta, _ = cpu_baseline(datasets, n_cpu, ncores)
+        cpu["all_cores"] = {"value": round(va, 3), "cores": ncores, "seconds": round(dta, 1)}
+
+    recon = recon_benchmark(dev, args.recon_epochs, args.recon_epochs if args.cpu_sample > 0 else 0) \
+        if args.recon_epochs > 0 else None
 
     flops_step = sum(pass_flops(*j.shape, True) * j.P for j in jobs)
     bytes_step = sum(pass_bytes(j.shape[0], j.shape[1], j.shape[3]) * j.P for j in jobs)
@@ -244,6 +323,7 @@ def main(argv=None):
                  "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
                  "data_gen_s": round(t_gen, 1)},
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kernels.items()},
+        "recon": recon,
     }
     print(json.dumps(out), flush=True)
     return 0
