@@ -47,6 +47,15 @@ extern "C" {
 #define PTY_LOSS_ML 1         /* Poisson: I - D log I */
 #define PTY_LOSS_LSQ 2        /* (I - D)^2 */
 #define PTY_LOSS_LSQ_SQRT_2 3 /* 0.5 (D - I)^2 / D for D > 0.5, I elsewhere */
+/* compressed data (objective.py:560-585): measured rows hold the J virtual
+ * detector coefficients D_j; the model's are c_j = sum_pix I * mask_j */
+#define PTY_LOSS_LSQ_COMPRESSED 4     /* (c - D)^2 */
+#define PTY_LOSS_GAUSS_COMPRESSED 5   /* (sqrt c - sqrt D)^2 */
+#define PTY_LOSS_POISSON_COMPRESSED 6 /* c - D log c */
+
+/* rows of `measured` */
+#define PTY_MEASURED_BY_POSITION 0 /* row p holds position p's pattern */
+#define PTY_MEASURED_BY_SEQUENCE 1 /* row i holds the pattern of sequence[i] (a streamed batch) */
 
 /* object layouts */
 #define PTY_LAYOUT_REFERENCE 0
@@ -97,6 +106,19 @@ typedef struct pty_problem {
   float tan_x, tan_y;    /* in-specimen tilt of the propagator */
   float damping_cutoff;  /* band-mask cutoff (utils.py:153) */
   float edge_multiplier; /* first-slice propagator power (>1 active) */
+  int n_masks;           /* J virtual detectors of the compressed losses */
+  /* per-batch preprocessing of raw patterns (utils.py:587-653, applied by
+   * objective.py:400-405 to every batch; not for compressed losses):
+   * roll by (data_shift_y, data_shift_x) and zero the edge band the
+   * reference zeroes, sum data_bin x data_bin blocks, scale by
+   * data_multiplier, zero-pad data_pad on each side.  raw_n: side of the raw
+   * patterns (0: n, no preprocessing unless other fields say so); the
+   * result must be n x n. */
+  int raw_n;
+  int data_shift_y, data_shift_x;
+  int data_bin;
+  int data_pad;
+  float data_multiplier; /* 0 is taken as 1 */
 } pty_problem;
 
 typedef struct pty_buffers {
@@ -104,7 +126,7 @@ typedef struct pty_buffers {
   const void* probe;         /* (M, n, n) */
   const int32_t* positions;  /* (P, 2) */
   const float* corrections;  /* (P, 2) */
-  const float* measured;     /* (P, n, n) */
+  const float* measured;     /* (rows, n, n), (rows, raw_n, raw_n) or (rows, n_masks); rows per measured_order */
   const int32_t* sequence;   /* (n_seq) position indices, each at most once */
   int n_seq;
   double* loss_sse;          /* out [2]: loss (x loss_weight), sse */
@@ -114,14 +136,21 @@ typedef struct pty_buffers {
   float* sim_patterns;       /* optional out (P, n, n) modelled intensity, or NULL */
   void* workspace;
   size_t workspace_bytes;
+  const float* masks;        /* (n_masks, n, n) virtual detectors, zero frequency at (n/2, n/2) */
+  int measured_order;        /* PTY_MEASURED_BY_* */
+  /* 1: add this call's gradients to those of the previous call made with the
+   * same problem and workspace (a pass split into batches whose patterns are
+   * streamed in); loss_sse still holds this call's sums only */
+  int accumulate;
 } pty_buffers;
 
 int pty_workspace_bytes(const pty_problem* p, size_t* bytes);
 
 /* One full forward + loss + adjoint pass over `sequence`:
  * replaces objective.loss_and_direction (pypty/objective.py:47-967) for
- * far-field multislice, lsq_sqrt / ml / lsq / lsq_sqrt_2 losses, object / probe / position
- * gradients, single slice distance, no tilts. */
+ * far-field multislice, the lsq_sqrt / ml / lsq / lsq_sqrt_2 losses and the
+ * three compressed ones, object / probe / position gradients, single slice
+ * distance, no tilts, with the reference's per-batch data preprocessing. */
 int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* stream);
 
 /* Multislice forward on pre-cut windows (pypty/multislice_core.py:9-82).

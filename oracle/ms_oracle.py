@@ -7,7 +7,8 @@ Arrays use the reference's own layouts:
   wavefield stash (B, N, N, S, M, K), exit wave (B, N, N, M, K).
 ``cdtype`` selects complex64 (the reference's "single" path, same op order)
 or complex128 (the FP64 oracle).  Supported subset: far field, single slice
-distance, no tilts, losses lsq_sqrt / ml / lsq / lsq_sqrt_2, object / probe /
+distance, no tilts, losses lsq_sqrt / ml / lsq / lsq_sqrt_2 and the compressed
+lsq / gauss / poisson (virtual detectors), object / probe /
 position gradients.
 """
 from __future__ import annotations
@@ -16,6 +17,8 @@ import numpy as np
 import scipy.fft as sf
 
 LSQ_SQRT, ML, LSQ, LSQ_SQRT_2 = "lsq_sqrt", "ml", "lsq", "lsq_sqrt_2"
+LSQ_COMPRESSED, GAUSS_COMPRESSED, POISSON_COMPRESSED = "lsq_compressed", "gauss_compressed", "poisson_compressed"
+COMPRESSED = (LSQ_COMPRESSED, GAUSS_COMPRESSED, POISSON_COMPRESSED)
 
 
 def _fdt(cdtype):
@@ -163,7 +166,7 @@ def scatteradd(full, my, mx, chop):
 def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *, dz, wavelength,
                        pixel_size_x, pixel_size_y, algorithm=LSQ_SQRT, loss_weight=1.0, cutoff=2 / 3,
                        edge=1.0, step_obj=True, step_probe=True, step_pos=False, compute_batch=64,
-                       cdtype=np.complex64, return_patterns=False):
+                       cdtype=np.complex64, return_patterns=False, masks=None):
     """objective.loss_and_direction restricted to the far-field multislice path.
 
     Follows pypty/objective.py:218-361 (grids, propagator, shift kernels),
@@ -173,6 +176,9 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     (probe-gradient band-limit).
     Returns loss, sse, obj_grad (Ny, Nx, S, K), probe_grad (N, N, M), pos_grad (P, 2)
     [, modelled intensity (P, N, N) if return_patterns].
+    For the compressed losses (objective.py:560-585) measured holds (P, J)
+    virtual-detector coefficients and masks the (J, N, N) detectors in the
+    DC-centred pattern layout.
     """
     fd = _fdt(cdtype)
     obj = np.asarray(obj, dtype=cdtype)
@@ -190,7 +196,10 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     obj_grad = np.zeros_like(obj)
     probe_grad = np.zeros((N, N, M), dtype=cdtype)
     pos_grad = np.zeros_like(corrections)
-    patterns = np.zeros(measured.shape, dtype=fd) if return_patterns else None
+    compressed = algorithm in COMPRESSED
+    if compressed:
+        masks = np.asarray(masks, dtype=fd)
+    patterns = np.zeros((measured.shape[0], N, N), dtype=fd) if return_patterns else None
     FP = fft2(probe[None], axes=(1, 2))
     loss, sse = fd(0), fd(0)
     seq = np.asarray(sequence)
@@ -245,6 +254,29 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
             loss = loss + np.sum(I[~nz])
             dI = np.ones_like(I)
             dI[nz] = I[nz] / meas[nz] - 1
+        elif compressed:
+            # objective.py:560-585: coefficients of the virtual detectors
+            c = np.sum(I[:, None, :, :] * masks[None], axis=(2, 3))
+            if algorithm == LSQ_COMPRESSED:
+                diff = c - meas
+                tterm = np.sum(diff ** 2)
+                loss = loss + tterm
+                sse = sse + tterm
+                dc = 2 * diff
+            elif algorithm == GAUSS_COMPRESSED:
+                ms, cs = np.sqrt(meas), np.sqrt(c)
+                sse = sse + np.sum((c - meas) ** 2)
+                dc = cs - ms
+                loss = loss + np.sum(dc ** 2)
+                cs[cs == 0] = 1
+                dc = dc / cs
+            else:
+                dc = c - meas
+                sse = sse + np.sum(dc ** 2)
+                c[c == 0] = 1
+                loss = loss + np.sum(c - meas * np.log(c))
+                dc = dc / c
+            dI = np.sum(masks[None] * dc[:, :, None, None], axis=1)
         else:
             raise ValueError(f"unsupported loss {algorithm}")
         if not coherent:
@@ -273,6 +305,39 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     if return_patterns:
         out = out + (patterns,)
     return out
+
+
+def preprocess_dataset(dataset, data_shift_vector=(0, 0), data_bin=1, data_pad=0, data_multiplier=1.0,
+                       upsample_pattern=1):
+    """pypty/utils.py:626-651 as objective.py:400-405 calls it per batch (not compressed, force_pad):
+    np.roll along y then x, each followed by zeroing the edge band the reference zeroes (rows [-sy:]
+    for sy > 0, [:-sy] for sy < 0); block sums of data_bin x data_bin pixels (the reference's loop
+    indexes an undefined name here, the evident intent is used); times data_multiplier; zero padding
+    of data_pad // upsample_pattern on each side."""
+    d = np.array(dataset, dtype=np.float64)
+    sy, sx = int(data_shift_vector[0]), int(data_shift_vector[1])
+    if sy != 0:
+        d = np.roll(d, sy, axis=1)
+        if sy > 0:
+            d[:, -sy:, :] = 0
+        else:
+            d[:, :-sy, :] = 0
+    if sx != 0:
+        d = np.roll(d, sx, axis=2)
+        if sx > 0:
+            d[:, :, -sx:] = 0
+        else:
+            d[:, :, :-sx] = 0
+    if data_bin > 1:
+        d2 = np.zeros((d.shape[0], d.shape[1] // data_bin, d.shape[2] // data_bin))
+        for i1 in range(data_bin):
+            for j1 in range(data_bin):
+                d2 += d[:, i1::data_bin, j1::data_bin]
+        d = d2
+    if data_multiplier != 1:
+        d = d * data_multiplier
+    pad = int(data_pad) // int(upsample_pattern)
+    return np.pad(d, [[0, 0], [pad, pad], [pad, pad]])
 
 
 # --------------------------------------------------------------- L-BFGS

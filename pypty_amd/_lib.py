@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libpypty_amd.so")
 PTY_OK = 0
 PTY_FFT_FWD, PTY_FFT_INV, PTY_FFT_FWD_SHIFT, PTY_FFT_INV_ISHIFT = 0, 1, 2, 3
 PTY_LOSS_LSQ_SQRT, PTY_LOSS_ML, PTY_LOSS_LSQ, PTY_LOSS_LSQ_SQRT_2 = 0, 1, 2, 3
+PTY_LOSS_LSQ_COMPRESSED, PTY_LOSS_GAUSS_COMPRESSED, PTY_LOSS_POISSON_COMPRESSED = 4, 5, 6
+PTY_MEASURED_BY_POSITION, PTY_MEASURED_BY_SEQUENCE = 0, 1
 PTY_LAYOUT_REFERENCE, PTY_LAYOUT_PLANAR = 0, 1
 
 EXPORTED = (
@@ -36,7 +38,9 @@ class PtyProblem(ctypes.Structure):
         ("slice_distance", ctypes.c_float), ("wavelength", ctypes.c_float),
         ("pixel_size_x", ctypes.c_float), ("pixel_size_y", ctypes.c_float),
         ("tan_x", ctypes.c_float), ("tan_y", ctypes.c_float), ("damping_cutoff", ctypes.c_float),
-        ("edge_multiplier", ctypes.c_float),
+        ("edge_multiplier", ctypes.c_float), ("n_masks", ctypes.c_int), ("raw_n", ctypes.c_int),
+        ("data_shift_y", ctypes.c_int), ("data_shift_x", ctypes.c_int), ("data_bin", ctypes.c_int),
+        ("data_pad", ctypes.c_int), ("data_multiplier", ctypes.c_float),
     ]
 
 
@@ -46,7 +50,8 @@ class PtyBuffers(ctypes.Structure):
         ("corrections", ctypes.c_void_p), ("measured", ctypes.c_void_p), ("sequence", ctypes.c_void_p),
         ("n_seq", ctypes.c_int), ("loss_sse", ctypes.c_void_p), ("obj_grad", ctypes.c_void_p),
         ("probe_grad", ctypes.c_void_p), ("pos_grad", ctypes.c_void_p), ("sim_patterns", ctypes.c_void_p),
-        ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
+        ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("masks", ctypes.c_void_p),
+        ("measured_order", ctypes.c_int), ("accumulate", ctypes.c_int),
     ]
 
 

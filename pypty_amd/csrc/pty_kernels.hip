@@ -321,7 +321,20 @@ struct BatchArgs {
   float2* obj_grad;         // planar
   float* pos_grad;          // (P, 2)
   int G;                    // probe-gradient groups
+  // measured data: rows by position index or by sequence slot; raw patterns
+  // preprocessed into meas_stage (batch slot order) when prep is set;
+  // virtual detectors of the compressed losses
+  int meas_order, prep, raw_n, sy, sx, bin, pad, J;
+  float mult;
+  float* meas_stage;        // (B, N, N)
+  const float* masks;       // (J, N, N)
 };
+
+// measured row of batch slot b (sequence entry b0 + b, position idx)
+__device__ __forceinline__ const float* meas_row(const BatchArgs& a, int b, int idx, size_t row_len) {
+  if (a.prep) return a.meas_stage + size_t(b) * row_len;
+  return a.meas + size_t(a.meas_order == PTY_MEASURED_BY_SEQUENCE ? a.b0 + b : idx) * row_len;
+}
 
 template <int N>
 __global__ __launch_bounds__(Tile<N>::NT) void k_patch(BatchArgs a) {
@@ -520,7 +533,7 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
     }
     return;
   }
-  const float* D = a.meas + size_t(idx) * N * N;
+  const float* D = meas_row(a, b, idx, size_t(N) * N);
   const float2* psi = a.Psi + size_t(b) * MK * N * N;
   const float scale = a.loss_weight * float(N) * float(N);
   double lsum = 0.0, ssum = 0.0;
@@ -572,6 +585,135 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
   if (threadIdx.x == 0) {
     a.loss_part[slot] = L;
     a.loss_part[slot + 1] = Ssum;
+  }
+}
+
+// Per-batch preprocessing of raw patterns (utils.py:587-653 as
+// objective.py:400-405 calls it): roll by (sy, sx) with the edge band that the
+// reference zeroes after each roll (rows [H-sy, H) for sy > 0, [0, -sy) for
+// sy < 0; columns likewise), sum of bin x bin blocks, times mult, zero pad of
+// `pad` on each side.  One thread per output pixel of the batch.
+template <int N>
+__global__ __launch_bounds__(256) void k_preprocess(BatchArgs a) {
+  const int b = blockIdx.x / ((N * N + 255) / 256);
+  const int e = (blockIdx.x % ((N * N + 255) / 256)) * 256 + threadIdx.x;
+  if (e >= N * N) return;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  float v = 0.f;
+  const int H = a.raw_n, nb = H / a.bin;
+  const int yb = e / N - a.pad, xb = e % N - a.pad;
+  if (idx >= 0 && yb >= 0 && yb < nb && xb >= 0 && xb < nb) {
+    const float* raw =
+        a.meas + size_t(a.meas_order == PTY_MEASURED_BY_SEQUENCE ? a.b0 + b : idx) * size_t(H) * H;
+    for (int i1 = 0; i1 < a.bin; ++i1) {
+      const int y = yb * a.bin + i1;
+      if ((a.sy > 0 && y >= H - a.sy) || (a.sy < 0 && y < -a.sy)) continue;
+      const int ys = ((y - a.sy) % H + H) % H;
+      for (int j1 = 0; j1 < a.bin; ++j1) {
+        const int x = xb * a.bin + j1;
+        if ((a.sx > 0 && x >= H - a.sx) || (a.sx < 0 && x < -a.sx)) continue;
+        v += raw[size_t(ys) * H + ((x - a.sx) % H + H) % H];
+      }
+    }
+    v *= a.mult;
+  }
+  a.meas_stage[size_t(b) * N * N + e] = v;
+}
+
+// Compressed losses (objective.py:560-585): the model's virtual detector
+// coefficients c_j = sum_pix I * mask_j (FP64 sums), the loss and sse of each
+// against the measured coefficients, and dL/dI = sum_j mask_j dL/dc_j.  One
+// workgroup per window; coef holds I between the passes.
+constexpr int kMaxMasks = 256;
+
+template <int N, bool RTL>
+__global__ __launch_bounds__(256) void k_loss_c(BatchArgs a) {
+  using T = Tile<N>;
+  constexpr int CH = LossSplit<N>::CHUNKS;
+  __shared__ double red[8];
+  __shared__ float dcs[kMaxMasks];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const int MK = a.M * a.K;
+  const size_t slot = size_t(a.b0 + b) * CH * 2;
+  float* coef = a.coef + size_t(b) * N * N;
+  if (tid < 2 * CH) a.loss_part[slot + tid] = 0.0;
+  if (idx < 0) {
+    for (int e = tid; e < N * N; e += blockDim.x) coef[e] = 0.f;
+    return;
+  }
+  auto pix = [](int e) {
+    int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
+    if constexpr (RTL) RT<N>::cspec(e, fy, fx);
+    return ((fy + N / 2) % N) * N + (fx + N / 2) % N;
+  };
+  const float2* psi = a.Psi + size_t(b) * MK * N * N;
+  for (int e = tid; e < N * N; e += blockDim.x) {
+    float I = 0.f;
+    for (int j = 0; j < MK; ++j) I += cabs2(psi[size_t(j) * N * N + e]);
+    if (MK > 1) I = I / float(MK);
+    coef[e] = I;
+    if (a.sim) a.sim[size_t(idx) * N * N + pix(e)] = I;
+  }
+  __syncthreads();
+  const float* D = meas_row(a, b, idx, size_t(a.J));
+  double lsum = 0.0, ssum = 0.0;  // meaningful in thread 0
+  constexpr int JC = 8;
+  for (int j0 = 0; j0 < a.J; j0 += JC) {
+    double acc[JC];
+#pragma unroll
+    for (int q = 0; q < JC; ++q) acc[q] = 0.0;
+    for (int e = tid; e < N * N; e += blockDim.x) {
+      const double I = coef[e];
+      const int p = pix(e);
+#pragma unroll
+      for (int q = 0; q < JC; ++q)
+        if (j0 + q < a.J) acc[q] += I * double(a.masks[size_t(j0 + q) * N * N + p]);
+    }
+#pragma unroll
+    for (int q = 0; q < JC; ++q) {
+      if (j0 + q >= a.J) break;
+      const double tot = block_sum(acc[q], red);
+      if (tid == 0) {
+        float c = float(tot);
+        const float d = D[j0 + q];
+        float dc;
+        if (a.algorithm == PTY_LOSS_LSQ_COMPRESSED) {
+          const float diff = c - d;
+          const double t2 = double(diff) * double(diff);
+          lsum += t2;
+          ssum += t2;
+          dc = 2.f * diff;
+        } else if (a.algorithm == PTY_LOSS_GAUSS_COMPRESSED) {
+          float cs = sqrtf(c);
+          const float r = c - d, dd = cs - sqrtf(d);
+          ssum += double(r) * double(r);
+          lsum += double(dd) * double(dd);
+          if (cs == 0.f) cs = 1.f;
+          dc = dd / cs;
+        } else {
+          const float r = c - d;
+          ssum += double(r) * double(r);
+          if (c == 0.f) c = 1.f;
+          lsum += double(c) - double(d) * double(logf(c));
+          dc = r / c;
+        }
+        dcs[j0 + q] = dc;
+      }
+    }
+  }
+  __syncthreads();
+  const float scale = a.loss_weight * float(N) * float(N);
+  for (int e = tid; e < N * N; e += blockDim.x) {
+    const int p = pix(e);
+    float l = 0.f;
+    for (int j = 0; j < a.J; ++j) l += a.masks[size_t(j) * N * N + p] * dcs[j];
+    if (MK > 1) l = l / float(MK);
+    coef[e] = l * scale;
+  }
+  if (tid == 0) {
+    a.loss_part[slot] = lsum;
+    a.loss_part[slot + 1] = ssum;
   }
 }
 
@@ -1075,14 +1217,15 @@ __global__ void k_pos_final(BatchArgs a) {
   a.pos_grad[2 * idx + 1] = float(x);
 }
 
-// part[0] = sum over the G group partials of the probe gradient, one thread
-// per element (in group order, so the sum is deterministic)
-__global__ __launch_bounds__(256) void k_probe_reduce(float2* __restrict__ part, int G, size_t per_group) {
+// sum over the G group partials of the probe gradient, one thread per
+// element (in group order, so the sum is deterministic)
+__global__ __launch_bounds__(256) void k_probe_reduce(const float2* __restrict__ part, int G, size_t per_group,
+                                                     float2* __restrict__ sum) {
   const size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (e >= per_group) return;
   float2 v = part[e];
   for (int g = 1; g < G; ++g) v = cadd(v, part[size_t(g) * per_group + e]);
-  part[e] = v;
+  sum[e] = v;
 }
 
 // probe_grad = ifft2(mask * fft2(sum_b interm)) (objective.py:927-933)
@@ -1251,8 +1394,8 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 
 // ---------------------------------------------------------- workspace
 struct Layout {
-  size_t H, H0, mask, maskc, fprobe, objp, gradp, Ob, Pp, stash, Psi, coef, Gbuf, gP, probe_part, pos_part, loss_part,
-      scratch, total;
+  size_t H, H0, mask, maskc, fprobe, objp, gradp, Ob, Pp, stash, Psi, coef, Gbuf, gP, probe_part, probe_sum,
+      pos_part, loss_part, meas_stage, scratch, total;
   int B, G;
 };
 inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
@@ -1284,6 +1427,16 @@ int batch_windows(const pty_problem* p) {
   return B;
 }
 
+bool compressed(int algorithm) {
+  return algorithm >= PTY_LOSS_LSQ_COMPRESSED && algorithm <= PTY_LOSS_POISSON_COMPRESSED;
+}
+// raw patterns need the per-batch preprocessing pass
+bool preprocessing(const pty_problem* p) {
+  if (compressed(p->algorithm)) return false;
+  return (p->raw_n != 0 && p->raw_n != p->n) || p->data_shift_y != 0 || p->data_shift_x != 0 || p->data_bin > 1 ||
+         p->data_pad != 0 || (p->data_multiplier != 0.f && p->data_multiplier != 1.f);
+}
+
 Layout plan_layout(const pty_problem* p) {
   Layout L{};
   const size_t n2 = size_t(p->n) * p->n;
@@ -1309,6 +1462,8 @@ Layout plan_layout(const pty_problem* p) {
   L.Gbuf = 0;
   L.gP = take(size_t(B) * M * K * n2 * 8);
   L.probe_part = take(size_t(G) * M * n2 * 8);
+  L.probe_sum = take(size_t(M) * n2 * 8);
+  L.meas_stage = preprocessing(p) ? take(size_t(B) * n2 * 4) : 0;
   L.pos_part = take(size_t(B) * M * 2 * 8);
   L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
   if (p->n > 128) {
@@ -1328,7 +1483,13 @@ int validate(const pty_problem* p) {
   if (!pty_supported_window(p->n) && p->n != 192 && p->n != 256) return PTY_ERR_SIZE;
   if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1 || p->compute_batch < 1) return PTY_ERR_ARG;
   if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
-  if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_LSQ_SQRT_2) return PTY_ERR_ARG;
+  if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_POISSON_COMPRESSED) return PTY_ERR_ARG;
+  if (compressed(p->algorithm) && (p->n_masks < 1 || p->n_masks > kMaxMasks)) return PTY_ERR_ARG;
+  if (preprocessing(p)) {
+    const int raw = p->raw_n ? p->raw_n : p->n, bin = std::max(p->data_bin, 1);
+    if (raw % bin != 0 || raw / bin + 2 * p->data_pad != p->n || p->data_pad < 0) return PTY_ERR_SIZE;
+    if (std::abs(p->data_shift_y) >= raw || std::abs(p->data_shift_x) >= raw) return PTY_ERR_ARG;
+  }
   return PTY_OK;
 }
 
@@ -1378,9 +1539,16 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
     else
       k_forward<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
   }
+  if (a.prep) {
+    TimeScope ts(KT_LOSS, s);
+    k_preprocess<N><<<a.B * ((N * N + 255) / 256), 256, 0, s>>>(a);
+  }
   {
     TimeScope ts(KT_LOSS, s);
-    k_loss<N, RTL><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
+    if (compressed(a.algorithm))
+      k_loss_c<N, RTL><<<a.B, 256, 0, s>>>(a);
+    else
+      k_loss<N, RTL><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
   }
   if (p->step_obj || p->step_probe || p->step_pos) {
     TimeScope ts(KT_ADJOINT, s);
@@ -1443,11 +1611,14 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     objp = tmp;
     gradp = reinterpret_cast<float2*>(ws + L.gradp);
   }
+  if (compressed(p->algorithm) && !bf->masks) return PTY_ERR_ARG;
+  if (bf->measured_order != PTY_MEASURED_BY_POSITION && bf->measured_order != PTY_MEASURED_BY_SEQUENCE)
+    return PTY_ERR_ARG;
   if (p->step_obj) {
     if (!bf->obj_grad) return PTY_ERR_ARG;
-    if ((rc = zero_fill(gradp, objelems * 8, s))) return rc;
+    if (!bf->accumulate && (rc = zero_fill(gradp, objelems * 8, s))) return rc;
   }
-  if (p->step_probe || p->step_pos) {
+  if ((p->step_probe || p->step_pos) && !bf->accumulate) {
     if ((rc = zero_fill(ws + L.probe_part, size_t(L.G) * M * n2 * 8, s))) return rc;
   }
   BatchArgs a{};
@@ -1465,6 +1636,14 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.pos_part = reinterpret_cast<double*>(ws + L.pos_part); a.loss_part = reinterpret_cast<double*>(ws + L.loss_part);
   a.sim = bf->sim_patterns; a.obj_grad = gradp; a.pos_grad = bf->pos_grad; a.G = L.G;
   a.scratch = L.scratch ? reinterpret_cast<float2*>(ws + L.scratch) : nullptr;
+  a.meas_order = bf->measured_order;
+  a.prep = preprocessing(p) ? 1 : 0;
+  a.raw_n = p->raw_n ? p->raw_n : N;
+  a.sy = p->data_shift_y; a.sx = p->data_shift_x; a.bin = std::max(p->data_bin, 1); a.pad = p->data_pad;
+  a.mult = (p->data_multiplier == 0.f) ? 1.f : p->data_multiplier;
+  a.meas_stage = L.meas_stage ? reinterpret_cast<float*>(ws + L.meas_stage) : nullptr;
+  a.masks = bf->masks;
+  a.J = compressed(p->algorithm) ? p->n_masks : 0;
   for (int b0 = 0; b0 < bf->n_seq; b0 += L.B) {
     a.b0 = b0;
     a.B = std::min(L.B, bf->n_seq - b0);
@@ -1480,8 +1659,9 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     if (!bf->probe_grad) return PTY_ERR_ARG;
     TimeScope ts(KT_PROBE_FINAL, s);
     const size_t per_group = size_t(M) * n2;
-    if (L.G > 1) k_probe_reduce<<<int((per_group + 255) / 256), 256, 0, s>>>(a.probe_part, L.G, per_group);
-    k_probe_final<N><<<M, T::NT, 0, s>>>(a.probe_part, 1, M, p->shift_active, mask,
+    float2* psum = reinterpret_cast<float2*>(ws + L.probe_sum);
+    k_probe_reduce<<<int((per_group + 255) / 256), 256, 0, s>>>(a.probe_part, L.G, per_group, psum);
+    k_probe_final<N><<<M, T::NT, 0, s>>>(psum, 1, M, p->shift_active, mask,
                                           static_cast<float2*>(bf->probe_grad), a.scratch);
   }
   if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE) {

@@ -2,12 +2,20 @@
 
 ``loss_and_direction`` keeps the reference's 77-argument signature and
 10-tuple result.  The far-field multislice path with the lsq_sqrt / ml / lsq /
-lsq_sqrt_2 losses and object, probe and position gradients runs entirely in the gfx950
-kernels behind ``pty_loss_and_direction``; arguments that select features
-outside that path (near field, tilts, compressed losses, static background,
-aberrations, beam current, phase plates, regularisers, data padding /
-binning / upsampling) raise ``NotImplementedError`` instead of silently
-computing something else.
+lsq_sqrt_2 losses, the lsq / gauss / poisson compressed losses (virtual
+detectors, objective.py:560-585) and object, probe and position gradients
+runs entirely in the gfx950 kernels behind ``pty_loss_and_direction``,
+including the per-batch data preprocessing (shift, bin, multiplier, pad;
+utils.py:587-653) and, with ``load_one_by_one`` and patterns in host memory,
+streaming of each batch's patterns through pinned buffers
+(objective.py:383-408).  Arguments that select features outside that path
+(near field, tilts, static background, aberrations, beam current, phase
+plates, regularisers, upsampling) raise ``NotImplementedError`` instead of
+silently computing something else.
+
+Binning follows the evident intent of utils.py:639-644 (sum of the
+data_bin x data_bin blocks); the reference's loop indexes an undefined name
+there and fails for data_bin > 1.
 
 ``PtychoPass`` is the allocation-owning engine under it: it caches the
 device workspace per problem shape so repeated passes (line-search trials,
@@ -24,7 +32,8 @@ import torch
 from . import _lib
 
 LOSSES = {"lsq_sqrt": _lib.PTY_LOSS_LSQ_SQRT, "ml": _lib.PTY_LOSS_ML, "lsq": _lib.PTY_LOSS_LSQ,
-          "lsq_sqrt_2": _lib.PTY_LOSS_LSQ_SQRT_2}
+          "lsq_sqrt_2": _lib.PTY_LOSS_LSQ_SQRT_2, "lsq_compressed": _lib.PTY_LOSS_LSQ_COMPRESSED,
+          "gauss_compressed": _lib.PTY_LOSS_GAUSS_COMPRESSED, "poisson_compressed": _lib.PTY_LOSS_POISSON_COMPRESSED}
 
 
 @dataclass(frozen=True)
@@ -54,6 +63,18 @@ class PassConfig:
     tan_y: float = 0.0
     damping_cutoff: float = 2.0 / 3.0
     edge_multiplier: float = 1.0
+    n_masks: int = 0
+    # per-batch preprocessing of raw patterns (pypty/utils.py:587-653)
+    raw_n: int = 0
+    data_shift_y: int = 0
+    data_shift_x: int = 0
+    data_bin: int = 1
+    data_pad: int = 0
+    data_multiplier: float = 1.0
+
+    @property
+    def compressed(self) -> bool:
+        return "compressed" in self.algorithm
 
     def to_c(self) -> _lib.PtyProblem:
         if self.algorithm not in LOSSES:
@@ -65,7 +86,9 @@ class PassConfig:
             int(bool(self.step_obj)), int(bool(self.step_probe)), int(bool(self.step_pos)),
             int(bool(self.shift_active)), float(self.loss_weight), float(self.slice_distance),
             float(self.wavelength), float(self.pixel_size_x), float(self.pixel_size_y), float(self.tan_x),
-            float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier))
+            float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier), int(self.n_masks),
+            int(self.raw_n), int(self.data_shift_y), int(self.data_shift_x), int(self.data_bin), int(self.data_pad),
+            float(self.data_multiplier))
 
 
 class PtychoPass:
@@ -99,9 +122,14 @@ class PtychoPass:
         self.loss_sse = torch.zeros(2, dtype=torch.float64, device=self.device)
 
     def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
-            pos_grad=None, sim_patterns=None, sync=True):
+            pos_grad=None, sim_patterns=None, sync=True, masks=None, measured_by_sequence=False, accumulate=False,
+            loss_sse=None):
         """Launch the pass.  probe_mnn is mode-major (M, n, n).  Returns (loss, sse) as floats
-        when ``sync`` else the device tensor [loss, sse]."""
+        when ``sync`` else the device tensor [loss, sse].
+
+        measured_by_sequence: row i of ``measured`` holds the pattern of sequence[i] (a streamed batch);
+        accumulate: add to the gradients of the previous call on this workspace; loss_sse: a device
+        float64 tensor of 2 elements to receive this call's sums (default: the engine's own)."""
         c = self.cfg
         for t, name in ((obj, "obj"), (probe_mnn, "probe"), (positions, "positions"), (measured, "measured"),
                         (sequence, "sequence")):
@@ -120,17 +148,75 @@ class PtychoPass:
             raise ValueError("corrections required")
         if c.step_pos and pos_grad is None:
             raise ValueError("pos_grad buffer required")
+        if c.compressed:
+            _lib.require_device_tensor(masks, "masks")
+            if masks.dtype != torch.float32 or tuple(masks.shape) != (c.n_masks, c.n, c.n):
+                raise ValueError(f"pypty_amd: masks must be float32 ({c.n_masks}, {c.n}, {c.n})")
+            if measured.shape[-1] != c.n_masks:
+                raise ValueError("pypty_amd: compressed data need one coefficient per virtual detector")
+        out = self.loss_sse if loss_sse is None else loss_sse
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         bufs = _lib.PtyBuffers(
             ptr(obj), ptr(probe_mnn), ptr(positions), ptr(corrections), ptr(measured), ptr(sequence),
-            int(sequence.numel()), ptr(self.loss_sse), ptr(obj_grad), ptr(probe_grad), ptr(pos_grad),
-            ptr(sim_patterns), ptr(self.workspace), self.workspace_bytes)
+            int(sequence.numel()), ptr(out), ptr(obj_grad), ptr(probe_grad), ptr(pos_grad),
+            ptr(sim_patterns), ptr(self.workspace), self.workspace_bytes, ptr(masks) if c.compressed else None,
+            _lib.PTY_MEASURED_BY_SEQUENCE if measured_by_sequence else _lib.PTY_MEASURED_BY_POSITION,
+            int(bool(accumulate)))
         rc = self.lib.pty_loss_and_direction(ctypes.byref(self.c_problem), ctypes.byref(bufs),
                                               _lib.stream_ptr(self.device))
         _lib.check(rc, "loss_and_direction")
         if not sync:
-            return self.loss_sse
-        loss, sse = self.loss_sse.tolist()
+            return out
+        loss, sse = out.tolist()
+        return loss, sse
+
+    def run_streamed(self, obj, probe_mnn, positions, corrections, measured_host, sequence_host, *, obj_grad=None,
+                     probe_grad=None, pos_grad=None, masks=None):
+        """The same pass with the measured patterns left in host memory (objective.py:383-408,
+        load_one_by_one): each batch's rows are gathered into one of two pinned buffers, copied to the
+        device on a side stream while the previous batch computes, and the batch runs as an
+        accumulating call over its slice of the sequence.  Returns (loss, sse)."""
+        c = self.cfg
+        seq = np.asarray(sequence_host, dtype=np.int64).ravel()
+        src = measured_host if isinstance(measured_host, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(measured_host, dtype=np.float32))
+        if src.is_cuda or src.dtype != torch.float32:
+            raise TypeError("pypty_amd: streamed patterns must be float32 in host memory")
+        B = max(1, int(c.compute_batch))
+        row_shape = tuple(src.shape[1:])
+        pinned = [torch.empty((B,) + row_shape, dtype=torch.float32).pin_memory() for _ in range(2)]
+        staged = [torch.empty((B,) + row_shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        copied = [torch.cuda.Event() for _ in range(2)]  # staged[slot] holds its batch
+        free = [torch.cuda.Event() for _ in range(2)]    # the batch that read staged[slot] has run
+        main = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(self.device)
+        starts = list(range(0, len(seq), B))
+        sums = torch.zeros((len(starts), 2), dtype=torch.float64, device=self.device)
+        seq_dev = torch.as_tensor(seq, dtype=torch.int32).to(self.device)
+
+        def stage(k):
+            rows = torch.from_numpy(seq[starts[k]:starts[k] + B])
+            slot = k % 2
+            copied[slot].synchronize()  # the pinned buffer's previous copy has finished
+            torch.index_select(src, 0, rows, out=pinned[slot][:len(rows)])
+            with torch.cuda.stream(side):
+                side.wait_event(free[slot])
+                staged[slot][:len(rows)].copy_(pinned[slot][:len(rows)], non_blocking=True)
+                copied[slot].record(side)
+
+        if starts:
+            stage(0)
+        for k, i0 in enumerate(starts):
+            slot = k % 2
+            sub = seq_dev[i0:i0 + B]
+            main.wait_event(copied[slot])
+            self.run(obj, probe_mnn, positions, corrections, staged[slot][:sub.numel()], sub, obj_grad=obj_grad,
+                     probe_grad=probe_grad, pos_grad=pos_grad, sync=False, masks=masks, measured_by_sequence=True,
+                     accumulate=k > 0, loss_sse=sums[k])
+            free[slot].record(main)
+            if k + 1 < len(starts):
+                stage(k + 1)  # the host gathers the next batch while this one computes
+        loss, sse = sums.sum(0).tolist()
         return loss, sse
 
 
@@ -198,11 +284,8 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     _unsupported("beam_current", beam_current is not None)
     _unsupported("phase_plate_in_h5", phase_plate_in_h5 is not None)
     _unsupported("phase_only_obj", bool(phase_only_obj))
-    _unsupported("data_pad", int(data_pad) != 0)
-    _unsupported("data_bin", int(data_bin) != 1)
     _unsupported("upsample_pattern", upsample_pattern != 1)
-    _unsupported("data_multiplier", float(data_multiplier) != 1.0)
-    _unsupported("data_shift_vector", any(int(v) != 0 for v in data_shift_vector))
+    _unsupported("data_pad without use_padded_intensity", int(data_pad) != 0 and not use_padded_intensity)
     _unsupported("data_simulation_flag", bool(data_simulation_flag))
     _unsupported("smooth_rolloff", float(smooth_rolloff) != 0.0)
     for name, w in (("fast_axis_reg_weight_positions", fast_axis_reg_weight_positions),
@@ -232,8 +315,23 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     probe_mnn = full_probe.to(device).permute(2, 0, 1).contiguous()
     positions = _as_device(this_pos_array, torch.int32, device)
     corrections = _as_device(this_pos_correction, torch.float32, device)
-    measured = _as_device(measured_array, torch.float32, device)
+    # load_one_by_one with patterns in host memory streams them per batch (objective.py:383-408)
+    stream = bool(load_one_by_one) and not (isinstance(measured_array, torch.Tensor) and measured_array.is_cuda)
+    if stream:
+        measured = (measured_array.detach().to(torch.float32).contiguous() if isinstance(measured_array, torch.Tensor)
+                    else torch.from_numpy(np.ascontiguousarray(measured_array, dtype=np.float32)))
+    else:
+        measured = _as_device(measured_array, torch.float32, device)
     seq = _as_device(this_chopped_sequence, torch.int32, device)
+    compressed = "compressed" in algorithm_type
+    masks_dev = None
+    if compressed:
+        if masks is None:
+            raise ValueError("pypty_amd.loss_and_direction: compressed losses need the virtual detectors (masks)")
+        masks_dev = _as_device(masks, torch.float32, device)
+    # per-batch preprocessing (utils.py:587-653; skipped for compressed data as there)
+    shift_y, shift_x = (0, 0) if compressed else (int(data_shift_vector[0]), int(data_shift_vector[1]))
+    pad = 0 if compressed else int(data_pad) // int(upsample_pattern)
     check_windows_inside(positions, seq, n, ny, nx)
     shift_active = bool(torch.any(corrections != 0).item()) or bool(this_step_pos_correction)
     cutoff = float(damping_cutoff_multislice)
@@ -245,13 +343,21 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         loss_weight=float(this_loss_weight), slice_distance=float(dist[0]), wavelength=float(this_wavelength),
         pixel_size_x=float(pixel_size_x_A), pixel_size_y=float(pixel_size_y_A),
         tan_x=float(tt[0, 3]), tan_y=float(tt[0, 2]), damping_cutoff=cutoff,
-        edge_multiplier=float(edge_slice_thickness_multiplier))
+        edge_multiplier=float(edge_slice_thickness_multiplier),
+        n_masks=int(masks_dev.shape[0]) if compressed else 0, raw_n=0 if compressed else int(measured.shape[-1]),
+        data_shift_y=shift_y, data_shift_x=shift_x, data_bin=1 if compressed else int(data_bin), data_pad=pad,
+        data_multiplier=1.0 if compressed else float(data_multiplier))
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None
     pos_grad = torch.zeros_like(corrections)
-    loss, sse = engine.run(obj, probe_mnn, positions, corrections, measured, seq, obj_grad=object_grad,
-                           probe_grad=probe_grad_mnn, pos_grad=pos_grad)
+    if stream:
+        loss, sse = engine.run_streamed(obj, probe_mnn, positions, corrections, measured, seq.cpu().numpy(),
+                                        obj_grad=object_grad, probe_grad=probe_grad_mnn, pos_grad=pos_grad,
+                                        masks=masks_dev)
+    else:
+        loss, sse = engine.run(obj, probe_mnn, positions, corrections, measured, seq, obj_grad=object_grad,
+                               probe_grad=probe_grad_mnn, pos_grad=pos_grad, masks=masks_dev)
     if loss != loss:
         raise ValueError("A very specific bad thing. Loss is Nan.")
     probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe

@@ -8,10 +8,23 @@ import numpy as np
 from oracle import ms_oracle as orc
 
 
+def virtual_detectors(n, J):
+    """J virtual detectors in the DC-centred pattern layout: a full-field one and J-1 annular rings
+    covering radii up to n/2."""
+    y, x = np.mgrid[0:n, 0:n]
+    r = np.hypot(y - n // 2, x - n // 2)
+    edges = np.linspace(0.0, n / 2, J)
+    masks = [np.ones((n, n))]
+    for j in range(J - 1):
+        masks.append(((r >= edges[j]) & (r < edges[j + 1])).astype(float))
+    return np.asarray(masks, np.float32)
+
+
 def make_problem(n=32, S=3, K=1, M=2, ny=80, nx=80, P=12, shift=True, seed=0, algorithm="lsq_sqrt",
-                 dz=2.0, wavelength=0.0196868, px=0.2, step_pos=None, noise=0.05):
+                 dz=2.0, wavelength=0.0196868, px=0.2, step_pos=None, noise=0.05, n_masks=5):
     """Weakly scattering object, aperture-limited probe modes, random scan, measured data
-    simulated by the FP64 oracle from a perturbed object plus multiplicative noise."""
+    simulated by the FP64 oracle from a perturbed object plus multiplicative noise (virtual-detector
+    coefficients for the compressed losses)."""
     rng = np.random.default_rng(seed)
     phase = rng.uniform(0.0, 0.6, (ny, nx, S, K))
     amp = 1.0 - 0.05 * rng.uniform(size=(ny, nx, S, K))
@@ -29,11 +42,15 @@ def make_problem(n=32, S=3, K=1, M=2, ny=80, nx=80, P=12, shift=True, seed=0, al
     true_obj = (obj * np.exp(1j * 0.1 * rng.standard_normal(obj.shape))).astype(np.complex128)
     _, _, _, _, _, I = orc.loss_and_direction(
         true_obj, probe.astype(np.complex128), pos, corr, np.zeros((P, n, n)), seq, dz=dz,
-        wavelength=wavelength, pixel_size_x=px, pixel_size_y=px, algorithm=algorithm, step_obj=False,
+        wavelength=wavelength, pixel_size_x=px, pixel_size_y=px, algorithm="lsq_sqrt", step_obj=False,
         step_probe=False, step_pos=False, cdtype=np.complex128, return_patterns=True)
+    masks = None
+    if algorithm in orc.COMPRESSED:
+        masks = virtual_detectors(n, n_masks)
+        I = np.einsum("pyx,jyx->pj", I, masks.astype(np.float64))
     meas = np.maximum(I * (1 + noise * rng.standard_normal(I.shape)), 0).astype(np.float32)
     return dict(obj=obj, probe=probe, pos=pos, corr=corr, meas=meas, seq=seq, dz=dz, wavelength=wavelength, px=px,
-                algorithm=algorithm, step_pos=bool(shift) if step_pos is None else step_pos)
+                algorithm=algorithm, step_pos=bool(shift) if step_pos is None else step_pos, masks=masks)
 
 
 def oracle_pass(pr, cdtype=np.complex64, step_obj=True, step_probe=True, compute_batch=5, loss_weight=1.0):
@@ -41,7 +58,7 @@ def oracle_pass(pr, cdtype=np.complex64, step_obj=True, step_probe=True, compute
         pr["obj"], pr["probe"], pr["pos"], pr["corr"], pr["meas"], pr["seq"], dz=pr["dz"],
         wavelength=pr["wavelength"], pixel_size_x=pr["px"], pixel_size_y=pr["px"], algorithm=pr["algorithm"],
         step_obj=step_obj, step_probe=step_probe, step_pos=pr["step_pos"], cdtype=cdtype,
-        compute_batch=compute_batch, loss_weight=loss_weight)
+        compute_batch=compute_batch, loss_weight=loss_weight, masks=pr.get("masks"))
 
 
 def reference_kwargs(**over):

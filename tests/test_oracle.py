@@ -149,6 +149,51 @@ def test_loss_closed_form_flat_probe(algorithm):
     assert np.isclose(out[1], (1 - I0) ** 2 + (n * n - 1), rtol=1e-10)
 
 
+@pytest.mark.parametrize("algorithm", ["lsq_compressed", "gauss_compressed", "poisson_compressed"])
+def test_compressed_loss_closed_form_flat_probe(algorithm):
+    """Flat probe modes as above (I = I0 at DC only) with three detectors: the full field (c = I0), the
+    DC pixel (c = I0) and everything else (c = 0); measured coefficients (1, 2, 3)."""
+    n, M = 16, 2
+    c = np.array([1.0, 2.0])
+    probe = np.ones((n, n, M), np.complex128) * c[None, None, :]
+    obj = np.ones((n, n, 1, 1), np.complex128)
+    masks = np.zeros((3, n, n))
+    masks[0] = 1.0
+    masks[1, n // 2, n // 2] = 1.0
+    masks[2] = 1.0 - masks[1]
+    meas = np.array([[1.0, 2.0, 3.0]])
+    out = orc.loss_and_direction(obj, probe, np.zeros((1, 2), int), np.zeros((1, 2)), meas, [0], dz=1.0,
+                                 wavelength=0.02, pixel_size_x=0.2, pixel_size_y=0.2, algorithm=algorithm,
+                                 step_obj=False, step_probe=False, cdtype=np.complex128, masks=masks)
+    I0 = np.mean((c * n * n) ** 2)
+    sse = (I0 - 1) ** 2 + (I0 - 2) ** 2 + 9.0
+    if algorithm == "lsq_compressed":
+        L = sse
+    elif algorithm == "gauss_compressed":
+        L = (np.sqrt(I0) - 1) ** 2 + (np.sqrt(I0) - np.sqrt(2)) ** 2 + 3.0
+    else:  # the zero coefficient is replaced by 1 (objective.py:582)
+        L = (I0 - np.log(I0)) + (I0 - 2 * np.log(I0)) + 1.0
+    assert np.isclose(out[0], L, rtol=1e-10), (out[0], L)
+    assert np.isclose(out[1], sse, rtol=1e-10)
+
+
+def test_preprocess_dataset_known_values():
+    """Hand-computed cases of utils.py:626-651: the roll keeps the wrapped rows and zeroes the band on
+    the other side (as the reference writes it), binning sums blocks, padding adds zero borders."""
+    x = np.arange(16, dtype=float).reshape(1, 4, 4)
+    y = orc.preprocess_dataset(x, (1, 0))
+    assert np.array_equal(y[0], np.array([x[0, 3], x[0, 0], x[0, 1], np.zeros(4)]))
+    y = orc.preprocess_dataset(x, (0, -1))
+    assert np.array_equal(y[0, :, 0], np.zeros(4)) and np.array_equal(y[0, :, 1:], np.roll(x[0], -1, 1)[:, 1:])
+    y = orc.preprocess_dataset(x, data_bin=2)
+    assert np.array_equal(y[0], np.array([[0 + 1 + 4 + 5, 2 + 3 + 6 + 7], [8 + 9 + 12 + 13, 10 + 11 + 14 + 15]]))
+    y = orc.preprocess_dataset(x, data_pad=2, data_multiplier=0.5)
+    assert y.shape == (1, 8, 8) and y.sum() == 0.5 * x.sum() and np.array_equal(y[0, 2:6, 2:6], 0.5 * x[0])
+    # shift then bin then pad compose in that order
+    y = orc.preprocess_dataset(x, (1, 0), data_bin=2, data_pad=1)
+    assert y.shape == (1, 4, 4) and y[0, 1, 1] == 12 + 13 + 0 + 1 and y[0, 2, 1] == 4 + 5 + 0 + 0
+
+
 # -------------------------------------------------------- finite differences
 def _fd_loss(pr, what, idx, delta, eps=1e-4):
     def L(shift):
@@ -160,7 +205,8 @@ def _fd_loss(pr, what, idx, delta, eps=1e-4):
     return (L(eps * delta) - L(-eps * delta)) / (2 * eps)
 
 
-@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "ml", "lsq", "lsq_sqrt_2"])
+@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "ml", "lsq", "lsq_sqrt_2", "lsq_compressed", "gauss_compressed",
+                                       "poisson_compressed"])
 def test_object_gradient_central_fd(algorithm):
     pr = make_problem(n=16, S=2, K=2, M=2, ny=40, nx=40, P=6, seed=11, algorithm=algorithm)
     pr["obj"] = pr["obj"].astype(np.complex128)
