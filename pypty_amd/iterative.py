@@ -324,6 +324,48 @@ def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos):
         np.save(os.path.join(folder, "cg.npy"), (pos.double() + corr.double()).cpu().numpy())
 
 
+def lambda_source(f):
+    """Source text of a lambda (``lambda e: ...``), the form params.pkl stores for callables
+    (pypty/utils.py:2315-2338); other values are returned unchanged."""
+    import ast
+    import inspect
+    import types
+
+    if not isinstance(f, types.LambdaType) or f.__name__ != "<lambda>":
+        return f
+    try:
+        src = inspect.getsource(f)
+    except (OSError, TypeError):
+        return repr(f)
+    start = src.find("lambda")
+    # the shortest prefix of the text from "lambda" on that parses as one expression
+    for end in range(len(src), start, -1):
+        piece = src[start:end].strip()
+        try:
+            node = ast.parse(piece, mode="eval")
+        except SyntaxError:
+            continue
+        if isinstance(node.body, ast.Lambda):
+            return piece
+    return src[start:].strip()
+
+
+def save_params(path, params, strip_dataset=True):
+    """params.pkl as pypty/utils.py:2432-2457 writes it: the parameter dict with callables as their
+    lambda source and, when strip_dataset, without the 'dataset' entry."""
+    import os
+    import pickle
+
+    def conv(v):  # lambdas inside [value, lambda] step lists too (the reference converts top-level ones)
+        return type(v)(conv(x) for x in v) if isinstance(v, (list, tuple)) else lambda_source(v)
+
+    out = {k: conv(v) for k, v in params.items() if not (strip_dataset and k == "dataset")}
+    if os.path.exists(path):
+        os.remove(path)
+    with open(path, "wb") as fh:
+        pickle.dump(out, fh)
+
+
 LOSS_CSV_FIELDS = ["epoch", "time / s", "loss", "sse", "initial step", "matching step", "N linesearch iterations",
                    "dir. derivative", "new dir. derivative", "Constraints contribution", "Free GiB", "Total GiB",
                    "Warnings"]
@@ -419,6 +461,9 @@ def run(pypty_params):
     save_log = get("save_loss_log", 1)
     if save_log == 2:
         raise NotImplementedError("pypty_amd.run: save_loss_log=2 (regulariser columns) is not supported")
+    if folder:
+        os.makedirs(folder, exist_ok=True)
+        save_params(os.path.join(folder, "params.pkl"), pypty_params, get("strip_dataset_from_params", True))
     _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
     reset_bfgs_history()
     t0 = time.time()

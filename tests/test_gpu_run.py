@@ -41,7 +41,7 @@ def test_run_reconstructs_and_writes_reference_files(tmp_path):
     e0 = metrics.nmse(np.zeros_like(truth), truth, mask)
     print(f"  loss {losses[0]:.4e} -> {losses[-1]:.4e}; NMSE {e0:.3f} -> {e:.3f}", flush=True)
     assert e < 0.7 * e0
-    for f in ("co.npy", "cp.npy", "cg.npy", "loss.csv"):
+    for f in ("co.npy", "cp.npy", "cg.npy", "loss.csv", "params.pkl"):
         assert os.path.exists(out + f), f
     co = np.load(out + "co.npy")
     assert co.shape[2:] == (1, 1) and co.dtype == np.complex64
