@@ -182,6 +182,26 @@ struct TableArgs {
 __device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b); }
 __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
 
+// Spectral order of H, H0, maskc, Psi and coef for the fused kernels of a
+// window size: 0 tile position order (LDS tile), 1 register-tile C-order
+// (64, 128 px), 2 global-slab C-order (192, 256 px).
+enum { LAY_POS = 0, LAY_RT = 1, LAY_GT = 2 };
+constexpr int kGtThreads = 512;
+template <int N>
+constexpr int kLayout = (N == 64 || N == 128) ? LAY_RT : ((N == 192 || N == 256) ? LAY_GT : LAY_POS);
+
+template <int N, int LAY>
+__host__ __device__ __forceinline__ void spec_of(int e, int& fy, int& fx) {
+  if constexpr (LAY == LAY_RT) {
+    RT<N>::cspec(e, fy, fx);
+  } else if constexpr (LAY == LAY_GT) {
+    GT<N, kGtThreads>::cspec(e, fy, fx);
+  } else {
+    fy = Tile<N>::pos2freq(e / N);
+    fx = Tile<N>::pos2freq(e % N);
+  }
+}
+
 template <int N>
 __device__ __forceinline__ float band_mask(const TableArgs& a, int fy, int fx) {
   const float qx = fftfreq_f(fx, N, a.px), qy = fftfreq_f(fy, N, a.py);
@@ -194,14 +214,14 @@ __device__ __forceinline__ float band_mask(const TableArgs& a, int fy, int fx) {
   return (q2 <= r_max) ? 1.f : 0.f;
 }
 
-template <int N, bool RTL>
+template <int N, int LAY>
 __global__ void k_tables(TableArgs a) {
   using T = Tile<N>;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N * N) return;
   a.mask[e] = band_mask<N>(a, T::pos2freq(e / N), T::pos2freq(e % N));
-  int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
-  if constexpr (RTL) RT<N>::cspec(e, fy, fx);
+  int fy, fx;
+  spec_of<N, LAY>(e, fy, fx);
   const float qx = fftfreq_f(fx, N, a.px), qy = fftfreq_f(fy, N, a.py);
   const float q2 = fadd(fmul(qx, qx), fmul(qy, qy));
   const float m = band_mask<N>(a, fy, fx);
@@ -515,7 +535,7 @@ struct LossSplit {
   static constexpr int PIX = N * N / CHUNKS;
 };
 
-template <int N, bool RTL>
+template <int N, int LAY>
 __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
   using T = Tile<N>;
   constexpr int CH = LossSplit<N>::CHUNKS, PIX = LossSplit<N>::PIX;
@@ -541,8 +561,8 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
     float I = 0.f;
     for (int j = 0; j < MK; ++j) I += cabs2(psi[size_t(j) * N * N + e]);
     if (MK > 1) I = I / float(MK);
-    int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
-    if constexpr (RTL) RT<N>::cspec(e, fy, fx);
+    int fy, fx;
+    spec_of<N, LAY>(e, fy, fx);
     const int ys = (fy + N / 2) % N, xs = (fx + N / 2) % N;
     const float d = D[ys * N + xs];
     float l;
@@ -626,7 +646,7 @@ __global__ __launch_bounds__(256) void k_preprocess(BatchArgs a) {
 // workgroup per window; coef holds I between the passes.
 constexpr int kMaxMasks = 256;
 
-template <int N, bool RTL>
+template <int N, int LAY>
 __global__ __launch_bounds__(256) void k_loss_c(BatchArgs a) {
   using T = Tile<N>;
   constexpr int CH = LossSplit<N>::CHUNKS;
@@ -643,8 +663,8 @@ __global__ __launch_bounds__(256) void k_loss_c(BatchArgs a) {
     return;
   }
   auto pix = [](int e) {
-    int fy = T::pos2freq(e / N), fx = T::pos2freq(e % N);
-    if constexpr (RTL) RT<N>::cspec(e, fy, fx);
+    int fy, fx;
+    spec_of<N, LAY>(e, fy, fx);
     return ((fy + N / 2) % N) * N + (fx + N / 2) % N;
   };
   const float2* psi = a.Psi + size_t(b) * MK * N * N;
@@ -838,7 +858,7 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
 // i*NT*4*W, so no per-register address arithmetic runs on the vector ALU
 // (a global load's immediate offset cannot hold i*NT*8 bytes).  The resource
 // covers exactly the block, so an access can never leave it.
-template <int N, typename E>
+template <int N, typename E, int NT = RT<N>::NT>
 struct RView {
   static_assert(sizeof(E) == 8 || sizeof(E) == 4, "float2 or float");
   __amdgpu_buffer_rsrc_t r;
@@ -847,14 +867,14 @@ struct RView {
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<E*>(block), 0, N * N * int(sizeof(E)), 0x00020000)),
         voff(tid * unsigned(sizeof(E))) {}
   __device__ __forceinline__ E operator[](int i) const {
-    const int so = i * RT<N>::NT * int(sizeof(E));
+    const int so = i * NT * int(sizeof(E));
     if constexpr (sizeof(E) == 8)
       return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, voff, so, 0));
     else
       return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, voff, so, 0));
   }
   __device__ __forceinline__ void store(int i, E v) const {
-    const int so = i * RT<N>::NT * int(sizeof(E));
+    const int so = i * NT * int(sizeof(E));
     if constexpr (sizeof(E) == 8) {
       using B = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
       __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B, v), r, voff, so, 0);
@@ -1053,26 +1073,317 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
   for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + 8 * n] = v[n];
 }
 
+// ------------------------------------------ global-slab pipeline (192, 256)
+// The per-batch steps for windows of 192 and 256 px over GT (pty_rt.hpp):
+// each phase runs in rounds of GT::LINES lines, the transposes go through
+// the workgroup's slab in `scratch`, LDS holds the wave-local exchanges.
+// Buffer orders as for the register tile (GT R-order / C-order).
+
+// the workgroup's slab: element (y, x) at byte (y*N + x)*8 of a buffer
+// resource over the slab, lane part in voff, uniform part in soff
+template <int N>
+struct Slab {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ explicit Slab(float2* p)
+      : r(__builtin_amdgcn_make_buffer_rsrc(p, 0, N * N * 8, 0x00020000)) {}
+  __device__ __forceinline__ float2 ld(unsigned voff, int soff) const {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+  __device__ __forceinline__ void st(unsigned voff, int soff, float2 v) const {
+    using B = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B, v), r, voff, soff, 0);
+  }
+};
+
+template <int N>
+struct GTLane {
+  int ln, j;               // line within a round, position within the line
+  unsigned vrow, vcol;     // slab lane offsets of the row / column phase
+  __device__ __forceinline__ GTLane() {
+    const int tid = threadIdx.x;
+    ln = 8 * (tid >> 6) + ((tid & 63) >> 3);
+    j = tid & 7;
+    vrow = unsigned(ln * N + j) * 8u;
+    vcol = unsigned(j * N + ln) * 8u;
+  }
+};
+
+// row spectrum of round r: register c*8 + k2 <-> slab (r*LINES + ln, 8*(CPT*k2 + c) + j)
+template <int N>
+__device__ __forceinline__ void gt_load_rowspec(float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
+  using G = GT<N, kGtThreads>;
+#pragma unroll
+  for (int c = 0; c < G::CPT; ++c)
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) v[c * 8 + k2] = sl.ld(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8);
+}
+template <int N>
+__device__ __forceinline__ void gt_store_rowspec(const float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
+  using G = GT<N, kGtThreads>;
+#pragma unroll
+  for (int c = 0; c < G::CPT; ++c)
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) sl.st(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8, v[c * 8 + k2]);
+}
+// column of round r: register n <-> slab (j + 8n, r*LINES + ln)
+template <int N>
+__device__ __forceinline__ void gt_load_col(float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
+  using G = GT<N, kGtThreads>;
+#pragma unroll
+  for (int n = 0; n < G::R1; ++n) v[n] = sl.ld(L.vcol, (8 * n * N + r * G::LINES) * 8);
+}
+template <int N>
+__device__ __forceinline__ void gt_store_col(const float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
+  using G = GT<N, kGtThreads>;
+#pragma unroll
+  for (int n = 0; n < G::R1; ++n) sl.st(L.vcol, (8 * n * N + r * G::LINES) * 8, v[n]);
+}
+
+// column phase of a band limit: forward column FFT, times maskc/N^2, inverse
+template <int N>
+__device__ __forceinline__ void gt_band_columns(float2* v, float2* X, const Slab<N>& sl, const GTLane<N>& L,
+                                                const float* maskc) {
+  using G = GT<N, kGtThreads>;
+  const RView<N, float, G::NT> mask(maskc, threadIdx.x);
+  const float sc = 1.0f / float(N * N);
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    gt_load_col<N>(v, sl, L, r);
+    rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+    for (int i = 0; i < G::R1; ++i) v[i] = cscale(v[i], mask[r * G::R1 + i] * sc);
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+    gt_store_col<N>(v, sl, L, r);
+  }
+}
+
+// k_patch for 192/256 px windows (objective.py:436-441)
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  const int planes = a.S * a.K;
+  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
+  const bool inside = window_inside(y0, x0, N, a.ny, a.nx);
+  const GTLane<N> L;
+  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  float2 v[G::R1];
+  const float2* obj = a.obj + size_t(plane) * a.ny * a.nx;
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    const int row = r * G::LINES + L.ln;
+    const float2* src = obj + (inside ? size_t(y0 + row) * a.nx + x0 + L.j : 0);
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) v[n] = inside ? src[8 * n] : make_float2(0.f, 0.f);
+    rt_line_fwd<N, true>(v, X, L.ln, L.j);
+    gt_store_rowspec<N>(v, sl, L, r);
+  }
+  __syncthreads();
+  gt_band_columns<N>(v, X, sl, L, a.maskc);
+  __syncthreads();
+  const RView<N, float2, G::NT> dst(a.Ob + (size_t(b) * planes + plane) * N * N, threadIdx.x);
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    gt_load_rowspec<N>(v, sl, L, r);
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) dst.store(r * G::R1 + n, v[n]);
+  }
+}
+
+// k_forward for 192/256 px windows (multislice_core.py:9-82, objective.py:501-512)
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int m = mk / a.K, k = mk % a.K;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
+  const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  float2 v[G::R1];
+  for (int s = 0; s < a.S; ++s) {
+    // row phase: inverse row transform of the previous slice, stash, object, forward row transform
+    const RView<N, float2, G::NT> U(st + size_t(s) * N * N, tid), O(ob + size_t(s) * a.K * N * N, tid);
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      if (s == 0) {
+        const float2* p = src + size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = p[8 * n];
+      } else {
+        gt_load_rowspec<N>(v, sl, L, r);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+      }
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) {
+        const int q = r * G::R1 + n;
+        if (a.write_stash) U.store(q, v[n]);
+        v[n] = cmul(v[n], O[q]);
+      }
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+      gt_store_rowspec<N>(v, sl, L, r);
+    }
+    __syncthreads();
+    // column phase: forward column transform, then the propagator and the
+    // inverse column transform, or the far-field band mask after the last slice
+    if (s < a.S - 1) {
+      const RView<N, float2, G::NT> H((s == 0) ? a.H0 : a.H, tid);
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        gt_load_col<N>(v, sl, L, r);
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+        for (int i = 0; i < G::R1; ++i) v[i] = cmul(v[i], H[r * G::R1 + i]);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+        gt_store_col<N>(v, sl, L, r);
+      }
+      __syncthreads();
+    } else {
+      const RView<N, float2, G::NT> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
+      const RView<N, float, G::NT> mask(a.maskc, tid);
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        gt_load_col<N>(v, sl, L, r);
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+        for (int i = 0; i < G::R1; ++i) out.store(r * G::R1 + i, cscale(v[i], mask[r * G::R1 + i]));
+      }
+    }
+  }
+}
+
+// k_adjoint for 192/256 px windows (objective.py:595-597, multislice_core.py:85-200)
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int k = mk % a.K;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  float2 v[G::R1];
+  {
+    // far-field adjoint: inverse column transforms of coef * Psi / N^2
+    const RView<N, float2, G::NT> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
+    const RView<N, float, G::NT> c(a.coef + size_t(b) * N * N, tid);
+    const float sc = 1.0f / float(N * N);
+    for (int r = 0; r < G::ROUNDS; ++r) {
+#pragma unroll
+      for (int i = 0; i < G::R1; ++i) v[i] = cscale(psi[r * G::R1 + i], c[r * G::R1 + i] * sc);
+      rt_line_inv<N, true>(v, X, L.ln, L.j);
+      gt_store_col<N>(v, sl, L, r);
+    }
+  }
+  __syncthreads();
+  // the stash slot of slice s is overwritten by this mode's window gradient
+  // conj(u_s) * g; k_objgrad_gt sums the modes
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  for (int s = a.S - 1; s >= 0; --s) {
+    const RView<N, float2, G::NT> U(st + size_t(s) * N * N, tid), O(ob + size_t(s) * a.K * N * N, tid);
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      gt_load_rowspec<N>(v, sl, L, r);
+      rt_line_inv<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) {
+        const int q = r * G::R1 + n;
+        if (a.step_obj) U.store(q, cmulc(v[n], U[q]));
+        v[n] = cmulc(v[n], O[q]);
+      }
+      if (s > 0) {
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        gt_store_rowspec<N>(v, sl, L, r);
+      } else {
+        float2* out = a.gP + (size_t(b) * MK + mk) * N * N + size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) out[8 * n] = v[n];
+      }
+    }
+    if (s > 0) {
+      __syncthreads();
+      const RView<N, float2, G::NT> H((s == 1) ? a.H0 : a.H, tid);
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        gt_load_col<N>(v, sl, L, r);
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+        for (int i = 0; i < G::R1; ++i) v[i] = cmulc(v[i], H[r * G::R1 + i]);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+        gt_store_col<N>(v, sl, L, r);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// k_objgrad for 192/256 px windows: mode sum from the R-order stash, band
+// limit, natural-order window gradient for k_gather_grad
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  const int planes = a.S * a.K;
+  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
+  const int sl_ = plane / a.K, k = plane % a.K;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  const size_t mstride = size_t(a.K) * a.S * N * N;
+  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.S + sl_) * N * N;
+  float2 v[G::R1];
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    {
+      const RView<N, float2, G::NT> G0(src, tid);
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) v[n] = G0[r * G::R1 + n];
+    }
+    for (int m = 1; m < a.M; ++m) {
+      const RView<N, float2, G::NT> Gm(src + m * mstride, tid);
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) v[n] = cadd(v[n], Gm[r * G::R1 + n]);
+    }
+    rt_line_fwd<N, true>(v, X, L.ln, L.j);
+    gt_store_rowspec<N>(v, sl, L, r);
+  }
+  __syncthreads();
+  gt_band_columns<N>(v, X, sl, L, a.maskc);
+  __syncthreads();
+  float2* Gout = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    gt_load_rowspec<N>(v, sl, L, r);
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+    float2* out = Gout + size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) out[8 * n] = v[n];
+  }
+}
+
 // Deterministic scatter-add of a batch's window gradients (multislice_core.py:192,
-// 206-230): one workgroup per GT x GT object tile of one plane sums, in window
+// 206-230): one workgroup per GTILE x GTILE object tile of one plane sums, in window
 // order, every window of the batch that overlaps the tile, then adds the sum
 // to the object gradient.  Tiles are disjoint, so no atomics are needed and
 // the result does not depend on scheduling.
-constexpr int GT = 32;
+constexpr int GTILE = 32;
 
 template <int N>
 __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
   __shared__ int wy0[256], wx0[256], wb[256];
-  const int tiles_x = (a.nx + GT - 1) / GT, tiles_y = (a.ny + GT - 1) / GT;
+  const int tiles_x = (a.nx + GTILE - 1) / GTILE, tiles_y = (a.ny + GTILE - 1) / GTILE;
   const int planes = a.S * a.K;
   const int plane = blockIdx.x / (tiles_x * tiles_y);
   const int tile = blockIdx.x % (tiles_x * tiles_y);
-  const int ty0 = (tile / tiles_x) * GT, tx0 = (tile % tiles_x) * GT;
-  const int cx = threadIdx.x % GT, ry = threadIdx.x / GT;  // 8 row groups of 4 rows
+  const int ty0 = (tile / tiles_x) * GTILE, tx0 = (tile % tiles_x) * GTILE;
+  const int cx = threadIdx.x % GTILE, ry = threadIdx.x / GTILE;  // 8 row groups of 4 rows
   const int x = tx0 + cx;
-  float2 acc[GT / 8];
+  float2 acc[GTILE / 8];
 #pragma unroll
-  for (int r = 0; r < GT / 8; ++r) acc[r] = make_float2(0.f, 0.f);
+  for (int r = 0; r < GTILE / 8; ++r) acc[r] = make_float2(0.f, 0.f);
   bool any = false;
   // windows in chunks of 256: each thread tests one window, overlapping ones are
   // compacted into LDS in window order, then all threads sum them
@@ -1085,7 +1396,7 @@ __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
       if (idx >= 0) {
         y0 = a.pos[2 * idx];
         x0 = a.pos[2 * idx + 1];
-        hit = window_inside(y0, x0, N, a.ny, a.nx) && !(y0 >= ty0 + GT || y0 + N <= ty0 || x0 >= tx0 + GT ||
+        hit = window_inside(y0, x0, N, a.ny, a.nx) && !(y0 >= ty0 + GTILE || y0 + N <= ty0 || x0 >= tx0 + GTILE ||
                                                         x0 + N <= tx0);
       }
     }
@@ -1114,7 +1425,7 @@ __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
       const float2* G = a.Ob + (size_t(wb[i]) * planes + plane) * N * N;
       const int wx = x - x0s;
 #pragma unroll
-      for (int r = 0; r < GT / 8; ++r) {
+      for (int r = 0; r < GTILE / 8; ++r) {
         const int wy = ty0 + ry + 8 * r - y0s;
         if (wx >= 0 && wx < N && wy >= 0 && wy < N) acc[r] = cadd(acc[r], G[wy * N + wx]);
       }
@@ -1124,7 +1435,7 @@ __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
   if (!any || x >= a.nx) return;
   float2* dst = a.obj_grad + size_t(plane) * a.ny * a.nx;
 #pragma unroll
-  for (int r = 0; r < GT / 8; ++r) {
+  for (int r = 0; r < GTILE / 8; ++r) {
     const int y = ty0 + ry + 8 * r;
     if (y < a.ny) {
       float2* d = &dst[size_t(y) * a.nx + x];
@@ -1493,16 +1804,13 @@ int validate(const pty_problem* p) {
   return PTY_OK;
 }
 
-// the fused kernels of windows of 64 and 128 px use the register tile
-template <int N>
-constexpr bool kRegTile = (N == 64 || N == 128);
 
 template <int N>
 int launch_tables(const pty_problem* p, float2* H, float2* H0, float* mask, float* maskc, hipStream_t s) {
   TimeScope ts(KT_TABLES, s);
   TableArgs a{H, H0, mask, maskc, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x,
               p->tan_y, p->damping_cutoff, p->edge_multiplier};
-  k_tables<N, kRegTile<N>><<<(N * N + 255) / 256, 256, 0, s>>>(a);
+  k_tables<N, kLayout<N>><<<(N * N + 255) / 256, 256, 0, s>>>(a);
   return check_launch();
 }
 
@@ -1518,13 +1826,15 @@ int zero_fill(void* ptr, size_t bytes, hipStream_t s) {
 template <int N>
 int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   using T = Tile<N>;
-  constexpr bool RTL = kRegTile<N>;
+  constexpr int LAY = kLayout<N>;
   const int planes = a.S * a.K;
   const int MK = a.M * a.K;
   {
     TimeScope ts(KT_PATCH, s);
-    if constexpr (RTL)
+    if constexpr (LAY == LAY_RT)
       k_patch_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
+    else if constexpr (LAY == LAY_GT)
+      k_patch_gt<N><<<a.B * planes, kGtThreads, 0, s>>>(a);
     else
       k_patch<N><<<a.B * planes, T::NT, 0, s>>>(a);
   }
@@ -1534,8 +1844,10 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   }
   {
     TimeScope ts(KT_FORWARD, s);
-    if constexpr (RTL)
+    if constexpr (LAY == LAY_RT)
       k_forward_rt<N><<<xcd_grid(a.B, MK), RT<N>::NT, 0, s>>>(a);
+    else if constexpr (LAY == LAY_GT)
+      k_forward_gt<N><<<xcd_grid(a.B, MK), kGtThreads, 0, s>>>(a);
     else
       k_forward<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
   }
@@ -1546,27 +1858,31 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   {
     TimeScope ts(KT_LOSS, s);
     if (compressed(a.algorithm))
-      k_loss_c<N, RTL><<<a.B, 256, 0, s>>>(a);
+      k_loss_c<N, LAY><<<a.B, 256, 0, s>>>(a);
     else
-      k_loss<N, RTL><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
+      k_loss<N, LAY><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
   }
   if (p->step_obj || p->step_probe || p->step_pos) {
     TimeScope ts(KT_ADJOINT, s);
-    if constexpr (RTL)
+    if constexpr (LAY == LAY_RT)
       k_adjoint_rt<N><<<xcd_grid(a.B, MK), RT<N>::NT, 0, s>>>(a);
+    else if constexpr (LAY == LAY_GT)
+      k_adjoint_gt<N><<<xcd_grid(a.B, MK), kGtThreads, 0, s>>>(a);
     else
       k_adjoint<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
   }
   if (p->step_obj) {
     {
       TimeScope ts(KT_OBJGRAD, s);
-      if constexpr (RTL)
+      if constexpr (LAY == LAY_RT)
         k_objgrad_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
+      else if constexpr (LAY == LAY_GT)
+        k_objgrad_gt<N><<<a.B * planes, kGtThreads, 0, s>>>(a);
       else
         k_objgrad<N><<<a.B * planes, T::NT, 0, s>>>(a);
     }
     TimeScope ts(KT_GATHER, s);
-    const int tiles = ((a.ny + GT - 1) / GT) * ((a.nx + GT - 1) / GT);
+    const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
     k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
   }
   if (p->step_probe || (p->step_pos && a.shift)) {

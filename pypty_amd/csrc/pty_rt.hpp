@@ -35,7 +35,9 @@ namespace pty {
 
 template <int N>
 struct RT {
-  static_assert(N == 64 || N == 128, "register tile: N = 64 or 128");
+  // 64 and 128 px: the whole window in registers; 192 and 256 px: the line
+  // split alone (GT below)
+  static_assert(N == 64 || N == 128 || N == 192 || N == 256, "register tile line split: N = 64, 128, 192, 256");
   static constexpr int R1 = N / 8;   // samples per thread and line
   static constexpr int CPT = R1 / 8; // DFT-8 groups per thread in stage 2
   static constexpr int NT = N * 8;   // threads per window
@@ -270,5 +272,32 @@ __device__ __forceinline__ void rt_fft2_inv(float2* v, float2* T, const RTIds<N>
   rt_cols_to_rows<N>(v, T, id);
   rt_line_inv<N, true>(v, T, id.row, id.rj);
 }
+
+// Windows of 192 and 256 px do not fit in one CU's registers and LDS.  The
+// same line split runs in rounds of LINES lines per workgroup, with the
+// workgroup's N x N slab in global memory (L2 / MALL resident) as the
+// transpose medium between the row phase and the column phase, and LDS only
+// for the wave-local exchanges.  Both phases use the row-phase lane mapping
+// (lane = j + 8 * line_in_wave); the slab is row-major, so a wavefront's
+// accesses are eight 64-byte segments in either phase.  The slab holds
+// natural rows after the column phase and row spectra (column positions,
+// frequency col_freq) after the row phase.
+//   R-order slot (r*R1 + i)*NT + tid: natural pixel (r*LINES + ln, j + 8i)
+//   C-order slot (r*R1 + i)*NT + tid: frequency (CPT*j + (i>>3) + R1*(i&7),
+//                                                 col_freq(r*LINES + ln))
+// with ln = 8*(tid/64) + (tid%64)/8 and j = tid%8.
+template <int N, int NT_>
+struct GT {
+  static constexpr int NT = NT_;
+  static constexpr int R1 = RT<N>::R1, CPT = RT<N>::CPT, LS = RT<N>::LS;
+  static constexpr int LINES = NT / 8;      // lines per round
+  static constexpr int ROUNDS = N / LINES;  // rounds per phase
+  static_assert(N % LINES == 0, "rounds");
+  __host__ __device__ __forceinline__ static void cspec(int e, int& fy, int& fx) {
+    const int q = e / NT, t = e % NT, r = q / R1, i = q % R1;
+    fx = RT<N>::col_freq(r * LINES + 8 * (t >> 6) + ((t & 63) >> 3));
+    fy = CPT * (t & 7) + (i >> 3) + R1 * (i & 7);
+  }
+};
 
 }  // namespace pty

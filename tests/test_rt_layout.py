@@ -88,6 +88,38 @@ def test_register_tile_computes_the_2d_dft(N):
     assert np.abs(Y / N ** 2 - X).max() <= 1e-12 * np.abs(X).max()
 
 
+@pytest.mark.parametrize("N", [192, 256])
+def test_line_split_of_large_windows(N):
+    """The 24- and 32-sample line splits (3 and 4 DFT-8 groups per thread) of the 192 / 256 px
+    global-slab tiles: one exchange per line through an LDS row of stride N + 8."""
+    R1, CPT, LS, xslot, col_freq = _maps(N)
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    T = np.full(LS, np.nan + 0j)
+    out = _line_fwd(N, [x[j::8] for j in range(8)], T, lambda p: p)
+    X = np.zeros(N, complex)
+    for j in range(8):
+        for c in range(CPT):
+            for k2 in range(8):
+                X[CPT * j + c + R1 * k2] = out[j][c * 8 + k2]
+    assert np.abs(X - np.fft.fft(x)).max() <= 1e-12 * np.abs(X).max()
+    back = _line_inv(N, out, T, lambda p: p)
+    y = np.zeros(N, complex)
+    for j in range(8):
+        y[j::8] = back[j]
+    assert np.abs(y / N - x).max() <= 1e-12 * np.abs(x).max()
+    assert sorted(col_freq(c) for c in range(N)) == list(range(N))
+    # exchanges of 8 lines per wavefront in rows of stride LS: 32 distinct bank pairs per half-wave
+    lanes = np.arange(64)
+    for k1 in range(R1):
+        a = np.array([(l >> 3) * LS + xslot(l & 7, k1) for l in lanes])
+        assert a.max() < 8 * LS and all(len(set((h % 32).tolist())) == 32 for h in (a[:32], a[32:]))
+    for c in range(CPT):
+        for jj in range(8):
+            a = np.array([(l >> 3) * LS + xslot(jj, CPT * (l & 7) + c) for l in lanes])
+            assert all(len(set((h % 32).tolist())) == 32 for h in (a[:32], a[32:]))
+
+
 @pytest.mark.parametrize("N", [64, 128])
 def test_register_tile_lds_bounds_and_banks(N):
     R1, CPT, LS, xslot, _ = _maps(N)
