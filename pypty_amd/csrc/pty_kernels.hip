@@ -186,7 +186,10 @@ __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b)
 // window size: 0 tile position order (LDS tile), 1 register-tile C-order
 // (64, 128 px), 2 global-slab C-order (192, 256 px).
 enum { LAY_POS = 0, LAY_RT = 1, LAY_GT = 2 };
-constexpr int kGtThreads = 512;
+#ifndef PTY_GT_THREADS
+#define PTY_GT_THREADS 512
+#endif
+constexpr int kGtThreads = PTY_GT_THREADS;  // threads of a 192/256 px slab workgroup
 template <int N>
 constexpr int kLayout = (N == 64 || N == 128) ? LAY_RT : ((N == 192 || N == 256) ? LAY_GT : LAY_POS);
 
@@ -1206,10 +1209,19 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
   const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
   float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
-  float2 v[G::R1];
+  // o holds the object registers of the current round; those of the next
+  // round (or the next slice's first round) are read while this round's
+  // forward row transform runs, and the propagator is read before the column
+  // transform that precedes its use
+  float2 v[G::R1], o[G::R1];
+  {
+    const RView<N, float2, G::NT> O(ob, tid);
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) o[n] = O[n];
+  }
   for (int s = 0; s < a.S; ++s) {
     // row phase: inverse row transform of the previous slice, stash, object, forward row transform
-    const RView<N, float2, G::NT> U(st + size_t(s) * N * N, tid), O(ob + size_t(s) * a.K * N * N, tid);
+    const RView<N, float2, G::NT> U(st + size_t(s) * N * N, tid);
     for (int r = 0; r < G::ROUNDS; ++r) {
       if (s == 0) {
         const float2* p = src + size_t(r * G::LINES + L.ln) * N + L.j;
@@ -1221,9 +1233,14 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
       }
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) {
-        const int q = r * G::R1 + n;
-        if (a.write_stash) U.store(q, v[n]);
-        v[n] = cmul(v[n], O[q]);
+        if (a.write_stash) U.store(r * G::R1 + n, v[n]);
+        v[n] = cmul(v[n], o[n]);
+      }
+      const int rn = (r + 1 < G::ROUNDS) ? r + 1 : 0, sn = (r + 1 < G::ROUNDS) ? s : s + 1;
+      if (sn < a.S) {
+        const RView<N, float2, G::NT> O(ob + size_t(sn) * a.K * N * N, tid);
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) o[n] = O[rn * G::R1 + n];
       }
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
       gt_store_rowspec<N>(v, sl, L, r);
@@ -1234,10 +1251,13 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
     if (s < a.S - 1) {
       const RView<N, float2, G::NT> H((s == 0) ? a.H0 : a.H, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        float2 h[G::R1];
+#pragma unroll
+        for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
         gt_load_col<N>(v, sl, L, r);
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
-        for (int i = 0; i < G::R1; ++i) v[i] = cmul(v[i], H[r * G::R1 + i]);
+        for (int i = 0; i < G::R1; ++i) v[i] = cmul(v[i], h[i]);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
         gt_store_col<N>(v, sl, L, r);
       }
@@ -1284,8 +1304,15 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
   __syncthreads();
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g; k_objgrad_gt sums the modes
+  // pu holds the stash registers of the current round, read one round ahead
   float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  float2 pu[G::R1];
+  if (a.step_obj) {
+    const RView<N, float2, G::NT> U(st + size_t(a.S - 1) * N * N, tid);
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) pu[n] = U[n];
+  }
   for (int s = a.S - 1; s >= 0; --s) {
     const RView<N, float2, G::NT> U(st + size_t(s) * N * N, tid), O(ob + size_t(s) * a.K * N * N, tid);
     for (int r = 0; r < G::ROUNDS; ++r) {
@@ -1294,8 +1321,14 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) {
         const int q = r * G::R1 + n;
-        if (a.step_obj) U.store(q, cmulc(v[n], U[q]));
+        if (a.step_obj) U.store(q, cmulc(v[n], pu[n]));
         v[n] = cmulc(v[n], O[q]);
+      }
+      const int rn = (r + 1 < G::ROUNDS) ? r + 1 : 0, sn = (r + 1 < G::ROUNDS) ? s : s - 1;
+      if (a.step_obj && sn >= 0) {
+        const RView<N, float2, G::NT> Un(st + size_t(sn) * N * N, tid);
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) pu[n] = Un[rn * G::R1 + n];
       }
       if (s > 0) {
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
@@ -1310,10 +1343,13 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
       __syncthreads();
       const RView<N, float2, G::NT> H((s == 1) ? a.H0 : a.H, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        float2 h[G::R1];
+#pragma unroll
+        for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
         gt_load_col<N>(v, sl, L, r);
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
-        for (int i = 0; i < G::R1; ++i) v[i] = cmulc(v[i], H[r * G::R1 + i]);
+        for (int i = 0; i < G::R1; ++i) v[i] = cmulc(v[i], h[i]);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
         gt_store_col<N>(v, sl, L, r);
       }
