@@ -1453,15 +1453,33 @@ __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
     }
     __syncthreads();
     any = any || n > 0;
-    // unrolled so that the loads of several windows are in flight together;
-    // the sum is still taken in window order
-#pragma unroll 4
-    for (int i = 0; i < n; ++i) {
+    // eight windows at a time: all their loads are issued before the sums, which
+    // are still taken in window order (a pixel a window does not cover adds 0)
+    constexpr int H = 8, RR = GTILE / 8;
+    int i = 0;
+    for (; i + H <= n; i += H) {
+      float2 t[H][RR];
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        const int y0s = wy0[i + h], wx = x - wx0[i + h];
+        const float2* G = a.Ob + (size_t(wb[i + h]) * planes + plane) * N * N;
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+          const int wy = ty0 + ry + 8 * r - y0s;
+          t[h][r] = (wx >= 0 && wx < N && wy >= 0 && wy < N) ? G[wy * N + wx] : make_float2(0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < H; ++h)
+#pragma unroll
+        for (int r = 0; r < RR; ++r) acc[r] = cadd(acc[r], t[h][r]);
+    }
+    for (; i < n; ++i) {
       const int y0s = wy0[i], x0s = wx0[i];
       const float2* G = a.Ob + (size_t(wb[i]) * planes + plane) * N * N;
       const int wx = x - x0s;
 #pragma unroll
-      for (int r = 0; r < GTILE / 8; ++r) {
+      for (int r = 0; r < RR; ++r) {
         const int wy = ty0 + ry + 8 * r - y0s;
         if (wx >= 0 && wx < N && wy >= 0 && wy < N) acc[r] = cadd(acc[r], G[wy * N + wx]);
       }
