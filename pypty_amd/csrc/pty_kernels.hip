@@ -853,7 +853,7 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
 // --------------------------------------------- register-tile pipeline
 // The same per-batch steps for N = 64 and 128 over the register tile of
 // pty_rt.hpp.  Buffer orders: Ob and stash R-order; Psi, coef, H, H0, maskc
-// C-order; Pp, gP and the band-limited window gradient natural order.
+// C-order; gP R-order (64, 128 px); Pp and the band-limited window gradient natural order.
 //
 // Register i of an R/C-order array is element i*NT + tid of one window's
 // N*N block.  It is accessed through a buffer resource on the block: the
@@ -1036,9 +1036,10 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
       rt_line_inv<N, true>(v, T, id.row, id.rj);
     }
   }
-  float2* out = a.gP + (size_t(b) * MK + mk) * N * N;
+  // probe-side gradient in R-order (read only by k_probegrad_rt)
+  const RView<N, float2> out(a.gP + (size_t(b) * MK + mk) * N * N, tid);
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) out[id.row * N + id.rj + 8 * n] = v[n];
+  for (int n = 0; n < R::R1; ++n) out.store(n, v[n]);
 }
 
 // k_objgrad over the register tile: mode sum from the R-order stash, band
@@ -1074,6 +1075,165 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
   float2* G = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + 8 * n] = v[n];
+}
+
+// ------------------------------------- register-tile probe side (64, 128)
+// Probe spectra (fprobe) and the spectral probe-gradient partials are in
+// C-order here, the probe-side gradient from the adjoint in R-order; the
+// shift kernel E = ex[fx] * ey[fy] * mask comes from two axis tables in LDS
+// (objective.py:309-313, 473).
+
+// exy[f] = exp(i * -2*pi*fftfreq(f) * dx) for f < N, exy[N + f] the same for dy
+// (indexed by frequency; the caller synchronises)
+template <int N>
+__device__ __forceinline__ void shift_tables_freq(float2* exy, float dy, float dx) {
+  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) {
+    const int f = i % N;
+    const float k = fmul(-6.2831855f, fftfreq_f(f, N, 1.f));
+    float sn, cs;
+    sincosf(fmul(k, i < N ? dx : dy), &sn, &cs);
+    exy[i] = make_float2(cs, sn);
+  }
+}
+
+// probe spectra, C-order, unnormalised
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_fprobe_rt(const float2* __restrict__ probe, float2* __restrict__ fprobe) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  const RTIds<N> id;
+  const float2* src = probe + size_t(blockIdx.x) * N * N;
+  float2 v[R::R1];
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + 8 * n];
+  rt_fft2_fwd<N>(v, T, id);
+  const RView<N, float2> out(fprobe + size_t(blockIdx.x) * N * N, threadIdx.x);
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) out.store(i, v[i]);
+}
+
+// shifted probes (objective.py:471-476): ifft2(fprobe * E), natural order
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  __shared__ float2 exy[2 * N];
+  const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+  const unsigned tid = threadIdx.x;
+  const RTIds<N> id;
+  shift_tables_freq<N>(exy, dy, dx);
+  __syncthreads();
+  const RView<N, float2> fp(a.fprobe + size_t(m) * N * N, tid);
+  const RView<N, float> mask(a.maskc, tid);
+  const float s = 1.0f / float(N * N);
+  float2 v[R::R1];
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) {
+    int fy, fx;
+    R::cspec(i * R::NT + tid, fy, fx);
+    const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mask[i]);
+    v[i] = cscale(cmul(fp[i], E), s);
+  }
+  rt_fft2_inv<N>(v, T, id);
+  float2* dst = a.Pp + (size_t(b) * a.M + m) * N * N;
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + 8 * n] = v[n];
+}
+
+// probe-side adjoint (objective.py:641-658, 697-703): per window and mode
+// the gradient summed over object modes is transformed, shifted back and
+// accumulated (C-order) into the group's partial, with the position gradient
+// -2/n^2 sum Re(K conj(FP) Q) in FP64; without a shift the natural-order
+// gradient itself is accumulated (R-order)
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  __shared__ float2 exy[2 * N];
+  __shared__ double red[16];
+  const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const unsigned tid = threadIdx.x;
+  const RTIds<N> id;
+  const RView<N, float2> acc(a.probe_part + (size_t(g) * a.M + m) * N * N, tid);
+  const RView<N, float2> fp(a.fprobe + size_t(m) * N * N, tid);
+  const RView<N, float> mask(a.maskc, tid);
+  for (int b = g; b < a.B; b += a.G) {
+    const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    float2 v[R::R1];
+    {
+      const RView<N, float2> g0(a.gP + (size_t(b) * a.M + m) * a.K * N * N, tid);
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) v[n] = g0[n];
+    }
+    for (int k = 1; k < a.K; ++k) {
+      const RView<N, float2> gk(a.gP + ((size_t(b) * a.M + m) * a.K + k) * N * N, tid);
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) v[n] = cadd(v[n], gk[n]);
+    }
+    if (!a.shift) {
+      // natural order: register n of the row phase is R-order slot n*NT + tid
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) acc.store(n, cadd(acc[n], v[n]));
+      continue;
+    }
+    const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+    __syncthreads();  // the previous window's table and transposes are done
+    shift_tables_freq<N>(exy, dy, dx);
+    rt_fft2_fwd<N>(v, T, id);  // its first barrier also publishes exy
+    double sy = 0.0, sx = 0.0;
+#pragma unroll
+    for (int i = 0; i < R::R1; ++i) {
+      int fy, fx;
+      R::cspec(i * R::NT + tid, fy, fx);
+      const float mk = mask[i];
+      const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mk);
+      const float2 Q = cmulc(v[i], E);
+      acc.store(i, cadd(acc[i], Q));
+      if (a.step_pos) {
+        const float2 Z = cmulc(Q, fp[i]);  // conj(FP) * Q
+        const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
+        const float ky = fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk);
+        sx += -double(kx) * double(Z.y);
+        sy += -double(ky) * double(Z.y);
+      }
+    }
+    if (a.step_pos) {
+      const double SY = block_sum(sy, red);
+      const double SX = block_sum(sx, red);
+      if (tid == 0) {
+        const double sh = -2.0 / (double(N) * double(N));
+        a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
+        a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
+      }
+    }
+  }
+}
+
+// probe_grad = ifft2(mask * fft2(sum)) (objective.py:927-933) from the reduced
+// partials: C-order spectrum (spectral) or R-order natural gradient
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_probe_final_rt(const float2* __restrict__ sum, int spectral,
+                                                              const float* __restrict__ maskc,
+                                                              float2* __restrict__ probe_grad) {
+  using R = RT<N>;
+  __shared__ float2 T[R::ELEMS];
+  const RTIds<N> id;
+  const unsigned tid = threadIdx.x;
+  const RView<N, float2> src(sum + size_t(blockIdx.x) * N * N, tid);
+  const RView<N, float> mask(maskc, tid);
+  float2 v[R::R1];
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) v[i] = src[i];
+  if (!spectral) rt_fft2_fwd<N>(v, T, id);
+  const float s = 1.0f / float(N * N);
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], mask[i] * s);
+  rt_fft2_inv<N>(v, T, id);
+  float2* dst = probe_grad + size_t(blockIdx.x) * N * N;
+#pragma unroll
+  for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + 8 * n] = v[n];
 }
 
 // ------------------------------------------ global-slab pipeline (192, 256)
@@ -1894,7 +2054,10 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   }
   if (a.shift) {
     TimeScope ts(KT_PROBE_SHIFT, s);
-    k_probe_shift<N><<<a.B * a.M, T::NT, 0, s>>>(a);
+    if constexpr (LAY == LAY_RT)
+      k_probe_shift_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
+    else
+      k_probe_shift<N><<<a.B * a.M, T::NT, 0, s>>>(a);
   }
   {
     TimeScope ts(KT_FORWARD, s);
@@ -1941,7 +2104,10 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   }
   if (p->step_probe || (p->step_pos && a.shift)) {
     TimeScope ts(KT_PROBEGRAD, s);
-    k_probegrad<N><<<a.G * a.M, T::NT, 0, s>>>(a);
+    if constexpr (LAY == LAY_RT)
+      k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+    else
+      k_probegrad<N><<<a.G * a.M, T::NT, 0, s>>>(a);
   }
   if (p->step_pos && a.shift) {
     TimeScope ts(KT_POS_FINAL, s);
@@ -1968,7 +2134,10 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   float2* fprobe = reinterpret_cast<float2*>(ws + L.fprobe);
   {
     TimeScope ts(KT_FPROBE, s);
-    k_fprobe<N><<<M, T::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
+    if constexpr (kLayout<N> == LAY_RT)
+      k_fprobe_rt<N><<<M, RT<N>::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe);
+    else
+      k_fprobe<N><<<M, T::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
                                      reinterpret_cast<float2*>(ws + L.scratch));
   }
   const float2* objp = static_cast<const float2*>(bf->obj);
@@ -2031,7 +2200,11 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     const size_t per_group = size_t(M) * n2;
     float2* psum = reinterpret_cast<float2*>(ws + L.probe_sum);
     k_probe_reduce<<<int((per_group + 255) / 256), 256, 0, s>>>(a.probe_part, L.G, per_group, psum);
-    k_probe_final<N><<<M, T::NT, 0, s>>>(psum, 1, M, p->shift_active, mask,
+    if constexpr (kLayout<N> == LAY_RT)
+      k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, p->shift_active, maskc,
+                                                   static_cast<float2*>(bf->probe_grad));
+    else
+      k_probe_final<N><<<M, T::NT, 0, s>>>(psum, 1, M, p->shift_active, mask,
                                           static_cast<float2*>(bf->probe_grad), a.scratch);
   }
   if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE) {
