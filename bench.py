@@ -173,6 +173,13 @@ class Job:
     """One dataset resident on the device with its engine and gradient buffers."""
 
     def __init__(self, ds, batch, dev):
+        self.ds = ds
+        self._setup(ds, batch, dev)
+
+    def set_batch(self, batch):
+        self._setup(self.ds, batch, self.dev)
+
+    def _setup(self, ds, batch, dev):
         import numpy as np
         import torch
 
@@ -200,9 +207,9 @@ class Job:
         self.pg = torch.empty_like(self.probe)
         self.posg = torch.zeros_like(self.corr)
 
-    def step(self):
+    def step(self, sync=True):
         return self.engine.run(self.obj, self.probe, self.pos, self.corr, self.meas, self.seq, obj_grad=self.og,
-                               probe_grad=self.pg, pos_grad=self.posg)
+                               probe_grad=self.pg, pos_grad=self.posg, sync=sync)
 
 
 def main(argv=None):
@@ -220,6 +227,10 @@ def main(argv=None):
                     help="also run a plain device copy of this size (for rocprofv3 counter scaling)")
     ap.add_argument("--recon-epochs", type=int, default=20,
                     help="epochs of the reconstruction wall-clock measurement (0: skip)")
+    ap.add_argument("--autotune", type=int, default=None,
+                    help="choose each dataset's windows per launch by timing (default: on for ragged)")
+    ap.add_argument("--graph", type=int, default=None,
+                    help="capture one step in a HIP graph and replay it (default: on for ragged)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (one-GPU boxes); run with --gpus 1", file=sys.stderr)
@@ -240,21 +251,58 @@ def main(argv=None):
     t_gen = time.perf_counter() - t_gen
     jobs = [Job(ds, args.batch, dev) for ds in datasets]
     P_total = sum(j.P for j in jobs)
+    ragged = args.config == "ragged"
+    use_tune = ragged if args.autotune is None else bool(args.autotune)
+    use_graph = ragged if args.graph is None else bool(args.graph)
+    tuned = None
+    if use_tune:
+        from pypty_amd import autotune
+
+        tuned = []
+        for j in jobs:
+            n, S, M, K = j.shape
+            key = autotune.shape_key(dev, n, S, K, M, j.P)
+
+            def make_pass(b, j=j):
+                j.set_batch(b)
+                return lambda: j.step(sync=False)
+
+            best = autotune.choose_batch(key, (64, 128, 256, 512), make_pass)
+            j.set_batch(best)
+            tuned.append(best)
 
     def step():
         out = None
         for j in jobs:
-            out = j.step()
+            out = j.step(sync=False)
         return out
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    graph = None
+    if use_graph:
+        # one whole step (every dataset's pass) captured once; the warm-up above allocated the workspaces
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            step()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            out_t = step()
+        graph.replay()
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        loss, sse = step()
+        if graph is not None:
+            graph.replay()
+        else:
+            out_t = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    loss, sse = out_t.tolist()
     value = P_total * args.steps / elapsed
 
     kernels = {}
@@ -312,7 +360,8 @@ def main(argv=None):
 
     flops_step = sum(pass_flops(*j.shape, True) * j.P for j in jobs)
     bytes_step = sum(pass_bytes(j.shape[0], j.shape[1], j.shape[3]) * j.P for j in jobs)
-    cfg_info = {"workload": WORKLOADS[args.config], "positions": P_total, "compute_batch": args.batch,
+    cfg_info = {"workload": WORKLOADS[args.config], "positions": P_total,
+                "compute_batch": tuned if tuned else args.batch, "hip_graph": bool(use_graph),
                 "datasets": [{"n": j.shape[0], "slices": j.shape[1], "probe_modes": j.shape[2],
                               "obj_modes": j.shape[3], "positions": j.P} for j in jobs]}
     out = {
