@@ -165,15 +165,16 @@ WORKLOADS = {
     "ragged": "8 seeded datasets, windows {96,128,192} px, 4-16 slices, 1-5 probe modes, scans 32x32-96x96, "
               "dose 1e3-1e5, one pass over every dataset per step",
     "thick": "thick object 1536x1536x40 dz1A, probe 256px x10 modes, 64x64 scan (4096 positions), "
-             "object+probe+position gradients",
+             "object+probe+position gradients, slice waves checkpointed every 8 slices and recomputed",
 }
 
 
 class Job:
     """One dataset resident on the device with its engine and gradient buffers."""
 
-    def __init__(self, ds, batch, dev):
+    def __init__(self, ds, batch, dev, checkpoint=0):
         self.ds = ds
+        self.checkpoint = checkpoint
         self._setup(ds, batch, dev)
 
     def set_batch(self, batch):
@@ -200,7 +201,7 @@ class Job:
                               n_positions=self.P, compute_batch=batch, algorithm="lsq_sqrt", planar=False,
                               step_obj=True, step_probe=True, step_pos=True, shift_active=True,
                               slice_distance=ds.dz_A, wavelength=ds.wavelength_A, pixel_size_x=ds.pixel_A,
-                              pixel_size_y=ds.pixel_A)
+                              pixel_size_y=ds.pixel_A, checkpoint_slices=self.checkpoint)
         self.dev = dev
         self.engine = PtychoPass(self.cfg, dev)
         self.og = torch.empty_like(self.obj)
@@ -231,6 +232,8 @@ def main(argv=None):
                     help="choose each dataset's windows per launch by timing (default: on for ragged)")
     ap.add_argument("--graph", type=int, default=None,
                     help="capture one step in a HIP graph and replay it (default: on for ragged)")
+    ap.add_argument("--checkpoint", type=int, default=None,
+                    help="keep every k-th slice's wave and recompute the rest (default: 8 for thick, else 0)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (one-GPU boxes); run with --gpus 1", file=sys.stderr)
@@ -249,7 +252,8 @@ def main(argv=None):
     else:
         datasets = [synth.config4(dev, n_scan=args.scan)]
     t_gen = time.perf_counter() - t_gen
-    jobs = [Job(ds, args.batch, dev) for ds in datasets]
+    ck = (8 if args.config == "thick" else 0) if args.checkpoint is None else args.checkpoint
+    jobs = [Job(ds, args.batch, dev, ck) for ds in datasets]
     P_total = sum(j.P for j in jobs)
     ragged = args.config == "ragged"
     use_tune = ragged if args.autotune is None else bool(args.autotune)
@@ -362,6 +366,7 @@ def main(argv=None):
     bytes_step = sum(pass_bytes(j.shape[0], j.shape[1], j.shape[3]) * j.P for j in jobs)
     cfg_info = {"workload": WORKLOADS[args.config], "positions": P_total,
                 "compute_batch": tuned if tuned else args.batch, "hip_graph": bool(use_graph),
+                "checkpoint_slices": ck,
                 "datasets": [{"n": j.shape[0], "slices": j.shape[1], "probe_modes": j.shape[2],
                               "obj_modes": j.shape[3], "positions": j.P} for j in jobs]}
     out = {

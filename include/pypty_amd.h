@@ -119,6 +119,17 @@ typedef struct pty_problem {
   int data_bin;
   int data_pad;
   float data_multiplier; /* 0 is taken as 1 */
+  /* Slice checkpointing (no counterpart in the reference, which keeps the
+   * incident wave of every slice, multislice_core.py:66): with
+   * 0 < checkpoint_slices < n_slices the forward keeps the wave entering
+   * every checkpoint_slices-th slice (and every wave of the last segment),
+   * and the object gradient recomputes each other segment of slices from its
+   * checkpoint before running the adjoint through it.  The wave storage per
+   * window and mode falls from n_slices slices to
+   * ceil(n_slices / checkpoint_slices) + checkpoint_slices, for up to one
+   * more forward pass of compute; the results are bit-identical to those
+   * without checkpointing.  0 keeps every slice. */
+  int checkpoint_slices;
 } pty_problem;
 
 typedef struct pty_buffers {

@@ -351,7 +351,36 @@ struct BatchArgs {
   float mult;
   float* meas_stage;        // (B, N, N)
   const float* masks;       // (J, N, N)
+  // slice range [s_lo, s_hi) of a forward, adjoint or objgrad launch (a whole
+  // pass: 0, S) and the stash blocks of st_slots slots per (window, mode).
+  // With r = s - s_lo the forward stores slice s in slot r / st_every when
+  // r < st_dense and r % st_every == 0, and in slot
+  // st_dense / st_every + r - st_dense when r >= st_dense (stash_slot); the
+  // adjoint and objgrad find slice s in slot r
+  int s_lo, s_hi, st_every, st_dense, st_slots;
+  // forward from a stored wave (segment recompute): the wave entering slice
+  // s_lo, in stash order, at fin + (b * M * K + mk) * fin_stride; such a
+  // launch writes no exit wave
+  const float2* fin;
+  size_t fin_stride;
+  // adjoint over a segment: the wave of slice s_hi - 1 is read from gstate
+  // when s_hi < S, and the wave propagated to slice s_lo - 1 is written there
+  // when s_lo > 0 (stash order, (B, M, K, N, N))
+  float2* gstate;
 };
+
+// a launch over every slice with one stash slot per slice
+inline void whole_pass(BatchArgs& a) {
+  a.s_lo = 0; a.s_hi = a.S; a.st_every = 1; a.st_dense = 0; a.st_slots = a.S;
+  a.fin = nullptr; a.fin_stride = 0; a.gstate = nullptr;
+}
+
+// forward stash slot of slice s, or -1 when it is not kept
+__device__ __forceinline__ int stash_slot(const BatchArgs& a, int s) {
+  const int r = s - a.s_lo;
+  if (r >= a.st_dense) return a.st_dense / a.st_every + (r - a.st_dense);
+  return r % a.st_every == 0 ? r / a.st_every : -1;
+}
 
 // measured row of batch slot b (sequence entry b0 + b, position idx)
 __device__ __forceinline__ const float* meas_row(const BatchArgs& a, int b, int idx, size_t row_len) {
@@ -480,26 +509,29 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_forward(BatchArgs a) {
   const int m = mk / a.K, k = mk % a.K;
   const int tid = threadIdx.x;
   const float2* tw = twiddles<N>();
-  const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
+  const float2* src = a.fin ? a.fin + (size_t(b) * MK + mk) * a.fin_stride
+                            : a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
 #pragma unroll 4
   for (int i = 0; i < T::PER_THREAD; ++i) {
     const int e = tid + i * T::NT;
     t[T::lds(e / N, e % N)] = src[e];
   }
-  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
-  for (int s = 0; s < a.S; ++s) {
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  for (int s = a.s_lo; s < a.s_hi; ++s) {
     __syncthreads();
     const float2* o = a.Ob + ((size_t(b) * a.S + s) * a.K + k) * N * N;
+    const int slot = a.write_stash ? stash_slot(a, s) : -1;
+    float2* u_out = slot >= 0 ? st + size_t(slot) * N * N : nullptr;
 #pragma unroll 4
     for (int i = 0; i < T::PER_THREAD; ++i) {
       const int e = tid + i * T::NT;
       const int l = T::lds(e / N, e % N);
       const float2 u = t[l];
-      if (a.write_stash) st[size_t(s) * N * N + e] = u;
+      if (u_out) u_out[e] = u;
       t[l] = cmul(u, o[e]);
     }
     __syncthreads();
-    if (s < a.S - 1) {
+    if (s < a.s_hi - 1) {
       tile_fft_fwd<N>(t, tw);
       const float2* H = (s == 0) ? a.H0 : a.H;
 #pragma unroll 4
@@ -512,6 +544,7 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_forward(BatchArgs a) {
       tile_fft_inv<N>(t, tw);
     }
   }
+  if (a.fin) return;  // segment recompute: the stash is the result
   float2* out = a.Psi + (size_t(b) * MK + mk) * N * N;
   if (!a.far_field) {
 #pragma unroll 4
@@ -752,7 +785,15 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_adjoint(BatchArgs a) {
   const int tid = threadIdx.x;
   const float2* tw = twiddles<N>();
   const float2* psi = a.Psi + (size_t(b) * MK + mk) * N * N;
-  if (a.far_field) {
+  if (a.s_hi < a.S) {
+    // a later segment left the wave of slice s_hi - 1
+    const float2* gs = a.gstate + (size_t(b) * MK + mk) * N * N;
+#pragma unroll 4
+    for (int i = 0; i < T::PER_THREAD; ++i) {
+      const int e = tid + i * T::NT;
+      t[T::lds(e / N, e % N)] = gs[e];
+    }
+  } else if (a.far_field) {
     const float* c = a.coef + size_t(b) * N * N;
     const float s = 1.0f / float(N * N);
 #pragma unroll 4
@@ -772,12 +813,12 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_adjoint(BatchArgs a) {
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g (the reference's waves_multislice *= dLoss); k_objgrad sums
   // the modes, so no two workgroups write the same element
-  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
-  for (int s = a.S - 1; s >= 0; --s) {
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  for (int s = a.s_hi - 1; s >= a.s_lo; --s) {
     __syncthreads();
     const size_t win = ((size_t(b) * a.S + s) * a.K + k) * N * N;
     const float2* o = a.Ob + win;
-    float2* u = st + size_t(s) * N * N;
+    float2* u = st + size_t(s - a.s_lo) * N * N;
 #pragma unroll 4
     for (int i = 0; i < T::PER_THREAD; ++i) {
       const int e = tid + i * T::NT;
@@ -800,12 +841,24 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_adjoint(BatchArgs a) {
       tile_fft_inv<N>(t, tw);
     }
   }
-  float2* out = a.gP + (size_t(b) * MK + mk) * N * N;
+  // the probe-side gradient, or the wave of slice s_lo - 1 for the next segment
+  float2* out = (a.s_lo > 0 ? a.gstate : a.gP) + (size_t(b) * MK + mk) * N * N;
 #pragma unroll 4
   for (int i = 0; i < T::PER_THREAD; ++i) {
     const int e = tid + i * T::NT;
     out[e] = t[T::lds(e / N, e % N)];
   }
+}
+
+// workgroup -> (window b, plane = s * K + k) of an objgrad launch over the
+// slices [s_lo, s_hi)
+__device__ __forceinline__ void objgrad_plane(const BatchArgs& a, int& b, int& plane, int& sl, int& k) {
+  const int splanes = (a.s_hi - a.s_lo) * a.K;
+  b = blockIdx.x / splanes;
+  const int q = blockIdx.x % splanes;
+  sl = a.s_lo + q / a.K;
+  k = q % a.K;
+  plane = sl * a.K + k;
 }
 
 // band-limit each window gradient and scatter-add it (multislice_core.py:184-192)
@@ -814,12 +867,12 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
   using T = Tile<N>;
   PTY_TILE(T, t, a.scratch);
   const int planes = a.S * a.K;
-  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
-  const int sl = plane / a.K, k = plane % a.K;
+  int b, plane, sl, k;
+  objgrad_plane(a, b, plane, sl, k);
   const int tid = threadIdx.x;
   // sum over probe modes of the per-mode window gradients left in the stash
-  const size_t mstride = size_t(a.K) * a.S * N * N;
-  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.S + sl) * N * N;
+  const size_t mstride = size_t(a.K) * a.st_slots * N * N;
+  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.st_slots + (sl - a.s_lo)) * N * N;
 #pragma unroll 4
   for (int i = 0; i < T::PER_THREAD; ++i) {
     const int e = tid + i * T::NT;
@@ -931,25 +984,32 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
   // during the column phase so that its latency hides behind the transforms,
   // and the propagator is read before the column transform that precedes its use
   float2 v[R::R1], o[R::R1];
+  if (a.fin) {
+    const RView<N, float2> I(a.fin + (size_t(b) * MK + mk) * a.fin_stride, tid);
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + 8 * n];
+    for (int n = 0; n < R::R1; ++n) v[n] = I[n];
+  } else {
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + 8 * n];
+  }
   const size_t ostride = size_t(a.K) * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
   {
-    const RView<N, float2> O(ob, tid);
+    const RView<N, float2> O(ob + size_t(a.s_lo) * ostride, tid);
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) o[n] = O[n];
   }
-  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
-  for (int s = 0; s < a.S; ++s) {
-    if (a.write_stash) {
-      const RView<N, float2> U(st + size_t(s) * N * N, tid);
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  for (int s = a.s_lo; s < a.s_hi; ++s) {
+    const int slot = a.write_stash ? stash_slot(a, s) : -1;
+    if (slot >= 0) {
+      const RView<N, float2> U(st + size_t(slot) * N * N, tid);
 #pragma unroll
       for (int n = 0; n < R::R1; ++n) U.store(n, v[n]);
     }
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) v[n] = cmul(v[n], o[n]);
-    if (s < a.S - 1) {
+    if (s < a.s_hi - 1) {
       rt_line_fwd<N, true>(v, T, id.row, id.rj);
       rt_rows_to_cols<N>(v, T, id);
       const RView<N, float2> H((s == 0) ? a.H0 : a.H, tid);
@@ -967,6 +1027,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
       rt_line_inv<N, true>(v, T, id.row, id.rj);
     }
   }
+  if (a.fin) return;  // segment recompute: the stash is the result
   rt_fft2_fwd<N>(v, T, id);
   const RView<N, float2> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
   const RView<N, float> mask(a.maskc, tid);
@@ -988,29 +1049,34 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
   const RTIds<N> id;
   const float sc = 1.0f / float(N * N);
   float2 v[R::R1];
-  {
+  if (a.s_hi < a.S) {
+    // a later segment left the wave of slice s_hi - 1
+    const RView<N, float2> gs(a.gstate + (size_t(b) * MK + mk) * N * N, tid);
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) v[n] = gs[n];
+  } else {
     const RView<N, float2> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
     const RView<N, float> c(a.coef + size_t(b) * N * N, tid);
 #pragma unroll
     for (int i = 0; i < R::R1; ++i) v[i] = cscale(psi[i], c[i] * sc);
+    rt_fft2_inv<N>(v, T, id);
   }
-  rt_fft2_inv<N>(v, T, id);
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g (k_objgrad_rt sums the modes); pu holds slot s, read one
   // column phase ahead like the forward's object window
-  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
   const size_t ostride = size_t(a.K) * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
   float2 pu[R::R1];
   if (a.step_obj) {
-    const RView<N, float2> U(st + size_t(a.S - 1) * N * N, tid);
+    const RView<N, float2> U(st + size_t(a.s_hi - 1 - a.s_lo) * N * N, tid);
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) pu[n] = U[n];
   }
-  for (int s = a.S - 1; s >= 0; --s) {
+  for (int s = a.s_hi - 1; s >= a.s_lo; --s) {
     const RView<N, float2> O(ob + size_t(s) * ostride, tid);
     if (a.step_obj) {
-      const RView<N, float2> U(st + size_t(s) * N * N, tid);
+      const RView<N, float2> U(st + size_t(s - a.s_lo) * N * N, tid);
 #pragma unroll
       for (int n = 0; n < R::R1; ++n) U.store(n, cmulc(v[n], pu[n]));
     }
@@ -1026,8 +1092,8 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
       rt_line_fwd<N, false>(v, T, id.col, id.cj);
 #pragma unroll
       for (int i = 0; i < R::R1; ++i) v[i] = cmulc(v[i], h[i]);
-      if (a.step_obj) {
-        const RView<N, float2> U(st + size_t(s - 1) * N * N, tid);
+      if (a.step_obj && s > a.s_lo) {
+        const RView<N, float2> U(st + size_t(s - 1 - a.s_lo) * N * N, tid);
 #pragma unroll
         for (int n = 0; n < R::R1; ++n) pu[n] = U[n];
       }
@@ -1036,8 +1102,9 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
       rt_line_inv<N, true>(v, T, id.row, id.rj);
     }
   }
-  // probe-side gradient in R-order (read only by k_probegrad_rt)
-  const RView<N, float2> out(a.gP + (size_t(b) * MK + mk) * N * N, tid);
+  // probe-side gradient in R-order (read only by k_probegrad_rt), or the wave
+  // of slice s_lo - 1 for the next segment
+  const RView<N, float2> out((a.s_lo > 0 ? a.gstate : a.gP) + (size_t(b) * MK + mk) * N * N, tid);
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) out.store(n, v[n]);
 }
@@ -1049,12 +1116,12 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
   using R = RT<N>;
   __shared__ float2 T[R::ELEMS];
   const int planes = a.S * a.K;
-  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
-  const int sl = plane / a.K, k = plane % a.K;
+  int b, plane, sl, k;
+  objgrad_plane(a, b, plane, sl, k);
   const unsigned tid = threadIdx.x;
   const RTIds<N> id;
-  const size_t mstride = size_t(a.K) * a.S * N * N;
-  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.S + sl) * N * N;
+  const size_t mstride = size_t(a.K) * a.st_slots * N * N;
+  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.st_slots + (sl - a.s_lo)) * N * N;
   float2 v[R::R1];
   {
     const RView<N, float2> G0(src, tid);
@@ -1368,22 +1435,30 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
   const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
-  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  const float2* fin = a.fin ? a.fin + (size_t(b) * MK + mk) * a.fin_stride : nullptr;
   // o holds the object registers of the current round; those of the next
   // round (or the next slice's first round) are read while this round's
   // forward row transform runs, and the propagator is read before the column
   // transform that precedes its use
   float2 v[G::R1], o[G::R1];
   {
-    const RView<N, float2, G::NT> O(ob, tid);
+    const RView<N, float2, G::NT> O(ob + size_t(a.s_lo) * a.K * N * N, tid);
 #pragma unroll
     for (int n = 0; n < G::R1; ++n) o[n] = O[n];
   }
-  for (int s = 0; s < a.S; ++s) {
+  for (int s = a.s_lo; s < a.s_hi; ++s) {
     // row phase: inverse row transform of the previous slice, stash, object, forward row transform
-    const RView<N, float2, G::NT> U(st + size_t(s) * N * N, tid);
+    const int slot = a.write_stash ? stash_slot(a, s) : -1;
+    const bool keep = slot >= 0;
+    const RView<N, float2, G::NT> U(st + size_t(keep ? slot : 0) * N * N, tid);
+    const bool last_of_recompute = fin && s == a.s_hi - 1;
     for (int r = 0; r < G::ROUNDS; ++r) {
-      if (s == 0) {
+      if (s == a.s_lo && fin) {
+        const RView<N, float2, G::NT> I(fin, tid);
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = I[r * G::R1 + n];
+      } else if (s == a.s_lo) {
         const float2* p = src + size_t(r * G::LINES + L.ln) * N + L.j;
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) v[n] = p[8 * n];
@@ -1393,11 +1468,12 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
       }
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) {
-        if (a.write_stash) U.store(r * G::R1 + n, v[n]);
+        if (keep) U.store(r * G::R1 + n, v[n]);
         v[n] = cmul(v[n], o[n]);
       }
+      if (last_of_recompute) continue;  // only the stash of this slice is wanted
       const int rn = (r + 1 < G::ROUNDS) ? r + 1 : 0, sn = (r + 1 < G::ROUNDS) ? s : s + 1;
-      if (sn < a.S) {
+      if (sn < a.s_hi) {
         const RView<N, float2, G::NT> O(ob + size_t(sn) * a.K * N * N, tid);
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) o[n] = O[rn * G::R1 + n];
@@ -1405,10 +1481,11 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
       gt_store_rowspec<N>(v, sl, L, r);
     }
+    if (last_of_recompute) break;
     __syncthreads();
     // column phase: forward column transform, then the propagator and the
     // inverse column transform, or the far-field band mask after the last slice
-    if (s < a.S - 1) {
+    if (s < a.s_hi - 1) {
       const RView<N, float2, G::NT> H((s == 0) ? a.H0 : a.H, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
         float2 h[G::R1];
@@ -1449,7 +1526,10 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
   const GTLane<N> L;
   const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   float2 v[G::R1];
-  {
+  // a later segment left the wave of slice s_hi - 1 (read in the first row phase)
+  const bool from_state = a.s_hi < a.S;
+  const RView<N, float2, G::NT> GS(a.gstate ? a.gstate + (size_t(b) * MK + mk) * N * N : a.Psi, tid);
+  if (!from_state) {
     // far-field adjoint: inverse column transforms of coef * Psi / N^2
     const RView<N, float2, G::NT> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
     const RView<N, float, G::NT> c(a.coef + size_t(b) * N * N, tid);
@@ -1460,24 +1540,29 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
       rt_line_inv<N, true>(v, X, L.ln, L.j);
       gt_store_col<N>(v, sl, L, r);
     }
+    __syncthreads();
   }
-  __syncthreads();
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g; k_objgrad_gt sums the modes
   // pu holds the stash registers of the current round, read one round ahead
-  float2* st = a.stash + (size_t(b) * MK + mk) * a.S * N * N;
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
   float2 pu[G::R1];
   if (a.step_obj) {
-    const RView<N, float2, G::NT> U(st + size_t(a.S - 1) * N * N, tid);
+    const RView<N, float2, G::NT> U(st + size_t(a.s_hi - 1 - a.s_lo) * N * N, tid);
 #pragma unroll
     for (int n = 0; n < G::R1; ++n) pu[n] = U[n];
   }
-  for (int s = a.S - 1; s >= 0; --s) {
-    const RView<N, float2, G::NT> U(st + size_t(s) * N * N, tid), O(ob + size_t(s) * a.K * N * N, tid);
+  for (int s = a.s_hi - 1; s >= a.s_lo; --s) {
+    const RView<N, float2, G::NT> U(st + size_t(s - a.s_lo) * N * N, tid), O(ob + size_t(s) * a.K * N * N, tid);
     for (int r = 0; r < G::ROUNDS; ++r) {
-      gt_load_rowspec<N>(v, sl, L, r);
-      rt_line_inv<N, true>(v, X, L.ln, L.j);
+      if (from_state && s == a.s_hi - 1) {
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = GS[r * G::R1 + n];
+      } else {
+        gt_load_rowspec<N>(v, sl, L, r);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+      }
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) {
         const int q = r * G::R1 + n;
@@ -1485,8 +1570,8 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
         v[n] = cmulc(v[n], O[q]);
       }
       const int rn = (r + 1 < G::ROUNDS) ? r + 1 : 0, sn = (r + 1 < G::ROUNDS) ? s : s - 1;
-      if (a.step_obj && sn >= 0) {
-        const RView<N, float2, G::NT> Un(st + size_t(sn) * N * N, tid);
+      if (a.step_obj && sn >= a.s_lo) {
+        const RView<N, float2, G::NT> Un(st + size_t(sn - a.s_lo) * N * N, tid);
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) pu[n] = Un[rn * G::R1 + n];
       }
@@ -1516,6 +1601,15 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
       __syncthreads();
     }
   }
+  if (a.s_lo > 0) {
+    // the wave propagated to slice s_lo - 1, as natural rows, for the next segment
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      gt_load_rowspec<N>(v, sl, L, r);
+      rt_line_inv<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) GS.store(r * G::R1 + n, v[n]);
+    }
+  }
 }
 
 // k_objgrad for 192/256 px windows: mode sum from the R-order stash, band
@@ -1525,13 +1619,13 @@ __global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
   using G = GT<N, kGtThreads>;
   __shared__ float2 X[G::LINES * G::LS];
   const int planes = a.S * a.K;
-  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
-  const int sl_ = plane / a.K, k = plane % a.K;
+  int b, plane, sl_, k;
+  objgrad_plane(a, b, plane, sl_, k);
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
   const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
-  const size_t mstride = size_t(a.K) * a.S * N * N;
-  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.S + sl_) * N * N;
+  const size_t mstride = size_t(a.K) * a.st_slots * N * N;
+  const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.st_slots + (sl_ - a.s_lo)) * N * N;
   float2 v[G::R1];
   for (int r = 0; r < G::ROUNDS; ++r) {
     {
@@ -1832,11 +1926,12 @@ struct TimedLaunch {
 };
 enum {
   KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
-  KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER, KT_COUNT
+  KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER,
+  KT_RECOMPUTE, KT_COUNT
 };
 const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
                                     "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
-                                    "loss_final", "zero", "scatteradd", "gather_grad"};
+                                    "loss_final", "zero", "scatteradd", "gather_grad", "recompute"};
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
@@ -1919,9 +2014,11 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 
 // ---------------------------------------------------------- workspace
 struct Layout {
-  size_t H, H0, mask, maskc, fprobe, objp, gradp, Ob, Pp, stash, Psi, coef, Gbuf, gP, probe_part, probe_sum,
-      pos_part, loss_part, meas_stage, scratch, total;
+  size_t H, H0, mask, maskc, fprobe, objp, gradp, Ob, Pp, stash, gstate, Psi, coef, Gbuf, gP, probe_part,
+      probe_sum, pos_part, loss_part, meas_stage, scratch, total;
   int B, G;
+  int ck, C;  // slices per checkpoint segment (0: every slice kept) and segments
+  int slots;  // stash slots per window and mode
 };
 inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -1950,6 +2047,13 @@ int batch_windows(const pty_problem* p) {
     B = std::min(B, cap);
   }
   return B;
+}
+
+// slices per checkpoint segment, or 0 when every slice's wave is kept (the
+// stash feeds only the object gradient)
+int checkpoint_every(const pty_problem* p) {
+  const int c = p->checkpoint_slices;
+  return (p->step_obj && c > 0 && c < p->n_slices) ? c : 0;
 }
 
 bool compressed(int algorithm) {
@@ -1981,7 +2085,14 @@ Layout plan_layout(const pty_problem* p) {
   L.gradp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
   L.Ob = take(size_t(B) * S * K * n2 * 8);
   L.Pp = take(size_t(B) * M * n2 * 8);
-  L.stash = take(size_t(B) * M * K * S * n2 * 8);
+  // every slice, or the checkpoints of all segments but the last followed by
+  // one segment of slices (the last segment's, left by the forward, then each
+  // recomputed segment's in turn)
+  L.ck = checkpoint_every(p);
+  L.C = L.ck ? (S + L.ck - 1) / L.ck : 0;
+  L.slots = L.ck ? L.C - 1 + L.ck : S;
+  L.stash = take(size_t(B) * M * K * L.slots * n2 * 8);
+  L.gstate = L.ck ? take(size_t(B) * M * K * n2 * 8) : 0;
   L.Psi = take(size_t(B) * M * K * n2 * 8);
   L.coef = take(size_t(B) * n2 * 4);
   L.Gbuf = 0;
@@ -2008,6 +2119,7 @@ int validate(const pty_problem* p) {
   if (!pty_supported_window(p->n) && p->n != 192 && p->n != 256) return PTY_ERR_SIZE;
   if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1 || p->compute_batch < 1) return PTY_ERR_ARG;
   if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
+  if (p->checkpoint_slices < 0) return PTY_ERR_ARG;
   if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_POISSON_COMPRESSED) return PTY_ERR_ARG;
   if (compressed(p->algorithm) && (p->n_masks < 1 || p->n_masks > kMaxMasks)) return PTY_ERR_ARG;
   if (preprocessing(p)) {
@@ -2038,11 +2150,56 @@ int zero_fill(void* ptr, size_t bytes, hipStream_t s) {
 }
 
 template <int N>
-int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
+void launch_forward(const BatchArgs& a, hipStream_t s) {
+  constexpr int LAY = kLayout<N>;
+  const int grid = xcd_grid(a.B, a.M * a.K);
+  if constexpr (LAY == LAY_RT)
+    k_forward_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
+  else if constexpr (LAY == LAY_GT)
+    k_forward_gt<N><<<grid, kGtThreads, 0, s>>>(a);
+  else
+    k_forward<N><<<grid, Tile<N>::NT, 0, s>>>(a);
+}
+
+template <int N>
+void launch_adjoint(const BatchArgs& a, hipStream_t s) {
+  constexpr int LAY = kLayout<N>;
+  const int grid = xcd_grid(a.B, a.M * a.K);
+  if constexpr (LAY == LAY_RT)
+    k_adjoint_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
+  else if constexpr (LAY == LAY_GT)
+    k_adjoint_gt<N><<<grid, kGtThreads, 0, s>>>(a);
+  else
+    k_adjoint<N><<<grid, Tile<N>::NT, 0, s>>>(a);
+}
+
+// one workgroup per (window, slice of [s_lo, s_hi), object mode)
+template <int N>
+void launch_objgrad(const BatchArgs& a, hipStream_t s) {
+  constexpr int LAY = kLayout<N>;
+  const int grid = a.B * (a.s_hi - a.s_lo) * a.K;
+  if constexpr (LAY == LAY_RT)
+    k_objgrad_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
+  else if constexpr (LAY == LAY_GT)
+    k_objgrad_gt<N><<<grid, kGtThreads, 0, s>>>(a);
+  else
+    k_objgrad<N><<<grid, Tile<N>::NT, 0, s>>>(a);
+}
+
+// One batch of windows.  With checkpoint segments (ck > 0, C segments) the
+// forward keeps, per window and mode, the wave entering segments 0..C-2 in
+// stash slots 0..C-2 and every wave of the last segment in the tail slots
+// C-1.. (a.st_dense).  The object gradient then walks the segments from the
+// last: recompute the segment's waves from its checkpoint into the tail
+// (except the last segment, already there), run the adjoint through the
+// segment (the wave crossing segments goes through a.gstate), and band-limit
+// the segment's window gradients.  Every launch computes exactly what the
+// whole-pass launch computes for those slices, in the same order.
+template <int N>
+int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
   using T = Tile<N>;
   constexpr int LAY = kLayout<N>;
   const int planes = a.S * a.K;
-  const int MK = a.M * a.K;
   {
     TimeScope ts(KT_PATCH, s);
     if constexpr (LAY == LAY_RT)
@@ -2061,12 +2218,7 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
   }
   {
     TimeScope ts(KT_FORWARD, s);
-    if constexpr (LAY == LAY_RT)
-      k_forward_rt<N><<<xcd_grid(a.B, MK), RT<N>::NT, 0, s>>>(a);
-    else if constexpr (LAY == LAY_GT)
-      k_forward_gt<N><<<xcd_grid(a.B, MK), kGtThreads, 0, s>>>(a);
-    else
-      k_forward<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
+    launch_forward<N>(a, s);
   }
   if (a.prep) {
     TimeScope ts(KT_LOSS, s);
@@ -2079,25 +2231,40 @@ int run_batch(BatchArgs& a, const pty_problem* p, hipStream_t s) {
     else
       k_loss<N, LAY><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
   }
-  if (p->step_obj || p->step_probe || p->step_pos) {
-    TimeScope ts(KT_ADJOINT, s);
-    if constexpr (LAY == LAY_RT)
-      k_adjoint_rt<N><<<xcd_grid(a.B, MK), RT<N>::NT, 0, s>>>(a);
-    else if constexpr (LAY == LAY_GT)
-      k_adjoint_gt<N><<<xcd_grid(a.B, MK), kGtThreads, 0, s>>>(a);
-    else
-      k_adjoint<N><<<xcd_grid(a.B, MK), T::NT, 0, s>>>(a);
+  if (p->step_obj && ck) {
+    const int C = (a.S + ck - 1) / ck;
+    for (int c = C - 1; c >= 0; --c) {
+      BatchArgs g = a;
+      g.s_lo = c * ck;
+      g.s_hi = std::min(a.S, g.s_lo + ck);
+      g.st_every = 1;
+      g.st_dense = 0;
+      g.stash = a.stash + size_t(C - 1) * N * N;  // the tail slots of each block
+      if (c < C - 1) {
+        TimeScope ts(KT_RECOMPUTE, s);
+        BatchArgs f = g;
+        f.fin = a.stash + size_t(c) * N * N;
+        f.fin_stride = size_t(a.st_slots) * N * N;
+        launch_forward<N>(f, s);
+      }
+      {
+        TimeScope ts(KT_ADJOINT, s);
+        launch_adjoint<N>(g, s);
+      }
+      TimeScope ts(KT_OBJGRAD, s);
+      launch_objgrad<N>(g, s);
+    }
+  } else {
+    if (p->step_obj || p->step_probe || p->step_pos) {
+      TimeScope ts(KT_ADJOINT, s);
+      launch_adjoint<N>(a, s);
+    }
+    if (p->step_obj) {
+      TimeScope ts(KT_OBJGRAD, s);
+      launch_objgrad<N>(a, s);
+    }
   }
   if (p->step_obj) {
-    {
-      TimeScope ts(KT_OBJGRAD, s);
-      if constexpr (LAY == LAY_RT)
-        k_objgrad_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
-      else if constexpr (LAY == LAY_GT)
-        k_objgrad_gt<N><<<a.B * planes, kGtThreads, 0, s>>>(a);
-      else
-        k_objgrad<N><<<a.B * planes, T::NT, 0, s>>>(a);
-    }
     TimeScope ts(KT_GATHER, s);
     const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
     k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
@@ -2163,7 +2330,7 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   BatchArgs a{};
   a.S = S; a.K = K; a.M = M; a.ny = p->obj_ny; a.nx = p->obj_nx; a.P = p->n_positions;
   a.algorithm = p->algorithm; a.step_obj = p->step_obj; a.step_probe = p->step_probe; a.step_pos = p->step_pos;
-  a.shift = p->shift_active; a.far_field = 1; a.write_stash = 1; a.scatter = 1;
+  a.shift = p->shift_active; a.far_field = 1; a.write_stash = p->step_obj; a.scatter = 1;
   a.loss_weight = p->loss_weight;
   a.obj = objp; a.probe = static_cast<const float2*>(bf->probe); a.fprobe = fprobe;
   a.pos = bf->positions; a.corr = bf->corrections; a.meas = bf->measured; a.seq = bf->sequence;
@@ -2183,11 +2350,19 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.meas_stage = L.meas_stage ? reinterpret_cast<float*>(ws + L.meas_stage) : nullptr;
   a.masks = bf->masks;
   a.J = compressed(p->algorithm) ? p->n_masks : 0;
+  whole_pass(a);
+  if (L.ck) {
+    // the forward keeps the checkpoints and every wave of the last segment
+    a.st_every = L.ck;
+    a.st_dense = (L.C - 1) * L.ck;
+    a.st_slots = L.slots;
+    a.gstate = reinterpret_cast<float2*>(ws + L.gstate);
+  }
   for (int b0 = 0; b0 < bf->n_seq; b0 += L.B) {
     a.b0 = b0;
     a.B = std::min(L.B, bf->n_seq - b0);
     a.G = std::min(L.G, a.B);
-    if ((rc = run_batch<N>(a, p, s))) return rc;
+    if ((rc = run_batch<N>(a, p, L.ck, s))) return rc;
   }
   {
     TimeScope ts(KT_LOSS_FINAL, s);
@@ -2245,6 +2420,7 @@ int ms_forward_n(const pty_problem* p, int batch, const void* probe_b, const voi
   a.stash = static_cast<float2*>(stash);
   a.Psi = static_cast<float2*>(exit_wave);
   a.H = H; a.H0 = H0; a.mask = mask;
+  whole_pass(a);
   TimeScope ts(KT_FORWARD, s);
   k_forward<N><<<xcd_grid(batch, a.M * a.K), T::NT, 0, s>>>(a);
   return check_launch();
@@ -2269,6 +2445,7 @@ int ms_adjoint_n(const pty_problem* p, int batch, void* g, void* stash, const vo
   a.Ob = const_cast<float2*>(static_cast<const float2*>(obj_b));
   a.Gbuf = static_cast<float2*>(grad_b);
   a.H = H; a.H0 = H0; a.mask = mask;
+  whole_pass(a);
   {
     TimeScope ts(KT_ADJOINT, s);
     k_adjoint<N><<<xcd_grid(batch, a.M * a.K), T::NT, 0, s>>>(a);

@@ -88,8 +88,11 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
                 slow_axis_reg_weight_positions, slow_axis_reg_weight_tilts, scan_size, scan_step_A,
                 current_deformation_reg_weight_positions, current_deformation_reg_weight_tilts, warnings,
                 data_simulation_flag, data_simulation_exitwave, pos_puzzling, probe_mod_reg_weight, probe_mod_tol,
-                use_padded_intensity, counter_failed_epochs):
+                use_padded_intensity, counter_failed_epochs, *, checkpoint_slices=0):
     """One L-BFGS epoch with Wolfe line search (pypty/iterative.py:440-1049).
+
+    checkpoint_slices (keyword only, not in the reference): passed to every
+    loss_and_direction call of the epoch.
 
     Returns (total_loss, sse, constraint_contributions, actual_step, count,
     d_value, new_d_value, warnings, counter_failed_epochs).
@@ -128,7 +131,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             current_deformation_reg_weight_tilts, fast_axis_reg_weight_tilts, aperture_mask, 0, current_window_weight,
             current_window, phase_norm_weight, abs_norm_weight, atv_weight, atv_q, atv_p, mixed_variance_weight,
             mixed_variance_sigma, wedge_mu, beta_wedge, probe_mod_reg_weight, probe_mod_tol, smart_memory,
-            print_flag, data_simulation_flag, data_simulation_exitwave, use_padded_intensity)
+            print_flag, data_simulation_flag, data_simulation_exitwave, use_padded_intensity,
+            checkpoint_slices=checkpoint_slices)
         loss, sse, og, pg, posg = out[0], out[1], out[2], out[3], out[4]
         if update_probe:
             pg = _clean(pg, cut, smooth_rolloff)
@@ -395,7 +399,9 @@ def run(pypty_params):
     wolfe_c1_constant, wolfe_c2_constant, max_count, reduce_factor, optimism,
     min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
-    save_loss_log, print_flag.  Keys selecting features outside the path raise
+    save_loss_log, print_flag, and checkpoint_slices (not in the reference:
+    keep every k-th slice's wave and recompute the rest in the adjoint).
+    Keys selecting features outside the path raise
     NotImplementedError through bfgs_update / loss_and_direction.  Writes
     co.npy, cp.npy, cg.npy and loss.csv like the reference; returns the final
     object, probe, positions and per-epoch losses (the reference returns None).
@@ -527,7 +533,8 @@ def run(pypty_params):
             slow_axis_reg_weight_tilts=0, scan_size=None, scan_step_A=1.0, current_deformation_reg_weight_positions=0,
             current_deformation_reg_weight_tilts=0, warnings=warnings, data_simulation_flag=False,
             data_simulation_exitwave=False, pos_puzzling=False, probe_mod_reg_weight=0, probe_mod_tol=0.05,
-            use_padded_intensity=True, counter_failed_epochs=counter_failed)
+            use_padded_intensity=True, counter_failed_epochs=counter_failed,
+            checkpoint_slices=int(get("checkpoint_slices", 0)))
         total_loss, sse, cc, actual_step, count, d_value, new_d_value, warnings, counter_failed = out
         is_first = False
         losses.append(total_loss)

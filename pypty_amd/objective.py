@@ -71,6 +71,9 @@ class PassConfig:
     data_bin: int = 1
     data_pad: int = 0
     data_multiplier: float = 1.0
+    # keep the wave entering every checkpoint_slices-th slice and recompute the
+    # others in the adjoint (0: keep every slice); the results are bit-identical
+    checkpoint_slices: int = 0
 
     @property
     def compressed(self) -> bool:
@@ -88,7 +91,7 @@ class PassConfig:
             float(self.wavelength), float(self.pixel_size_x), float(self.pixel_size_y), float(self.tan_x),
             float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier), int(self.n_masks),
             int(self.raw_n), int(self.data_shift_y), int(self.data_shift_x), int(self.data_bin), int(self.data_pad),
-            float(self.data_multiplier))
+            float(self.data_multiplier), int(self.checkpoint_slices))
 
 
 class PtychoPass:
@@ -261,7 +264,7 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
                        probe_reg_weight, current_window_weight, current_window, phase_norm_weight, abs_norm_weight,
                        atv_weight, atv_q, atv_p, mixed_variance_weight, mixed_variance_sigma, wedge_mu, beta_wedge,
                        probe_mod_reg_weight, probe_mod_tol, smart_memory, print_flag, data_simulation_flag,
-                       data_simulation_exitwave, use_padded_intensity):
+                       data_simulation_exitwave, use_padded_intensity, *, checkpoint_slices=0):
     """Drop-in for pypty.objective.loss_and_direction (pypty/objective.py:47).
 
     this_obj (Ny, Nx, S, K) complex64 and full_probe (N, N, M) complex64 are
@@ -271,6 +274,10 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     static_background_grad, aberrations_array_grad, beam_current_grad,
     constraint_contributions) like the reference; unused gradients are 0 /
     zero tensors as there.
+
+    checkpoint_slices (keyword only, not in the reference): keep the wave
+    entering every k-th slice and recompute the others in the adjoint, for
+    objects whose full wave stash does not fit; the results are identical.
     """
     _unsupported("recon_type", recon_type != "far_field")
     _unsupported("propmethod", propmethod != "multislice")
@@ -346,7 +353,7 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         edge_multiplier=float(edge_slice_thickness_multiplier),
         n_masks=int(masks_dev.shape[0]) if compressed else 0, raw_n=0 if compressed else int(measured.shape[-1]),
         data_shift_y=shift_y, data_shift_x=shift_x, data_bin=1 if compressed else int(data_bin), data_pad=pad,
-        data_multiplier=1.0 if compressed else float(data_multiplier))
+        data_multiplier=1.0 if compressed else float(data_multiplier), checkpoint_slices=int(checkpoint_slices))
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None

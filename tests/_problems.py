@@ -85,9 +85,12 @@ def reference_kwargs(**over):
 
 
 def call_loss_and_direction(fn, **kw):
-    """Call a loss_and_direction with the reference's positional argument order."""
-    names = list(inspect.signature(fn).parameters)
-    return fn(*[kw[k] for k in names])
+    """Call a loss_and_direction with the reference's positional argument order; keyword-only
+    extensions are passed by name when given."""
+    params = inspect.signature(fn).parameters.values()
+    pos = [kw[p.name] for p in params if p.kind == p.POSITIONAL_OR_KEYWORD]
+    extra = {p.name: kw[p.name] for p in params if p.kind == p.KEYWORD_ONLY and p.name in kw}
+    return fn(*pos, **extra)
 
 
 def rel(a, b):

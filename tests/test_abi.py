@@ -83,3 +83,32 @@ def test_workspace_size_formula(lib):
     n2 = 128 * 128 * 8
     stash = 256 * 5 * 10 * n2
     assert stash < size.value < 3 * stash
+
+
+def test_checkpoint_segments_shrink_the_workspace(lib):
+    """Thick sample, 256 px windows, 40 slices, 10 modes, 512 windows: keeping every 8th slice's wave
+    (4 checkpoints + an 8-slice segment + the wave carried between segments) needs 13 wave slots per
+    window and mode instead of 40."""
+    from pypty_amd import _lib
+
+    def ws(ck, step_obj=1):
+        p = _lib.PtyProblem()
+        p.n, p.n_slices, p.n_obj_modes, p.n_probe_modes = 256, 40, 1, 10
+        p.obj_ny = p.obj_nx = 1536
+        p.compute_batch, p.n_positions, p.obj_layout = 512, 16384, 1
+        p.step_obj, p.checkpoint_slices = step_obj, ck
+        size = ctypes.c_size_t(0)
+        rc = lib.pty_workspace_bytes(ctypes.byref(p), ctypes.byref(size))
+        return rc, size.value
+
+    rc, full = ws(0)
+    assert rc == 0
+    rc, ck8 = ws(8)
+    assert rc == 0
+    # one wave slot for every window and mode of a launch; windows larger than 128 px cap the
+    # launch so that one tile per workgroup fits in 2 GiB
+    windows = min(512, max(8, (2 << 30) // (40 * 256 * 256 * 8)))
+    slot = windows * 10 * 256 * 256 * 8
+    assert full - ck8 == (40 - 13) * slot
+    assert ws(40)[1] == full and ws(0, step_obj=0)[1] == ws(8, step_obj=0)[1]  # inactive cases keep every slice
+    assert ws(-1)[0] == -1
