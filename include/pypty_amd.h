@@ -224,6 +224,41 @@ void pty_timing_enable(int on);
 int pty_timing_read(const char** names, double* total_ms, int64_t* launches, int cap);
 void pty_timing_reset(void);
 
+/* ---- Learned warm start (no counterpart in the reference: a small CNN that
+ * maps a diffraction pattern to the projected phase around the probe, whose
+ * stitched predictions replace the flat starting object `obj` of
+ * pypty/iterative.py:56-250).  BF16 operands are raw bit patterns
+ * (uint16), rounded to nearest even. */
+
+/* C (M x N, row stride ldc, FP32) = op(A) x op(B) with BF16 operands and FP32
+ * accumulation on the matrix cores.  op(A)[m][k] = ta ? A[k*lda + m] :
+ * A[m*lda + k]; op(B)[k][n] = tb ? B[n*ldb + k] : B[k*ldb + n].  A long
+ * reduction over few output tiles is split into as many parts (up to 256) as
+ * `partial` (partial_elems floats) holds M x N partial products of, summed in
+ * a fixed order; without `partial` it runs unsplit. */
+int pty_gemm_bf16(int ta, int tb, int M, int N, int K, const void* A, int lda, const void* B, int ldb, float* C,
+                  int ldc, float* partial, size_t partial_elems, void* stream);
+/* cols[p][(ky*ks + kx)*C + c] (BF16) = X[b][oy*stride - pad + ky][ox*stride - pad + kx][c]
+ * (0 outside) for X (batch, H, W, C) FP32 and p = (b*Ho + oy)*Wo + ox. */
+int pty_cnn_im2col(const float* X, int batch, int H, int W, int C, int ks, int stride, int pad, void* cols,
+                   void* stream);
+/* the adjoint of im2col: dX (batch, H, W, C) = sum of the dcols entries that read each element */
+int pty_cnn_col2im(const float* dcols, int batch, int H, int W, int C, int ks, int stride, int pad, float* dX,
+                   void* stream);
+/* A[p][c] = act(Y[p][c] + bias[c]), act = ReLU if relu else identity; A (FP32) and Abf (BF16) optional */
+int pty_cnn_bias_act(const float* Y, int64_t P, int C, const float* bias, int relu, float* A, void* Abf,
+                     void* stream);
+/* dY = dA * [A > 0] (relu) or dA, as FP32 and BF16 */
+int pty_cnn_act_grad(const float* dA, const float* A, int64_t n, int relu, float* dY, void* dYbf, void* stream);
+/* db[c] = sum over p of dY[p][c] */
+int pty_cnn_bias_grad(const float* dY, int64_t P, int C, float* db, void* stream);
+int pty_cnn_to_bf16(const float* x, int64_t n, void* y, void* stream);
+/* loss[0] = mean (Y - T)^2 (FP64), dY = 2 (Y - T) / n */
+int pty_cnn_mse(const float* Y, const float* T, int64_t n, float* dY, double* loss, void* stream);
+/* one bias-corrected Adam step on FP32 weights w (BF16 copy wbf optional) */
+int pty_cnn_adam(float* w, void* wbf, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+                 float beta2, float eps, int step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,8 @@ EXPORTED = (
     "pty_scatteradd_c64", "pty_workspace_bytes", "pty_loss_and_direction", "pty_tables_bytes",
     "pty_ms_forward", "pty_ms_adjoint", "pty_timing_enable", "pty_timing_read", "pty_timing_reset",
     "pty_fourier_clean_c64", "pty_plane_fft2_c64", "pty_multidot_c64", "pty_axpby_c64", "pty_phase_grad_c64",
+    "pty_gemm_bf16", "pty_cnn_im2col", "pty_cnn_col2im", "pty_cnn_bias_act", "pty_cnn_act_grad",
+    "pty_cnn_bias_grad", "pty_cnn_to_bf16", "pty_cnn_mse", "pty_cnn_adam",
 )
 
 
@@ -112,6 +114,20 @@ def load(path: str | None = None) -> ctypes.CDLL:
     lib.pty_timing_read.argtypes = [P(ctypes.c_char_p), P(ctypes.c_double), P(ctypes.c_int64), i]
     lib.pty_timing_reset.restype = None
     lib.pty_timing_reset.argtypes = []
+    f = ctypes.c_float
+    for name, args in (
+            ("pty_gemm_bf16", [i, i, i, i, i, vp, i, vp, i, vp, i, vp, sz, vp]),
+            ("pty_cnn_im2col", [vp, i, i, i, i, i, i, i, vp, vp]),
+            ("pty_cnn_col2im", [vp, i, i, i, i, i, i, i, vp, vp]),
+            ("pty_cnn_bias_act", [vp, i64, i, vp, i, vp, vp, vp]),
+            ("pty_cnn_act_grad", [vp, vp, i64, i, vp, vp, vp]),
+            ("pty_cnn_bias_grad", [vp, i64, i, vp, vp]),
+            ("pty_cnn_to_bf16", [vp, i64, vp, vp]),
+            ("pty_cnn_mse", [vp, vp, i64, vp, vp, vp]),
+            ("pty_cnn_adam", [vp, vp, vp, vp, vp, i64, f, f, f, f, i, vp])):
+        fn = getattr(lib, name)
+        fn.restype = i
+        fn.argtypes = args
     _lib = lib
     return lib
 

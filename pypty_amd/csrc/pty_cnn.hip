@@ -1,0 +1,316 @@
+// Learned warm start: the kernels of a small convolutional network that maps
+// a log-scaled diffraction pattern to the projected phase around the probe.
+// The network has no counterpart in the reference; it produces an initial
+// object for the reconstruction (the reference's `obj` parameter,
+// pypty/iterative.py:56-250).
+//
+// Every layer is one GEMM on the matrix cores with BF16 operands and FP32
+// accumulation (v_mfma_f32_32x32x8_bf16): a convolution as im2col followed by
+// cols x W^T, its backward as dY^T x cols (weights) and dY x W then col2im
+// (input); a dense layer is the same with one pixel per pattern.  Activations
+// are pixel-major with channels innermost (NHWC).  Weights are FP32 masters
+// with a BF16 copy for the products.  All reductions run in a fixed order, so
+// results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "pypty_amd.h"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bf16 bits, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+int launch_status() {
+  return hipGetLastError() == hipSuccess ? PTY_OK : PTY_ERR_HIP;
+}
+
+// C[z] (M x N, row stride ldc) = op(A) x op(B) over k in [z*kchunk, (z+1)*kchunk)
+//   op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
+//   op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
+// Workgroup: 4 wavefronts, a 64 x 64 tile of C; each wavefront one 32 x 32
+// block accumulated by v_mfma_f32_32x32x8_bf16.  Operand lanes: lane l holds
+// A[m0 + l%32][k + 4*(l/32) + v] and B[k + 4*(l/32) + v][n0 + l%32], v = 0..3;
+// accumulator r of lane l is C[m0 + 8*(r/4) + 4*(l/32) + r%4][n0 + l%32].
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
+                                                   const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
+                                                   int ldc, size_t zstride, int kchunk) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = blockIdx.y * 64 + (wave >> 1) * 32, n0 = blockIdx.x * 64 + (wave & 1) * 32;
+  const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
+  const int li = lane & 31, kq = (lane >> 5) * 4;
+  const int m = m0 + li, n = n0 + li;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  for (int k = k_lo; k < k_hi; k += 8) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int kk = k + kq + v;
+      const bool kin = kk < k_hi;
+      a[v] = (kin && m < M) ? short(TA ? A[size_t(kk) * lda + m] : A[size_t(m) * lda + kk]) : short(0);
+      b[v] = (kin && n < N) ? short(TB ? B[size_t(n) * ldb + kk] : B[size_t(kk) * ldb + n]) : short(0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  float* Cz = C + size_t(blockIdx.z) * zstride;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (row < M && n < N) Cz[size_t(row) * ldc + n] = acc[r];
+  }
+}
+
+// C[m][n] = sum over z of part[z][m][n] (z ascending)
+__global__ void k_splitk_sum(const float* __restrict__ part, int splits, int M, int N, float* __restrict__ C,
+                             int ldc) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(M) * N) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += part[size_t(z) * M * N + i];
+  C[(i / N) * ldc + i % N] = s;
+}
+
+// cols[p][(ky*ks + kx)*C + c] = X[b][oy*st - pad + ky][ox*st - pad + kx][c] (0 outside),
+// p = (b*Ho + oy)*Wo + ox
+__global__ void k_im2col(const float* __restrict__ X, int Bn, int H, int W, int C, int ks, int st, int pad, int Ho,
+                         int Wo, uint16_t* __restrict__ cols) {
+  const int Kc = ks * ks * C;
+  const size_t total = size_t(Bn) * Ho * Wo * Kc;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+    const size_t p = i / Kc;
+    const int q = int(i % Kc);
+    const int c = q % C, t = q / C, kx = t % ks, ky = t / ks;
+    const int ox = int(p % Wo), oy = int((p / Wo) % Ho), b = int(p / (size_t(Wo) * Ho));
+    const int iy = oy * st - pad + ky, ix = ox * st - pad + kx;
+    const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
+    cols[i] = in ? to_bf16(X[((size_t(b) * H + iy) * W + ix) * C + c]) : uint16_t(0);
+  }
+}
+
+// dX[b][iy][ix][c] = sum over the taps (ky, kx) that read (iy, ix) of
+// dcols[(b, oy, ox)][(ky*ks + kx)*C + c], taps in (ky, kx) order
+__global__ void k_col2im(const float* __restrict__ dcols, int Bn, int H, int W, int C, int ks, int st, int pad, int Ho,
+                         int Wo, float* __restrict__ dX) {
+  const int Kc = ks * ks * C;
+  const size_t total = size_t(Bn) * H * W * C;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+    const int c = int(i % C);
+    const int ix = int((i / C) % W), iy = int((i / (size_t(C) * W)) % H), b = int(i / (size_t(C) * W * H));
+    float s = 0.f;
+    for (int ky = 0; ky < ks; ++ky) {
+      const int ty = iy + pad - ky;
+      if (ty < 0 || ty % st) continue;
+      const int oy = ty / st;
+      if (oy >= Ho) continue;
+      for (int kx = 0; kx < ks; ++kx) {
+        const int tx = ix + pad - kx;
+        if (tx < 0 || tx % st) continue;
+        const int ox = tx / st;
+        if (ox >= Wo) continue;
+        s += dcols[((size_t(b) * Ho + oy) * Wo + ox) * Kc + (ky * ks + kx) * C + c];
+      }
+    }
+    dX[i] = s;
+  }
+}
+
+// A[p][c] = act(Y[p][c] + bias[c]) (act: ReLU or identity); optional FP32
+// and BF16 outputs
+__global__ void k_bias_act(const float* __restrict__ Y, size_t P, int C, const float* __restrict__ bias, int relu,
+                           float* __restrict__ A, uint16_t* __restrict__ Abf) {
+  const size_t total = P * C;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+    float v = Y[i] + bias[i % C];
+    if (relu) v = v > 0.f ? v : 0.f;
+    if (A) A[i] = v;
+    if (Abf) Abf[i] = to_bf16(v);
+  }
+}
+
+// dY = dA * relu'(A) (or dA); FP32 and BF16 copies
+__global__ void k_act_grad(const float* __restrict__ dA, const float* __restrict__ A, size_t n, int relu,
+                           float* __restrict__ dY, uint16_t* __restrict__ dYbf) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const float g = (relu && !(A[i] > 0.f)) ? 0.f : dA[i];
+    dY[i] = g;
+    dYbf[i] = to_bf16(g);
+  }
+}
+
+// db[c] = sum over p of dY[p][c]: one workgroup per channel, thread t sums
+// p = t, t + 256, ... in order, then a fixed tree
+__global__ __launch_bounds__(256) void k_bias_grad(const float* __restrict__ dY, size_t P, int C,
+                                                   float* __restrict__ db) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (size_t p = threadIdx.x; p < P; p += 256) s += dY[p * C + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] = red[0];
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    y[i] = to_bf16(x[i]);
+}
+
+// mean squared error over n outputs: loss[0] = sum (y - t)^2 / n (FP64, one
+// workgroup, fixed order), dY = 2 (y - t) / n
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const float* __restrict__ T, size_t n,
+                                             float* __restrict__ dY, double* __restrict__ loss) {
+  __shared__ double red[256];
+  double s = 0.0;
+  const float sc = 2.0f / float(n);
+  for (size_t i = threadIdx.x; i < n; i += 256) {
+    const float d = Y[i] - T[i];
+    dY[i] = sc * d;
+    s += double(d) * double(d);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = red[0] / double(n);
+}
+
+// Adam (Kingma & Ba 2015, bias-corrected) on FP32 masters; writes the BF16 copy
+__global__ void k_adam(float* __restrict__ w, uint16_t* __restrict__ wbf, const float* __restrict__ g,
+                       float* __restrict__ m, float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wbf) wbf[i] = to_bf16(wi);
+  }
+}
+
+int grid_for(size_t n) { return int(std::min<size_t>((n + 255) / 256, 65536)); }
+
+}  // namespace
+
+extern "C" {
+
+int pty_gemm_bf16(int ta, int tb, int M, int N, int K, const void* A, int lda, const void* B, int ldb, float* C,
+                  int ldc, float* partial, size_t partial_elems, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || ldc < N) return PTY_ERR_ARG;
+  if (lda < (ta ? M : K) || ldb < (tb ? K : N)) return PTY_ERR_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int gx = (N + 63) / 64, gy = (M + 63) / 64;
+  // split the reduction when the output has few tiles and the reduction is long
+  int splits = 1;
+  if (gx * gy < 64 && K >= 4096) splits = std::min(256, std::max(1, K / 2048));
+  if (splits > 1) {
+    // as many as the partial buffer holds
+    const size_t fit = partial ? partial_elems / (size_t(M) * N) : 0;
+    splits = int(std::min<size_t>(size_t(splits), fit));
+    if (splits < 2) splits = 1;
+  }
+  const int kchunk = ((K + splits - 1) / splits + 7) / 8 * 8;
+  splits = (K + kchunk - 1) / kchunk;
+  const dim3 grid(gx, gy, splits);
+  const auto* a = static_cast<const uint16_t*>(A);
+  const auto* b = static_cast<const uint16_t*>(B);
+  float* out = splits > 1 ? partial : C;
+  const int ldo = splits > 1 ? N : ldc;
+  const size_t zs = splits > 1 ? size_t(M) * N : 0;
+  if (ta && tb) k_gemm_bf16<true, true><<<grid, 256, 0, s>>>(M, N, K, a, lda, b, ldb, out, ldo, zs, kchunk);
+  else if (ta) k_gemm_bf16<true, false><<<grid, 256, 0, s>>>(M, N, K, a, lda, b, ldb, out, ldo, zs, kchunk);
+  else if (tb) k_gemm_bf16<false, true><<<grid, 256, 0, s>>>(M, N, K, a, lda, b, ldb, out, ldo, zs, kchunk);
+  else k_gemm_bf16<false, false><<<grid, 256, 0, s>>>(M, N, K, a, lda, b, ldb, out, ldo, zs, kchunk);
+  if (splits > 1) k_splitk_sum<<<grid_for(size_t(M) * N), 256, 0, s>>>(partial, splits, M, N, C, ldc);
+  return launch_status();
+}
+
+int pty_cnn_im2col(const float* X, int batch, int H, int W, int C, int ks, int stride, int pad, void* cols,
+                   void* stream) {
+  if (batch <= 0 || H <= 0 || W <= 0 || C <= 0 || ks <= 0 || stride <= 0 || pad < 0) return PTY_ERR_ARG;
+  const int Ho = (H + 2 * pad - ks) / stride + 1, Wo = (W + 2 * pad - ks) / stride + 1;
+  if (Ho <= 0 || Wo <= 0) return PTY_ERR_ARG;
+  const size_t total = size_t(batch) * Ho * Wo * ks * ks * C;
+  k_im2col<<<grid_for(total), 256, 0, static_cast<hipStream_t>(stream)>>>(X, batch, H, W, C, ks, stride, pad, Ho, Wo,
+                                                                          static_cast<uint16_t*>(cols));
+  return launch_status();
+}
+
+int pty_cnn_col2im(const float* dcols, int batch, int H, int W, int C, int ks, int stride, int pad, float* dX,
+                   void* stream) {
+  if (batch <= 0 || H <= 0 || W <= 0 || C <= 0 || ks <= 0 || stride <= 0 || pad < 0) return PTY_ERR_ARG;
+  const int Ho = (H + 2 * pad - ks) / stride + 1, Wo = (W + 2 * pad - ks) / stride + 1;
+  if (Ho <= 0 || Wo <= 0) return PTY_ERR_ARG;
+  const size_t total = size_t(batch) * H * W * C;
+  k_col2im<<<grid_for(total), 256, 0, static_cast<hipStream_t>(stream)>>>(dcols, batch, H, W, C, ks, stride, pad, Ho,
+                                                                          Wo, dX);
+  return launch_status();
+}
+
+int pty_cnn_bias_act(const float* Y, int64_t P, int C, const float* bias, int relu, float* A, void* Abf,
+                     void* stream) {
+  if (P <= 0 || C <= 0 || !Y || !bias) return PTY_ERR_ARG;
+  k_bias_act<<<grid_for(size_t(P) * C), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      Y, size_t(P), C, bias, relu, A, static_cast<uint16_t*>(Abf));
+  return launch_status();
+}
+
+int pty_cnn_act_grad(const float* dA, const float* A, int64_t n, int relu, float* dY, void* dYbf, void* stream) {
+  if (n <= 0 || !dA || !dY || !dYbf || (relu && !A)) return PTY_ERR_ARG;
+  k_act_grad<<<grid_for(size_t(n)), 256, 0, static_cast<hipStream_t>(stream)>>>(dA, A, size_t(n), relu, dY,
+                                                                                static_cast<uint16_t*>(dYbf));
+  return launch_status();
+}
+
+int pty_cnn_bias_grad(const float* dY, int64_t P, int C, float* db, void* stream) {
+  if (P <= 0 || C <= 0 || !dY || !db) return PTY_ERR_ARG;
+  k_bias_grad<<<C, 256, 0, static_cast<hipStream_t>(stream)>>>(dY, size_t(P), C, db);
+  return launch_status();
+}
+
+int pty_cnn_to_bf16(const float* x, int64_t n, void* y, void* stream) {
+  if (n <= 0 || !x || !y) return PTY_ERR_ARG;
+  k_to_bf16<<<grid_for(size_t(n)), 256, 0, static_cast<hipStream_t>(stream)>>>(x, size_t(n),
+                                                                               static_cast<uint16_t*>(y));
+  return launch_status();
+}
+
+int pty_cnn_mse(const float* Y, const float* T, int64_t n, float* dY, double* loss, void* stream) {
+  if (n <= 0 || !Y || !T || !dY || !loss) return PTY_ERR_ARG;
+  k_mse<<<1, 256, 0, static_cast<hipStream_t>(stream)>>>(Y, T, size_t(n), dY, loss);
+  return launch_status();
+}
+
+int pty_cnn_adam(float* w, void* wbf, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+                 float beta2, float eps, int step, void* stream) {
+  if (n <= 0 || !w || !g || !m || !v || step < 1) return PTY_ERR_ARG;
+  const float c1 = 1.f - powf(beta1, float(step)), c2 = 1.f - powf(beta2, float(step));
+  k_adam<<<grid_for(size_t(n)), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      w, static_cast<uint16_t*>(wbf), g, m, v, size_t(n), lr, beta1, beta2, eps, c1, c2);
+  return launch_status();
+}
+
+}  // extern "C"

@@ -86,7 +86,8 @@ def random_atom_object(ny, nx, n_slices, pixel_A=0.2, dz_A=2.0, density=0.1, sig
 
 def make_probe(n, pixel_A, acc_voltage_kv=300.0, aperture_mrad=20.0, defocus_A=100.0, n_modes=1,
                defocus_spread_A=20.0, seed=1101):
-    """Aperture-limited, defocused probe (n, n, M) complex64.
+    """Aperture-limited, defocused probe (n, n, M) complex64, centred in the window
+    (fftshift of the inverse transform, as pypty/initialize.py:977 builds its probe).
 
     Mode m carries an extra seeded defocus offset ~ N(0, defocus_spread_A)
     (mode 0 none) and a weight 1/(m+1).  Normalised so that sum |probe|^2 = 1 over all modes.
@@ -101,7 +102,7 @@ def make_probe(n, pixel_A, acc_voltage_kv=300.0, aperture_mrad=20.0, defocus_A=1
     for m in range(n_modes):
         df = defocus_A + (rng.normal(0.0, defocus_spread_A) if m > 0 else 0.0)
         spec = ap * np.exp(-1j * np.pi * df * lam * q2) / (m + 1)
-        probe[:, :, m] = np.fft.ifft2(spec)
+        probe[:, :, m] = np.fft.fftshift(np.fft.ifft2(spec))
     probe /= np.sqrt(np.sum(np.abs(probe) ** 2))
     return probe.astype(np.complex64)
 
