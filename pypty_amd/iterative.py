@@ -363,7 +363,9 @@ def save_params(path, params, strip_dataset=True):
     def conv(v):  # lambdas inside [value, lambda] step lists too (the reference converts top-level ones)
         return type(v)(conv(x) for x in v) if isinstance(v, (list, tuple)) else lambda_source(v)
 
-    out = {k: conv(v) for k, v in params.items() if not (strip_dataset and k == "dataset")}
+    # epoch_callback is this driver's own hook, not a reconstruction parameter
+    out = {k: conv(v) for k, v in params.items()
+           if not (strip_dataset and k == "dataset") and k != "epoch_callback"}
     if os.path.exists(path):
         os.remove(path)
     with open(path, "wb") as fh:
@@ -399,8 +401,10 @@ def run(pypty_params):
     wolfe_c1_constant, wolfe_c2_constant, max_count, reduce_factor, optimism,
     min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
-    save_loss_log, print_flag, and checkpoint_slices (not in the reference:
-    keep every k-th slice's wave and recompute the rest in the adjoint).
+    save_loss_log, print_flag, and two keys not in the reference:
+    checkpoint_slices (keep every k-th slice's wave and recompute the rest in
+    the adjoint) and epoch_callback (called as f(epoch, obj, probe) with the
+    device arrays after every epoch, e.g. to track an error metric).
     Keys selecting features outside the path raise
     NotImplementedError through bfgs_update / loss_and_direction.  Writes
     co.npy, cp.npy, cg.npy and loss.csv like the reference; returns the final
@@ -479,6 +483,7 @@ def run(pypty_params):
     losses = []
     is_first = True
     actual_step_default = get("update_step_bfgs", 1)
+    epoch_callback = get("epoch_callback", None)
     for epoch in range(int(get("epoch_prev", 0)), int(get("epoch_max", 200))):
         warnings = ""
         s_obj = _per_epoch(get("update_obj", 1), epoch)
@@ -550,6 +555,8 @@ def run(pypty_params):
                                "N linesearch iterations": count + 1, "dir. derivative": d_value,
                                "new dir. derivative": new_d_value, "Constraints contribution": 0.0,
                                "Free GiB": free / 1024 ** 3, "Total GiB": total / 1024 ** 3, "Warnings": warnings})
+        if epoch_callback is not None:
+            epoch_callback(epoch, obj, probe)
     _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
     result = {"obj": obj, "probe": probe, "positions": positions.double() + positions_correction.double(),
               "losses": losses, "time_s": time.time() - t0}

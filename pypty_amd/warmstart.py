@@ -172,13 +172,16 @@ class WarmStartNet:
                                              self.step_count, s), "adam")
 
     # ------------------------------------------------------------------ use
-    def train(self, x_all: torch.Tensor, t_all: torch.Tensor, epochs=5, batch=256, lr=1e-3, seed=0, log=None):
-        """Adam on shuffled minibatches (seeded order); returns the mean loss of each epoch."""
+    def train(self, x_all: torch.Tensor, t_all: torch.Tensor, epochs=5, batch=256, lr=1e-3, seed=0, log=None,
+              lr_end=None):
+        """Adam on shuffled minibatches (seeded order), the learning rate decaying geometrically from
+        lr to lr_end (default: constant) over the epochs; returns the mean loss of each epoch."""
         g = torch.Generator()
         g.manual_seed(int(seed))
         n = x_all.shape[0]
         history = []
         for ep in range(epochs):
+            lr_ep = lr if lr_end is None or epochs < 2 else lr * (lr_end / lr) ** (ep / (epochs - 1))
             order = torch.randperm(n, generator=g).to(self.dev)
             tot = torch.zeros((), dtype=torch.float64, device=self.dev)
             steps = 0
@@ -186,7 +189,7 @@ class WarmStartNet:
                 idx = order[i:i + batch]
                 self.forward(x_all.index_select(0, idx))
                 tot += self.backward(t_all.index_select(0, idx))[0]
-                self.adam(lr=lr)
+                self.adam(lr=lr_ep)
                 steps += 1
             history.append(float(tot) / max(steps, 1))
             if log:
@@ -240,10 +243,11 @@ def simulate_pairs(n_pairs, n=128, patch=32, n_slices=10, pixel_A=0.2, dz_A=2.0,
     return torch.cat(xs), torch.cat(ts)
 
 
-def warm_start_object(net: WarmStartNet, measured, corners, obj_shape):
+def warm_start_object(net: WarmStartNet, measured, corners, obj_shape, sigma_px=None):
     """Initial object (Ny, Nx, S, K) complex64 from the network's phase patches: overlapping
-    patches are averaged, the projected phase is spread evenly over the slices, and pixels that no
-    patch covers keep the flat start (phase 0)."""
+    patches are averaged with a Gaussian weight around each patch centre (sigma_px, default an
+    eighth of the patch; the network is surest near the probe), the projected phase is spread
+    evenly over the slices, and pixels that no patch covers keep the flat start (phase 0)."""
     ny, nx, S, K = obj_shape
     n, patch = net.n_in, net.patch
     meas = measured if isinstance(measured, torch.Tensor) else torch.from_numpy(np.asarray(measured, np.float32))
@@ -252,11 +256,14 @@ def warm_start_object(net: WarmStartNet, measured, corners, obj_shape):
     cnt = np.zeros((ny, nx), np.float64)
     c0 = (n - patch) // 2
     iy, ix = np.meshgrid(np.arange(patch), np.arange(patch), indexing="ij")
+    sig = patch / 8.0 if sigma_px is None else float(sigma_px)
+    r2 = (iy - (patch - 1) / 2.0) ** 2 + (ix - (patch - 1) / 2.0) ** 2
+    w = np.exp(-r2 / (2 * sig * sig))
     for (y0, x0), p in zip(np.asarray(corners), pred):
         ys, xs = y0 + c0 + iy, x0 + c0 + ix
         ok = (ys >= 0) & (ys < ny) & (xs >= 0) & (xs < nx)
-        np.add.at(acc, (ys[ok], xs[ok]), p[ok])
-        np.add.at(cnt, (ys[ok], xs[ok]), 1.0)
-    phase = np.where(cnt > 0, acc / np.maximum(cnt, 1.0), 0.0)
+        np.add.at(acc, (ys[ok], xs[ok]), (w * p)[ok])
+        np.add.at(cnt, (ys[ok], xs[ok]), w[ok])
+    phase = np.where(cnt > 1e-6, acc / np.maximum(cnt, 1e-6), 0.0)
     obj = np.exp(1j * phase / (S * K))[:, :, None, None] * np.ones((1, 1, S, K))
     return obj.astype(np.complex64)
