@@ -125,7 +125,7 @@ def warm_start_benchmark(dev, pairs, train_epochs, rec_epochs, step0=1e-3):
     t_sim = time.perf_counter() - t0
     net = warmstart.WarmStartNet(n, patch, dev, seed=0)
     t0 = time.perf_counter()
-    hist = net.train(x, t, epochs=train_epochs, batch=batch, lr=1e-3, seed=1)
+    hist = net.train(x, t, epochs=train_epochs, batch=batch, lr=1e-3, seed=1, lr_end=1e-4)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     steps = train_epochs * (pairs // batch)
@@ -174,6 +174,14 @@ def warm_start_benchmark(dev, pairs, train_epochs, rec_epochs, step0=1e-3):
     target = plain[-1][1]
     t_warm = next((tt for tt, e in warm if e <= target), None)
     mask0 = metrics.fov_mask(truth.shape, pos, n)
+    # the same comparison with the plain run stopped earlier: target = its NMSE after k epochs
+    by_epochs = []
+    for k in range(5, rec_epochs + 1, 5):
+        tk, ek = plain[k - 1]
+        tw = next((tt for tt, e in warm if e <= ek), None)
+        by_epochs.append({"plain_epochs": k, "target_nmse": round(ek, 4), "plain_s": round(tk, 3),
+                          "warm_s": None if tw is None else round(tw, 3),
+                          "warm_over_plain": None if tw is None else round(tw / tk, 3)})
     return {"workload": "multislice object 512x512x10, probe 128px x5 modes, 64x64 scan, lsq_sqrt, object "
                         f"updates, first L-BFGS step {step0}",
             "network": "4 conv 4x4/2 (1-16-32-32-64) + dense 4096-1024 + dense 1024-1024, BF16 MFMA GEMMs",
@@ -185,7 +193,7 @@ def warm_start_benchmark(dev, pairs, train_epochs, rec_epochs, step0=1e-3):
             "start_nmse_warm": round(metrics.nmse(metrics.projected_phase(warm_obj), truth, mask0), 4),
             "warm_build_s": round(t_build, 3), "warm_s_to_target": None if t_warm is None else round(t_warm, 3),
             "warm_over_plain": None if t_warm is None else round(t_warm / plain[-1][0], 3),
-            "warm_final_nmse": round(warm[-1][1], 4)}
+            "warm_final_nmse": round(warm[-1][1], 4), "by_plain_epochs": by_epochs}
 
 
 def recon_benchmark(dev, epochs, cpu_epochs):
@@ -340,8 +348,8 @@ def main(argv=None):
                     help="capture one step in a HIP graph and replay it (default: on for ragged)")
     ap.add_argument("--warm-start-pairs", type=int, default=50000,
                     help="training pairs of the learned warm-start measurement (multislice workload; 0: skip)")
-    ap.add_argument("--warm-start-epochs", type=int, default=20)
-    ap.add_argument("--warm-recon-epochs", type=int, default=15)
+    ap.add_argument("--warm-start-epochs", type=int, default=40)
+    ap.add_argument("--warm-recon-epochs", type=int, default=20)
     ap.add_argument("--checkpoint", type=int, default=None,
                     help="keep every k-th slice's wave and recompute the rest (default: 8 for thick, else 0)")
     args = ap.parse_args(argv)

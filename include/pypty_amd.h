@@ -250,11 +250,15 @@ int pty_cnn_bias_act(const float* Y, int64_t P, int C, const float* bias, int re
                      void* stream);
 /* dY = dA * [A > 0] (relu) or dA, as FP32 and BF16 */
 int pty_cnn_act_grad(const float* dA, const float* A, int64_t n, int relu, float* dY, void* dYbf, void* stream);
-/* db[c] = sum over p of dY[p][c] */
-int pty_cnn_bias_grad(const float* dY, int64_t P, int C, float* db, void* stream);
+/* db[c] = sum over p of dY[p][c] in a fixed order; partial: device scratch of
+ * partial_elems floats (C per block of rows, up to 2048 blocks) */
+int pty_cnn_bias_grad(const float* dY, int64_t P, int C, float* db, float* partial, size_t partial_elems,
+                      void* stream);
 int pty_cnn_to_bf16(const float* x, int64_t n, void* y, void* stream);
-/* loss[0] = mean (Y - T)^2 (FP64), dY = 2 (Y - T) / n */
-int pty_cnn_mse(const float* Y, const float* T, int64_t n, float* dY, double* loss, void* stream);
+/* loss[0] = mean (Y - T)^2 (FP64, fixed order), dY = 2 (Y - T) / n; partial:
+ * device scratch of partial_elems doubles (up to 1024 used) */
+int pty_cnn_mse(const float* Y, const float* T, int64_t n, float* dY, double* loss, double* partial,
+                size_t partial_elems, void* stream);
 /* one bias-corrected Adam step on FP32 weights w (BF16 copy wbf optional) */
 int pty_cnn_adam(float* w, void* wbf, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
                  float beta2, float eps, int step, void* stream);

@@ -121,9 +121,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
             ("pty_cnn_col2im", [vp, i, i, i, i, i, i, i, vp, vp]),
             ("pty_cnn_bias_act", [vp, i64, i, vp, i, vp, vp, vp]),
             ("pty_cnn_act_grad", [vp, vp, i64, i, vp, vp, vp]),
-            ("pty_cnn_bias_grad", [vp, i64, i, vp, vp]),
+            ("pty_cnn_bias_grad", [vp, i64, i, vp, vp, sz, vp]),
             ("pty_cnn_to_bf16", [vp, i64, vp, vp]),
-            ("pty_cnn_mse", [vp, vp, i64, vp, vp, vp]),
+            ("pty_cnn_mse", [vp, vp, i64, vp, vp, vp, sz, vp]),
             ("pty_cnn_adam", [vp, vp, vp, vp, vp, i64, f, f, f, f, i, vp])):
         fn = getattr(lib, name)
         fn.restype = i

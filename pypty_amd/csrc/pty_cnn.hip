@@ -56,14 +56,29 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  // the four k values of a lane are contiguous in memory for !TA (A) and TB
+  // (B): one 8-byte load when they are all in range and aligned
+  const bool avec = !TA && (lda % 4 == 0), bvec = TB && (ldb % 4 == 0);
   for (int k = k_lo; k < k_hi; k += 8) {
     bf16x4 a, b;
+    const int kb = k + kq;
+    if (avec && kb + 3 < k_hi && m < M) {
+      a = *reinterpret_cast<const bf16x4*>(A + size_t(m) * lda + kb);
+    } else {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int kk = k + kq + v;
-      const bool kin = kk < k_hi;
-      a[v] = (kin && m < M) ? short(TA ? A[size_t(kk) * lda + m] : A[size_t(m) * lda + kk]) : short(0);
-      b[v] = (kin && n < N) ? short(TB ? B[size_t(n) * ldb + kk] : B[size_t(kk) * ldb + n]) : short(0);
+      for (int v = 0; v < 4; ++v) {
+        const int kk = kb + v;
+        a[v] = (kk < k_hi && m < M) ? short(TA ? A[size_t(kk) * lda + m] : A[size_t(m) * lda + kk]) : short(0);
+      }
+    }
+    if (bvec && kb + 3 < k_hi && n < N) {
+      b = *reinterpret_cast<const bf16x4*>(B + size_t(n) * ldb + kb);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int kk = kb + v;
+        b[v] = (kk < k_hi && n < N) ? short(TB ? B[size_t(n) * ldb + kk] : B[size_t(kk) * ldb + n]) : short(0);
+      }
     }
     acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
   }
@@ -86,42 +101,44 @@ __global__ void k_splitk_sum(const float* __restrict__ part, int splits, int M, 
 }
 
 // cols[p][(ky*ks + kx)*C + c] = X[b][oy*st - pad + ky][ox*st - pad + kx][c] (0 outside),
-// p = (b*Ho + oy)*Wo + ox
+// p = (b*Ho + oy)*Wo + ox; one thread per (p, tap) copies the tap's C channels
 __global__ void k_im2col(const float* __restrict__ X, int Bn, int H, int W, int C, int ks, int st, int pad, int Ho,
                          int Wo, uint16_t* __restrict__ cols) {
-  const int Kc = ks * ks * C;
-  const size_t total = size_t(Bn) * Ho * Wo * Kc;
+  const int taps = ks * ks, Kc = taps * C;
+  const size_t total = size_t(Bn) * Ho * Wo * taps;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
-    const size_t p = i / Kc;
-    const int q = int(i % Kc);
-    const int c = q % C, t = q / C, kx = t % ks, ky = t / ks;
+    const size_t p = i / taps;
+    const int tap = int(i % taps), kx = tap % ks, ky = tap / ks;
     const int ox = int(p % Wo), oy = int((p / Wo) % Ho), b = int(p / (size_t(Wo) * Ho));
     const int iy = oy * st - pad + ky, ix = ox * st - pad + kx;
-    const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
-    cols[i] = in ? to_bf16(X[((size_t(b) * H + iy) * W + ix) * C + c]) : uint16_t(0);
+    uint16_t* dst = cols + p * Kc + tap * C;
+    if (iy >= 0 && iy < H && ix >= 0 && ix < W) {
+      const float* src = X + ((size_t(b) * H + iy) * W + ix) * C;
+      for (int c = 0; c < C; ++c) dst[c] = to_bf16(src[c]);
+    } else {
+      for (int c = 0; c < C; ++c) dst[c] = 0;
+    }
   }
 }
 
 // dX[b][iy][ix][c] = sum over the taps (ky, kx) that read (iy, ix) of
-// dcols[(b, oy, ox)][(ky*ks + kx)*C + c], taps in (ky, kx) order
+// dcols[(b, oy, ox)][(ky*ks + kx)*C + c], taps in (ky, kx) order; only the
+// taps with ky = (iy + pad) mod st + j*st (likewise kx) can read a pixel
 __global__ void k_col2im(const float* __restrict__ dcols, int Bn, int H, int W, int C, int ks, int st, int pad, int Ho,
                          int Wo, float* __restrict__ dX) {
   const int Kc = ks * ks * C;
   const size_t total = size_t(Bn) * H * W * C;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
     const int c = int(i % C);
-    const int ix = int((i / C) % W), iy = int((i / (size_t(C) * W)) % H), b = int(i / (size_t(C) * W * H));
+    const size_t pix = i / C;
+    const int ix = int(pix % W), iy = int((pix / W) % H), b = int(pix / (size_t(W) * H));
     float s = 0.f;
-    for (int ky = 0; ky < ks; ++ky) {
-      const int ty = iy + pad - ky;
-      if (ty < 0 || ty % st) continue;
-      const int oy = ty / st;
-      if (oy >= Ho) continue;
-      for (int kx = 0; kx < ks; ++kx) {
-        const int tx = ix + pad - kx;
-        if (tx < 0 || tx % st) continue;
-        const int ox = tx / st;
-        if (ox >= Wo) continue;
+    for (int ky = (iy + pad) % st; ky < ks; ky += st) {
+      const int oy = (iy + pad - ky) / st;
+      if (oy < 0 || oy >= Ho) continue;
+      for (int kx = (ix + pad) % st; kx < ks; kx += st) {
+        const int ox = (ix + pad - kx) / st;
+        if (ox < 0 || ox >= Wo) continue;
         s += dcols[((size_t(b) * Ho + oy) * Wo + ox) * Kc + (ky * ks + kx) * C + c];
       }
     }
@@ -152,21 +169,47 @@ __global__ void k_act_grad(const float* __restrict__ dA, const float* __restrict
   }
 }
 
-// db[c] = sum over p of dY[p][c]: one workgroup per channel, thread t sums
-// p = t, t + 256, ... in order, then a fixed tree
-__global__ __launch_bounds__(256) void k_bias_grad(const float* __restrict__ dY, size_t P, int C,
-                                                   float* __restrict__ db) {
+// db[c] = sum over p of dY[p][c] in two fixed-order stages.  Stage 1: block g
+// sums rows [g*R, (g+1)*R); thread t owns column t % TC (and every TC-th after
+// it when C > 256) and rows t / TC + k * (256 / TC), so each pass reads 256
+// consecutive floats; row groups are then added in order through LDS.
+// Stage 2: part[g][c] summed over g in order.
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ dY, size_t P, int C, size_t R,
+                                                     float* __restrict__ part) {
+  __shared__ float red[256];
+  const int t = threadIdx.x;
+  const int TC = C < 256 ? C : 256, groups = 256 / TC;
+  const int c0 = t % TC, rg = t / TC;
+  const size_t r0 = size_t(blockIdx.x) * R, r1 = r0 + R < P ? r0 + R : P;
+  for (int cb = 0; cb < C; cb += TC) {
+    const int c = cb + c0;
+    float s = 0.f;
+    if (rg < groups && c < C)
+      for (size_t r = r0 + rg; r < r1; r += groups) s += dY[r * C + c];
+    red[t] = s;
+    __syncthreads();
+    if (t < TC) {
+      float acc = 0.f;
+      for (int g = 0; g < groups; ++g) acc += red[g * TC + t];
+      if (cb + t < C) part[size_t(blockIdx.x) * C + cb + t] = acc;
+    }
+    __syncthreads();
+  }
+}
+// one workgroup per column: thread t sums g = t, t + 256, ... in order, then a fixed tree
+__global__ __launch_bounds__(256) void k_colsum_final(const float* __restrict__ part, int G, int C,
+                                                      float* __restrict__ out) {
   __shared__ float red[256];
   const int c = blockIdx.x;
   float s = 0.f;
-  for (size_t p = threadIdx.x; p < P; p += 256) s += dY[p * C + c];
+  for (int g = threadIdx.x; g < G; g += 256) s += part[size_t(g) * C + c];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) db[c] = red[0];
+  if (threadIdx.x == 0) out[c] = red[0];
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
@@ -174,14 +217,15 @@ __global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __res
     y[i] = to_bf16(x[i]);
 }
 
-// mean squared error over n outputs: loss[0] = sum (y - t)^2 / n (FP64, one
-// workgroup, fixed order), dY = 2 (y - t) / n
-__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const float* __restrict__ T, size_t n,
-                                             float* __restrict__ dY, double* __restrict__ loss) {
+// mean squared error over n outputs, dY = 2 (y - t) / n; FP64 partial sums
+// of (y - t)^2 per block (fixed order: strided per thread, then a tree),
+// loss[0] = their ordered sum / n
+__global__ __launch_bounds__(256) void k_mse_part(const float* __restrict__ Y, const float* __restrict__ T, size_t n,
+                                                  float* __restrict__ dY, double* __restrict__ part) {
   __shared__ double red[256];
   double s = 0.0;
   const float sc = 2.0f / float(n);
-  for (size_t i = threadIdx.x; i < n; i += 256) {
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
     const float d = Y[i] - T[i];
     dY[i] = sc * d;
     s += double(d) * double(d);
@@ -192,7 +236,13 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const 
     if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss[0] = red[0] / double(n);
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void k_mse_final(const double* __restrict__ part, int G, size_t n, double* __restrict__ loss) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int g = 0; g < G; ++g) s += part[g];
+  loss[0] = s / double(n);
 }
 
 // Adam (Kingma & Ba 2015, bias-corrected) on FP32 masters; writes the BF16 copy
@@ -253,7 +303,7 @@ int pty_cnn_im2col(const float* X, int batch, int H, int W, int C, int ks, int s
   if (batch <= 0 || H <= 0 || W <= 0 || C <= 0 || ks <= 0 || stride <= 0 || pad < 0) return PTY_ERR_ARG;
   const int Ho = (H + 2 * pad - ks) / stride + 1, Wo = (W + 2 * pad - ks) / stride + 1;
   if (Ho <= 0 || Wo <= 0) return PTY_ERR_ARG;
-  const size_t total = size_t(batch) * Ho * Wo * ks * ks * C;
+  const size_t total = size_t(batch) * Ho * Wo * ks * ks;
   k_im2col<<<grid_for(total), 256, 0, static_cast<hipStream_t>(stream)>>>(X, batch, H, W, C, ks, stride, pad, Ho, Wo,
                                                                           static_cast<uint16_t*>(cols));
   return launch_status();
@@ -285,9 +335,19 @@ int pty_cnn_act_grad(const float* dA, const float* A, int64_t n, int relu, float
   return launch_status();
 }
 
-int pty_cnn_bias_grad(const float* dY, int64_t P, int C, float* db, void* stream) {
-  if (P <= 0 || C <= 0 || !dY || !db) return PTY_ERR_ARG;
-  k_bias_grad<<<C, 256, 0, static_cast<hipStream_t>(stream)>>>(dY, size_t(P), C, db);
+int pty_cnn_bias_grad(const float* dY, int64_t P, int C, float* db, float* partial, size_t partial_elems,
+                      void* stream) {
+  if (P <= 0 || C <= 0 || !dY || !db || !partial) return PTY_ERR_ARG;
+  // about 64 rows per thread-row-group, at most as many blocks as the partials hold
+  const int groups = 256 / (C < 256 ? C : 256);
+  size_t G = std::min<size_t>((size_t(P) + size_t(groups) * 64 - 1) / (size_t(groups) * 64), 2048);
+  G = std::min<size_t>(G, partial_elems / size_t(C));
+  if (G < 1) return PTY_ERR_WORKSPACE;
+  const size_t R = (size_t(P) + G - 1) / G;
+  G = (size_t(P) + R - 1) / R;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  k_colsum_part<<<int(G), 256, 0, s>>>(dY, size_t(P), C, R, partial);
+  k_colsum_final<<<C, 256, 0, s>>>(partial, int(G), C, db);
   return launch_status();
 }
 
@@ -298,9 +358,13 @@ int pty_cnn_to_bf16(const float* x, int64_t n, void* y, void* stream) {
   return launch_status();
 }
 
-int pty_cnn_mse(const float* Y, const float* T, int64_t n, float* dY, double* loss, void* stream) {
-  if (n <= 0 || !Y || !T || !dY || !loss) return PTY_ERR_ARG;
-  k_mse<<<1, 256, 0, static_cast<hipStream_t>(stream)>>>(Y, T, size_t(n), dY, loss);
+int pty_cnn_mse(const float* Y, const float* T, int64_t n, float* dY, double* loss, double* partial,
+                size_t partial_elems, void* stream) {
+  if (n <= 0 || !Y || !T || !dY || !loss || !partial || partial_elems < 1) return PTY_ERR_ARG;
+  const int G = int(std::min<size_t>({(size_t(n) + 4095) / 4096, partial_elems, 1024}));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  k_mse_part<<<G, 256, 0, s>>>(Y, T, size_t(n), dY, partial);
+  k_mse_final<<<1, 64, 0, s>>>(partial, G, size_t(n), loss);
   return launch_status();
 }
 

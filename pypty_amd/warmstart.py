@@ -108,6 +108,7 @@ class WarmStartNet:
         # split-reduction partials of the long, narrow GEMMs (weight gradients)
         self.partial = torch.empty(1 << 24, dtype=f32, device=self.dev)
         self.loss = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        self.loss_part = torch.empty(1024, dtype=torch.float64, device=self.dev)
         self._buf_B = B
 
     # ----------------------------------------------------------- forward/backward
@@ -144,14 +145,15 @@ class WarmStartNet:
         s = self._s()
         out = self.A[-1]
         _lib.check(self.lib.pty_cnn_mse(_p(out), _p(target.contiguous()), out.numel(), _p(self.dA[-1]),
-                                        _p(self.loss), s), "mse")
+                                        _p(self.loss), _p(self.loss_part), self.loss_part.numel(), s), "mse")
         for li in range(len(self.spec) - 1, -1, -1):
             kind, cin, cout, relu, hw = self.spec[li]
             P = self.Y[li].shape[0]
             Kc = self.W[li].shape[1]
             _lib.check(self.lib.pty_cnn_act_grad(_p(self.dA[li]), _p(self.A[li]), P * cout, int(relu),
                                                  _p(self.dY[li]), _p(self.dYbf[li]), s), "act_grad")
-            _lib.check(self.lib.pty_cnn_bias_grad(_p(self.dY[li]), P, cout, _p(self.gb[li]), s), "bias_grad")
+            _lib.check(self.lib.pty_cnn_bias_grad(_p(self.dY[li]), P, cout, _p(self.gb[li]), _p(self.partial),
+                                                  self.partial.numel(), s), "bias_grad")
             self._gemm(1, 0, cout, Kc, P, self.dYbf[li], cout, self._input_cols(li), Kc, self.gW[li], Kc)
             if li > 0:
                 self._gemm(0, 0, P, Kc, cout, self.dYbf[li], cout, self.Wbf[li], Kc, self.dcols[li], Kc)
