@@ -264,12 +264,16 @@ def recon_benchmark(dev, epochs, cpu_epochs):
             step = r[2]
             cpu_curve.append((t_cpu, metrics.nmse(metrics.projected_phase(state["obj"]), truth, mask_c)))
         # the two runs take slightly different paths; both are timed to the looser of their final NMSEs
+        # (from the flat start they can need different numbers of epochs, so both are reported)
         target = max(nmse_gpu, cpu_curve[-1][1])
-        tg = next(t for t, e in gpu_curve if e <= target)
-        tc = next(t for t, e in cpu_curve if e <= target)
+        ig = next(i for i, (t, e) in enumerate(gpu_curve) if e <= target)
+        ic = next(i for i, (t, e) in enumerate(cpu_curve) if e <= target)
+        tg, tc = gpu_curve[ig][0], cpu_curve[ic][0]
         out.update({"cpu_epochs": cpu_epochs, "cpu_s": round(t_cpu, 3), "cpu_nmse": round(cpu_curve[-1][1], 5),
                     "cpu_cores": 1, "target_nmse": round(target, 5), "gpu_s_to_target": round(tg, 4),
-                    "cpu_s_to_target": round(tc, 3), "speedup_to_target": round(tc / tg, 1)})
+                    "gpu_epochs_to_target": ig + 1, "cpu_s_to_target": round(tc, 3), "cpu_epochs_to_target": ic + 1,
+                    "speedup_to_target": round(tc / tg, 1), "speedup_per_epoch": round((t_cpu / cpu_epochs) /
+                                                                                       (t_gpu / epochs), 1)})
     return out
 
 
