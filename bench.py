@@ -354,6 +354,8 @@ def main(argv=None):
                     help="training pairs of the learned warm-start measurement (multislice workload; 0: skip)")
     ap.add_argument("--warm-start-epochs", type=int, default=40)
     ap.add_argument("--warm-recon-epochs", type=int, default=20)
+    ap.add_argument("--streams", type=int, default=None,
+                    help="streams the datasets of a step are spread over (default: 2 for ragged, else 1)")
     ap.add_argument("--checkpoint", type=int, default=None,
                     help="keep every k-th slice's wave and recompute the rest (default: 8 for thick, else 0)")
     args = ap.parse_args(argv)
@@ -397,10 +399,24 @@ def main(argv=None):
             j.set_batch(best)
             tuned.append(best)
 
+    # datasets on several default-priority streams (each has its own workspace), joined at the end
+    n_streams = max(1, min(len(jobs), (2 if ragged else 1) if args.streams is None else args.streams))
+    lanes = [torch.cuda.Stream(dev) for _ in range(n_streams)] if n_streams > 1 else []
+
     def step():
         out = None
-        for j in jobs:
-            out = j.step(sync=False)
+        if not lanes:
+            for j in jobs:
+                out = j.step(sync=False)
+            return out
+        main = torch.cuda.current_stream(dev)
+        for ln in lanes:
+            ln.wait_stream(main)
+        for i, j in enumerate(jobs):
+            with torch.cuda.stream(lanes[i % n_streams]):
+                out = j.step(sync=False)
+        for ln in lanes:
+            main.wait_stream(ln)
         return out
 
     for _ in range(args.warmup):
@@ -490,6 +506,7 @@ def main(argv=None):
     bytes_step = sum(pass_bytes(j.shape[0], j.shape[1], j.shape[3]) * j.P for j in jobs)
     cfg_info = {"workload": WORKLOADS[args.config], "positions": P_total,
                 "compute_batch": tuned if tuned else args.batch, "hip_graph": bool(use_graph),
+                "streams": n_streams,
                 "checkpoint_slices": ck,
                 "datasets": [{"n": j.shape[0], "slices": j.shape[1], "probe_modes": j.shape[2],
                               "obj_modes": j.shape[3], "positions": j.P} for j in jobs]}
