@@ -239,10 +239,12 @@ void pty_timing_reset(void);
 int pty_gemm_bf16(int ta, int tb, int M, int N, int K, const void* A, int lda, const void* B, int ldb, float* C,
                   int ldc, float* partial, size_t partial_elems, void* stream);
 /* cols[p][(ky*ks + kx)*C + c] (BF16) = X[b][oy*stride - pad + ky][ox*stride - pad + kx][c]
- * (0 outside) for X (batch, H, W, C) FP32 and p = (b*Ho + oy)*Wo + ox. */
+ * (0 outside) for X (batch, H, W, C) FP32 and p = (b*Ho + oy)*Wo + ox; ks*ks*C must be a
+ * multiple of 8, X and cols 16-byte aligned. */
 int pty_cnn_im2col(const float* X, int batch, int H, int W, int C, int ks, int stride, int pad, void* cols,
                    void* stream);
-/* the adjoint of im2col: dX (batch, H, W, C) = sum of the dcols entries that read each element */
+/* the adjoint of im2col: dX (batch, H, W, C) = sum of the dcols entries that read each element
+ * (C a multiple of 4, dcols and dX 16-byte aligned) */
 int pty_cnn_col2im(const float* dcols, int batch, int H, int W, int C, int ks, int stride, int pad, float* dX,
                    void* stream);
 /* A[p][c] = act(Y[p][c] + bias[c]), act = ReLU if relu else identity; A (FP32) and Abf (BF16) optional */

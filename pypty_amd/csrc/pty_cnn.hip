@@ -40,52 +40,94 @@ int launch_status() {
 // C[z] (M x N, row stride ldc) = op(A) x op(B) over k in [z*kchunk, (z+1)*kchunk)
 //   op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
 //   op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
-// Workgroup: 4 wavefronts, a 64 x 64 tile of C; each wavefront one 32 x 32
-// block accumulated by v_mfma_f32_32x32x8_bf16.  Operand lanes: lane l holds
+// Workgroup: 4 wavefronts and a 64 x 64 tile of C, K in steps of 32.  Each
+// step stages op(A) and op(B)^T as [row][k] tiles in LDS (one 16-byte global
+// load per thread and operand, row pitch GLP) while the previous step's
+// tiles feed the matrix cores; each wavefront owns a 32 x 32 block and
+// accumulates v_mfma_f32_32x32x8_bf16.  Operand lanes: lane l holds
 // A[m0 + l%32][k + 4*(l/32) + v] and B[k + 4*(l/32) + v][n0 + l%32], v = 0..3;
 // accumulator r of lane l is C[m0 + 8*(r/4) + 4*(l/32) + r%4][n0 + l%32].
+constexpr int GBK = 32, GLP = GBK + 4;  // k per step; LDS row pitch (bf16), 72 bytes
+
+// 8 bf16 of the 64 x 32 tile X[row][k] (rows r0.., k k0..), X[row][k] =
+// TRANS ? X[k*ld + row] : X[row*ld + k], zero outside rows < R and k < k_hi.
+// Thread t: !TRANS row t/4, k 8*(t%4)..+7; TRANS k t/8, rows 8*(t%8)..+7.
+template <bool TRANS>
+__device__ __forceinline__ uint4 tile_load(const uint16_t* __restrict__ X, int ld, int R, int r0, int k0, int k_hi,
+                                           bool vec) {
+  const int t = threadIdx.x;
+  uint4 out;
+  uint16_t* v = reinterpret_cast<uint16_t*>(&out);
+  if (!TRANS) {
+    const int r = r0 + (t >> 2), k = k0 + 8 * (t & 3);
+    if (vec && r < R && k + 7 < k_hi) return *reinterpret_cast<const uint4*>(X + size_t(r) * ld + k);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (r < R && k + i < k_hi) ? X[size_t(r) * ld + k + i] : uint16_t(0);
+  } else {
+    const int k = k0 + (t >> 3), r = r0 + 8 * (t & 7);
+    if (vec && k < k_hi && r + 7 < R) return *reinterpret_cast<const uint4*>(X + size_t(k) * ld + r);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (k < k_hi && r + i < R) ? X[size_t(k) * ld + r + i] : uint16_t(0);
+  }
+  return out;
+}
+// the same 8 values into the LDS tile S[row][k] (pitch GLP)
+template <bool TRANS>
+__device__ __forceinline__ void tile_store(uint16_t* S, uint4 val) {
+  const int t = threadIdx.x;
+  const uint16_t* v = reinterpret_cast<const uint16_t*>(&val);
+  if (!TRANS) {
+    uint2* d = reinterpret_cast<uint2*>(S + (t >> 2) * GLP + 8 * (t & 3));
+    d[0] = make_uint2(val.x, val.y);
+    d[1] = make_uint2(val.z, val.w);
+  } else {
+    const int k = t >> 3, r = 8 * (t & 7);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) S[(r + i) * GLP + k] = v[i];
+  }
+}
+
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
                                                    const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
                                                    int ldc, size_t zstride, int kchunk) {
+  __shared__ __align__(16) uint16_t As[64 * GLP];
+  __shared__ __align__(16) uint16_t Bs[64 * GLP];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m0 = blockIdx.y * 64 + (wave >> 1) * 32, n0 = blockIdx.x * 64 + (wave & 1) * 32;
+  const int bm = blockIdx.y * 64, bn = blockIdx.x * 64;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
   const int li = lane & 31, kq = (lane >> 5) * 4;
-  const int m = m0 + li, n = n0 + li;
+  // 16-byte global loads need 8-element aligned rows and base
+  const bool avec = (lda % 8 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+  const bool bvec = (ldb % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  // the four k values of a lane are contiguous in memory for !TA (A) and TB
-  // (B): one 8-byte load when they are all in range and aligned
-  const bool avec = !TA && (lda % 4 == 0), bvec = TB && (ldb % 4 == 0);
-  for (int k = k_lo; k < k_hi; k += 8) {
-    bf16x4 a, b;
-    const int kb = k + kq;
-    if (avec && kb + 3 < k_hi && m < M) {
-      a = *reinterpret_cast<const bf16x4*>(A + size_t(m) * lda + kb);
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int kk = kb + v;
-        a[v] = (kk < k_hi && m < M) ? short(TA ? A[size_t(kk) * lda + m] : A[size_t(m) * lda + kk]) : short(0);
-      }
+  // op(B)^T[n][k] = TB ? B[n*ldb + k] : B[k*ldb + n]
+  uint4 va = tile_load<TA>(A, lda, M, bm, k_lo, k_hi, avec);
+  uint4 vb = tile_load<!TB>(B, ldb, N, bn, k_lo, k_hi, bvec);
+  for (int k0 = k_lo; k0 < k_hi; k0 += GBK) {
+    __syncthreads();  // the previous step's fragments have been read
+    tile_store<TA>(As, va);
+    tile_store<!TB>(Bs, vb);
+    __syncthreads();
+    if (k0 + GBK < k_hi) {
+      va = tile_load<TA>(A, lda, M, bm, k0 + GBK, k_hi, avec);
+      vb = tile_load<!TB>(B, ldb, N, bn, k0 + GBK, k_hi, bvec);
     }
-    if (bvec && kb + 3 < k_hi && n < N) {
-      b = *reinterpret_cast<const bf16x4*>(B + size_t(n) * ldb + kb);
-    } else {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int kk = kb + v;
-        b[v] = (kk < k_hi && n < N) ? short(TB ? B[size_t(n) * ldb + kk] : B[size_t(kk) * ldb + n]) : short(0);
-      }
+    for (int kk = 0; kk < GBK; kk += 8) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(As + (wm + li) * GLP + kk + kq);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(Bs + (wn + li) * GLP + kk + kq);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
   }
   float* Cz = C + size_t(blockIdx.z) * zstride;
+  const int n = bn + wn + li;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const int row = m0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int row = bm + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (row < M && n < N) Cz[size_t(row) * ldc + n] = acc[r];
   }
 }
@@ -101,48 +143,72 @@ __global__ void k_splitk_sum(const float* __restrict__ part, int splits, int M, 
 }
 
 // cols[p][(ky*ks + kx)*C + c] = X[b][oy*st - pad + ky][ox*st - pad + kx][c] (0 outside),
-// p = (b*Ho + oy)*Wo + ox; one thread per (p, tap) copies the tap's C channels
+// p = (b*Ho + oy)*Wo + ox.  One thread writes 8 consecutive columns of a row
+// with one 16-byte store (Kc % 8 == 0); with C % 8 == 0 they are 8 channels
+// of one tap, read as two 16-byte loads.
+__device__ __forceinline__ float im2col_at(const float* X, int b, int H, int W, int C, int ks, int st, int pad, int oy,
+                                           int ox, int q) {
+  const int c = q % C, tap = q / C, kx = tap % ks, ky = tap / ks;
+  const int iy = oy * st - pad + ky, ix = ox * st - pad + kx;
+  return (iy >= 0 && iy < H && ix >= 0 && ix < W) ? X[((size_t(b) * H + iy) * W + ix) * C + c] : 0.f;
+}
 __global__ void k_im2col(const float* __restrict__ X, int Bn, int H, int W, int C, int ks, int st, int pad, int Ho,
                          int Wo, uint16_t* __restrict__ cols) {
-  const int taps = ks * ks, Kc = taps * C;
-  const size_t total = size_t(Bn) * Ho * Wo * taps;
+  const int Kc = ks * ks * C, per_row = Kc / 8;
+  const size_t total = size_t(Bn) * Ho * Wo * per_row;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
-    const size_t p = i / taps;
-    const int tap = int(i % taps), kx = tap % ks, ky = tap / ks;
+    const size_t p = i / per_row;
+    const int q0 = int(i % per_row) * 8;
     const int ox = int(p % Wo), oy = int((p / Wo) % Ho), b = int(p / (size_t(Wo) * Ho));
-    const int iy = oy * st - pad + ky, ix = ox * st - pad + kx;
-    uint16_t* dst = cols + p * Kc + tap * C;
-    if (iy >= 0 && iy < H && ix >= 0 && ix < W) {
-      const float* src = X + ((size_t(b) * H + iy) * W + ix) * C;
-      for (int c = 0; c < C; ++c) dst[c] = to_bf16(src[c]);
+    float v[8];
+    if (C % 8 == 0) {
+      const int c0 = q0 % C, tap = q0 / C, kx = tap % ks, ky = tap / ks;
+      const int iy = oy * st - pad + ky, ix = ox * st - pad + kx;
+      if (iy >= 0 && iy < H && ix >= 0 && ix < W) {
+        const float4* src = reinterpret_cast<const float4*>(X + ((size_t(b) * H + iy) * W + ix) * C + c0);
+        const float4 a = src[0], c = src[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = 0.f;
+      }
     } else {
-      for (int c = 0; c < C; ++c) dst[c] = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = im2col_at(X, b, H, W, C, ks, st, pad, oy, ox, q0 + j);
     }
+    uint4 out;
+    uint16_t* o = reinterpret_cast<uint16_t*>(&out);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = to_bf16(v[j]);
+    *reinterpret_cast<uint4*>(cols + p * Kc + q0) = out;
   }
 }
 
 // dX[b][iy][ix][c] = sum over the taps (ky, kx) that read (iy, ix) of
 // dcols[(b, oy, ox)][(ky*ks + kx)*C + c], taps in (ky, kx) order; only the
-// taps with ky = (iy + pad) mod st + j*st (likewise kx) can read a pixel
+// taps with ky = (iy + pad) mod st + j*st (likewise kx) can read a pixel.
+// One thread per (pixel, 4 channels) with 16-byte loads and stores (C % 4 == 0).
 __global__ void k_col2im(const float* __restrict__ dcols, int Bn, int H, int W, int C, int ks, int st, int pad, int Ho,
                          int Wo, float* __restrict__ dX) {
-  const int Kc = ks * ks * C;
-  const size_t total = size_t(Bn) * H * W * C;
+  const int Kc = ks * ks * C, C4 = C / 4;
+  const size_t total = size_t(Bn) * H * W * C4;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
-    const int c = int(i % C);
-    const size_t pix = i / C;
+    const int c = int(i % C4) * 4;
+    const size_t pix = i / C4;
     const int ix = int(pix % W), iy = int((pix / W) % H), b = int(pix / (size_t(W) * H));
-    float s = 0.f;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int ky = (iy + pad) % st; ky < ks; ky += st) {
       const int oy = (iy + pad - ky) / st;
       if (oy < 0 || oy >= Ho) continue;
       for (int kx = (ix + pad) % st; kx < ks; kx += st) {
         const int ox = (ix + pad - kx) / st;
         if (ox < 0 || ox >= Wo) continue;
-        s += dcols[((size_t(b) * Ho + oy) * Wo + ox) * Kc + (ky * ks + kx) * C + c];
+        const float4 d =
+            *reinterpret_cast<const float4*>(dcols + ((size_t(b) * Ho + oy) * Wo + ox) * Kc + (ky * ks + kx) * C + c);
+        s.x += d.x; s.y += d.y; s.z += d.z; s.w += d.w;
       }
     }
-    dX[i] = s;
+    *reinterpret_cast<float4*>(dX + pix * C + c) = s;
   }
 }
 
@@ -303,7 +369,8 @@ int pty_cnn_im2col(const float* X, int batch, int H, int W, int C, int ks, int s
   if (batch <= 0 || H <= 0 || W <= 0 || C <= 0 || ks <= 0 || stride <= 0 || pad < 0) return PTY_ERR_ARG;
   const int Ho = (H + 2 * pad - ks) / stride + 1, Wo = (W + 2 * pad - ks) / stride + 1;
   if (Ho <= 0 || Wo <= 0) return PTY_ERR_ARG;
-  const size_t total = size_t(batch) * Ho * Wo * ks * ks;
+  if ((ks * ks * C) % 8 != 0) return PTY_ERR_ARG;  // 8 columns per thread
+  const size_t total = size_t(batch) * Ho * Wo * (ks * ks * C / 8);
   k_im2col<<<grid_for(total), 256, 0, static_cast<hipStream_t>(stream)>>>(X, batch, H, W, C, ks, stride, pad, Ho, Wo,
                                                                           static_cast<uint16_t*>(cols));
   return launch_status();
@@ -314,7 +381,8 @@ int pty_cnn_col2im(const float* dcols, int batch, int H, int W, int C, int ks, i
   if (batch <= 0 || H <= 0 || W <= 0 || C <= 0 || ks <= 0 || stride <= 0 || pad < 0) return PTY_ERR_ARG;
   const int Ho = (H + 2 * pad - ks) / stride + 1, Wo = (W + 2 * pad - ks) / stride + 1;
   if (Ho <= 0 || Wo <= 0) return PTY_ERR_ARG;
-  const size_t total = size_t(batch) * H * W * C;
+  if (C % 4 != 0) return PTY_ERR_ARG;  // 4 channels per thread
+  const size_t total = size_t(batch) * H * W * (C / 4);
   k_col2im<<<grid_for(total), 256, 0, static_cast<hipStream_t>(stream)>>>(dcols, batch, H, W, C, ks, stride, pad, Ho,
                                                                           Wo, dX);
   return launch_status();

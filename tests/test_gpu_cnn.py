@@ -51,23 +51,29 @@ def test_gemm_bf16_matches_numpy(ta, tb, M, N, K):
     assert e < 1e-5
 
 
-def test_im2col_col2im_match_oracle():
+@pytest.mark.parametrize("C", [1, 5, 8, 16])
+def test_im2col_col2im_match_oracle(C):
     import torch
 
     from pypty_amd import _lib
 
-    print("[im2col] B=3 H=10 W=12 C=5", flush=True)
+    print(f"[im2col] B=3 H=10 W=12 C={C}", flush=True)
     lib = _lib.load()
-    rng = np.random.default_rng(11)
-    X = rng.standard_normal((3, 10, 12, 5)).astype(np.float32)
-    cols = torch.empty((3 * 5 * 6, 80), dtype=torch.int16, device="cuda")
+    rng = np.random.default_rng(11 + C)
+    X = rng.standard_normal((3, 10, 12, C)).astype(np.float32)
+    Kc = 16 * C
+    cols = torch.empty((3 * 5 * 6, Kc), dtype=torch.int16, device="cuda")
     s = _lib.stream_ptr()
-    assert lib.pty_cnn_im2col(torch.from_numpy(X).cuda().data_ptr(), 3, 10, 12, 5, 4, 2, 1, cols.data_ptr(), s) == 0
+    assert lib.pty_cnn_im2col(torch.from_numpy(X).cuda().data_ptr(), 3, 10, 12, C, 4, 2, 1, cols.data_ptr(), s) == 0
     got = (cols.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32)
     assert np.array_equal(got, cnn.bf16_round(cnn.im2col(X)))
-    D = rng.standard_normal((90, 80)).astype(np.float32)
-    dX = torch.empty((3, 10, 12, 5), dtype=torch.float32, device="cuda")
-    assert lib.pty_cnn_col2im(torch.from_numpy(D).cuda().data_ptr(), 3, 10, 12, 5, 4, 2, 1, dX.data_ptr(), s) == 0
+    D = rng.standard_normal((90, Kc)).astype(np.float32)
+    dX = torch.empty((3, 10, 12, C), dtype=torch.float32, device="cuda")
+    rc = lib.pty_cnn_col2im(torch.from_numpy(D).cuda().data_ptr(), 3, 10, 12, C, 4, 2, 1, dX.data_ptr(), s)
+    if C % 4:
+        assert rc == -1  # four channels per thread
+        return
+    assert rc == 0
     assert np.allclose(dX.cpu().numpy(), cnn.col2im(D, X.shape), rtol=1e-6, atol=1e-6)
 
 
