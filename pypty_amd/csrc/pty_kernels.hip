@@ -190,8 +190,9 @@ enum { LAY_POS = 0, LAY_RT = 1, LAY_GT = 2 };
 #define PTY_GT_THREADS 512
 #endif
 constexpr int kGtThreads = PTY_GT_THREADS;  // threads of a 192/256 px slab workgroup
+constexpr int kLayoutOf(int n) { return (n == 64 || n == 128) ? LAY_RT : ((n == 192 || n == 256) ? LAY_GT : LAY_POS); }
 template <int N>
-constexpr int kLayout = (N == 64 || N == 128) ? LAY_RT : ((N == 192 || N == 256) ? LAY_GT : LAY_POS);
+constexpr int kLayout = kLayoutOf(N);
 
 template <int N, int LAY>
 __host__ __device__ __forceinline__ void spec_of(int e, int& fy, int& fx) {
@@ -1223,9 +1224,14 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
   const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
   const unsigned tid = threadIdx.x;
   const RTIds<N> id;
+  // the group's partial sum stays in registers across its windows (one read
+  // and one write of the slot per launch; the slot carries earlier batches)
   const RView<N, float2> acc(a.probe_part + (size_t(g) * a.M + m) * N * N, tid);
   const RView<N, float2> fp(a.fprobe + size_t(m) * N * N, tid);
   const RView<N, float> mask(a.maskc, tid);
+  float2 sum[R::R1];
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) sum[i] = acc[i];
   for (int b = g; b < a.B; b += a.G) {
     const int idx = seq_index(a.seq, a.b0 + b, a.P);
     float2 v[R::R1];
@@ -1242,7 +1248,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
     if (!a.shift) {
       // natural order: register n of the row phase is R-order slot n*NT + tid
 #pragma unroll
-      for (int n = 0; n < R::R1; ++n) acc.store(n, cadd(acc[n], v[n]));
+      for (int n = 0; n < R::R1; ++n) sum[n] = cadd(sum[n], v[n]);
       continue;
     }
     const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
@@ -1257,7 +1263,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       const float mk = mask[i];
       const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mk);
       const float2 Q = cmulc(v[i], E);
-      acc.store(i, cadd(acc[i], Q));
+      sum[i] = cadd(sum[i], Q);
       if (a.step_pos) {
         const float2 Z = cmulc(Q, fp[i]);  // conj(FP) * Q
         const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
@@ -1276,6 +1282,8 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       }
     }
   }
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) acc.store(i, sum[i]);
 }
 
 // probe_grad = ifft2(mask * fft2(sum)) (objective.py:927-933) from the reduced
@@ -2071,8 +2079,12 @@ Layout plan_layout(const pty_problem* p) {
   const size_t n2 = size_t(p->n) * p->n;
   const int B = batch_windows(p);
   const int S = p->n_slices, K = p->n_obj_modes, M = p->n_probe_modes;
-  // probe-gradient groups: enough workgroups (G * M) to fill the chip
-  const int G = std::min(B, std::max(64, (1024 + M - 1) / M));
+  // probe-gradient groups: the register tile keeps a group's sum in
+  // registers, so one workgroup per CU (G * M <= 256 CUs) walks several
+  // windows; the other tiles add into the slot in memory per window and take
+  // enough groups (G * M >= 1024 workgroups) to fill the chip
+  const int G = kLayoutOf(p->n) == LAY_RT ? std::min(B, std::max(1, 256 / M))
+                                          : std::min(B, std::max(64, (1024 + M - 1) / M));
   const size_t objsz = size_t(p->obj_ny) * p->obj_nx * S * K * 8;
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t r = o; o += al(bytes); return r; };
