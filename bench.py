@@ -403,9 +403,9 @@ def main(argv=None):
     n_streams = max(1, min(len(jobs), (2 if ragged else 1) if args.streams is None else args.streams))
     lanes = [torch.cuda.Stream(dev) for _ in range(n_streams)] if n_streams > 1 else []
 
-    def step():
+    def step(concurrent=True):
         out = None
-        if not lanes:
+        if not lanes or not concurrent:
             for j in jobs:
                 out = j.step(sync=False)
             return out
@@ -451,10 +451,11 @@ def main(argv=None):
     roofline = None
     if not args.no_kernel_timing:
         # same steps again with HIP events around every launch, for the per-kernel breakdown
+        # (datasets one after the other: events on concurrent streams would time each other's kernels)
         _lib.timing_reset()
         _lib.timing_enable(True)
         for _ in range(args.steps):
-            step()
+            step(concurrent=False)
         torch.cuda.synchronize()
         kernels = {k: v for k, v in _lib.timing_read().items() if v[1] > 0}
         _lib.timing_enable(False)

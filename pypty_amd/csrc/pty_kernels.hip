@@ -190,7 +190,9 @@ enum { LAY_POS = 0, LAY_RT = 1, LAY_GT = 2 };
 #define PTY_GT_THREADS 512
 #endif
 constexpr int kGtThreads = PTY_GT_THREADS;  // threads of a 192/256 px slab workgroup
-constexpr int kLayoutOf(int n) { return (n == 64 || n == 128) ? LAY_RT : ((n == 192 || n == 256) ? LAY_GT : LAY_POS); }
+constexpr int kLayoutOf(int n) {
+  return (n == 48 || n == 64 || n == 96 || n == 128) ? LAY_RT : ((n == 192 || n == 256) ? LAY_GT : LAY_POS);
+}
 template <int N>
 constexpr int kLayout = kLayoutOf(N);
 
@@ -956,7 +958,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
   float2 v[R::R1];
   const float2* src = a.obj + size_t(plane) * a.ny * a.nx + (inside ? size_t(y0 + id.row) * a.nx + x0 + id.rj : 0);
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) v[n] = inside ? src[8 * n] : make_float2(0.f, 0.f);
+  for (int n = 0; n < R::R1; ++n) v[n] = inside ? src[R::T * n] : make_float2(0.f, 0.f);
   rt_fft2_fwd<N>(v, T, id);
   const float sc = 1.0f / float(N * N);
   const RView<N, float> mask(a.maskc, tid);
@@ -991,7 +993,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
     for (int n = 0; n < R::R1; ++n) v[n] = I[n];
   } else {
 #pragma unroll
-    for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + 8 * n];
+    for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + R::T * n];
   }
   const size_t ostride = size_t(a.K) * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
@@ -1142,7 +1144,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
   rt_fft2_inv<N>(v, T, id);
   float2* G = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + 8 * n] = v[n];
+  for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + R::T * n] = v[n];
 }
 
 // ------------------------------------- register-tile probe side (64, 128)
@@ -1173,7 +1175,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_fprobe_rt(const float2* __restric
   const float2* src = probe + size_t(blockIdx.x) * N * N;
   float2 v[R::R1];
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + 8 * n];
+  for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + R::T * n];
   rt_fft2_fwd<N>(v, T, id);
   const RView<N, float2> out(fprobe + size_t(blockIdx.x) * N * N, threadIdx.x);
 #pragma unroll
@@ -1207,7 +1209,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
   rt_fft2_inv<N>(v, T, id);
   float2* dst = a.Pp + (size_t(b) * a.M + m) * N * N;
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + 8 * n] = v[n];
+  for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + R::T * n] = v[n];
 }
 
 // probe-side adjoint (objective.py:641-658, 697-703): per window and mode
@@ -1308,7 +1310,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_final_rt(const float2* __re
   rt_fft2_inv<N>(v, T, id);
   float2* dst = probe_grad + size_t(blockIdx.x) * N * N;
 #pragma unroll
-  for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + 8 * n] = v[n];
+  for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + R::T * n] = v[n];
 }
 
 // ------------------------------------------ global-slab pipeline (192, 256)

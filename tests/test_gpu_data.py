@@ -46,7 +46,7 @@ def _compare(got, ref, ref64, step_pos):
 
 
 @pytest.mark.parametrize("algorithm", ["lsq_compressed", "gauss_compressed", "poisson_compressed"])
-@pytest.mark.parametrize("n,S,M,shift", [(32, 2, 2, True), (64, 3, 1, True), (128, 2, 2, True)])
+@pytest.mark.parametrize("n,S,M,shift", [(32, 2, 2, True), (64, 3, 1, True), (96, 2, 2, True), (128, 2, 2, True)])
 def test_compressed_losses_match_oracle(algorithm, n, S, M, shift):
     import torch
 

@@ -51,6 +51,8 @@ def test_fft_modes_match_numpy(n, mode):
     dict(n=64, S=3, K=2, M=2, shift=True, algorithm="lsq_sqrt_2", ny=128, nx=128, P=6),
     dict(n=192, S=4, K=2, M=3, shift=True, algorithm="ml", ny=240, nx=240, P=5),
     dict(n=256, S=3, K=1, M=1, shift=False, algorithm="lsq", ny=300, nx=300, P=4),
+    dict(n=96, S=2, K=2, M=3, shift=True, algorithm="ml", ny=144, nx=144, P=6),
+    dict(n=48, S=3, K=1, M=2, shift=True, algorithm="lsq_sqrt", ny=96, nx=96, P=8),
 ])
 def test_loss_and_direction_matches_oracle(case):
     torch = _torch()

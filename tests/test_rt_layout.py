@@ -1,98 +1,123 @@
-"""CPU emulation of the register-tile layout of pypty_amd/csrc/pty_rt.hpp (N = 64, 128).
+"""CPU emulation of the register-tile layout of pypty_amd/csrc/pty_rt.hpp (N = 48, 64, 96, 128;
+T = 8 threads per line when 64 divides N, else 4).
 
 The index maps below restate the header's (xslot, col_freq, the row/column phase lane
 assignment, the transpose positions).  The emulation checks that the two-phase line split
 computes the 2D DFT exactly (against numpy), that the inverse path returns the input, that every
-LDS address stays inside the N x (N+8) tile, and that each exchange / transpose instruction puts
-the 32 lanes of a half-wavefront on 32 distinct 8-byte bank pairs.
+LDS address stays inside the N x LS tile, and the bank behaviour of each exchange / transpose
+instruction: with T = 8 the 32 lanes of a half-wavefront fall on 32 distinct 8-byte bank pairs;
+with T = 4 the row phase does, and the column phase puts at most two lanes on a bank pair.
 """
 import numpy as np
 import pytest
 
 
 def _maps(N):
-    R1 = N // 8
-    CPT = R1 // 8
-    LS = N + 8
-    xslot = lambda j, k1: k1 * 8 + (j ^ (k1 // CPT))  # noqa: E731
-    col_freq = lambda col: CPT * (col & 7) + ((col >> 3) % CPT) + R1 * ((col >> 3) // CPT)  # noqa: E731
-    return R1, CPT, LS, xslot, col_freq
+    T = 8 if N % 64 == 0 else 4
+    R1 = N // T
+    CPT = R1 // T
+    LS = N + (8 if T == 8 else 4)
+    xslot = lambda j, k1: k1 * T + (j ^ (k1 // CPT))  # noqa: E731
+    col_freq = lambda col: CPT * (col % T) + ((col // T) % CPT) + R1 * ((col // T) // CPT)  # noqa: E731
+    return T, R1, CPT, LS, xslot, col_freq
 
 
-def _line_fwd(N, xs, T, at):
-    R1, CPT, LS, xslot, _ = _maps(N)
-    for j in range(8):
+def _line_fwd(N, xs, Tl, at):
+    T, R1, CPT, LS, xslot, _ = _maps(N)
+    for j in range(T):
         v = np.fft.fft(xs[j]) * np.exp(-2j * np.pi * j * np.arange(R1) / N)
         for k1 in range(R1):
-            T[at(xslot(j, k1))] = v[k1]
+            Tl[at(xslot(j, k1))] = v[k1]
     out = []
-    for j in range(8):
+    for j in range(T):
         u = np.zeros(R1, complex)
         for c in range(CPT):
-            u[c * 8:(c + 1) * 8] = np.fft.fft([T[at(xslot(jj, CPT * j + c))] for jj in range(8)])
+            u[c * T:(c + 1) * T] = np.fft.fft([Tl[at(xslot(jj, CPT * j + c))] for jj in range(T)])
         out.append(u)
     return out
 
 
-def _line_inv(N, vs, T, at):
-    R1, CPT, LS, xslot, _ = _maps(N)
-    for j in range(8):
+def _line_inv(N, vs, Tl, at):
+    T, R1, CPT, LS, xslot, _ = _maps(N)
+    for j in range(T):
         for c in range(CPT):
-            y = np.fft.ifft(vs[j][c * 8:(c + 1) * 8]) * 8
-            for jj in range(8):
-                T[at(xslot(jj, CPT * j + c))] = y[jj]
+            y = np.fft.ifft(vs[j][c * T:(c + 1) * T]) * T
+            for jj in range(T):
+                Tl[at(xslot(jj, CPT * j + c))] = y[jj]
     out = []
-    for j in range(8):
-        z = np.array([T[at(xslot(j, k1))] for k1 in range(R1)]) * np.exp(2j * np.pi * j * np.arange(R1) / N)
+    for j in range(T):
+        z = np.array([Tl[at(xslot(j, k1))] for k1 in range(R1)]) * np.exp(2j * np.pi * j * np.arange(R1) / N)
         out.append(np.fft.ifft(z) * R1)
     return out
 
 
-@pytest.mark.parametrize("N", [64, 128])
+@pytest.mark.parametrize("N", [48, 64, 96, 128])
 def test_register_tile_computes_the_2d_dft(N):
-    R1, CPT, LS, xslot, col_freq = _maps(N)
+    T, R1, CPT, LS, xslot, col_freq = _maps(N)
     rng = np.random.default_rng(N)
     X = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
-    T = np.full(N * LS, np.nan + 0j)
+    Tl = np.full(N * LS, np.nan + 0j)
     for y in range(N):  # row phase, then the transpose write
-        out = _line_fwd(N, [X[y, rj::8] for rj in range(8)], T, lambda p, y=y: y * LS + p)
-        for rj in range(8):
+        out = _line_fwd(N, [X[y, rj::T] for rj in range(T)], Tl, lambda p, y=y: y * LS + p)
+        for rj in range(T):
             for c in range(CPT):
-                for k2 in range(8):
-                    T[y * LS + 8 * (CPT * k2 + c) + rj] = out[rj][c * 8 + k2]
+                for k2 in range(T):
+                    Tl[y * LS + T * (CPT * k2 + c) + rj] = out[rj][c * T + k2]
     F = np.zeros((N, N), complex)
     for col in range(N):  # column phase
-        vs = [np.array([T[(cj + 8 * n) * LS + col] for n in range(R1)]) for cj in range(8)]
-        out = _line_fwd(N, vs, T, lambda p, col=col: p * LS + col)
-        for cj in range(8):
+        vs = [np.array([Tl[(cj + T * n) * LS + col] for n in range(R1)]) for cj in range(T)]
+        out = _line_fwd(N, vs, Tl, lambda p, col=col: p * LS + col)
+        for cj in range(T):
             for c in range(CPT):
-                for k2 in range(8):
-                    F[CPT * cj + c + R1 * k2, col_freq(col)] = out[cj][c * 8 + k2]
+                for k2 in range(T):
+                    F[CPT * cj + c + R1 * k2, col_freq(col)] = out[cj][c * T + k2]
     assert np.abs(F - np.fft.fft2(X)).max() <= 1e-12 * np.abs(F).max()
     # inverse: column phase, transpose, row phase
     Y = np.zeros((N, N), complex)
-    T[:] = 0
+    Tl[:] = 0
     for col in range(N):
         fx = col_freq(col)
-        vs = [np.array([F[CPT * cj + c + R1 * k2, fx] for c in range(CPT) for k2 in range(8)]) for cj in range(8)]
-        out = _line_inv(N, vs, T, lambda p, col=col: p * LS + col)
-        for cj in range(8):
+        vs = [np.array([F[CPT * cj + c + R1 * k2, fx] for c in range(CPT) for k2 in range(T)]) for cj in range(T)]
+        out = _line_inv(N, vs, Tl, lambda p, col=col: p * LS + col)
+        for cj in range(T):
             for n in range(R1):
-                T[(cj + 8 * n) * LS + col] = out[cj][n]
+                Tl[(cj + T * n) * LS + col] = out[cj][n]
     for y in range(N):
-        vs = [np.array([T[y * LS + 8 * (CPT * k2 + c) + rj] for c in range(CPT) for k2 in range(8)])
-              for rj in range(8)]
-        out = _line_inv(N, vs, T, lambda p, y=y: y * LS + p)
-        for rj in range(8):
-            Y[y, rj::8] = out[rj]
+        vs = [np.array([Tl[y * LS + T * (CPT * k2 + c) + rj] for c in range(CPT) for k2 in range(T)])
+              for rj in range(T)]
+        out = _line_inv(N, vs, Tl, lambda p, y=y: y * LS + p)
+        for rj in range(T):
+            Y[y, rj::T] = out[rj]
     assert np.abs(Y / N ** 2 - X).max() <= 1e-12 * np.abs(X).max()
+
+
+def _rslot(N, y, x):
+    T = 8 if N % 64 == 0 else 4
+    L, NT = 64 // T, N * T
+    return (x // T) * NT + (y // L) * 64 + (y % L) * T + (x % T)
+
+
+@pytest.mark.parametrize("N", [48, 64, 96, 128])
+def test_r_and_c_orders_are_permutations(N):
+    """R-order slots of all pixels and C-order frequencies of all slots cover the window once."""
+    T = 8 if N % 64 == 0 else 4
+    L, NT = 64 // T, N * T
+    _, R1, CPT, _, _, col_freq = _maps(N)
+    slots = {_rslot(N, y, x) for y in range(N) for x in range(N)}
+    assert slots == set(range(N * N))
+    freqs = set()
+    for e in range(N * N):
+        i, t = e // NT, e % NT
+        lane = t & 63
+        freqs.add((CPT * (lane // L) + i // T + R1 * (i % T), col_freq((t >> 6) * L + lane % L)))
+    assert len(freqs) == N * N
 
 
 @pytest.mark.parametrize("N", [192, 256])
 def test_line_split_of_large_windows(N):
     """The 24- and 32-sample line splits (3 and 4 DFT-8 groups per thread) of the 192 / 256 px
     global-slab tiles: one exchange per line through an LDS row of stride N + 8."""
-    R1, CPT, LS, xslot, col_freq = _maps(N)
+    _, R1, CPT, LS, xslot, col_freq = _maps(N)
     rng = np.random.default_rng(N)
     x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
     T = np.full(LS, np.nan + 0j)
@@ -120,27 +145,30 @@ def test_line_split_of_large_windows(N):
             assert all(len(set((h % 32).tolist())) == 32 for h in (a[:32], a[32:]))
 
 
-@pytest.mark.parametrize("N", [64, 128])
+@pytest.mark.parametrize("N", [48, 64, 96, 128])
 def test_register_tile_lds_bounds_and_banks(N):
-    R1, CPT, LS, xslot, _ = _maps(N)
+    T, R1, CPT, LS, xslot, _ = _maps(N)
+    L = 64 // T
 
-    def check(addrs):
+    def check(addrs, most):
         a = np.asarray(addrs)
         assert a.min() >= 0 and a.max() < N * LS
         for half in (a[:32], a[32:]):
-            assert len(set((half % 32).tolist())) == 32
+            _, counts = np.unique(half % 32, return_counts=True)
+            assert counts.max() <= most
 
+    col_most = 1 if T == 8 else 2  # lanes per 8-byte bank pair in the column phase
     lanes = np.arange(64)
-    for w in range(N // 8):
-        # row phase: lane = j + 8*row_in_wave; column phase: lane = col_in_wave + 8*j
+    for w in range(N // L):
+        # row phase: lane = j + T*row_in_wave; column phase: lane = col_in_wave + L*j
         for k1 in range(R1):
-            check([(8 * w + (l >> 3)) * LS + xslot(l & 7, k1) for l in lanes])
-            check([xslot(l >> 3, k1) * LS + 8 * w + (l & 7) for l in lanes])
+            check([(L * w + l // T) * LS + xslot(l % T, k1) for l in lanes], 1)
+            check([xslot(l // L, k1) * LS + L * w + l % L for l in lanes], col_most)
         for c in range(CPT):
-            for jj in range(8):
-                check([(8 * w + (l >> 3)) * LS + xslot(jj, CPT * (l & 7) + c) for l in lanes])
-                check([xslot(jj, CPT * (l >> 3) + c) * LS + 8 * w + (l & 7) for l in lanes])
-            for k2 in range(8):
-                check([(8 * w + (l >> 3)) * LS + 8 * (CPT * k2 + c) + (l & 7) for l in lanes])
+            for jj in range(T):
+                check([(L * w + l // T) * LS + xslot(jj, CPT * (l % T) + c) for l in lanes], 1)
+                check([xslot(jj, CPT * (l // L) + c) * LS + L * w + l % L for l in lanes], col_most)
+            for k2 in range(T):
+                check([(L * w + l // T) * LS + T * (CPT * k2 + c) + l % T for l in lanes], 1)
         for n in range(R1):
-            check([((l >> 3) + 8 * n) * LS + 8 * w + (l & 7) for l in lanes])
+            check([(l // L + T * n) * LS + L * w + l % L for l in lanes], col_most)
