@@ -168,9 +168,10 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_fft2_tiles(float2* __restrict__
 // (objective.py:337-345), its first-slice power H^e (multislice_core.py:73)
 // and the band mask q^2 <= (max(qx)*cutoff)^2 (utils.py:153-199).  The mask
 // is stored in tile position order; H, H^e and maskc in the order of the
-// fused forward/adjoint kernels: position order for the LDS tile, C-order
-// for the register tile (N = 64, 128).  H and H^e carry the 1/n^2 of the
-// inverse FFT.
+// fused forward/adjoint kernels: C-order of the register tile (N <= 128) or
+// of the slab tile (192, 256); position order (LAY_POS) is the order of the
+// LDS tile, kept for its kernels.  H and H^e carry the 1/n^2 of the inverse
+// FFT.
 struct TableArgs {
   float2* H;
   float2* H0;
@@ -183,15 +184,15 @@ __device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b)
 __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
 
 // Spectral order of H, H0, maskc, Psi and coef for the fused kernels of a
-// window size: 0 tile position order (LDS tile), 1 register-tile C-order
-// (64, 128 px), 2 global-slab C-order (192, 256 px).
+// window size: 1 register-tile C-order (16 to 128 px), 2 global-slab C-order
+// (192, 256 px); 0 is the LDS tile's position order.
 enum { LAY_POS = 0, LAY_RT = 1, LAY_GT = 2 };
 #ifndef PTY_GT_THREADS
 #define PTY_GT_THREADS 512
 #endif
 constexpr int kGtThreads = PTY_GT_THREADS;  // threads of a 192/256 px slab workgroup
 constexpr int kLayoutOf(int n) {
-  return (n == 48 || n == 64 || n == 96 || n == 128) ? LAY_RT : ((n == 192 || n == 256) ? LAY_GT : LAY_POS);
+  return (n <= 128) ? LAY_RT : LAY_GT;
 }
 template <int N>
 constexpr int kLayout = kLayoutOf(N);
@@ -907,9 +908,9 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_objgrad(BatchArgs a) {
 }
 
 // --------------------------------------------- register-tile pipeline
-// The same per-batch steps for N = 64 and 128 over the register tile of
-// pty_rt.hpp.  Buffer orders: Ob and stash R-order; Psi, coef, H, H0, maskc
-// C-order; gP R-order (64, 128 px); Pp and the band-limited window gradient natural order.
+// The same per-batch steps for N = 16 to 128 over the register tile of
+// pty_rt.hpp.  Buffer orders: Ob, stash and gP R-order; Psi, coef, H, H0,
+// maskc C-order; Pp and the band-limited window gradient natural order.
 //
 // Register i of an R/C-order array is element i*NT + tid of one window's
 // N*N block.  It is accessed through a buffer resource on the block: the
@@ -1147,7 +1148,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
   for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + R::T * n] = v[n];
 }
 
-// ------------------------------------- register-tile probe side (64, 128)
+// --------------------------------- register-tile probe side (16 to 128 px)
 // Probe spectra (fprobe) and the spectral probe-gradient partials are in
 // C-order here, the probe-side gradient from the adjoint in R-order; the
 // shift kernel E = ex[fx] * ey[fy] * mask comes from two axis tables in LDS

@@ -1,4 +1,4 @@
-// Register-resident window tile for N = 48, 64, 96 and 128 (gfx950).
+// Register-resident window tile for N = 16, 32, 48, 64, 96 and 128 (gfx950).
 //
 // Each thread keeps R1 = N/T samples of one line in registers; the T threads
 // of a line (T = 8 for N a multiple of 64, else 4) are T lanes of the same
@@ -38,10 +38,10 @@ namespace pty {
 
 template <int N>
 struct RT {
-  // 48 to 128 px: the whole window in registers; 192 and 256 px: the line
+  // 16 to 128 px: the whole window in registers; 192 and 256 px: the line
   // split alone (GT below)
-  static_assert(N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256,
-                "register tile line split: N = 48, 64, 96, 128, 192, 256");
+  static_assert(N == 16 || N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256,
+                "register tile line split: N = 16, 32, 48, 64, 96, 128, 192, 256");
   static constexpr int T = (N % 64 == 0) ? 8 : 4;  // threads per line
   static constexpr int L = 64 / T;                 // lines per wavefront
   static constexpr int R1 = N / T;                 // samples per thread and line

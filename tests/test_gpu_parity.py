@@ -53,6 +53,7 @@ def test_fft_modes_match_numpy(n, mode):
     dict(n=256, S=3, K=1, M=1, shift=False, algorithm="lsq", ny=300, nx=300, P=4),
     dict(n=96, S=2, K=2, M=3, shift=True, algorithm="ml", ny=144, nx=144, P=6),
     dict(n=48, S=3, K=1, M=2, shift=True, algorithm="lsq_sqrt", ny=96, nx=96, P=8),
+    dict(n=16, S=3, K=1, M=2, shift=True, algorithm="lsq_sqrt", ny=48, nx=48, P=9),
 ])
 def test_loss_and_direction_matches_oracle(case):
     torch = _torch()

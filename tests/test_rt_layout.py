@@ -1,4 +1,4 @@
-"""CPU emulation of the register-tile layout of pypty_amd/csrc/pty_rt.hpp (N = 48, 64, 96, 128;
+"""CPU emulation of the register-tile layout of pypty_amd/csrc/pty_rt.hpp (N = 16 ... 128;
 T = 8 threads per line when 64 divides N, else 4).
 
 The index maps below restate the header's (xslot, col_freq, the row/column phase lane
@@ -51,7 +51,7 @@ def _line_inv(N, vs, Tl, at):
     return out
 
 
-@pytest.mark.parametrize("N", [48, 64, 96, 128])
+@pytest.mark.parametrize("N", [16, 32, 48, 64, 96, 128])
 def test_register_tile_computes_the_2d_dft(N):
     T, R1, CPT, LS, xslot, col_freq = _maps(N)
     rng = np.random.default_rng(N)
@@ -97,7 +97,7 @@ def _rslot(N, y, x):
     return (x // T) * NT + (y // L) * 64 + (y % L) * T + (x % T)
 
 
-@pytest.mark.parametrize("N", [48, 64, 96, 128])
+@pytest.mark.parametrize("N", [16, 32, 48, 64, 96, 128])
 def test_r_and_c_orders_are_permutations(N):
     """R-order slots of all pixels and C-order frequencies of all slots cover the window once."""
     T = 8 if N % 64 == 0 else 4
@@ -145,7 +145,7 @@ def test_line_split_of_large_windows(N):
             assert all(len(set((h % 32).tolist())) == 32 for h in (a[:32], a[32:]))
 
 
-@pytest.mark.parametrize("N", [48, 64, 96, 128])
+@pytest.mark.parametrize("N", [16, 32, 48, 64, 96, 128])
 def test_register_tile_lds_bounds_and_banks(N):
     T, R1, CPT, LS, xslot, _ = _maps(N)
     L = 64 // T
