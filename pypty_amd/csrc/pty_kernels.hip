@@ -1665,6 +1665,212 @@ __global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
   }
 }
 
+// ------------------------------- slab-tile probe side (192, 256 px)
+// As on the register tile: probe spectra and the spectral probe-gradient
+// partials in slab C-order, the shift factor from two axis tables in LDS.
+// Without a shift the partials hold the natural-order gradient in slab
+// R-order.  Each workgroup transposes through its slab in `scratch`.
+
+// probe spectra, slab C-order, unnormalised
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_fprobe_gt(const float2* __restrict__ probe,
+                                                          float2* __restrict__ fprobe, float2* __restrict__ scratch) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  const GTLane<N> L;
+  const Slab<N> sl(scratch + size_t(blockIdx.x) * N * N);
+  const float2* src = probe + size_t(blockIdx.x) * N * N;
+  float2 v[G::R1];
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    const float2* p = src + size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) v[n] = p[8 * n];
+    rt_line_fwd<N, true>(v, X, L.ln, L.j);
+    gt_store_rowspec<N>(v, sl, L, r);
+  }
+  __syncthreads();
+  const RView<N, float2, G::NT> out(fprobe + size_t(blockIdx.x) * N * N, threadIdx.x);
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    gt_load_col<N>(v, sl, L, r);
+    rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+    for (int i = 0; i < G::R1; ++i) out.store(r * G::R1 + i, v[i]);
+  }
+}
+
+// shifted probes (objective.py:471-476): ifft2(fprobe * E), natural order
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  __shared__ float2 exy[2 * N];
+  const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  shift_tables_freq<N>(exy, dy, dx);
+  __syncthreads();
+  const RView<N, float2, G::NT> fp(a.fprobe + size_t(m) * N * N, tid);
+  const RView<N, float, G::NT> mask(a.maskc, tid);
+  const float s = 1.0f / float(N * N);
+  float2 v[G::R1];
+  for (int r = 0; r < G::ROUNDS; ++r) {
+#pragma unroll
+    for (int i = 0; i < G::R1; ++i) {
+      const int q = r * G::R1 + i;
+      int fy, fx;
+      G::cspec(q * G::NT + tid, fy, fx);
+      const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mask[q]);
+      v[i] = cscale(cmul(fp[q], E), s);
+    }
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+    gt_store_col<N>(v, sl, L, r);
+  }
+  __syncthreads();
+  float2* dst = a.Pp + (size_t(b) * a.M + m) * N * N;
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    gt_load_rowspec<N>(v, sl, L, r);
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+    float2* out = dst + size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) out[8 * n] = v[n];
+  }
+}
+
+// probe-side adjoint (objective.py:641-658, 697-703) for 192/256 px windows:
+// the gradient summed over object modes (natural order from k_adjoint_gt) is
+// transformed, shifted back and added (C-order) to the group's partial, with
+// the position gradient -2/n^2 sum Re(K conj(FP) Q) in FP64; without a shift
+// the natural-order gradient itself is added (slab R-order)
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  __shared__ float2 exy[2 * N];
+  __shared__ double red[16];
+  const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  const RView<N, float2, G::NT> acc(a.probe_part + (size_t(g) * a.M + m) * N * N, tid);
+  const RView<N, float2, G::NT> fp(a.fprobe + size_t(m) * N * N, tid);
+  const RView<N, float, G::NT> mask(a.maskc, tid);
+  float2 v[G::R1];
+  for (int b = g; b < a.B; b += a.G) {
+    const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    const float2* gp = a.gP + (size_t(b) * a.M + m) * a.K * N * N;
+    if (!a.shift) {
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        const size_t row = size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = gp[row + 8 * n];
+        for (int k = 1; k < a.K; ++k)
+#pragma unroll
+          for (int n = 0; n < G::R1; ++n) v[n] = cadd(v[n], gp[size_t(k) * N * N + row + 8 * n]);
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) acc.store(r * G::R1 + n, cadd(acc[r * G::R1 + n], v[n]));
+      }
+      continue;
+    }
+    const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+    __syncthreads();  // the previous window's slab and table reads are done
+    shift_tables_freq<N>(exy, dy, dx);
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      const size_t row = size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) v[n] = gp[row + 8 * n];
+      for (int k = 1; k < a.K; ++k)
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = cadd(v[n], gp[size_t(k) * N * N + row + 8 * n]);
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+      gt_store_rowspec<N>(v, sl, L, r);
+    }
+    __syncthreads();  // the slab holds the row spectra; the table is published
+    double sy = 0.0, sx = 0.0;
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      gt_load_col<N>(v, sl, L, r);
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+      for (int i = 0; i < G::R1; ++i) {
+        const int q = r * G::R1 + i;
+        int fy, fx;
+        G::cspec(q * G::NT + tid, fy, fx);
+        const float mk = mask[q];
+        const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mk);
+        const float2 Q = cmulc(v[i], E);
+        acc.store(q, cadd(acc[q], Q));
+        if (a.step_pos) {
+          const float2 Z = cmulc(Q, fp[q]);  // conj(FP) * Q
+          const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
+          const float ky = fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk);
+          sx += -double(kx) * double(Z.y);
+          sy += -double(ky) * double(Z.y);
+        }
+      }
+    }
+    if (a.step_pos) {
+      const double SY = block_sum(sy, red);
+      const double SX = block_sum(sx, red);
+      if (tid == 0) {
+        const double sh = -2.0 / (double(N) * double(N));
+        a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
+        a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
+      }
+    }
+  }
+}
+
+// probe_grad = ifft2(mask * fft2(sum)) (objective.py:927-933) from the reduced
+// partials: slab C-order spectrum (spectral) or slab R-order natural gradient
+template <int N>
+__global__ __launch_bounds__(kGtThreads) void k_probe_final_gt(const float2* __restrict__ sum, int spectral,
+                                                               const float* __restrict__ maskc,
+                                                               float2* __restrict__ probe_grad,
+                                                               float2* __restrict__ scratch) {
+  using G = GT<N, kGtThreads>;
+  __shared__ float2 X[G::LINES * G::LS];
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  const Slab<N> sl(scratch + size_t(blockIdx.x) * N * N);
+  const RView<N, float2, G::NT> src(sum + size_t(blockIdx.x) * N * N, tid);
+  const RView<N, float, G::NT> mask(maskc, tid);
+  const float s = 1.0f / float(N * N);
+  float2 v[G::R1];
+  if (!spectral) {
+    for (int r = 0; r < G::ROUNDS; ++r) {
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) v[n] = src[r * G::R1 + n];
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+      gt_store_rowspec<N>(v, sl, L, r);
+    }
+    __syncthreads();
+  }
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    if (spectral) {
+#pragma unroll
+      for (int i = 0; i < G::R1; ++i) v[i] = src[r * G::R1 + i];
+    } else {
+      gt_load_col<N>(v, sl, L, r);
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+    }
+#pragma unroll
+    for (int i = 0; i < G::R1; ++i) v[i] = cscale(v[i], mask[r * G::R1 + i] * s);
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+    gt_store_col<N>(v, sl, L, r);
+  }
+  __syncthreads();
+  float2* dst = probe_grad + size_t(blockIdx.x) * N * N;
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    gt_load_rowspec<N>(v, sl, L, r);
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+    float2* out = dst + size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) out[8 * n] = v[n];
+  }
+}
+
 // Deterministic scatter-add of a batch's window gradients (multislice_core.py:192,
 // 206-230): one workgroup per GTILE x GTILE object tile of one plane sums, in window
 // order, every window of the batch that overlaps the tile, then adds the sum
@@ -2228,6 +2434,8 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
     TimeScope ts(KT_PROBE_SHIFT, s);
     if constexpr (LAY == LAY_RT)
       k_probe_shift_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
+    else if constexpr (LAY == LAY_GT)
+      k_probe_shift_gt<N><<<a.B * a.M, kGtThreads, 0, s>>>(a);
     else
       k_probe_shift<N><<<a.B * a.M, T::NT, 0, s>>>(a);
   }
@@ -2288,6 +2496,8 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
     TimeScope ts(KT_PROBEGRAD, s);
     if constexpr (LAY == LAY_RT)
       k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+    else if constexpr (LAY == LAY_GT)
+      k_probegrad_gt<N><<<a.G * a.M, kGtThreads, 0, s>>>(a);
     else
       k_probegrad<N><<<a.G * a.M, T::NT, 0, s>>>(a);
   }
@@ -2318,6 +2528,9 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     TimeScope ts(KT_FPROBE, s);
     if constexpr (kLayout<N> == LAY_RT)
       k_fprobe_rt<N><<<M, RT<N>::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe);
+    else if constexpr (kLayout<N> == LAY_GT)
+      k_fprobe_gt<N><<<M, kGtThreads, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
+                                             reinterpret_cast<float2*>(ws + L.scratch));
     else
       k_fprobe<N><<<M, T::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
                                      reinterpret_cast<float2*>(ws + L.scratch));
@@ -2393,6 +2606,9 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     if constexpr (kLayout<N> == LAY_RT)
       k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, p->shift_active, maskc,
                                                    static_cast<float2*>(bf->probe_grad));
+    else if constexpr (kLayout<N> == LAY_GT)
+      k_probe_final_gt<N><<<M, kGtThreads, 0, s>>>(psum, p->shift_active, maskc,
+                                                   static_cast<float2*>(bf->probe_grad), a.scratch);
     else
       k_probe_final<N><<<M, T::NT, 0, s>>>(psum, 1, M, p->shift_active, mask,
                                           static_cast<float2*>(bf->probe_grad), a.scratch);
