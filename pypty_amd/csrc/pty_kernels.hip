@@ -166,16 +166,13 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_fft2_tiles(float2* __restrict__
 // --------------------------------------------------------------- tables
 // Propagator H = exp(-i*pi*dz*(lambda*q^2 + 2*(qx*tx + qy*ty))) * mask
 // (objective.py:337-345), its first-slice power H^e (multislice_core.py:73)
-// and the band mask q^2 <= (max(qx)*cutoff)^2 (utils.py:153-199).  The mask
-// is stored in tile position order; H, H^e and maskc in the order of the
-// fused forward/adjoint kernels: C-order of the register tile (N <= 128) or
-// of the slab tile (192, 256); position order (LAY_POS) is the order of the
-// LDS tile, kept for its kernels.  H and H^e carry the 1/n^2 of the inverse
-// FFT.
+// and the band mask q^2 <= (max(qx)*cutoff)^2 (utils.py:153-199), maskc,
+// stored in the order of the fused forward/adjoint kernels: C-order of the
+// register tile (N <= 128) or of the slab tile (192, 256).  H and H^e carry
+// the 1/n^2 of the inverse FFT.
 struct TableArgs {
   float2* H;
   float2* H0;
-  float* mask;
   float* maskc;
   float dz, lambda, px, py, tx, ty, cutoff, edge;
 };
@@ -184,9 +181,9 @@ __device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b)
 __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
 
 // Spectral order of H, H0, maskc, Psi and coef for the fused kernels of a
-// window size: 1 register-tile C-order (16 to 128 px), 2 global-slab C-order
-// (192, 256 px); 0 is the LDS tile's position order.
-enum { LAY_POS = 0, LAY_RT = 1, LAY_GT = 2 };
+// window size: register-tile C-order (16 to 128 px) or global-slab C-order
+// (192, 256 px).
+enum { LAY_RT = 1, LAY_GT = 2 };
 #ifndef PTY_GT_THREADS
 #define PTY_GT_THREADS 512
 #endif
@@ -199,14 +196,10 @@ constexpr int kLayout = kLayoutOf(N);
 
 template <int N, int LAY>
 __host__ __device__ __forceinline__ void spec_of(int e, int& fy, int& fx) {
-  if constexpr (LAY == LAY_RT) {
+  if constexpr (LAY == LAY_RT)
     RT<N>::cspec(e, fy, fx);
-  } else if constexpr (LAY == LAY_GT) {
+  else
     GT<N, kGtThreads>::cspec(e, fy, fx);
-  } else {
-    fy = Tile<N>::pos2freq(e / N);
-    fx = Tile<N>::pos2freq(e % N);
-  }
 }
 
 template <int N>
@@ -223,10 +216,8 @@ __device__ __forceinline__ float band_mask(const TableArgs& a, int fy, int fx) {
 
 template <int N, int LAY>
 __global__ void k_tables(TableArgs a) {
-  using T = Tile<N>;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N * N) return;
-  a.mask[e] = band_mask<N>(a, T::pos2freq(e / N), T::pos2freq(e % N));
   int fy, fx;
   spec_of<N, LAY>(e, fy, fx);
   const float qx = fftfreq_f(fx, N, a.px), qy = fftfreq_f(fy, N, a.py);
@@ -390,116 +381,6 @@ __device__ __forceinline__ int stash_slot(const BatchArgs& a, int s) {
 __device__ __forceinline__ const float* meas_row(const BatchArgs& a, int b, int idx, size_t row_len) {
   if (a.prep) return a.meas_stage + size_t(b) * row_len;
   return a.meas + size_t(a.meas_order == PTY_MEASURED_BY_SEQUENCE ? a.b0 + b : idx) * row_len;
-}
-
-template <int N>
-__global__ __launch_bounds__(Tile<N>::NT) void k_patch(BatchArgs a) {
-  using T = Tile<N>;
-  PTY_TILE(T, t, a.scratch);
-  const int planes = a.S * a.K;
-  const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
-  const int idx = seq_index(a.seq, a.b0 + b, a.P);
-  const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
-  const bool inside = window_inside(y0, x0, N, a.ny, a.nx);
-  const float2* src = a.obj + size_t(plane) * a.ny * a.nx;
-  const int tid = threadIdx.x;
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT, y = e / N, x = e % N;
-    t[T::lds(y, x)] = inside ? src[size_t(y0 + y) * a.nx + (x0 + x)] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  const float2* tw = twiddles<N>();
-  tile_fft_fwd<N>(t, tw);
-  const float s = 1.0f / float(N * N);
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT, py = e / N, px = e % N;
-    t[T::lds(py, px)] = cscale(t[T::lds(py, px)], a.mask[e] * s);
-  }
-  __syncthreads();
-  tile_fft_inv<N>(t, tw);
-  float2* dst = a.Ob + (size_t(b) * planes + plane) * N * N;
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT;
-    dst[e] = t[T::lds(e / N, e % N)];
-  }
-}
-
-// spectrum of each probe mode, position order, unnormalised
-template <int N>
-__global__ __launch_bounds__(Tile<N>::NT) void k_fprobe(const float2* __restrict__ probe, float2* __restrict__ fprobe,
-                                                        float2* __restrict__ scratch) {
-  using T = Tile<N>;
-  PTY_TILE(T, t, scratch);
-  const int m = blockIdx.x, tid = threadIdx.x;
-  const float2* src = probe + size_t(m) * N * N;
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT;
-    t[T::lds(e / N, e % N)] = src[e];
-  }
-  __syncthreads();
-  tile_fft_fwd<N>(t, twiddles<N>());
-  float2* dst = fprobe + size_t(m) * N * N;
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT;
-    dst[e] = t[T::lds(e / N, e % N)];
-  }
-}
-
-// Shift kernel E = exp(Kx*dx + Ky*dy) * mask with K = -2*pi*i*fftfreq * mask
-// (objective.py:309-313, 473).  Where the mask is 1 the exponent is a sum of
-// an x and a y term, so E = ex[px] * ey[py] * mask with the two axis factors
-// exp(-2*pi*i*fftfreq*d) tabulated once per window in LDS (2N sincos instead
-// of N^2); where the mask is 0, E is 0 either way.
-template <int N>
-__device__ __forceinline__ float2 shift_axis(int p, float d) {
-  using T = Tile<N>;
-  const float k = fmul(-6.2831855f, fftfreq_f(T::pos2freq(p), N, 1.f));
-  float sn, cs;
-  sincosf(fmul(k, d), &sn, &cs);
-  return make_float2(cs, sn);
-}
-// exy[0..N) = x factors for dx, exy[N..2N) = y factors for dy; the caller
-// synchronises before use
-template <int N>
-__device__ __forceinline__ void shift_tables(float2* exy, float dy, float dx) {
-  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) exy[i] = i < N ? shift_axis<N>(i, dx) : shift_axis<N>(i - N, dy);
-}
-__device__ __forceinline__ float2 shift_at(const float2* exy, int n, int py, int px, float m) {
-  return cscale(cmul(exy[px], exy[n + py]), m);
-}
-
-template <int N>
-__global__ __launch_bounds__(Tile<N>::NT) void k_probe_shift(BatchArgs a) {
-  using T = Tile<N>;
-  PTY_TILE(T, t, a.scratch);
-  const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
-  const int idx = seq_index(a.seq, a.b0 + b, a.P);
-  const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
-  const float2* src = a.fprobe + size_t(m) * N * N;
-  const float s = 1.0f / float(N * N);
-  const int tid = threadIdx.x;
-  __shared__ float2 exy[2 * N];
-  shift_tables<N>(exy, dy, dx);
-  __syncthreads();
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT, py = e / N, px = e % N;
-    const float2 E = shift_at(exy, N, py, px, a.mask[e]);
-    t[T::lds(py, px)] = cscale(cmul(src[e], E), s);
-  }
-  __syncthreads();
-  tile_fft_inv<N>(t, twiddles<N>());
-  float2* dst = a.Pp + (size_t(b) * a.M + m) * N * N;
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT;
-    dst[e] = t[T::lds(e / N, e % N)];
-  }
 }
 
 template <int N>
@@ -1154,6 +1035,11 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
 // shift kernel E = ex[fx] * ey[fy] * mask comes from two axis tables in LDS
 // (objective.py:309-313, 473).
 
+// Shift kernel E = exp(Kx*dx + Ky*dy) * mask with K = -2*pi*i*fftfreq * mask
+// (objective.py:309-313, 473).  Where the mask is 1 the exponent is a sum of
+// an x and a y term, so E = ex[px] * ey[py] * mask with the two axis factors
+// exp(-2*pi*i*fftfreq*d) tabulated once per window in LDS (2N sincos instead
+// of N^2); where the mask is 0, E is 0 either way.
 // exy[f] = exp(i * -2*pi*fftfreq(f) * dx) for f < N, exy[N + f] the same for dy
 // (indexed by frequency; the caller synchronises)
 template <int N>
@@ -1969,75 +1855,6 @@ __global__ __launch_bounds__(256) void k_gather_grad(BatchArgs a) {
   }
 }
 
-// Probe-side adjoint (objective.py:641-658, 697-703): per window and mode
-// the gradient summed over object modes is shifted back, the position
-// gradient -2/n^2 * sum Re(K * conj(FP) * Q) is accumulated in FP64, and the
-// spectrum Q (or the gradient itself when no shift is active) is summed over
-// the group's windows in registers.
-template <int N>
-__global__ __launch_bounds__(Tile<N>::NT) void k_probegrad(BatchArgs a) {
-  using T = Tile<N>;
-  PTY_TILE(T, t, a.scratch);
-  __shared__ double red[16];
-  __shared__ float2 exy[2 * N];
-  const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
-  const int tid = threadIdx.x;
-  const float2* tw = twiddles<N>();
-  const float2* fp = a.fprobe + size_t(m) * N * N;
-  // this workgroup is the only writer of its partial-sum slot
-  float2* acc = a.probe_part + (size_t(g) * a.M + m) * N * N;
-  for (int b = g; b < a.B; b += a.G) {
-    const int idx = seq_index(a.seq, a.b0 + b, a.P);
-    const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
-    if (a.shift) shift_tables<N>(exy, dy, dx);
-    // sum over object modes
-#pragma unroll 4
-    for (int i = 0; i < T::PER_THREAD; ++i) {
-      const int e = tid + i * T::NT;
-      float2 v = make_float2(0.f, 0.f);
-      for (int k = 0; k < a.K; ++k) v = cadd(v, a.gP[((size_t(b) * a.M + m) * a.K + k) * N * N + e]);
-      t[T::lds(e / N, e % N)] = v;
-    }
-    __syncthreads();
-    if (a.shift) {
-      tile_fft_fwd<N>(t, tw);
-      double sy = 0.0, sx = 0.0;
-#pragma unroll 4
-      for (int i = 0; i < T::PER_THREAD; ++i) {
-        const int e = tid + i * T::NT, py = e / N, px = e % N;
-        const float mk = a.mask[e];
-        const float2 E = shift_at(exy, N, py, px, mk);
-        const float2 Q = cmulc(t[T::lds(py, px)], E);
-        acc[e] = cadd(acc[e], Q);
-        if (a.step_pos) {
-          const float2 Z = cmulc(Q, fp[e]);  // conj(FP) * Q
-          const float kx = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(px), N, 1.f)), mk);
-          const float ky = fmul(fmul(-6.2831855f, fftfreq_f(T::pos2freq(py), N, 1.f)), mk);
-          // Re(i*k * Z) = -k * Im(Z)
-          sx += -double(kx) * double(Z.y);
-          sy += -double(ky) * double(Z.y);
-        }
-      }
-      if (a.step_pos) {
-        const double SY = block_sum(sy, red);
-        const double SX = block_sum(sx, red);
-        if (tid == 0) {
-          const double sh = -2.0 / (double(N) * double(N));
-          a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
-          a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
-        }
-      }
-    } else {
-#pragma unroll 4
-      for (int i = 0; i < T::PER_THREAD; ++i) {
-        const int e = tid + i * T::NT;
-        acc[e] = cadd(acc[e], t[T::lds(e / N, e % N)]);
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // pos_grad[idx] = sum over probe modes (assigned per window, objective.py:658)
 __global__ void k_pos_final(BatchArgs a) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2062,44 +1879,6 @@ __global__ __launch_bounds__(256) void k_probe_reduce(const float2* __restrict__
   float2 v = part[e];
   for (int g = 1; g < G; ++g) v = cadd(v, part[size_t(g) * per_group + e]);
   sum[e] = v;
-}
-
-// probe_grad = ifft2(mask * fft2(sum_b interm)) (objective.py:927-933)
-template <int N>
-__global__ __launch_bounds__(Tile<N>::NT) void k_probe_final(const float2* __restrict__ part, int G, int M,
-                                                             int spectral, const float* __restrict__ mask,
-                                                             float2* __restrict__ probe_grad,
-                                                             float2* __restrict__ scratch) {
-  using T = Tile<N>;
-  PTY_TILE(T, t, scratch);
-  const int m = blockIdx.x, tid = threadIdx.x;
-  const float2* tw = twiddles<N>();
-  const float s = 1.0f / float(N * N);
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT;
-    float2 v = make_float2(0.f, 0.f);
-    for (int g = 0; g < G; ++g) v = cadd(v, part[(size_t(g) * M + m) * N * N + e]);
-    t[T::lds(e / N, e % N)] = spectral ? cscale(v, mask[e] * s) : v;
-  }
-  __syncthreads();
-  if (!spectral) {
-    tile_fft_fwd<N>(t, tw);
-#pragma unroll 4
-    for (int i = 0; i < T::PER_THREAD; ++i) {
-      const int e = tid + i * T::NT;
-      const int l = T::lds(e / N, e % N);
-      t[l] = cscale(t[l], mask[e] * s);
-    }
-    __syncthreads();
-  }
-  tile_fft_inv<N>(t, tw);
-  float2* dst = probe_grad + size_t(m) * N * N;
-#pragma unroll 4
-  for (int i = 0; i < T::PER_THREAD; ++i) {
-    const int e = tid + i * T::NT;
-    dst[e] = t[T::lds(e / N, e % N)];
-  }
 }
 
 __global__ __launch_bounds__(256) void k_loss_final(const double* __restrict__ part, int n, double w,
@@ -2231,7 +2010,7 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 
 // ---------------------------------------------------------- workspace
 struct Layout {
-  size_t H, H0, mask, maskc, fprobe, objp, gradp, Ob, Pp, stash, gstate, Psi, coef, Gbuf, gP, probe_part,
+  size_t H, H0, maskc, fprobe, objp, gradp, Ob, Pp, stash, gstate, Psi, coef, Gbuf, gP, probe_part,
       probe_sum, pos_part, loss_part, meas_stage, scratch, total;
   int B, G;
   int ck, C;  // slices per checkpoint segment (0: every slice kept) and segments
@@ -2299,7 +2078,6 @@ Layout plan_layout(const pty_problem* p) {
   auto take = [&](size_t bytes) { size_t r = o; o += al(bytes); return r; };
   L.H = take(n2 * 8);
   L.H0 = take(n2 * 8);
-  L.mask = take(n2 * 4);
   L.maskc = take(n2 * 4);
   L.fprobe = take(size_t(M) * n2 * 8);
   L.objp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
@@ -2353,9 +2131,9 @@ int validate(const pty_problem* p) {
 
 
 template <int N>
-int launch_tables(const pty_problem* p, float2* H, float2* H0, float* mask, float* maskc, hipStream_t s) {
+int launch_tables(const pty_problem* p, float2* H, float2* H0, float* maskc, hipStream_t s) {
   TimeScope ts(KT_TABLES, s);
-  TableArgs a{H, H0, mask, maskc, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x,
+  TableArgs a{H, H0, maskc, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x,
               p->tan_y, p->damping_cutoff, p->edge_multiplier};
   k_tables<N, kLayout<N>><<<(N * N + 255) / 256, 256, 0, s>>>(a);
   return check_launch();
@@ -2376,10 +2154,8 @@ void launch_forward(const BatchArgs& a, hipStream_t s) {
   const int grid = xcd_grid(a.B, a.M * a.K);
   if constexpr (LAY == LAY_RT)
     k_forward_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
-  else if constexpr (LAY == LAY_GT)
-    k_forward_gt<N><<<grid, kGtThreads, 0, s>>>(a);
   else
-    k_forward<N><<<grid, Tile<N>::NT, 0, s>>>(a);
+    k_forward_gt<N><<<grid, kGtThreads, 0, s>>>(a);
 }
 
 template <int N>
@@ -2388,10 +2164,8 @@ void launch_adjoint(const BatchArgs& a, hipStream_t s) {
   const int grid = xcd_grid(a.B, a.M * a.K);
   if constexpr (LAY == LAY_RT)
     k_adjoint_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
-  else if constexpr (LAY == LAY_GT)
-    k_adjoint_gt<N><<<grid, kGtThreads, 0, s>>>(a);
   else
-    k_adjoint<N><<<grid, Tile<N>::NT, 0, s>>>(a);
+    k_adjoint_gt<N><<<grid, kGtThreads, 0, s>>>(a);
 }
 
 // one workgroup per (window, slice of [s_lo, s_hi), object mode)
@@ -2401,10 +2175,8 @@ void launch_objgrad(const BatchArgs& a, hipStream_t s) {
   const int grid = a.B * (a.s_hi - a.s_lo) * a.K;
   if constexpr (LAY == LAY_RT)
     k_objgrad_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
-  else if constexpr (LAY == LAY_GT)
-    k_objgrad_gt<N><<<grid, kGtThreads, 0, s>>>(a);
   else
-    k_objgrad<N><<<grid, Tile<N>::NT, 0, s>>>(a);
+    k_objgrad_gt<N><<<grid, kGtThreads, 0, s>>>(a);
 }
 
 // One batch of windows.  With checkpoint segments (ck > 0, C segments) the
@@ -2418,26 +2190,21 @@ void launch_objgrad(const BatchArgs& a, hipStream_t s) {
 // whole-pass launch computes for those slices, in the same order.
 template <int N>
 int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
-  using T = Tile<N>;
   constexpr int LAY = kLayout<N>;
   const int planes = a.S * a.K;
   {
     TimeScope ts(KT_PATCH, s);
     if constexpr (LAY == LAY_RT)
       k_patch_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
-    else if constexpr (LAY == LAY_GT)
-      k_patch_gt<N><<<a.B * planes, kGtThreads, 0, s>>>(a);
     else
-      k_patch<N><<<a.B * planes, T::NT, 0, s>>>(a);
+      k_patch_gt<N><<<a.B * planes, kGtThreads, 0, s>>>(a);
   }
   if (a.shift) {
     TimeScope ts(KT_PROBE_SHIFT, s);
     if constexpr (LAY == LAY_RT)
       k_probe_shift_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
-    else if constexpr (LAY == LAY_GT)
-      k_probe_shift_gt<N><<<a.B * a.M, kGtThreads, 0, s>>>(a);
     else
-      k_probe_shift<N><<<a.B * a.M, T::NT, 0, s>>>(a);
+      k_probe_shift_gt<N><<<a.B * a.M, kGtThreads, 0, s>>>(a);
   }
   {
     TimeScope ts(KT_FORWARD, s);
@@ -2496,10 +2263,8 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
     TimeScope ts(KT_PROBEGRAD, s);
     if constexpr (LAY == LAY_RT)
       k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
-    else if constexpr (LAY == LAY_GT)
-      k_probegrad_gt<N><<<a.G * a.M, kGtThreads, 0, s>>>(a);
     else
-      k_probegrad<N><<<a.G * a.M, T::NT, 0, s>>>(a);
+      k_probegrad_gt<N><<<a.G * a.M, kGtThreads, 0, s>>>(a);
   }
   if (p->step_pos && a.shift) {
     TimeScope ts(KT_POS_FINAL, s);
@@ -2510,7 +2275,6 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
 
 template <int N>
 int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_t s) {
-  using T = Tile<N>;
   const Layout L = plan_layout(p);
   if (!bf->workspace || bf->workspace_bytes < L.total) return PTY_ERR_WORKSPACE;
   char* ws = static_cast<char*>(bf->workspace);
@@ -2520,20 +2284,16 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   int rc;
   float2* H = reinterpret_cast<float2*>(ws + L.H);
   float2* H0 = reinterpret_cast<float2*>(ws + L.H0);
-  float* mask = reinterpret_cast<float*>(ws + L.mask);
   float* maskc = reinterpret_cast<float*>(ws + L.maskc);
-  if ((rc = launch_tables<N>(p, H, H0, mask, maskc, s))) return rc;
+  if ((rc = launch_tables<N>(p, H, H0, maskc, s))) return rc;
   float2* fprobe = reinterpret_cast<float2*>(ws + L.fprobe);
   {
     TimeScope ts(KT_FPROBE, s);
     if constexpr (kLayout<N> == LAY_RT)
       k_fprobe_rt<N><<<M, RT<N>::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe);
-    else if constexpr (kLayout<N> == LAY_GT)
+    else
       k_fprobe_gt<N><<<M, kGtThreads, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
                                              reinterpret_cast<float2*>(ws + L.scratch));
-    else
-      k_fprobe<N><<<M, T::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
-                                     reinterpret_cast<float2*>(ws + L.scratch));
   }
   const float2* objp = static_cast<const float2*>(bf->obj);
   float2* gradp = static_cast<float2*>(bf->obj_grad);
@@ -2562,7 +2322,7 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.loss_weight = p->loss_weight;
   a.obj = objp; a.probe = static_cast<const float2*>(bf->probe); a.fprobe = fprobe;
   a.pos = bf->positions; a.corr = bf->corrections; a.meas = bf->measured; a.seq = bf->sequence;
-  a.H = H; a.H0 = H0; a.mask = mask; a.maskc = maskc;
+  a.H = H; a.H0 = H0; a.maskc = maskc;
   a.Ob = reinterpret_cast<float2*>(ws + L.Ob); a.Pp = reinterpret_cast<float2*>(ws + L.Pp);
   a.stash = reinterpret_cast<float2*>(ws + L.stash); a.Psi = reinterpret_cast<float2*>(ws + L.Psi);
   a.coef = reinterpret_cast<float*>(ws + L.coef); a.Gbuf = nullptr;
@@ -2606,12 +2366,9 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     if constexpr (kLayout<N> == LAY_RT)
       k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, p->shift_active, maskc,
                                                    static_cast<float2*>(bf->probe_grad));
-    else if constexpr (kLayout<N> == LAY_GT)
+    else
       k_probe_final_gt<N><<<M, kGtThreads, 0, s>>>(psum, p->shift_active, maskc,
                                                    static_cast<float2*>(bf->probe_grad), a.scratch);
-    else
-      k_probe_final<N><<<M, T::NT, 0, s>>>(psum, 1, M, p->shift_active, mask,
-                                          static_cast<float2*>(bf->probe_grad), a.scratch);
   }
   if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE) {
     TimeScope ts(KT_LAYOUT, s);
