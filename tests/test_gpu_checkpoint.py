@@ -1,9 +1,9 @@
 """Slice checkpointing: a pass that keeps only every k-th slice's wave and recomputes each segment in
 the adjoint gives the loss and every gradient of the pass that keeps all slices, bit for bit (the
 recompute runs the same instructions on the same stored values, and the wave crossing a segment
-boundary is stored exactly).  Covers the LDS tile (32, 96 px), the register tile (64, 128 px) and
-the global-slab tile (192, 256 px), uneven last segments, object modes, several batches per pass and
-the degenerate k = 1."""
+boundary is stored exactly).  Covers the register tile with four threads per line (32, 96 px) and
+with eight (64, 128 px), the global-slab tile (192, 256 px), uneven last segments, object modes,
+several batches per pass and the degenerate k = 1."""
 import numpy as np
 import pytest
 
