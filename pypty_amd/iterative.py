@@ -482,13 +482,14 @@ def run(pypty_params):
     prev_steps = 0
     losses = []
     is_first = True
-    actual_step_default = get("update_step_bfgs", 1)
+    step_param = get("update_step_bfgs", 1)  # first line-search step, a value or a callable of the epoch
     epoch_callback = get("epoch_callback", None)
     for epoch in range(int(get("epoch_prev", 0)), int(get("epoch_max", 200))):
         warnings = ""
         s_obj = _per_epoch(get("update_obj", 1), epoch)
         s_probe = _per_epoch(get("update_probe", 1), epoch)
         s_pos = _per_epoch(get("update_probe_pos", 0), epoch)
+        first_step = _per_epoch(step_param, epoch)  # per epoch, as pypty/iterative.py:325-336 reads it
         # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, probe after 6
         if counter_failed >= 2:
             fail_pos = epoch
@@ -533,7 +534,7 @@ def run(pypty_params):
             current_window_weight=0, dynamically_resize_yx_object=False, lazy_clean=False,
             current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=0,
             xp=None, remove_fft_cache=False, is_first_epoch=is_first,
-            hist_length=_per_epoch(get("hist_length", 10), epoch), actual_step=actual_step_default,
+            hist_length=_per_epoch(get("hist_length", 10), epoch), actual_step=first_step,
             fast_axis_reg_weight_positions=0, fast_axis_reg_weight_tilts=0, slow_axis_reg_weight_positions=0,
             slow_axis_reg_weight_tilts=0, scan_size=None, scan_step_A=1.0, current_deformation_reg_weight_positions=0,
             current_deformation_reg_weight_tilts=0, warnings=warnings, data_simulation_flag=False,
@@ -551,7 +552,7 @@ def run(pypty_params):
         if save_log:
             free, total = torch.cuda.mem_get_info(dev)
             _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,
-                               "initial step": actual_step_default, "matching step": actual_step,
+                               "initial step": first_step, "matching step": actual_step,
                                "N linesearch iterations": count + 1, "dir. derivative": d_value,
                                "new dir. derivative": new_d_value, "Constraints contribution": 0.0,
                                "Free GiB": free / 1024 ** 3, "Total GiB": total / 1024 ** 3, "Warnings": warnings})

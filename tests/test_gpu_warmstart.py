@@ -1,7 +1,8 @@
 """The learned warm start end to end: train the network on the GPU on patterns the multislice kernels
 simulate, stitch its predictions for a held-out multislice dataset (different object and probes),
-and show what it gains over the flat start: a lower starting NMSE, and the plain reconstruction's
-final NMSE reached in fewer L-BFGS epochs from the warm start."""
+and show what it gains over the flat start: a lower starting NMSE, and, with a constant small first
+line-search step (update_step_bfgs = 1e-3), the plain reconstruction's final NMSE reached in fewer
+L-BFGS epochs from the warm start."""
 import numpy as np
 import pytest
 
