@@ -174,6 +174,7 @@ struct TableArgs {
   float2* H;
   float2* H0;
   float* maskc;
+  int* zcol;  // (N) 1 where column fx of the band mask is zero for every fy
   float dz, lambda, px, py, tx, ty, cutoff, edge;
 };
 // float32 products/sums without contraction, in numpy's order
@@ -241,6 +242,11 @@ __global__ void k_tables(TableArgs a) {
     a.H0[e] = a.H[e];
   }
   a.maskc[e] = m;
+  if (e < N && a.zcol) {
+    unsigned any = 0;
+    for (int fy = 0; fy < N; ++fy) any |= unsigned(band_mask<N>(a, fy, e) != 0.f);
+    a.zcol[e] = any ? 0 : 1;
+  }
 }
 
 // Position-order tables from a caller-supplied propagator (natural order,
@@ -322,6 +328,7 @@ struct BatchArgs {
   const float2* H0;
   const float* mask;        // position order
   const float* maskc;       // order of H (C-order for the register tile)
+  const int* zcol;          // (N) columns fx where the band mask, H and H0 are zero, or null
   // per-batch buffers
   float2* Ob;               // (B, S, K, N, N) band-limited windows
   float2* Pp;               // (B, M, N, N) shifted probes
@@ -825,6 +832,44 @@ struct RView {
   }
 };
 
+// Zero columns.  k_tables marks in zcol[fx] the spectrum columns fx in which
+// the band mask, and with it H and H0, is zero for every fy; the default
+// mask, a disc of radius 2/3 of the largest frequency, zeroes about a third
+// of them.  A spectrum multiplied by these tables is exactly zero there, and
+// so is its inverse column transform: a wavefront whose columns are all
+// marked skips their transforms.  col is the thread's column position in the
+// column phase (frequency col_freq(col)).
+template <int N>
+__device__ __forceinline__ bool wave_cols_zero(const int* zcol, int col) {
+  return zcol != nullptr && __ballot(zcol[RT<N>::col_freq(col)] == 0) == 0ull;
+}
+
+// Band limit of a window held in the row phase, in place: forward 2D
+// transform, times maskc / N^2, inverse 2D transform.  Wavefronts whose
+// columns the mask zeroes skip their column transforms.
+template <int N>
+__device__ __forceinline__ void rt_band_limit(float2* v, float2* T, const RTIds<N>& id, const float* maskc,
+                                              const int* zcol) {
+  using R = RT<N>;
+  const unsigned tid = threadIdx.x;
+  const bool zc = wave_cols_zero<N>(zcol, id.col);
+  rt_line_fwd<N, true>(v, T, id.row, id.rj);
+  rt_rows_to_cols<N>(v, T, id);
+  if (zc) {
+#pragma unroll
+    for (int i = 0; i < R::R1; ++i) v[i] = make_float2(0.f, 0.f);
+  } else {
+    rt_line_fwd<N, false>(v, T, id.col, id.cj);
+    const float sc = 1.0f / float(N * N);
+    const RView<N, float> mask(maskc, tid);
+#pragma unroll
+    for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], mask[i] * sc);
+    rt_line_inv<N, false>(v, T, id.col, id.cj);
+  }
+  rt_cols_to_rows<N>(v, T, id);
+  rt_line_inv<N, true>(v, T, id.row, id.rj);
+}
+
 // k_patch over the register tile (objective.py:436-441)
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
@@ -841,12 +886,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
   const float2* src = a.obj + size_t(plane) * a.ny * a.nx + (inside ? size_t(y0 + id.row) * a.nx + x0 + id.rj : 0);
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) v[n] = inside ? src[R::T * n] : make_float2(0.f, 0.f);
-  rt_fft2_fwd<N>(v, T, id);
-  const float sc = 1.0f / float(N * N);
-  const RView<N, float> mask(a.maskc, tid);
-#pragma unroll
-  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], mask[i] * sc);
-  rt_fft2_inv<N>(v, T, id);
+  rt_band_limit<N>(v, T, id, a.maskc, a.zcol);
   const RView<N, float2> dst(a.Ob + (size_t(b) * planes + plane) * N * N, tid);
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) dst.store(n, v[n]);
@@ -885,6 +925,9 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
     for (int n = 0; n < R::R1; ++n) o[n] = O[n];
   }
   float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  // wavefronts whose columns the propagators (the band mask) zero skip the
+  // column transforms of the slices (of the far field)
+  const bool zh = wave_cols_zero<N>(a.zcol, id.col), zm = zh;
   for (int s = a.s_lo; s < a.s_hi; ++s) {
     const int slot = a.write_stash ? stash_slot(a, s) : -1;
     if (slot >= 0) {
@@ -897,27 +940,41 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
     if (s < a.s_hi - 1) {
       rt_line_fwd<N, true>(v, T, id.row, id.rj);
       rt_rows_to_cols<N>(v, T, id);
-      const RView<N, float2> H((s == 0) ? a.H0 : a.H, tid);
-      float2 h[R::R1];
-#pragma unroll
-      for (int i = 0; i < R::R1; ++i) h[i] = H[i];
-      rt_line_fwd<N, false>(v, T, id.col, id.cj);
-#pragma unroll
-      for (int i = 0; i < R::R1; ++i) v[i] = cmul(v[i], h[i]);
       const RView<N, float2> O(ob + size_t(s + 1) * ostride, tid);
+      if (zh) {
 #pragma unroll
-      for (int n = 0; n < R::R1; ++n) o[n] = O[n];
-      rt_line_inv<N, false>(v, T, id.col, id.cj);
+        for (int n = 0; n < R::R1; ++n) o[n] = O[n];
+#pragma unroll
+        for (int i = 0; i < R::R1; ++i) v[i] = make_float2(0.f, 0.f);
+      } else {
+        const RView<N, float2> H((s == 0) ? a.H0 : a.H, tid);
+        float2 h[R::R1];
+#pragma unroll
+        for (int i = 0; i < R::R1; ++i) h[i] = H[i];
+        rt_line_fwd<N, false>(v, T, id.col, id.cj);
+#pragma unroll
+        for (int i = 0; i < R::R1; ++i) v[i] = cmul(v[i], h[i]);
+#pragma unroll
+        for (int n = 0; n < R::R1; ++n) o[n] = O[n];
+        rt_line_inv<N, false>(v, T, id.col, id.cj);
+      }
       rt_cols_to_rows<N>(v, T, id);
       rt_line_inv<N, true>(v, T, id.row, id.rj);
     }
   }
   if (a.fin) return;  // segment recompute: the stash is the result
-  rt_fft2_fwd<N>(v, T, id);
+  rt_line_fwd<N, true>(v, T, id.row, id.rj);
+  rt_rows_to_cols<N>(v, T, id);
   const RView<N, float2> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
-  const RView<N, float> mask(a.maskc, tid);
+  if (zm) {
 #pragma unroll
-  for (int i = 0; i < R::R1; ++i) out.store(i, cscale(v[i], mask[i]));
+    for (int i = 0; i < R::R1; ++i) out.store(i, make_float2(0.f, 0.f));
+  } else {
+    rt_line_fwd<N, false>(v, T, id.col, id.cj);
+    const RView<N, float> mask(a.maskc, tid);
+#pragma unroll
+    for (int i = 0; i < R::R1; ++i) out.store(i, cscale(v[i], mask[i]));
+  }
 }
 
 // k_adjoint over the register tile (objective.py:595-597, multislice_core.py:85-200)
@@ -934,17 +991,27 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
   const RTIds<N> id;
   const float sc = 1.0f / float(N * N);
   float2 v[R::R1];
+  // as in the forward: columns zero in the propagators, and in the band mask
+  // (where the far field, and so its gradient, is zero)
+  const bool zh = wave_cols_zero<N>(a.zcol, id.col);
   if (a.s_hi < a.S) {
     // a later segment left the wave of slice s_hi - 1
     const RView<N, float2> gs(a.gstate + (size_t(b) * MK + mk) * N * N, tid);
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) v[n] = gs[n];
   } else {
-    const RView<N, float2> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
-    const RView<N, float> c(a.coef + size_t(b) * N * N, tid);
+    if (zh) {
 #pragma unroll
-    for (int i = 0; i < R::R1; ++i) v[i] = cscale(psi[i], c[i] * sc);
-    rt_fft2_inv<N>(v, T, id);
+      for (int i = 0; i < R::R1; ++i) v[i] = make_float2(0.f, 0.f);
+    } else {
+      const RView<N, float2> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
+      const RView<N, float> c(a.coef + size_t(b) * N * N, tid);
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) v[i] = cscale(psi[i], c[i] * sc);
+      rt_line_inv<N, false>(v, T, id.col, id.cj);
+    }
+    rt_cols_to_rows<N>(v, T, id);
+    rt_line_inv<N, true>(v, T, id.row, id.rj);
   }
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g (k_objgrad_rt sums the modes); pu holds slot s, read one
@@ -970,19 +1037,24 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
     if (s > 0) {
       rt_line_fwd<N, true>(v, T, id.row, id.rj);
       rt_rows_to_cols<N>(v, T, id);
-      const RView<N, float2> H((s == 1) ? a.H0 : a.H, tid);
-      float2 h[R::R1];
+      if (zh) {
 #pragma unroll
-      for (int i = 0; i < R::R1; ++i) h[i] = H[i];
-      rt_line_fwd<N, false>(v, T, id.col, id.cj);
+        for (int i = 0; i < R::R1; ++i) v[i] = make_float2(0.f, 0.f);
+      } else {
+        const RView<N, float2> H((s == 1) ? a.H0 : a.H, tid);
+        float2 h[R::R1];
 #pragma unroll
-      for (int i = 0; i < R::R1; ++i) v[i] = cmulc(v[i], h[i]);
+        for (int i = 0; i < R::R1; ++i) h[i] = H[i];
+        rt_line_fwd<N, false>(v, T, id.col, id.cj);
+#pragma unroll
+        for (int i = 0; i < R::R1; ++i) v[i] = cmulc(v[i], h[i]);
+      }
       if (a.step_obj && s > a.s_lo) {
         const RView<N, float2> U(st + size_t(s - 1 - a.s_lo) * N * N, tid);
 #pragma unroll
         for (int n = 0; n < R::R1; ++n) pu[n] = U[n];
       }
-      rt_line_inv<N, false>(v, T, id.col, id.cj);
+      if (!zh) rt_line_inv<N, false>(v, T, id.col, id.cj);
       rt_cols_to_rows<N>(v, T, id);
       rt_line_inv<N, true>(v, T, id.row, id.rj);
     }
@@ -1018,12 +1090,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) v[n] = cadd(v[n], Gm[n]);
   }
-  rt_fft2_fwd<N>(v, T, id);
-  const float sc = 1.0f / float(N * N);
-  const RView<N, float> mask(a.maskc, tid);
-#pragma unroll
-  for (int i = 0; i < R::R1; ++i) v[i] = cscale(v[i], mask[i] * sc);
-  rt_fft2_inv<N>(v, T, id);
+  rt_band_limit<N>(v, T, id, a.maskc, a.zcol);
   float2* G = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) G[id.row * N + id.rj + R::T * n] = v[n];
@@ -1086,14 +1153,22 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
   const RView<N, float> mask(a.maskc, tid);
   const float s = 1.0f / float(N * N);
   float2 v[R::R1];
+  // E, and so the spectrum, is zero on the columns the band mask zeroes
+  if (wave_cols_zero<N>(a.zcol, id.col)) {
 #pragma unroll
-  for (int i = 0; i < R::R1; ++i) {
-    int fy, fx;
-    R::cspec(i * R::NT + tid, fy, fx);
-    const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mask[i]);
-    v[i] = cscale(cmul(fp[i], E), s);
+    for (int i = 0; i < R::R1; ++i) v[i] = make_float2(0.f, 0.f);
+  } else {
+#pragma unroll
+    for (int i = 0; i < R::R1; ++i) {
+      int fy, fx;
+      R::cspec(i * R::NT + tid, fy, fx);
+      const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mask[i]);
+      v[i] = cscale(cmul(fp[i], E), s);
+    }
+    rt_line_inv<N, false>(v, T, id.col, id.cj);
   }
-  rt_fft2_inv<N>(v, T, id);
+  rt_cols_to_rows<N>(v, T, id);
+  rt_line_inv<N, true>(v, T, id.row, id.rj);
   float2* dst = a.Pp + (size_t(b) * a.M + m) * N * N;
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + R::T * n] = v[n];
@@ -1266,14 +1341,36 @@ __device__ __forceinline__ void gt_store_col(const float2* v, const Slab<N>& sl,
   for (int n = 0; n < G::R1; ++n) sl.st(L.vcol, (8 * n * N + r * G::LINES) * 8, v[n]);
 }
 
-// column phase of a band limit: forward column FFT, times maskc/N^2, inverse
+// bit r set: the wavefront's columns of round r are all marked zero
+template <int N>
+__device__ __forceinline__ unsigned gt_cols_zero(const int* zcol, const GTLane<N>& L) {
+  using G = GT<N, kGtThreads>;
+  unsigned z = 0;
+#pragma unroll
+  for (int r = 0; r < G::ROUNDS; ++r) z |= unsigned(wave_cols_zero<N>(zcol, r * G::LINES + L.ln)) << r;
+  return z;
+}
+template <int N>
+__device__ __forceinline__ void gt_zero_col(float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
+#pragma unroll
+  for (int i = 0; i < GT<N, kGtThreads>::R1; ++i) v[i] = make_float2(0.f, 0.f);
+  gt_store_col<N>(v, sl, L, r);
+}
+
+// column phase of a band limit: forward column FFT, times maskc/N^2, inverse;
+// zero columns of the mask are written as zeros
 template <int N>
 __device__ __forceinline__ void gt_band_columns(float2* v, float2* X, const Slab<N>& sl, const GTLane<N>& L,
-                                                const float* maskc) {
+                                                const float* maskc, const int* zcol) {
   using G = GT<N, kGtThreads>;
   const RView<N, float, G::NT> mask(maskc, threadIdx.x);
   const float sc = 1.0f / float(N * N);
+  const unsigned zm = gt_cols_zero<N>(zcol, L);
   for (int r = 0; r < G::ROUNDS; ++r) {
+    if ((zm >> r) & 1u) {
+      gt_zero_col<N>(v, sl, L, r);
+      continue;
+    }
     gt_load_col<N>(v, sl, L, r);
     rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
@@ -1306,7 +1403,7 @@ __global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
     gt_store_rowspec<N>(v, sl, L, r);
   }
   __syncthreads();
-  gt_band_columns<N>(v, X, sl, L, a.maskc);
+  gt_band_columns<N>(v, X, sl, L, a.maskc, a.zcol);
   __syncthreads();
   const RView<N, float2, G::NT> dst(a.Ob + (size_t(b) * planes + plane) * N * N, threadIdx.x);
   for (int r = 0; r < G::ROUNDS; ++r) {
@@ -1334,6 +1431,8 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
   float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
   const float2* fin = a.fin ? a.fin + (size_t(b) * MK + mk) * a.fin_stride : nullptr;
+  // rounds whose columns the propagators (the band mask) zero
+  const unsigned zh = gt_cols_zero<N>(a.zcol, L), zm = zh;
   // o holds the object registers of the current round; those of the next
   // round (or the next slice's first round) are read while this round's
   // forward row transform runs, and the propagator is read before the column
@@ -1385,6 +1484,10 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
     if (s < a.s_hi - 1) {
       const RView<N, float2, G::NT> H((s == 0) ? a.H0 : a.H, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        if ((zh >> r) & 1u) {
+          gt_zero_col<N>(v, sl, L, r);
+          continue;
+        }
         float2 h[G::R1];
 #pragma unroll
         for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
@@ -1400,6 +1503,11 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
       const RView<N, float2, G::NT> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
       const RView<N, float, G::NT> mask(a.maskc, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        if ((zm >> r) & 1u) {
+#pragma unroll
+          for (int i = 0; i < G::R1; ++i) out.store(r * G::R1 + i, make_float2(0.f, 0.f));
+          continue;
+        }
         gt_load_col<N>(v, sl, L, r);
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
@@ -1426,12 +1534,19 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
   // a later segment left the wave of slice s_hi - 1 (read in the first row phase)
   const bool from_state = a.s_hi < a.S;
   const RView<N, float2, G::NT> GS(a.gstate ? a.gstate + (size_t(b) * MK + mk) * N * N : a.Psi, tid);
+  // as in the forward: columns zero in the propagators, and in the band mask
+  // (where the far field, and so its gradient, is zero)
+  const unsigned zh = gt_cols_zero<N>(a.zcol, L);
   if (!from_state) {
     // far-field adjoint: inverse column transforms of coef * Psi / N^2
     const RView<N, float2, G::NT> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
     const RView<N, float, G::NT> c(a.coef + size_t(b) * N * N, tid);
     const float sc = 1.0f / float(N * N);
     for (int r = 0; r < G::ROUNDS; ++r) {
+      if ((zh >> r) & 1u) {
+        gt_zero_col<N>(v, sl, L, r);
+        continue;
+      }
 #pragma unroll
       for (int i = 0; i < G::R1; ++i) v[i] = cscale(psi[r * G::R1 + i], c[r * G::R1 + i] * sc);
       rt_line_inv<N, true>(v, X, L.ln, L.j);
@@ -1485,6 +1600,10 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
       __syncthreads();
       const RView<N, float2, G::NT> H((s == 1) ? a.H0 : a.H, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        if ((zh >> r) & 1u) {
+          gt_zero_col<N>(v, sl, L, r);
+          continue;
+        }
         float2 h[G::R1];
 #pragma unroll
         for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
@@ -1539,7 +1658,7 @@ __global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
     gt_store_rowspec<N>(v, sl, L, r);
   }
   __syncthreads();
-  gt_band_columns<N>(v, X, sl, L, a.maskc);
+  gt_band_columns<N>(v, X, sl, L, a.maskc, a.zcol);
   __syncthreads();
   float2* Gout = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
   for (int r = 0; r < G::ROUNDS; ++r) {
@@ -1602,7 +1721,13 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
   const RView<N, float, G::NT> mask(a.maskc, tid);
   const float s = 1.0f / float(N * N);
   float2 v[G::R1];
+  // E, and so the spectrum, is zero on the columns the band mask zeroes
+  const unsigned zm = gt_cols_zero<N>(a.zcol, L);
   for (int r = 0; r < G::ROUNDS; ++r) {
+    if ((zm >> r) & 1u) {
+      gt_zero_col<N>(v, sl, L, r);
+      continue;
+    }
 #pragma unroll
     for (int i = 0; i < G::R1; ++i) {
       const int q = r * G::R1 + i;
@@ -1644,6 +1769,8 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
   const RView<N, float2, G::NT> fp(a.fprobe + size_t(m) * N * N, tid);
   const RView<N, float, G::NT> mask(a.maskc, tid);
   float2 v[G::R1];
+  // Q = conj(E) * spectrum is zero on the columns the band mask zeroes
+  const unsigned zm = gt_cols_zero<N>(a.zcol, L);
   for (int b = g; b < a.B; b += a.G) {
     const int idx = seq_index(a.seq, a.b0 + b, a.P);
     const float2* gp = a.gP + (size_t(b) * a.M + m) * a.K * N * N;
@@ -1676,6 +1803,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
     __syncthreads();  // the slab holds the row spectra; the table is published
     double sy = 0.0, sx = 0.0;
     for (int r = 0; r < G::ROUNDS; ++r) {
+      if ((zm >> r) & 1u) continue;
       gt_load_col<N>(v, sl, L, r);
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
@@ -2010,7 +2138,7 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 
 // ---------------------------------------------------------- workspace
 struct Layout {
-  size_t H, H0, maskc, fprobe, objp, gradp, Ob, Pp, stash, gstate, Psi, coef, Gbuf, gP, probe_part,
+  size_t H, H0, maskc, zcol, fprobe, objp, gradp, Ob, Pp, stash, gstate, Psi, coef, Gbuf, gP, probe_part,
       probe_sum, pos_part, loss_part, meas_stage, scratch, total;
   int B, G;
   int ck, C;  // slices per checkpoint segment (0: every slice kept) and segments
@@ -2079,6 +2207,7 @@ Layout plan_layout(const pty_problem* p) {
   L.H = take(n2 * 8);
   L.H0 = take(n2 * 8);
   L.maskc = take(n2 * 4);
+  L.zcol = take(size_t(p->n) * 4);
   L.fprobe = take(size_t(M) * n2 * 8);
   L.objp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
   L.gradp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
@@ -2131,9 +2260,9 @@ int validate(const pty_problem* p) {
 
 
 template <int N>
-int launch_tables(const pty_problem* p, float2* H, float2* H0, float* maskc, hipStream_t s) {
+int launch_tables(const pty_problem* p, float2* H, float2* H0, float* maskc, int* zcol, hipStream_t s) {
   TimeScope ts(KT_TABLES, s);
-  TableArgs a{H, H0, maskc, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x,
+  TableArgs a{H, H0, maskc, zcol, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x,
               p->tan_y, p->damping_cutoff, p->edge_multiplier};
   k_tables<N, kLayout<N>><<<(N * N + 255) / 256, 256, 0, s>>>(a);
   return check_launch();
@@ -2285,7 +2414,8 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   float2* H = reinterpret_cast<float2*>(ws + L.H);
   float2* H0 = reinterpret_cast<float2*>(ws + L.H0);
   float* maskc = reinterpret_cast<float*>(ws + L.maskc);
-  if ((rc = launch_tables<N>(p, H, H0, maskc, s))) return rc;
+  int* zcol = reinterpret_cast<int*>(ws + L.zcol);
+  if ((rc = launch_tables<N>(p, H, H0, maskc, zcol, s))) return rc;
   float2* fprobe = reinterpret_cast<float2*>(ws + L.fprobe);
   {
     TimeScope ts(KT_FPROBE, s);
@@ -2322,7 +2452,7 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.loss_weight = p->loss_weight;
   a.obj = objp; a.probe = static_cast<const float2*>(bf->probe); a.fprobe = fprobe;
   a.pos = bf->positions; a.corr = bf->corrections; a.meas = bf->measured; a.seq = bf->sequence;
-  a.H = H; a.H0 = H0; a.maskc = maskc;
+  a.H = H; a.H0 = H0; a.maskc = maskc; a.zcol = zcol;
   a.Ob = reinterpret_cast<float2*>(ws + L.Ob); a.Pp = reinterpret_cast<float2*>(ws + L.Pp);
   a.stash = reinterpret_cast<float2*>(ws + L.stash); a.Psi = reinterpret_cast<float2*>(ws + L.Psi);
   a.coef = reinterpret_cast<float*>(ws + L.coef); a.Gbuf = nullptr;
