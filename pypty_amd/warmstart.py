@@ -245,27 +245,38 @@ def simulate_pairs(n_pairs, n=128, patch=32, n_slices=10, pixel_A=0.2, dz_A=2.0,
     return torch.cat(xs), torch.cat(ts)
 
 
-def warm_start_object(net: WarmStartNet, measured, corners, obj_shape, sigma_px=None):
+def warm_start_object(net: WarmStartNet, measured, corners, obj_shape, sigma_px=None, chunk=256):
     """Initial object (Ny, Nx, S, K) complex64 from the network's phase patches: overlapping
     patches are averaged with a Gaussian weight around each patch centre (sigma_px, default an
     eighth of the patch; the network is surest near the probe), the projected phase is spread
-    evenly over the slices, and pixels that no patch covers keep the flat start (phase 0)."""
+    evenly over the slices, and pixels that no patch covers keep the flat start (phase 0).
+
+    Everything runs on the network's device: patterns go up `chunk` at a time and are predicted
+    as they arrive; the weighted patches (w * phase, w) are summed as one complex scatter-add
+    (``pty_scatteradd_c64``) into a frame padded by a patch on every side, so patches that reach
+    past the object edge are clipped."""
     ny, nx, S, K = obj_shape
-    n, patch = net.n_in, net.patch
-    meas = measured if isinstance(measured, torch.Tensor) else torch.from_numpy(np.asarray(measured, np.float32))
-    pred = net.predict(log_scale(meas.to(net.dev).float())).cpu().numpy().reshape(-1, patch, patch)
-    acc = np.zeros((ny, nx), np.float64)
-    cnt = np.zeros((ny, nx), np.float64)
-    c0 = (n - patch) // 2
-    iy, ix = np.meshgrid(np.arange(patch), np.arange(patch), indexing="ij")
+    n, patch, dev = net.n_in, net.patch, net.dev
+    corners = np.asarray(corners)
+    P = len(corners)
+    pred = torch.empty((P, patch * patch), dtype=torch.float32, device=dev)
+    for i in range(0, P, chunk):
+        m = measured[i:i + chunk]
+        m = m if isinstance(m, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(m, np.float32))
+        pred[i:i + m.shape[0]] = net.forward(log_scale(m.to(dev).float()))
     sig = patch / 8.0 if sigma_px is None else float(sigma_px)
-    r2 = (iy - (patch - 1) / 2.0) ** 2 + (ix - (patch - 1) / 2.0) ** 2
-    w = np.exp(-r2 / (2 * sig * sig))
-    for (y0, x0), p in zip(np.asarray(corners), pred):
-        ys, xs = y0 + c0 + iy, x0 + c0 + ix
-        ok = (ys >= 0) & (ys < ny) & (xs >= 0) & (xs < nx)
-        np.add.at(acc, (ys[ok], xs[ok]), (w * p)[ok])
-        np.add.at(cnt, (ys[ok], xs[ok]), w[ok])
-    phase = np.where(cnt > 1e-6, acc / np.maximum(cnt, 1e-6), 0.0)
-    obj = np.exp(1j * phase / (S * K))[:, :, None, None] * np.ones((1, 1, S, K))
-    return obj.astype(np.complex64)
+    r = torch.arange(patch, device=dev, dtype=torch.float32) - (patch - 1) / 2.0
+    w = torch.exp(-(r[:, None] ** 2 + r[None, :] ** 2) / (2 * sig * sig))
+    wp = pred.view(P, patch, patch) * w
+    patches = torch.complex(wp, w.expand_as(wp)).contiguous()  # (P, 1 plane, patch, patch)
+    pad, c0 = patch, (n - patch) // 2
+    full = torch.zeros((ny + 2 * pad, nx + 2 * pad), dtype=torch.complex64, device=dev)
+    cor = torch.from_numpy((corners.astype(np.int64) + c0 + pad).astype(np.int32)).to(dev).contiguous()
+    rc = net.lib.pty_scatteradd_c64(_p(full), ny + 2 * pad, nx + 2 * pad, 1, _lib.PTY_LAYOUT_PLANAR, _p(patches),
+                                    P, patch, _p(cor), _lib.stream_ptr(dev))
+    _lib.check(rc, "warm_start_object scatter-add")
+    full = full[pad:pad + ny, pad:pad + nx]
+    acc, cnt = full.real, full.imag
+    phase = torch.where(cnt > 1e-6, acc / torch.clamp(cnt, min=1e-6), torch.zeros_like(acc))
+    obj = torch.polar(torch.ones_like(phase), phase / (S * K))
+    return obj[:, :, None, None].expand(ny, nx, S, K).cpu().numpy()
