@@ -393,7 +393,8 @@ def run(pypty_params):
     """Iterative reconstruction (mirror of pypty.iterative.run, pypty/iterative.py:27-436).
 
     Reads the keys of the far-field multislice path with the reference's
-    defaults: dataset (array, or data_path to a .npy), probe (N, N, M),
+    defaults: dataset (array, a torch tensor used in place on the device, or
+    data_path to a .npy), probe (N, N, M),
     positions (P, 2) in pixels, obj, acc_voltage, pixel_size_x_A/y_A,
     slice_distances, num_slices, algorithm, epoch_max, epoch_prev, update_obj,
     update_probe, update_probe_pos (values or callables of the epoch),
@@ -422,7 +423,11 @@ def run(pypty_params):
         if not path.endswith(".npy"):
             raise NotImplementedError("pypty_amd.run: data_path must be a .npy file (HDF5 is not supported)")
         dataset = np.load(path)
-    dataset = np.asarray(dataset, dtype=np.float32)
+    if isinstance(dataset, torch.Tensor):
+        # already resident (for example uploaded once for a warm start): used in place on the device
+        dataset = dataset.to(dev, torch.float32).contiguous()
+    else:
+        dataset = np.asarray(dataset, dtype=np.float32)
     if dataset.ndim == 4:
         dataset = dataset.reshape(dataset.shape[0] * dataset.shape[1], dataset.shape[2], dataset.shape[3])
     if get("probe", None) is None:
@@ -464,7 +469,7 @@ def run(pypty_params):
     positions_correction = torch.from_numpy(corr.astype(np.float32)).to(dev)
     tilts = np.zeros((1, 6), np.float32)
     tilts_correction = np.zeros((1, 6), np.float32)
-    measured = torch.from_numpy(dataset).to(dev)
+    measured = dataset if isinstance(dataset, torch.Tensor) else torch.from_numpy(dataset).to(dev)
     folder = get("output_folder", "")
     print_flag = get("print_flag", 3)
     save_inter = get("save_inter_checkpoints", True)
