@@ -130,6 +130,18 @@ typedef struct pty_problem {
    * more forward pass of compute; the results are bit-identical to those
    * without checkpointing.  0 keeps every slice. */
   int checkpoint_slices;
+  /* Batch lanes (no counterpart in the reference, which runs its batches one
+   * after another, objective.py:383-725): with lanes = 2 consecutive batches
+   * alternate between the caller's stream and a library stream of the same
+   * device and default priority, each lane with its own per-batch buffers,
+   * so that one batch's memory-bound kernels overlap the next one's
+   * compute-bound kernels.  The kernels that add into the object, probe and
+   * position gradients still run in batch order (events order them across
+   * the lanes), so the results are bit-identical to lanes = 1.  The call
+   * returns with all work joined to the caller's stream.  A call on a
+   * stream that is being captured runs one lane.  0 or 1: one stream.  The
+   * workspace holds one set of per-batch buffers per lane. */
+  int lanes;
 } pty_problem;
 
 typedef struct pty_buffers {

@@ -43,7 +43,7 @@ class PtyProblem(ctypes.Structure):
         ("edge_multiplier", ctypes.c_float), ("n_masks", ctypes.c_int), ("raw_n", ctypes.c_int),
         ("data_shift_y", ctypes.c_int), ("data_shift_x", ctypes.c_int), ("data_bin", ctypes.c_int),
         ("data_pad", ctypes.c_int), ("data_multiplier", ctypes.c_float),
-        ("checkpoint_slices", ctypes.c_int),
+        ("checkpoint_slices", ctypes.c_int), ("lanes", ctypes.c_int),
     ]
 
 

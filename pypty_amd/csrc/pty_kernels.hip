@@ -27,6 +27,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -2137,9 +2139,14 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 }
 
 // ---------------------------------------------------------- workspace
+// Shared buffers first, then one block of per-batch buffers per lane: lane l's
+// copy of a per-batch buffer lies lane_bytes * l after lane 0's offset (an
+// offset of 0 marks a buffer that is not needed).
 struct Layout {
-  size_t H, H0, maskc, zcol, fprobe, objp, gradp, Ob, Pp, stash, gstate, Psi, coef, Gbuf, gP, probe_part,
-      probe_sum, pos_part, loss_part, meas_stage, scratch, total;
+  size_t H, H0, maskc, zcol, fprobe, objp, gradp, probe_part, probe_sum, loss_part;  // shared
+  size_t Ob, Pp, stash, gstate, Psi, coef, gP, pos_part, meas_stage, scratch;        // per lane
+  size_t lane_bytes, total;
+  int lanes;
   int B, G;
   int ck, C;  // slices per checkpoint segment (0: every slice kept) and segments
   int slots;  // stash slots per window and mode
@@ -2211,6 +2218,10 @@ Layout plan_layout(const pty_problem* p) {
   L.fprobe = take(size_t(M) * n2 * 8);
   L.objp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
   L.gradp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
+  L.probe_part = take(size_t(G) * M * n2 * 8);
+  L.probe_sum = take(size_t(M) * n2 * 8);
+  L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
+  const size_t lane0 = o;
   L.Ob = take(size_t(B) * S * K * n2 * 8);
   L.Pp = take(size_t(B) * M * n2 * 8);
   // every slice, or the checkpoints of all segments but the last followed by
@@ -2223,20 +2234,18 @@ Layout plan_layout(const pty_problem* p) {
   L.gstate = L.ck ? take(size_t(B) * M * K * n2 * 8) : 0;
   L.Psi = take(size_t(B) * M * K * n2 * 8);
   L.coef = take(size_t(B) * n2 * 4);
-  L.Gbuf = 0;
   L.gP = take(size_t(B) * M * K * n2 * 8);
-  L.probe_part = take(size_t(G) * M * n2 * 8);
-  L.probe_sum = take(size_t(M) * n2 * 8);
   L.meas_stage = preprocessing(p) ? take(size_t(B) * n2 * 4) : 0;
   L.pos_part = take(size_t(B) * M * 2 * 8);
-  L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
   if (p->n > 128) {
     // one tile per workgroup of the largest launch
     const size_t grid = std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K, size_t(G) * M,
                                           size_t(M)});
     L.scratch = take(grid * n2 * 8);
   }
-  L.total = o;
+  L.lane_bytes = o - lane0;
+  L.lanes = p->lanes >= 2 ? 2 : 1;
+  L.total = lane0 + size_t(L.lanes) * L.lane_bytes;
   L.B = B;
   L.G = G;
   return L;
@@ -2248,6 +2257,7 @@ int validate(const pty_problem* p) {
   if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1 || p->compute_batch < 1) return PTY_ERR_ARG;
   if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
   if (p->checkpoint_slices < 0) return PTY_ERR_ARG;
+  if (p->lanes < 0 || p->lanes > 2) return PTY_ERR_ARG;
   if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_POISSON_COMPRESSED) return PTY_ERR_ARG;
   if (compressed(p->algorithm) && (p->n_masks < 1 || p->n_masks > kMaxMasks)) return PTY_ERR_ARG;
   if (preprocessing(p)) {
@@ -2258,6 +2268,37 @@ int validate(const pty_problem* p) {
   return PTY_OK;
 }
 
+
+// The library stream that carries lane 1 of the batches issued on a caller
+// stream, one per (device, caller stream), created on first use at the
+// default priority and kept for the life of the process, with the events that
+// fork, order and join the lanes.  A call holds the state's mutex while it
+// enqueues.  A call on a capturing stream runs one lane.
+struct LaneState {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, acc[2] = {nullptr, nullptr};
+  std::mutex mu;
+};
+std::mutex g_lane_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<LaneState>> g_lanes;
+
+// the caller's lane state, or null (one lane) when none can be made now
+LaneState* lane_state(hipStream_t caller) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(caller, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+  std::lock_guard<std::mutex> g(g_lane_mu);
+  auto& slot = g_lanes[{dev, caller}];
+  if (!slot) {
+    auto st = std::make_unique<LaneState>();
+    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    for (hipEvent_t* e : {&st->fork, &st->join, &st->acc[0], &st->acc[1]})
+      if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    slot = std::move(st);
+  }
+  return slot.get();
+}
 
 template <int N>
 int launch_tables(const pty_problem* p, float2* H, float2* H0, float* maskc, int* zcol, hipStream_t s) {
@@ -2318,7 +2359,11 @@ void launch_objgrad(const BatchArgs& a, hipStream_t s) {
 // the segment's window gradients.  Every launch computes exactly what the
 // whole-pass launch computes for those slices, in the same order.
 template <int N>
-int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
+// One batch on stream s.  The kernels that add into buffers shared by all
+// batches (object gradient, probe partials, position gradients) wait for
+// wait_acc (the previous batch's, on the other lane) and then record done_acc.
+int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s, hipEvent_t wait_acc = nullptr,
+              hipEvent_t done_acc = nullptr) {
   constexpr int LAY = kLayout<N>;
   const int planes = a.S * a.K;
   {
@@ -2383,6 +2428,7 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
       launch_objgrad<N>(a, s);
     }
   }
+  if (wait_acc && hipStreamWaitEvent(s, wait_acc, 0) != hipSuccess) return PTY_ERR_HIP;
   if (p->step_obj) {
     TimeScope ts(KT_GATHER, s);
     const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
@@ -2399,6 +2445,7 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s) {
     TimeScope ts(KT_POS_FINAL, s);
     k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a);
   }
+  if (done_acc && hipEventRecord(done_acc, s) != hipSuccess) return PTY_ERR_HIP;
   return check_launch();
 }
 
@@ -2476,12 +2523,38 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     a.st_slots = L.slots;
     a.gstate = reinterpret_cast<float2*>(ws + L.gstate);
   }
-  for (int b0 = 0; b0 < bf->n_seq; b0 += L.B) {
-    a.b0 = b0;
-    a.B = std::min(L.B, bf->n_seq - b0);
-    a.G = std::min(L.G, a.B);
-    if ((rc = run_batch<N>(a, p, L.ck, s))) return rc;
+  // batches alternate between the lanes; lane 1 runs on the library stream
+  // paired with s, with its own per-batch buffers
+  const int n_batches = (bf->n_seq + L.B - 1) / L.B;
+  LaneState* ln = (L.lanes == 2 && n_batches >= 2) ? lane_state(s) : nullptr;
+  std::unique_lock<std::mutex> lane_lock;
+  BatchArgs al[2] = {a, a};
+  hipStream_t lst[2] = {s, s};
+  if (ln) {
+    lane_lock = std::unique_lock<std::mutex>(ln->mu);
+    BatchArgs& b1 = al[1];
+    auto off = [&](auto* ptr) { return ptr ? reinterpret_cast<decltype(ptr)>(reinterpret_cast<char*>(ptr) + L.lane_bytes) : ptr; };
+    b1.Ob = off(b1.Ob); b1.Pp = off(b1.Pp); b1.stash = off(b1.stash); b1.gstate = off(b1.gstate);
+    b1.Psi = off(b1.Psi); b1.coef = off(b1.coef); b1.gP = off(b1.gP); b1.pos_part = off(b1.pos_part);
+    b1.meas_stage = off(b1.meas_stage); b1.scratch = off(b1.scratch);
+    lst[1] = ln->stream;
+    if (hipEventRecord(ln->fork, s) != hipSuccess || hipStreamWaitEvent(ln->stream, ln->fork, 0) != hipSuccess)
+      return PTY_ERR_HIP;
   }
+  rc = PTY_OK;
+  for (int i = 0; i < n_batches && rc == PTY_OK; ++i) {
+    const int l = ln ? (i & 1) : 0;
+    BatchArgs& x = al[l];
+    x.b0 = i * L.B;
+    x.B = std::min(L.B, bf->n_seq - x.b0);
+    x.G = std::min(L.G, x.B);
+    hipEvent_t wait_acc = (ln && i > 0) ? ln->acc[l ^ 1] : nullptr, done_acc = ln ? ln->acc[l] : nullptr;
+    rc = run_batch<N>(x, p, L.ck, lst[l], wait_acc, done_acc);
+  }
+  // the lane's work joins s also when a launch failed
+  if (ln && (hipEventRecord(ln->join, ln->stream) != hipSuccess || hipStreamWaitEvent(s, ln->join, 0) != hipSuccess))
+    return PTY_ERR_HIP;
+  if (rc) return rc;
   {
     TimeScope ts(KT_LOSS_FINAL, s);
     k_loss_final<<<1, 256, 0, s>>>(a.loss_part, bf->n_seq * LossSplit<N>::CHUNKS, double(p->loss_weight),

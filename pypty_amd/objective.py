@@ -74,6 +74,8 @@ class PassConfig:
     # keep the wave entering every checkpoint_slices-th slice and recompute the
     # others in the adjoint (0: keep every slice); the results are bit-identical
     checkpoint_slices: int = 0
+    # consecutive batches alternate between two streams (bit-identical results)
+    lanes: int = 2
 
     @property
     def compressed(self) -> bool:
@@ -91,7 +93,7 @@ class PassConfig:
             float(self.wavelength), float(self.pixel_size_x), float(self.pixel_size_y), float(self.tan_x),
             float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier), int(self.n_masks),
             int(self.raw_n), int(self.data_shift_y), int(self.data_shift_x), int(self.data_bin), int(self.data_pad),
-            float(self.data_multiplier), int(self.checkpoint_slices))
+            float(self.data_multiplier), int(self.checkpoint_slices), int(self.lanes))
 
 
 class PtychoPass:
@@ -353,7 +355,8 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         edge_multiplier=float(edge_slice_thickness_multiplier),
         n_masks=int(masks_dev.shape[0]) if compressed else 0, raw_n=0 if compressed else int(measured.shape[-1]),
         data_shift_y=shift_y, data_shift_x=shift_x, data_bin=1 if compressed else int(data_bin), data_pad=pad,
-        data_multiplier=1.0 if compressed else float(data_multiplier), checkpoint_slices=int(checkpoint_slices))
+        data_multiplier=1.0 if compressed else float(data_multiplier), checkpoint_slices=int(checkpoint_slices),
+        lanes=1 if stream else 2)  # a streamed pass runs one batch per call
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None

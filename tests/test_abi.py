@@ -112,3 +112,29 @@ def test_checkpoint_segments_shrink_the_workspace(lib):
     assert full - ck8 == (40 - 13) * slot
     assert ws(40)[1] == full and ws(0, step_obj=0)[1] == ws(8, step_obj=0)[1]  # inactive cases keep every slice
     assert ws(-1)[0] == -1
+
+
+def test_batch_lanes_add_one_set_of_per_batch_buffers(lib):
+    """lanes = 2 adds exactly one more block of the per-batch buffers (the shared tables, probe
+    partials and loss partials stay single); other values are rejected."""
+    from pypty_amd import _lib
+
+    def ws(lanes, n_slices=10):
+        p = _lib.PtyProblem()
+        p.n, p.n_slices, p.n_obj_modes, p.n_probe_modes = 128, n_slices, 1, 5
+        p.obj_ny = p.obj_nx = 512
+        p.compute_batch, p.n_positions, p.obj_layout, p.lanes = 256, 4096, 1, lanes
+        size = ctypes.c_size_t(0)
+        rc = lib.pty_workspace_bytes(ctypes.byref(p), ctypes.byref(size))
+        return rc, size.value
+
+    rc1, one = ws(1)
+    rc2, two = ws(2)
+    assert rc1 == rc2 == 0 and ws(0) == (0, one)
+    stash = 256 * 5 * 10 * 128 * 128 * 8
+    assert stash < two - one < one
+    # a slice more adds a stash slot per window and mode and a band-limited window per window to
+    # every lane (planar object layout: no object copies)
+    slice_block = (256 * 5 + 256) * 128 * 128 * 8
+    assert (ws(2, 11)[1] - two) == 2 * slice_block and (ws(1, 11)[1] - one) == slice_block
+    assert ws(3)[0] == -1 and ws(-1)[0] == -1

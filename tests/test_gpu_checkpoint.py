@@ -22,7 +22,7 @@ CASES = [
 ]
 
 
-def _pass(pr, n, S, K, M, P, batch, ck, dev):
+def _pass(pr, n, S, K, M, P, batch, ck, dev, lanes=2):
     import torch
 
     from pypty_amd.objective import PassConfig, PtychoPass
@@ -31,7 +31,7 @@ def _pass(pr, n, S, K, M, P, batch, ck, dev):
                      obj_nx=pr["obj"].shape[1], n_positions=P, compute_batch=batch, algorithm="lsq_sqrt",
                      step_obj=True, step_probe=True, step_pos=True, shift_active=True, slice_distance=pr["dz"],
                      wavelength=pr["wavelength"], pixel_size_x=pr["px"], pixel_size_y=pr["px"],
-                     checkpoint_slices=ck)
+                     checkpoint_slices=ck, lanes=lanes)
     obj = torch.from_numpy(pr["obj"]).to(dev)
     probe = torch.from_numpy(np.ascontiguousarray(pr["probe"].transpose(2, 0, 1))).to(dev)
     og, pg = torch.empty_like(obj), torch.empty_like(probe)
@@ -59,9 +59,9 @@ def test_checkpointed_pass_is_bit_identical(n, S, K, M, P, batch, ck):
         assert torch.equal(a, b), name
     assert float(full[1].abs().max()) > 0 and float(full[3].abs().max()) > 0
     # wave slots per window and mode: S without checkpoints, else the checkpoints of all segments
-    # but the last, one k-slice segment and the wave carried between segments
+    # but the last, one k-slice segment and the wave carried between segments; one set per batch lane
     slots = (-(-S // ck) - 1) + ck + 1
-    assert full[4] - seg[4] == (S - slots) * batch * M * K * n * n * 8
+    assert full[4] - seg[4] == 2 * (S - slots) * batch * M * K * n * n * 8
 
 
 def test_drop_in_checkpoint_keyword():

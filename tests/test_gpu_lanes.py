@@ -1,0 +1,95 @@
+"""Batch lanes: a pass whose consecutive batches alternate between the caller's stream and the
+library's second stream gives the loss and every gradient of the one-stream pass bit for bit (the
+kernels that add into the shared gradients run in batch order across the lanes).  Covers the
+register tile with four and eight threads per line, the slab tile, object modes, checkpointed
+passes, an odd number of batches, a single batch (no second lane), and a HIP graph captured from a
+two-lane engine on a stream whose lanes were in use eagerly (a capturing call runs one lane),
+replayed with eager work between the replays."""
+import numpy as np
+import pytest
+
+from tests._problems import make_problem
+from tests.test_gpu_checkpoint import _pass
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # n, S, K, M, P, batch, k
+    (32, 3, 2, 2, 11, 4, 0),
+    (96, 4, 1, 2, 9, 2, 0),
+    (128, 3, 1, 3, 8, 3, 0),
+    (128, 5, 1, 2, 7, 2, 2),
+    (192, 3, 1, 2, 5, 2, 0),
+    (256, 2, 1, 1, 4, 2, 0),
+    (64, 2, 1, 2, 5, 8, 0),  # one batch
+]
+
+
+@pytest.mark.parametrize("n,S,K,M,P,batch,ck", CASES)
+def test_two_lanes_are_bit_identical_to_one(n, S, K, M, P, batch, ck):
+    import torch
+
+    print(f"[lanes] n={n} S={S} K={K} M={M} P={P} batch={batch} k={ck}", flush=True)
+    pr = make_problem(n=n, S=S, K=K, M=M, ny=n + 40, nx=n + 40, P=P, shift=True, seed=131 + n + ck)
+    dev = torch.device("cuda", 0)
+    one = _pass(pr, n, S, K, M, P, batch, ck, dev, lanes=1)
+    two = _pass(pr, n, S, K, M, P, batch, ck, dev, lanes=2)
+    names = ("loss", "object gradient", "probe gradient", "position gradient")
+    for name, a, b in zip(names, one[:4], two[:4]):
+        assert torch.isfinite(a).all(), name
+        assert torch.equal(a, b), name
+    assert float(one[1].abs().max()) > 0 and float(one[3].abs().max()) > 0
+    assert two[4] > one[4]  # a second set of per-batch buffers
+
+
+def test_graph_captured_from_a_two_lane_engine_replays_bit_identically():
+    import torch
+
+    from pypty_amd.objective import PassConfig, PtychoPass
+
+    n, S, M, P, batch = 128, 2, 2, 10, 3
+    print(f"[lanes] graph n={n} S={S} M={M} P={P} batch={batch}", flush=True)
+    dev = torch.device("cuda", 0)
+    pr = make_problem(n=n, S=S, K=1, M=M, ny=n + 40, nx=n + 40, P=P, shift=True, seed=171)
+    t = dict(obj=torch.from_numpy(pr["obj"]).to(dev),
+             probe=torch.from_numpy(np.ascontiguousarray(pr["probe"].transpose(2, 0, 1))).to(dev),
+             pos=torch.from_numpy(pr["pos"]).to(dev), corr=torch.from_numpy(pr["corr"]).to(dev),
+             meas=torch.from_numpy(pr["meas"]).to(dev), seq=torch.arange(P, dtype=torch.int32, device=dev))
+
+    def engine(lanes):
+        cfg = PassConfig(n=n, n_slices=S, n_obj_modes=1, n_probe_modes=M, obj_ny=n + 40, obj_nx=n + 40,
+                         n_positions=P, compute_batch=batch, algorithm="lsq_sqrt", step_obj=True, step_probe=True,
+                         step_pos=True, shift_active=True, slice_distance=pr["dz"], wavelength=pr["wavelength"],
+                         pixel_size_x=pr["px"], pixel_size_y=pr["px"], lanes=lanes)
+        eng = PtychoPass(cfg, dev)
+        out = dict(og=torch.empty_like(t["obj"]), pg=torch.empty_like(t["probe"]), posg=torch.zeros_like(t["corr"]))
+
+        def run():
+            return eng.run(t["obj"], t["probe"], t["pos"], t["corr"], t["meas"], t["seq"], obj_grad=out["og"],
+                           probe_grad=out["pg"], pos_grad=out["posg"], sync=False)
+
+        return run, out
+
+    run1, out1 = engine(1)
+    ls = run1()
+    torch.cuda.synchronize()
+    ref = (ls.clone(), out1["og"].clone(), out1["pg"].clone(), out1["posg"].clone())
+    run2, out2 = engine(2)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run2()  # eager use of the capture stream, with its second lane
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        lt = run2()
+    for rep in range(4):
+        for x in (out2["og"], out2["pg"], out2["posg"]):
+            x.fill_(float("nan"))  # replays must rewrite every output
+        y = torch.randn(512, 512, device="cuda")
+        (y @ y).sum().item()  # unrelated eager work and a synchronisation between replays
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip((lt, out2["og"], out2["pg"], out2["posg"]), ref):
+            assert torch.equal(a, b), rep
