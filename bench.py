@@ -8,8 +8,8 @@ positions, 256 windows per launch, Poisson data at 1e4 e-.  --config ragged
 runs eight mixed-shape datasets (windows 96-192 px) one after the other per
 step; --config thick the 1536x1536x40 object with a 256 px, 10-mode probe.
 The loss/sse readback of every pass is inside the timed region.  Each pass runs
-with two batch lanes (--lanes): consecutive batches alternate between two
-streams, so one batch's memory-bound kernels overlap the next one's compute.
+with four batch lanes (--lanes): batch i runs on stream i % 4, so one batch's
+memory-bound kernels overlap another's compute.
 The per-kernel breakdown and the roofline come from a second run of the same
 steps with one lane, where every launch has the GPU to itself.
 
@@ -323,7 +323,7 @@ WORKLOADS = {
 class Job:
     """One dataset resident on the device with its engine and gradient buffers."""
 
-    def __init__(self, ds, batch, dev, checkpoint=0, lanes=2):
+    def __init__(self, ds, batch, dev, checkpoint=0, lanes=4):
         self.ds = ds
         self.checkpoint = checkpoint
         self.lanes = lanes
@@ -394,9 +394,9 @@ def main(argv=None):
     ap.add_argument("--warm-recon-epochs", type=int, default=20)
     ap.add_argument("--streams", type=int, default=None,
                     help="streams the datasets of a step are spread over (default: 2 for ragged, else 1)")
-    ap.add_argument("--lanes", type=int, default=None, choices=(1, 2),
-                    help="batch lanes of each pass (consecutive batches alternate between two streams); "
-                         "default 2, and 1 for --config ragged")
+    ap.add_argument("--lanes", type=int, default=None, choices=(1, 2, 3, 4),
+                    help="batch lanes of each pass (batch i runs on stream i %% lanes); "
+                         "default 4, and 1 for --config ragged")
     ap.add_argument("--checkpoint", type=int, default=None,
                     help="keep every k-th slice's wave and recompute the rest (default: 8 for thick, else 0)")
     args = ap.parse_args(argv)
@@ -418,7 +418,7 @@ def main(argv=None):
         datasets = [synth.config4(dev, n_scan=args.scan)]
     t_gen = time.perf_counter() - t_gen
     if args.lanes is None:
-        args.lanes = 1 if args.config == "ragged" else 2
+        args.lanes = 1 if args.config == "ragged" else 4
     if args.config == "ragged" and args.lanes > 1:
         # two lanes under this workload (eight datasets over two streams, autotuned, captured in one
         # graph) ended once in a host segmentation fault whose cause is not known; it runs one lane

@@ -131,11 +131,11 @@ typedef struct pty_problem {
    * without checkpointing.  0 keeps every slice. */
   int checkpoint_slices;
   /* Batch lanes (no counterpart in the reference, which runs its batches one
-   * after another, objective.py:383-725): with lanes = 2 consecutive batches
-   * alternate between the caller's stream and a library stream of the same
-   * device and default priority, each lane with its own per-batch buffers,
-   * so that one batch's memory-bound kernels overlap the next one's
-   * compute-bound kernels.  The kernels that add into the object, probe and
+   * after another, objective.py:383-725): with lanes = L in 2..4, batch i
+   * runs in lane i % L, lane 0 on the caller's stream and lanes 1..L-1 on
+   * library streams of the same device and default priority, each lane with
+   * its own per-batch buffers, so that one batch's memory-bound kernels
+   * overlap another's compute-bound kernels.  The kernels that add into the object, probe and
    * position gradients still run in batch order (events order them across
    * the lanes), so the results are bit-identical to lanes = 1.  The call
    * returns with all work joined to the caller's stream.  A call on a

@@ -191,6 +191,7 @@ enum { LAY_RT = 1, LAY_GT = 2 };
 #define PTY_GT_THREADS 512
 #endif
 constexpr int kGtThreads = PTY_GT_THREADS;  // threads of a 192/256 px slab workgroup
+constexpr int kMaxLanes = 4;                 // batch lanes of a pass (pty_problem.lanes)
 constexpr int kLayoutOf(int n) {
   return (n <= 128) ? LAY_RT : LAY_GT;
 }
@@ -2244,7 +2245,7 @@ Layout plan_layout(const pty_problem* p) {
     L.scratch = take(grid * n2 * 8);
   }
   L.lane_bytes = o - lane0;
-  L.lanes = p->lanes >= 2 ? 2 : 1;
+  L.lanes = std::max(1, p->lanes);
   L.total = lane0 + size_t(L.lanes) * L.lane_bytes;
   L.B = B;
   L.G = G;
@@ -2257,7 +2258,7 @@ int validate(const pty_problem* p) {
   if (p->n_slices < 1 || p->n_obj_modes < 1 || p->n_probe_modes < 1 || p->compute_batch < 1) return PTY_ERR_ARG;
   if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
   if (p->checkpoint_slices < 0) return PTY_ERR_ARG;
-  if (p->lanes < 0 || p->lanes > 2) return PTY_ERR_ARG;
+  if (p->lanes < 0 || p->lanes > kMaxLanes) return PTY_ERR_ARG;
   if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_POISSON_COMPRESSED) return PTY_ERR_ARG;
   if (compressed(p->algorithm) && (p->n_masks < 1 || p->n_masks > kMaxMasks)) return PTY_ERR_ARG;
   if (preprocessing(p)) {
@@ -2269,14 +2270,14 @@ int validate(const pty_problem* p) {
 }
 
 
-// The library stream that carries lane 1 of the batches issued on a caller
-// stream, one per (device, caller stream), created on first use at the
-// default priority and kept for the life of the process, with the events that
-// fork, order and join the lanes.  A call holds the state's mutex while it
-// enqueues.  A call on a capturing stream runs one lane.
+// The library streams that carry lanes 1 .. kMaxLanes-1 of the batches issued
+// on a caller stream, one set per (device, caller stream), created on first
+// use at the default priority and kept for the life of the process, with the
+// events that fork, order and join the lanes.  A call holds the state's mutex
+// while it enqueues.  A call on a capturing stream runs one lane.
 struct LaneState {
-  hipStream_t stream = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, acc[2] = {nullptr, nullptr};
+  hipStream_t stream[kMaxLanes] = {};  // [0] unused: lane 0 is the caller's stream
+  hipEvent_t fork = nullptr, join[kMaxLanes] = {}, acc[kMaxLanes] = {};
   std::mutex mu;
 };
 std::mutex g_lane_mu;
@@ -2292,9 +2293,13 @@ LaneState* lane_state(hipStream_t caller) {
   auto& slot = g_lanes[{dev, caller}];
   if (!slot) {
     auto st = std::make_unique<LaneState>();
-    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    for (hipEvent_t* e : {&st->fork, &st->join, &st->acc[0], &st->acc[1]})
-      if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    for (int l = 1; l < kMaxLanes; ++l)
+      if (hipStreamCreateWithFlags(&st->stream[l], hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&st->fork, hipEventDisableTiming) != hipSuccess) return nullptr;
+    for (int l = 0; l < kMaxLanes; ++l)
+      if (hipEventCreateWithFlags(&st->join[l], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&st->acc[l], hipEventDisableTiming) != hipSuccess)
+        return nullptr;
     slot = std::move(st);
   }
   return slot.get();
@@ -2523,37 +2528,45 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     a.st_slots = L.slots;
     a.gstate = reinterpret_cast<float2*>(ws + L.gstate);
   }
-  // batches alternate between the lanes; lane 1 runs on the library stream
-  // paired with s, with its own per-batch buffers
+  // batch i runs in lane i % lanes: lane 0 on s, lane l > 0 on the library
+  // stream l paired with s, each lane with its own per-batch buffers
   const int n_batches = (bf->n_seq + L.B - 1) / L.B;
-  LaneState* ln = (L.lanes == 2 && n_batches >= 2) ? lane_state(s) : nullptr;
+  const int lanes = std::min(L.lanes, n_batches);
+  LaneState* ln = lanes >= 2 ? lane_state(s) : nullptr;
+  const int nl = ln ? lanes : 1;
   std::unique_lock<std::mutex> lane_lock;
-  BatchArgs al[2] = {a, a};
-  hipStream_t lst[2] = {s, s};
+  BatchArgs al[kMaxLanes];
+  hipStream_t lst[kMaxLanes];
+  for (int l = 0; l < nl; ++l) {
+    BatchArgs& x = al[l];
+    x = a;
+    const size_t d = size_t(l) * L.lane_bytes;
+    auto off = [d](auto* ptr) { return ptr ? reinterpret_cast<decltype(ptr)>(reinterpret_cast<char*>(ptr) + d) : ptr; };
+    x.Ob = off(x.Ob); x.Pp = off(x.Pp); x.stash = off(x.stash); x.gstate = off(x.gstate);
+    x.Psi = off(x.Psi); x.coef = off(x.coef); x.gP = off(x.gP); x.pos_part = off(x.pos_part);
+    x.meas_stage = off(x.meas_stage); x.scratch = off(x.scratch);
+    lst[l] = l ? ln->stream[l] : s;
+  }
   if (ln) {
     lane_lock = std::unique_lock<std::mutex>(ln->mu);
-    BatchArgs& b1 = al[1];
-    auto off = [&](auto* ptr) { return ptr ? reinterpret_cast<decltype(ptr)>(reinterpret_cast<char*>(ptr) + L.lane_bytes) : ptr; };
-    b1.Ob = off(b1.Ob); b1.Pp = off(b1.Pp); b1.stash = off(b1.stash); b1.gstate = off(b1.gstate);
-    b1.Psi = off(b1.Psi); b1.coef = off(b1.coef); b1.gP = off(b1.gP); b1.pos_part = off(b1.pos_part);
-    b1.meas_stage = off(b1.meas_stage); b1.scratch = off(b1.scratch);
-    lst[1] = ln->stream;
-    if (hipEventRecord(ln->fork, s) != hipSuccess || hipStreamWaitEvent(ln->stream, ln->fork, 0) != hipSuccess)
-      return PTY_ERR_HIP;
+    if (hipEventRecord(ln->fork, s) != hipSuccess) return PTY_ERR_HIP;
+    for (int l = 1; l < nl; ++l)
+      if (hipStreamWaitEvent(lst[l], ln->fork, 0) != hipSuccess) return PTY_ERR_HIP;
   }
   rc = PTY_OK;
   for (int i = 0; i < n_batches && rc == PTY_OK; ++i) {
-    const int l = ln ? (i & 1) : 0;
+    const int l = i % nl;
     BatchArgs& x = al[l];
     x.b0 = i * L.B;
     x.B = std::min(L.B, bf->n_seq - x.b0);
     x.G = std::min(L.G, x.B);
-    hipEvent_t wait_acc = (ln && i > 0) ? ln->acc[l ^ 1] : nullptr, done_acc = ln ? ln->acc[l] : nullptr;
+    hipEvent_t wait_acc = (ln && i > 0) ? ln->acc[(i - 1) % nl] : nullptr, done_acc = ln ? ln->acc[l] : nullptr;
     rc = run_batch<N>(x, p, L.ck, lst[l], wait_acc, done_acc);
   }
-  // the lane's work joins s also when a launch failed
-  if (ln && (hipEventRecord(ln->join, ln->stream) != hipSuccess || hipStreamWaitEvent(s, ln->join, 0) != hipSuccess))
-    return PTY_ERR_HIP;
+  // the lanes' work joins s also when a launch failed
+  for (int l = 1; l < nl; ++l)
+    if (hipEventRecord(ln->join[l], lst[l]) != hipSuccess || hipStreamWaitEvent(s, ln->join[l], 0) != hipSuccess)
+      return PTY_ERR_HIP;
   if (rc) return rc;
   {
     TimeScope ts(KT_LOSS_FINAL, s);

@@ -74,8 +74,8 @@ class PassConfig:
     # keep the wave entering every checkpoint_slices-th slice and recompute the
     # others in the adjoint (0: keep every slice); the results are bit-identical
     checkpoint_slices: int = 0
-    # consecutive batches alternate between two streams (bit-identical results)
-    lanes: int = 2
+    # batch i runs on stream i % lanes of up to four (bit-identical results)
+    lanes: int = 4
 
     @property
     def compressed(self) -> bool:
@@ -356,7 +356,7 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         n_masks=int(masks_dev.shape[0]) if compressed else 0, raw_n=0 if compressed else int(measured.shape[-1]),
         data_shift_y=shift_y, data_shift_x=shift_x, data_bin=1 if compressed else int(data_bin), data_pad=pad,
         data_multiplier=1.0 if compressed else float(data_multiplier), checkpoint_slices=int(checkpoint_slices),
-        lanes=1 if stream else 2)  # a streamed pass runs one batch per call
+        lanes=1 if stream else 4)  # a streamed pass runs one batch per call
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None

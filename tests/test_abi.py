@@ -115,8 +115,8 @@ def test_checkpoint_segments_shrink_the_workspace(lib):
 
 
 def test_batch_lanes_add_one_set_of_per_batch_buffers(lib):
-    """lanes = 2 adds exactly one more block of the per-batch buffers (the shared tables, probe
-    partials and loss partials stay single); other values are rejected."""
+    """Every lane beyond the first adds exactly one more block of the per-batch buffers (the shared
+    tables, probe partials and loss partials stay single); more than four lanes are rejected."""
     from pypty_amd import _lib
 
     def ws(lanes, n_slices=10):
@@ -137,4 +137,5 @@ def test_batch_lanes_add_one_set_of_per_batch_buffers(lib):
     # every lane (planar object layout: no object copies)
     slice_block = (256 * 5 + 256) * 128 * 128 * 8
     assert (ws(2, 11)[1] - two) == 2 * slice_block and (ws(1, 11)[1] - one) == slice_block
-    assert ws(3)[0] == -1 and ws(-1)[0] == -1
+    assert ws(3)[1] - two == two - one and ws(4)[1] - ws(3)[1] == two - one
+    assert ws(5)[0] == -1 and ws(-1)[0] == -1
