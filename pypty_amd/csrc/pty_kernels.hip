@@ -877,6 +877,7 @@ __device__ __forceinline__ void rt_band_limit(float2* v, float2* T, const RTIds<
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   const int planes = a.S * a.K;
   const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
@@ -899,6 +900,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   const int MK = a.M * a.K;
   int b, mk;
@@ -984,6 +986,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   const int MK = a.M * a.K;
   int b, mk;
@@ -1074,6 +1077,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_objgrad_rt(BatchArgs a) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   const int planes = a.S * a.K;
   int b, plane, sl, k;
@@ -1127,6 +1131,7 @@ __device__ __forceinline__ void shift_tables_freq(float2* exy, float dy, float d
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_fprobe_rt(const float2* __restrict__ probe, float2* __restrict__ fprobe) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   const RTIds<N> id;
   const float2* src = probe + size_t(blockIdx.x) * N * N;
@@ -1143,6 +1148,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_fprobe_rt(const float2* __restric
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   __shared__ float2 exy[2 * N];
   const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
@@ -1185,6 +1191,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   __shared__ float2 exy[2 * N];
   __shared__ double red[16];
@@ -1260,6 +1267,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_final_rt(const float2* __re
                                                               const float* __restrict__ maskc,
                                                               float2* __restrict__ probe_grad) {
   using R = RT<N>;
+  rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
   const RTIds<N> id;
   const unsigned tid = threadIdx.x;
@@ -1387,6 +1395,7 @@ __device__ __forceinline__ void gt_band_columns(float2* v, float2* X, const Slab
 template <int N>
 __global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int planes = a.S * a.K;
   const int b = blockIdx.x / planes, plane = blockIdx.x % planes;
@@ -1421,6 +1430,7 @@ __global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int MK = a.M * a.K;
   int b, mk;
@@ -1524,6 +1534,7 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int MK = a.M * a.K;
   int b, mk;
@@ -1636,6 +1647,7 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int planes = a.S * a.K;
   int b, plane, sl_, k;
@@ -1684,6 +1696,7 @@ template <int N>
 __global__ __launch_bounds__(kGtThreads) void k_fprobe_gt(const float2* __restrict__ probe,
                                                           float2* __restrict__ fprobe, float2* __restrict__ scratch) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const GTLane<N> L;
   const Slab<N> sl(scratch + size_t(blockIdx.x) * N * N);
@@ -1710,6 +1723,7 @@ __global__ __launch_bounds__(kGtThreads) void k_fprobe_gt(const float2* __restri
 template <int N>
 __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   __shared__ float2 exy[2 * N];
   const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
@@ -1761,6 +1775,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
 template <int N>
 __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   __shared__ float2 exy[2 * N];
   __shared__ double red[16];
@@ -1847,6 +1862,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_final_gt(const float2* __r
                                                                float2* __restrict__ probe_grad,
                                                                float2* __restrict__ scratch) {
   using G = GT<N, kGtThreads>;
+  rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
