@@ -151,7 +151,7 @@ def simulate(obj, probe, positions, corrections, pixel_A, dz_A, wavelength_A, do
     cfg = PassConfig(n=n, n_slices=S, n_obj_modes=K, n_probe_modes=M, obj_ny=ny, obj_nx=nx, n_positions=P,
                      compute_batch=min(compute_batch, P), step_obj=False, step_probe=False, step_pos=False,
                      shift_active=bool(np.any(corrections != 0)), slice_distance=dz_A, wavelength=wavelength_A,
-                     pixel_size_x=pixel_A, pixel_size_y=pixel_A)
+                     pixel_size_x=pixel_A, pixel_size_y=pixel_A, lanes=1)  # one stream: profiles see whole launches
     eng = PtychoPass(cfg, dev)
     sim = torch.zeros((P, n, n), dtype=torch.float32, device=dev)
     eng.run(torch.from_numpy(obj).to(dev), torch.from_numpy(np.ascontiguousarray(probe.transpose(2, 0, 1))).to(dev),
