@@ -205,6 +205,9 @@ def warm_start_benchmark(dev, pairs, train_epochs, rec_epochs, step0=1e-3):
         return {"plain_s_total": round(plain[-1][0], 3), "target_nmse": round(target, 4),
                 "plain_s_to_target": round(t_plain, 3), "warm_s_to_target": None if t_warm is None else round(t_warm, 3),
                 "warm_over_plain": None if t_warm is None else round(t_warm / t_plain, 3),
+                # the warm run's first arrival at plain's final NMSE against plain's whole run (the
+                # wall-clock plain takes to produce that final NMSE)
+                "warm_over_plain_run": None if t_warm is None else round(t_warm / plain[-1][0], 3),
                 "warm_final_nmse": round(warm[-1][1], 4), "by_plain_epochs": by_epochs,
                 "curves_s_nmse": {"plain": [[round(t, 3), round(e, 4)] for t, e in plain],
                                   "warm": [[round(t, 3), round(e, 4)] for t, e in warm]}}
