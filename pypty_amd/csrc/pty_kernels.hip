@@ -2309,13 +2309,23 @@ LaneState* lane_state(hipStream_t caller) {
   auto& slot = g_lanes[{dev, caller}];
   if (!slot) {
     auto st = std::make_unique<LaneState>();
-    for (int l = 1; l < kMaxLanes; ++l)
-      if (hipStreamCreateWithFlags(&st->stream[l], hipStreamNonBlocking) != hipSuccess) return nullptr;
-    if (hipEventCreateWithFlags(&st->fork, hipEventDisableTiming) != hipSuccess) return nullptr;
-    for (int l = 0; l < kMaxLanes; ++l)
-      if (hipEventCreateWithFlags(&st->join[l], hipEventDisableTiming) != hipSuccess ||
-          hipEventCreateWithFlags(&st->acc[l], hipEventDisableTiming) != hipSuccess)
-        return nullptr;
+    bool ok = hipEventCreateWithFlags(&st->fork, hipEventDisableTiming) == hipSuccess;
+    for (int l = 1; ok && l < kMaxLanes; ++l)
+      ok = hipStreamCreateWithFlags(&st->stream[l], hipStreamNonBlocking) == hipSuccess;
+    for (int l = 0; ok && l < kMaxLanes; ++l)
+      ok = hipEventCreateWithFlags(&st->join[l], hipEventDisableTiming) == hipSuccess &&
+           hipEventCreateWithFlags(&st->acc[l], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      // release what was made; the call runs one lane
+      for (int l = 1; l < kMaxLanes; ++l)
+        if (st->stream[l]) (void)hipStreamDestroy(st->stream[l]);
+      for (hipEvent_t e : st->join)
+        if (e) (void)hipEventDestroy(e);
+      for (hipEvent_t e : st->acc)
+        if (e) (void)hipEventDestroy(e);
+      if (st->fork) (void)hipEventDestroy(st->fork);
+      return nullptr;
+    }
     slot = std::move(st);
   }
   return slot.get();
