@@ -404,7 +404,8 @@ def main(argv=None):
                     help="keep every k-th slice's wave and recompute the rest (default: 8 for thick, else 0)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (one-GPU boxes); run with --gpus 1", file=sys.stderr)
+        print("bench.py: the pass is measured on one GPU only (the multi-GPU split is a design, not built); "
+              "run with --gpus 1", file=sys.stderr)
         return 2
 
     import torch

@@ -6,7 +6,7 @@ starting object of the reconstruction.
 The reference has no learned component.  Its reconstruction starts from the
 object given in ``pypty_params['obj']`` (pypty/iterative.py:56-250, ones by
 default); ``warm_start_object`` builds that array.  The plain start stays the
-fallback and the iterative reconstruction stays the judge of the error.
+fallback, and the iterative reconstruction still decides the final error.
 
 Network (activations pixel-major, channels innermost; ``layer_spec``):
 four 4x4 stride-2 convolutions 1 -> 16 -> 32 -> 32 -> 64 channels with ReLU,
