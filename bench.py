@@ -449,8 +449,11 @@ def main(argv=None):
             best = autotune.choose_batch(key, (64, 128, 256, 512), make_pass)
             j.set_batch(best)
             tuned.append(best)
+        from pypty_amd.objective import PtychoPass
 
-    # datasets on several default-priority streams (each has its own workspace), joined at the end
+        PtychoPass.release_workspaces()  # the candidates' workspaces; the chosen engines take theirs next
+
+    # datasets on several default-priority streams (a workspace per stream and shape), joined at the end
     n_streams = max(1, min(len(jobs), (2 if ragged else 1) if args.streams is None else args.streams))
     lanes = [torch.cuda.Stream(dev) for _ in range(n_streams)] if n_streams > 1 else []
 

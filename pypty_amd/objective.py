@@ -99,9 +99,12 @@ class PassConfig:
 class PtychoPass:
     """One forward + loss + adjoint pass over a sequence of positions, on one HIP device.
 
-    Workspaces are cached per (device, size) for the life of the process, so
-    repeated passes of the same problem shapes allocate nothing; passes on one
-    stream may share a workspace.  ``release_workspaces()`` drops the cache.
+    Workspaces are cached per (device, size, stream) for the life of the process, so
+    repeated passes of the same problem shapes allocate nothing.  An engine takes its
+    workspace at its first run, from the cache entry of the stream that run is issued on,
+    and keeps it: engines of one shape share a workspace only when they start on the same
+    stream, whose order keeps their passes apart, and engines started on different streams
+    never share one.  ``release_workspaces()`` drops the cache (engines keep theirs).
     """
 
     _workspaces: dict = {}
@@ -118,12 +121,7 @@ class PtychoPass:
         nbytes = ctypes.c_size_t(0)
         _lib.check(self.lib.pty_workspace_bytes(ctypes.byref(self.c_problem), ctypes.byref(nbytes)), "workspace")
         self.workspace_bytes = int(nbytes.value)
-        key = (self.device, self.workspace_bytes)
-        ws = PtychoPass._workspaces.get(key)
-        if ws is None:
-            ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
-            PtychoPass._workspaces[key] = ws
-        self.workspace = ws
+        self.workspace = None  # taken at the first run
         self.loss_sse = torch.zeros(2, dtype=torch.float64, device=self.device)
 
     def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
@@ -159,6 +157,13 @@ class PtychoPass:
                 raise ValueError(f"pypty_amd: masks must be float32 ({c.n_masks}, {c.n}, {c.n})")
             if measured.shape[-1] != c.n_masks:
                 raise ValueError("pypty_amd: compressed data need one coefficient per virtual detector")
+        if self.workspace is None:
+            key = (self.device, self.workspace_bytes, _lib.stream_ptr(self.device))
+            ws = PtychoPass._workspaces.get(key)
+            if ws is None:
+                ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
+                PtychoPass._workspaces[key] = ws
+            self.workspace = ws
         out = self.loss_sse if loss_sse is None else loss_sse
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         bufs = _lib.PtyBuffers(
