@@ -142,6 +142,10 @@ typedef struct pty_problem {
    * stream that is being captured runs one lane.  0 or 1: one stream.  The
    * workspace holds one set of per-batch buffers per lane. */
   int lanes;
+  /* 1: also return the gradient of the static background
+   * (objective.py:588-589, this_step_static_background); needs
+   * pty_buffers.static_background and static_background_grad. */
+  int step_static_background;
 } pty_problem;
 
 typedef struct pty_buffers {
@@ -165,6 +169,15 @@ typedef struct pty_buffers {
    * same problem and workspace (a pass split into batches whose patterns are
    * streamed in); loss_sse still holds this call's sums only */
   int accumulate;
+  /* (n, n) static background b, zero frequency at (n/2, n/2), or NULL
+   * (objective.py:359-361, 525-526): the modelled pattern is the mode-mean
+   * intensity plus b^2 times the band mask.  Not with the compressed losses. */
+  const float* static_background;
+  /* (n, n) gradient with respect to b, summed over the pass
+   * (objective.py:377, 588-589): 2 * b * loss_weight * the sum of dL/dI over
+   * the positions, with dL/dI as the reference forms it (carrying the
+   * 1/(M*K) of the mode mean when there is more than one mode). */
+  float* static_background_grad;
 } pty_buffers;
 
 int pty_workspace_bytes(const pty_problem* p, size_t* bytes);

@@ -166,7 +166,8 @@ def scatteradd(full, my, mx, chop):
 def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *, dz, wavelength,
                        pixel_size_x, pixel_size_y, algorithm=LSQ_SQRT, loss_weight=1.0, cutoff=2 / 3,
                        edge=1.0, step_obj=True, step_probe=True, step_pos=False, compute_batch=64,
-                       cdtype=np.complex64, return_patterns=False, masks=None):
+                       cdtype=np.complex64, return_patterns=False, masks=None, static_background=None,
+                       step_static_background=False):
     """objective.loss_and_direction restricted to the far-field multislice path.
 
     Follows pypty/objective.py:218-361 (grids, propagator, shift kernels),
@@ -179,6 +180,12 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     For the compressed losses (objective.py:560-585) measured holds (P, J)
     virtual-detector coefficients and masks the (J, N, N) detectors in the
     DC-centred pattern layout.
+    static_background (N, N), DC-centred, follows objective.py:359-361 (b^2
+    times the DC-centred band mask), :525-526 (added to the pattern after the
+    simulated-pattern copy) and :377, :588-589 (gradient 2 b w sum_b dL/dI,
+    with dL/dI after the 1/(M*K) of :586-587, not band-limited); when it is
+    given the gradient (zeros unless step_static_background) is appended to
+    the result.
     """
     fd = _fdt(cdtype)
     obj = np.asarray(obj, dtype=cdtype)
@@ -200,6 +207,9 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     if compressed:
         masks = np.asarray(masks, dtype=fd)
     patterns = np.zeros((measured.shape[0], N, N), dtype=fd) if return_patterns else None
+    bg = None if static_background is None else np.asarray(static_background, dtype=fd)
+    bg_square = None if bg is None else (bg[None] ** 2) * mask_s[None]
+    bg_grad = None if bg is None else np.zeros((N, N), dtype=fd)
     FP = fft2(probe[None], axes=(1, 2))
     loss, sse = fd(0), fd(0)
     seq = np.asarray(sequence)
@@ -224,6 +234,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
         I = I[:, :, :, 0, 0] if coherent else np.mean(I, axis=(-1, -2))
         if return_patterns:
             patterns[tcs] = I
+        if bg_square is not None:
+            I = I + bg_square
         if algorithm == LSQ_SQRT:
             sse = sse + np.sum((meas - I) ** 2)
             Is, Ds = np.sqrt(I), np.sqrt(meas)
@@ -281,6 +293,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
             raise ValueError(f"unsupported loss {algorithm}")
         if not coherent:
             dI = dI / (M * K)
+        if step_static_background and bg is not None:
+            bg_grad = bg_grad + np.sum(dI, 0) * 2 * bg * loss_weight
         dWave = dI[:, :, :, None, None] * Psi * (loss_weight * N * N)
         g = ifft2_ishift(dWave, axes=(1, 2))
         G, interm = multislice_grads(g, waves, chop, H, mask, edge, step_obj)
@@ -304,6 +318,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     out = (float(loss), float(sse), obj_grad, probe_grad, pos_grad)
     if return_patterns:
         out = out + (patterns,)
+    if bg is not None:
+        out = out + (bg_grad,)
     return out
 
 

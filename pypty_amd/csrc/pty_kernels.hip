@@ -356,6 +356,10 @@ struct BatchArgs {
   float mult;
   float* meas_stage;        // (B, N, N)
   const float* masks;       // (J, N, N)
+  // static background (N, N), zero frequency at (N/2, N/2), or null; its
+  // gradient, summed over the pass, or null when not wanted
+  const float* bg;
+  float* bg_grad;
   // slice range [s_lo, s_hi) of a forward, adjoint or objgrad launch (a whole
   // pass: 0, S) and the stash blocks of st_slots slots per (window, mode).
   // With r = s - s_lo the forward stores slice s in slot r / st_every when
@@ -496,6 +500,11 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
     spec_of<N, LAY>(e, fy, fx);
     const int ys = (fy + N / 2) % N, xs = (fx + N / 2) % N;
     const float d = D[ys * N + xs];
+    const float I_wave = I;  // the simulated pattern leaves the background out (objective.py:517-526)
+    if (a.bg) {
+      const float bgv = a.bg[ys * N + xs];
+      I += bgv * bgv * a.maskc[e];
+    }
     float l;
     if (a.algorithm == PTY_LOSS_ML) {
       const float diff = I - d;
@@ -529,7 +538,7 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
     }
     if (MK > 1) l = l / float(MK);
     a.coef[size_t(b) * N * N + e] = l * scale;
-    if (a.sim) a.sim[size_t(idx) * N * N + ys * N + xs] = I;
+    if (a.sim) a.sim[size_t(idx) * N * N + ys * N + xs] = I_wave;
   }
   const double L = block_sum(lsum, red);
   const double Ssum = block_sum(ssum, red);
@@ -537,6 +546,21 @@ __global__ __launch_bounds__(256) void k_loss(BatchArgs a) {
     a.loss_part[slot] = L;
     a.loss_part[slot + 1] = Ssum;
   }
+}
+
+// Gradient of the static background (objective.py:588-589): per pixel, the
+// batch's dL/dI summed in batch order, times 2 * b * loss_weight, added to the
+// pass sum.  coef holds dL/dI * loss_weight * N^2 (zero for invalid entries).
+template <int N, int LAY>
+__global__ __launch_bounds__(256) void k_bg_grad(BatchArgs a) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= N * N) return;
+  float sum = 0.f;
+  for (int b = 0; b < a.B; ++b) sum += a.coef[size_t(b) * N * N + e];
+  int fy, fx;
+  spec_of<N, LAY>(e, fy, fx);
+  const int p = ((fy + N / 2) % N) * N + (fx + N / 2) % N;
+  a.bg_grad[p] += sum * (2.f / float(N * N)) * a.bg[p];
 }
 
 // Per-batch preprocessing of raw patterns (utils.py:587-653 as
@@ -2476,6 +2500,10 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s, hipEven
     TimeScope ts(KT_POS_FINAL, s);
     k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a);
   }
+  if (a.bg_grad) {
+    TimeScope ts(KT_LOSS, s);
+    k_bg_grad<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(a);
+  }
   if (done_acc && hipEventRecord(done_acc, s) != hipSuccess) return PTY_ERR_HIP;
   return check_launch();
 }
@@ -2514,6 +2542,11 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     gradp = reinterpret_cast<float2*>(ws + L.gradp);
   }
   if (compressed(p->algorithm) && !bf->masks) return PTY_ERR_ARG;
+  if (bf->static_background && compressed(p->algorithm)) return PTY_ERR_ARG;
+  if (p->step_static_background && (!bf->static_background || !bf->static_background_grad)) return PTY_ERR_ARG;
+  if (p->step_static_background && !bf->accumulate &&
+      (rc = zero_fill(bf->static_background_grad, n2 * sizeof(float), s)))
+    return rc;
   if (bf->measured_order != PTY_MEASURED_BY_POSITION && bf->measured_order != PTY_MEASURED_BY_SEQUENCE)
     return PTY_ERR_ARG;
   if (p->step_obj) {
@@ -2546,6 +2579,8 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
   a.meas_stage = L.meas_stage ? reinterpret_cast<float*>(ws + L.meas_stage) : nullptr;
   a.masks = bf->masks;
   a.J = compressed(p->algorithm) ? p->n_masks : 0;
+  a.bg = bf->static_background;
+  a.bg_grad = p->step_static_background ? bf->static_background_grad : nullptr;
   whole_pass(a);
   if (L.ck) {
     // the forward keeps the checkpoints and every wave of the last segment

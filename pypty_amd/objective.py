@@ -6,20 +6,21 @@ lsq_sqrt_2 losses, the lsq / gauss / poisson compressed losses (virtual
 detectors, objective.py:560-585) and object, probe and position gradients
 runs entirely in the gfx950 kernels behind ``pty_loss_and_direction``,
 including the per-batch data preprocessing (shift, bin, multiplier, pad;
-utils.py:587-653) and, with ``load_one_by_one`` and patterns in host memory,
-streaming of each batch's patterns through pinned buffers
-(objective.py:383-408).  Arguments that select features outside that path
-(near field, tilts, static background, aberrations, beam current, phase
-plates, regularisers, upsampling) raise ``NotImplementedError`` instead of
-silently computing something else.
+utils.py:587-653), the static background and its gradient
+(objective.py:359-361, 525-526, 588-589) and, with ``load_one_by_one`` and
+patterns in host memory, streaming of each batch's patterns through pinned
+buffers (objective.py:383-408).  Arguments that select features outside that
+path (near field, tilts, aberrations, beam current, phase plates,
+regularisers, upsampling) raise ``NotImplementedError`` instead of silently
+computing something else.
 
 Binning follows the evident intent of utils.py:639-644 (sum of the
 data_bin x data_bin blocks); the reference's loop indexes an undefined name
 there and fails for data_bin > 1.
 
 ``PtychoPass`` is the allocation-owning engine under it: it caches the
-device workspace per problem shape so repeated passes (line-search trials,
-epochs, benchmark steps) allocate nothing.
+device workspace per problem shape and stream so repeated passes
+(line-search trials, epochs, benchmark steps) allocate nothing.
 """
 from __future__ import annotations
 
@@ -76,6 +77,8 @@ class PassConfig:
     checkpoint_slices: int = 0
     # batch i runs on stream i % lanes of up to four (bit-identical results)
     lanes: int = 4
+    # also return the gradient of the static background (objective.py:588-589)
+    step_static_background: bool = False
 
     @property
     def compressed(self) -> bool:
@@ -93,7 +96,8 @@ class PassConfig:
             float(self.wavelength), float(self.pixel_size_x), float(self.pixel_size_y), float(self.tan_x),
             float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier), int(self.n_masks),
             int(self.raw_n), int(self.data_shift_y), int(self.data_shift_x), int(self.data_bin), int(self.data_pad),
-            float(self.data_multiplier), int(self.checkpoint_slices), int(self.lanes))
+            float(self.data_multiplier), int(self.checkpoint_slices), int(self.lanes),
+            int(bool(self.step_static_background)))
 
 
 class PtychoPass:
@@ -126,13 +130,16 @@ class PtychoPass:
 
     def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
             pos_grad=None, sim_patterns=None, sync=True, masks=None, measured_by_sequence=False, accumulate=False,
-            loss_sse=None):
+            loss_sse=None, static_background=None, static_background_grad=None):
         """Launch the pass.  probe_mnn is mode-major (M, n, n).  Returns (loss, sse) as floats
         when ``sync`` else the device tensor [loss, sse].
 
         measured_by_sequence: row i of ``measured`` holds the pattern of sequence[i] (a streamed batch);
         accumulate: add to the gradients of the previous call on this workspace; loss_sse: a device
-        float64 tensor of 2 elements to receive this call's sums (default: the engine's own)."""
+        float64 tensor of 2 elements to receive this call's sums (default: the engine's own);
+        static_background: (n, n) float32, zero frequency at (n/2, n/2), added squared and band-limited
+        to the modelled patterns (objective.py:359-361, 525-526); static_background_grad: (n, n)
+        float32 receiving its gradient when the config steps it (objective.py:588-589)."""
         c = self.cfg
         for t, name in ((obj, "obj"), (probe_mnn, "probe"), (positions, "positions"), (measured, "measured"),
                         (sequence, "sequence")):
@@ -164,6 +171,15 @@ class PtychoPass:
                 ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
                 PtychoPass._workspaces[key] = ws
             self.workspace = ws
+        for t, name in ((static_background, "static_background"), (static_background_grad, "static_background_grad")):
+            if t is not None:
+                _lib.require_device_tensor(t, name)
+                if t.dtype != torch.float32 or tuple(t.shape) != (c.n, c.n) or not t.is_contiguous():
+                    raise ValueError(f"pypty_amd: {name} must be contiguous float32 ({c.n}, {c.n})")
+        if static_background is not None and c.compressed:
+            raise NotImplementedError("pypty_amd: a static background with the compressed losses")
+        if c.step_static_background and (static_background is None or static_background_grad is None):
+            raise ValueError("static_background and static_background_grad required")
         out = self.loss_sse if loss_sse is None else loss_sse
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         bufs = _lib.PtyBuffers(
@@ -171,7 +187,8 @@ class PtychoPass:
             int(sequence.numel()), ptr(out), ptr(obj_grad), ptr(probe_grad), ptr(pos_grad),
             ptr(sim_patterns), ptr(self.workspace), self.workspace_bytes, ptr(masks) if c.compressed else None,
             _lib.PTY_MEASURED_BY_SEQUENCE if measured_by_sequence else _lib.PTY_MEASURED_BY_POSITION,
-            int(bool(accumulate)))
+            int(bool(accumulate)), ptr(static_background),
+            ptr(static_background_grad) if c.step_static_background else None)
         rc = self.lib.pty_loss_and_direction(ctypes.byref(self.c_problem), ctypes.byref(bufs),
                                               _lib.stream_ptr(self.device))
         _lib.check(rc, "loss_and_direction")
@@ -181,7 +198,8 @@ class PtychoPass:
         return loss, sse
 
     def run_streamed(self, obj, probe_mnn, positions, corrections, measured_host, sequence_host, *, obj_grad=None,
-                     probe_grad=None, pos_grad=None, masks=None):
+                     probe_grad=None, pos_grad=None, masks=None, static_background=None,
+                     static_background_grad=None):
         """The same pass with the measured patterns left in host memory (objective.py:383-408,
         load_one_by_one): each batch's rows are gathered into one of two pinned buffers, copied to the
         device on a side stream while the previous batch computes, and the batch runs as an
@@ -222,7 +240,8 @@ class PtychoPass:
             main.wait_event(copied[slot])
             self.run(obj, probe_mnn, positions, corrections, staged[slot][:sub.numel()], sub, obj_grad=obj_grad,
                      probe_grad=probe_grad, pos_grad=pos_grad, sync=False, masks=masks, measured_by_sequence=True,
-                     accumulate=k > 0, loss_sse=sums[k])
+                     accumulate=k > 0, loss_sse=sums[k], static_background=static_background,
+                     static_background_grad=static_background_grad)
             free[slot].record(main)
             if k + 1 < len(starts):
                 stage(k + 1)  # the host gathers the next batch while this one computes
@@ -291,8 +310,8 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     _unsupported("algorithm_type", algorithm_type not in LOSSES)
     _unsupported("this_step_tilts", bool(this_step_tilts))
     _unsupported("tilt_mode", bool(tilt_mode))
-    _unsupported("static_background", static_background is not None and not np.isscalar(static_background))
-    _unsupported("this_step_static_background", bool(this_step_static_background))
+    _unsupported("static_background with a compressed loss",
+                 static_background is not None and "compressed" in str(algorithm_type))
     _unsupported("aberration_marker", aberration_marker is not None)
     _unsupported("probe_marker", probe_marker is not None)
     _unsupported("beam_current", beam_current is not None)
@@ -337,6 +356,17 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     else:
         measured = _as_device(measured_array, torch.float32, device)
     seq = _as_device(this_chopped_sequence, torch.int32, device)
+    # static background (objective.py:359-361, 525-526, 588-589): a scalar acts on every pixel
+    bg = bg_grad = None
+    step_bg = static_background is not None and bool(this_step_static_background)
+    if static_background is not None:
+        if np.isscalar(static_background):
+            bg = torch.full((n, n), float(static_background), dtype=torch.float32, device=device)
+        else:
+            bg = _as_device(static_background, torch.float32, device)
+            _unsupported("static_background not of the window's size", tuple(bg.shape) != (n, n))
+        if step_bg:
+            bg_grad = torch.empty((n, n), dtype=torch.float32, device=device)
     compressed = "compressed" in algorithm_type
     masks_dev = None
     if compressed:
@@ -361,7 +391,8 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         n_masks=int(masks_dev.shape[0]) if compressed else 0, raw_n=0 if compressed else int(measured.shape[-1]),
         data_shift_y=shift_y, data_shift_x=shift_x, data_bin=1 if compressed else int(data_bin), data_pad=pad,
         data_multiplier=1.0 if compressed else float(data_multiplier), checkpoint_slices=int(checkpoint_slices),
-        lanes=1 if stream else 4)  # a streamed pass runs one batch per call
+        lanes=1 if stream else 4,  # a streamed pass runs one batch per call
+        step_static_background=step_bg)
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None
@@ -369,13 +400,17 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     if stream:
         loss, sse = engine.run_streamed(obj, probe_mnn, positions, corrections, measured, seq.cpu().numpy(),
                                         obj_grad=object_grad, probe_grad=probe_grad_mnn, pos_grad=pos_grad,
-                                        masks=masks_dev)
+                                        masks=masks_dev, static_background=bg, static_background_grad=bg_grad)
     else:
         loss, sse = engine.run(obj, probe_mnn, positions, corrections, measured, seq, obj_grad=object_grad,
-                               probe_grad=probe_grad_mnn, pos_grad=pos_grad, masks=masks_dev)
+                               probe_grad=probe_grad_mnn, pos_grad=pos_grad, masks=masks_dev,
+                               static_background=bg, static_background_grad=bg_grad)
     if loss != loss:
         raise ValueError("A very specific bad thing. Loss is Nan.")
     probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe
                   else torch.zeros_like(full_probe))
     tilts_grad = torch.zeros(tt.shape, dtype=torch.float32, device=device)
-    return loss, sse, object_grad, probe_grad, pos_grad, tilts_grad, 0, 0, 0, np.array([0])
+    # objective.py:377: zeros of the background's shape for an array, 0 for a scalar or none
+    if bg_grad is None:
+        bg_grad = (torch.zeros_like(bg) if bg is not None and not np.isscalar(static_background) else 0)
+    return loss, sse, object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, 0, 0, np.array([0])

@@ -1,0 +1,88 @@
+"""The oracle's static background (pypty/objective.py:359-361, 525-526, 588-589), pinned by:
+a zero background changing nothing; central finite differences of the FP64 loss at pixels inside
+the band (the reference's gradient carries the 1/(M*K) of the mode mean, so with several modes it
+is the derivative divided by M*K); and, outside the band, where the model pattern is exactly zero
+and the background is masked out of the loss, the closed form that the data alone fix."""
+import numpy as np
+import pytest
+
+from oracle import ms_oracle as orc
+from tests._problems import make_problem, rel
+
+
+def _run(pr, bg, step=True, algorithm=None, cdtype=np.complex128, patterns=False):
+    return orc.loss_and_direction(
+        pr["obj"], pr["probe"], pr["pos"], pr["corr"], pr["meas"], pr["seq"], dz=pr["dz"],
+        wavelength=pr["wavelength"], pixel_size_x=pr["px"], pixel_size_y=pr["px"],
+        algorithm=algorithm or pr["algorithm"], step_obj=True, step_probe=True, step_pos=pr["step_pos"],
+        cdtype=cdtype, compute_batch=3, loss_weight=0.7, static_background=bg, step_static_background=step,
+        return_patterns=patterns)
+
+
+def _background(pr, n, seed):
+    rng = np.random.default_rng(seed)
+    return (0.3 * np.sqrt(pr["meas"].mean()) * rng.uniform(0.5, 1.5, (n, n))).astype(np.float64)
+
+
+def _band(n):
+    _, _, _, mask_s, _ = orc.create_spatial_frequencies(0.2, 0.2, n, 2 / 3, 0.0, np.float64)
+    return mask_s > 0
+
+
+def test_zero_background_changes_nothing():
+    n = 16
+    pr = make_problem(n=n, S=2, M=2, ny=40, nx=40, P=5, seed=301)
+    base = _run(pr, None, patterns=True)
+    with_bg = _run(pr, np.zeros((n, n)), patterns=True)
+    assert with_bg[0] == base[0] and with_bg[1] == base[1]
+    for a, b in zip(base[2:6], with_bg[2:6]):
+        assert np.array_equal(a, b)
+    assert not np.any(with_bg[6])
+
+
+def test_simulated_patterns_leave_the_background_out():
+    n = 16
+    pr = make_problem(n=n, S=2, M=1, ny=40, nx=40, P=4, seed=305)
+    base = _run(pr, None, patterns=True)
+    with_bg = _run(pr, _background(pr, n, 1), patterns=True)
+    assert np.array_equal(base[5], with_bg[5])
+    assert with_bg[0] != base[0]
+
+
+@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "ml", "lsq", "lsq_sqrt_2"])
+@pytest.mark.parametrize("M", [1, 2])
+def test_background_gradient_central_fd_inside_the_band(algorithm, M):
+    n = 16
+    pr = make_problem(n=n, S=2, M=M, ny=40, nx=40, P=4, seed=311 + M, algorithm=algorithm)
+    bg = _background(pr, n, 7)
+    grad = _run(pr, bg)[5]
+    band = np.argwhere(_band(n))
+    rng = np.random.default_rng(3)
+    for y, x in band[rng.choice(len(band), 5, replace=False)]:
+        eps = 1e-4 * bg[y, x]
+        up, dn = bg.copy(), bg.copy()
+        up[y, x] += eps
+        dn[y, x] -= eps
+        fd = (_run(pr, up, step=False)[0] - _run(pr, dn, step=False)[0]) / (2 * eps)
+        assert abs(grad[y, x] * M - fd) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(grad).max() * M), (y, x, grad[y, x], fd)
+
+
+@pytest.mark.parametrize("algorithm", ["lsq_sqrt", "lsq"])
+def test_background_gradient_outside_the_band_closed_form(algorithm):
+    n = 16
+    pr = make_problem(n=n, S=2, M=1, ny=40, nx=40, P=5, seed=321, algorithm=algorithm)
+    bg = _background(pr, n, 9)
+    out = _run(pr, bg)
+    grad = out[5]
+    outside = ~_band(n)
+    assert outside.sum() > 0
+    meas = pr["meas"].astype(np.float64)
+    # the model pattern is zero there: dL/dI = -sqrt(D) (lsq_sqrt, with the reference's 0 -> 1
+    # divisor) or -2 D (lsq)
+    dI = -np.sqrt(meas) if algorithm == "lsq_sqrt" else -2 * meas
+    want = 2 * bg * 0.7 * dI.sum(0)
+    assert rel(grad[outside], want[outside]) < 1e-12
+    # and the loss does not depend on the background there
+    bg2 = bg.copy()
+    bg2[outside] *= 3
+    assert _run(pr, bg2, step=False)[0] == out[0]
