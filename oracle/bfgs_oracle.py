@@ -39,11 +39,13 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         return orc.fourier_clean(x, cutoff=cut, fdtype=fd).astype(cdtype)
 
     def evaluate(o, p, c):
+        # a fixed static background (data["static_background"]) enters the loss only
         loss, sse, og, pg, posg = orc.loss_and_direction(
             o, p, state["positions"], c, data["measured"], data["sequence"], dz=data["dz"],
             wavelength=data["wavelength"], pixel_size_x=data["px"], pixel_size_y=data["px"],
             algorithm=data.get("algorithm", "lsq_sqrt"), cutoff=cutoff, step_obj=update_obj,
-            step_probe=update_probe, step_pos=update_pos, compute_batch=data.get("compute_batch", 64), cdtype=cdtype)
+            step_probe=update_probe, step_pos=update_pos, compute_batch=data.get("compute_batch", 64), cdtype=cdtype,
+            static_background=data.get("static_background"))[:5]
         if update_probe:
             pg = clean(pg)
         if update_obj:

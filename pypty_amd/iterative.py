@@ -5,7 +5,8 @@ tuple and its module-global state (``obj``, ``probe``, ``positions``,
 ``positions_correction``, ``tilts``, ``tilts_correction``,
 ``static_background``, ``aberrations_array``, ``beam_current``,
 ``history_bfgs``).  Supported variables: object, probe and sub-pixel
-positions (the ones the benchmark configurations refine).  Every large
+positions (the ones the benchmark configurations refine); a static
+background held in ``static_background`` enters the loss unrefined.  Every large
 operation runs on the HIP device through this package's kernels:
 loss and gradients (``objective.loss_and_direction``), the band-limits
 (``utils.fourier_clean``, plane FFT kernels), the inner products of the
@@ -124,7 +125,7 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             measured_array, algorithm_type, this_wavelength, update_probe, update_obj, update_pos, False, masks,
             pixel_size_x_A, pixel_size_y_A, recon_type, Cs, defocus_array, alpha_near_field, damping_cutoff_multislice,
             0, propmethod, this_chopped_sequence, load_one_by_one, data_multiplier, data_pad, phase_plate_in_h5,
-            this_loss_weight, data_bin, data_shift_vector, upsample_pattern, None, False, tilt_mode,
+            this_loss_weight, data_bin, data_shift_vector, upsample_pattern, static_background, False, tilt_mode,
             aberration_marker, probe_marker, None, compute_batch, phase_only_obj, None, False, False, default_float,
             default_complex, xp, is_first_epoch, scan_size, scan_step_A, fast_axis_reg_weight_positions,
             slow_axis_reg_weight_positions, slow_axis_reg_weight_tilts, current_deformation_reg_weight_positions,
@@ -389,6 +390,21 @@ def _log_loss(folder, row):
         csv.DictWriter(fh, fieldnames=LOSS_CSV_FIELDS).writerow(row)
 
 
+def _static_background_from_nothing(value, n, cutoff):
+    """pypty/utils.py:694-731 without padding or upsampling: None stays None, an array is used as
+    given, a scalar s (at least 1e-7) becomes 0.1 + 0.9 x uniform noise on (n, n) scaled to total s,
+    zero where the DC-centred normalised frequency radius exceeds cutoff / 2."""
+    if value is None or isinstance(value, (np.ndarray, torch.Tensor)):
+        return value
+    total = max(float(value), 1e-7)
+    bg = 0.1 + 0.9 * np.random.rand(n, n)
+    bg *= total / np.sum(bg)
+    f = np.fft.fftshift(np.fft.fftfreq(n))
+    r = np.sum(np.array(np.meshgrid(f, f, indexing="xy")) ** 2, axis=0) ** 0.5
+    bg[r > cutoff * 0.5] = 0
+    return bg.astype(np.float32)
+
+
 def run(pypty_params):
     """Iterative reconstruction (mirror of pypty.iterative.run, pypty/iterative.py:27-436).
 
@@ -402,7 +418,8 @@ def run(pypty_params):
     wolfe_c1_constant, wolfe_c2_constant, max_count, reduce_factor, optimism,
     min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
-    save_loss_log, print_flag, and two keys not in the reference:
+    save_loss_log, print_flag, static_background (held fixed: update_static_background
+    must stay 0), and two keys not in the reference:
     checkpoint_slices (keep every k-th slice's wave and recompute the rest in
     the adjoint) and epoch_callback (called as f(epoch, obj, probe) with the
     device arrays after every epoch, e.g. to track an error metric).
@@ -414,7 +431,7 @@ def run(pypty_params):
     import os
     import time
 
-    global obj, probe, positions, positions_correction, tilts, tilts_correction, history_bfgs
+    global obj, probe, positions, positions_correction, tilts, tilts_correction, static_background, history_bfgs
     get = pypty_params.get
     dev = torch.device("cuda", torch.cuda.current_device())
     dataset = get("dataset", None)
@@ -469,6 +486,7 @@ def run(pypty_params):
     positions_correction = torch.from_numpy(corr.astype(np.float32)).to(dev)
     tilts = np.zeros((1, 6), np.float32)
     tilts_correction = np.zeros((1, 6), np.float32)
+    static_background = _static_background_from_nothing(get("static_background", None), n, float(cutoff))
     measured = dataset if isinstance(dataset, torch.Tensor) else torch.from_numpy(dataset).to(dev)
     folder = get("output_folder", "")
     print_flag = get("print_flag", 3)
@@ -533,7 +551,8 @@ def run(pypty_params):
             max_count=get("max_count", 20), reduce_factor=np.float32(get("reduce_factor", 0.1)),
             optimism=get("optimism", 3), mixed_variance_weight=0, mixed_variance_sigma=0.5, data_bin=1,
             data_shift_vector=[0, 0], smart_memory=False, default_float=np.float32, default_complex=np.complex64,
-            default_int=np.int32, upsample_pattern=1, this_step_static_background=0, tilt_mode=0,
+            default_int=np.int32, upsample_pattern=1,
+            this_step_static_background=_per_epoch(get("update_static_background", 0), epoch), tilt_mode=0,
             fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=None, this_step_aberrations_array=0,
             probe_marker=None, compute_batch=get("compute_batch", 256), current_window=None,
             current_window_weight=0, dynamically_resize_yx_object=False, lazy_clean=False,

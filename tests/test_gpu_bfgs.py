@@ -60,3 +60,44 @@ def test_three_epochs_match_oracle(step_pos):
             assert e_pos < 1e-4
     # the loss decreased over the accepted epochs
     assert hist["prev_loss"] < ro[0] * 1.0000001 or not ro[6]
+
+
+def test_epochs_with_a_fixed_static_background_match_oracle():
+    """A static background held in the module state (pypty/iterative.py:136, 272) enters every loss
+    evaluation of the epoch and is not refined."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 2 epochs with a fixed static background", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=True, seed=23, step_pos=False, noise=0.02)
+    bg = (0.3 * np.sqrt(pr["meas"].mean()) * np.random.default_rng(4).uniform(0.5, 1.5, (32, 32))).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5, "static_background": bg}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.static_background = torch.from_numpy(bg).to(dev)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    try:
+        for epoch in range(2):
+            np.random.seed(200 + epoch)
+            ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, actual_step=step_o)
+            np.random.seed(200 + epoch)
+            rg = call_loss_and_direction(iterative.bfgs_update,
+                                         **bfgs_kwargs(pr, actual_step=step_g, is_first_epoch=(epoch == 0)))
+            step_o, step_g = ro[2], rg[3]
+            print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; count {rg[4]}/{ro[3]}", flush=True)
+            assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+            assert rg[4] == ro[3]
+            assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+            assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
+    finally:
+        iterative.static_background = None

@@ -86,3 +86,26 @@ def test_background_gradient_outside_the_band_closed_form(algorithm):
     bg2 = bg.copy()
     bg2[outside] *= 3
     assert _run(pr, bg2, step=False)[0] == out[0]
+
+
+def test_static_background_start_from_a_scalar():
+    """pypty/utils.py:720-731: a scalar total becomes 0.1-1.0 uniform noise scaled to that total, then
+    zeroed beyond half the cutoff in DC-centred normalised frequency; arrays and None pass through."""
+    from pypty_amd.iterative import _static_background_from_nothing
+
+    assert _static_background_from_nothing(None, 16, 2 / 3) is None
+    a = np.ones((16, 16), np.float32)
+    assert _static_background_from_nothing(a, 16, 2 / 3) is a
+    np.random.seed(0)
+    bg = _static_background_from_nothing(5.0, 16, 2 / 3)
+    f = np.fft.fftshift(np.fft.fftfreq(16))
+    r = np.hypot(f[None, :], f[:, None])
+    inside = r <= (2 / 3) * 0.5
+    assert bg.dtype == np.float32 and bg.shape == (16, 16)
+    assert not np.any(bg[~inside]) and np.all(bg[inside] > 0)
+    assert bg[inside].max() / bg[inside].min() <= 10.0 + 1e-5
+    assert bg.sum() < 5.0
+    np.random.seed(0)
+    raw = 0.1 + 0.9 * np.random.rand(16, 16)
+    assert np.allclose(bg[inside], (raw * 5.0 / raw.sum())[inside], rtol=1e-6)
+    assert _static_background_from_nothing(0.0, 16, 2 / 3).sum() <= 1e-7 * (1 + 1e-6)
