@@ -15,10 +15,11 @@ from . import ms_oracle as orc
 
 def new_history():
     """pypty/iterative.py:1055-1103 (the variables this restatement supports)."""
-    return {"obj_hist_s": [], "probe_hist_s": [], "positions_hist_s": [], "obj_hist_y": [], "probe_hist_y": [],
-            "positions_hist_y": [], "rho_hist": [], "prev_loss": None, "prev_sse": None, "obj_grad": None,
-            "probe_grad": None, "pos_grad": None, "empty_hist": True, "gamma_obj": 1, "gamma_probe": 1,
-            "gamma_pos": 1}
+    return {"obj_hist_s": [], "probe_hist_s": [], "positions_hist_s": [], "static_background_hist_s": [],
+            "obj_hist_y": [], "probe_hist_y": [], "positions_hist_y": [], "static_background_hist_y": [],
+            "rho_hist": [], "prev_loss": None, "prev_sse": None, "obj_grad": None, "probe_grad": None,
+            "pos_grad": None, "static_background_grad": None, "empty_hist": True, "gamma_obj": 1, "gamma_probe": 1,
+            "gamma_pos": 1, "gamma_static": 1}
 
 
 def _d(a, b):
@@ -30,99 +31,102 @@ def _d(a, b):
 
 def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_pos=False, actual_step=1.0,
                cutoff=2 / 3, update_extra_cut=0.005, wolfe_c1=0.1, wolfe_c2=0.9, max_count=20, reduce_factor=0.1,
-               optimism=3, min_step=1e-20, hist_length=10, cdtype=np.complex64):
-    """One epoch; mutates state and hist; returns (loss, sse, actual_step, count, d_value, new_d_value, accepted)."""
+               optimism=3, min_step=1e-20, hist_length=10, cdtype=np.complex64, update_bg=False):
+    """One epoch; mutates state and hist; returns (loss, sse, actual_step, count, d_value, new_d_value, accepted).
+
+    A static background is state["bg"] (N, N) when present; it enters every loss evaluation and,
+    with update_bg, is refined as the fourth variable (iterative.py:480, 569, 604-671, 688, 769,
+    855-862, 898-903, 1030), a real block like the positions."""
     fd = np.float32 if np.dtype(cdtype) == np.complex64 else np.float64
     cut = float(fd(cutoff) - fd(update_extra_cut))
+    bg0 = state.get("bg", data.get("static_background"))
+    update_bg = bool(update_bg) and bg0 is not None
+    on = (update_obj, update_probe, update_pos, update_bg)
+    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s")
+    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y")
+    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static")
+    for k in S_KEYS + Y_KEYS:
+        hist.setdefault(k, [0] * len(hist["rho_hist"]))
+    hist.setdefault("gamma_static", 1)
 
     def clean(x):
         return orc.fourier_clean(x, cutoff=cut, fdtype=fd).astype(cdtype)
 
-    def evaluate(o, p, c):
-        # a fixed static background (data["static_background"]) enters the loss only
-        loss, sse, og, pg, posg = orc.loss_and_direction(
+    def evaluate(v):
+        o, p, c, b = v
+        out = orc.loss_and_direction(
             o, p, state["positions"], c, data["measured"], data["sequence"], dz=data["dz"],
             wavelength=data["wavelength"], pixel_size_x=data["px"], pixel_size_y=data["px"],
             algorithm=data.get("algorithm", "lsq_sqrt"), cutoff=cutoff, step_obj=update_obj,
             step_probe=update_probe, step_pos=update_pos, compute_batch=data.get("compute_batch", 64), cdtype=cdtype,
-            static_background=data.get("static_background"))[:5]
+            static_background=b, step_static_background=update_bg)
+        loss, sse, og, pg, posg = out[:5]
         if update_probe:
             pg = clean(pg)
         if update_obj:
             og = clean(og)
-        return loss, sse, og, pg, posg
+        return loss, sse, [og, pg, posg, out[5] if update_bg else None]
 
-    def ddot(uo, up, ux, go, gp, gx):
-        d = 0.0
-        if update_obj:
-            d += _d(uo, go)
-        if update_probe:
-            d += _d(up, gp)
-        if update_pos:
-            d += _d(ux, gx)
-        return d
+    def ddot(u, g):
+        return sum(_d(a, b) for a, b, f in zip(u, g, on) if f)
 
     if update_probe:
         state["probe"] = clean(state["probe"])
     if update_obj:
         state["obj"] = clean(state["obj"])
-    obj, probe, corr = state["obj"], state["probe"], state["corr"]
+    x = [state["obj"], state["probe"], state["corr"], bg0]
     if hist["empty_hist"]:
-        loss, sse, og, pg, posg = evaluate(obj, probe, corr)
-        uo = -og if update_obj else None
+        loss, sse, g = evaluate(x)
+        uo = -g[0] if update_obj else None
         dvo = _d(uo, uo) if update_obj else 0.0
         if dvo == 0:
             dvo = 1.0
-        up = -pg * fd(dvo / _d(pg, pg)) if update_probe else None
-        ux = -posg * fd(dvo / _d(posg, posg)) if update_pos else None
+        u = [uo] + [-g[j] * fd(dvo / _d(g[j], g[j])) if on[j] else None for j in (1, 2, 3)]
         hist["empty_hist"] = False
     else:
         loss, sse = hist["prev_loss"], hist["prev_sse"]
-        og, pg, posg = hist["obj_grad"], hist["probe_grad"], hist["pos_grad"]
+        g = [hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist.get("static_background_grad")]
         if update_probe:
-            pg = clean(pg)
-        q = [og.copy() if update_obj else None, pg.copy() if update_probe else None,
-             posg.copy() if update_pos else None]
+            g[1] = clean(g[1])
+        q = [g[j].copy() if on[j] else None for j in range(4)]
         rhos = hist["rho_hist"]
         alphas = []
         for i in range(len(rhos) - 1, -1, -1):
-            a = rhos[i] * ddot(hist["obj_hist_s"][i], hist["probe_hist_s"][i], hist["positions_hist_s"][i], *q)
+            a = rhos[i] * ddot([hist[k][i] for k in S_KEYS], q)
             alphas.append(a)
-            for j, key in enumerate(("obj_hist_y", "probe_hist_y", "positions_hist_y")):
+            for j, key in enumerate(Y_KEYS):
                 if q[j] is not None:
                     q[j] = q[j] - fd(a) * hist[key][i]
         alphas = alphas[::-1]
-        for j, key in enumerate(("gamma_obj", "gamma_probe", "gamma_pos")):
-            g = hist[key]
-            if q[j] is not None and g != 0 and g == g:
-                q[j] = q[j] * fd(g)
+        for j, key in enumerate(G_KEYS):
+            gm = hist[key]
+            if q[j] is not None and gm != 0 and gm == gm:
+                q[j] = q[j] * fd(gm)
         for i in range(len(rhos)):
-            beta = alphas[i] - rhos[i] * ddot(hist["obj_hist_y"][i], hist["probe_hist_y"][i],
-                                              hist["positions_hist_y"][i], *q)
-            for j, key in enumerate(("obj_hist_s", "probe_hist_s", "positions_hist_s")):
+            beta = alphas[i] - rhos[i] * ddot([hist[k][i] for k in Y_KEYS], q)
+            for j, key in enumerate(S_KEYS):
                 if q[j] is not None:
                     q[j] = q[j] + fd(beta) * hist[key][i]
-        uo, up, ux = [None if v is None else -v for v in q]
+        u = [None if v is None else -v for v in q]
         if update_obj:
-            uo = clean(uo)
-    d_value = ddot(uo, up, ux, og, pg, posg)
+            u[0] = clean(u[0])
+    d_value = ddot(u, g)
     if d_value >= 0:
-        uo = -og if update_obj else None
-        up = -pg if update_probe else None
-        ux = -posg if update_pos else None
-        gs = [hist["gamma_obj"], hist["gamma_probe"], hist["gamma_pos"]]
-        uo = uo * fd(gs[0]) if (update_obj and gs[0] > 0 and gs[0] == gs[0]) else uo
-        up = up * fd(gs[1]) if (update_probe and gs[1] > 0 and gs[1] == gs[1]) else up
-        ux = ux * fd(gs[2]) if (update_pos and gs[2] > 0 and gs[2] == gs[2]) else ux
-        d_value = ddot(uo, up, ux, og, pg, posg)
+        u = [-g[j] if on[j] else None for j in range(4)]
+        for j, key in enumerate(G_KEYS):
+            gm = hist[key]
+            if on[j] and gm > 0 and gm == gm:
+                u[j] = u[j] * fd(gm)
+        d_value = ddot(u, g)
     count = -1
     while True:
         count += 1
-        nobj = clean(obj + fd(actual_step) * uo) if update_obj else obj
-        nprobe = probe + fd(actual_step) * up if update_probe else probe
-        ncorr = (corr + fd(actual_step) * ux).astype(fd) if update_pos else corr
-        nloss, nsse, nog, npg, nposg = evaluate(nobj, nprobe, ncorr)
-        nd = ddot(uo, up, ux, nog, npg, nposg)
+        nx = [clean(x[0] + fd(actual_step) * u[0]) if update_obj else x[0],
+              x[1] + fd(actual_step) * u[1] if update_probe else x[1],
+              (x[2] + fd(actual_step) * u[2]).astype(fd) if update_pos else x[2],
+              (x[3] + fd(actual_step) * u[3]).astype(fd) if update_bg else x[3]]
+        nloss, nsse, ng = evaluate(nx)
+        nd = ddot(u, ng)
         w1 = orc.wolfe_1(loss, nloss, d_value, actual_step, wolfe_c1)
         w2 = orc.wolfe_2(d_value, nd, wolfe_c2)
         if w1 and w2:
@@ -141,29 +145,27 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         hist.clear()
         hist.update(new_history())
         return loss, sse, actual_step, count, d_value, nd, False
-    s = [fd(actual_step) * uo if update_obj else 0, fd(actual_step) * up if update_probe else 0,
-         fd(actual_step) * ux if update_pos else 0]
-    y = [nog - og if update_obj else 0, npg - pg if update_probe else 0, nposg - posg if update_pos else 0]
-    for j, (ks, ky) in enumerate((("obj_hist_s", "obj_hist_y"), ("probe_hist_s", "probe_hist_y"),
-                                  ("positions_hist_s", "positions_hist_y"))):
-        hist[ks].append(s[j])
-        hist[ky].append(y[j])
+    s = [fd(actual_step) * u[j] if on[j] else 0 for j in range(4)]
+    y = [ng[j] - g[j] if on[j] else 0 for j in range(4)]
+    for j in range(4):
+        hist[S_KEYS[j]].append(s[j])
+        hist[Y_KEYS[j]].append(y[j])
     hist["prev_loss"], hist["prev_sse"] = nloss, nsse
-    hist["obj_grad"], hist["probe_grad"], hist["pos_grad"] = nog, npg, nposg
+    hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist["static_background_grad"] = ng
     tot = 0.0
-    for j, (key, on) in enumerate((("gamma_obj", update_obj), ("gamma_probe", update_probe),
-                                   ("gamma_pos", update_pos))):
-        if on:
+    for j, key in enumerate(G_KEYS):
+        if on[j]:
             ys = _d(y[j], s[j])
             yy = _d(y[j], y[j]) / (2.0 if np.iscomplexobj(y[j]) else 1.0)
             tot += ys
             hist[key] = ys / yy
     hist["rho_hist"].append(1.0 / tot)
     while len(hist["obj_hist_s"]) > hist_length:
-        for key in ("obj_hist_s", "probe_hist_s", "positions_hist_s", "obj_hist_y", "probe_hist_y",
-                    "positions_hist_y", "rho_hist"):
+        for key in S_KEYS + Y_KEYS + ("rho_hist",):
             hist[key].pop(0)
-    state["obj"], state["probe"] = nobj, nprobe
+    state["obj"], state["probe"] = nx[0], nx[1]
     if update_pos:
-        state["corr"] = ncorr
+        state["corr"] = nx[2]
+    if update_bg:
+        state["bg"] = nx[3]
     return loss, sse, actual_step, count, d_value, nd, True

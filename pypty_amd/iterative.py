@@ -5,8 +5,9 @@ tuple and its module-global state (``obj``, ``probe``, ``positions``,
 ``positions_correction``, ``tilts``, ``tilts_correction``,
 ``static_background``, ``aberrations_array``, ``beam_current``,
 ``history_bfgs``).  Supported variables: object, probe and sub-pixel
-positions (the ones the benchmark configurations refine); a static
-background held in ``static_background`` enters the loss unrefined.  Every large
+positions (the ones the benchmark configurations refine), and the static
+background held in ``static_background`` (in every loss evaluation; refined
+with ``this_step_static_background``).  Every large
 operation runs on the HIP device through this package's kernels:
 loss and gradients (``objective.loss_and_direction``), the band-limits
 (``utils.fourier_clean``, plane FFT kernels), the inner products of the
@@ -98,9 +99,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     Returns (total_loss, sse, constraint_contributions, actual_step, count,
     d_value, new_d_value, warnings, counter_failed_epochs).
     """
-    global obj, probe, positions_correction, history_bfgs
+    global obj, probe, positions_correction, static_background, history_bfgs
     _unsupported("this_step_tilts", bool(this_step_tilts))
-    _unsupported("this_step_static_background", bool(this_step_static_background))
     _unsupported("this_step_aberrations_array", bool(this_step_aberrations_array))
     _unsupported("this_beam_current_step", bool(this_beam_current_step))
     _unsupported("phase_only_obj", bool(phase_only_obj))
@@ -116,16 +116,30 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     update_obj = this_step_obj > 0
     update_probe = this_step_probe > 0
     update_pos = this_step_pos_correction > 0
+    # the static background, a real (N, N) block refined like the positions (iterative.py:480, 569, 604-671,
+    # 688, 769, 855-862, 898-903, 1030)
+    update_bg = this_step_static_background > 0
+    if update_bg and static_background is None:
+        raise ValueError("pypty_amd.bfgs_update: refining the static background needs a static_background")
+    if static_background is not None and not isinstance(static_background, torch.Tensor) \
+            and not np.isscalar(static_background):
+        static_background = torch.as_tensor(np.asarray(static_background, np.float32), device=obj.device)
+    if update_bg and not isinstance(static_background, torch.Tensor):
+        static_background = torch.full((probe.shape[0], probe.shape[1]), float(static_background),
+                                       dtype=torch.float32, device=obj.device)
+    for key in ("static_background_hist_s", "static_background_hist_y"):
+        history_bfgs.setdefault(key, [0] * len(history_bfgs["rho_hist"]))
+    history_bfgs.setdefault("gamma_static", 1)
     cut = damping_cutoff_multislice - update_extra_cut
     clean = not lazy_clean
 
-    def evaluate(o, p, pc):
+    def evaluate(o, p, pc, b):
         out = objective.loss_and_direction(
             o, p, positions, pc, tilts, tilts_correction, this_slice_distances, edge_slice_thickness_multiplier,
             measured_array, algorithm_type, this_wavelength, update_probe, update_obj, update_pos, False, masks,
             pixel_size_x_A, pixel_size_y_A, recon_type, Cs, defocus_array, alpha_near_field, damping_cutoff_multislice,
             0, propmethod, this_chopped_sequence, load_one_by_one, data_multiplier, data_pad, phase_plate_in_h5,
-            this_loss_weight, data_bin, data_shift_vector, upsample_pattern, static_background, False, tilt_mode,
+            this_loss_weight, data_bin, data_shift_vector, upsample_pattern, b, update_bg, tilt_mode,
             aberration_marker, probe_marker, None, compute_batch, phase_only_obj, None, False, False, default_float,
             default_complex, xp, is_first_epoch, scan_size, scan_step_A, fast_axis_reg_weight_positions,
             slow_axis_reg_weight_positions, slow_axis_reg_weight_tilts, current_deformation_reg_weight_positions,
@@ -139,88 +153,75 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             pg = _clean(pg, cut, smooth_rolloff)
         if clean and update_obj:
             og = _clean(og, cut, smooth_rolloff)
-        return loss, sse, og, pg, posg, out[9]
+        return loss, sse, og, pg, posg, out[9], (out[6] if update_bg else None)
 
     if update_probe:
         probe = _clean(probe, cut, smooth_rolloff)
     if update_obj and clean:
         obj = _clean(obj, cut, smooth_rolloff)
 
-    def direction_dot(uo, up, ux, go, gp, gx):
+    on = (update_obj, update_probe, update_pos, update_bg)
+    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s")
+    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y")
+    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static")
+
+    def direction_dot(u, g):
         d = 0.0
-        if update_obj:
-            d += _dot2(uo, go)
-        if update_probe:
-            d += _dot2(up, gp)
-        if update_pos:
-            d += _dot2(ux, gx)
+        for a, b, f in zip(u, g, on):
+            if f:
+                d += _dot2(a, b)
         return d
 
     if empty_hist:
-        total_loss, this_sse, og, pg, posg, cc = evaluate(obj, probe, positions_correction)
+        total_loss, this_sse, og, pg, posg, cc, bgg = evaluate(obj, probe, positions_correction, static_background)
+        g = [og, pg, posg, bgg]
         u_obj = _neg(og) if update_obj else None
         d_val_obj = _dot2(u_obj, u_obj) if update_obj else 0.0
         if d_val_obj == 0:
             d_val_obj = 1.0
-        u_probe = lbfgs.lincomb(pg, -d_val_obj / _dot2(pg, pg)) if update_probe else None
-        u_pos = lbfgs.lincomb(posg, -d_val_obj / _dot2(posg, posg)) if update_pos else None
+        u = [u_obj] + [lbfgs.lincomb(g[j], -d_val_obj / _dot2(g[j], g[j])) if on[j] else None for j in (1, 2, 3)]
         history_bfgs["empty_hist"] = False
     else:
         total_loss, this_sse = history_bfgs["prev_loss"], history_bfgs["prev_sse"]
-        og, pg, posg = history_bfgs["obj_grad"], history_bfgs["probe_grad"], history_bfgs["pos_grad"]
+        g = [history_bfgs["obj_grad"], history_bfgs["probe_grad"], history_bfgs["pos_grad"],
+             history_bfgs.get("static_background_grad")]
         cc = history_bfgs["constraint_contributions"]
         if update_probe:
-            pg = _clean(pg, cut, smooth_rolloff)
+            g[1] = _clean(g[1], cut, smooth_rolloff)
         rhos = history_bfgs["rho_hist"]
-        q_obj = og.clone() if update_obj else None
-        q_probe = pg.clone() if update_probe else None
-        q_pos = posg.clone() if update_pos else None
+        q = [g[j].clone() if on[j] else None for j in range(4)]
         alphas = []
         for i in range(len(rhos) - 1, -1, -1):
-            siTq = direction_dot(history_bfgs["obj_hist_s"][i], history_bfgs["probe_hist_s"][i],
-                                 history_bfgs["positions_hist_s"][i], q_obj, q_probe, q_pos)
-            a = rhos[i] * siTq
+            a = rhos[i] * direction_dot([history_bfgs[k][i] for k in S_KEYS], q)
             alphas.append(a)
-            if update_obj:
-                lbfgs.axpby(q_obj, history_bfgs["obj_hist_y"][i], 1.0, -a)
-            if update_probe:
-                lbfgs.axpby(q_probe, history_bfgs["probe_hist_y"][i], 1.0, -a)
-            if update_pos:
-                lbfgs.axpby(q_pos, history_bfgs["positions_hist_y"][i], 1.0, -a)
+            for j, key in enumerate(Y_KEYS):
+                if on[j]:
+                    lbfgs.axpby(q[j], history_bfgs[key][i], 1.0, -a)
         alphas = alphas[::-1]
-        for q, key, on in ((q_obj, "gamma_obj", update_obj), (q_probe, "gamma_probe", update_probe),
-                           (q_pos, "gamma_pos", update_pos)):
-            g = history_bfgs[key]
-            if on and g != 0 and g == g:
-                lbfgs.axpby(q, None, g, 0.0)
+        for j, key in enumerate(G_KEYS):
+            gm = history_bfgs[key]
+            if on[j] and gm != 0 and gm == gm:
+                lbfgs.axpby(q[j], None, gm, 0.0)
         for i in range(len(rhos)):
-            yiTr = direction_dot(history_bfgs["obj_hist_y"][i], history_bfgs["probe_hist_y"][i],
-                                 history_bfgs["positions_hist_y"][i], q_obj, q_probe, q_pos)
-            beta = alphas[i] - rhos[i] * yiTr
-            if update_obj:
-                lbfgs.axpby(q_obj, history_bfgs["obj_hist_s"][i], 1.0, beta)
-            if update_probe:
-                lbfgs.axpby(q_probe, history_bfgs["probe_hist_s"][i], 1.0, beta)
-            if update_pos:
-                lbfgs.axpby(q_pos, history_bfgs["positions_hist_s"][i], 1.0, beta)
-        u_obj = lbfgs.axpby(q_obj, None, -1.0, 0.0) if update_obj else None
-        u_probe = lbfgs.axpby(q_probe, None, -1.0, 0.0) if update_probe else None
-        u_pos = lbfgs.axpby(q_pos, None, -1.0, 0.0) if update_pos else None
+            beta = alphas[i] - rhos[i] * direction_dot([history_bfgs[k][i] for k in Y_KEYS], q)
+            for j, key in enumerate(S_KEYS):
+                if on[j]:
+                    lbfgs.axpby(q[j], history_bfgs[key][i], 1.0, beta)
+        u = [lbfgs.axpby(q[j], None, -1.0, 0.0) if on[j] else None for j in range(4)]
         if clean and update_obj:
-            u_obj = _clean(u_obj, cut, smooth_rolloff)
-    d_value = direction_dot(u_obj, u_probe, u_pos, og, pg, posg)
+            u[0] = _clean(u[0], cut, smooth_rolloff)
+    d_value = direction_dot(u, g)
     if d_value >= 0:
         sys.stdout.write("\n\n\nWarning! Positive dir-derivative, using normal gradient-descent!\n\n\n\n\n\n")
         warnings += "\n\\Positive dir-derivative, using normal gradient-descent!"
-        u_obj = _neg(og) if update_obj else None
-        u_probe = _neg(pg) if update_probe else None
-        u_pos = _neg(posg) if update_pos else None
-        for u, key, on in ((u_obj, "gamma_obj", update_obj), (u_probe, "gamma_probe", update_probe),
-                           (u_pos, "gamma_pos", update_pos)):
-            g = history_bfgs[key]
-            if on and g > 0 and g == g:
-                lbfgs.axpby(u, None, g, 0.0)
-        d_value = direction_dot(u_obj, u_probe, u_pos, og, pg, posg)
+        u = [_neg(g[j]) if on[j] else None for j in range(4)]
+        for j, key in enumerate(G_KEYS):
+            gm = history_bfgs[key]
+            if on[j] and gm > 0 and gm == gm:
+                lbfgs.axpby(u[j], None, gm, 0.0)
+        d_value = direction_dot(u, g)
+    u_obj, u_probe, u_pos, u_bg = u
+    og, pg, posg, bgg = g
     count = -1
     new_d_value = d_value
     this_wolfe_1 = False
@@ -235,8 +236,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         new_probe = lbfgs.axpby(probe.clone(), u_probe, 1.0, actual_step) if update_probe else probe
         new_pc = lbfgs.axpby(positions_correction.clone(), u_pos, 1.0, actual_step) if update_pos \
             else positions_correction
-        new_total_loss, new_sse, nog, npg, nposg, ncc = evaluate(new_obj, new_probe, new_pc)
-        new_d_value = direction_dot(u_obj, u_probe, u_pos, nog, npg, nposg)
+        new_bg = lbfgs.axpby(static_background.clone(), u_bg, 1.0, actual_step) if update_bg \
+            else static_background
+        new_total_loss, new_sse, nog, npg, nposg, ncc, nbgg = evaluate(new_obj, new_probe, new_pc, new_bg)
+        new_d_value = direction_dot(u, (nog, npg, nposg, nbgg))
         this_wolfe_1 = utils.wolfe_1(total_loss, new_total_loss, d_value, actual_step, wolfe_c1_constant)
         this_wolfe_2 = utils.wolfe_2(d_value, new_d_value, wolfe_c2_constant)
         if print_flag >= 3:
@@ -273,22 +276,25 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     s_obj = lbfgs.lincomb(u_obj, actual_step) if update_obj else 0
     s_probe = lbfgs.lincomb(u_probe, actual_step) if update_probe else 0
     s_pos = lbfgs.lincomb(u_pos, actual_step) if update_pos else 0
+    s_bg = lbfgs.lincomb(u_bg, actual_step) if update_bg else 0
     y_obj = lbfgs.axpby(nog.clone(), og, 1.0, -1.0) if update_obj else 0
     y_probe = lbfgs.axpby(npg.clone(), pg, 1.0, -1.0) if update_probe else 0
     y_pos = lbfgs.axpby(nposg.clone(), posg, 1.0, -1.0) if update_pos else 0
+    y_bg = lbfgs.axpby(nbgg.clone(), bgg, 1.0, -1.0) if update_bg else 0
     for key, v in (("obj_hist_s", s_obj), ("probe_hist_s", s_probe), ("positions_hist_s", s_pos),
-                   ("obj_hist_y", y_obj), ("probe_hist_y", y_probe), ("positions_hist_y", y_pos)):
+                   ("static_background_hist_s", s_bg), ("obj_hist_y", y_obj), ("probe_hist_y", y_probe),
+                   ("positions_hist_y", y_pos), ("static_background_hist_y", y_bg)):
         h[key].append(v)
-    for key in ("tilts_hist_s", "beam_current_hist_s", "aberrations_hist_s", "static_background_hist_s",
-                "tilts_hist_y", "beam_current_hist_y", "aberrations_hist_y", "static_background_hist_y"):
+    for key in ("tilts_hist_s", "beam_current_hist_s", "aberrations_hist_s",
+                "tilts_hist_y", "beam_current_hist_y", "aberrations_hist_y"):
         h[key].append(0)
     h["prev_loss"], h["prev_sse"] = new_total_loss, new_sse
-    h["obj_grad"], h["probe_grad"], h["pos_grad"] = nog, npg, nposg
+    h["obj_grad"], h["probe_grad"], h["pos_grad"], h["static_background_grad"] = nog, npg, nposg, nbgg
     h["constraint_contributions"] = ncc
     yiTsi_tot = 0.0
-    for y, s, key, on in ((y_obj, s_obj, "gamma_obj", update_obj), (y_probe, s_probe, "gamma_probe", update_probe),
-                          (y_pos, s_pos, "gamma_pos", update_pos)):
-        if on:
+    for y, s, key, flag in ((y_obj, s_obj, "gamma_obj", update_obj), (y_probe, s_probe, "gamma_probe", update_probe),
+                            (y_pos, s_pos, "gamma_pos", update_pos), (y_bg, s_bg, "gamma_static", update_bg)):
+        if flag:
             (yy,) = lbfgs.dots([(y, y)])
             ys = _dot2(y, s)
             yiTsi_tot += ys
@@ -302,6 +308,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     obj = new_obj
     if update_pos:
         positions_correction = new_pc
+    if update_bg:
+        static_background = new_bg
     return total_loss, this_sse, cc, actual_step, count, d_value, new_d_value, warnings, counter_failed_epochs
 
 
@@ -418,8 +426,8 @@ def run(pypty_params):
     wolfe_c1_constant, wolfe_c2_constant, max_count, reduce_factor, optimism,
     min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
-    save_loss_log, print_flag, static_background (held fixed: update_static_background
-    must stay 0), and two keys not in the reference:
+    save_loss_log, print_flag, static_background and update_static_background, and two keys
+    not in the reference:
     checkpoint_slices (keep every k-th slice's wave and recompute the rest in
     the adjoint) and epoch_callback (called as f(epoch, obj, probe) with the
     device arrays after every epoch, e.g. to track an error metric).
@@ -512,6 +520,7 @@ def run(pypty_params):
         s_obj = _per_epoch(get("update_obj", 1), epoch)
         s_probe = _per_epoch(get("update_probe", 1), epoch)
         s_pos = _per_epoch(get("update_probe_pos", 0), epoch)
+        s_bg = _per_epoch(get("update_static_background", 0), epoch) if static_background is not None else 0
         first_step = _per_epoch(step_param, epoch)  # per epoch, as pypty/iterative.py:325-336 reads it
         # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, probe after 6
         if counter_failed >= 2:
@@ -524,7 +533,7 @@ def run(pypty_params):
             s_pos = 0
         if abs(fail_probe - epoch) < 10:
             s_probe = 0
-        steps = sum(int(x != 0) for x in (s_obj, s_probe, s_pos))
+        steps = sum(int(x != 0) for x in (s_obj, s_probe, s_pos, s_bg))
         if epoch != int(get("epoch_prev", 0)) and steps != prev_steps:
             reset_bfgs_history()
             warnings += "\nResetting the history!"
@@ -552,7 +561,7 @@ def run(pypty_params):
             optimism=get("optimism", 3), mixed_variance_weight=0, mixed_variance_sigma=0.5, data_bin=1,
             data_shift_vector=[0, 0], smart_memory=False, default_float=np.float32, default_complex=np.complex64,
             default_int=np.int32, upsample_pattern=1,
-            this_step_static_background=_per_epoch(get("update_static_background", 0), epoch), tilt_mode=0,
+            this_step_static_background=s_bg, tilt_mode=0,
             fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=None, this_step_aberrations_array=0,
             probe_marker=None, compute_batch=get("compute_batch", 256), current_window=None,
             current_window_weight=0, dynamically_resize_yx_object=False, lazy_clean=False,

@@ -101,3 +101,52 @@ def test_epochs_with_a_fixed_static_background_match_oracle():
             assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
     finally:
         iterative.static_background = None
+
+
+def test_epochs_refining_the_static_background_match_oracle():
+    """The static background refined as a fourth variable with object and probe (iterative.py:480,
+    569-1030): losses, line-search counts and the updated object, probe and background follow the
+    oracle over three epochs."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 3 epochs refining object, probe and static background", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=True, seed=29, step_pos=False, noise=0.02)
+    bg = (0.3 * np.sqrt(pr["meas"].mean()) * np.random.default_rng(6).uniform(0.5, 1.5, (32, 32))).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy(),
+             "bg": bg.copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.static_background = torch.from_numpy(bg.copy()).to(dev)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    try:
+        for epoch in range(3):
+            np.random.seed(300 + epoch)
+            ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, update_bg=True, actual_step=step_o)
+            np.random.seed(300 + epoch)
+            rg = call_loss_and_direction(iterative.bfgs_update,
+                                         **bfgs_kwargs(pr, actual_step=step_g, is_first_epoch=(epoch == 0),
+                                                       this_step_static_background=1))
+            step_o, step_g = ro[2], rg[3]
+            e_bg = rel(iterative.static_background.cpu().numpy(), state["bg"])
+            print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; step {rg[3]:.3e}/{ro[2]:.3e}; "
+                  f"count {rg[4]}/{ro[3]}; rel err background {e_bg:.2e}", flush=True)
+            assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+            assert rg[4] == ro[3]
+            assert abs(rg[3] - ro[2]) <= 1e-6 * abs(ro[2])
+            assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+            assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
+            assert e_bg < 1e-4
+        assert float(np.abs(state["bg"] - bg).max()) > 0  # the background moved
+    finally:
+        iterative.static_background = None

@@ -109,3 +109,35 @@ def test_static_background_start_from_a_scalar():
     raw = 0.1 + 0.9 * np.random.rand(16, 16)
     assert np.allclose(bg[inside], (raw * 5.0 / raw.sum())[inside], rtol=1e-6)
     assert _static_background_from_nothing(0.0, 16, 2 / 3).sum() <= 1e-7 * (1 + 1e-6)
+
+
+def test_oracle_lbfgs_refines_the_background_towards_the_true_one():
+    """Data made of the model patterns of the current object plus a known background: refining the
+    background alone (iterative.py:480, 569-1030 as the oracle restates them) lowers the loss every
+    epoch and moves the background inside the band towards the true one."""
+    from oracle import bfgs_oracle
+
+    n = 16
+    pr = make_problem(n=n, S=2, M=1, ny=40, nx=40, P=6, seed=331)
+    I = orc.loss_and_direction(pr["obj"], pr["probe"], pr["pos"], pr["corr"], np.zeros((6, n, n)), pr["seq"],
+                               dz=pr["dz"], wavelength=pr["wavelength"], pixel_size_x=pr["px"], pixel_size_y=pr["px"],
+                               step_obj=False, step_probe=False, cdtype=np.complex128, return_patterns=True)[5]
+    band = _band(n)
+    bg_true = _background(pr, n, 11) * band
+    meas = I + (bg_true ** 2)[None]
+    state = {"obj": pr["obj"].astype(np.complex128), "probe": pr["probe"].astype(np.complex128),
+             "positions": pr["pos"], "corr": pr["corr"].astype(np.float64), "bg": 0.5 * bg_true}
+    data = {"measured": meas, "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"], "px": pr["px"],
+            "compute_batch": 4, "algorithm": "lsq"}
+    hist = bfgs_oracle.new_history()
+    err0 = rel(state["bg"][band], bg_true[band])
+    losses, step = [], 1.0
+    np.random.seed(1)
+    for _ in range(15):
+        loss, _, step, _, _, _, ok = bfgs_oracle.bfgs_epoch(state, hist, data, update_obj=False, update_probe=False,
+                                                            update_bg=True, actual_step=step, cdtype=np.complex128)
+        assert ok
+        losses.append(loss)
+    assert all(b < a for a, b in zip(losses, losses[1:]))
+    assert rel(state["bg"][band], bg_true[band]) < 0.05 * err0
+    assert not np.any(state["bg"][~band])
