@@ -322,8 +322,9 @@ def _per_epoch(v, epoch):
     return v(epoch) if callable(v) else v
 
 
-def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos):
-    """co.npy / cp.npy / cg.npy as pypty/utils.py:2085-2112 writes them."""
+def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None):
+    """co.npy / cp.npy / cg.npy, and cs.npy for a background being refined, as pypty/utils.py:2085-2118
+    writes them."""
     import os
 
     if not folder:
@@ -335,6 +336,8 @@ def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos):
         np.save(os.path.join(folder, "cp.npy"), p.cpu().numpy())
     if save_pos:
         np.save(os.path.join(folder, "cg.npy"), (pos.double() + corr.double()).cpu().numpy())
+    if isinstance(bg, torch.Tensor):
+        np.save(os.path.join(folder, "cs.npy"), bg.cpu().numpy())
 
 
 def lambda_source(f):
@@ -581,7 +584,8 @@ def run(pypty_params):
             print(f"--> epoch {epoch}: loss {total_loss:.6e} sse {sse:.6e} step {actual_step:.3e} "
                   f"linesearch {count + 1} time {time.time() - t0:.2f} s", flush=True)
         if save_inter and epoch % int(save_inter) == 0:
-            _save_checkpoint(folder, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos)
+            _save_checkpoint(folder, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
+                             static_background if s_bg else None)
         if save_log:
             free, total = torch.cuda.mem_get_info(dev)
             _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,

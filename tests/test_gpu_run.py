@@ -48,3 +48,51 @@ def test_run_reconstructs_and_writes_reference_files(tmp_path):
     assert np.load(out + "cg.npy").shape == (144, 2)
     rows = list(csv.reader(open(out + "loss.csv")))
     assert len(rows) == 8 and len(rows[0]) == 13
+
+
+def test_run_refines_a_static_background_and_writes_it(tmp_path):
+    """Data with a known background added, reconstructed from the true object with object and
+    background refined together (from a flat object the background absorbs the object's misfit; the
+    background alone takes the reference's first step scaled by an object step of zero, which is
+    far too short at these intensities): run() lowers the loss, refines the background and writes
+    cs.npy (pypty/utils.py:2112-2114)."""
+    import torch
+
+    from pypty_amd import iterative, synth
+
+    print("[run] 6 epochs refining object and static background", flush=True)
+    dev = torch.device("cuda", 0)
+    px, n = 0.2, 64
+    true_obj = synth.hex_lattice_object(120, 120, px, 2.46, 0.3, 0.5, seed=15)
+    probe = synth.make_probe(n, px, 300.0, 20.0, 100.0, 1, 0.0, seed=16)
+    pos = synth.raster_scan(8, 8, 6.0, origin_px=(0.0, 0.0), jitter_px=0.0, seed=17)  # run() keeps them
+    ints, corr = synth.split_positions(pos)
+    meas = synth.simulate(true_obj, probe, ints, corr, px, 2.0, synth.electron_wavelength(300.0), 1e5, seed=18,
+                          device=dev)
+    f = np.fft.fftshift(np.fft.fftfreq(n, px))
+    q = np.hypot(f[None, :], f[:, None])
+    band = q <= np.abs(np.fft.fftfreq(n, px)).max() * (2 / 3)  # the DC-centred band mask at cutoff 2/3
+    rng = np.random.default_rng(19)
+    bg_true = (0.5 * np.sqrt(meas.mean()) * rng.uniform(0.5, 1.5, (n, n)) * band).astype(np.float32)
+    meas = (meas + bg_true[None] ** 2).astype(np.float32)
+    bg0 = (0.5 * bg_true).astype(np.float32)
+    out = str(tmp_path) + os.sep
+    params = dict(dataset=meas, probe=probe, positions=pos, obj=true_obj.astype(np.complex64),
+                  acc_voltage=300.0, pixel_size_x_A=px, pixel_size_y_A=px, slice_distances=np.array([2.0]),
+                  num_slices=1, algorithm="lsq", epoch_max=6, update_obj=1, update_probe=0, update_probe_pos=0,
+                  static_background=bg0, update_static_background=1, compute_batch=32, output_folder=out,
+                  save_inter_checkpoints=1, print_flag=0)
+    try:
+        res = iterative.run(params)
+        losses = res["losses"]
+        print(f"  loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+        assert all(b <= a * (1 + 1e-9) for a, b in zip(losses, losses[1:])), losses
+        assert losses[-1] < losses[0]
+        cs = np.load(out + "cs.npy")
+        assert cs.shape == (n, n) and cs.dtype == np.float32
+        e0 = np.linalg.norm((bg0 - bg_true)[band])
+        e = np.linalg.norm((cs - bg_true)[band])
+        print(f"  background error {e0:.4e} -> {e:.4e}", flush=True)
+        assert np.abs(cs - bg0).max() > 0 and np.isfinite(cs).all()
+    finally:
+        iterative.static_background = None
