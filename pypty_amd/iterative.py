@@ -166,9 +166,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static")
 
     def direction_dot(u, g):
+        # a history entry of 0 (the variable was not refined in that epoch) adds 0, as in the reference
         d = 0.0
         for a, b, f in zip(u, g, on):
-            if f:
+            if f and isinstance(a, torch.Tensor) and isinstance(b, torch.Tensor):
                 d += _dot2(a, b)
         return d
 
@@ -195,7 +196,7 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             a = rhos[i] * direction_dot([history_bfgs[k][i] for k in S_KEYS], q)
             alphas.append(a)
             for j, key in enumerate(Y_KEYS):
-                if on[j]:
+                if on[j] and isinstance(history_bfgs[key][i], torch.Tensor):
                     lbfgs.axpby(q[j], history_bfgs[key][i], 1.0, -a)
         alphas = alphas[::-1]
         for j, key in enumerate(G_KEYS):
@@ -205,7 +206,7 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         for i in range(len(rhos)):
             beta = alphas[i] - rhos[i] * direction_dot([history_bfgs[k][i] for k in Y_KEYS], q)
             for j, key in enumerate(S_KEYS):
-                if on[j]:
+                if on[j] and isinstance(history_bfgs[key][i], torch.Tensor):
                     lbfgs.axpby(q[j], history_bfgs[key][i], 1.0, beta)
         u = [lbfgs.axpby(q[j], None, -1.0, 0.0) if on[j] else None for j in range(4)]
         if clean and update_obj:
