@@ -238,6 +238,15 @@ int pty_multidot_c64(const void* const* a, const void* const* b, int npairs, int
  * the axpy steps of the two-loop recursion and of the trial point. */
 int pty_axpby_c64(void* y, const void* x, int64_t n, float alpha, float beta, void* stream);
 
+/* L1 norms of the object's phase and log-amplitude (pypty/objective.py:1286-1320,
+ * compute_full_l1_constraint, called at :811-830): *out = phase_weight *
+ * sum |arg obj| + abs_weight * sum |log |obj|| (FP64 sums), and, when grad is
+ * not NULL, grad += 0.5 (abs_weight sign(log |obj|) + i phase_weight
+ * sign(arg obj)) / conj(obj).  A weight <= 0 contributes nothing.  n complex64
+ * elements; partial: device scratch of 512 doubles; out: device, 1 double. */
+int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float phase_weight, void* grad,
+                          double* out, double* partial, void* stream);
+
 /* out = 2 Re(-i conj(z) g): Wirtinger gradient -> phase gradient
  * (pypty/utils.py:260-276, complex_grad_to_phase_grad); out float32, n elements. */
 int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream);

@@ -323,6 +323,23 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     return out
 
 
+def full_l1_constraint(obj, abs_norm_weight, phase_norm_weight):
+    """pypty/objective.py:1286-1320 (compute_full_l1_constraint with return_direction):
+    reg = phase_w sum |angle O| + abs_w sum |log |O||, grad = 0.5 (abs_w sign(log |O|) +
+    i phase_w sign(angle O)) / conj(O); a weight <= 0 contributes nothing."""
+    reg = 0.0
+    grad = np.zeros_like(obj)
+    if phase_norm_weight > 0:
+        ph = np.angle(obj)
+        reg += phase_norm_weight * np.sum(np.abs(ph))
+        grad = grad + 1j * phase_norm_weight * np.sign(ph)
+    if abs_norm_weight > 0:
+        la = np.log(np.abs(obj))
+        reg += abs_norm_weight * np.sum(np.abs(la))
+        grad = grad + abs_norm_weight * np.sign(la)
+    return float(reg), (0.5 * grad / np.conj(obj)).astype(obj.dtype)
+
+
 def preprocess_dataset(dataset, data_shift_vector=(0, 0), data_bin=1, data_pad=0, data_multiplier=1.0,
                        upsample_pattern=1):
     """pypty/utils.py:626-651 as objective.py:400-405 calls it per batch (not compressed, force_pad):

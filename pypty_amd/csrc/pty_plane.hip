@@ -321,6 +321,42 @@ __global__ __launch_bounds__(256) void k_axpby(float* __restrict__ y, const floa
     y[i] = (alpha == 0.f ? 0.f : alpha * y[i]) + (x ? beta * x[i] : 0.f);
 }
 
+// L1 norms of the object's phase and log-amplitude (objective.py:1286-1320):
+// per block the FP64 sum of phase_w |arg O| + abs_w |log |O||, and, when grad
+// is set, grad += 0.5 (abs_w sign(log |O|) + i phase_w sign(arg O)) / conj(O).
+// A weight <= 0 contributes nothing.
+__global__ __launch_bounds__(256) void k_l1(const float2* __restrict__ obj, size_t n, float abs_w, float phase_w,
+                                            float2* __restrict__ grad, double* __restrict__ partial) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const float2 o = obj[i];
+    float cre = 0.f, cim = 0.f;
+    if (phase_w > 0.f) {
+      const float ph = atan2f(o.y, o.x);
+      s += double(phase_w) * double(fabsf(ph));
+      cim = 0.5f * phase_w * float((ph > 0.f) - (ph < 0.f));
+    }
+    if (abs_w > 0.f) {
+      const float la = logf(sqrtf(o.x * o.x + o.y * o.y));
+      s += double(abs_w) * double(fabsf(la));
+      cre = 0.5f * abs_w * float((la > 0.f) - (la < 0.f));
+    }
+    if (grad) {
+      // c / conj(o) = c o / |o|^2
+      const float r2 = o.x * o.x + o.y * o.y;
+      float2 g = grad[i];
+      g.x += (cre * o.x - cim * o.y) / r2;
+      g.y += (cre * o.y + cim * o.x) / r2;
+      grad[i] = g;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
 __global__ __launch_bounds__(256) void k_phase_grad(const float2* __restrict__ g, const float2* __restrict__ z,
                                                     float* __restrict__ out, size_t n) {
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
@@ -427,6 +463,17 @@ int pty_axpby_c64(void* y, const void* x, int64_t n, float alpha, float beta, vo
   const size_t nf = size_t(n) * 2;
   const int grid = int(std::min<size_t>((nf + 255) / 256, 8192));
   k_axpby<<<grid, 256, 0, s>>>(static_cast<float*>(y), static_cast<const float*>(x), nf, alpha, beta);
+  return plane_launch_check();
+}
+
+int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float phase_weight, void* grad,
+                          double* out, double* partial, void* stream) {
+  if (!obj || n < 0 || !out || !partial) return PTY_ERR_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int cols = 512;
+  k_l1<<<cols, 256, 0, s>>>(static_cast<const float2*>(obj), size_t(n), abs_weight, phase_weight,
+                            static_cast<float2*>(grad), partial);
+  k_reduce_rows<<<1, 256, 0, s>>>(partial, cols, out);
   return plane_launch_check();
 }
 

@@ -10,9 +10,10 @@ utils.py:587-653), the static background and its gradient
 (objective.py:359-361, 525-526, 588-589) and, with ``load_one_by_one`` and
 patterns in host memory, streaming of each batch's patterns through pinned
 buffers (objective.py:383-408).  Arguments that select features outside that
-path (near field, tilts, aberrations, beam current, phase plates,
-regularisers, upsampling) raise ``NotImplementedError`` instead of silently
-computing something else.
+path (near field, tilts, aberrations, beam current, phase plates, the
+regularisers other than the L1 norms of the object's phase and log-amplitude,
+upsampling) raise ``NotImplementedError`` instead of silently computing
+something else.
 
 Binning follows the evident intent of utils.py:639-644 (sum of the
 data_bin x data_bin blocks); the reference's loop indexes an undefined name
@@ -249,6 +250,31 @@ class PtychoPass:
         return loss, sse
 
 
+def compute_full_l1_constraint(object, abs_norm_weight, phase_norm_weight, grad_mask=None, return_direction=True,
+                               smart_memory=False, *, out_grad=None):
+    """Mirror of pypty/objective.py:1286-1320: L1 norms of the phase (phase_norm_weight) and of the
+    log-amplitude (abs_norm_weight) of a complex64 object on the device, through
+    ``pty_l1_constraint_c64``.  Returns (reg_term, grad): grad is 0.5 (abs_w sign(log |O|) +
+    i phase_w sign(arg O)) / conj(O), or None without return_direction.  A weight <= 0 contributes
+    nothing, as there.  out_grad (keyword only, not in the reference): add the gradient into this
+    tensor instead of a new one.  grad_mask is unused, as in the reference."""
+    _lib.require_device_tensor(object, "object")
+    if object.dtype != torch.complex64 or not object.is_contiguous():
+        raise TypeError("pypty_amd: the object must be contiguous complex64")
+    grad = None
+    if return_direction:
+        grad = out_grad if out_grad is not None else torch.zeros_like(object)
+        if grad.shape != object.shape or grad.dtype != torch.complex64 or not grad.is_contiguous():
+            raise ValueError("pypty_amd: out_grad must be a contiguous complex64 tensor of the object's shape")
+    out = torch.empty(1, dtype=torch.float64, device=object.device)
+    partial = torch.empty(512, dtype=torch.float64, device=object.device)
+    rc = _lib.load().pty_l1_constraint_c64(object.data_ptr(), object.numel(), float(abs_norm_weight),
+                                           float(phase_norm_weight), grad.data_ptr() if grad is not None else None,
+                                           out.data_ptr(), partial.data_ptr(), _lib.stream_ptr(object.device))
+    _lib.check(rc, "l1_constraint")
+    return float(out.item()), grad
+
+
 def check_windows_inside(positions: torch.Tensor, sequence: torch.Tensor, n: int, ny: int, nx: int) -> None:
     """Raise if a window [y, y+n) x [x, x+n) of the sequence leaves the object.
 
@@ -324,7 +350,6 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     for name, w in (("fast_axis_reg_weight_positions", fast_axis_reg_weight_positions),
                     ("slow_axis_reg_weight_positions", slow_axis_reg_weight_positions),
                     ("probe_reg_weight", probe_reg_weight), ("current_window_weight", current_window_weight),
-                    ("phase_norm_weight", phase_norm_weight), ("abs_norm_weight", abs_norm_weight),
                     ("atv_weight", atv_weight), ("mixed_variance_weight", mixed_variance_weight),
                     ("wedge_mu", wedge_mu), ("probe_mod_reg_weight", probe_mod_reg_weight)):
         _unsupported(name, bool(w))
@@ -410,7 +435,16 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe
                   else torch.zeros_like(full_probe))
     tilts_grad = torch.zeros(tt.shape, dtype=torch.float32, device=device)
+    # constraints (objective.py:742-921): the L1 norms of the object's phase and log-amplitude are the
+    # fifth and sixth of the reference's fourteen contributions; they add to the loss (after the loss
+    # weight) and to the object gradient
+    constraint_contributions = np.zeros(14)
+    for k, (wa, wp) in ((4, (0.0, phase_norm_weight)), (5, (abs_norm_weight, 0.0))):
+        if wa != 0 or wp != 0:
+            reg, _ = compute_full_l1_constraint(obj, wa, wp, None, True, smart_memory, out_grad=object_grad)
+            loss += reg
+            constraint_contributions[k] = reg
     # objective.py:377: zeros of the background's shape for an array, 0 for a scalar or none
     if bg_grad is None:
         bg_grad = (torch.zeros_like(bg) if bg is not None and not np.isscalar(static_background) else 0)
-    return loss, sse, object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, 0, 0, np.array([0])
+    return loss, sse, object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, 0, 0, constraint_contributions

@@ -141,3 +141,23 @@ def test_oracle_lbfgs_refines_the_background_towards_the_true_one():
     assert all(b < a for a, b in zip(losses, losses[1:]))
     assert rel(state["bg"][band], bg_true[band]) < 0.05 * err0
     assert not np.any(state["bg"][~band])
+
+
+def test_l1_constraint_closed_form_and_central_fd():
+    """pypty/objective.py:1286-1320: on a constant object r e^{i phi} the terms are n (w_p |phi| +
+    w_a |log r|); for a generic object the gradient g satisfies dL = 2 Re sum conj(g) dO (the
+    convention of every object gradient here), checked by central differences; weights <= 0 add
+    nothing."""
+    n = 200
+    o = np.full(n, 0.8 * np.exp(-0.7j), np.complex128)
+    reg, _ = orc.full_l1_constraint(o, 0.3, 0.2)
+    assert abs(reg - n * (0.2 * 0.7 + 0.3 * abs(np.log(0.8)))) < 1e-12 * n
+    assert orc.full_l1_constraint(o, -1.0, 0.0)[0] == 0.0 and not np.any(orc.full_l1_constraint(o, 0.0, -2.0)[1])
+    rng = np.random.default_rng(12)
+    o = (rng.uniform(0.5, 1.5, n) * np.exp(1j * rng.uniform(-2.5, 2.5, n))).astype(np.complex128)
+    reg, g = orc.full_l1_constraint(o, 0.3, 0.2)
+    for _ in range(4):
+        d = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        eps = 1e-7
+        fd = (orc.full_l1_constraint(o + eps * d, 0.3, 0.2)[0] - orc.full_l1_constraint(o - eps * d, 0.3, 0.2)[0]) / (2 * eps)
+        assert abs(fd - 2 * np.real(np.sum(np.conj(g) * d))) < 1e-6 * abs(fd)
