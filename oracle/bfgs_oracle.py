@@ -61,6 +61,12 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
             step_probe=update_probe, step_pos=update_pos, compute_batch=data.get("compute_batch", 64), cdtype=cdtype,
             static_background=b, step_static_background=update_bg)
         loss, sse, og, pg, posg = out[:5]
+        # L1 constraints of the object (objective.py:811-830), in the loss and object gradient
+        for wa, wp in ((0.0, data.get("phase_norm_weight", 0)), (data.get("abs_norm_weight", 0), 0.0)):
+            if wa != 0 or wp != 0:
+                reg, gl = orc.full_l1_constraint(o, wa, wp)
+                loss += reg
+                og = (og + gl).astype(og.dtype)
         if update_probe:
             pg = clean(pg)
         if update_obj:

@@ -430,7 +430,8 @@ def run(pypty_params):
     wolfe_c1_constant, wolfe_c2_constant, max_count, reduce_factor, optimism,
     min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
-    save_loss_log, print_flag, static_background and update_static_background, and two keys
+    save_loss_log, print_flag, static_background and update_static_background,
+    phase_norm_weight and abs_norm_weight (values or callables of the epoch), and two keys
     not in the reference:
     checkpoint_slices (keep every k-th slice's wave and recompute the rest in
     the adjoint) and epoch_callback (called as f(epoch, obj, probe) with the
@@ -550,7 +551,9 @@ def run(pypty_params):
             algorithm_type=get("algorithm", "lsq_sqrt"), this_slice_distances=slice_distances,
             edge_slice_thickness_multiplier=1.0, this_step_probe=s_probe, this_step_obj=s_obj,
             this_step_pos_correction=s_pos, this_step_tilts=0, measured_array=measured, this_wavelength=wavelength,
-            masks=None, pixel_size_x_A=px, pixel_size_y_A=py, phase_norm_weight=0, abs_norm_weight=0,
+            masks=None, pixel_size_x_A=px, pixel_size_y_A=py,
+            phase_norm_weight=_per_epoch(get("phase_norm_weight", 0), epoch),
+            abs_norm_weight=_per_epoch(get("abs_norm_weight", 0), epoch),
             stepsize_threshold_low=get("min_step", 1e-20), probe_reg_weight=0, aperture_mask=None,
             recon_type="far_field", defocus_array=np.array([0.0]), Cs=0.0, alpha_near_field=0.0,
             damping_cutoff_multislice=cutoff, smooth_rolloff=rolloff,

@@ -150,3 +150,41 @@ def test_epochs_refining_the_static_background_match_oracle():
         assert float(np.abs(state["bg"] - bg).max()) > 0  # the background moved
     finally:
         iterative.static_background = None
+
+
+def test_epochs_with_l1_constraints_match_oracle():
+    """phase_norm_weight and abs_norm_weight (objective.py:811-830) enter every loss evaluation and
+    object gradient of the epoch."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 2 epochs with L1 constraints", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=True, seed=33, step_pos=False, noise=0.02)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5, "phase_norm_weight": 5.0,
+            "abs_norm_weight": 20.0}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    for epoch in range(2):
+        np.random.seed(400 + epoch)
+        ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, actual_step=step_o)
+        np.random.seed(400 + epoch)
+        rg = call_loss_and_direction(iterative.bfgs_update,
+                                     **bfgs_kwargs(pr, actual_step=step_g, is_first_epoch=(epoch == 0),
+                                                   phase_norm_weight=5.0, abs_norm_weight=20.0))
+        step_o, step_g = ro[2], rg[3]
+        print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; count {rg[4]}/{ro[3]}", flush=True)
+        assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+        assert rg[4] == ro[3]
+        assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+        assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
