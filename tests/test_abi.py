@@ -139,3 +139,33 @@ def test_batch_lanes_add_one_set_of_per_batch_buffers(lib):
     assert (ws(2, 11)[1] - two) == 2 * slice_block and (ws(1, 11)[1] - one) == slice_block
     assert ws(3)[1] - two == two - one and ws(4)[1] - ws(3)[1] == two - one
     assert ws(5)[0] == -1 and ws(-1)[0] == -1
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """The ctypes mirrors of pty_problem and pty_buffers have the header's size and field offsets
+    (a C program built with gcc from the header prints them)."""
+    import shutil
+    import subprocess
+
+    from pypty_amd import _lib
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pypty_amd.h"', "int main(void) {"]
+    for cname, cls in (("pty_problem", _lib.PtyProblem), ("pty_buffers", _lib.PtyBuffers)):
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for row in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, value = row.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in (("pty_problem", _lib.PtyProblem), ("pty_buffers", _lib.PtyBuffers)):
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
