@@ -23,7 +23,10 @@ def new_history():
 
 
 def _d(a, b):
-    """2 sum Re(a conj(b)) for complex blocks, sum a b for real ones (iterative.py:597, 652, 683-690)."""
+    """2 sum Re(a conj(b)) for complex blocks, sum a b for real ones (iterative.py:597, 652, 683-690);
+    a history entry 0 of a variable not refined in its epoch contributes 0."""
+    if not isinstance(a, np.ndarray) or not isinstance(b, np.ndarray):
+        return 0.0
     if np.iscomplexobj(a):
         return 2.0 * float(np.sum(np.real(a * np.conj(b)), dtype=np.float64))
     return float(np.sum(a.astype(np.float64) * b.astype(np.float64)))
@@ -71,7 +74,8 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
             pg = clean(pg)
         if update_obj:
             og = clean(og)
-        return loss, sse, [og, pg, posg, out[5] if update_bg else None]
+        # objective.py:376-377: zeros of the background's shape when it is not refined
+        return loss, sse, [og, pg, posg, out[5] if update_bg else (None if b is None else np.zeros_like(b))]
 
     def ddot(u, g):
         return sum(_d(a, b) for a, b, f in zip(u, g, on) if f)
@@ -92,6 +96,8 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
     else:
         loss, sse = hist["prev_loss"], hist["prev_sse"]
         g = [hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist.get("static_background_grad")]
+        if update_bg and g[3] is None:
+            g[3] = np.zeros_like(bg0)
         if update_probe:
             g[1] = clean(g[1])
         q = [g[j].copy() if on[j] else None for j in range(4)]

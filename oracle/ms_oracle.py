@@ -340,6 +340,11 @@ def full_l1_constraint(obj, abs_norm_weight, phase_norm_weight):
     return float(reg), (0.5 * grad / np.conj(obj)).astype(obj.dtype)
 
 
+def complex_grad_to_phase_grad(grad, array):
+    """pypty/utils.py:260-276: dL/dphase = 2 Re(-i conj(z) dL/dz*) for z = |z| exp(i phase)."""
+    return 2 * np.real(-1j * np.conj(array) * grad)
+
+
 def preprocess_dataset(dataset, data_shift_vector=(0, 0), data_bin=1, data_pad=0, data_multiplier=1.0,
                        upsample_pattern=1):
     """pypty/utils.py:626-651 as objective.py:400-405 calls it per batch (not compressed, force_pad):

@@ -153,7 +153,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             pg = _clean(pg, cut, smooth_rolloff)
         if clean and update_obj:
             og = _clean(og, cut, smooth_rolloff)
-        return loss, sse, og, pg, posg, out[9], (out[6] if update_bg else None)
+        # an unrefined background has a zero gradient, as objective.py:376-377 returns it, so a history
+        # entry stored in such an epoch serves an epoch that starts refining it without a reset
+        bgg = out[6] if update_bg else (torch.zeros_like(b) if isinstance(b, torch.Tensor) else 0)
+        return loss, sse, og, pg, posg, out[9], bgg
 
     if update_probe:
         probe = _clean(probe, cut, smooth_rolloff)
@@ -186,6 +189,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         total_loss, this_sse = history_bfgs["prev_loss"], history_bfgs["prev_sse"]
         g = [history_bfgs["obj_grad"], history_bfgs["probe_grad"], history_bfgs["pos_grad"],
              history_bfgs.get("static_background_grad")]
+        if update_bg and not isinstance(g[3], torch.Tensor):
+            g[3] = torch.zeros_like(static_background)  # a background held fixed until now
         cc = history_bfgs["constraint_contributions"]
         if update_probe:
             g[1] = _clean(g[1], cut, smooth_rolloff)
@@ -514,8 +519,8 @@ def run(pypty_params):
     reset_bfgs_history()
     t0 = time.time()
     counter_failed = 0
-    fail_pos, fail_probe = np.inf, np.inf
-    prev_steps = 0
+    fail_pos, fail_bg, fail_probe = np.inf, np.inf, np.inf
+    prev_steps = prev_constraints = 0
     losses = []
     is_first = True
     step_param = get("update_step_bfgs", 1)  # first line-search step, a value or a callable of the epoch
@@ -527,33 +532,43 @@ def run(pypty_params):
         s_pos = _per_epoch(get("update_probe_pos", 0), epoch)
         s_bg = _per_epoch(get("update_static_background", 0), epoch) if static_background is not None else 0
         first_step = _per_epoch(step_param, epoch)  # per epoch, as pypty/iterative.py:325-336 reads it
-        # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, probe after 6
+        # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, the static
+        # background after 4, the probe after 6 (tilts and aberrations are not refined here)
         if counter_failed >= 2:
             fail_pos = epoch
             warnings += "\nWARNING: Scan grid refinement was manually disabled for the next 10 epochs!"
+            if counter_failed >= 4:
+                fail_bg = epoch
+                warnings += "\nWARNING: Static background refinement was manually disabled for the next 10 epochs!"
             if counter_failed >= 6:
                 fail_probe = epoch
                 warnings += "\nWARNING: Probe refinement was manually disabled for the next 10 epochs!"
         if abs(fail_pos - epoch) < 10:
             s_pos = 0
+        if abs(fail_bg - epoch) < 10:
+            s_bg = 0
         if abs(fail_probe - epoch) < 10:
             s_probe = 0
-        steps = sum(int(x != 0) for x in (s_obj, s_probe, s_pos, s_bg))
-        if epoch != int(get("epoch_prev", 0)) and steps != prev_steps:
-            reset_bfgs_history()
-            warnings += "\nResetting the history!"
-        prev_steps = steps
-        seq = np.sort(np.asarray(_per_epoch(seq_param, epoch))) if seq_param is not None else seq0
         lw = _per_epoch(get("loss_weight", 1), epoch)
         if lw == "mean":
             lw = 1 / dataset.shape[0]
+        w_phase = _per_epoch(get("phase_norm_weight", 0), epoch)
+        w_abs = _per_epoch(get("abs_norm_weight", 0), epoch)
+        # the history is reset when the objective changes: a different set of refined variables or of
+        # non-zero constraint weights (pypty/iterative.py:383-389; of its weights run() takes these)
+        steps = sum(int(x != 0) for x in (s_obj, s_probe, s_pos, s_bg))
+        constraints = sum(int(x != 0) for x in (lw, w_phase, w_abs))
+        if epoch != int(get("epoch_prev", 0)) and (steps != prev_steps or constraints != prev_constraints):
+            reset_bfgs_history()
+            warnings += "\nResetting the history!"
+        prev_steps, prev_constraints = steps, constraints
+        seq = np.sort(np.asarray(_per_epoch(seq_param, epoch))) if seq_param is not None else seq0
         out = bfgs_update(
             algorithm_type=get("algorithm", "lsq_sqrt"), this_slice_distances=slice_distances,
             edge_slice_thickness_multiplier=1.0, this_step_probe=s_probe, this_step_obj=s_obj,
             this_step_pos_correction=s_pos, this_step_tilts=0, measured_array=measured, this_wavelength=wavelength,
             masks=None, pixel_size_x_A=px, pixel_size_y_A=py,
-            phase_norm_weight=_per_epoch(get("phase_norm_weight", 0), epoch),
-            abs_norm_weight=_per_epoch(get("abs_norm_weight", 0), epoch),
+            phase_norm_weight=w_phase, abs_norm_weight=w_abs,
             stepsize_threshold_low=get("min_step", 1e-20), probe_reg_weight=0, aperture_mask=None,
             recon_type="far_field", defocus_array=np.array([0.0]), Cs=0.0, alpha_near_field=0.0,
             damping_cutoff_multislice=cutoff, smooth_rolloff=rolloff,
@@ -595,7 +610,8 @@ def run(pypty_params):
             _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,
                                "initial step": first_step, "matching step": actual_step,
                                "N linesearch iterations": count + 1, "dir. derivative": d_value,
-                               "new dir. derivative": new_d_value, "Constraints contribution": 0.0,
+                               "new dir. derivative": new_d_value,
+                               "Constraints contribution": float(np.sum(cc)),
                                "Free GiB": free / 1024 ** 3, "Total GiB": total / 1024 ** 3, "Warnings": warnings})
         if epoch_callback is not None:
             epoch_callback(epoch, obj, probe)

@@ -132,10 +132,13 @@ def scatteradd(full, masky, maskx, chop):
     (corner + arange); chop (B, n, n, S, K); full (Ny, Nx, S, K) complex64.
     """
     B, n = chop.shape[0], chop.shape[1]
+    masky = torch.as_tensor(masky, device=chop.device).broadcast_to((B, n, n))
+    maskx = torch.as_tensor(maskx, device=chop.device).broadcast_to((B, n, n))
     corners = torch.stack((masky[:, 0, 0], maskx[:, 0, 0]), dim=1).to(torch.int32).contiguous()
     ar = torch.arange(n, device=chop.device)
-    if not (torch.equal(masky, corners[:, 0, None, None] + ar[None, :, None]) and
-            torch.equal(maskx, corners[:, 1, None, None] + ar[None, None, :])):
+    ey = (corners[:, 0, None, None].long() + ar[None, :, None]).expand(B, n, n)
+    ex = (corners[:, 1, None, None].long() + ar[None, None, :]).expand(B, n, n)
+    if not (torch.equal(masky.long(), ey) and torch.equal(maskx.long(), ex)):
         raise NotImplementedError("pypty_amd.scatteradd: index grids must be contiguous windows")
     windows = chop.reshape(B, n, n, -1).permute(0, 3, 1, 2).to(torch.complex64).contiguous()
     _scatter_windows(full, windows, corners)

@@ -191,9 +191,10 @@ def config0(device=None, seed=1000):
     return Dataset(obj, probe, ints, corr, meas, px, dz, lam, 1e4)
 
 
-def ragged_datasets(device=None, seed=2000, n_datasets=8):
+def ragged_datasets(device=None, seed=2000, n_datasets=8, max_scan=None):
     """Eight independent seeded datasets of mixed shapes: windows {96, 128, 192} px, 4-16 slices,
-    1-5 probe modes, raster scans 32x32 .. 96x96 (step 6 px), dose log-uniform in 1e3 .. 1e5 e-."""
+    1-5 probe modes, raster scans 32x32 .. 96x96 (step 6 px), dose log-uniform in 1e3 .. 1e5 e-.
+    ``max_scan`` caps the scan side (same shapes, fewer positions, smaller objects)."""
     out = []
     px, dz = 0.2, 2.0
     lam = electron_wavelength(300.0)
@@ -203,6 +204,8 @@ def ragged_datasets(device=None, seed=2000, n_datasets=8):
         S = int(rng.integers(4, 17))
         M = int(rng.integers(1, 6))
         scan = int(rng.integers(32, 97))
+        if max_scan is not None:
+            scan = min(scan, int(max_scan))
         dose = float(10 ** rng.uniform(3, 5))
         step = 6.0
         side = int(np.ceil(((scan - 1) * step + n + 4) / 32.0) * 32)

@@ -188,3 +188,54 @@ def test_epochs_with_l1_constraints_match_oracle():
         assert rg[4] == ro[3]
         assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
         assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
+
+
+def test_probe_swapped_for_background_without_a_history_reset_matches_oracle():
+    """Epoch 0 refines object and probe with the background held fixed; epoch 1 refines object and
+    background (the number of refined variables is unchanged, so run() would not reset the history).
+    The stored background gradient of epoch 0 is zero (objective.py:376-377), and epoch 1 follows
+    the oracle."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] probe swapped for the static background without a reset", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=True, seed=37, step_pos=False, noise=0.02)
+    bg = (0.3 * np.sqrt(pr["meas"].mean()) * np.random.default_rng(8).uniform(0.5, 1.5, (32, 32))).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy(),
+             "bg": bg.copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.static_background = torch.from_numpy(bg.copy()).to(dev)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    try:
+        for epoch, (probe_on, bg_on) in enumerate(((True, False), (False, True))):
+            np.random.seed(500 + epoch)
+            ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_probe=probe_on, update_pos=False, update_bg=bg_on,
+                                        actual_step=step_o)
+            np.random.seed(500 + epoch)
+            rg = call_loss_and_direction(iterative.bfgs_update,
+                                         **bfgs_kwargs(pr, actual_step=step_g, is_first_epoch=(epoch == 0),
+                                                       this_step_probe=int(probe_on),
+                                                       this_step_static_background=int(bg_on)))
+            step_o, step_g = ro[2], rg[3]
+            print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; count {rg[4]}/{ro[3]}", flush=True)
+            assert ro[6], "the oracle's epoch was accepted"
+            assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+            assert rg[4] == ro[3]
+            assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+            assert rel(iterative.static_background.cpu().numpy(), state["bg"]) < 1e-4
+            if epoch == 0:
+                g = iterative.history_bfgs["static_background_grad"]
+                assert isinstance(g, torch.Tensor) and float(g.abs().max()) == 0.0
+    finally:
+        iterative.static_background = None

@@ -96,3 +96,51 @@ def test_run_refines_a_static_background_and_writes_it(tmp_path):
         assert np.abs(cs - bg0).max() > 0 and np.isfinite(cs).all()
     finally:
         iterative.static_background = None
+
+
+def test_run_resets_the_history_when_an_l1_weight_switches_on(tmp_path):
+    """A phase_norm_weight given as a callable that switches on at epoch 3 changes the objective: run()
+    resets the L-BFGS history there (pypty/iterative.py:383-389), and loss.csv's constraints column
+    holds the sum of the epoch's constraint contributions (pypty/utils.py:1926), zero before."""
+    import torch
+
+    from pypty_amd import iterative, synth
+
+    print("[run] L1 phase weight switching on at epoch 3", flush=True)
+    dev = torch.device("cuda", 0)
+    px = 0.2
+    true_obj = synth.hex_lattice_object(128, 128, px, 2.46, 0.3, 0.5, seed=15)
+    probe = synth.make_probe(64, px, 300.0, 20.0, 100.0, 1, 0.0, seed=16)
+    pos = synth.raster_scan(8, 8, 6.0, origin_px=(8.0, 8.0), jitter_px=0.3, seed=17)
+    ints, corr = synth.split_positions(pos)
+    meas = synth.simulate(true_obj, probe, ints, corr, px, 2.0, synth.electron_wavelength(300.0), 1e5, seed=18,
+                          device=dev)
+    out = str(tmp_path) + os.sep
+    params = dict(dataset=meas, probe=probe, positions=pos, obj=true_obj.copy(), acc_voltage=300.0,
+                  pixel_size_x_A=px, pixel_size_y_A=px, slice_distances=np.array([2.0]), num_slices=1,
+                  algorithm="lsq_sqrt", epoch_max=6, update_obj=1, update_probe=0, update_probe_pos=0,
+                  compute_batch=32, output_folder=out, save_inter_checkpoints=0, print_flag=0,
+                  phase_norm_weight=lambda epoch: 0.0 if epoch < 3 else 50.0)
+    resets = []
+    real_reset = iterative.reset_bfgs_history
+    epoch_now = [None]
+
+    def cb(ep, obj, pr):
+        epoch_now[0] = ep + 1
+
+    def counting_reset():
+        resets.append(epoch_now[0])
+        real_reset()
+
+    iterative.reset_bfgs_history = counting_reset
+    try:
+        iterative.run(dict(params, epoch_callback=cb))
+    finally:
+        iterative.reset_bfgs_history = real_reset
+    rows = list(csv.reader(open(out + "loss.csv")))
+    assert len(rows) == 6
+    cons = [float(r[9]) for r in rows]
+    print(f"  constraints column {cons}; history resets before epochs {resets}", flush=True)
+    assert all(c == 0.0 for c in cons[:3]) and all(c > 0.0 for c in cons[3:])
+    assert "Resetting the history" in rows[3][12]
+    assert 3 in resets
