@@ -1,31 +1,33 @@
 """Throughput of the multislice forward + loss + adjoint pass on one MI355X.
 
 One step = one full loss-and-gradient pass (object, probe and position
-gradients, probe sub-pixel shift active) over every scan position.  The
-default workload is the multislice configuration: object 512x512x10 (dz 2 A,
-0.2 A pixels), 128 px probe with 5 incoherent modes, 64x64 raster = 4096
-positions, 256 windows per launch, Poisson data at 1e4 e-.  --config ragged
-runs eight mixed-shape datasets (windows 96-192 px) one after the other per
-step; --config thick the 1536x1536x40 object with a 256 px, 10-mode probe.
-The loss/sse readback of every pass is inside the timed region.  Each pass runs
-with four batch lanes (--lanes): batch i runs on stream i % 4, so one batch's
-memory-bound kernels overlap another's compute.
-The per-kernel breakdown and the roofline come from a second run of the same
-steps with one lane, where every launch has the GPU to itself.
+gradients, probe sub-pixel shift active) over every scan position of every
+dataset of the workload, followed by the readback of every dataset's loss and
+sse.  patterns/s = positions / median per-step wall time (after --warmup
+untimed steps).  The default workload is the ragged one: eight seeded datasets
+(windows 96/128/192 px, 4-16 slices, 1-5 probe modes, 39,320 positions), each
+with autotuned windows per launch, spread over two streams and replayed as one
+captured HIP graph per step.  --also adds further workloads to the same line
+(default: the multislice configuration, object 512x512x10, 128 px probe with
+5 modes, 4096 positions, four batch lanes); --config thick is the
+1536x1536x40 object with a 256 px, 10-mode probe over 16384 positions.
 
-Prints one JSON line: patterns/s (= positions x steps / seconds), the
-roofline of the dominant kernel (FP32 flops per launch / HIP-event average
-duration), the CPU baseline (the oracle's complex64 numpy path on a bounded
-sample of positions, timed on this host, with scipy.fft on one worker as PyPty
-calls it and on all host cores), and "recon": the wall-clock of a plain
-L-BFGS reconstruction of the single-slice workload (256x256 hexagonal-lattice
-phase object, 64 px probe, 16x16 scan, 20 epochs, object updates) through the
-run() driver on the GPU (after an untimed 2-epoch warm-up run) against the
-oracle's L-BFGS on one CPU core, with the NMSE each reaches and the CPU time to
-reach the GPU's final NMSE.  With the multislice workload, "warm_start" adds the
-metric's second half: the learned warm start (a CNN trained on the GPU on
-patterns simulated by the forward kernels) against the plain flat start, as
-the wall-clock each needs to reach the plain reconstruction's final NMSE.
+The per-kernel breakdown and the roofline come from a second run of a few
+steps with one lane and the datasets one after the other, with HIP events
+around every launch on its stream.  The roofline is the FP32 vector roof of
+the dominant kernel kind: algorithmic flops of its launches / their summed
+duration.
+
+Prints one JSON line: patterns/s, the roofline, the CPU baseline (the
+oracle's complex64 numpy path on the first --cpu-sample positions of every
+dataset, timed on this host with scipy.fft on one worker, as PyPty calls it,
+and on all host cores; the faster is the baseline), "recon" (wall-clock of a
+plain L-BFGS reconstruction of the single-slice workload through the run()
+driver on the GPU against the oracle's L-BFGS on one CPU core, with the NMSE
+each reaches) and "warm_start" (the learned warm start, a CNN trained on the
+GPU on patterns simulated by the forward kernels, against the plain flat
+start, as the wall-clock each needs to reach the plain reconstruction's final
+NMSE).
 """
 from __future__ import annotations
 
@@ -314,62 +316,149 @@ def recon_benchmark(dev, epochs, cpu_epochs):
 
 
 WORKLOADS = {
+    "ragged": "8 seeded datasets (seeds 2000-2007), windows {96,128,192} px, 4-16 slices, 1-5 probe modes, "
+              "scans 32x32-96x96 (39,320 positions), dose 1e3-1e5, one full pass over every dataset per step, "
+              "autotuned windows per launch, HIP-graph replay over two streams",
     "multislice": "multislice object 512x512x10 dz2A, probe 128px x5 modes, 64x64 scan (4096 positions), "
                   "batch 256, lsq_sqrt, object+probe+position gradients",
-    "ragged": "8 seeded datasets, windows {96,128,192} px, 4-16 slices, 1-5 probe modes, scans 32x32-96x96, "
-              "dose 1e3-1e5, one pass over every dataset per step",
-    "thick": "thick object 1536x1536x40 dz1A, probe 256px x10 modes, 64x64 scan (4096 positions), "
+    "thick": "thick object 1536x1536x40 dz1A, probe 256px x10 modes, 128x128 scan (16384 positions), batch 512, "
              "object+probe+position gradients, slice waves checkpointed every 8 slices and recomputed",
 }
+# name of each workload in profiles/ file names
+PROFILE_TAG = {"ragged": "ragged_8datasets_n96-192", "multislice": "multislice_512x512x10_n128_m5",
+               "thick": "thick_1536x1536x40_n256_m10"}
 
 
-class Job:
-    """One dataset resident on the device with its engine and gradient buffers."""
+def build_workload(name, dev, args):
+    """The datasets and the RaggedStep of a workload, tuned and (for ragged) graph-captured."""
+    from pypty_amd import ragged, synth
 
-    def __init__(self, ds, batch, dev, checkpoint=0, lanes=4):
-        self.ds = ds
-        self.checkpoint = checkpoint
-        self.lanes = lanes
-        self._setup(ds, batch, dev)
+    t0 = time.perf_counter()
+    if name == "ragged":
+        datasets = synth.ragged_datasets(dev)
+        kw = dict(batch=256, streams=2, lanes=1, checkpoint=0)
+        tune, graph = True, True
+    elif name == "multislice":
+        datasets = [synth.config1(dev, n_scan=args.scan)]
+        kw = dict(batch=256, streams=1, lanes=4, checkpoint=0)
+        tune, graph = False, False
+    else:
+        datasets = [synth.config4(dev, n_scan=args.thick_scan)]
+        kw = dict(batch=512, streams=1, lanes=4, checkpoint=8)
+        tune, graph = False, False
+    t_gen = time.perf_counter() - t0
+    step = ragged.RaggedStep(datasets, dev, **kw)
+    if tune:
+        step.tune((64, 128, 256, 512))
+    info = {"lanes": kw["lanes"], "streams": step.n_streams, "checkpoint_slices": kw["checkpoint"],
+            "hip_graph": graph, "data_gen_s": round(t_gen, 1)}
+    return datasets, step, graph, info
 
-    def set_batch(self, batch):
-        self._setup(self.ds, batch, self.dev)
 
-    def set_lanes(self, lanes):
-        self.lanes = lanes
-        self._setup(self.ds, self.cfg.compute_batch, self.dev)
+def time_steps(step, graph, steps, warmup):
+    """Per-step wall times: each step is one full pass over every dataset (the graph replay, or the
+    eager launches) followed by the readback of every dataset's loss and sse, which synchronises."""
+    import torch
 
-    def _setup(self, ds, batch, dev):
-        import numpy as np
-        import torch
+    for _ in range(warmup):
+        step.run()
+        step.losses().tolist()
+    if graph:
+        step.capture()
+        step.replay()
+        step.losses().tolist()
+    torch.cuda.synchronize()
+    times, losses = [], None
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        if graph:
+            step.replay()
+        else:
+            step.run()
+        losses = step.losses().tolist()
+        times.append(time.perf_counter() - t0)
+    return times, losses
 
-        from pypty_amd.objective import PassConfig, PtychoPass
 
-        ny, nx, S, K = ds.obj.shape
-        n, _, M = ds.probe.shape
-        self.P = ds.positions.shape[0]
-        self.shape = (n, S, M, K)
-        # reconstruction starts from a flat object; data come from the true one
-        self.obj = torch.ones((ny, nx, S, K), dtype=torch.complex64, device=dev)
-        self.probe = torch.from_numpy(np.ascontiguousarray(ds.probe.transpose(2, 0, 1))).to(dev)
-        self.pos = torch.from_numpy(ds.positions).to(dev)
-        self.corr = torch.from_numpy(ds.corrections).to(dev)
-        self.meas = torch.from_numpy(ds.measured).to(dev)
-        self.seq = torch.arange(self.P, dtype=torch.int32, device=dev)
-        self.cfg = PassConfig(n=n, n_slices=S, n_obj_modes=K, n_probe_modes=M, obj_ny=ny, obj_nx=nx,
-                              n_positions=self.P, compute_batch=batch, algorithm="lsq_sqrt", planar=False,
-                              step_obj=True, step_probe=True, step_pos=True, shift_active=True,
-                              slice_distance=ds.dz_A, wavelength=ds.wavelength_A, pixel_size_x=ds.pixel_A,
-                              pixel_size_y=ds.pixel_A, checkpoint_slices=self.checkpoint, lanes=self.lanes)
-        self.dev = dev
-        self.engine = PtychoPass(self.cfg, dev)
-        self.og = torch.empty_like(self.obj)
-        self.pg = torch.empty_like(self.probe)
-        self.posg = torch.zeros_like(self.corr)
+def kernel_breakdown(step, steps):
+    """The same steps again, eagerly, datasets one after the other and one batch lane, with HIP events
+    around every launch on the stream it is issued on: total ms and launches per kernel kind."""
+    import torch
 
-    def step(self, sync=True):
-        return self.engine.run(self.obj, self.probe, self.pos, self.corr, self.meas, self.seq, obj_grad=self.og,
-                               probe_grad=self.pg, pos_grad=self.posg, sync=sync)
+    from pypty_amd import _lib
+
+    lanes = [j.lanes for j in step.jobs]
+    for j in step.jobs:
+        if j.lanes != 1:
+            j.set_lanes(1)
+    step.run(concurrent=False)  # the one-lane engines take their workspaces
+    torch.cuda.synchronize()
+    _lib.timing_reset()
+    _lib.timing_enable(True)
+    for _ in range(steps):
+        step.run(concurrent=False)
+    torch.cuda.synchronize()
+    kernels = {k: v for k, v in _lib.timing_read().items() if v[1] > 0}
+    _lib.timing_enable(False)
+    for j, ln in zip(step.jobs, lanes):
+        if ln != 1:
+            j.set_lanes(ln)
+    return kernels
+
+
+def roofline_of(kernels, step, steps, tag):
+    """FP32 roofline of the dominant kernel kind: algorithmic flops of all its launches in a step
+    (kernel_flops per window x windows) over the HIP-event time of those launches."""
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    tot_ms, launches = kernels[dom]
+    avg_s = tot_ms / launches / 1e3
+    flops_step = sum(kernel_flops(dom, *j.shape) * j.P for j in step.jobs)
+    per_launch = flops_step * steps / launches
+    achieved = per_launch / avg_s / 1e12
+    out = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
+           "avg_launch_ms": round(avg_s * 1e3, 4), "launches_per_step": launches // steps,
+           "note": "FP32 vector ALU roof (the FFT-dominated path uses VALU and LDS, not MFMA); "
+                   "peak = MI355X FP32 157.3 TFLOP/s"}
+    # HBM bytes per launch of this kernel at this workload, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE (scaled by a plain-copy calibration) and committed under profiles/
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{tag}.txt")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            for line in fh:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    out["traffic"] = float(line.split(":", 1)[1])
+    return out
+
+
+def measure(name, dev, args, kernel_timing=True):
+    """Throughput of one workload: patterns/s = positions / median step time."""
+    import statistics
+
+    datasets, step, graph, info = build_workload(name, dev, args)
+    times, losses = time_steps(step, graph, args.steps, args.warmup)
+    med = statistics.median(times)
+    P = step.positions
+    res = {"workload": WORKLOADS[name], "positions": P, "value": round(P / med, 2),
+           "ms_per_step": round(med * 1e3, 3), "mean_ms_per_step": round(statistics.fmean(times) * 1e3, 3),
+           "min_ms_per_step": round(min(times) * 1e3, 3),
+           "compute_batch": step.tuned if step.tuned else [j.cfg.compute_batch for j in step.jobs], **info,
+           "datasets": [{"n": j.shape[0], "slices": j.shape[1], "probe_modes": j.shape[2], "obj_modes": j.shape[3],
+                         "positions": j.P} for j in step.jobs],
+           "loss_sse_last_dataset": losses[-1]}
+    flops_step = sum(pass_flops(*j.shape, True) * j.P for j in step.jobs)
+    bytes_step = sum(pass_bytes(j.shape[0], j.shape[1], j.shape[3]) * j.P for j in step.jobs)
+    res["alg_gflop_per_pattern"] = round(flops_step / P / 1e9, 4)
+    res["alg_tflops"] = round(flops_step / med / 1e12, 3)
+    res["pass_frac_fp32"] = round(flops_step / med / 1e12 / FP32_PEAK_TFLOPS, 4)
+    res["hbm_frac_alg"] = round(bytes_step / med / (HBM_PEAK_GBS * 1e9), 4)
+    kernels = {}
+    if kernel_timing:
+        ksteps = max(1, min(args.steps, args.kernel_steps))
+        kernels = kernel_breakdown(step, ksteps)
+        res["roofline"] = roofline_of(kernels, step, ksteps, PROFILE_TAG[name])
+        res["kernels_ms_per_step"] = {k: round(v[0] / ksteps, 3) for k, v in kernels.items()}
+    return datasets, step, res
 
 
 def main(argv=None):
@@ -377,31 +466,23 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--config", choices=sorted(WORKLOADS), default="multislice")
-    ap.add_argument("--scan", type=int, default=64, help="scan grid side of the single-dataset workloads")
-    ap.add_argument("--cpu-sample", type=int, default=160,
-                    help="positions timed on the CPU (per dataset for the ragged workload; 0: skip)")
+    ap.add_argument("--config", choices=sorted(WORKLOADS), default="ragged")
+    ap.add_argument("--also", default="multislice",
+                    help="comma-separated further workloads measured beside the headline ('' for none)")
+    ap.add_argument("--scan", type=int, default=64, help="scan grid side of the multislice workload")
+    ap.add_argument("--thick-scan", type=int, default=128, help="scan grid side of the thick workload")
+    ap.add_argument("--cpu-sample", type=int, default=32,
+                    help="positions per dataset timed on the CPU (0: skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--kernel-steps", type=int, default=3, help="steps of the per-kernel HIP-event pass")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for rocprofv3 counter scaling)")
     ap.add_argument("--recon-epochs", type=int, default=20,
                     help="epochs of the reconstruction wall-clock measurement (0: skip)")
-    ap.add_argument("--autotune", type=int, default=None,
-                    help="choose each dataset's windows per launch by timing (default: on for ragged)")
-    ap.add_argument("--graph", type=int, default=None,
-                    help="capture one step in a HIP graph and replay it (default: on for ragged)")
     ap.add_argument("--warm-start-pairs", type=int, default=50000,
-                    help="training pairs of the learned warm-start measurement (multislice workload; 0: skip)")
+                    help="training pairs of the learned warm-start measurement (0: skip)")
     ap.add_argument("--warm-start-epochs", type=int, default=40)
     ap.add_argument("--warm-recon-epochs", type=int, default=20)
-    ap.add_argument("--streams", type=int, default=None,
-                    help="streams the datasets of a step are spread over (default: 2 for ragged, else 1)")
-    ap.add_argument("--lanes", type=int, default=None, choices=(1, 2, 3, 4),
-                    help="batch lanes of each pass (batch i runs on stream i %% lanes); "
-                         "default 4, and 1 for --config ragged")
-    ap.add_argument("--checkpoint", type=int, default=None,
-                    help="keep every k-th slice's wave and recompute the rest (default: 8 for thick, else 0)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the pass is measured on one GPU only (the multi-GPU split is a design, not built); "
@@ -410,131 +491,17 @@ def main(argv=None):
 
     import torch
 
-    from pypty_amd import _lib, synth
-
     dev = torch.device("cuda", 0)
-    t_gen = time.perf_counter()
-    if args.config == "multislice":
-        datasets = [synth.config1(dev, n_scan=args.scan)]
-    elif args.config == "ragged":
-        datasets = synth.ragged_datasets(dev)
-    else:
-        datasets = [synth.config4(dev, n_scan=args.scan)]
-    t_gen = time.perf_counter() - t_gen
-    if args.lanes is None:
-        args.lanes = 1 if args.config == "ragged" else 4
-    if args.config == "ragged" and args.lanes > 1:
-        # two lanes under this workload (eight datasets over two streams, autotuned, captured in one
-        # graph) ended once in a host segmentation fault whose cause is not known; it runs one lane
-        sys.exit("bench.py: --config ragged runs with --lanes 1")
-    ck = (8 if args.config == "thick" else 0) if args.checkpoint is None else args.checkpoint
-    jobs = [Job(ds, args.batch, dev, ck, args.lanes) for ds in datasets]
-    P_total = sum(j.P for j in jobs)
-    ragged = args.config == "ragged"
-    use_tune = ragged if args.autotune is None else bool(args.autotune)
-    use_graph = ragged if args.graph is None else bool(args.graph)
-    tuned = None
-    if use_tune:
-        from pypty_amd import autotune
-
-        tuned = []
-        for j in jobs:
-            n, S, M, K = j.shape
-            key = autotune.shape_key(dev, n, S, K, M, j.P)
-
-            def make_pass(b, j=j):
-                j.set_batch(b)
-                return lambda: j.step(sync=False)
-
-            best = autotune.choose_batch(key, (64, 128, 256, 512), make_pass)
-            j.set_batch(best)
-            tuned.append(best)
-        from pypty_amd.objective import PtychoPass
-
-        PtychoPass.release_workspaces()  # the candidates' workspaces; the chosen engines take theirs next
-
-    # datasets on several default-priority streams (a workspace per stream and shape), joined at the end
-    n_streams = max(1, min(len(jobs), (2 if ragged else 1) if args.streams is None else args.streams))
-    lanes = [torch.cuda.Stream(dev) for _ in range(n_streams)] if n_streams > 1 else []
-
-    def step(concurrent=True):
-        out = None
-        if not lanes or not concurrent:
-            for j in jobs:
-                out = j.step(sync=False)
-            return out
-        main = torch.cuda.current_stream(dev)
-        for ln in lanes:
-            ln.wait_stream(main)
-        for i, j in enumerate(jobs):
-            with torch.cuda.stream(lanes[i % n_streams]):
-                out = j.step(sync=False)
-        for ln in lanes:
-            main.wait_stream(ln)
-        return out
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    graph = None
-    if use_graph:
-        # one whole step (every dataset's pass) captured once; the warm-up above allocated the workspaces
-        side = torch.cuda.Stream(dev)
-        side.wait_stream(torch.cuda.current_stream(dev))
-        with torch.cuda.stream(side):
-            step()
-        torch.cuda.current_stream(dev).wait_stream(side)
-        torch.cuda.synchronize()
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
-            out_t = step()
-        graph.replay()
-        torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        if graph is not None:
-            graph.replay()
-        else:
-            out_t = step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    loss, sse = out_t.tolist()
-    value = P_total * args.steps / elapsed
-
-    kernels = {}
-    roofline = None
-    if not args.no_kernel_timing:
-        # same steps again with HIP events around every launch, for the per-kernel breakdown
-        # (datasets one after the other and one batch lane: events on concurrent streams would time
-        # each other's kernels)
-        if args.lanes > 1:
-            for j in jobs:
-                j.set_lanes(1)
-        _lib.timing_reset()
-        _lib.timing_enable(True)
-        for _ in range(args.steps):
-            step(concurrent=False)
-        torch.cuda.synchronize()
-        kernels = {k: v for k, v in _lib.timing_read().items() if v[1] > 0}
-        _lib.timing_enable(False)
-        dom = max(kernels, key=lambda k: kernels[k][0])
-        tot_ms, launches = kernels[dom]
-        avg_s = tot_ms / launches / 1e3
-        # algorithmic flops of all launches of the dominant kernel in one step, per launch
-        flops_step = sum(kernel_flops(dom, *j.shape) * j.P for j in jobs)
-        per_launch = flops_step * args.steps / launches
-        achieved = per_launch / avg_s / 1e12
-        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
-                    "avg_launch_ms": round(avg_s * 1e3, 4), "launches_per_step": launches // args.steps}
-        # HBM bytes per launch of this kernel at this workload, measured with rocprofv3
-        # FETCH_SIZE / WRITE_SIZE (scaled by a plain-copy calibration) and committed under profiles/
-        tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_multislice_512x512x10_n128_m5.txt")
-        if args.config == "multislice" and os.path.exists(tf) and args.batch == 256 and args.scan == 64:
-            with open(tf) as fh:
-                for line in fh:
-                    if line.startswith("hbm_bytes_per_launch:"):
-                        roofline["traffic"] = float(line.split(":", 1)[1])
+    datasets, step, head = measure(args.config, dev, args, kernel_timing=not args.no_kernel_timing)
+    print(f"[bench] {args.config}: {head['value']:.1f} patterns/s, median {head['ms_per_step']:.2f} ms per step",
+          file=sys.stderr, flush=True)
+    extra = {}
+    for name in [x for x in args.also.split(",") if x and x != args.config]:
+        _, st, r = measure(name, dev, args, kernel_timing=not args.no_kernel_timing)
+        extra[name] = r
+        del st
+        print(f"[bench] {name}: {r['value']:.1f} patterns/s", file=sys.stderr, flush=True)
+    del step
 
     if args.copy_calibration_mib:
         a = torch.empty(args.copy_calibration_mib << 20, dtype=torch.uint8, device=dev)
@@ -544,43 +511,39 @@ def main(argv=None):
 
     cpu = None
     if args.cpu_sample > 0:
-        n_cpu = args.cpu_sample if args.config != "ragged" else max(1, args.cpu_sample // 10)
-        if args.config == "thick":
-            n_cpu = max(1, args.cpu_sample // 20)
-        v1, dt1, npos = cpu_baseline(datasets, n_cpu, 1)
-        cpu = {"value": round(v1, 3), "unit": "patterns/s", "cores": 1, "kind": "port",
-               "sample": f"oracle complex64 numpy/scipy.fft path (PyPty op order), first {n_cpu} positions of "
-                         f"each of the {len(datasets)} dataset(s) ({npos} patterns), scipy workers=1, {dt1:.1f} s"}
-        # the same sample with scipy.fft on all host cores (numpy's elementwise work stays on one core)
+        v1, dt1, npos = cpu_baseline(datasets, args.cpu_sample, 1)
         ncores = max(1, min(os.cpu_count() or 1, 16))
-        va, dta, _ = cpu_baseline(datasets, n_cpu, ncores)
-        cpu["all_cores"] = {"value": round(va, 3), "cores": ncores, "seconds": round(dta, 1)}
+        va, dta, _ = cpu_baseline(datasets, args.cpu_sample, ncores)
+        faster = max((v1, 1, dt1), (va, ncores, dta))
+        cpu = {"value": round(faster[0], 3), "unit": "patterns/s", "cores": faster[1], "kind": "port",
+               "sample": f"oracle complex64 numpy/scipy.fft path (PyPty op order), first {args.cpu_sample} "
+                         f"positions of each of the {len(datasets)} dataset(s) ({npos} patterns); the faster of "
+                         f"scipy.fft workers=1 ({v1:.2f}/s, {dt1:.1f} s) and workers={ncores} "
+                         f"({va:.2f}/s, {dta:.1f} s); numpy's elementwise work runs on one core",
+               "one_core": {"value": round(v1, 3), "seconds": round(dt1, 1)},
+               "all_cores": {"value": round(va, 3), "cores": ncores, "seconds": round(dta, 1)}}
+        print(f"[bench] cpu baseline {cpu['value']:.2f} patterns/s", file=sys.stderr, flush=True)
 
     recon = recon_benchmark(dev, args.recon_epochs, args.recon_epochs if args.cpu_sample > 0 else 0) \
         if args.recon_epochs > 0 else None
     warm = warm_start_benchmark(dev, args.warm_start_pairs, args.warm_start_epochs, args.warm_recon_epochs) \
-        if args.config == "multislice" and args.warm_start_pairs > 0 else None
+        if args.warm_start_pairs > 0 else None
 
-    flops_step = sum(pass_flops(*j.shape, True) * j.P for j in jobs)
-    bytes_step = sum(pass_bytes(j.shape[0], j.shape[1], j.shape[3]) * j.P for j in jobs)
-    cfg_info = {"workload": WORKLOADS[args.config], "positions": P_total,
-                "compute_batch": tuned if tuned else args.batch, "hip_graph": bool(use_graph),
-                "streams": n_streams, "batch_lanes": args.lanes,
-                "checkpoint_slices": ck,
-                "datasets": [{"n": j.shape[0], "slices": j.shape[1], "probe_modes": j.shape[2],
-                              "obj_modes": j.shape[3], "positions": j.P} for j in jobs]}
+    cfg_info = {"workload": head["workload"], "positions": head["positions"], "compute_batch": head["compute_batch"],
+                "hip_graph": head["hip_graph"], "streams": head["streams"], "batch_lanes": head["lanes"],
+                "checkpoint_slices": head["checkpoint_slices"], "datasets": head["datasets"]}
     out = {
-        "metric": METRIC, "value": round(value, 2), "unit": "patterns/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "metric": METRIC, "value": head["value"], "unit": "patterns/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "complex64", "data": "synthetic",
-        "config": cfg_info, "roofline": roofline, "cpu_baseline": cpu,
-        "pass": {"loss_last_dataset": loss, "sse_last_dataset": sse,
-                 "alg_gflop_per_pattern": round(flops_step / P_total / 1e9, 4),
-                 "alg_tflops": round(flops_step * args.steps / elapsed / 1e12, 3),
-                 "hbm_frac_alg": round(bytes_step * args.steps / elapsed / (HBM_PEAK_GBS * 1e9), 4),
-                 "gpu_over_cpu": round(value / cpu["value"], 1) if cpu else None,
-                 "data_gen_s": round(t_gen, 1)},
-        "kernels_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kernels.items()},
+        "config": cfg_info, "roofline": head.get("roofline"), "cpu_baseline": cpu,
+        "timing": "patterns/s = positions / median of the per-step wall times; a step is one full pass over "
+                  "every dataset and the readback of every dataset's loss (synchronising)",
+        "pass": {k: head[k] for k in ("mean_ms_per_step", "min_ms_per_step", "alg_gflop_per_pattern", "alg_tflops",
+                                      "pass_frac_fp32", "hbm_frac_alg", "loss_sse_last_dataset", "data_gen_s")},
+        "gpu_over_cpu": round(head["value"] / cpu["value"], 1) if cpu else None,
+        "kernels_ms_per_step": head.get("kernels_ms_per_step", {}),
+        "other_workloads": extra,
         "recon": recon,
         "warm_start": warm,
     }

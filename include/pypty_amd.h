@@ -251,6 +251,23 @@ int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float ph
  * (pypty/utils.py:260-276, complex_grad_to_phase_grad); out float32, n elements. */
 int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream);
 
+/* Host-side check of the launch plan of pty_loss_and_direction (no
+ * counterpart in the reference, whose objective.py:393-710 loop is serial):
+ * `calls` passes of problem p over n_seq positions on up to `lanes` batch
+ * lanes, issued back to back on one caller stream, are recorded instead of
+ * launched (no HIP call is made).  Every buffer range each launch reads or
+ * writes is checked to lie in its workspace buffer (per-batch buffers in the
+ * launch's lane block), every pair of launches on different lanes that
+ * touch one range, one of them writing, to be ordered by the pass's events,
+ * and the batches to cover the sequence once.  Returns the number of
+ * violations (0: none) or a negative status; report (report_bytes) receives
+ * one line per violation, the plan itself with verbose.  omit plants faults
+ * in the recorded plan only, to test the check (bit 0: no waits on the batch
+ * ordering events, 1: on the fork, 2: on the joins; bit 3: a stash one window
+ * smaller); 0 checks the plan the library issues. */
+int pty_plan_check(const pty_problem* p, int n_seq, int lanes, int calls, int accumulate, int verbose, int omit,
+                   char* report, size_t report_bytes);
+
 /* Kernel timing (HIP events around every launch of the library). */
 void pty_timing_enable(int on);
 /* Writes up to `cap` entries: names (static strings), total ms, launches;

@@ -25,7 +25,7 @@ EXPORTED = (
     "pty_scatteradd_c64", "pty_workspace_bytes", "pty_loss_and_direction", "pty_tables_bytes",
     "pty_ms_forward", "pty_ms_adjoint", "pty_timing_enable", "pty_timing_read", "pty_timing_reset",
     "pty_fourier_clean_c64", "pty_plane_fft2_c64", "pty_multidot_c64", "pty_axpby_c64", "pty_phase_grad_c64",
-    "pty_l1_constraint_c64",
+    "pty_l1_constraint_c64", "pty_plan_check",
     "pty_gemm_bf16", "pty_cnn_im2col", "pty_cnn_col2im", "pty_cnn_bias_act", "pty_cnn_act_grad",
     "pty_cnn_bias_grad", "pty_cnn_to_bf16", "pty_cnn_mse", "pty_cnn_adam",
 )
@@ -113,6 +113,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
     lib.pty_phase_grad_c64.argtypes = [vp, vp, vp, i64, vp]
     lib.pty_l1_constraint_c64.restype = i
     lib.pty_l1_constraint_c64.argtypes = [vp, i64, ctypes.c_float, ctypes.c_float, vp, vp, vp, vp]
+    lib.pty_plan_check.restype = i
+    lib.pty_plan_check.argtypes = [P(PtyProblem), i, i, i, i, i, i, ctypes.c_char_p, sz]
     lib.pty_timing_enable.restype = None
     lib.pty_timing_enable.argtypes = [i]
     lib.pty_timing_read.restype = i
@@ -177,3 +179,15 @@ def timing_read() -> dict:
     cnt = (ctypes.c_int64 * cap)()
     n = lib.pty_timing_read(names, ms, cnt, cap)
     return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n, cap))}
+
+
+def plan_check(problem: "PtyProblem", n_seq: int, lanes: int, calls: int = 1, accumulate: bool = False,
+               verbose: bool = False, omit: int = 0):
+    """Host-side check of the launch plan of pty_loss_and_direction (no GPU needed):
+    returns (violations, report text).  ``omit`` plants faults in the recorded plan only."""
+    buf = ctypes.create_string_buffer(1 << 22)
+    n = load().pty_plan_check(ctypes.byref(problem), int(n_seq), int(lanes), int(calls), int(bool(accumulate)),
+                              int(bool(verbose)), int(omit), buf, len(buf))
+    if n < 0:
+        check(n, "plan_check")
+    return n, buf.value.decode()

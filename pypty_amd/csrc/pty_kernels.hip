@@ -30,6 +30,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "pty_fft.hpp"
@@ -2094,11 +2095,12 @@ struct TimedLaunch {
 enum {
   KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
   KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER,
-  KT_RECOMPUTE, KT_COUNT
+  KT_RECOMPUTE, KT_PREPROCESS, KT_BG_GRAD, KT_PROBE_REDUCE, KT_COUNT
 };
 const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
                                     "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
-                                    "loss_final", "zero", "scatteradd", "gather_grad", "recompute"};
+                                    "loss_final", "zero", "scatteradd", "gather_grad", "recompute", "preprocess",
+                                    "bg_grad", "probe_reduce"};
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
@@ -2179,18 +2181,25 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
   return check_launch();
 }
 
-// ---------------------------------------------------------- workspace
-// Shared buffers first, then one block of per-batch buffers per lane: lane l's
-// copy of a per-batch buffer lies lane_bytes * l after lane 0's offset (an
-// offset of 0 marks a buffer that is not needed).
+// Workspace buffers.  Shared buffers first, then one block of per-batch
+// buffers per lane: lane l's copy of a per-batch buffer lies lane_bytes * l
+// after lane 0's (a buffer of 0 bytes is not needed by the problem).
+enum {
+  WB_H = 0, WB_H0, WB_MASKC, WB_ZCOL, WB_FPROBE, WB_OBJP, WB_GRADP, WB_PROBE_PART, WB_PROBE_SUM, WB_LOSS_PART,
+  WB_OB, WB_PP, WB_STASH, WB_GSTATE, WB_PSI, WB_COEF, WB_GP, WB_MEAS_STAGE, WB_POS_PART, WB_SCRATCH, WB_COUNT
+};
+constexpr int kFirstLaneBuffer = WB_OB;
+const char* kBufNames[WB_COUNT] = {"H", "H0", "maskc", "zcol", "fprobe", "objp", "gradp", "probe_part",
+                                   "probe_sum", "loss_part", "Ob", "Pp", "stash", "gstate", "Psi", "coef", "gP",
+                                   "meas_stage", "pos_part", "scratch"};
 struct Layout {
-  size_t H, H0, maskc, zcol, fprobe, objp, gradp, probe_part, probe_sum, loss_part;  // shared
-  size_t Ob, Pp, stash, gstate, Psi, coef, gP, pos_part, meas_stage, scratch;        // per lane
-  size_t lane_bytes, total;
+  size_t off[WB_COUNT], bytes[WB_COUNT];  // lane 0's offset for the per-lane buffers
+  size_t lane0, lane_bytes, total;
   int lanes;
   int B, G;
   int ck, C;  // slices per checkpoint segment (0: every slice kept) and segments
   int slots;  // stash slots per window and mode
+  int scratch_tiles;
 };
 inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -2251,42 +2260,45 @@ Layout plan_layout(const pty_problem* p) {
                                           : std::min(B, std::max(64, (1024 + M - 1) / M));
   const size_t objsz = size_t(p->obj_ny) * p->obj_nx * S * K * 8;
   size_t o = 0;
-  auto take = [&](size_t bytes) { size_t r = o; o += al(bytes); return r; };
-  L.H = take(n2 * 8);
-  L.H0 = take(n2 * 8);
-  L.maskc = take(n2 * 4);
-  L.zcol = take(size_t(p->n) * 4);
-  L.fprobe = take(size_t(M) * n2 * 8);
-  L.objp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
-  L.gradp = (p->obj_layout == PTY_LAYOUT_REFERENCE) ? take(objsz) : 0;
-  L.probe_part = take(size_t(G) * M * n2 * 8);
-  L.probe_sum = take(size_t(M) * n2 * 8);
-  L.loss_part = take(size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
-  const size_t lane0 = o;
-  L.Ob = take(size_t(B) * S * K * n2 * 8);
-  L.Pp = take(size_t(B) * M * n2 * 8);
+  auto take = [&](int id, size_t bytes) {
+    L.off[id] = bytes ? o : 0;
+    L.bytes[id] = bytes;
+    o += al(bytes);
+  };
+  take(WB_H, n2 * 8);
+  take(WB_H0, n2 * 8);
+  take(WB_MASKC, n2 * 4);
+  take(WB_ZCOL, size_t(p->n) * 4);
+  take(WB_FPROBE, size_t(M) * n2 * 8);
+  take(WB_OBJP, p->obj_layout == PTY_LAYOUT_REFERENCE ? objsz : 0);
+  take(WB_GRADP, p->obj_layout == PTY_LAYOUT_REFERENCE ? objsz : 0);
+  take(WB_PROBE_PART, size_t(G) * M * n2 * 8);
+  take(WB_PROBE_SUM, size_t(M) * n2 * 8);
+  take(WB_LOSS_PART, size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
+  L.lane0 = o;
+  take(WB_OB, size_t(B) * S * K * n2 * 8);
+  take(WB_PP, size_t(B) * M * n2 * 8);
   // every slice, or the checkpoints of all segments but the last followed by
   // one segment of slices (the last segment's, left by the forward, then each
   // recomputed segment's in turn)
   L.ck = checkpoint_every(p);
   L.C = L.ck ? (S + L.ck - 1) / L.ck : 0;
   L.slots = L.ck ? L.C - 1 + L.ck : S;
-  L.stash = take(size_t(B) * M * K * L.slots * n2 * 8);
-  L.gstate = L.ck ? take(size_t(B) * M * K * n2 * 8) : 0;
-  L.Psi = take(size_t(B) * M * K * n2 * 8);
-  L.coef = take(size_t(B) * n2 * 4);
-  L.gP = take(size_t(B) * M * K * n2 * 8);
-  L.meas_stage = preprocessing(p) ? take(size_t(B) * n2 * 4) : 0;
-  L.pos_part = take(size_t(B) * M * 2 * 8);
-  if (p->n > 128) {
-    // one tile per workgroup of the largest launch
-    const size_t grid = std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K, size_t(G) * M,
-                                          size_t(M)});
-    L.scratch = take(grid * n2 * 8);
-  }
-  L.lane_bytes = o - lane0;
+  take(WB_STASH, size_t(B) * M * K * L.slots * n2 * 8);
+  take(WB_GSTATE, L.ck ? size_t(B) * M * K * n2 * 8 : 0);
+  take(WB_PSI, size_t(B) * M * K * n2 * 8);
+  take(WB_COEF, size_t(B) * n2 * 4);
+  take(WB_GP, size_t(B) * M * K * n2 * 8);
+  take(WB_MEAS_STAGE, preprocessing(p) ? size_t(B) * n2 * 4 : 0);
+  take(WB_POS_PART, size_t(B) * M * 2 * 8);
+  // windows larger than 128 px: one tile per workgroup of the largest launch
+  L.scratch_tiles = p->n > 128 ? int(std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K,
+                                                       size_t(G) * M, size_t(M)}))
+                               : 0;
+  take(WB_SCRATCH, size_t(L.scratch_tiles) * n2 * 8);
+  L.lane_bytes = o - L.lane0;
   L.lanes = std::max(1, p->lanes);
-  L.total = lane0 + size_t(L.lanes) * L.lane_bytes;
+  L.total = L.lane0 + size_t(L.lanes) * L.lane_bytes;
   L.B = B;
   L.G = G;
   return L;
@@ -2309,68 +2321,90 @@ int validate(const pty_problem* p) {
   return PTY_OK;
 }
 
+// the buffers a pass needs, checked before anything is launched
+int validate_buffers(const pty_problem* p, const pty_buffers* b) {
+  if (!b || !b->obj || !b->probe || !b->positions || !b->measured || !b->sequence || !b->loss_sse) return PTY_ERR_ARG;
+  if (b->n_seq < 1 || b->n_seq > p->n_positions) return PTY_ERR_ARG;
+  if ((p->shift_active || p->step_pos) && !b->corrections) return PTY_ERR_ARG;
+  if (p->step_pos && !b->pos_grad) return PTY_ERR_ARG;
+  if (p->step_obj && !b->obj_grad) return PTY_ERR_ARG;
+  if (p->step_probe && !b->probe_grad) return PTY_ERR_ARG;
+  if (compressed(p->algorithm) && !b->masks) return PTY_ERR_ARG;
+  if (b->static_background && compressed(p->algorithm)) return PTY_ERR_ARG;
+  if (p->step_static_background && (!b->static_background || !b->static_background_grad)) return PTY_ERR_ARG;
+  if (b->measured_order != PTY_MEASURED_BY_POSITION && b->measured_order != PTY_MEASURED_BY_SEQUENCE)
+    return PTY_ERR_ARG;
+  return PTY_OK;
+}
 
-// The library streams that carry lanes 1 .. kMaxLanes-1 of the batches issued
-// on a caller stream, one set per (device, caller stream), created on first
-// use at the default priority and kept for the life of the process, with the
-// events that fork, order and join the lanes.  A call holds the state's mutex
-// while it enqueues.  A call on a capturing stream runs one lane.
-struct LaneState {
-  hipStream_t stream[kMaxLanes] = {};  // [0] unused: lane 0 is the caller's stream
-  hipEvent_t fork = nullptr, join[kMaxLanes] = {}, acc[kMaxLanes] = {};
-  std::mutex mu;
+// ------------------------------------------------------------- issuing
+// Events of a pass: the fork from the caller's stream, a join per lane and,
+// per lane, the event recorded after a batch's additions into the buffers
+// every batch shares (the next batch, on the next lane, waits for it).
+enum { EV_FORK = 0, EV_JOIN = 1, EV_ACC = 1 + kMaxLanes, EV_COUNT = 1 + 2 * kMaxLanes };
+enum { OP_RECORD = 1000, OP_WAIT = 1001 };
+// sub-kinds of a fill or layout launch
+enum { SUB_NONE = 0, SUB_TO_PLANAR, SUB_TO_REFERENCE };
+
+// One step of a pass: a launch of kind KT_* on a lane with its arguments, or
+// an event record / wait (OP_RECORD / OP_WAIT of event ev).  A fill records
+// its buffer (p0, n0).
+struct Op {
+  int kind, lane, ev, sub;
+  BatchArgs a;
+  const void* p0;
+  size_t n0;
 };
-std::mutex g_lane_mu;
-std::map<std::pair<int, hipStream_t>, std::unique_ptr<LaneState>> g_lanes;
 
-// the caller's lane state, or null (one lane) when none can be made now
-LaneState* lane_state(hipStream_t caller) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(caller, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
-  std::lock_guard<std::mutex> g(g_lane_mu);
-  auto& slot = g_lanes[{dev, caller}];
-  if (!slot) {
-    auto st = std::make_unique<LaneState>();
-    bool ok = hipEventCreateWithFlags(&st->fork, hipEventDisableTiming) == hipSuccess;
-    for (int l = 1; ok && l < kMaxLanes; ++l)
-      ok = hipStreamCreateWithFlags(&st->stream[l], hipStreamNonBlocking) == hipSuccess;
-    for (int l = 0; ok && l < kMaxLanes; ++l)
-      ok = hipEventCreateWithFlags(&st->join[l], hipEventDisableTiming) == hipSuccess &&
-           hipEventCreateWithFlags(&st->acc[l], hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
-      // release what was made; the call runs one lane
-      for (int l = 1; l < kMaxLanes; ++l)
-        if (st->stream[l]) (void)hipStreamDestroy(st->stream[l]);
-      for (hipEvent_t e : st->join)
-        if (e) (void)hipEventDestroy(e);
-      for (hipEvent_t e : st->acc)
-        if (e) (void)hipEventDestroy(e);
-      if (st->fork) (void)hipEventDestroy(st->fork);
-      return nullptr;
-    }
-    slot = std::move(st);
+// Issues the steps of a pass to the lanes' streams: lane 0 is the caller's
+// stream, lanes 1.. the library streams paired with it.
+struct Enqueue {
+  hipStream_t st[kMaxLanes] = {};
+  hipEvent_t ev[EV_COUNT] = {};
+  template <class F>
+  int launch(int kind, int lane, const BatchArgs&, F&& f, int = SUB_NONE, const void* = nullptr, size_t = 0) {
+    TimeScope ts(kind, st[lane]);
+    f(st[lane]);
+    return check_launch();
   }
-  return slot.get();
-}
+  int record(int lane, int e) { return hipEventRecord(ev[e], st[lane]) == hipSuccess ? PTY_OK : PTY_ERR_HIP; }
+  int wait(int lane, int e) { return hipStreamWaitEvent(st[lane], ev[e], 0) == hipSuccess ? PTY_OK : PTY_ERR_HIP; }
+};
 
-template <int N>
-int launch_tables(const pty_problem* p, float2* H, float2* H0, float* maskc, int* zcol, hipStream_t s) {
-  TimeScope ts(KT_TABLES, s);
-  TableArgs a{H, H0, maskc, zcol, p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y, p->tan_x,
-              p->tan_y, p->damping_cutoff, p->edge_multiplier};
-  k_tables<N, kLayout<N>><<<(N * N + 255) / 256, 256, 0, s>>>(a);
-  return check_launch();
-}
+// Records the steps instead of issuing them: the plan that plan_check() walks
+// on the host (no HIP call is made).
+struct Recorder {
+  std::vector<Op> ops;
+  template <class F>
+  int launch(int kind, int lane, const BatchArgs& a, F&&, int sub = SUB_NONE, const void* p0 = nullptr,
+             size_t n0 = 0) {
+    ops.push_back(Op{kind, lane, -1, sub, a, p0, n0});
+    return PTY_OK;
+  }
+  int record(int lane, int e) {
+    ops.push_back(Op{OP_RECORD, lane, e, SUB_NONE, BatchArgs{}, nullptr, 0});
+    return PTY_OK;
+  }
+  int wait(int lane, int e) {
+    ops.push_back(Op{OP_WAIT, lane, e, SUB_NONE, BatchArgs{}, nullptr, 0});
+    return PTY_OK;
+  }
+};
 
-int zero_fill(void* ptr, size_t bytes, hipStream_t s) {
+#define PTY_TRY(expr)          \
+  do {                         \
+    const int rc_ = (expr);    \
+    if (rc_) return rc_;       \
+  } while (0)
+
+template <class X>
+int issue_zero(X& x, int lane, const BatchArgs& a, void* ptr, size_t bytes) {
   if (bytes == 0) return PTY_OK;
-  TimeScope ts(KT_ZERO, s);
-  const size_t nf = bytes / 4;
-  const int grid = int(std::min<size_t>((nf + 255) / 256, 65536));
-  k_zero<<<grid, 256, 0, s>>>(static_cast<float*>(ptr), nf);
-  return check_launch();
+  return x.launch(KT_ZERO, lane, a, [&](hipStream_t s) {
+    const size_t nf = bytes / 4;
+    const int grid = int(std::min<size_t>((nf + 255) / 256, 65536));
+    k_zero<<<grid, 256, 0, s>>>(static_cast<float*>(ptr), nf);
+  }, SUB_NONE, ptr, bytes);
 }
 
 template <int N>
@@ -2404,52 +2438,46 @@ void launch_objgrad(const BatchArgs& a, hipStream_t s) {
     k_objgrad_gt<N><<<grid, kGtThreads, 0, s>>>(a);
 }
 
-// One batch of windows.  With checkpoint segments (ck > 0, C segments) the
-// forward keeps, per window and mode, the wave entering segments 0..C-2 in
-// stash slots 0..C-2 and every wave of the last segment in the tail slots
-// C-1.. (a.st_dense).  The object gradient then walks the segments from the
-// last: recompute the segment's waves from its checkpoint into the tail
-// (except the last segment, already there), run the adjoint through the
-// segment (the wave crossing segments goes through a.gstate), and band-limit
-// the segment's window gradients.  Every launch computes exactly what the
-// whole-pass launch computes for those slices, in the same order.
-template <int N>
-// One batch on stream s.  The kernels that add into buffers shared by all
-// batches (object gradient, probe partials, position gradients) wait for
-// wait_acc (the previous batch's, on the other lane) and then record done_acc.
-int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s, hipEvent_t wait_acc = nullptr,
-              hipEvent_t done_acc = nullptr) {
+// One batch of windows on one lane.  With checkpoint segments (ck > 0, C
+// segments) the forward keeps, per window and mode, the wave entering
+// segments 0..C-2 in stash slots 0..C-2 and every wave of the last segment in
+// the tail slots C-1.. (a.st_dense).  The object gradient then walks the
+// segments from the last: recompute the segment's waves from its checkpoint
+// into the tail (except the last segment, already there), run the adjoint
+// through the segment (the wave crossing segments goes through a.gstate), and
+// band-limit the segment's window gradients.  Every launch computes exactly
+// what the whole-pass launch computes for those slices, in the same order.
+// The launches that add into buffers shared by all batches (object gradient,
+// probe partials, position and background gradients) wait for event wait_ev
+// (the previous batch's, on its lane; -1: none) and then record done_ev.
+template <int N, class X>
+int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x, int wait_ev, int done_ev) {
   constexpr int LAY = kLayout<N>;
   const int planes = a.S * a.K;
-  {
-    TimeScope ts(KT_PATCH, s);
+  PTY_TRY(x.launch(KT_PATCH, lane, a, [&](hipStream_t s) {
     if constexpr (LAY == LAY_RT)
       k_patch_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
     else
       k_patch_gt<N><<<a.B * planes, kGtThreads, 0, s>>>(a);
-  }
-  if (a.shift) {
-    TimeScope ts(KT_PROBE_SHIFT, s);
-    if constexpr (LAY == LAY_RT)
-      k_probe_shift_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
-    else
-      k_probe_shift_gt<N><<<a.B * a.M, kGtThreads, 0, s>>>(a);
-  }
-  {
-    TimeScope ts(KT_FORWARD, s);
-    launch_forward<N>(a, s);
-  }
-  if (a.prep) {
-    TimeScope ts(KT_LOSS, s);
-    k_preprocess<N><<<a.B * ((N * N + 255) / 256), 256, 0, s>>>(a);
-  }
-  {
-    TimeScope ts(KT_LOSS, s);
+  }));
+  if (a.shift)
+    PTY_TRY(x.launch(KT_PROBE_SHIFT, lane, a, [&](hipStream_t s) {
+      if constexpr (LAY == LAY_RT)
+        k_probe_shift_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
+      else
+        k_probe_shift_gt<N><<<a.B * a.M, kGtThreads, 0, s>>>(a);
+    }));
+  PTY_TRY(x.launch(KT_FORWARD, lane, a, [&](hipStream_t s) { launch_forward<N>(a, s); }));
+  if (a.prep)
+    PTY_TRY(x.launch(KT_PREPROCESS, lane, a, [&](hipStream_t s) {
+      k_preprocess<N><<<a.B * ((N * N + 255) / 256), 256, 0, s>>>(a);
+    }));
+  PTY_TRY(x.launch(KT_LOSS, lane, a, [&](hipStream_t s) {
     if (compressed(a.algorithm))
       k_loss_c<N, LAY><<<a.B, 256, 0, s>>>(a);
     else
       k_loss<N, LAY><<<a.B * LossSplit<N>::CHUNKS, 256, 0, s>>>(a);
-  }
+  }));
   if (p->step_obj && ck) {
     const int C = (a.S + ck - 1) / ck;
     for (int c = C - 1; c >= 0; --c) {
@@ -2460,123 +2488,75 @@ int run_batch(BatchArgs& a, const pty_problem* p, int ck, hipStream_t s, hipEven
       g.st_dense = 0;
       g.stash = a.stash + size_t(C - 1) * N * N;  // the tail slots of each block
       if (c < C - 1) {
-        TimeScope ts(KT_RECOMPUTE, s);
         BatchArgs f = g;
         f.fin = a.stash + size_t(c) * N * N;
         f.fin_stride = size_t(a.st_slots) * N * N;
-        launch_forward<N>(f, s);
+        PTY_TRY(x.launch(KT_RECOMPUTE, lane, f, [&](hipStream_t s) { launch_forward<N>(f, s); }));
       }
-      {
-        TimeScope ts(KT_ADJOINT, s);
-        launch_adjoint<N>(g, s);
-      }
-      TimeScope ts(KT_OBJGRAD, s);
-      launch_objgrad<N>(g, s);
+      PTY_TRY(x.launch(KT_ADJOINT, lane, g, [&](hipStream_t s) { launch_adjoint<N>(g, s); }));
+      PTY_TRY(x.launch(KT_OBJGRAD, lane, g, [&](hipStream_t s) { launch_objgrad<N>(g, s); }));
     }
   } else {
-    if (p->step_obj || p->step_probe || p->step_pos) {
-      TimeScope ts(KT_ADJOINT, s);
-      launch_adjoint<N>(a, s);
-    }
-    if (p->step_obj) {
-      TimeScope ts(KT_OBJGRAD, s);
-      launch_objgrad<N>(a, s);
-    }
+    if (p->step_obj || p->step_probe || p->step_pos)
+      PTY_TRY(x.launch(KT_ADJOINT, lane, a, [&](hipStream_t s) { launch_adjoint<N>(a, s); }));
+    if (p->step_obj) PTY_TRY(x.launch(KT_OBJGRAD, lane, a, [&](hipStream_t s) { launch_objgrad<N>(a, s); }));
   }
-  if (wait_acc && hipStreamWaitEvent(s, wait_acc, 0) != hipSuccess) return PTY_ERR_HIP;
-  if (p->step_obj) {
-    TimeScope ts(KT_GATHER, s);
-    const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
-    k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
-  }
-  if (p->step_probe || (p->step_pos && a.shift)) {
-    TimeScope ts(KT_PROBEGRAD, s);
-    if constexpr (LAY == LAY_RT)
-      k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
-    else
-      k_probegrad_gt<N><<<a.G * a.M, kGtThreads, 0, s>>>(a);
-  }
-  if (p->step_pos && a.shift) {
-    TimeScope ts(KT_POS_FINAL, s);
-    k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a);
-  }
-  if (a.bg_grad) {
-    TimeScope ts(KT_LOSS, s);
-    k_bg_grad<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(a);
-  }
-  if (done_acc && hipEventRecord(done_acc, s) != hipSuccess) return PTY_ERR_HIP;
-  return check_launch();
+  if (wait_ev >= 0) PTY_TRY(x.wait(lane, wait_ev));
+  if (p->step_obj)
+    PTY_TRY(x.launch(KT_GATHER, lane, a, [&](hipStream_t s) {
+      const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
+      k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
+    }));
+  if (p->step_probe || (p->step_pos && a.shift))
+    PTY_TRY(x.launch(KT_PROBEGRAD, lane, a, [&](hipStream_t s) {
+      if constexpr (LAY == LAY_RT)
+        k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+      else
+        k_probegrad_gt<N><<<a.G * a.M, kGtThreads, 0, s>>>(a);
+    }));
+  if (p->step_pos && a.shift)
+    PTY_TRY(x.launch(KT_POS_FINAL, lane, a, [&](hipStream_t s) { k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
+  if (a.bg_grad)
+    PTY_TRY(x.launch(KT_BG_GRAD, lane, a, [&](hipStream_t s) {
+      k_bg_grad<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(a);
+    }));
+  if (done_ev >= 0) PTY_TRY(x.record(lane, done_ev));
+  return PTY_OK;
 }
 
+// The arguments shared by every batch of a pass (lane 0's per-batch buffers).
 template <int N>
-int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_t s) {
-  const Layout L = plan_layout(p);
-  if (!bf->workspace || bf->workspace_bytes < L.total) return PTY_ERR_WORKSPACE;
-  char* ws = static_cast<char*>(bf->workspace);
-  const int S = p->n_slices, K = p->n_obj_modes, M = p->n_probe_modes, planes = S * K;
-  const size_t n2 = size_t(N) * N;
-  const size_t objelems = size_t(p->obj_ny) * p->obj_nx * planes;
-  int rc;
-  float2* H = reinterpret_cast<float2*>(ws + L.H);
-  float2* H0 = reinterpret_cast<float2*>(ws + L.H0);
-  float* maskc = reinterpret_cast<float*>(ws + L.maskc);
-  int* zcol = reinterpret_cast<int*>(ws + L.zcol);
-  if ((rc = launch_tables<N>(p, H, H0, maskc, zcol, s))) return rc;
-  float2* fprobe = reinterpret_cast<float2*>(ws + L.fprobe);
-  {
-    TimeScope ts(KT_FPROBE, s);
-    if constexpr (kLayout<N> == LAY_RT)
-      k_fprobe_rt<N><<<M, RT<N>::NT, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe);
-    else
-      k_fprobe_gt<N><<<M, kGtThreads, 0, s>>>(static_cast<const float2*>(bf->probe), fprobe,
-                                             reinterpret_cast<float2*>(ws + L.scratch));
-  }
-  const float2* objp = static_cast<const float2*>(bf->obj);
-  float2* gradp = static_cast<float2*>(bf->obj_grad);
-  const int lgrid = int(std::min<size_t>((objelems + 255) / 256, 65536));
-  if (p->obj_layout == PTY_LAYOUT_REFERENCE) {
-    float2* tmp = reinterpret_cast<float2*>(ws + L.objp);
-    TimeScope ts(KT_LAYOUT, s);
-    k_ref_to_planar<<<lgrid, 256, 0, s>>>(objp, tmp, p->obj_ny, p->obj_nx, planes);
-    objp = tmp;
-    gradp = reinterpret_cast<float2*>(ws + L.gradp);
-  }
-  if (compressed(p->algorithm) && !bf->masks) return PTY_ERR_ARG;
-  if (bf->static_background && compressed(p->algorithm)) return PTY_ERR_ARG;
-  if (p->step_static_background && (!bf->static_background || !bf->static_background_grad)) return PTY_ERR_ARG;
-  if (p->step_static_background && !bf->accumulate &&
-      (rc = zero_fill(bf->static_background_grad, n2 * sizeof(float), s)))
-    return rc;
-  if (bf->measured_order != PTY_MEASURED_BY_POSITION && bf->measured_order != PTY_MEASURED_BY_SEQUENCE)
-    return PTY_ERR_ARG;
-  if (p->step_obj) {
-    if (!bf->obj_grad) return PTY_ERR_ARG;
-    if (!bf->accumulate && (rc = zero_fill(gradp, objelems * 8, s))) return rc;
-  }
-  if ((p->step_probe || p->step_pos) && !bf->accumulate) {
-    if ((rc = zero_fill(ws + L.probe_part, size_t(L.G) * M * n2 * 8, s))) return rc;
-  }
+BatchArgs pass_args(const pty_problem* p, const pty_buffers* bf, const Layout& L, char* ws) {
   BatchArgs a{};
-  a.S = S; a.K = K; a.M = M; a.ny = p->obj_ny; a.nx = p->obj_nx; a.P = p->n_positions;
+  auto at = [&](int id) { return L.bytes[id] ? ws + L.off[id] : nullptr; };
+  a.S = p->n_slices; a.K = p->n_obj_modes; a.M = p->n_probe_modes; a.ny = p->obj_ny; a.nx = p->obj_nx;
+  a.P = p->n_positions;
   a.algorithm = p->algorithm; a.step_obj = p->step_obj; a.step_probe = p->step_probe; a.step_pos = p->step_pos;
   a.shift = p->shift_active; a.far_field = 1; a.write_stash = p->step_obj; a.scatter = 1;
   a.loss_weight = p->loss_weight;
-  a.obj = objp; a.probe = static_cast<const float2*>(bf->probe); a.fprobe = fprobe;
+  const bool ref = p->obj_layout == PTY_LAYOUT_REFERENCE;
+  a.obj = ref ? reinterpret_cast<const float2*>(at(WB_OBJP)) : static_cast<const float2*>(bf->obj);
+  a.probe = static_cast<const float2*>(bf->probe);
+  a.fprobe = reinterpret_cast<float2*>(at(WB_FPROBE));
   a.pos = bf->positions; a.corr = bf->corrections; a.meas = bf->measured; a.seq = bf->sequence;
-  a.H = H; a.H0 = H0; a.maskc = maskc; a.zcol = zcol;
-  a.Ob = reinterpret_cast<float2*>(ws + L.Ob); a.Pp = reinterpret_cast<float2*>(ws + L.Pp);
-  a.stash = reinterpret_cast<float2*>(ws + L.stash); a.Psi = reinterpret_cast<float2*>(ws + L.Psi);
-  a.coef = reinterpret_cast<float*>(ws + L.coef); a.Gbuf = nullptr;
-  a.gP = reinterpret_cast<float2*>(ws + L.gP); a.probe_part = reinterpret_cast<float2*>(ws + L.probe_part);
-  a.pos_part = reinterpret_cast<double*>(ws + L.pos_part); a.loss_part = reinterpret_cast<double*>(ws + L.loss_part);
-  a.sim = bf->sim_patterns; a.obj_grad = gradp; a.pos_grad = bf->pos_grad; a.G = L.G;
-  a.scratch = L.scratch ? reinterpret_cast<float2*>(ws + L.scratch) : nullptr;
+  a.H = reinterpret_cast<float2*>(at(WB_H)); a.H0 = reinterpret_cast<float2*>(at(WB_H0));
+  a.maskc = reinterpret_cast<float*>(at(WB_MASKC)); a.zcol = reinterpret_cast<int*>(at(WB_ZCOL));
+  a.Ob = reinterpret_cast<float2*>(at(WB_OB)); a.Pp = reinterpret_cast<float2*>(at(WB_PP));
+  a.stash = reinterpret_cast<float2*>(at(WB_STASH)); a.Psi = reinterpret_cast<float2*>(at(WB_PSI));
+  a.coef = reinterpret_cast<float*>(at(WB_COEF)); a.Gbuf = nullptr;
+  a.gP = reinterpret_cast<float2*>(at(WB_GP)); a.probe_part = reinterpret_cast<float2*>(at(WB_PROBE_PART));
+  a.pos_part = reinterpret_cast<double*>(at(WB_POS_PART));
+  a.loss_part = reinterpret_cast<double*>(at(WB_LOSS_PART));
+  a.sim = bf->sim_patterns;
+  a.obj_grad = ref ? reinterpret_cast<float2*>(at(WB_GRADP)) : static_cast<float2*>(bf->obj_grad);
+  a.pos_grad = bf->pos_grad; a.G = L.G;
+  a.scratch = reinterpret_cast<float2*>(at(WB_SCRATCH));
   a.meas_order = bf->measured_order;
   a.prep = preprocessing(p) ? 1 : 0;
   a.raw_n = p->raw_n ? p->raw_n : N;
   a.sy = p->data_shift_y; a.sx = p->data_shift_x; a.bin = std::max(p->data_bin, 1); a.pad = p->data_pad;
   a.mult = (p->data_multiplier == 0.f) ? 1.f : p->data_multiplier;
-  a.meas_stage = L.meas_stage ? reinterpret_cast<float*>(ws + L.meas_stage) : nullptr;
+  a.meas_stage = reinterpret_cast<float*>(at(WB_MEAS_STAGE));
   a.masks = bf->masks;
   a.J = compressed(p->algorithm) ? p->n_masks : 0;
   a.bg = bf->static_background;
@@ -2587,72 +2567,475 @@ int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_
     a.st_every = L.ck;
     a.st_dense = (L.C - 1) * L.ck;
     a.st_slots = L.slots;
-    a.gstate = reinterpret_cast<float2*>(ws + L.gstate);
+    a.gstate = reinterpret_cast<float2*>(at(WB_GSTATE));
   }
-  // batch i runs in lane i % lanes: lane 0 on s, lane l > 0 on the library
-  // stream l paired with s, each lane with its own per-batch buffers
+  return a;
+}
+
+// The whole pass over nl lanes: tables, probe spectra and the planar object
+// copy on lane 0; batch i on lane i % nl, each lane with its own per-batch
+// buffers; the shared additions of the batches in batch order (events); the
+// lanes join lane 0, which reduces the loss, the probe gradient and copies
+// the object gradient back.
+template <int N, class X>
+int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int nl, X& x) {
+  constexpr int LAY = kLayout<N>;
+  const int M = p->n_probe_modes, planes = p->n_slices * p->n_obj_modes;
+  const size_t n2 = size_t(N) * N;
+  const size_t objelems = size_t(p->obj_ny) * p->obj_nx * planes;
+  const int lgrid = int(std::min<size_t>((objelems + 255) / 256, 65536));
+  char* ws = static_cast<char*>(bf->workspace);
+  const BatchArgs a = pass_args<N>(p, bf, L, ws);
+  PTY_TRY(x.launch(KT_TABLES, 0, a, [&](hipStream_t s) {
+    TableArgs t{const_cast<float2*>(a.H), const_cast<float2*>(a.H0), const_cast<float*>(a.maskc),
+                const_cast<int*>(a.zcol), p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y,
+                p->tan_x, p->tan_y, p->damping_cutoff, p->edge_multiplier};
+    k_tables<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(t);
+  }));
+  PTY_TRY(x.launch(KT_FPROBE, 0, a, [&](hipStream_t s) {
+    if constexpr (LAY == LAY_RT)
+      k_fprobe_rt<N><<<M, RT<N>::NT, 0, s>>>(a.probe, const_cast<float2*>(a.fprobe));
+    else
+      k_fprobe_gt<N><<<M, kGtThreads, 0, s>>>(a.probe, const_cast<float2*>(a.fprobe), a.scratch);
+  }));
+  if (p->obj_layout == PTY_LAYOUT_REFERENCE)
+    PTY_TRY(x.launch(KT_LAYOUT, 0, a, [&](hipStream_t s) {
+      k_ref_to_planar<<<lgrid, 256, 0, s>>>(static_cast<const float2*>(bf->obj), const_cast<float2*>(a.obj),
+                                             p->obj_ny, p->obj_nx, planes);
+    }, SUB_TO_PLANAR));
+  if (p->step_static_background && !bf->accumulate)
+    PTY_TRY(issue_zero(x, 0, a, bf->static_background_grad, n2 * sizeof(float)));
+  if (p->step_obj && !bf->accumulate) PTY_TRY(issue_zero(x, 0, a, a.obj_grad, objelems * 8));
+  if ((p->step_probe || p->step_pos) && !bf->accumulate)
+    PTY_TRY(issue_zero(x, 0, a, a.probe_part, size_t(L.G) * M * n2 * 8));
+  // lane l's arguments: its own copies of the per-batch buffers
+  BatchArgs al[kMaxLanes];
+  for (int l = 0; l < nl; ++l) {
+    BatchArgs& y = al[l];
+    y = a;
+    const size_t d = size_t(l) * L.lane_bytes;
+    auto off = [d](auto* ptr) { return ptr ? reinterpret_cast<decltype(ptr)>(reinterpret_cast<char*>(ptr) + d) : ptr; };
+    y.Ob = off(y.Ob); y.Pp = off(y.Pp); y.stash = off(y.stash); y.gstate = off(y.gstate);
+    y.Psi = off(y.Psi); y.coef = off(y.coef); y.gP = off(y.gP); y.pos_part = off(y.pos_part);
+    y.meas_stage = off(y.meas_stage); y.scratch = off(y.scratch);
+  }
+  if (nl > 1) {
+    PTY_TRY(x.record(0, EV_FORK));
+    for (int l = 1; l < nl; ++l) PTY_TRY(x.wait(l, EV_FORK));
+  }
+  const int n_batches = (bf->n_seq + L.B - 1) / L.B;
+  int rc = PTY_OK;
+  for (int i = 0; i < n_batches && rc == PTY_OK; ++i) {
+    const int l = i % nl;
+    BatchArgs y = al[l];
+    y.b0 = i * L.B;
+    y.B = std::min(L.B, bf->n_seq - y.b0);
+    y.G = std::min(L.G, y.B);
+    const int wait_ev = (nl > 1 && i > 0) ? EV_ACC + (i - 1) % nl : -1, done_ev = nl > 1 ? EV_ACC + l : -1;
+    rc = issue_batch<N>(y, p, L.ck, l, x, wait_ev, done_ev);
+  }
+  // the lanes' work joins lane 0 also when a launch failed
+  for (int l = 1; l < nl; ++l) {
+    PTY_TRY(x.record(l, EV_JOIN + l));
+    PTY_TRY(x.wait(0, EV_JOIN + l));
+  }
+  if (rc) return rc;
+  PTY_TRY(x.launch(KT_LOSS_FINAL, 0, a, [&](hipStream_t s) {
+    k_loss_final<<<1, 256, 0, s>>>(a.loss_part, bf->n_seq * LossSplit<N>::CHUNKS, double(p->loss_weight),
+                                   bf->loss_sse);
+  }));
+  if (p->step_probe) {
+    float2* psum = reinterpret_cast<float2*>(ws + L.off[WB_PROBE_SUM]);
+    const size_t per_group = size_t(M) * n2;
+    PTY_TRY(x.launch(KT_PROBE_REDUCE, 0, a, [&](hipStream_t s) {
+      k_probe_reduce<<<int((per_group + 255) / 256), 256, 0, s>>>(a.probe_part, L.G, per_group, psum);
+    }));
+    PTY_TRY(x.launch(KT_PROBE_FINAL, 0, a, [&](hipStream_t s) {
+      if constexpr (LAY == LAY_RT)
+        k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, p->shift_active, a.maskc,
+                                                     static_cast<float2*>(bf->probe_grad));
+      else
+        k_probe_final_gt<N><<<M, kGtThreads, 0, s>>>(psum, p->shift_active, a.maskc,
+                                                     static_cast<float2*>(bf->probe_grad), a.scratch);
+    }));
+  }
+  if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE)
+    PTY_TRY(x.launch(KT_LAYOUT, 0, a, [&](hipStream_t s) {
+      k_planar_to_ref<<<lgrid, 256, 0, s>>>(a.obj_grad, static_cast<float2*>(bf->obj_grad), p->obj_ny, p->obj_nx,
+                                             planes);
+    }, SUB_TO_REFERENCE));
+  return PTY_OK;
+}
+
+// The library streams that carry lanes 1 .. kMaxLanes-1 of the batches issued
+// on a caller stream, one set per (device, caller stream), created on first
+// use at the default priority and kept for the life of the process, with the
+// events that fork, order and join the lanes.  A call holds the state's mutex
+// while it enqueues.  A call runs one lane when its stream is being captured,
+// or while the lanes of another caller stream on the device still have work
+// in flight (so lanes of several caller streams never run at the same time).
+struct LaneState {
+  hipStream_t stream[kMaxLanes] = {};  // [0] unused: lane 0 is the caller's stream
+  hipEvent_t ev[EV_COUNT] = {};
+  int dev = 0;
+  std::mutex mu;
+};
+std::mutex g_lane_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<LaneState>> g_lanes;
+
+// the caller's lane state, or null (one lane) when none can be used now
+LaneState* lane_state(hipStream_t caller) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(caller, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+  std::lock_guard<std::mutex> g(g_lane_mu);
+  for (auto& kv : g_lanes) {
+    if (kv.first.first != dev || kv.first.second == caller) continue;
+    for (int l = 1; l < kMaxLanes; ++l)
+      if (hipEventQuery(kv.second->ev[EV_JOIN + l]) != hipSuccess) return nullptr;  // busy (or failed)
+  }
+  auto& slot = g_lanes[{dev, caller}];
+  if (!slot) {
+    auto st = std::make_unique<LaneState>();
+    st->dev = dev;
+    bool ok = true;
+    for (int l = 1; ok && l < kMaxLanes; ++l)
+      ok = hipStreamCreateWithFlags(&st->stream[l], hipStreamNonBlocking) == hipSuccess;
+    for (int e = 0; ok && e < EV_COUNT; ++e) ok = hipEventCreateWithFlags(&st->ev[e], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      // release what was made; the call runs one lane
+      for (int l = 1; l < kMaxLanes; ++l)
+        if (st->stream[l]) (void)hipStreamDestroy(st->stream[l]);
+      for (hipEvent_t e : st->ev)
+        if (e) (void)hipEventDestroy(e);
+      g_lanes.erase({dev, caller});
+      return nullptr;
+    }
+    slot = std::move(st);
+  }
+  return slot.get();
+}
+
+template <int N>
+int loss_and_direction_n(const pty_problem* p, const pty_buffers* bf, hipStream_t s) {
+  const Layout L = plan_layout(p);
+  if (!bf->workspace || bf->workspace_bytes < L.total) return PTY_ERR_WORKSPACE;
   const int n_batches = (bf->n_seq + L.B - 1) / L.B;
   const int lanes = std::min(L.lanes, n_batches);
   LaneState* ln = lanes >= 2 ? lane_state(s) : nullptr;
   const int nl = ln ? lanes : 1;
+  Enqueue x;
+  x.st[0] = s;
   std::unique_lock<std::mutex> lane_lock;
-  BatchArgs al[kMaxLanes];
-  hipStream_t lst[kMaxLanes];
-  for (int l = 0; l < nl; ++l) {
-    BatchArgs& x = al[l];
-    x = a;
-    const size_t d = size_t(l) * L.lane_bytes;
-    auto off = [d](auto* ptr) { return ptr ? reinterpret_cast<decltype(ptr)>(reinterpret_cast<char*>(ptr) + d) : ptr; };
-    x.Ob = off(x.Ob); x.Pp = off(x.Pp); x.stash = off(x.stash); x.gstate = off(x.gstate);
-    x.Psi = off(x.Psi); x.coef = off(x.coef); x.gP = off(x.gP); x.pos_part = off(x.pos_part);
-    x.meas_stage = off(x.meas_stage); x.scratch = off(x.scratch);
-    lst[l] = l ? ln->stream[l] : s;
-  }
   if (ln) {
     lane_lock = std::unique_lock<std::mutex>(ln->mu);
-    if (hipEventRecord(ln->fork, s) != hipSuccess) return PTY_ERR_HIP;
-    for (int l = 1; l < nl; ++l)
-      if (hipStreamWaitEvent(lst[l], ln->fork, 0) != hipSuccess) return PTY_ERR_HIP;
+    for (int l = 1; l < nl; ++l) x.st[l] = ln->stream[l];
+    for (int e = 0; e < EV_COUNT; ++e) x.ev[e] = ln->ev[e];
   }
-  rc = PTY_OK;
-  for (int i = 0; i < n_batches && rc == PTY_OK; ++i) {
-    const int l = i % nl;
-    BatchArgs& x = al[l];
-    x.b0 = i * L.B;
-    x.B = std::min(L.B, bf->n_seq - x.b0);
-    x.G = std::min(L.G, x.B);
-    hipEvent_t wait_acc = (ln && i > 0) ? ln->acc[(i - 1) % nl] : nullptr, done_acc = ln ? ln->acc[l] : nullptr;
-    rc = run_batch<N>(x, p, L.ck, lst[l], wait_acc, done_acc);
+  return issue_pass<N>(p, bf, L, nl, x);
+}
+
+// ------------------------------------------------------------ plan check
+// A pass's plan checked on the host: every buffer range a launch reads or
+// writes (from its arguments, as the kernel indexes them) stays inside its
+// buffer, per-batch buffers inside the lane's block, every pair of launches
+// on different lanes whose ranges overlap with a write is ordered by the
+// events, and the batches cover the sequence once.
+struct Access {
+  uintptr_t lo, hi;  // [lo, hi) bytes
+  bool write;
+  int buf;           // WB_* or an external buffer (kExt + PB_*)
+};
+enum { PB_OBJ = 0, PB_PROBE, PB_MEASURED, PB_LOSS_SSE, PB_OBJ_GRAD, PB_PROBE_GRAD, PB_POS_GRAD, PB_SIM,
+       PB_BG_GRAD, PB_COUNT };
+constexpr int kExt = 100;
+
+// the bytes of every range an op touches; gt: windows larger than 128 px
+void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const Layout& L, std::vector<Access>& out) {
+  out.clear();
+  if (o.kind == OP_RECORD || o.kind == OP_WAIT) return;
+  const BatchArgs& a = o.a;
+  const int n = p->n, S = a.S, K = a.K, M = a.M, MK = M * K;
+  const size_t n2 = size_t(n) * n, c8 = n2 * 8, c4 = n2 * 4;
+  const bool gt = n > 128;
+  const size_t objbytes = size_t(p->obj_ny) * p->obj_nx * S * K * 8;
+  auto ws = [&](int id) { return reinterpret_cast<uintptr_t>(bf->workspace) + L.off[id]; };
+  auto add = [&](const void* base, size_t bytes, bool w, int buf) {
+    if (base && bytes) out.push_back({reinterpret_cast<uintptr_t>(base), reinterpret_cast<uintptr_t>(base) + bytes, w, buf});
+  };
+  auto shared = [&](int id, bool w) { add(reinterpret_cast<void*>(ws(id)), L.bytes[id], w, id); };
+  auto tiles = [&](int count) { if (gt) add(a.scratch, size_t(count) * c8, true, WB_SCRATCH); };
+  const int B = a.B;
+  // stash slots a launch over [s_lo, s_hi) touches per (window, mode): from
+  // slot 0 to the last one it addresses, within blocks of st_slots
+  auto stash_range = [&](const float2* base, int last_slot, bool w) {
+    if (!base) return;
+    add(base, ((size_t(B) * MK - 1) * a.st_slots + last_slot + 1) * c8, w, WB_STASH);
+  };
+  switch (o.kind) {
+    case KT_TABLES:
+      shared(WB_H, true); shared(WB_H0, true); shared(WB_MASKC, true); shared(WB_ZCOL, true);
+      break;
+    case KT_FPROBE:
+      add(bf->probe, size_t(M) * c8, false, kExt + PB_PROBE);
+      shared(WB_FPROBE, true);
+      tiles(M);
+      break;
+    case KT_LAYOUT:
+      if (o.sub == SUB_TO_PLANAR) {
+        add(bf->obj, objbytes, false, kExt + PB_OBJ);
+        shared(WB_OBJP, true);
+      } else {
+        shared(WB_GRADP, false);
+        add(bf->obj_grad, objbytes, true, kExt + PB_OBJ_GRAD);
+      }
+      break;
+    case KT_ZERO:
+      add(o.p0, o.n0, true, -1);
+      break;
+    case KT_PATCH:
+      add(a.obj, objbytes, false, p->obj_layout == PTY_LAYOUT_REFERENCE ? WB_OBJP : kExt + PB_OBJ);
+      shared(WB_MASKC, false); shared(WB_ZCOL, false);
+      add(a.Ob, size_t(B) * S * K * c8, true, WB_OB);
+      tiles(B * S * K);
+      break;
+    case KT_PROBE_SHIFT:
+      shared(WB_FPROBE, false); shared(WB_MASKC, false); shared(WB_ZCOL, false);
+      add(a.Pp, size_t(B) * M * c8, true, WB_PP);
+      tiles(B * M);
+      break;
+    case KT_FORWARD:
+    case KT_RECOMPUTE: {
+      add(a.Ob, size_t(B) * S * K * c8, false, WB_OB);
+      if (a.shift) add(a.Pp, size_t(B) * M * c8, false, WB_PP);
+      shared(WB_H, false); shared(WB_H0, false); shared(WB_MASKC, false); shared(WB_ZCOL, false);
+      int last = -1;  // the last slot the forward stores
+      for (int s = a.s_lo; s < a.s_hi; ++s) {
+        const int r = s - a.s_lo;
+        const int slot = r >= a.st_dense ? a.st_dense / a.st_every + (r - a.st_dense)
+                                         : (r % a.st_every == 0 ? r / a.st_every : -1);
+        last = std::max(last, slot);
+      }
+      if (a.write_stash && last >= 0) stash_range(a.stash, last, true);
+      if (a.fin) {
+        // the checkpoint of each (window, mode): fin + (b * MK + mk) * fin_stride
+        add(a.fin, ((size_t(B) * MK - 1) * a.fin_stride + n2) * 8, false, WB_STASH);
+      } else {
+        add(a.Psi, size_t(B) * MK * c8, true, WB_PSI);
+      }
+      tiles(xcd_grid(B, MK));
+      break;
+    }
+    case KT_PREPROCESS:
+      add(a.meas_stage, size_t(B) * c4, true, WB_MEAS_STAGE);
+      break;
+    case KT_LOSS:
+      add(a.Psi, size_t(B) * MK * c8, false, WB_PSI);
+      if (a.prep) add(a.meas_stage, size_t(B) * c4, false, WB_MEAS_STAGE);
+      add(a.coef, size_t(B) * c4, true, WB_COEF);
+      add(reinterpret_cast<const char*>(a.loss_part) + size_t(a.b0) * 2 * 8 * loss_chunks(n),
+          size_t(B) * 2 * 8 * loss_chunks(n), true, WB_LOSS_PART);
+      if (a.sim) add(reinterpret_cast<const char*>(a.sim), size_t(p->n_positions) * c4, true, kExt + PB_SIM);
+      break;
+    case KT_ADJOINT:
+      if (a.s_hi == S) {
+        add(a.Psi, size_t(B) * MK * c8, false, WB_PSI);
+        add(a.coef, size_t(B) * c4, false, WB_COEF);
+      } else {
+        add(a.gstate, size_t(B) * MK * c8, false, WB_GSTATE);
+      }
+      add(a.Ob, size_t(B) * S * K * c8, false, WB_OB);
+      shared(WB_H, false); shared(WB_H0, false); shared(WB_ZCOL, false);
+      if (a.step_obj) stash_range(a.stash, a.s_hi - 1 - a.s_lo, true);
+      add(a.s_lo > 0 ? a.gstate : a.gP, size_t(B) * MK * c8, true, a.s_lo > 0 ? WB_GSTATE : WB_GP);
+      tiles(xcd_grid(B, MK));
+      break;
+    case KT_OBJGRAD:
+      stash_range(a.stash, a.s_hi - 1 - a.s_lo, false);
+      shared(WB_MASKC, false); shared(WB_ZCOL, false);
+      add(a.Ob, size_t(B) * S * K * c8, true, WB_OB);
+      tiles(B * (a.s_hi - a.s_lo) * K);
+      break;
+    case KT_GATHER:
+      add(a.Ob, size_t(B) * S * K * c8, false, WB_OB);
+      add(a.obj_grad, objbytes, true, p->obj_layout == PTY_LAYOUT_REFERENCE ? WB_GRADP : kExt + PB_OBJ_GRAD);
+      break;
+    case KT_PROBEGRAD:
+      add(a.gP, size_t(B) * MK * c8, false, WB_GP);
+      shared(WB_FPROBE, false); shared(WB_MASKC, false);
+      add(a.probe_part, size_t(a.G) * M * c8, true, WB_PROBE_PART);
+      if (a.step_pos && a.shift) add(a.pos_part, size_t(B) * M * 2 * 8, true, WB_POS_PART);
+      tiles(a.G * M);
+      break;
+    case KT_POS_FINAL:
+      add(a.pos_part, size_t(B) * M * 2 * 8, false, WB_POS_PART);
+      add(a.pos_grad, size_t(p->n_positions) * 2 * 4, true, kExt + PB_POS_GRAD);
+      break;
+    case KT_BG_GRAD:
+      add(a.coef, size_t(B) * c4, false, WB_COEF);
+      add(a.bg_grad, c4, true, kExt + PB_BG_GRAD);
+      break;
+    case KT_LOSS_FINAL:
+      add(a.loss_part, size_t(bf->n_seq) * 2 * 8 * loss_chunks(n), false, WB_LOSS_PART);
+      add(bf->loss_sse, 16, true, kExt + PB_LOSS_SSE);
+      break;
+    case KT_PROBE_REDUCE:
+      shared(WB_PROBE_PART, false);
+      shared(WB_PROBE_SUM, true);
+      break;
+    case KT_PROBE_FINAL:
+      shared(WB_PROBE_SUM, false); shared(WB_MASKC, false);
+      add(bf->probe_grad, size_t(M) * c8, true, kExt + PB_PROBE_GRAD);
+      tiles(M);
+      break;
+    default:
+      break;
   }
-  // the lanes' work joins s also when a launch failed
-  for (int l = 1; l < nl; ++l)
-    if (hipEventRecord(ln->join[l], lst[l]) != hipSuccess || hipStreamWaitEvent(s, ln->join[l], 0) != hipSuccess)
-      return PTY_ERR_HIP;
+}
+
+const char* op_name(int kind) {
+  if (kind == OP_RECORD) return "record";
+  if (kind == OP_WAIT) return "wait";
+  return (kind >= 0 && kind < KT_COUNT) ? kKindNames[kind] : "?";
+}
+
+// Checks `calls` passes of one problem issued one after another on one caller
+// stream (lane state and workspace reused), each over n_seq positions on nl
+// lanes.  Returns the number of violations; a description of each (and, with
+// verbose, the plan) is appended to report.
+// omit (for testing the check itself; the issued passes never omit anything):
+// bit 0 drops the waits on the batches' ordering events from the recorded
+// plan, bit 1 the waits on the fork, bit 2 the waits on the joins, bit 3
+// checks the ranges against a stash one window smaller than the workspace's.
+template <int N>
+int plan_check_n(const pty_problem* p, int n_seq, int nl, int calls, int accumulate, int verbose, int omit,
+                 std::string& report) {
+  Layout L = plan_layout(p);
+  // distinct fake address ranges, 1 TiB apart: the workspace and each external buffer
+  constexpr uintptr_t kSpan = uintptr_t(1) << 40;
+  pty_buffers bf{};
+  auto fake = [&](int i) { return reinterpret_cast<void*>(kSpan * uintptr_t(i + 1)); };
+  bf.obj = fake(1 + PB_OBJ); bf.probe = fake(1 + PB_PROBE); bf.positions = static_cast<const int*>(fake(40));
+  bf.corrections = static_cast<const float*>(fake(41)); bf.measured = static_cast<const float*>(fake(1 + PB_MEASURED));
+  bf.sequence = static_cast<const int*>(fake(42)); bf.n_seq = n_seq;
+  bf.loss_sse = static_cast<double*>(fake(1 + PB_LOSS_SSE)); bf.obj_grad = fake(1 + PB_OBJ_GRAD);
+  bf.probe_grad = fake(1 + PB_PROBE_GRAD); bf.pos_grad = static_cast<float*>(fake(1 + PB_POS_GRAD));
+  bf.workspace = fake(0); bf.workspace_bytes = L.total; bf.masks = static_cast<const float*>(fake(43));
+  bf.measured_order = PTY_MEASURED_BY_POSITION; bf.accumulate = accumulate;
+  if (p->step_static_background) {
+    bf.static_background = static_cast<const float*>(fake(44));
+    bf.static_background_grad = static_cast<float*>(fake(1 + PB_BG_GRAD));
+  }
+  int rc = validate_buffers(p, &bf);
   if (rc) return rc;
-  {
-    TimeScope ts(KT_LOSS_FINAL, s);
-    k_loss_final<<<1, 256, 0, s>>>(a.loss_part, bf->n_seq * LossSplit<N>::CHUNKS, double(p->loss_weight),
-                                   bf->loss_sse);
+  const int n_batches = (n_seq + L.B - 1) / L.B;
+  nl = std::max(1, std::min({nl, L.lanes, n_batches}));
+  std::vector<Op> ops;
+  for (int c = 0; c < calls; ++c) {
+    Recorder r;
+    PTY_TRY(issue_pass<N>(p, &bf, L, nl, r));
+    for (const Op& o : r.ops) {
+      const bool drop = o.kind == OP_WAIT && (((omit & 1) && o.ev >= EV_ACC) || ((omit & 2) && o.ev == EV_FORK) ||
+                                              ((omit & 4) && o.ev >= EV_JOIN && o.ev < EV_ACC));
+      if (!drop) ops.push_back(o);
+    }
   }
-  if (p->step_probe) {
-    if (!bf->probe_grad) return PTY_ERR_ARG;
-    TimeScope ts(KT_PROBE_FINAL, s);
-    const size_t per_group = size_t(M) * n2;
-    float2* psum = reinterpret_cast<float2*>(ws + L.probe_sum);
-    k_probe_reduce<<<int((per_group + 255) / 256), 256, 0, s>>>(a.probe_part, L.G, per_group, psum);
-    if constexpr (kLayout<N> == LAY_RT)
-      k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, p->shift_active, maskc,
-                                                   static_cast<float2*>(bf->probe_grad));
-    else
-      k_probe_final_gt<N><<<M, kGtThreads, 0, s>>>(psum, p->shift_active, maskc,
-                                                   static_cast<float2*>(bf->probe_grad), a.scratch);
+  if (omit & 8) L.bytes[WB_STASH] -= L.bytes[WB_STASH] / L.B;
+  int bad = 0;
+  char line[512];
+  auto note = [&](const char* s) {
+    report += s;
+    report += "\n";
+  };
+  // vector clocks: per lane, per event; an op's stamp is its lane's clock
+  std::vector<std::vector<int>> vc(ops.size());
+  std::vector<std::vector<int>> lane(kMaxLanes, std::vector<int>(kMaxLanes, 0));
+  std::vector<std::vector<int>> evc(EV_COUNT, std::vector<int>(kMaxLanes, 0));
+  std::vector<std::vector<Access>> acc(ops.size());
+  const uintptr_t ws0 = reinterpret_cast<uintptr_t>(bf.workspace);
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const Op& o = ops[i];
+    auto& lc = lane[o.lane];
+    if (o.kind == OP_RECORD) {
+      evc[o.ev] = lc;
+    } else if (o.kind == OP_WAIT) {
+      for (int l = 0; l < kMaxLanes; ++l) lc[l] = std::max(lc[l], evc[o.ev][l]);
+    } else {
+      lc[o.lane] += 1;
+      vc[i] = lc;
+      op_accesses(o, p, &bf, L, acc[i]);
+    }
+    if (verbose) {
+      snprintf(line, sizeof line, "op %zu lane %d %s%s b0 %d B %d slices [%d,%d) ev %d", i, o.lane, op_name(o.kind),
+               o.kind == KT_LAYOUT ? (o.sub == SUB_TO_PLANAR ? " to-planar" : " to-reference") : "", o.a.b0, o.a.B,
+               o.a.s_lo, o.a.s_hi, o.ev);
+      note(line);
+    }
+    // bounds: workspace ranges inside their buffer (per-lane buffers: the op's lane's copy)
+    for (const Access& x : acc[i]) {
+      if (x.lo < ws0 || x.lo >= ws0 + kSpan) continue;  // an external buffer
+      if (x.hi > ws0 + L.total) {
+        snprintf(line, sizeof line, "OUT OF WORKSPACE: op %zu %s lane %d touches [%zu, %zu) of %zu bytes", i,
+                 op_name(o.kind), o.lane, size_t(x.lo - ws0), size_t(x.hi - ws0), L.total);
+        note(line);
+        ++bad;
+        continue;
+      }
+      if (x.buf < 0 || x.buf >= WB_COUNT) continue;  // a fill: checked against the buffers below
+      const size_t base = L.off[x.buf] + (x.buf >= kFirstLaneBuffer ? size_t(o.lane) * L.lane_bytes : 0);
+      if (x.lo - ws0 < base || x.hi - ws0 > base + L.bytes[x.buf]) {
+        snprintf(line, sizeof line, "OUT OF BUFFER: op %zu %s lane %d touches %s [%zu, %zu), its buffer is [%zu, %zu)",
+                 i, op_name(o.kind), o.lane, kBufNames[x.buf], size_t(x.lo - ws0), size_t(x.hi - ws0), base,
+                 base + L.bytes[x.buf]);
+        note(line);
+        ++bad;
+      }
+    }
   }
-  if (p->step_obj && p->obj_layout == PTY_LAYOUT_REFERENCE) {
-    TimeScope ts(KT_LAYOUT, s);
-    k_planar_to_ref<<<lgrid, 256, 0, s>>>(gradp, static_cast<float2*>(bf->obj_grad), p->obj_ny, p->obj_nx,
-                                           planes);
+  // races: overlapping ranges on different lanes, one of them written, not ordered by the events
+  for (size_t j = 0; j < ops.size(); ++j) {
+    if (vc[j].empty()) continue;
+    for (size_t i = 0; i < j; ++i) {
+      if (vc[i].empty() || ops[i].lane == ops[j].lane) continue;
+      const int li = ops[i].lane;
+      if (vc[j][li] >= vc[i][li]) continue;  // i happens before j
+      for (const Access& x : acc[i])
+        for (const Access& y : acc[j])
+          if ((x.write || y.write) && x.lo < y.hi && y.lo < x.hi) {
+            snprintf(line, sizeof line, "RACE: op %zu %s (lane %d, b0 %d) and op %zu %s (lane %d, b0 %d) on [%zu, %zu)",
+                     i, op_name(ops[i].kind), li, ops[i].a.b0, j, op_name(ops[j].kind), ops[j].lane, ops[j].a.b0,
+                     size_t(std::max(x.lo, y.lo) - ws0), size_t(std::min(x.hi, y.hi) - ws0));
+            note(line);
+            ++bad;
+          }
+    }
   }
-  return check_launch();
+  // coverage: the patch launches of each call cover [0, n_seq) once, in order
+  int call = 0, at = 0;
+  for (const Op& o : ops) {
+    if (o.kind != KT_PATCH) continue;
+    if (at == n_seq) {
+      ++call;
+      at = 0;
+    }
+    if (o.a.b0 != at || o.a.B < 1 || o.a.b0 + o.a.B > n_seq) {
+      snprintf(line, sizeof line, "COVERAGE: call %d: batch at %d of %d windows, expected one at %d", call, o.a.b0,
+               o.a.B, at);
+      note(line);
+      ++bad;
+    }
+    at = o.a.b0 + o.a.B;
+  }
+  if (call != calls - 1 || at != n_seq) {
+    snprintf(line, sizeof line, "COVERAGE: the batches end at call %d, window %d of %d", call, at, n_seq);
+    note(line);
+    ++bad;
+  }
+  snprintf(line, sizeof line, "plan: %zu steps, %d lanes, %d batches of up to %d windows per call, %d call(s); %d "
+           "violation(s)", ops.size(), nl, n_batches, L.B, calls, bad);
+  note(line);
+  return bad;
 }
 
 template <int N>
@@ -2813,6 +3196,32 @@ int pty_ms_adjoint(const pty_problem* p, int batch, void* g, void* stash, const 
   if (batch < 1 || !g || !stash || !obj_b || !propagator || !mask || !grad_b || !tables) return PTY_ERR_ARG;
   hipStream_t s = static_cast<hipStream_t>(stream);
   PTY_DISPATCH_N(p->n, ms_adjoint_n, p, batch, g, stash, obj_b, propagator, mask, grad_b, tables, s);
+}
+
+int pty_plan_check(const pty_problem* p, int n_seq, int lanes, int calls, int accumulate, int verbose, int omit,
+                   char* report, size_t report_bytes) {
+  const int rc = validate(p);
+  if (rc) return rc;
+  if (n_seq < 1 || n_seq > p->n_positions || lanes < 1 || calls < 1) return PTY_ERR_ARG;
+  std::string text;
+  int bad;
+  switch (p->n) {
+    case 16: bad = plan_check_n<16>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    case 32: bad = plan_check_n<32>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    case 48: bad = plan_check_n<48>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    case 64: bad = plan_check_n<64>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    case 96: bad = plan_check_n<96>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    case 128: bad = plan_check_n<128>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    case 192: bad = plan_check_n<192>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    case 256: bad = plan_check_n<256>(p, n_seq, lanes, calls, accumulate, verbose, omit, text); break;
+    default: return PTY_ERR_SIZE;
+  }
+  if (report && report_bytes) {
+    const size_t k = std::min(text.size(), report_bytes - 1);
+    memcpy(report, text.data(), k);
+    report[k] = 0;
+  }
+  return bad;
 }
 
 void pty_timing_enable(int on) {

@@ -105,11 +105,10 @@ class PtychoPass:
     """One forward + loss + adjoint pass over a sequence of positions, on one HIP device.
 
     Workspaces are cached per (device, size, stream) for the life of the process, so
-    repeated passes of the same problem shapes allocate nothing.  An engine takes its
-    workspace at its first run, from the cache entry of the stream that run is issued on,
-    and keeps it: engines of one shape share a workspace only when they start on the same
-    stream, whose order keeps their passes apart, and engines started on different streams
-    never share one.  ``release_workspaces()`` drops the cache (engines keep theirs).
+    repeated passes of the same problem shapes allocate nothing.  Every run takes the
+    workspace of the stream it is issued on: passes that share a workspace are issued on
+    one stream, whose order keeps them apart, and passes on different streams never share
+    one, whichever engine issues them.  ``release_workspaces()`` drops the cache.
     """
 
     _workspaces: dict = {}
@@ -126,7 +125,7 @@ class PtychoPass:
         nbytes = ctypes.c_size_t(0)
         _lib.check(self.lib.pty_workspace_bytes(ctypes.byref(self.c_problem), ctypes.byref(nbytes)), "workspace")
         self.workspace_bytes = int(nbytes.value)
-        self.workspace = None  # taken at the first run
+        self.workspace = None  # the workspace of the last run
         self.loss_sse = torch.zeros(2, dtype=torch.float64, device=self.device)
 
     def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
@@ -165,13 +164,12 @@ class PtychoPass:
                 raise ValueError(f"pypty_amd: masks must be float32 ({c.n_masks}, {c.n}, {c.n})")
             if measured.shape[-1] != c.n_masks:
                 raise ValueError("pypty_amd: compressed data need one coefficient per virtual detector")
-        if self.workspace is None:
-            key = (self.device, self.workspace_bytes, _lib.stream_ptr(self.device))
-            ws = PtychoPass._workspaces.get(key)
-            if ws is None:
-                ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
-                PtychoPass._workspaces[key] = ws
-            self.workspace = ws
+        key = (self.device, self.workspace_bytes, _lib.stream_ptr(self.device))
+        ws = PtychoPass._workspaces.get(key)
+        if ws is None:
+            ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
+            PtychoPass._workspaces[key] = ws
+        self.workspace = ws  # the last run's
         for t, name in ((static_background, "static_background"), (static_background_grad, "static_background_grad")):
             if t is not None:
                 _lib.require_device_tensor(t, name)

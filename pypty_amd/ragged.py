@@ -116,7 +116,7 @@ class RaggedStep:
             best = autotune.choose_batch(key, candidates, make_pass)
             j.set_batch(best)
             self.tuned.append(best)
-        # the candidates' workspaces; the chosen engines take theirs at their first run
+        # the candidates' workspaces; the chosen engines' runs take theirs
         PtychoPass.release_workspaces()
         return self.tuned
 
@@ -139,7 +139,8 @@ class RaggedStep:
         return torch.stack([j.loss_sse for j in self.jobs])
 
     def capture(self):
-        """Record one step in a HIP graph.  The engines must have run once (workspaces taken)."""
+        """Record one step in a HIP graph (the datasets on their streams, whose workspaces the eager
+        steps already hold)."""
         side = torch.cuda.Stream(self.dev)
         side.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(side):
