@@ -348,7 +348,15 @@ def build_workload(name, dev, args):
         tune, graph = False, False
     t_gen = time.perf_counter() - t0
     step = ragged.RaggedStep(datasets, dev, **kw)
-    if tune:
+    if args.batches:
+        # windows per launch given per dataset (e.g. a tuned choice, to skip the tuner in profiled runs)
+        given = [int(b) for b in args.batches.split(",")]
+        if len(given) != len(step.jobs):
+            raise SystemExit(f"bench.py: --batches needs {len(step.jobs)} values")
+        for j, b in zip(step.jobs, given):
+            j.set_batch(b)
+        step.tuned = given
+    elif tune:
         step.tune((64, 128, 256, 512))
     info = {"lanes": kw["lanes"], "streams": step.n_streams, "checkpoint_slices": kw["checkpoint"],
             "hip_graph": graph, "data_gen_s": round(t_gen, 1)}
@@ -468,13 +476,16 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", choices=sorted(WORKLOADS), default="ragged")
     ap.add_argument("--also", default="multislice",
-                    help="comma-separated further workloads measured beside the headline ('' for none)")
+                    help="comma-separated further workloads measured beside the headline ('none' for none)")
     ap.add_argument("--scan", type=int, default=64, help="scan grid side of the multislice workload")
     ap.add_argument("--thick-scan", type=int, default=128, help="scan grid side of the thick workload")
     ap.add_argument("--cpu-sample", type=int, default=32,
                     help="positions per dataset timed on the CPU (0: skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=3, help="steps of the per-kernel HIP-event pass")
+    ap.add_argument("--batches", default="",
+                    help="comma-separated windows per launch of each dataset of the headline workload "
+                         "(instead of the autotuner)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for rocprofv3 counter scaling)")
     ap.add_argument("--recon-epochs", type=int, default=20,
@@ -496,7 +507,7 @@ def main(argv=None):
     print(f"[bench] {args.config}: {head['value']:.1f} patterns/s, median {head['ms_per_step']:.2f} ms per step",
           file=sys.stderr, flush=True)
     extra = {}
-    for name in [x for x in args.also.split(",") if x and x != args.config]:
+    for name in [x for x in args.also.split(",") if x and x not in (args.config, "none")]:
         _, st, r = measure(name, dev, args, kernel_timing=not args.no_kernel_timing)
         extra[name] = r
         del st

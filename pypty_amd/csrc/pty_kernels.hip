@@ -1264,11 +1264,13 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       const float2 Q = cmulc(v[i], E);
       sum[i] = cadd(sum[i], Q);
       if (a.step_pos) {
-        const float2 Z = cmulc(Q, fp[i]);  // conj(FP) * Q
+        // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
+        const float2 F = fp[i];
+        const double zy = double(Q.y) * double(F.x) - double(Q.x) * double(F.y);
         const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
         const float ky = fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk);
-        sx += -double(kx) * double(Z.y);
-        sy += -double(ky) * double(Z.y);
+        sx += -double(kx) * zy;
+        sy += -double(ky) * zy;
       }
     }
     if (a.step_pos) {
@@ -1859,11 +1861,13 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
         const float2 Q = cmulc(v[i], E);
         acc.store(q, cadd(acc[q], Q));
         if (a.step_pos) {
-          const float2 Z = cmulc(Q, fp[q]);  // conj(FP) * Q
+          // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
+          const float2 F = fp[q];
+          const double zy = double(Q.y) * double(F.x) - double(Q.x) * double(F.y);
           const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
           const float ky = fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk);
-          sx += -double(kx) * double(Z.y);
-          sy += -double(ky) * double(Z.y);
+          sx += -double(kx) * zy;
+          sy += -double(ky) * zy;
         }
       }
     }

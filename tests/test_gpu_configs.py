@@ -207,6 +207,7 @@ def test_ragged_step_graph_replay_matches_eager_and_oracle():
     ds = big.ds
     start = (np.abs(ds.obj) * np.exp(0.7j * np.angle(ds.obj))).astype(np.complex64)
     big.obj.copy_(torch.from_numpy(start).to(dev))
+    big.posg.zero_()  # a pass assigns the gradients of its sequence's positions only (objective.py:658)
     loss, sse = big.run(sync=True, sequence=torch.from_numpy(seq.astype(np.int32)).to(dev))
     kw = dict(dz=ds.dz_A, wavelength=ds.wavelength_A, pixel_size_x=ds.pixel_A, pixel_size_y=ds.pixel_A,
               step_obj=True, step_probe=True, step_pos=True, compute_batch=4)
