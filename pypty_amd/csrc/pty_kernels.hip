@@ -3170,10 +3170,7 @@ int pty_workspace_bytes(const pty_problem* p, size_t* bytes) {
 int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* stream) {
   int rc = validate(p);
   if (rc) return rc;
-  if (!b || !b->obj || !b->probe || !b->positions || !b->measured || !b->sequence || !b->loss_sse) return PTY_ERR_ARG;
-  if (b->n_seq < 1 || b->n_seq > p->n_positions) return PTY_ERR_ARG;
-  if ((p->shift_active || p->step_pos) && !b->corrections) return PTY_ERR_ARG;
-  if (p->step_pos && !b->pos_grad) return PTY_ERR_ARG;
+  if ((rc = validate_buffers(p, b))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   PTY_DISPATCH_FUSED(p->n, loss_and_direction_n, p, b, s);
 }
