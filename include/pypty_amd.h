@@ -251,6 +251,21 @@ int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float ph
  * (pypty/utils.py:260-276, complex_grad_to_phase_grad); out float32, n elements. */
 int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream);
 
+/* The same pass without a caller workspace and without any allocation (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  b->workspace is ignored: the workspace is a device
+ * array of PTY_STATIC_WORKSPACE_BYTES inside the library, one per GPU, held
+ * until the process ends.  Problems whose one-lane workspace
+ * (pty_workspace_bytes with lanes = 1) exceeds it are rejected with
+ * PTY_ERR_WORKSPACE.  The pass runs one batch lane and calls no hipMalloc,
+ * hipFree or allocating routine.  Every call on one GPU shares that GPU's
+ * array, which keeps the last call's contents: calls that overlap on one GPU
+ * race (issue them on one stream), and b->accumulate adds to the gradients of
+ * the previous call of the same problem on that GPU. */
+#define PTY_STATIC_WORKSPACE_BYTES (64u << 20)
+int pty_loss_and_direction_static(const pty_problem* p, const pty_buffers* b, void* stream);
+
 /* Host-side check of the launch plan of pty_loss_and_direction (no
  * counterpart in the reference, whose objective.py:393-710 loop is serial):
  * `calls` passes of problem p over n_seq positions on up to `lanes` batch

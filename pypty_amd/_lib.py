@@ -19,13 +19,15 @@ PTY_LOSS_LSQ_SQRT, PTY_LOSS_ML, PTY_LOSS_LSQ, PTY_LOSS_LSQ_SQRT_2 = 0, 1, 2, 3
 PTY_LOSS_LSQ_COMPRESSED, PTY_LOSS_GAUSS_COMPRESSED, PTY_LOSS_POISSON_COMPRESSED = 4, 5, 6
 PTY_MEASURED_BY_POSITION, PTY_MEASURED_BY_SEQUENCE = 0, 1
 PTY_LAYOUT_REFERENCE, PTY_LAYOUT_PLANAR = 0, 1
+PTY_ERR_WORKSPACE = -5
+PTY_STATIC_WORKSPACE_BYTES = 64 << 20  # the workspace array of pty_loss_and_direction_static
 
 EXPORTED = (
     "pty_status_string", "pty_version", "pty_supported_window", "pty_fft2_c64", "pty_bandlimit_c64",
     "pty_scatteradd_c64", "pty_workspace_bytes", "pty_loss_and_direction", "pty_tables_bytes",
     "pty_ms_forward", "pty_ms_adjoint", "pty_timing_enable", "pty_timing_read", "pty_timing_reset",
     "pty_fourier_clean_c64", "pty_plane_fft2_c64", "pty_multidot_c64", "pty_axpby_c64", "pty_phase_grad_c64",
-    "pty_l1_constraint_c64", "pty_plan_check",
+    "pty_l1_constraint_c64", "pty_plan_check", "pty_loss_and_direction_static",
     "pty_gemm_bf16", "pty_cnn_im2col", "pty_cnn_col2im", "pty_cnn_bias_act", "pty_cnn_act_grad",
     "pty_cnn_bias_grad", "pty_cnn_to_bf16", "pty_cnn_mse", "pty_cnn_adam",
 )
@@ -113,6 +115,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
     lib.pty_phase_grad_c64.argtypes = [vp, vp, vp, i64, vp]
     lib.pty_l1_constraint_c64.restype = i
     lib.pty_l1_constraint_c64.argtypes = [vp, i64, ctypes.c_float, ctypes.c_float, vp, vp, vp, vp]
+    lib.pty_loss_and_direction_static.restype = i
+    lib.pty_loss_and_direction_static.argtypes = [P(PtyProblem), P(PtyBuffers), vp]
     lib.pty_plan_check.restype = i
     lib.pty_plan_check.argtypes = [P(PtyProblem), i, i, i, i, i, i, ctypes.c_char_p, sz]
     lib.pty_timing_enable.restype = None

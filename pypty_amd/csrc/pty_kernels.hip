@@ -2116,7 +2116,7 @@ struct TimeScope {
   hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
   bool on;
-  TimeScope(int k, hipStream_t st) : kind(k), s(st), on(g_timing) {
+  TimeScope(int k, hipStream_t st, bool allowed = true) : kind(k), s(st), on(g_timing && allowed) {
     if (on) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
@@ -2365,9 +2365,10 @@ struct Op {
 struct Enqueue {
   hipStream_t st[kMaxLanes] = {};
   hipEvent_t ev[EV_COUNT] = {};
+  bool timed = true;  // HIP-event timing of the launches when pty_timing_enable is on
   template <class F>
   int launch(int kind, int lane, const BatchArgs&, F&& f, int = SUB_NONE, const void* = nullptr, size_t = 0) {
-    TimeScope ts(kind, st[lane]);
+    TimeScope ts(kind, st[lane], timed);
     f(st[lane]);
     return check_launch();
   }
@@ -2719,6 +2720,38 @@ LaneState* lane_state(hipStream_t caller) {
     slot = std::move(st);
   }
   return slot.get();
+}
+
+// The workspace of pty_loss_and_direction_static: a file-scope device array,
+// one copy per GPU the library runs on, held until the process ends.  Host
+// code never passes the array's name to a kernel or copy: it uses the
+// device address that hipGetSymbolAddress returns for the current device.
+constexpr size_t kStaticWorkspaceBytes = size_t(PTY_STATIC_WORKSPACE_BYTES);
+static __device__ __align__(256) char scratch[kStaticWorkspaceBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of `scratch`, looked up once per device
+int scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return PTY_ERR_HIP;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return PTY_ERR_HIP;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PTY_OK;
+}
+
+template <int N>
+int loss_and_direction_static_n(const pty_problem* p, const pty_buffers* bf, hipStream_t s) {
+  Enqueue x;
+  x.st[0] = s;
+  x.timed = false;  // no timing events are created
+  return issue_pass<N>(p, bf, plan_layout(p), 1, x);
 }
 
 template <int N>
@@ -3173,6 +3206,23 @@ int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* str
   if ((rc = validate_buffers(p, b))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   PTY_DISPATCH_FUSED(p->n, loss_and_direction_n, p, b, s);
+}
+
+int pty_loss_and_direction_static(const pty_problem* p, const pty_buffers* b, void* stream) {
+  int rc = validate(p);
+  if (rc) return rc;
+  if (!b) return PTY_ERR_ARG;
+  // one lane: no library streams or events are made
+  pty_problem q = *p;
+  q.lanes = 1;
+  if (plan_layout(&q).total > kStaticWorkspaceBytes) return PTY_ERR_WORKSPACE;
+  pty_buffers c = *b;
+  c.workspace_bytes = kStaticWorkspaceBytes;
+  c.workspace = nullptr;
+  if ((rc = scratch_address(&c.workspace))) return rc;
+  if ((rc = validate_buffers(&q, &c))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PTY_DISPATCH_FUSED(q.n, loss_and_direction_static_n, &q, &c, s);
 }
 
 int pty_tables_bytes(int n, size_t* bytes) {
