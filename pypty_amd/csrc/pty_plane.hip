@@ -206,8 +206,10 @@ __global__ __launch_bounds__(256) void k_plane_cols_band(float2* __restrict__ da
   for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
     const int y = i / nc, c = i % nc;
     const int fy = plan_freq(py, y), fx = plan_freq(px, x0 + c);
-    const double vy = double((2 * fy < ny) ? fy : fy - ny) / double(ny);
-    const double vx = double((2 * fx < nx) ? fx : fx - nx) / double(nx);
+    // numpy.fft.fftfreq(n) = k * (1 / n), rounded as numpy does (a pixel on the circle stays on
+    // the reference's side of it)
+    const double vy = double((2 * fy < ny) ? fy : fy - ny) * (1.0 / double(ny));
+    const double vx = double((2 * fx < nx) ? fx : fx - nx) * (1.0 / double(nx));
     const double r = vx * vx + vy * vy;
     float m = (r <= max_r * max_r) ? 1.f : 0.f;
     if (rolloff != 0.0 && r > r0 && m > 0.f)

@@ -2,7 +2,9 @@
 
 Same call signatures as the reference: ``fft2(arr, axes=(0,1),
 overwrite_x=False, xp=None)``.  ``arr`` is a complex64 torch tensor on a HIP
-device; the two transformed axes are moved innermost, the batch of n x n
+device, a numpy array (copied to the device; the result is a numpy array) or a
+device array of another library such as CuPy (viewed through DLPack; the
+result is of that library); the two transformed axes are moved innermost, the batch of n x n
 tiles is transformed by the LDS-resident gfx950 kernel (pty_fft2_c64) and the
 axes are moved back.  numpy convention: forward unnormalised, inverse 1/n^2.
 ``xp`` is accepted for signature compatibility and ignored.
@@ -11,12 +13,21 @@ from __future__ import annotations
 
 import torch
 
-from . import _lib
+from . import _arrays, _lib
 
 
-def _run(arr: torch.Tensor, axes, mode: int, overwrite_x: bool) -> torch.Tensor:
-    if not isinstance(arr, torch.Tensor) or arr.device.type != "cuda":
-        raise TypeError("pypty_amd.fft: input must be a torch tensor on a HIP device")
+def _run(arr, axes, mode: int, overwrite_x: bool):
+    kind = _arrays.kind_of(arr)
+    if kind != _arrays.TORCH:
+        # numpy (copied to the device) or a device array of another library (viewed through DLPack);
+        # the result comes back as the caller's type
+        t = _arrays.to_torch(arr)
+        if t.dtype == torch.complex128 or not t.is_complex():
+            t = t.to(torch.complex64)
+        out = _run(t, axes, mode, overwrite_x and kind == _arrays.FOREIGN)
+        return _arrays.like(out, kind, arr)
+    if arr.device.type != "cuda":
+        raise TypeError("pypty_amd.fft: input must be on a HIP device")
     if arr.dtype != torch.complex64:
         raise TypeError(f"pypty_amd.fft: complex64 required, got {arr.dtype}")
     nd = arr.dim()

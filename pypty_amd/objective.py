@@ -31,7 +31,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import _lib
+from . import _arrays, _lib
 
 LOSSES = {"lsq_sqrt": _lib.PTY_LOSS_LSQ_SQRT, "ml": _lib.PTY_LOSS_ML, "lsq": _lib.PTY_LOSS_LSQ,
           "lsq_sqrt_2": _lib.PTY_LOSS_LSQ_SQRT_2, "lsq_compressed": _lib.PTY_LOSS_LSQ_COMPRESSED,
@@ -288,9 +288,14 @@ def check_windows_inside(positions: torch.Tensor, sequence: torch.Tensor, n: int
 
 
 def _as_device(x, dtype, device):
-    if isinstance(x, torch.Tensor):
-        return x.to(device=device, dtype=dtype).contiguous()
-    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device).contiguous()
+    return _arrays.to_torch(x, device, dtype).contiguous()
+
+
+def _host(x, dtype):
+    """A small argument (distances, tilts) as a numpy array, from any array library."""
+    if isinstance(x, torch.Tensor) or _arrays.kind_of(x) == _arrays.FOREIGN:
+        return _arrays.to_torch(x).cpu().numpy().astype(dtype)
+    return np.asarray(x, dtype=dtype)
 
 
 def _unsupported(name, cond):
@@ -317,13 +322,15 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
                        data_simulation_exitwave, use_padded_intensity, *, checkpoint_slices=0):
     """Drop-in for pypty.objective.loss_and_direction (pypty/objective.py:47).
 
-    this_obj (Ny, Nx, S, K) complex64 and full_probe (N, N, M) complex64 are
-    torch tensors on the HIP device; positions (P, 2) int, corrections (P, 2)
-    float32 and measured_array (P, N, N) float32 may be torch tensors or numpy
-    arrays.  Returns (loss, sse, object_grad, probe_grad, pos_grad, tilts_grad,
-    static_background_grad, aberrations_array_grad, beam_current_grad,
-    constraint_contributions) like the reference; unused gradients are 0 /
-    zero tensors as there.
+    this_obj (Ny, Nx, S, K) and full_probe (N, N, M) complex, positions (P, 2)
+    int, corrections (P, 2) and measured_array (P, N, N) float may be torch
+    tensors, numpy arrays (as the reference's CPU path passes them; copied to the
+    device, the measured patterns once while unchanged) or device arrays of
+    another library such as CuPy (viewed through DLPack).  Returns (loss, sse,
+    object_grad, probe_grad, pos_grad, tilts_grad, static_background_grad,
+    aberrations_array_grad, beam_current_grad, constraint_contributions) like
+    the reference, the arrays of this_obj's type; unused gradients are 0 / zero
+    arrays as there.
 
     checkpoint_slices (keyword only, not in the reference): keep the wave
     entering every k-th slice and recompute the others in the adjoint, for
@@ -351,24 +358,23 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
                     ("atv_weight", atv_weight), ("mixed_variance_weight", mixed_variance_weight),
                     ("wedge_mu", wedge_mu), ("probe_mod_reg_weight", probe_mod_reg_weight)):
         _unsupported(name, bool(w))
-    dist = np.asarray(this_distances.cpu() if isinstance(this_distances, torch.Tensor) else this_distances,
-                      dtype=np.float64).ravel()
+    dist = _host(this_distances, np.float64).ravel()
     _unsupported("per-slice distances", dist.shape[0] != 1)
-    tilts = np.asarray(this_tilt_array.cpu() if isinstance(this_tilt_array, torch.Tensor) else this_tilt_array,
-                       dtype=np.float64)
-    tcorr = np.asarray(this_tilts_correction.cpu() if isinstance(this_tilts_correction, torch.Tensor)
-                       else this_tilts_correction, dtype=np.float64)
-    tt = tilts + tcorr
+    tt = _host(this_tilt_array, np.float64) + _host(this_tilts_correction, np.float64)
     _unsupported("per-position tilts", tt.shape[0] != 1)
     _unsupported("tilts before/after the specimen", bool(np.any(tt[0, [0, 1, 4, 5]] != 0)))
 
-    if not isinstance(this_obj, torch.Tensor) or this_obj.device.type != "cuda":
-        raise TypeError("pypty_amd: this_obj must be a torch tensor on a HIP device")
-    device = this_obj.device
-    obj = this_obj.contiguous()
+    # the caller's array type (numpy, torch, or a device array of another library such as CuPy)
+    # is the type of the returned gradients
+    kind, template = _arrays.kind_of(this_obj), this_obj
+    obj = _arrays.to_torch(this_obj, dtype=torch.complex64).contiguous()
+    if obj.device.type != "cuda":
+        raise TypeError("pypty_amd: no HIP device for this_obj")
+    device = obj.device
     ny, nx, S, K = obj.shape
+    full_probe = _arrays.to_torch(full_probe, device, torch.complex64)
     n, _, M = full_probe.shape
-    probe_mnn = full_probe.to(device).permute(2, 0, 1).contiguous()
+    probe_mnn = full_probe.permute(2, 0, 1).contiguous()
     positions = _as_device(this_pos_array, torch.int32, device)
     corrections = _as_device(this_pos_correction, torch.float32, device)
     # load_one_by_one with patterns in host memory streams them per batch (objective.py:383-408)
@@ -377,7 +383,9 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         measured = (measured_array.detach().to(torch.float32).contiguous() if isinstance(measured_array, torch.Tensor)
                     else torch.from_numpy(np.ascontiguousarray(measured_array, dtype=np.float32)))
     else:
-        measured = _as_device(measured_array, torch.float32, device)
+        # host patterns are copied to the device once and reused while unchanged (they are the same
+        # array every epoch)
+        measured = _arrays.stage(measured_array, device, torch.float32).contiguous()
     seq = _as_device(this_chopped_sequence, torch.int32, device)
     # static background (objective.py:359-361, 525-526, 588-589): a scalar acts on every pixel
     bg = bg_grad = None
@@ -445,4 +453,5 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     # objective.py:377: zeros of the background's shape for an array, 0 for a scalar or none
     if bg_grad is None:
         bg_grad = (torch.zeros_like(bg) if bg is not None and not np.isscalar(static_background) else 0)
-    return loss, sse, object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, 0, 0, constraint_contributions
+    out = [_arrays.like(t, kind, template) for t in (object_grad, probe_grad, pos_grad, tilts_grad, bg_grad)]
+    return (loss, sse, out[0], out[1], out[2], out[3], out[4], 0, 0, constraint_contributions)

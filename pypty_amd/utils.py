@@ -12,7 +12,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import _lib
+from . import _arrays, _lib
 
 
 def _planar(array: torch.Tensor):
@@ -43,14 +43,20 @@ def fourier_clean_planar(planes: torch.Tensor, cutoff: float, rolloff: float = 0
 def fourier_clean(array, cutoff=0.66, mask=None, rolloff=0, default_float=None, xp=None):
     """Band-limit over axes (0, 1) (pypty/utils.py:102-150): ifft2_ishift(mask * shift_fft2(array)).
 
-    ``array`` is a complex64 torch tensor (Ny, Nx[, ...]) on a HIP device;
+    ``array`` (Ny, Nx[, ...]) is a complex64 torch tensor on a HIP device, a
+    numpy array or another library's device array (returned as the same type);
     ``mask`` (an explicit mask instead of the cutoff circle) is not on the
-    accelerated path.  Returns a new tensor like the reference.
+    accelerated path.  Returns a new array like the reference.
     """
     if cutoff is None and mask is None:
         return array
     if mask is not None:
         raise NotImplementedError("pypty_amd.fourier_clean: explicit masks are not on the accelerated path")
+    kind = _arrays.kind_of(array)
+    if kind != _arrays.TORCH:
+        # numpy (copied to the device) or another library's device array (DLPack), returned as such
+        out = fourier_clean(_arrays.to_torch(array, dtype=torch.complex64), cutoff, None, rolloff)
+        return _arrays.like(out, kind, array)
     flat, back = _planar(array)
     if flat.data_ptr() == array.data_ptr():
         flat = flat.clone()
@@ -85,7 +91,13 @@ def create_spatial_frequencies(px, py, shape, damping_cutoff_multislice, smooth_
 
 
 def complex_grad_to_phase_grad(grad, array, xp=None):
-    """2 Re(-i conj(array) grad) (pypty/utils.py:260-276)."""
+    """2 Re(-i conj(array) grad) (pypty/utils.py:260-276); numpy or another library's device arrays
+    come back as the same type."""
+    kind = _arrays.kind_of(grad)
+    if kind != _arrays.TORCH:
+        out = complex_grad_to_phase_grad(_arrays.to_torch(grad, dtype=torch.complex64).contiguous(),
+                                         _arrays.to_torch(array, dtype=torch.complex64).contiguous())
+        return _arrays.like(out, kind, grad)
     for t, name in ((grad, "grad"), (array, "array")):
         _lib.require_device_tensor(t, name)
     if grad.shape != array.shape or grad.dtype != torch.complex64 or array.dtype != torch.complex64:
