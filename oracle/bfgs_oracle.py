@@ -70,6 +70,18 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
                 reg, gl = orc.full_l1_constraint(o, wa, wp)
                 loss += reg
                 og = (og + gl).astype(og.dtype)
+        # further constraint terms (objective.py:765-921): data["constraints"](obj, probe) returns
+        # (term, object gradient or None, probe gradient or None); the probe gradient is band-limited
+        # with the pass's (objective.py:927-933)
+        if data.get("constraints") is not None:
+            reg, go, gp = data["constraints"](o, p)
+            loss += reg
+            if go is not None:
+                og = (og + go).astype(og.dtype)
+            if gp is not None and update_probe:
+                _, _, _, _, m = orc.create_spatial_frequencies(data["px"], data["px"], p.shape[0], cutoff, 0.0,
+                                                                 np.float32)
+                pg = (pg + orc.ifft2(orc.fft2(gp) * m[:, :, None])).astype(pg.dtype)
         if update_probe:
             pg = clean(pg)
         if update_obj:

@@ -108,7 +108,6 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     _unsupported("tune_only_probe_abs", bool(tune_only_probe_abs))
     _unsupported("do_charge_flip", bool(do_charge_flip))
     _unsupported("keep_probe_states_orthogonal", bool(keep_probe_states_orthogonal) and probe.shape[-1] > 1)
-    _unsupported("probe_reg_weight=inf", probe_reg_weight == np.inf)
     _unsupported("gaussian filters", bool(current_gaussian_filter) or bool(current_apply_gaussian_filter_amplitude))
     _unsupported("dynamically_resize_yx_object", bool(dynamically_resize_yx_object) and bool(this_step_pos_correction))
     _unsupported("pos_puzzling", bool(pos_puzzling) and bool(this_step_pos_correction))
@@ -132,6 +131,18 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     history_bfgs.setdefault("gamma_static", 1)
     cut = damping_cutoff_multislice - update_extra_cut
     clean = not lazy_clean
+    # an infinite probe_reg_weight band-limits the probe and its gradient to the aperture instead of
+    # penalising what lies outside it (iterative.py:490-496, 526-530)
+    clean_grad_ap = probe_reg_weight is not None and np.isinf(float(probe_reg_weight))
+
+    def _clean_aperture(x):
+        if isinstance(aperture_mask, (np.ndarray, torch.Tensor)):
+            return utils.fourier_clean(x, mask=aperture_mask)
+        return utils.fourier_clean(x, cutoff=aperture_mask)
+
+    if clean_grad_ap:
+        probe_reg_weight = 0
+        probe = _clean_aperture(probe)
 
     def evaluate(o, p, pc, b):
         out = objective.loss_and_direction(
@@ -143,12 +154,15 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             aberration_marker, probe_marker, None, compute_batch, phase_only_obj, None, False, False, default_float,
             default_complex, xp, is_first_epoch, scan_size, scan_step_A, fast_axis_reg_weight_positions,
             slow_axis_reg_weight_positions, slow_axis_reg_weight_tilts, current_deformation_reg_weight_positions,
-            current_deformation_reg_weight_tilts, fast_axis_reg_weight_tilts, aperture_mask, 0, current_window_weight,
+            current_deformation_reg_weight_tilts, fast_axis_reg_weight_tilts, aperture_mask, probe_reg_weight,
+            current_window_weight,
             current_window, phase_norm_weight, abs_norm_weight, atv_weight, atv_q, atv_p, mixed_variance_weight,
             mixed_variance_sigma, wedge_mu, beta_wedge, probe_mod_reg_weight, probe_mod_tol, smart_memory,
             print_flag, data_simulation_flag, data_simulation_exitwave, use_padded_intensity,
             checkpoint_slices=checkpoint_slices)
         loss, sse, og, pg, posg = out[0], out[1], out[2], out[3], out[4]
+        if clean_grad_ap and update_probe:
+            pg = _clean_aperture(pg)
         if update_probe:
             pg = _clean(pg, cut, smooth_rolloff)
         if clean and update_obj:
@@ -554,10 +568,23 @@ def run(pypty_params):
             lw = 1 / dataset.shape[0]
         w_phase = _per_epoch(get("phase_norm_weight", 0), epoch)
         w_abs = _per_epoch(get("abs_norm_weight", 0), epoch)
+        # the constraint weights (pypty/iterative.py:112, 124-126, 188-210), values or callables of the epoch
+        cw = {k: _per_epoch(get(k, d), epoch) for k, d in (
+            ("fast_axis_reg_weight_positions", 0), ("slow_axis_reg_weight_positions", 0),
+            ("deformation_reg_weight_positions", 0), ("probe_reg_constraint_weight", 0), ("window_weight", 0),
+            ("atv_weight", 0), ("mixed_variance_weight", 0), ("mixed_variance_sigma", 0.5), ("wedge_mu", 0),
+            ("beta_wedge", 0), ("probe_mod_reg_weight", 0), ("probe_mod_tol", 0.05))}
+        window = get("window", None)
+        this_window = None
+        if cw["window_weight"] > 0 and window is not None:
+            # a pair (r0, r_max) is a tapered window, else the window itself (iterative.py:410-416)
+            this_window = (utils.get_window(n, window[0], window[1]) if len(window) == 2
+                           else np.asarray(window, np.float32))
         # the history is reset when the objective changes: a different set of refined variables or of
         # non-zero constraint weights (pypty/iterative.py:383-389; of its weights run() takes these)
         steps = sum(int(x != 0) for x in (s_obj, s_probe, s_pos, s_bg))
-        constraints = sum(int(x != 0) for x in (lw, w_phase, w_abs))
+        constraints = sum(int(x != 0) for x in (lw, w_phase, w_abs, *[cw[k] for k in cw if k not in (
+            "mixed_variance_sigma", "beta_wedge", "probe_mod_tol")], cw["mixed_variance_sigma"]))
         if epoch != int(get("epoch_prev", 0)) and (steps != prev_steps or constraints != prev_constraints):
             reset_bfgs_history()
             warnings += "\nResetting the history!"
@@ -569,31 +596,34 @@ def run(pypty_params):
             this_step_pos_correction=s_pos, this_step_tilts=0, measured_array=measured, this_wavelength=wavelength,
             masks=None, pixel_size_x_A=px, pixel_size_y_A=py,
             phase_norm_weight=w_phase, abs_norm_weight=w_abs,
-            stepsize_threshold_low=get("min_step", 1e-20), probe_reg_weight=0, aperture_mask=None,
+            stepsize_threshold_low=get("min_step", 1e-20), probe_reg_weight=cw["probe_reg_constraint_weight"], aperture_mask=get("aperture_mask", None),
             recon_type="far_field", defocus_array=np.array([0.0]), Cs=0.0, alpha_near_field=0.0,
             damping_cutoff_multislice=cutoff, smooth_rolloff=rolloff,
             update_extra_cut=np.float32(get("update_extra_cut", 0.005)), keep_probe_states_orthogonal=False,
             do_charge_flip=False, cf_delta_phase=0.1, cf_delta_abs=0.01, cf_beta_phase=-0.95, cf_beta_abs=-0.95,
-            phase_only_obj=False, wedge_mu=0, beta_wedge=0,
+            phase_only_obj=False, wedge_mu=cw["wedge_mu"], beta_wedge=cw["beta_wedge"],
             wolfe_c1_constant=_per_epoch(get("wolfe_c1_constant", 0.1), epoch),
-            wolfe_c2_constant=_per_epoch(get("wolfe_c2_constant", 0.9), epoch), atv_weight=0, atv_q=1, atv_p=2,
+            wolfe_c2_constant=_per_epoch(get("wolfe_c2_constant", 0.9), epoch), atv_weight=cw["atv_weight"], atv_q=get("atv_q", 1), atv_p=get("atv_p", 2),
             tune_only_probe_phase=False, propmethod="multislice", this_chopped_sequence=seq, load_one_by_one=False,
             data_multiplier=1, data_pad=0, phase_plate_in_h5=None, print_flag=print_flag, this_loss_weight=lw,
             max_count=get("max_count", 20), reduce_factor=np.float32(get("reduce_factor", 0.1)),
-            optimism=get("optimism", 3), mixed_variance_weight=0, mixed_variance_sigma=0.5, data_bin=1,
+            optimism=get("optimism", 3), mixed_variance_weight=cw["mixed_variance_weight"],
+            mixed_variance_sigma=cw["mixed_variance_sigma"], data_bin=1,
             data_shift_vector=[0, 0], smart_memory=False, default_float=np.float32, default_complex=np.complex64,
             default_int=np.int32, upsample_pattern=1,
             this_step_static_background=s_bg, tilt_mode=0,
             fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=None, this_step_aberrations_array=0,
-            probe_marker=None, compute_batch=get("compute_batch", 256), current_window=None,
-            current_window_weight=0, dynamically_resize_yx_object=False, lazy_clean=False,
+            probe_marker=None, compute_batch=get("compute_batch", 256), current_window=this_window,
+            current_window_weight=cw["window_weight"], dynamically_resize_yx_object=False, lazy_clean=False,
             current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=0,
             xp=None, remove_fft_cache=False, is_first_epoch=is_first,
             hist_length=_per_epoch(get("hist_length", 10), epoch), actual_step=first_step,
-            fast_axis_reg_weight_positions=0, fast_axis_reg_weight_tilts=0, slow_axis_reg_weight_positions=0,
-            slow_axis_reg_weight_tilts=0, scan_size=None, scan_step_A=1.0, current_deformation_reg_weight_positions=0,
+            fast_axis_reg_weight_positions=cw["fast_axis_reg_weight_positions"], fast_axis_reg_weight_tilts=0,
+            slow_axis_reg_weight_positions=cw["slow_axis_reg_weight_positions"],
+            slow_axis_reg_weight_tilts=0, scan_size=get("scan_size", None), scan_step_A=get("scan_step_A", 1),
+            current_deformation_reg_weight_positions=cw["deformation_reg_weight_positions"],
             current_deformation_reg_weight_tilts=0, warnings=warnings, data_simulation_flag=False,
-            data_simulation_exitwave=False, pos_puzzling=False, probe_mod_reg_weight=0, probe_mod_tol=0.05,
+            data_simulation_exitwave=False, pos_puzzling=False, probe_mod_reg_weight=cw["probe_mod_reg_weight"], probe_mod_tol=cw["probe_mod_tol"],
             use_padded_intensity=True, counter_failed_epochs=counter_failed,
             checkpoint_slices=int(get("checkpoint_slices", 0)))
         total_loss, sse, cc, actual_step, count, d_value, new_d_value, warnings, counter_failed = out

@@ -31,7 +31,9 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import _arrays, _lib
+from . import _arrays, _lib, constraints
+from . import fft as pfft
+from . import utils
 
 LOSSES = {"lsq_sqrt": _lib.PTY_LOSS_LSQ_SQRT, "ml": _lib.PTY_LOSS_ML, "lsq": _lib.PTY_LOSS_LSQ,
           "lsq_sqrt_2": _lib.PTY_LOSS_LSQ_SQRT_2, "lsq_compressed": _lib.PTY_LOSS_LSQ_COMPRESSED,
@@ -352,12 +354,11 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     _unsupported("data_pad without use_padded_intensity", int(data_pad) != 0 and not use_padded_intensity)
     _unsupported("data_simulation_flag", bool(data_simulation_flag))
     _unsupported("smooth_rolloff", float(smooth_rolloff) != 0.0)
-    for name, w in (("fast_axis_reg_weight_positions", fast_axis_reg_weight_positions),
-                    ("slow_axis_reg_weight_positions", slow_axis_reg_weight_positions),
-                    ("probe_reg_weight", probe_reg_weight), ("current_window_weight", current_window_weight),
-                    ("atv_weight", atv_weight), ("mixed_variance_weight", mixed_variance_weight),
-                    ("wedge_mu", wedge_mu), ("probe_mod_reg_weight", probe_mod_reg_weight)):
-        _unsupported(name, bool(w))
+    _unsupported("probe_reg_weight=inf", probe_reg_weight is not None and np.isinf(float(probe_reg_weight)))
+    pos_weights = (fast_axis_reg_weight_positions, current_deformation_reg_weight_positions,
+                   slow_axis_reg_weight_positions)
+    if this_step_pos_correction and any(w != 0 for w in pos_weights) and scan_size is None:
+        raise ValueError("pypty_amd.loss_and_direction: the scan-grid constraints on the positions need scan_size")
     dist = _host(this_distances, np.float64).ravel()
     _unsupported("per-slice distances", dist.shape[0] != 1)
     tt = _host(this_tilt_array, np.float64) + _host(this_tilts_correction, np.float64)
@@ -441,15 +442,63 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe
                   else torch.zeros_like(full_probe))
     tilts_grad = torch.zeros(tt.shape, dtype=torch.float32, device=device)
-    # constraints (objective.py:742-921): the L1 norms of the object's phase and log-amplitude are the
-    # fifth and sixth of the reference's fourteen contributions; they add to the loss (after the loss
-    # weight) and to the object gradient
-    constraint_contributions = np.zeros(14)
-    for k, (wa, wp) in ((4, (0.0, phase_norm_weight)), (5, (abs_norm_weight, 0.0))):
-        if wa != 0 or wp != 0:
-            reg, _ = compute_full_l1_constraint(obj, wa, wp, None, True, smart_memory, out_grad=object_grad)
-            loss += reg
-            constraint_contributions[k] = reg
+    # constraints (objective.py:765-921), the reference's fourteen contributions in its order; each
+    # adds to the loss (after the loss weight) and to the gradient of what it constrains.  The probe
+    # ones are band-limited with the probe gradient (objective.py:927-933), the tilt ones need tilt
+    # refinement, which is outside this path.
+    cc = np.zeros(14)
+    pos_abs = None
+    if this_step_pos_correction and any(w != 0 for w in pos_weights):
+        pos_abs = positions.to(torch.float64) + corrections.to(torch.float64)
+    pos_terms = ((0, constraints.compute_fast_axis_constraint_on_grid, fast_axis_reg_weight_positions),
+                 (1, constraints.compute_deformation_constraint_on_grid, current_deformation_reg_weight_positions),
+                 (11, constraints.compute_slow_axis_constraint_on_grid, slow_axis_reg_weight_positions))
+    probe_reg = []
+    for k in range(14):
+        term, grad = 0.0, None
+        if k in (0, 1, 11):
+            fn, w = next((f, w) for kk, f, w in pos_terms if kk == k)
+            if pos_abs is not None and w != 0:
+                term, grad = fn(pos_abs, scan_size, w)
+                pos_grad += grad.to(pos_grad.dtype)
+        elif k in (4, 5):
+            wa, wp = (0.0, phase_norm_weight) if k == 4 else (abs_norm_weight, 0.0)
+            if wa != 0 or wp != 0:
+                term, _ = compute_full_l1_constraint(obj, wa, wp, None, True, smart_memory, out_grad=object_grad)
+        elif k == 6 and this_step_probe and probe_reg_weight:
+            term, grad = constraints.compute_probe_constraint(full_probe, aperture_mask, probe_reg_weight, True)
+            probe_reg.append(grad)
+        elif k == 7 and this_step_probe and current_window_weight:
+            term, grad = constraints.compute_window_constraint(full_probe, current_window, current_window_weight)
+            probe_reg.append(grad)
+        elif k == 8 and atv_weight:
+            term, grad = constraints.compute_atv_constraint(obj, atv_weight, atv_q, atv_p, pixel_size_x_A,
+                                                            pixel_size_y_A, None, True, smart_memory)
+            object_grad += grad
+        elif k == 9 and mixed_variance_weight and K > 1:
+            term, grad = constraints.compute_mixed_object_variance_constraint(obj, mixed_variance_weight,
+                                                                              mixed_variance_sigma, True, smart_memory)
+            object_grad += grad
+        elif k == 10 and wedge_mu:
+            term, grad = constraints.compute_missing_wedge_constraint(obj, pixel_size_x_A, pixel_size_y_A,
+                                                                      float(dist[0]), beta_wedge, wedge_mu)
+            object_grad += grad
+        elif k == 13 and this_step_probe and probe_mod_reg_weight:
+            term, grad = constraints.probe_modulation_constraint(full_probe, scan_step_A, pixel_size_x_A,
+                                                                 pixel_size_y_A, probe_mod_tol, probe_mod_reg_weight,
+                                                                 cutoff)
+            probe_reg.append(grad)
+        loss += term
+        cc[k] = term
+    if probe_reg:
+        # the probe constraints enter before the probe gradient's band limit, which is linear
+        mask_i = utils.create_spatial_frequencies(float(pixel_size_x_A), float(pixel_size_y_A), n, cutoff, 0.0)[4]
+        total = probe_reg[0].to(torch.complex64)
+        for g in probe_reg[1:]:
+            total = total + g.to(torch.complex64)
+        spec = pfft.fft2(total.contiguous(), axes=(0, 1)) * mask_i[:, :, None]
+        probe_grad += pfft.ifft2(spec.contiguous(), axes=(0, 1))
+    constraint_contributions = cc
     # objective.py:377: zeros of the background's shape for an array, 0 for a scalar or none
     if bg_grad is None:
         bg_grad = (torch.zeros_like(bg) if bg is not None and not np.isscalar(static_background) else 0)

@@ -45,18 +45,24 @@ def fourier_clean(array, cutoff=0.66, mask=None, rolloff=0, default_float=None, 
 
     ``array`` (Ny, Nx[, ...]) is a complex64 torch tensor on a HIP device, a
     numpy array or another library's device array (returned as the same type);
-    ``mask`` (an explicit mask instead of the cutoff circle) is not on the
-    accelerated path.  Returns a new array like the reference.
+    ``mask`` (DC-centred, as there) replaces the cutoff circle.  Returns a new
+    array like the reference.
     """
     if cutoff is None and mask is None:
         return array
-    if mask is not None:
-        raise NotImplementedError("pypty_amd.fourier_clean: explicit masks are not on the accelerated path")
     kind = _arrays.kind_of(array)
     if kind != _arrays.TORCH:
         # numpy (copied to the device) or another library's device array (DLPack), returned as such
-        out = fourier_clean(_arrays.to_torch(array, dtype=torch.complex64), cutoff, None, rolloff)
+        out = fourier_clean(_arrays.to_torch(array, dtype=torch.complex64), cutoff, mask, rolloff)
         return _arrays.like(out, kind, array)
+    if mask is not None:
+        # an explicit mask in the DC-centred layout (utils.py:142-146)
+        from . import fft as pfft
+
+        m = _arrays.to_torch(mask, array.device, torch.float32)
+        m = m.reshape(m.shape + (1,) * (array.dim() - m.dim()))
+        spec = pfft.shift_fft2(array.contiguous(), axes=(0, 1)) * m
+        return pfft.ifft2_ishift(spec.contiguous(), axes=(0, 1))
     flat, back = _planar(array)
     if flat.data_ptr() == array.data_ptr():
         flat = flat.clone()
@@ -88,6 +94,25 @@ def create_spatial_frequencies(px, py, shape, damping_cutoff_multislice, smooth_
     dev = torch.device("cuda", torch.cuda.current_device())
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
     return t(q2), t(qx), t(qy), t(np.fft.fftshift(m)), t(m)
+
+
+def get_window(shape, r0, r_max, inverted=True, xp=None):
+    """Cosine-tapered square window (pypty/utils.py:656-690), host setup: 1 inside |x|, |y| <= r0
+    (radii relative to the half-width), falling to 0 at r_max; inverted (the default) gives 1 - that,
+    the region the real-space probe constraint penalises."""
+    x = np.arange(shape, dtype=np.float64)
+    x = x - np.mean(x)
+    r_max = r_max * np.max(x)
+    r0 = r0 * np.max(x)
+    x, y = np.meshgrid(x, x, indexing="xy")
+    mask = np.ones_like(x)
+    if r0 < r_max:
+        for a in (np.abs(x), np.abs(y)):
+            sel = a > r0
+            mask[sel] *= (0.5 * (1 + np.cos(3.141592654 * (a - r0) / (r_max - r0))))[sel]
+    mask[np.abs(x) > r_max] = 0
+    mask[np.abs(y) > r_max] = 0
+    return 1 - mask if inverted else mask
 
 
 def complex_grad_to_phase_grad(grad, array, xp=None):

@@ -239,3 +239,55 @@ def test_probe_swapped_for_background_without_a_history_reset_matches_oracle():
                 assert isinstance(g, torch.Tensor) and float(g.abs().max()) == 0.0
     finally:
         iterative.static_background = None
+
+
+def test_epochs_with_object_and_probe_constraints_match_oracle():
+    """ATV and missing-wedge terms on the object, the aperture and window terms on the probe
+    (objective.py:850-905), in every loss evaluation of two L-BFGS epochs."""
+    import torch
+
+    from oracle import constraints_oracle as co
+    from pypty_amd import iterative
+
+    print("[bfgs] 2 epochs with object and probe constraints", flush=True)
+    pr = make_problem(n=32, S=3, K=1, M=2, ny=64, nx=64, P=10, shift=True, seed=39, step_pos=False, noise=0.02)
+    rng = np.random.default_rng(9)
+    pr["probe"] = (pr["probe"] + 0.05 * np.abs(pr["probe"]).mean() * (
+        rng.standard_normal(pr["probe"].shape) + 1j * rng.standard_normal(pr["probe"].shape))).astype(np.complex64)
+    window = rng.uniform(0, 1, (32, 32)).astype(np.float32)
+    dev = torch.device("cuda", 0)
+
+    def constraints(o, p):
+        t1, g1 = co.atv_constraint(o, 0.5, 1, 2)
+        t2, g2 = co.missing_wedge_constraint(o, pr["px"], pr["px"], pr["dz"], 2.0, 0.2)
+        t3, g3 = co.probe_constraint(p, 0.35, 20.0)
+        t4, g4 = co.window_constraint(p, window, 0.05)
+        return t1 + t2 + t3 + t4, g1 + g2, g3 + g4
+
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5, "constraints": constraints}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    for epoch in range(2):
+        np.random.seed(600 + epoch)
+        ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, actual_step=step_o)
+        np.random.seed(600 + epoch)
+        rg = call_loss_and_direction(iterative.bfgs_update, **bfgs_kwargs(
+            pr, actual_step=step_g, is_first_epoch=(epoch == 0), atv_weight=0.5, atv_q=1, atv_p=2, wedge_mu=0.2,
+            beta_wedge=2.0, probe_reg_weight=20.0, aperture_mask=0.35, current_window_weight=0.05,
+            current_window=window))
+        step_o, step_g = ro[2], rg[3]
+        print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; count {rg[4]}/{ro[3]}", flush=True)
+        assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+        assert rg[4] == ro[3]
+        assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+        assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
+        assert float(rg[2][8]) > 0 and float(rg[2][6]) > 0  # the ATV and aperture contributions
