@@ -146,6 +146,10 @@ typedef struct pty_problem {
    * (objective.py:588-589, this_step_static_background); needs
    * pty_buffers.static_background and static_background_grad. */
   int step_static_background;
+  /* 1: also return the gradient of the beam current (objective.py:688-698,
+   * this_beam_current_step); needs pty_buffers.beam_current and
+   * beam_current_grad. */
+  int step_beam_current;
 } pty_problem;
 
 typedef struct pty_buffers {
@@ -178,6 +182,15 @@ typedef struct pty_buffers {
    * the positions, with dL/dI as the reference forms it (carrying the
    * 1/(M*K) of the mode mean when there is more than one mode). */
   float* static_background_grad;
+  /* (n_positions) beam current or NULL (objective.py:446-450): the probe of
+   * position i, before its sub-pixel shift, is |beam_current[i]| times the
+   * probe. */
+  const float* beam_current;
+  /* (n_positions) out: rows of `sequence` assigned 2 sign(|bc|) times the
+   * sum of Re(dL/dP* conj(P)) over the window and the probe modes, with P
+   * the unscaled probe (objective.py:694-696; sign(|bc|) is 1 unless the
+   * current is 0); the other rows zero unless accumulate. */
+  float* beam_current_grad;
 } pty_buffers;
 
 int pty_workspace_bytes(const pty_problem* p, size_t* bytes);

@@ -167,7 +167,7 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
                        pixel_size_x, pixel_size_y, algorithm=LSQ_SQRT, loss_weight=1.0, cutoff=2 / 3,
                        edge=1.0, step_obj=True, step_probe=True, step_pos=False, compute_batch=64,
                        cdtype=np.complex64, return_patterns=False, masks=None, static_background=None,
-                       step_static_background=False):
+                       step_static_background=False, beam_current=None, step_beam_current=False):
     """objective.loss_and_direction restricted to the far-field multislice path.
 
     Follows pypty/objective.py:218-361 (grids, propagator, shift kernels),
@@ -186,6 +186,14 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     with dL/dI after the 1/(M*K) of :586-587, not band-limited); when it is
     given the gradient (zeros unless step_static_background) is appended to
     the result.
+    beam_current (P,) follows objective.py:446-450 (each position's probe,
+    before its shift, times |beam_current|) and :688-698 (gradient
+    2 sign(|bc|) sum Re(dL/dP* conj(P)) of the unscaled probe, then the probe
+    gradient times |bc|); when it is given its gradient (P,) (zeros unless
+    step_beam_current, rows outside the sequence zero) is appended last.  The
+    reference's one-mode branch conjugates the caller's probe in place
+    (objective.py:692) and so feeds the conjugate to the next batch; this
+    restatement keeps the probe as given, as its many-mode branch does.
     """
     fd = _fdt(cdtype)
     obj = np.asarray(obj, dtype=cdtype)
@@ -210,6 +218,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     bg = None if static_background is None else np.asarray(static_background, dtype=fd)
     bg_square = None if bg is None else (bg[None] ** 2) * mask_s[None]
     bg_grad = None if bg is None else np.zeros((N, N), dtype=fd)
+    bc = None if beam_current is None else np.asarray(beam_current, dtype=fd)
+    bc_grad = None if bc is None else np.zeros_like(bc)
     FP = fft2(probe[None], axes=(1, 2))
     loss, sse = fd(0), fd(0)
     seq = np.asarray(sequence)
@@ -221,11 +231,16 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
         chop = obj[my, mx, :, :]
         chop = ifft2(fft2(chop, axes=(1, 2)) * mask[None, :, :, None, None], axes=(1, 2))
         this_probe = probe[None]
+        FPb = FP
+        if bc is not None:
+            thisbc = np.abs(bc[tcs])
+            this_probe = this_probe * thisbc[:, None, None, None]
+            FPb = FP * thisbc[:, None, None, None]
         E = None
         if shift_flag:
             corr = corrections[tcs]
             E = np.exp(shx[None] * corr[:, 1, None, None] + shy[None] * corr[:, 0, None, None]) * mask[None]
-            this_probe = ifft2(FP * E[..., None], axes=(1, 2))
+            this_probe = ifft2(FPb * E[..., None], axes=(1, 2))
         if this_probe.shape[0] == 1 and len(tcs) > 1:
             this_probe = np.repeat(this_probe, len(tcs), axis=0)
         waves, exitw = multislice(this_probe, chop, H, edge)
@@ -304,10 +319,14 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
             Fg = fft2(interm, axes=(1, 2)) * np.conj(E)[..., None]
             interm = ifft2(Fg, axes=(1, 2))
             if step_pos:
-                smg = np.conj(FP) * Fg
+                smg = np.conj(FPb) * Fg
                 sh = -2 / (N * N)
                 yx = np.stack((shy, shx), axis=0)  # (2, N, N)
                 pos_grad[tcs] = sh * np.sum(np.real(yx[None, :, :, :, None] * smg[:, None]), axis=(2, 3, 4))
+        if bc is not None:
+            if step_beam_current:
+                bc_grad[tcs] = 2 * np.sign(thisbc) * np.sum(np.real(interm * np.conj(probe[None])), axis=(1, 2, 3))
+            interm = interm * thisbc[:, None, None, None]
         if step_probe:
             probe_grad += np.sum(interm, axis=0)
     loss = loss * loss_weight
@@ -320,6 +339,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
         out = out + (patterns,)
     if bg is not None:
         out = out + (bg_grad,)
+    if bc is not None:
+        out = out + (bc_grad,)
     return out
 
 

@@ -47,7 +47,7 @@ class PtyProblem(ctypes.Structure):
         ("data_shift_y", ctypes.c_int), ("data_shift_x", ctypes.c_int), ("data_bin", ctypes.c_int),
         ("data_pad", ctypes.c_int), ("data_multiplier", ctypes.c_float),
         ("checkpoint_slices", ctypes.c_int), ("lanes", ctypes.c_int),
-        ("step_static_background", ctypes.c_int),
+        ("step_static_background", ctypes.c_int), ("step_beam_current", ctypes.c_int),
     ]
 
 
@@ -60,6 +60,7 @@ class PtyBuffers(ctypes.Structure):
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("masks", ctypes.c_void_p),
         ("measured_order", ctypes.c_int), ("accumulate", ctypes.c_int),
         ("static_background", ctypes.c_void_p), ("static_background_grad", ctypes.c_void_p),
+        ("beam_current", ctypes.c_void_p), ("beam_current_grad", ctypes.c_void_p),
     ]
 
 

@@ -361,6 +361,11 @@ struct BatchArgs {
   // gradient, summed over the pass, or null when not wanted
   const float* bg;
   float* bg_grad;
+  // beam current (P) scaling each position's probe by |bc| (objective.py:446-450), or null;
+  // with bc_grad the per-window partials of its gradient (B, M) and the gradient (P)
+  const float* bc;
+  double* bc_part;
+  float* bc_grad;
   // slice range [s_lo, s_hi) of a forward, adjoint or objgrad launch (a whole
   // pass: 0, S) and the stash blocks of st_slots slots per (window, mode).
   // With r = s - s_lo the forward stores slice s in slot r / st_every when
@@ -390,6 +395,17 @@ __device__ __forceinline__ int stash_slot(const BatchArgs& a, int s) {
   const int r = s - a.s_lo;
   if (r >= a.st_dense) return a.st_dense / a.st_every + (r - a.st_dense);
   return r % a.st_every == 0 ? r / a.st_every : -1;
+}
+
+// |beam current| of position idx, the factor of its probe (1 without a beam current)
+__device__ __forceinline__ float bc_scale(const BatchArgs& a, int idx) {
+  return (a.bc && idx >= 0) ? fabsf(a.bc[idx]) : 1.f;
+}
+
+// 2 sign(|bc|) of position idx (objective.py:694-696): the factor of its
+// gradient sum Re(dL/dP* conj(P)); 0 for a zero current
+__device__ __forceinline__ double bc_factor(const BatchArgs& a, int idx) {
+  return (idx >= 0 && a.bc[idx] != 0.f) ? 2.0 : 0.0;
 }
 
 // measured row of batch slot b (sequence entry b0 + b, position idx)
@@ -944,8 +960,10 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) v[n] = I[n];
   } else {
+    // the shifted probes carry the beam current already
+    const float bcs = a.shift ? 1.f : bc_scale(a, seq_index(a.seq, a.b0 + b, a.P));
 #pragma unroll
-    for (int n = 0; n < R::R1; ++n) v[n] = src[id.row * N + id.rj + R::T * n];
+    for (int n = 0; n < R::R1; ++n) v[n] = cscale(src[id.row * N + id.rj + R::T * n], bcs);
   }
   const size_t ostride = size_t(a.K) * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
@@ -1185,7 +1203,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
   __syncthreads();
   const RView<N, float2> fp(a.fprobe + size_t(m) * N * N, tid);
   const RView<N, float> mask(a.maskc, tid);
-  const float s = 1.0f / float(N * N);
+  const float s = bc_scale(a, idx) / float(N * N);
   float2 v[R::R1];
   // E, and so the spectrum, is zero on the columns the band mask zeroes
   if (wave_cols_zero<N>(a.zcol, id.col)) {
@@ -1244,17 +1262,30 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
 #pragma unroll
       for (int n = 0; n < R::R1; ++n) v[n] = cadd(v[n], gk[n]);
     }
+    const float bcs = bc_scale(a, idx);
     if (!a.shift) {
-      // natural order: register n of the row phase is R-order slot n*NT + tid
+      // natural order: register n of the row phase is R-order slot n*NT + tid,
+      // element (row, rj + T*n)
+      double sb = 0.0;
 #pragma unroll
-      for (int n = 0; n < R::R1; ++n) sum[n] = cadd(sum[n], v[n]);
+      for (int n = 0; n < R::R1; ++n) {
+        sum[n] = cadd(sum[n], cscale(v[n], bcs));
+        if (a.bc_part) {
+          const float2 Pn = a.probe[size_t(m) * N * N + id.row * N + id.rj + R::T * n];
+          sb += double(v[n].x) * double(Pn.x) + double(v[n].y) * double(Pn.y);
+        }
+      }
+      if (a.bc_part) {
+        const double SB = block_sum(sb, red);
+        if (tid == 0) a.bc_part[size_t(b) * a.M + m] = bc_factor(a, idx) * SB;
+      }
       continue;
     }
     const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
     __syncthreads();  // the previous window's table and transposes are done
     shift_tables_freq<N>(exy, dy, dx);
     rt_fft2_fwd<N>(v, T, id);  // its first barrier also publishes exy
-    double sy = 0.0, sx = 0.0;
+    double sy = 0.0, sx = 0.0, sb = 0.0;
 #pragma unroll
     for (int i = 0; i < R::R1; ++i) {
       int fy, fx;
@@ -1262,10 +1293,12 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       const float mk = mask[i];
       const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mk);
       const float2 Q = cmulc(v[i], E);
-      sum[i] = cadd(sum[i], Q);
-      if (a.step_pos) {
-        // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
+      sum[i] = cadd(sum[i], cscale(Q, bcs));
+      if (a.step_pos || a.bc_part) {
         const float2 F = fp[i];
+        // Re(Q * conj(FP)): the beam current's sum, Parseval's form of the real-space one
+        sb += double(Q.x) * double(F.x) + double(Q.y) * double(F.y);
+        // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
         const double zy = double(Q.y) * double(F.x) - double(Q.x) * double(F.y);
         const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
         const float ky = fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk);
@@ -1277,10 +1310,15 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       const double SY = block_sum(sy, red);
       const double SX = block_sum(sx, red);
       if (tid == 0) {
-        const double sh = -2.0 / (double(N) * double(N));
+        // the shifted probe's spectrum is |bc| FP
+        const double sh = -2.0 * double(bcs) / (double(N) * double(N));
         a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
         a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
       }
+    }
+    if (a.bc_part) {
+      const double SB = block_sum(sb, red);
+      if (tid == 0) a.bc_part[size_t(b) * a.M + m] = bc_factor(a, idx) / (double(N) * double(N)) * SB;
     }
   }
 #pragma unroll
@@ -1468,6 +1506,8 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
   const GTLane<N> L;
   const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
+  // the shifted probes carry the beam current already
+  const float bcs = a.shift ? 1.f : bc_scale(a, seq_index(a.seq, a.b0 + b, a.P));
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
   float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
   const float2* fin = a.fin ? a.fin + (size_t(b) * MK + mk) * a.fin_stride : nullptr;
@@ -1497,7 +1537,7 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
       } else if (s == a.s_lo) {
         const float2* p = src + size_t(r * G::LINES + L.ln) * N + L.j;
 #pragma unroll
-        for (int n = 0; n < G::R1; ++n) v[n] = p[8 * n];
+        for (int n = 0; n < G::R1; ++n) v[n] = cscale(p[8 * n], bcs);
       } else {
         gt_load_rowspec<N>(v, sl, L, r);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
@@ -1763,7 +1803,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
   __syncthreads();
   const RView<N, float2, G::NT> fp(a.fprobe + size_t(m) * N * N, tid);
   const RView<N, float, G::NT> mask(a.maskc, tid);
-  const float s = 1.0f / float(N * N);
+  const float s = bc_scale(a, idx) / float(N * N);
   float2 v[G::R1];
   // E, and so the spectrum, is zero on the columns the band mask zeroes
   const unsigned zm = gt_cols_zero<N>(a.zcol, L);
@@ -1818,8 +1858,10 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
   const unsigned zm = gt_cols_zero<N>(a.zcol, L);
   for (int b = g; b < a.B; b += a.G) {
     const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    const float bcs = bc_scale(a, idx);
     const float2* gp = a.gP + (size_t(b) * a.M + m) * a.K * N * N;
     if (!a.shift) {
+      double sb = 0.0;
       for (int r = 0; r < G::ROUNDS; ++r) {
         const size_t row = size_t(r * G::LINES + L.ln) * N + L.j;
 #pragma unroll
@@ -1828,7 +1870,17 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
 #pragma unroll
           for (int n = 0; n < G::R1; ++n) v[n] = cadd(v[n], gp[size_t(k) * N * N + row + 8 * n]);
 #pragma unroll
-        for (int n = 0; n < G::R1; ++n) acc.store(r * G::R1 + n, cadd(acc[r * G::R1 + n], v[n]));
+        for (int n = 0; n < G::R1; ++n) {
+          acc.store(r * G::R1 + n, cadd(acc[r * G::R1 + n], cscale(v[n], bcs)));
+          if (a.bc_part) {
+            const float2 Pn = a.probe[size_t(m) * N * N + row + 8 * n];
+            sb += double(v[n].x) * double(Pn.x) + double(v[n].y) * double(Pn.y);
+          }
+        }
+      }
+      if (a.bc_part) {
+        const double SB = block_sum(sb, red);
+        if (tid == 0) a.bc_part[size_t(b) * a.M + m] = bc_factor(a, idx) * SB;
       }
       continue;
     }
@@ -1846,7 +1898,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
       gt_store_rowspec<N>(v, sl, L, r);
     }
     __syncthreads();  // the slab holds the row spectra; the table is published
-    double sy = 0.0, sx = 0.0;
+    double sy = 0.0, sx = 0.0, sb = 0.0;
     for (int r = 0; r < G::ROUNDS; ++r) {
       if ((zm >> r) & 1u) continue;
       gt_load_col<N>(v, sl, L, r);
@@ -1859,10 +1911,12 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
         const float mk = mask[q];
         const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mk);
         const float2 Q = cmulc(v[i], E);
-        acc.store(q, cadd(acc[q], Q));
-        if (a.step_pos) {
-          // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
+        acc.store(q, cadd(acc[q], cscale(Q, bcs)));
+        if (a.step_pos || a.bc_part) {
           const float2 F = fp[q];
+          // Re(Q * conj(FP)): the beam current's sum, Parseval's form of the real-space one
+          sb += double(Q.x) * double(F.x) + double(Q.y) * double(F.y);
+          // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
           const double zy = double(Q.y) * double(F.x) - double(Q.x) * double(F.y);
           const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
           const float ky = fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk);
@@ -1875,10 +1929,15 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
       const double SY = block_sum(sy, red);
       const double SX = block_sum(sx, red);
       if (tid == 0) {
-        const double sh = -2.0 / (double(N) * double(N));
+        // the shifted probe's spectrum is |bc| FP
+        const double sh = -2.0 * double(bcs) / (double(N) * double(N));
         a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
         a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
       }
+    }
+    if (a.bc_part) {
+      const double SB = block_sum(sb, red);
+      if (tid == 0) a.bc_part[size_t(b) * a.M + m] = bc_factor(a, idx) / (double(N) * double(N)) * SB;
     }
   }
 }
@@ -2046,6 +2105,17 @@ __global__ void k_pos_final(BatchArgs a) {
   a.pos_grad[2 * idx + 1] = float(x);
 }
 
+// beam_current_grad[idx] = sum over probe modes (assigned per window, objective.py:694-696)
+__global__ void k_bc_final(BatchArgs a) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.B) return;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  if (idx < 0) return;
+  double s = 0.0;
+  for (int m = 0; m < a.M; ++m) s += a.bc_part[size_t(b) * a.M + m];
+  a.bc_grad[idx] = float(s);
+}
+
 // sum over the G group partials of the probe gradient, one thread per
 // element (in group order, so the sum is deterministic)
 __global__ __launch_bounds__(256) void k_probe_reduce(const float2* __restrict__ part, int G, size_t per_group,
@@ -2099,12 +2169,12 @@ struct TimedLaunch {
 enum {
   KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
   KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER,
-  KT_RECOMPUTE, KT_PREPROCESS, KT_BG_GRAD, KT_PROBE_REDUCE, KT_COUNT
+  KT_RECOMPUTE, KT_PREPROCESS, KT_BG_GRAD, KT_PROBE_REDUCE, KT_BC_FINAL, KT_COUNT
 };
 const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
                                     "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
                                     "loss_final", "zero", "scatteradd", "gather_grad", "recompute", "preprocess",
-                                    "bg_grad", "probe_reduce"};
+                                    "bg_grad", "probe_reduce", "bc_final"};
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
@@ -2190,12 +2260,13 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 // after lane 0's (a buffer of 0 bytes is not needed by the problem).
 enum {
   WB_H = 0, WB_H0, WB_MASKC, WB_ZCOL, WB_FPROBE, WB_OBJP, WB_GRADP, WB_PROBE_PART, WB_PROBE_SUM, WB_LOSS_PART,
-  WB_OB, WB_PP, WB_STASH, WB_GSTATE, WB_PSI, WB_COEF, WB_GP, WB_MEAS_STAGE, WB_POS_PART, WB_SCRATCH, WB_COUNT
+  WB_OB, WB_PP, WB_STASH, WB_GSTATE, WB_PSI, WB_COEF, WB_GP, WB_MEAS_STAGE, WB_POS_PART, WB_BC_PART, WB_SCRATCH,
+  WB_COUNT
 };
 constexpr int kFirstLaneBuffer = WB_OB;
 const char* kBufNames[WB_COUNT] = {"H", "H0", "maskc", "zcol", "fprobe", "objp", "gradp", "probe_part",
                                    "probe_sum", "loss_part", "Ob", "Pp", "stash", "gstate", "Psi", "coef", "gP",
-                                   "meas_stage", "pos_part", "scratch"};
+                                   "meas_stage", "pos_part", "bc_part", "scratch"};
 struct Layout {
   size_t off[WB_COUNT], bytes[WB_COUNT];  // lane 0's offset for the per-lane buffers
   size_t lane0, lane_bytes, total;
@@ -2295,6 +2366,7 @@ Layout plan_layout(const pty_problem* p) {
   take(WB_GP, size_t(B) * M * K * n2 * 8);
   take(WB_MEAS_STAGE, preprocessing(p) ? size_t(B) * n2 * 4 : 0);
   take(WB_POS_PART, size_t(B) * M * 2 * 8);
+  take(WB_BC_PART, p->step_beam_current ? size_t(B) * M * 8 : 0);
   // windows larger than 128 px: one tile per workgroup of the largest launch
   L.scratch_tiles = p->n > 128 ? int(std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K,
                                                        size_t(G) * M, size_t(M)}))
@@ -2336,6 +2408,7 @@ int validate_buffers(const pty_problem* p, const pty_buffers* b) {
   if (compressed(p->algorithm) && !b->masks) return PTY_ERR_ARG;
   if (b->static_background && compressed(p->algorithm)) return PTY_ERR_ARG;
   if (p->step_static_background && (!b->static_background || !b->static_background_grad)) return PTY_ERR_ARG;
+  if (p->step_beam_current && (!b->beam_current || !b->beam_current_grad)) return PTY_ERR_ARG;
   if (b->measured_order != PTY_MEASURED_BY_POSITION && b->measured_order != PTY_MEASURED_BY_SEQUENCE)
     return PTY_ERR_ARG;
   return PTY_OK;
@@ -2502,7 +2575,7 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
       PTY_TRY(x.launch(KT_OBJGRAD, lane, g, [&](hipStream_t s) { launch_objgrad<N>(g, s); }));
     }
   } else {
-    if (p->step_obj || p->step_probe || p->step_pos)
+    if (p->step_obj || p->step_probe || p->step_pos || p->step_beam_current)
       PTY_TRY(x.launch(KT_ADJOINT, lane, a, [&](hipStream_t s) { launch_adjoint<N>(a, s); }));
     if (p->step_obj) PTY_TRY(x.launch(KT_OBJGRAD, lane, a, [&](hipStream_t s) { launch_objgrad<N>(a, s); }));
   }
@@ -2512,7 +2585,7 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
       const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
       k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
     }));
-  if (p->step_probe || (p->step_pos && a.shift))
+  if (p->step_probe || (p->step_pos && a.shift) || p->step_beam_current)
     PTY_TRY(x.launch(KT_PROBEGRAD, lane, a, [&](hipStream_t s) {
       if constexpr (LAY == LAY_RT)
         k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
@@ -2521,6 +2594,8 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
     }));
   if (p->step_pos && a.shift)
     PTY_TRY(x.launch(KT_POS_FINAL, lane, a, [&](hipStream_t s) { k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
+  if (p->step_beam_current)
+    PTY_TRY(x.launch(KT_BC_FINAL, lane, a, [&](hipStream_t s) { k_bc_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
   if (a.bg_grad)
     PTY_TRY(x.launch(KT_BG_GRAD, lane, a, [&](hipStream_t s) {
       k_bg_grad<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(a);
@@ -2566,6 +2641,9 @@ BatchArgs pass_args(const pty_problem* p, const pty_buffers* bf, const Layout& L
   a.J = compressed(p->algorithm) ? p->n_masks : 0;
   a.bg = bf->static_background;
   a.bg_grad = p->step_static_background ? bf->static_background_grad : nullptr;
+  a.bc = bf->beam_current;
+  a.bc_part = p->step_beam_current ? reinterpret_cast<double*>(at(WB_BC_PART)) : nullptr;
+  a.bc_grad = p->step_beam_current ? bf->beam_current_grad : nullptr;
   whole_pass(a);
   if (L.ck) {
     // the forward keeps the checkpoints and every wave of the last segment
@@ -2610,8 +2688,10 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     }, SUB_TO_PLANAR));
   if (p->step_static_background && !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, bf->static_background_grad, n2 * sizeof(float)));
+  if (p->step_beam_current && !bf->accumulate)
+    PTY_TRY(issue_zero(x, 0, a, bf->beam_current_grad, size_t(p->n_positions) * sizeof(float)));
   if (p->step_obj && !bf->accumulate) PTY_TRY(issue_zero(x, 0, a, a.obj_grad, objelems * 8));
-  if ((p->step_probe || p->step_pos) && !bf->accumulate)
+  if ((p->step_probe || p->step_pos || p->step_beam_current) && !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, a.probe_part, size_t(L.G) * M * n2 * 8));
   // lane l's arguments: its own copies of the per-batch buffers
   BatchArgs al[kMaxLanes];
@@ -2622,6 +2702,7 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     auto off = [d](auto* ptr) { return ptr ? reinterpret_cast<decltype(ptr)>(reinterpret_cast<char*>(ptr) + d) : ptr; };
     y.Ob = off(y.Ob); y.Pp = off(y.Pp); y.stash = off(y.stash); y.gstate = off(y.gstate);
     y.Psi = off(y.Psi); y.coef = off(y.coef); y.gP = off(y.gP); y.pos_part = off(y.pos_part);
+    y.bc_part = off(y.bc_part);
     y.meas_stage = off(y.meas_stage); y.scratch = off(y.scratch);
   }
   if (nl > 1) {
@@ -2785,7 +2866,7 @@ struct Access {
   int buf;           // WB_* or an external buffer (kExt + PB_*)
 };
 enum { PB_OBJ = 0, PB_PROBE, PB_MEASURED, PB_LOSS_SSE, PB_OBJ_GRAD, PB_PROBE_GRAD, PB_POS_GRAD, PB_SIM,
-       PB_BG_GRAD, PB_COUNT };
+       PB_BG_GRAD, PB_BC, PB_BC_GRAD, PB_COUNT };
 constexpr int kExt = 100;
 
 // the bytes of every range an op touches; gt: windows larger than 128 px
@@ -2839,12 +2920,14 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       break;
     case KT_PROBE_SHIFT:
       shared(WB_FPROBE, false); shared(WB_MASKC, false); shared(WB_ZCOL, false);
+      add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
       add(a.Pp, size_t(B) * M * c8, true, WB_PP);
       tiles(B * M);
       break;
     case KT_FORWARD:
     case KT_RECOMPUTE: {
       add(a.Ob, size_t(B) * S * K * c8, false, WB_OB);
+      if (!a.shift) add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
       if (a.shift) add(a.Pp, size_t(B) * M * c8, false, WB_PP);
       shared(WB_H, false); shared(WB_H0, false); shared(WB_MASKC, false); shared(WB_ZCOL, false);
       int last = -1;  // the last slot the forward stores
@@ -2903,7 +2986,14 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       shared(WB_FPROBE, false); shared(WB_MASKC, false);
       add(a.probe_part, size_t(a.G) * M * c8, true, WB_PROBE_PART);
       if (a.step_pos && a.shift) add(a.pos_part, size_t(B) * M * 2 * 8, true, WB_POS_PART);
+      add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
+      add(a.bc_part, size_t(B) * M * 8, true, WB_BC_PART);
+      if (a.bc_part && !a.shift) add(bf->probe, size_t(M) * c8, false, kExt + PB_PROBE);
       tiles(a.G * M);
+      break;
+    case KT_BC_FINAL:
+      add(a.bc_part, size_t(B) * M * 8, false, WB_BC_PART);
+      add(a.bc_grad, size_t(p->n_positions) * 4, true, kExt + PB_BC_GRAD);
       break;
     case KT_POS_FINAL:
       add(a.pos_part, size_t(B) * M * 2 * 8, false, WB_POS_PART);
@@ -2963,6 +3053,10 @@ int plan_check_n(const pty_problem* p, int n_seq, int nl, int calls, int accumul
   if (p->step_static_background) {
     bf.static_background = static_cast<const float*>(fake(44));
     bf.static_background_grad = static_cast<float*>(fake(1 + PB_BG_GRAD));
+  }
+  if (p->step_beam_current) {
+    bf.beam_current = static_cast<const float*>(fake(1 + PB_BC));
+    bf.beam_current_grad = static_cast<float*>(fake(1 + PB_BC_GRAD));
   }
   int rc = validate_buffers(p, &bf);
   if (rc) return rc;

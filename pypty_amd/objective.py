@@ -82,6 +82,8 @@ class PassConfig:
     lanes: int = 4
     # also return the gradient of the static background (objective.py:588-589)
     step_static_background: bool = False
+    # also return the gradient of the beam current (objective.py:688-698)
+    step_beam_current: bool = False
 
     @property
     def compressed(self) -> bool:
@@ -100,7 +102,7 @@ class PassConfig:
             float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier), int(self.n_masks),
             int(self.raw_n), int(self.data_shift_y), int(self.data_shift_x), int(self.data_bin), int(self.data_pad),
             float(self.data_multiplier), int(self.checkpoint_slices), int(self.lanes),
-            int(bool(self.step_static_background)))
+            int(bool(self.step_static_background)), int(bool(self.step_beam_current)))
 
 
 class PtychoPass:
@@ -132,7 +134,8 @@ class PtychoPass:
 
     def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
             pos_grad=None, sim_patterns=None, sync=True, masks=None, measured_by_sequence=False, accumulate=False,
-            loss_sse=None, static_background=None, static_background_grad=None):
+            loss_sse=None, static_background=None, static_background_grad=None, beam_current=None,
+            beam_current_grad=None):
         """Launch the pass.  probe_mnn is mode-major (M, n, n).  Returns (loss, sse) as floats
         when ``sync`` else the device tensor [loss, sse].
 
@@ -141,7 +144,10 @@ class PtychoPass:
         float64 tensor of 2 elements to receive this call's sums (default: the engine's own);
         static_background: (n, n) float32, zero frequency at (n/2, n/2), added squared and band-limited
         to the modelled patterns (objective.py:359-361, 525-526); static_background_grad: (n, n)
-        float32 receiving its gradient when the config steps it (objective.py:588-589)."""
+        float32 receiving its gradient when the config steps it (objective.py:588-589);
+        beam_current: (P,) float32, each position's probe taken |beam_current| times before its shift
+        (objective.py:446-450); beam_current_grad: (P,) float32 receiving its gradient on the rows of
+        the sequence when the config steps it (objective.py:688-698)."""
         c = self.cfg
         for t, name in ((obj, "obj"), (probe_mnn, "probe"), (positions, "positions"), (measured, "measured"),
                         (sequence, "sequence")):
@@ -181,6 +187,13 @@ class PtychoPass:
             raise NotImplementedError("pypty_amd: a static background with the compressed losses")
         if c.step_static_background and (static_background is None or static_background_grad is None):
             raise ValueError("static_background and static_background_grad required")
+        for t, name in ((beam_current, "beam_current"), (beam_current_grad, "beam_current_grad")):
+            if t is not None:
+                _lib.require_device_tensor(t, name)
+                if t.dtype != torch.float32 or tuple(t.shape) != (c.n_positions,) or not t.is_contiguous():
+                    raise ValueError(f"pypty_amd: {name} must be contiguous float32 ({c.n_positions},)")
+        if c.step_beam_current and (beam_current is None or beam_current_grad is None):
+            raise ValueError("beam_current and beam_current_grad required")
         out = self.loss_sse if loss_sse is None else loss_sse
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         bufs = _lib.PtyBuffers(
@@ -189,7 +202,8 @@ class PtychoPass:
             ptr(sim_patterns), ptr(self.workspace), self.workspace_bytes, ptr(masks) if c.compressed else None,
             _lib.PTY_MEASURED_BY_SEQUENCE if measured_by_sequence else _lib.PTY_MEASURED_BY_POSITION,
             int(bool(accumulate)), ptr(static_background),
-            ptr(static_background_grad) if c.step_static_background else None)
+            ptr(static_background_grad) if c.step_static_background else None, ptr(beam_current),
+            ptr(beam_current_grad) if c.step_beam_current else None)
         rc = self.lib.pty_loss_and_direction(ctypes.byref(self.c_problem), ctypes.byref(bufs),
                                               _lib.stream_ptr(self.device))
         _lib.check(rc, "loss_and_direction")
@@ -200,7 +214,7 @@ class PtychoPass:
 
     def run_streamed(self, obj, probe_mnn, positions, corrections, measured_host, sequence_host, *, obj_grad=None,
                      probe_grad=None, pos_grad=None, masks=None, static_background=None,
-                     static_background_grad=None):
+                     static_background_grad=None, beam_current=None, beam_current_grad=None):
         """The same pass with the measured patterns left in host memory (objective.py:383-408,
         load_one_by_one): each batch's rows are gathered into one of two pinned buffers, copied to the
         device on a side stream while the previous batch computes, and the batch runs as an
@@ -242,7 +256,8 @@ class PtychoPass:
             self.run(obj, probe_mnn, positions, corrections, staged[slot][:sub.numel()], sub, obj_grad=obj_grad,
                      probe_grad=probe_grad, pos_grad=pos_grad, sync=False, masks=masks, measured_by_sequence=True,
                      accumulate=k > 0, loss_sse=sums[k], static_background=static_background,
-                     static_background_grad=static_background_grad)
+                     static_background_grad=static_background_grad, beam_current=beam_current,
+                     beam_current_grad=beam_current_grad)
             free[slot].record(main)
             if k + 1 < len(starts):
                 stage(k + 1)  # the host gathers the next batch while this one computes
@@ -347,7 +362,6 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
                  static_background is not None and "compressed" in str(algorithm_type))
     _unsupported("aberration_marker", aberration_marker is not None)
     _unsupported("probe_marker", probe_marker is not None)
-    _unsupported("beam_current", beam_current is not None)
     _unsupported("phase_plate_in_h5", phase_plate_in_h5 is not None)
     _unsupported("phase_only_obj", bool(phase_only_obj))
     _unsupported("upsample_pattern", upsample_pattern != 1)
@@ -399,6 +413,15 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
             _unsupported("static_background not of the window's size", tuple(bg.shape) != (n, n))
         if step_bg:
             bg_grad = torch.empty((n, n), dtype=torch.float32, device=device)
+    # beam current (objective.py:446-450, 688-698): each position's probe times |beam_current|
+    bc = bc_grad = None
+    step_bc = beam_current is not None and bool(this_beam_current_step)
+    if beam_current is not None:
+        bc = _as_device(beam_current, torch.float32, device).reshape(-1).contiguous()
+        if bc.numel() != positions.shape[0]:
+            raise ValueError("pypty_amd.loss_and_direction: beam_current needs one value per position")
+        if step_bc:
+            bc_grad = torch.empty_like(bc)
     compressed = "compressed" in algorithm_type
     masks_dev = None
     if compressed:
@@ -424,7 +447,7 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         data_shift_y=shift_y, data_shift_x=shift_x, data_bin=1 if compressed else int(data_bin), data_pad=pad,
         data_multiplier=1.0 if compressed else float(data_multiplier), checkpoint_slices=int(checkpoint_slices),
         lanes=1 if stream else 4,  # a streamed pass runs one batch per call
-        step_static_background=step_bg)
+        step_static_background=step_bg, step_beam_current=step_bc)
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None
@@ -432,11 +455,13 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     if stream:
         loss, sse = engine.run_streamed(obj, probe_mnn, positions, corrections, measured, seq.cpu().numpy(),
                                         obj_grad=object_grad, probe_grad=probe_grad_mnn, pos_grad=pos_grad,
-                                        masks=masks_dev, static_background=bg, static_background_grad=bg_grad)
+                                        masks=masks_dev, static_background=bg, static_background_grad=bg_grad,
+                                        beam_current=bc, beam_current_grad=bc_grad)
     else:
         loss, sse = engine.run(obj, probe_mnn, positions, corrections, measured, seq, obj_grad=object_grad,
                                probe_grad=probe_grad_mnn, pos_grad=pos_grad, masks=masks_dev,
-                               static_background=bg, static_background_grad=bg_grad)
+                               static_background=bg, static_background_grad=bg_grad, beam_current=bc,
+                               beam_current_grad=bc_grad)
     if loss != loss:
         raise ValueError("A very specific bad thing. Loss is Nan.")
     probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe
@@ -502,5 +527,9 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     # objective.py:377: zeros of the background's shape for an array, 0 for a scalar or none
     if bg_grad is None:
         bg_grad = (torch.zeros_like(bg) if bg is not None and not np.isscalar(static_background) else 0)
-    out = [_arrays.like(t, kind, template) for t in (object_grad, probe_grad, pos_grad, tilts_grad, bg_grad)]
-    return (loss, sse, out[0], out[1], out[2], out[3], out[4], 0, 0, constraint_contributions)
+    # objective.py:378: zeros of the current's shape when there is one, else 0
+    if bc_grad is None:
+        bc_grad = torch.zeros_like(bc) if bc is not None else 0
+    out = [_arrays.like(t, kind, template)
+           for t in (object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, bc_grad)]
+    return (loss, sse, out[0], out[1], out[2], out[3], out[4], 0, out[5], constraint_contributions)

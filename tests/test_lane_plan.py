@@ -21,9 +21,11 @@ from pypty_amd.objective import PassConfig
 
 
 def _cfg(n, S, K, M, P, batch, ck=0, lanes=2, **kw):
-    return PassConfig(n=n, n_slices=S, n_obj_modes=K, n_probe_modes=M, obj_ny=n + 48, obj_nx=n + 48,
-                      n_positions=P, compute_batch=batch, step_obj=True, step_probe=True, step_pos=True,
-                      shift_active=True, checkpoint_slices=ck, lanes=lanes, **kw)
+    args = dict(n=n, n_slices=S, n_obj_modes=K, n_probe_modes=M, obj_ny=n + 48, obj_nx=n + 48, n_positions=P,
+                compute_batch=batch, step_obj=True, step_probe=True, step_pos=True, shift_active=True,
+                checkpoint_slices=ck, lanes=lanes)
+    args.update(kw)
+    return PassConfig(**args)
 
 
 def _check(cfg, n_seq=None, lanes=None, calls=1, accumulate=False):
@@ -56,6 +58,7 @@ def test_every_tile_family(n, S, K, M, P, batch, ck, lanes):
     dict(algorithm="lsq_compressed", n_masks=5), dict(algorithm="ml", data_shift_y=2, data_bin=2, raw_n=64,
                                                       data_pad=0, data_multiplier=2.0),
     dict(step_static_background=True), dict(planar=True), dict(algorithm="lsq_sqrt_2"),
+    dict(step_beam_current=True), dict(step_beam_current=True, shift_active=False, step_pos=False),
 ])
 def test_data_format_options(extra):
     n = 32 if extra.get("raw_n") != 64 else 32
