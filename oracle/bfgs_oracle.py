@@ -16,10 +16,11 @@ from . import ms_oracle as orc
 def new_history():
     """pypty/iterative.py:1055-1103 (the variables this restatement supports)."""
     return {"obj_hist_s": [], "probe_hist_s": [], "positions_hist_s": [], "static_background_hist_s": [],
-            "obj_hist_y": [], "probe_hist_y": [], "positions_hist_y": [], "static_background_hist_y": [],
-            "rho_hist": [], "prev_loss": None, "prev_sse": None, "obj_grad": None, "probe_grad": None,
-            "pos_grad": None, "static_background_grad": None, "empty_hist": True, "gamma_obj": 1, "gamma_probe": 1,
-            "gamma_pos": 1, "gamma_static": 1}
+            "beam_current_hist_s": [], "obj_hist_y": [], "probe_hist_y": [], "positions_hist_y": [],
+            "static_background_hist_y": [], "beam_current_hist_y": [], "rho_hist": [], "prev_loss": None,
+            "prev_sse": None, "obj_grad": None, "probe_grad": None, "pos_grad": None, "static_background_grad": None,
+            "beam_current_grad": None, "empty_hist": True, "gamma_obj": 1, "gamma_probe": 1, "gamma_pos": 1,
+            "gamma_static": 1, "gamma_beam_current": 1}
 
 
 def _d(a, b):
@@ -34,35 +35,41 @@ def _d(a, b):
 
 def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_pos=False, actual_step=1.0,
                cutoff=2 / 3, update_extra_cut=0.005, wolfe_c1=0.1, wolfe_c2=0.9, max_count=20, reduce_factor=0.1,
-               optimism=3, min_step=1e-20, hist_length=10, cdtype=np.complex64, update_bg=False):
+               optimism=3, min_step=1e-20, hist_length=10, cdtype=np.complex64, update_bg=False, update_bc=False):
     """One epoch; mutates state and hist; returns (loss, sse, actual_step, count, d_value, new_d_value, accepted).
 
     A static background is state["bg"] (N, N) when present; it enters every loss evaluation and,
     with update_bg, is refined as the fourth variable (iterative.py:480, 569, 604-671, 688, 769,
-    855-862, 898-903, 1030), a real block like the positions."""
+    855-862, 898-903, 1030), a real block like the positions.  A beam current is state["bc"] (P,)
+    when present; it enters every loss evaluation and, with update_bc, is refined as the fifth
+    (iterative.py:481, 571, 591-673, 690, 771, 853-870, 910-915, 1029)."""
     fd = np.float32 if np.dtype(cdtype) == np.complex64 else np.float64
     cut = float(fd(cutoff) - fd(update_extra_cut))
     bg0 = state.get("bg", data.get("static_background"))
     update_bg = bool(update_bg) and bg0 is not None
-    on = (update_obj, update_probe, update_pos, update_bg)
-    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s")
-    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y")
-    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static")
+    bc0 = state.get("bc", data.get("beam_current"))
+    update_bc = bool(update_bc) and bc0 is not None
+    on = (update_obj, update_probe, update_pos, update_bg, update_bc)
+    NB = len(on)
+    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s", "beam_current_hist_s")
+    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y", "beam_current_hist_y")
+    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static", "gamma_beam_current")
     for k in S_KEYS + Y_KEYS:
         hist.setdefault(k, [0] * len(hist["rho_hist"]))
     hist.setdefault("gamma_static", 1)
+    hist.setdefault("gamma_beam_current", 1)
 
     def clean(x):
         return orc.fourier_clean(x, cutoff=cut, fdtype=fd).astype(cdtype)
 
     def evaluate(v):
-        o, p, c, b = v
+        o, p, c, b, cur = v
         out = orc.loss_and_direction(
             o, p, state["positions"], c, data["measured"], data["sequence"], dz=data["dz"],
             wavelength=data["wavelength"], pixel_size_x=data["px"], pixel_size_y=data["px"],
             algorithm=data.get("algorithm", "lsq_sqrt"), cutoff=cutoff, step_obj=update_obj,
             step_probe=update_probe, step_pos=update_pos, compute_batch=data.get("compute_batch", 64), cdtype=cdtype,
-            static_background=b, step_static_background=update_bg)
+            static_background=b, step_static_background=update_bg, beam_current=cur, step_beam_current=update_bc)
         loss, sse, og, pg, posg = out[:5]
         # L1 constraints of the object (objective.py:811-830), in the loss and object gradient
         for wa, wp in ((0.0, data.get("phase_norm_weight", 0)), (data.get("abs_norm_weight", 0), 0.0)):
@@ -86,8 +93,12 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
             pg = clean(pg)
         if update_obj:
             og = clean(og)
-        # objective.py:376-377: zeros of the background's shape when it is not refined
-        return loss, sse, [og, pg, posg, out[5] if update_bg else (None if b is None else np.zeros_like(b))]
+        # objective.py:376-378: zeros of the background's (current's) shape when it is not refined
+        extra = list(out[5:])
+        bgg = extra.pop(0) if b is not None else None
+        bcg = extra.pop(0) if cur is not None else None
+        return loss, sse, [og, pg, posg, bgg if update_bg else (None if b is None else np.zeros_like(b)),
+                           bcg if update_bc else (None if cur is None else np.zeros_like(cur))]
 
     def ddot(u, g):
         return sum(_d(a, b) for a, b, f in zip(u, g, on) if f)
@@ -96,23 +107,26 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         state["probe"] = clean(state["probe"])
     if update_obj:
         state["obj"] = clean(state["obj"])
-    x = [state["obj"], state["probe"], state["corr"], bg0]
+    x = [state["obj"], state["probe"], state["corr"], bg0, bc0]
     if hist["empty_hist"]:
         loss, sse, g = evaluate(x)
         uo = -g[0] if update_obj else None
         dvo = _d(uo, uo) if update_obj else 0.0
         if dvo == 0:
             dvo = 1.0
-        u = [uo] + [-g[j] * fd(dvo / _d(g[j], g[j])) if on[j] else None for j in (1, 2, 3)]
+        u = [uo] + [-g[j] * fd(dvo / _d(g[j], g[j])) if on[j] else None for j in range(1, NB)]
         hist["empty_hist"] = False
     else:
         loss, sse = hist["prev_loss"], hist["prev_sse"]
-        g = [hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist.get("static_background_grad")]
+        g = [hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist.get("static_background_grad"),
+             hist.get("beam_current_grad")]
         if update_bg and g[3] is None:
             g[3] = np.zeros_like(bg0)
+        if update_bc and g[4] is None:
+            g[4] = np.zeros_like(bc0)
         if update_probe:
             g[1] = clean(g[1])
-        q = [g[j].copy() if on[j] else None for j in range(4)]
+        q = [g[j].copy() if on[j] else None for j in range(NB)]
         rhos = hist["rho_hist"]
         alphas = []
         for i in range(len(rhos) - 1, -1, -1):
@@ -136,7 +150,7 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
             u[0] = clean(u[0])
     d_value = ddot(u, g)
     if d_value >= 0:
-        u = [-g[j] if on[j] else None for j in range(4)]
+        u = [-g[j] if on[j] else None for j in range(NB)]
         for j, key in enumerate(G_KEYS):
             gm = hist[key]
             if on[j] and gm > 0 and gm == gm:
@@ -148,7 +162,8 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         nx = [clean(x[0] + fd(actual_step) * u[0]) if update_obj else x[0],
               x[1] + fd(actual_step) * u[1] if update_probe else x[1],
               (x[2] + fd(actual_step) * u[2]).astype(fd) if update_pos else x[2],
-              (x[3] + fd(actual_step) * u[3]).astype(fd) if update_bg else x[3]]
+              (x[3] + fd(actual_step) * u[3]).astype(fd) if update_bg else x[3],
+              (x[4] + fd(actual_step) * u[4]).astype(fd) if update_bc else x[4]]
         nloss, nsse, ng = evaluate(nx)
         nd = ddot(u, ng)
         w1 = orc.wolfe_1(loss, nloss, d_value, actual_step, wolfe_c1)
@@ -169,13 +184,14 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         hist.clear()
         hist.update(new_history())
         return loss, sse, actual_step, count, d_value, nd, False
-    s = [fd(actual_step) * u[j] if on[j] else 0 for j in range(4)]
-    y = [ng[j] - g[j] if on[j] else 0 for j in range(4)]
-    for j in range(4):
+    s = [fd(actual_step) * u[j] if on[j] else 0 for j in range(NB)]
+    y = [ng[j] - g[j] if on[j] else 0 for j in range(NB)]
+    for j in range(NB):
         hist[S_KEYS[j]].append(s[j])
         hist[Y_KEYS[j]].append(y[j])
     hist["prev_loss"], hist["prev_sse"] = nloss, nsse
-    hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist["static_background_grad"] = ng
+    (hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist["static_background_grad"],
+     hist["beam_current_grad"]) = ng
     tot = 0.0
     for j, key in enumerate(G_KEYS):
         if on[j]:
@@ -192,4 +208,6 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         state["corr"] = nx[2]
     if update_bg:
         state["bg"] = nx[3]
+    if update_bc:
+        state["bc"] = nx[4]
     return loss, sse, actual_step, count, d_value, nd, True

@@ -5,9 +5,11 @@ tuple and its module-global state (``obj``, ``probe``, ``positions``,
 ``positions_correction``, ``tilts``, ``tilts_correction``,
 ``static_background``, ``aberrations_array``, ``beam_current``,
 ``history_bfgs``).  Supported variables: object, probe and sub-pixel
-positions (the ones the benchmark configurations refine), and the static
+positions (the ones the benchmark configurations refine), the static
 background held in ``static_background`` (in every loss evaluation; refined
-with ``this_step_static_background``).  Every large
+with ``this_step_static_background``) and the per-position beam current held
+in ``beam_current`` (likewise; refined with ``this_beam_current_step``).
+Every large
 operation runs on the HIP device through this package's kernels:
 loss and gradients (``objective.loss_and_direction``), the band-limits
 (``utils.fourier_clean``, plane FFT kernels), the inner products of the
@@ -22,7 +24,7 @@ import sys
 import numpy as np
 import torch
 
-from . import lbfgs, objective, utils
+from . import _arrays, lbfgs, objective, utils
 
 obj = None
 probe = None
@@ -99,10 +101,9 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     Returns (total_loss, sse, constraint_contributions, actual_step, count,
     d_value, new_d_value, warnings, counter_failed_epochs).
     """
-    global obj, probe, positions_correction, static_background, history_bfgs
+    global obj, probe, positions_correction, static_background, beam_current, history_bfgs
     _unsupported("this_step_tilts", bool(this_step_tilts))
     _unsupported("this_step_aberrations_array", bool(this_step_aberrations_array))
-    _unsupported("this_beam_current_step", bool(this_beam_current_step))
     _unsupported("phase_only_obj", bool(phase_only_obj))
     _unsupported("tune_only_probe_phase", bool(tune_only_probe_phase))
     _unsupported("tune_only_probe_abs", bool(tune_only_probe_abs))
@@ -126,9 +127,18 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     if update_bg and not isinstance(static_background, torch.Tensor):
         static_background = torch.full((probe.shape[0], probe.shape[1]), float(static_background),
                                        dtype=torch.float32, device=obj.device)
-    for key in ("static_background_hist_s", "static_background_hist_y"):
+    # the beam current, a real (P,) block (iterative.py:481, 571, 591-673, 690, 771, 853-870, 910-915,
+    # 1029); held in every loss evaluation when there is one
+    update_bc = this_beam_current_step > 0
+    if update_bc and beam_current is None:
+        raise ValueError("pypty_amd.bfgs_update: refining the beam current needs a beam_current")
+    if beam_current is not None and not isinstance(beam_current, torch.Tensor):
+        beam_current = torch.as_tensor(np.asarray(beam_current, np.float32), device=obj.device)
+    for key in ("static_background_hist_s", "static_background_hist_y", "beam_current_hist_s",
+                "beam_current_hist_y"):
         history_bfgs.setdefault(key, [0] * len(history_bfgs["rho_hist"]))
     history_bfgs.setdefault("gamma_static", 1)
+    history_bfgs.setdefault("gamma_beam_current", 1)
     cut = damping_cutoff_multislice - update_extra_cut
     clean = not lazy_clean
     # an infinite probe_reg_weight band-limits the probe and its gradient to the aperture instead of
@@ -144,14 +154,14 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         probe_reg_weight = 0
         probe = _clean_aperture(probe)
 
-    def evaluate(o, p, pc, b):
+    def evaluate(o, p, pc, b, c):
         out = objective.loss_and_direction(
             o, p, positions, pc, tilts, tilts_correction, this_slice_distances, edge_slice_thickness_multiplier,
             measured_array, algorithm_type, this_wavelength, update_probe, update_obj, update_pos, False, masks,
             pixel_size_x_A, pixel_size_y_A, recon_type, Cs, defocus_array, alpha_near_field, damping_cutoff_multislice,
             0, propmethod, this_chopped_sequence, load_one_by_one, data_multiplier, data_pad, phase_plate_in_h5,
             this_loss_weight, data_bin, data_shift_vector, upsample_pattern, b, update_bg, tilt_mode,
-            aberration_marker, probe_marker, None, compute_batch, phase_only_obj, None, False, False, default_float,
+            aberration_marker, probe_marker, None, compute_batch, phase_only_obj, c, update_bc, False, default_float,
             default_complex, xp, is_first_epoch, scan_size, scan_step_A, fast_axis_reg_weight_positions,
             slow_axis_reg_weight_positions, slow_axis_reg_weight_tilts, current_deformation_reg_weight_positions,
             current_deformation_reg_weight_tilts, fast_axis_reg_weight_tilts, aperture_mask, probe_reg_weight,
@@ -170,17 +180,19 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         # an unrefined background has a zero gradient, as objective.py:376-377 returns it, so a history
         # entry stored in such an epoch serves an epoch that starts refining it without a reset
         bgg = out[6] if update_bg else (torch.zeros_like(b) if isinstance(b, torch.Tensor) else 0)
-        return loss, sse, og, pg, posg, out[9], bgg
+        bcg = out[8] if update_bc else (torch.zeros_like(c) if isinstance(c, torch.Tensor) else 0)
+        return loss, sse, og, pg, posg, out[9], bgg, bcg
 
     if update_probe:
         probe = _clean(probe, cut, smooth_rolloff)
     if update_obj and clean:
         obj = _clean(obj, cut, smooth_rolloff)
 
-    on = (update_obj, update_probe, update_pos, update_bg)
-    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s")
-    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y")
-    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static")
+    on = (update_obj, update_probe, update_pos, update_bg, update_bc)
+    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s", "beam_current_hist_s")
+    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y", "beam_current_hist_y")
+    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static", "gamma_beam_current")
+    NB = len(on)
 
     def direction_dot(u, g):
         # a history entry of 0 (the variable was not refined in that epoch) adds 0, as in the reference
@@ -191,25 +203,29 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         return d
 
     if empty_hist:
-        total_loss, this_sse, og, pg, posg, cc, bgg = evaluate(obj, probe, positions_correction, static_background)
-        g = [og, pg, posg, bgg]
+        total_loss, this_sse, og, pg, posg, cc, bgg, bcg = evaluate(obj, probe, positions_correction,
+                                                                    static_background, beam_current)
+        g = [og, pg, posg, bgg, bcg]
         u_obj = _neg(og) if update_obj else None
         d_val_obj = _dot2(u_obj, u_obj) if update_obj else 0.0
         if d_val_obj == 0:
             d_val_obj = 1.0
-        u = [u_obj] + [lbfgs.lincomb(g[j], -d_val_obj / _dot2(g[j], g[j])) if on[j] else None for j in (1, 2, 3)]
+        u = [u_obj] + [lbfgs.lincomb(g[j], -d_val_obj / _dot2(g[j], g[j])) if on[j] else None
+                       for j in range(1, NB)]
         history_bfgs["empty_hist"] = False
     else:
         total_loss, this_sse = history_bfgs["prev_loss"], history_bfgs["prev_sse"]
         g = [history_bfgs["obj_grad"], history_bfgs["probe_grad"], history_bfgs["pos_grad"],
-             history_bfgs.get("static_background_grad")]
+             history_bfgs.get("static_background_grad"), history_bfgs.get("beam_current_grad")]
         if update_bg and not isinstance(g[3], torch.Tensor):
             g[3] = torch.zeros_like(static_background)  # a background held fixed until now
+        if update_bc and not isinstance(g[4], torch.Tensor):
+            g[4] = torch.zeros_like(beam_current)  # a current held fixed until now
         cc = history_bfgs["constraint_contributions"]
         if update_probe:
             g[1] = _clean(g[1], cut, smooth_rolloff)
         rhos = history_bfgs["rho_hist"]
-        q = [g[j].clone() if on[j] else None for j in range(4)]
+        q = [g[j].clone() if on[j] else None for j in range(NB)]
         alphas = []
         for i in range(len(rhos) - 1, -1, -1):
             a = rhos[i] * direction_dot([history_bfgs[k][i] for k in S_KEYS], q)
@@ -227,21 +243,21 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             for j, key in enumerate(S_KEYS):
                 if on[j] and isinstance(history_bfgs[key][i], torch.Tensor):
                     lbfgs.axpby(q[j], history_bfgs[key][i], 1.0, beta)
-        u = [lbfgs.axpby(q[j], None, -1.0, 0.0) if on[j] else None for j in range(4)]
+        u = [lbfgs.axpby(q[j], None, -1.0, 0.0) if on[j] else None for j in range(NB)]
         if clean and update_obj:
             u[0] = _clean(u[0], cut, smooth_rolloff)
     d_value = direction_dot(u, g)
     if d_value >= 0:
         sys.stdout.write("\n\n\nWarning! Positive dir-derivative, using normal gradient-descent!\n\n\n\n\n\n")
         warnings += "\n\\Positive dir-derivative, using normal gradient-descent!"
-        u = [_neg(g[j]) if on[j] else None for j in range(4)]
+        u = [_neg(g[j]) if on[j] else None for j in range(NB)]
         for j, key in enumerate(G_KEYS):
             gm = history_bfgs[key]
             if on[j] and gm > 0 and gm == gm:
                 lbfgs.axpby(u[j], None, gm, 0.0)
         d_value = direction_dot(u, g)
-    u_obj, u_probe, u_pos, u_bg = u
-    og, pg, posg, bgg = g
+    u_obj, u_probe, u_pos, u_bg, u_bc = u
+    og, pg, posg, bgg, bcg = g
     count = -1
     new_d_value = d_value
     this_wolfe_1 = False
@@ -258,8 +274,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             else positions_correction
         new_bg = lbfgs.axpby(static_background.clone(), u_bg, 1.0, actual_step) if update_bg \
             else static_background
-        new_total_loss, new_sse, nog, npg, nposg, ncc, nbgg = evaluate(new_obj, new_probe, new_pc, new_bg)
-        new_d_value = direction_dot(u, (nog, npg, nposg, nbgg))
+        new_bc = lbfgs.axpby(beam_current.clone(), u_bc, 1.0, actual_step) if update_bc else beam_current
+        new_total_loss, new_sse, nog, npg, nposg, ncc, nbgg, nbcg = evaluate(new_obj, new_probe, new_pc, new_bg,
+                                                                             new_bc)
+        new_d_value = direction_dot(u, (nog, npg, nposg, nbgg, nbcg))
         this_wolfe_1 = utils.wolfe_1(total_loss, new_total_loss, d_value, actual_step, wolfe_c1_constant)
         this_wolfe_2 = utils.wolfe_2(d_value, new_d_value, wolfe_c2_constant)
         if print_flag >= 3:
@@ -297,23 +315,27 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     s_probe = lbfgs.lincomb(u_probe, actual_step) if update_probe else 0
     s_pos = lbfgs.lincomb(u_pos, actual_step) if update_pos else 0
     s_bg = lbfgs.lincomb(u_bg, actual_step) if update_bg else 0
+    s_bc = lbfgs.lincomb(u_bc, actual_step) if update_bc else 0
     y_obj = lbfgs.axpby(nog.clone(), og, 1.0, -1.0) if update_obj else 0
     y_probe = lbfgs.axpby(npg.clone(), pg, 1.0, -1.0) if update_probe else 0
     y_pos = lbfgs.axpby(nposg.clone(), posg, 1.0, -1.0) if update_pos else 0
     y_bg = lbfgs.axpby(nbgg.clone(), bgg, 1.0, -1.0) if update_bg else 0
+    y_bc = lbfgs.axpby(nbcg.clone(), bcg, 1.0, -1.0) if update_bc else 0
     for key, v in (("obj_hist_s", s_obj), ("probe_hist_s", s_probe), ("positions_hist_s", s_pos),
-                   ("static_background_hist_s", s_bg), ("obj_hist_y", y_obj), ("probe_hist_y", y_probe),
-                   ("positions_hist_y", y_pos), ("static_background_hist_y", y_bg)):
+                   ("static_background_hist_s", s_bg), ("beam_current_hist_s", s_bc), ("obj_hist_y", y_obj),
+                   ("probe_hist_y", y_probe), ("positions_hist_y", y_pos), ("static_background_hist_y", y_bg),
+                   ("beam_current_hist_y", y_bc)):
         h[key].append(v)
-    for key in ("tilts_hist_s", "beam_current_hist_s", "aberrations_hist_s",
-                "tilts_hist_y", "beam_current_hist_y", "aberrations_hist_y"):
+    for key in ("tilts_hist_s", "aberrations_hist_s", "tilts_hist_y", "aberrations_hist_y"):
         h[key].append(0)
     h["prev_loss"], h["prev_sse"] = new_total_loss, new_sse
     h["obj_grad"], h["probe_grad"], h["pos_grad"], h["static_background_grad"] = nog, npg, nposg, nbgg
+    h["beam_current_grad"] = nbcg
     h["constraint_contributions"] = ncc
     yiTsi_tot = 0.0
     for y, s, key, flag in ((y_obj, s_obj, "gamma_obj", update_obj), (y_probe, s_probe, "gamma_probe", update_probe),
-                            (y_pos, s_pos, "gamma_pos", update_pos), (y_bg, s_bg, "gamma_static", update_bg)):
+                            (y_pos, s_pos, "gamma_pos", update_pos), (y_bg, s_bg, "gamma_static", update_bg),
+                            (y_bc, s_bc, "gamma_beam_current", update_bc)):
         if flag:
             (yy,) = lbfgs.dots([(y, y)])
             ys = _dot2(y, s)
@@ -330,6 +352,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         positions_correction = new_pc
     if update_bg:
         static_background = new_bg
+    if update_bc:
+        beam_current = new_bc
     return total_loss, this_sse, cc, actual_step, count, d_value, new_d_value, warnings, counter_failed_epochs
 
 
@@ -342,9 +366,9 @@ def _per_epoch(v, epoch):
     return v(epoch) if callable(v) else v
 
 
-def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None):
-    """co.npy / cp.npy / cg.npy, and cs.npy for a background being refined, as pypty/utils.py:2085-2118
-    writes them."""
+def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None):
+    """co.npy / cp.npy / cg.npy, cs.npy for a background and cb.npy for a beam current being refined, as
+    pypty/utils.py:2085-2118 writes them."""
     import os
 
     if not folder:
@@ -358,6 +382,8 @@ def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg
         np.save(os.path.join(folder, "cg.npy"), (pos.double() + corr.double()).cpu().numpy())
     if isinstance(bg, torch.Tensor):
         np.save(os.path.join(folder, "cs.npy"), bg.cpu().numpy())
+    if isinstance(bc, torch.Tensor):
+        np.save(os.path.join(folder, "cb.npy"), bc.cpu().numpy())
 
 
 def lambda_source(f):
@@ -450,6 +476,7 @@ def run(pypty_params):
     min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
     save_loss_log, print_flag, static_background and update_static_background,
+    beam_current and update_beam_current,
     phase_norm_weight and abs_norm_weight (values or callables of the epoch), and two keys
     not in the reference:
     checkpoint_slices (keep every k-th slice's wave and recompute the rest in
@@ -463,7 +490,8 @@ def run(pypty_params):
     import os
     import time
 
-    global obj, probe, positions, positions_correction, tilts, tilts_correction, static_background, history_bfgs
+    global obj, probe, positions, positions_correction, tilts, tilts_correction, static_background, beam_current
+    global history_bfgs
     get = pypty_params.get
     dev = torch.device("cuda", torch.cuda.current_device())
     dataset = get("dataset", None)
@@ -519,6 +547,9 @@ def run(pypty_params):
     tilts = np.zeros((1, 6), np.float32)
     tilts_correction = np.zeros((1, 6), np.float32)
     static_background = _static_background_from_nothing(get("static_background", None), n, float(cutoff))
+    beam_current = get("beam_current", None)
+    if beam_current is not None:
+        beam_current = _arrays.to_torch(beam_current, dev, torch.float32).reshape(-1).contiguous()
     measured = dataset if isinstance(dataset, torch.Tensor) else torch.from_numpy(dataset).to(dev)
     folder = get("output_folder", "")
     print_flag = get("print_flag", 3)
@@ -545,6 +576,7 @@ def run(pypty_params):
         s_probe = _per_epoch(get("update_probe", 1), epoch)
         s_pos = _per_epoch(get("update_probe_pos", 0), epoch)
         s_bg = _per_epoch(get("update_static_background", 0), epoch) if static_background is not None else 0
+        s_bc = _per_epoch(get("update_beam_current", 0), epoch)
         first_step = _per_epoch(step_param, epoch)  # per epoch, as pypty/iterative.py:325-336 reads it
         # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, the static
         # background after 4, the probe after 6 (tilts and aberrations are not refined here)
@@ -582,13 +614,16 @@ def run(pypty_params):
                            else np.asarray(window, np.float32))
         # the history is reset when the objective changes: a different set of refined variables or of
         # non-zero constraint weights (pypty/iterative.py:383-389; of its weights run() takes these)
-        steps = sum(int(x != 0) for x in (s_obj, s_probe, s_pos, s_bg))
+        steps = sum(int(x != 0) for x in (s_bc, s_obj, s_probe, s_pos, s_bg))
         constraints = sum(int(x != 0) for x in (lw, w_phase, w_abs, *[cw[k] for k in cw if k not in (
             "mixed_variance_sigma", "beta_wedge", "probe_mod_tol")], cw["mixed_variance_sigma"]))
         if epoch != int(get("epoch_prev", 0)) and (steps != prev_steps or constraints != prev_constraints):
             reset_bfgs_history()
             warnings += "\nResetting the history!"
         prev_steps, prev_constraints = steps, constraints
+        if s_bc:
+            # ones for a missing current, a short one padded (pypty/iterative.py:417)
+            beam_current = utils.try_to_initialize_beam_current(beam_current, dataset.shape)
         seq = np.sort(np.asarray(_per_epoch(seq_param, epoch))) if seq_param is not None else seq0
         out = bfgs_update(
             algorithm_type=get("algorithm", "lsq_sqrt"), this_slice_distances=slice_distances,
@@ -615,7 +650,7 @@ def run(pypty_params):
             fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=None, this_step_aberrations_array=0,
             probe_marker=None, compute_batch=get("compute_batch", 256), current_window=this_window,
             current_window_weight=cw["window_weight"], dynamically_resize_yx_object=False, lazy_clean=False,
-            current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=0,
+            current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=s_bc,
             xp=None, remove_fft_cache=False, is_first_epoch=is_first,
             hist_length=_per_epoch(get("hist_length", 10), epoch), actual_step=first_step,
             fast_axis_reg_weight_positions=cw["fast_axis_reg_weight_positions"], fast_axis_reg_weight_tilts=0,
@@ -634,7 +669,7 @@ def run(pypty_params):
                   f"linesearch {count + 1} time {time.time() - t0:.2f} s", flush=True)
         if save_inter and epoch % int(save_inter) == 0:
             _save_checkpoint(folder, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
-                             static_background if s_bg else None)
+                             static_background if s_bg else None, beam_current if s_bc else None)
         if save_log:
             free, total = torch.cuda.mem_get_info(dev)
             _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,
@@ -647,5 +682,5 @@ def run(pypty_params):
             epoch_callback(epoch, obj, probe)
     _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
     result = {"obj": obj, "probe": probe, "positions": positions.double() + positions_correction.double(),
-              "losses": losses, "time_s": time.time() - t0}
+              "beam_current": beam_current, "losses": losses, "time_s": time.time() - t0}
     return result

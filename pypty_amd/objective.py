@@ -416,10 +416,15 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     # beam current (objective.py:446-450, 688-698): each position's probe times |beam_current|
     bc = bc_grad = None
     step_bc = beam_current is not None and bool(this_beam_current_step)
+    bc_full = None
     if beam_current is not None:
-        bc = _as_device(beam_current, torch.float32, device).reshape(-1).contiguous()
-        if bc.numel() != positions.shape[0]:
-            raise ValueError("pypty_amd.loss_and_direction: beam_current needs one value per position")
+        # a current longer than the scan (as utils.try_to_initialize_beam_current pads it) keeps its
+        # extra entries, which no position indexes and whose gradient is zero
+        bc_full = _as_device(beam_current, torch.float32, device).reshape(-1)
+        P = positions.shape[0]
+        if bc_full.numel() < P:
+            raise ValueError("pypty_amd.loss_and_direction: beam_current needs a value per position")
+        bc = bc_full[:P].contiguous()
         if step_bc:
             bc_grad = torch.empty_like(bc)
     compressed = "compressed" in algorithm_type
@@ -528,8 +533,13 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     if bg_grad is None:
         bg_grad = (torch.zeros_like(bg) if bg is not None and not np.isscalar(static_background) else 0)
     # objective.py:378: zeros of the current's shape when there is one, else 0
-    if bc_grad is None:
-        bc_grad = torch.zeros_like(bc) if bc is not None else 0
+    if bc is None:
+        bc_grad = 0
+    elif bc_grad is None or bc_full.numel() != bc.numel():
+        full = torch.zeros_like(bc_full)
+        if bc_grad is not None:
+            full[:bc.numel()] = bc_grad
+        bc_grad = full
     out = [_arrays.like(t, kind, template)
            for t in (object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, bc_grad)]
     return (loss, sse, out[0], out[1], out[2], out[3], out[4], 0, out[5], constraint_contributions)

@@ -142,3 +142,21 @@ def wolfe_1(value, new_value, d_value, step, wolfe_c1=0.5):
 def wolfe_2(d_value, new_d_value, wolfe_c2=0.9):
     """Curvature condition (pypty/utils.py:350-368)."""
     return -1 * new_d_value <= -1 * d_value * wolfe_c2
+
+
+def try_to_initialize_beam_current(beam_current, measured_data_shape, default_float=np.float32, xp=None):
+    """A beam current of ones, one per pattern, when there is none (pypty/utils.py:1766-1789).  A
+    shorter one is padded with ones by the pattern count minus its length on each side, as numpy's
+    pad with a scalar width does there, so its values move up by that count.  Returns a device
+    float32 tensor (the input's array type is not kept: bfgs_update holds it on the device)."""
+    n = int(measured_data_shape[0])
+    if beam_current is None:
+        return torch.ones(n, dtype=torch.float32, device=torch.device("cuda", torch.cuda.current_device()))
+    bc = _arrays.to_torch(beam_current, dtype=torch.float32)
+    if bc.device.type != "cuda":
+        bc = bc.to(torch.device("cuda", torch.cuda.current_device()))
+    bc = bc.reshape(-1)
+    if bc.shape[0] < n:
+        w = n - bc.shape[0]
+        bc = torch.nn.functional.pad(bc, (w, w), value=1.0)
+    return bc.contiguous()

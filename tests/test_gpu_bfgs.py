@@ -152,6 +152,57 @@ def test_epochs_refining_the_static_background_match_oracle():
         iterative.static_background = None
 
 
+def test_epochs_refining_the_beam_current_match_oracle():
+    """The beam current refined as a fifth variable with object, probe and positions (iterative.py:481,
+    571-1029), then held for an epoch (its history entries 0, the scaling still on): losses,
+    line-search counts and the updated variables follow the oracle."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 4 epochs refining object, probe, positions and beam current", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=True, seed=31, noise=0.02)
+    bc = np.random.default_rng(7).uniform(0.8, 1.2, 10).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy(),
+             "bc": bc.copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.beam_current = torch.from_numpy(bc.copy()).to(dev)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    try:
+        for epoch in range(4):
+            bc_on = epoch < 3
+            np.random.seed(400 + epoch)
+            ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=True, update_bc=bc_on, actual_step=step_o)
+            np.random.seed(400 + epoch)
+            rg = call_loss_and_direction(iterative.bfgs_update,
+                                         **bfgs_kwargs(pr, actual_step=step_g, is_first_epoch=(epoch == 0),
+                                                       this_beam_current_step=int(bc_on)))
+            step_o, step_g = ro[2], rg[3]
+            e_bc = rel(iterative.beam_current.cpu().numpy(), state["bc"])
+            print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; step {rg[3]:.3e}/{ro[2]:.3e}; "
+                  f"count {rg[4]}/{ro[3]}; rel err beam current {e_bc:.2e}", flush=True)
+            assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+            assert rg[4] == ro[3]
+            assert abs(rg[3] - ro[2]) <= 1e-5 * abs(ro[2])
+            assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+            assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
+            assert rel(iterative.positions_correction.cpu().numpy(), state["corr"]) < 1e-4
+            assert e_bc < 1e-4
+        assert float(np.abs(state["bc"] - bc).max()) > 0  # the current moved
+    finally:
+        iterative.beam_current = None
+
+
 def test_epochs_with_l1_constraints_match_oracle():
     """phase_norm_weight and abs_norm_weight (objective.py:811-830) enter every loss evaluation and
     object gradient of the epoch."""

@@ -144,3 +144,65 @@ def test_run_resets_the_history_when_an_l1_weight_switches_on(tmp_path):
     assert all(c == 0.0 for c in cons[:3]) and all(c > 0.0 for c in cons[3:])
     assert "Resetting the history" in rows[3][12]
     assert 3 in resets
+
+
+def test_run_refines_the_beam_current_and_writes_it(tmp_path):
+    """Patterns whose intensities fluctuate by a per-position current squared, reconstructed from the
+    true object with object and beam current refined from a current of ones (pypty/iterative.py:417
+    starts it so): run() lowers the loss monotonically, moves the current's relative fluctuations
+    towards the true ones and writes cb.npy (pypty/utils.py:2096-2099)."""
+    import torch
+
+    from pypty_amd import iterative, synth
+
+    print("[run] 6 epochs refining object and beam current", flush=True)
+    dev = torch.device("cuda", 0)
+    px, n = 0.2, 64
+    true_obj = synth.hex_lattice_object(120, 120, px, 2.46, 0.3, 0.5, seed=15)
+    probe = synth.make_probe(n, px, 300.0, 20.0, 100.0, 1, 0.0, seed=16)
+    pos = synth.raster_scan(8, 8, 6.0, origin_px=(0.0, 0.0), jitter_px=0.0, seed=17)
+    ints, corr = synth.split_positions(pos)
+    meas = synth.simulate(true_obj, probe, ints, corr, px, 2.0, synth.electron_wavelength(300.0), 1e5, seed=18,
+                          device=dev)
+    bc_true = np.random.default_rng(20).uniform(0.85, 1.15, meas.shape[0]).astype(np.float32)
+    meas = (meas * bc_true[:, None, None] ** 2).astype(np.float32)
+    out = str(tmp_path) + os.sep
+    params = dict(dataset=meas, probe=probe, positions=pos, obj=true_obj.astype(np.complex64),
+                  acc_voltage=300.0, pixel_size_x_A=px, pixel_size_y_A=px, slice_distances=np.array([2.0]),
+                  num_slices=1, algorithm="lsq_sqrt", epoch_max=6, update_obj=1, update_probe=0,
+                  update_probe_pos=0, update_beam_current=1, compute_batch=32, output_folder=out,
+                  save_inter_checkpoints=1, print_flag=0)
+    try:
+        res = iterative.run(params)
+        losses = res["losses"]
+        print(f"  loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+        assert all(b <= a * (1 + 1e-9) for a, b in zip(losses, losses[1:])), losses
+        assert losses[-1] < losses[0]
+        cb = np.load(out + "cb.npy")
+        assert cb.shape == (meas.shape[0],) and cb.dtype == np.float32
+        np.testing.assert_array_equal(cb, res["beam_current"].cpu().numpy())
+        # the probe is not scaled to the dose, so the current also takes up the global intensity scale:
+        # compare the currents relative to their means
+        rel_true = bc_true / bc_true.mean()
+        e0 = np.linalg.norm(1.0 - rel_true)
+        e = np.linalg.norm(np.abs(cb) / np.abs(cb).mean() - rel_true)
+        print(f"  mean current {np.abs(cb).mean():.4e}; relative current error {e0:.4e} -> {e:.4e}", flush=True)
+        assert e < 0.5 * e0
+    finally:
+        iterative.beam_current = None
+
+
+def test_beam_current_initialisation_pads_like_the_reference():
+    """utils.try_to_initialize_beam_current (pypty/utils.py:1766-1789): ones for none; a short current
+    padded with ones by the shortfall on both sides (numpy's pad with a scalar width)."""
+    import torch
+
+    from pypty_amd import utils
+
+    ones = utils.try_to_initialize_beam_current(None, (7, 32, 32))
+    assert ones.is_cuda and ones.dtype == torch.float32 and torch.equal(ones.cpu(), torch.ones(7))
+    short = np.array([2.0, 3.0, 4.0], np.float32)
+    got = utils.try_to_initialize_beam_current(short, (5, 32, 32)).cpu().numpy()
+    np.testing.assert_array_equal(got, np.pad(short, pad_width=2, mode="constant", constant_values=1))
+    full = np.arange(5, dtype=np.float32)
+    np.testing.assert_array_equal(utils.try_to_initialize_beam_current(full, (5, 32, 32)).cpu().numpy(), full)
