@@ -31,6 +31,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pty_fft.hpp"
@@ -188,10 +189,10 @@ __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b)
 // window size: register-tile C-order (16 to 128 px) or global-slab C-order
 // (192, 256 px).
 enum { LAY_RT = 1, LAY_GT = 2 };
-#ifndef PTY_GT_THREADS
-#define PTY_GT_THREADS 512
-#endif
-constexpr int kGtThreads = PTY_GT_THREADS;  // threads of a 192/256 px slab workgroup
+// threads of a 192 / 256 px workgroup: 768 at 192 px (two rounds of 96
+// lines), 512 at 256 px (four rounds of 64 lines)
+template <int N>
+constexpr int kGtThreads = (N == 192) ? 768 : 512;
 constexpr int kMaxLanes = 4;                 // batch lanes of a pass (pty_problem.lanes)
 constexpr int kLayoutOf(int n) {
   return (n <= 128) ? LAY_RT : LAY_GT;
@@ -204,7 +205,7 @@ __host__ __device__ __forceinline__ void spec_of(int e, int& fy, int& fx) {
   if constexpr (LAY == LAY_RT)
     RT<N>::cspec(e, fy, fx);
   else
-    GT<N, kGtThreads>::cspec(e, fy, fx);
+    GT<N, kGtThreads<N>>::cspec(e, fy, fx);
 }
 
 template <int N>
@@ -1387,62 +1388,177 @@ struct GTLane {
 };
 
 // row spectrum of round r: register c*8 + k2 <-> slab (r*LINES + ln, 8*(CPT*k2 + c) + j)
-template <int N>
-__device__ __forceinline__ void gt_load_rowspec(float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
-  using G = GT<N, kGtThreads>;
-#pragma unroll
-  for (int c = 0; c < G::CPT; ++c)
-#pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) v[c * 8 + k2] = sl.ld(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8);
-}
-template <int N>
-__device__ __forceinline__ void gt_store_rowspec(const float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
-  using G = GT<N, kGtThreads>;
-#pragma unroll
-  for (int c = 0; c < G::CPT; ++c)
-#pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) sl.st(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8, v[c * 8 + k2]);
-}
 // column of round r: register n <-> slab (j + 8n, r*LINES + ln)
+// The global slab (256 px): the phases meet in workspace memory, so a phase
+// change is a workgroup barrier.
 template <int N>
-__device__ __forceinline__ void gt_load_col(float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
-  using G = GT<N, kGtThreads>;
+struct GSlab {
+  using G = GT<N, kGtThreads<N>>;
+  Slab<N> sl;
+  __device__ __forceinline__ explicit GSlab(float2* p) : sl(p) {}
+  __device__ __forceinline__ void load_rowspec(float2* v, const GTLane<N>& L, int r) const {
 #pragma unroll
-  for (int n = 0; n < G::R1; ++n) v[n] = sl.ld(L.vcol, (8 * n * N + r * G::LINES) * 8);
+    for (int c = 0; c < G::CPT; ++c)
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) v[c * 8 + k2] = sl.ld(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8);
+  }
+  __device__ __forceinline__ void store_rowspec(const float2* v, const GTLane<N>& L, int r) {
+#pragma unroll
+    for (int c = 0; c < G::CPT; ++c)
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) sl.st(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8, v[c * 8 + k2]);
+  }
+  __device__ __forceinline__ void load_col(float2* v, const GTLane<N>& L, int r) const {
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) v[n] = sl.ld(L.vcol, (8 * n * N + r * G::LINES) * 8);
+  }
+  __device__ __forceinline__ void store_col(const float2* v, const GTLane<N>& L, int r) {
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) sl.st(L.vcol, (8 * n * N + r * G::LINES) * 8, v[n]);
+  }
+  __device__ __forceinline__ void to_cols(float2*, const GTLane<N>&) { __syncthreads(); }
+  __device__ __forceinline__ void to_rows(float2*, const GTLane<N>&) { __syncthreads(); }
+};
+
+// The 192 px tile in registers (768 threads, two rounds of 96 lines, 24
+// samples per thread and round: 96 VGPRs).  A phase change moves it through
+// LDS in two passes of half the tile (96 rows of stride 200, 150 KB), the LDS
+// that the line transforms use for their wave-local exchanges:
+//   to_cols, pass p: every thread writes row p*96 + ln (its own line, so no
+//   wavefront touches another's exchange rows); barrier; it reads rows
+//   j + 8n (n < 12) of its columns ln and 96 + ln, the samples 12p .. 12p+11
+//   of both its column rounds; barrier
+//   to_rows: barrier (the column exchanges of the other wavefronts are done);
+//   then per pass p the reverse
+// Registers follow the slab's orders exactly, so every kernel written
+// against the slab runs unchanged on it.  The kernels unroll their round
+// loops, so d[r] is always a register.
+struct RegSlab {
+  static constexpr int N = 192;
+  using G = GT<N, kGtThreads<N>>;
+  static constexpr int R1 = G::R1, CPT = G::CPT, LINES = G::LINES, LS = G::LS, H = R1 / 2;
+  static_assert(G::ROUNDS == 2 && 8 * H == LINES, "two rounds of 96 lines");
+  float2 d[2][R1];
+  __device__ __forceinline__ explicit RegSlab(float2*) {}
+  __device__ __forceinline__ void load_rowspec(float2* v, const GTLane<N>&, int r) {
+#pragma unroll
+    for (int i = 0; i < R1; ++i) v[i] = d[r][i];
+  }
+  __device__ __forceinline__ void store_rowspec(const float2* v, const GTLane<N>&, int r) {
+#pragma unroll
+    for (int i = 0; i < R1; ++i) d[r][i] = v[i];
+  }
+  __device__ __forceinline__ void load_col(float2* v, const GTLane<N>&, int r) {
+#pragma unroll
+    for (int i = 0; i < R1; ++i) v[i] = d[r][i];
+  }
+  __device__ __forceinline__ void store_col(const float2* v, const GTLane<N>&, int r) {
+#pragma unroll
+    for (int i = 0; i < R1; ++i) d[r][i] = v[i];
+  }
+  __device__ __forceinline__ void to_cols(float2* X, const GTLane<N>& L) {
+    float2 t[2][2][H];  // [pass][column round][sample]
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      float2* w = X + L.ln * LS + L.j;
+#pragma unroll
+      for (int c = 0; c < CPT; ++c)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) w[8 * (CPT * k2 + c)] = d[p][c * 8 + k2];
+      __syncthreads();
+      const float2* q0 = X + L.j * LS + L.ln;
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int n = 0; n < H; ++n) t[p][q][n] = q0[8 * n * LS + q * LINES];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int n = 0; n < R1; ++n) d[q][n] = n < H ? t[0][q][n] : t[1][q][n - H];
+  }
+  __device__ __forceinline__ void to_rows(float2* X, const GTLane<N>& L) {
+    float2 t[2][R1];
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      float2* q0 = X + L.j * LS + L.ln;
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int n = 0; n < H; ++n) q0[8 * n * LS + q * LINES] = d[q][n + H * p];
+      __syncthreads();
+      const float2* w = X + L.ln * LS + L.j;
+#pragma unroll
+      for (int c = 0; c < CPT; ++c)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) t[p][c * 8 + k2] = w[8 * (CPT * k2 + c)];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int i = 0; i < R1; ++i) d[r][i] = t[r][i];
+  }
+};
+
+template <int N>
+using SlabOf = std::conditional_t<N == 192, RegSlab, GSlab<N>>;
+
+// Between the rounds of a phase on the register tile: the rounds are
+// independent, and a schedule that interleaves two rounds' transforms needs
+// both rounds' temporaries on top of the tile's 96 registers.
+template <class SL>
+__device__ __forceinline__ void round_fence() {
+  if constexpr (std::is_same_v<SL, RegSlab>) __builtin_amdgcn_sched_barrier(0);
 }
-template <int N>
-__device__ __forceinline__ void gt_store_col(const float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
-  using G = GT<N, kGtThreads>;
-#pragma unroll
-  for (int n = 0; n < G::R1; ++n) sl.st(L.vcol, (8 * n * N + r * G::LINES) * 8, v[n]);
+
+template <int N, class SL>
+__device__ __forceinline__ void gt_load_rowspec(float2* v, SL& sl, const GTLane<N>& L, int r) {
+  sl.load_rowspec(v, L, r);
+}
+template <int N, class SL>
+__device__ __forceinline__ void gt_store_rowspec(const float2* v, SL& sl, const GTLane<N>& L, int r) {
+  sl.store_rowspec(v, L, r);
+}
+template <int N, class SL>
+__device__ __forceinline__ void gt_load_col(float2* v, SL& sl, const GTLane<N>& L, int r) {
+  sl.load_col(v, L, r);
+}
+template <int N, class SL>
+__device__ __forceinline__ void gt_store_col(const float2* v, SL& sl, const GTLane<N>& L, int r) {
+  sl.store_col(v, L, r);
 }
 
 // bit r set: the wavefront's columns of round r are all marked zero
 template <int N>
 __device__ __forceinline__ unsigned gt_cols_zero(const int* zcol, const GTLane<N>& L) {
-  using G = GT<N, kGtThreads>;
+  using G = GT<N, kGtThreads<N>>;
   unsigned z = 0;
 #pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) z |= unsigned(wave_cols_zero<N>(zcol, r * G::LINES + L.ln)) << r;
   return z;
 }
-template <int N>
-__device__ __forceinline__ void gt_zero_col(float2* v, const Slab<N>& sl, const GTLane<N>& L, int r) {
+template <int N, class SL>
+__device__ __forceinline__ void gt_zero_col(float2* v, SL& sl, const GTLane<N>& L, int r) {
 #pragma unroll
-  for (int i = 0; i < GT<N, kGtThreads>::R1; ++i) v[i] = make_float2(0.f, 0.f);
+  for (int i = 0; i < GT<N, kGtThreads<N>>::R1; ++i) v[i] = make_float2(0.f, 0.f);
   gt_store_col<N>(v, sl, L, r);
 }
 
 // column phase of a band limit: forward column FFT, times maskc/N^2, inverse;
 // zero columns of the mask are written as zeros
-template <int N>
-__device__ __forceinline__ void gt_band_columns(float2* v, float2* X, const Slab<N>& sl, const GTLane<N>& L,
+template <int N, class SL>
+__device__ __forceinline__ void gt_band_columns(float2* v, float2* X, SL& sl, const GTLane<N>& L,
                                                 const float* maskc, const int* zcol) {
-  using G = GT<N, kGtThreads>;
+  using G = GT<N, kGtThreads<N>>;
   const RView<N, float, G::NT> mask(maskc, threadIdx.x);
   const float sc = 1.0f / float(N * N);
   const unsigned zm = gt_cols_zero<N>(zcol, L);
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     if ((zm >> r) & 1u) {
       gt_zero_col<N>(v, sl, L, r);
       continue;
@@ -1458,8 +1574,8 @@ __device__ __forceinline__ void gt_band_columns(float2* v, float2* X, const Slab
 
 // k_patch for 192/256 px windows (objective.py:436-441)
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
-  using G = GT<N, kGtThreads>;
+__global__ __launch_bounds__(kGtThreads<N>) void k_patch_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int planes = a.S * a.K;
@@ -1468,10 +1584,12 @@ __global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
   const int y0 = idx >= 0 ? a.pos[2 * idx] : -1, x0 = idx >= 0 ? a.pos[2 * idx + 1] : -1;
   const bool inside = window_inside(y0, x0, N, a.ny, a.nx);
   const GTLane<N> L;
-  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  SlabOf<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   float2 v[G::R1];
   const float2* obj = a.obj + size_t(plane) * a.ny * a.nx;
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     const int row = r * G::LINES + L.ln;
     const float2* src = obj + (inside ? size_t(y0 + row) * a.nx + x0 + L.j : 0);
 #pragma unroll
@@ -1479,11 +1597,13 @@ __global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
     rt_line_fwd<N, true>(v, X, L.ln, L.j);
     gt_store_rowspec<N>(v, sl, L, r);
   }
-  __syncthreads();
+  sl.to_cols(X, L);
   gt_band_columns<N>(v, X, sl, L, a.maskc, a.zcol);
-  __syncthreads();
+  sl.to_rows(X, L);
   const RView<N, float2, G::NT> dst(a.Ob + (size_t(b) * planes + plane) * N * N, threadIdx.x);
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     gt_load_rowspec<N>(v, sl, L, r);
     rt_line_inv<N, true>(v, X, L.ln, L.j);
 #pragma unroll
@@ -1493,8 +1613,8 @@ __global__ __launch_bounds__(kGtThreads) void k_patch_gt(BatchArgs a) {
 
 // k_forward for 192/256 px windows (multislice_core.py:9-82, objective.py:501-512)
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
-  using G = GT<N, kGtThreads>;
+__global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int MK = a.M * a.K;
@@ -1504,7 +1624,7 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
   const int m = mk / a.K, k = mk % a.K;
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
-  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  GSlab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
   // the shifted probes carry the beam current already
   const float bcs = a.shift ? 1.f : bc_scale(a, seq_index(a.seq, a.b0 + b, a.P));
@@ -1530,6 +1650,7 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
     const RView<N, float2, G::NT> U(st + size_t(keep ? slot : 0) * N * N, tid);
     const bool last_of_recompute = fin && s == a.s_hi - 1;
     for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
       if (s == a.s_lo && fin) {
         const RView<N, float2, G::NT> I(fin, tid);
 #pragma unroll
@@ -1558,12 +1679,13 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
       gt_store_rowspec<N>(v, sl, L, r);
     }
     if (last_of_recompute) break;
-    __syncthreads();
+    sl.to_cols(X, L);
     // column phase: forward column transform, then the propagator and the
     // inverse column transform, or the far-field band mask after the last slice
     if (s < a.s_hi - 1) {
       const RView<N, float2, G::NT> H((s == 0) ? a.H0 : a.H, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
         if ((zh >> r) & 1u) {
           gt_zero_col<N>(v, sl, L, r);
           continue;
@@ -1578,11 +1700,12 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
         rt_line_inv<N, true>(v, X, L.ln, L.j);
         gt_store_col<N>(v, sl, L, r);
       }
-      __syncthreads();
+      sl.to_rows(X, L);
     } else {
       const RView<N, float2, G::NT> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
       const RView<N, float, G::NT> mask(a.maskc, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
         if ((zm >> r) & 1u) {
 #pragma unroll
           for (int i = 0; i < G::R1; ++i) out.store(r * G::R1 + i, make_float2(0.f, 0.f));
@@ -1599,8 +1722,8 @@ __global__ __launch_bounds__(kGtThreads) void k_forward_gt(BatchArgs a) {
 
 // k_adjoint for 192/256 px windows (objective.py:595-597, multislice_core.py:85-200)
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
-  using G = GT<N, kGtThreads>;
+__global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int MK = a.M * a.K;
@@ -1610,7 +1733,7 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
   const int k = mk % a.K;
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
-  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  GSlab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   float2 v[G::R1];
   // a later segment left the wave of slice s_hi - 1 (read in the first row phase)
   const bool from_state = a.s_hi < a.S;
@@ -1624,6 +1747,7 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
     const RView<N, float, G::NT> c(a.coef + size_t(b) * N * N, tid);
     const float sc = 1.0f / float(N * N);
     for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
       if ((zh >> r) & 1u) {
         gt_zero_col<N>(v, sl, L, r);
         continue;
@@ -1633,7 +1757,7 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
       rt_line_inv<N, true>(v, X, L.ln, L.j);
       gt_store_col<N>(v, sl, L, r);
     }
-    __syncthreads();
+    sl.to_rows(X, L);
   }
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g; k_objgrad_gt sums the modes
@@ -1649,6 +1773,7 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
   for (int s = a.s_hi - 1; s >= a.s_lo; --s) {
     const RView<N, float2, G::NT> U(st + size_t(s - a.s_lo) * N * N, tid), O(ob + size_t(s) * a.K * N * N, tid);
     for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
       if (from_state && s == a.s_hi - 1) {
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) v[n] = GS[r * G::R1 + n];
@@ -1678,9 +1803,10 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
       }
     }
     if (s > 0) {
-      __syncthreads();
+      sl.to_cols(X, L);
       const RView<N, float2, G::NT> H((s == 1) ? a.H0 : a.H, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
         if ((zh >> r) & 1u) {
           gt_zero_col<N>(v, sl, L, r);
           continue;
@@ -1695,12 +1821,13 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
         rt_line_inv<N, true>(v, X, L.ln, L.j);
         gt_store_col<N>(v, sl, L, r);
       }
-      __syncthreads();
+      sl.to_rows(X, L);
     }
   }
   if (a.s_lo > 0) {
     // the wave propagated to slice s_lo - 1, as natural rows, for the next segment
     for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
       gt_load_rowspec<N>(v, sl, L, r);
       rt_line_inv<N, true>(v, X, L.ln, L.j);
 #pragma unroll
@@ -1712,8 +1839,8 @@ __global__ __launch_bounds__(kGtThreads) void k_adjoint_gt(BatchArgs a) {
 // k_objgrad for 192/256 px windows: mode sum from the R-order stash, band
 // limit, natural-order window gradient for k_gather_grad
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
-  using G = GT<N, kGtThreads>;
+__global__ __launch_bounds__(kGtThreads<N>) void k_objgrad_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const int planes = a.S * a.K;
@@ -1721,11 +1848,13 @@ __global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
   objgrad_plane(a, b, plane, sl_, k);
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
-  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  SlabOf<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   const size_t mstride = size_t(a.K) * a.st_slots * N * N;
   const float2* src = a.stash + ((size_t(b) * a.M * a.K + k) * a.st_slots + (sl_ - a.s_lo)) * N * N;
   float2 v[G::R1];
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     {
       const RView<N, float2, G::NT> G0(src, tid);
 #pragma unroll
@@ -1739,11 +1868,13 @@ __global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
     rt_line_fwd<N, true>(v, X, L.ln, L.j);
     gt_store_rowspec<N>(v, sl, L, r);
   }
-  __syncthreads();
+  sl.to_cols(X, L);
   gt_band_columns<N>(v, X, sl, L, a.maskc, a.zcol);
-  __syncthreads();
+  sl.to_rows(X, L);
   float2* Gout = (a.scatter ? a.Ob : a.Gbuf) + (size_t(b) * planes + plane) * N * N;
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     gt_load_rowspec<N>(v, sl, L, r);
     rt_line_inv<N, true>(v, X, L.ln, L.j);
     float2* out = Gout + size_t(r * G::LINES + L.ln) * N + L.j;
@@ -1760,25 +1891,29 @@ __global__ __launch_bounds__(kGtThreads) void k_objgrad_gt(BatchArgs a) {
 
 // probe spectra, slab C-order, unnormalised
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_fprobe_gt(const float2* __restrict__ probe,
+__global__ __launch_bounds__(kGtThreads<N>) void k_fprobe_gt(const float2* __restrict__ probe,
                                                           float2* __restrict__ fprobe, float2* __restrict__ scratch) {
-  using G = GT<N, kGtThreads>;
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const GTLane<N> L;
-  const Slab<N> sl(scratch + size_t(blockIdx.x) * N * N);
+  SlabOf<N> sl(scratch + size_t(blockIdx.x) * N * N);
   const float2* src = probe + size_t(blockIdx.x) * N * N;
   float2 v[G::R1];
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     const float2* p = src + size_t(r * G::LINES + L.ln) * N + L.j;
 #pragma unroll
     for (int n = 0; n < G::R1; ++n) v[n] = p[8 * n];
     rt_line_fwd<N, true>(v, X, L.ln, L.j);
     gt_store_rowspec<N>(v, sl, L, r);
   }
-  __syncthreads();
+  sl.to_cols(X, L);
   const RView<N, float2, G::NT> out(fprobe + size_t(blockIdx.x) * N * N, threadIdx.x);
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     gt_load_col<N>(v, sl, L, r);
     rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
@@ -1788,8 +1923,8 @@ __global__ __launch_bounds__(kGtThreads) void k_fprobe_gt(const float2* __restri
 
 // shifted probes (objective.py:471-476): ifft2(fprobe * E), natural order
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
-  using G = GT<N, kGtThreads>;
+__global__ __launch_bounds__(kGtThreads<N>) void k_probe_shift_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   __shared__ float2 exy[2 * N];
@@ -1798,7 +1933,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
   const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
-  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  SlabOf<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   shift_tables_freq<N>(exy, dy, dx);
   __syncthreads();
   const RView<N, float2, G::NT> fp(a.fprobe + size_t(m) * N * N, tid);
@@ -1807,7 +1942,9 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
   float2 v[G::R1];
   // E, and so the spectrum, is zero on the columns the band mask zeroes
   const unsigned zm = gt_cols_zero<N>(a.zcol, L);
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     if ((zm >> r) & 1u) {
       gt_zero_col<N>(v, sl, L, r);
       continue;
@@ -1823,9 +1960,11 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
     rt_line_inv<N, true>(v, X, L.ln, L.j);
     gt_store_col<N>(v, sl, L, r);
   }
-  __syncthreads();
+  sl.to_rows(X, L);
   float2* dst = a.Pp + (size_t(b) * a.M + m) * N * N;
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     gt_load_rowspec<N>(v, sl, L, r);
     rt_line_inv<N, true>(v, X, L.ln, L.j);
     float2* out = dst + size_t(r * G::LINES + L.ln) * N + L.j;
@@ -1840,8 +1979,8 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_shift_gt(BatchArgs a) {
 // the position gradient -2/n^2 sum Re(K conj(FP) Q) in FP64; without a shift
 // the natural-order gradient itself is added (slab R-order)
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
-  using G = GT<N, kGtThreads>;
+__global__ __launch_bounds__(kGtThreads<N>) void k_probegrad_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   __shared__ float2 exy[2 * N];
@@ -1849,7 +1988,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
   const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
-  const Slab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  GSlab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   const RView<N, float2, G::NT> acc(a.probe_part + (size_t(g) * a.M + m) * N * N, tid);
   const RView<N, float2, G::NT> fp(a.fprobe + size_t(m) * N * N, tid);
   const RView<N, float, G::NT> mask(a.maskc, tid);
@@ -1863,6 +2002,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
     if (!a.shift) {
       double sb = 0.0;
       for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
         const size_t row = size_t(r * G::LINES + L.ln) * N + L.j;
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) v[n] = gp[row + 8 * n];
@@ -1888,6 +2028,7 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
     __syncthreads();  // the previous window's slab and table reads are done
     shift_tables_freq<N>(exy, dy, dx);
     for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
       const size_t row = size_t(r * G::LINES + L.ln) * N + L.j;
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) v[n] = gp[row + 8 * n];
@@ -1897,9 +2038,10 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
       gt_store_rowspec<N>(v, sl, L, r);
     }
-    __syncthreads();  // the slab holds the row spectra; the table is published
+    sl.to_cols(X, L);  // the slab holds the row spectra; the table is published
     double sy = 0.0, sx = 0.0, sb = 0.0;
     for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
       if ((zm >> r) & 1u) continue;
       gt_load_col<N>(v, sl, L, r);
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
@@ -1945,30 +2087,34 @@ __global__ __launch_bounds__(kGtThreads) void k_probegrad_gt(BatchArgs a) {
 // probe_grad = ifft2(mask * fft2(sum)) (objective.py:927-933) from the reduced
 // partials: slab C-order spectrum (spectral) or slab R-order natural gradient
 template <int N>
-__global__ __launch_bounds__(kGtThreads) void k_probe_final_gt(const float2* __restrict__ sum, int spectral,
+__global__ __launch_bounds__(kGtThreads<N>) void k_probe_final_gt(const float2* __restrict__ sum, int spectral,
                                                                const float* __restrict__ maskc,
                                                                float2* __restrict__ probe_grad,
                                                                float2* __restrict__ scratch) {
-  using G = GT<N, kGtThreads>;
+  using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
-  const Slab<N> sl(scratch + size_t(blockIdx.x) * N * N);
+  SlabOf<N> sl(scratch + size_t(blockIdx.x) * N * N);
   const RView<N, float2, G::NT> src(sum + size_t(blockIdx.x) * N * N, tid);
   const RView<N, float, G::NT> mask(maskc, tid);
   const float s = 1.0f / float(N * N);
   float2 v[G::R1];
   if (!spectral) {
+#pragma unroll
     for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) v[n] = src[r * G::R1 + n];
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
       gt_store_rowspec<N>(v, sl, L, r);
     }
-    __syncthreads();
+    sl.to_cols(X, L);
   }
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     if (spectral) {
 #pragma unroll
       for (int i = 0; i < G::R1; ++i) v[i] = src[r * G::R1 + i];
@@ -1981,9 +2127,11 @@ __global__ __launch_bounds__(kGtThreads) void k_probe_final_gt(const float2* __r
     rt_line_inv<N, true>(v, X, L.ln, L.j);
     gt_store_col<N>(v, sl, L, r);
   }
-  __syncthreads();
+  sl.to_rows(X, L);
   float2* dst = probe_grad + size_t(blockIdx.x) * N * N;
+#pragma unroll
   for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<decltype(sl)>();
     gt_load_rowspec<N>(v, sl, L, r);
     rt_line_inv<N, true>(v, X, L.ln, L.j);
     float2* out = dst + size_t(r * G::LINES + L.ln) * N + L.j;
@@ -2492,7 +2640,7 @@ void launch_forward(const BatchArgs& a, hipStream_t s) {
   if constexpr (LAY == LAY_RT)
     k_forward_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
   else
-    k_forward_gt<N><<<grid, kGtThreads, 0, s>>>(a);
+    k_forward_gt<N><<<grid, kGtThreads<N>, 0, s>>>(a);
 }
 
 template <int N>
@@ -2502,7 +2650,7 @@ void launch_adjoint(const BatchArgs& a, hipStream_t s) {
   if constexpr (LAY == LAY_RT)
     k_adjoint_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
   else
-    k_adjoint_gt<N><<<grid, kGtThreads, 0, s>>>(a);
+    k_adjoint_gt<N><<<grid, kGtThreads<N>, 0, s>>>(a);
 }
 
 // one workgroup per (window, slice of [s_lo, s_hi), object mode)
@@ -2513,7 +2661,7 @@ void launch_objgrad(const BatchArgs& a, hipStream_t s) {
   if constexpr (LAY == LAY_RT)
     k_objgrad_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
   else
-    k_objgrad_gt<N><<<grid, kGtThreads, 0, s>>>(a);
+    k_objgrad_gt<N><<<grid, kGtThreads<N>, 0, s>>>(a);
 }
 
 // One batch of windows on one lane.  With checkpoint segments (ck > 0, C
@@ -2536,14 +2684,14 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
     if constexpr (LAY == LAY_RT)
       k_patch_rt<N><<<a.B * planes, RT<N>::NT, 0, s>>>(a);
     else
-      k_patch_gt<N><<<a.B * planes, kGtThreads, 0, s>>>(a);
+      k_patch_gt<N><<<a.B * planes, kGtThreads<N>, 0, s>>>(a);
   }));
   if (a.shift)
     PTY_TRY(x.launch(KT_PROBE_SHIFT, lane, a, [&](hipStream_t s) {
       if constexpr (LAY == LAY_RT)
         k_probe_shift_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
       else
-        k_probe_shift_gt<N><<<a.B * a.M, kGtThreads, 0, s>>>(a);
+        k_probe_shift_gt<N><<<a.B * a.M, kGtThreads<N>, 0, s>>>(a);
     }));
   PTY_TRY(x.launch(KT_FORWARD, lane, a, [&](hipStream_t s) { launch_forward<N>(a, s); }));
   if (a.prep)
@@ -2590,7 +2738,7 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
       if constexpr (LAY == LAY_RT)
         k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
       else
-        k_probegrad_gt<N><<<a.G * a.M, kGtThreads, 0, s>>>(a);
+        k_probegrad_gt<N><<<a.G * a.M, kGtThreads<N>, 0, s>>>(a);
     }));
   if (p->step_pos && a.shift)
     PTY_TRY(x.launch(KT_POS_FINAL, lane, a, [&](hipStream_t s) { k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
@@ -2679,7 +2827,7 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     if constexpr (LAY == LAY_RT)
       k_fprobe_rt<N><<<M, RT<N>::NT, 0, s>>>(a.probe, const_cast<float2*>(a.fprobe));
     else
-      k_fprobe_gt<N><<<M, kGtThreads, 0, s>>>(a.probe, const_cast<float2*>(a.fprobe), a.scratch);
+      k_fprobe_gt<N><<<M, kGtThreads<N>, 0, s>>>(a.probe, const_cast<float2*>(a.fprobe), a.scratch);
   }));
   if (p->obj_layout == PTY_LAYOUT_REFERENCE)
     PTY_TRY(x.launch(KT_LAYOUT, 0, a, [&](hipStream_t s) {
@@ -2741,7 +2889,7 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
         k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, p->shift_active, a.maskc,
                                                      static_cast<float2*>(bf->probe_grad));
       else
-        k_probe_final_gt<N><<<M, kGtThreads, 0, s>>>(psum, p->shift_active, a.maskc,
+        k_probe_final_gt<N><<<M, kGtThreads<N>, 0, s>>>(psum, p->shift_active, a.maskc,
                                                      static_cast<float2*>(bf->probe_grad), a.scratch);
     }));
   }
