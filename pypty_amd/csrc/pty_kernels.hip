@@ -1511,7 +1511,11 @@ using SlabOf = std::conditional_t<N == 192, RegSlab, GSlab<N>>;
 // both rounds' temporaries on top of the tile's 96 registers.
 template <class SL>
 __device__ __forceinline__ void round_fence() {
-  if constexpr (std::is_same_v<SL, RegSlab>) __builtin_amdgcn_sched_barrier(0);
+  if constexpr (std::is_same_v<SL, RegSlab>) {
+    // nor may a round's table or object reads move up into the previous round
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+  }
 }
 
 template <int N, class SL>
@@ -1637,8 +1641,10 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
   // round (or the next slice's first round) are read while this round's
   // forward row transform runs, and the propagator is read before the column
   // transform that precedes its use
+  // at 192 px (768 threads, 168 registers) reading ahead costs spills
+  constexpr bool kAhead = N != 192;
   float2 v[G::R1], o[G::R1];
-  {
+  if constexpr (kAhead) {
     const RView<N, float2, G::NT> O(ob + size_t(a.s_lo) * a.K * N * N, tid);
 #pragma unroll
     for (int n = 0; n < G::R1; ++n) o[n] = O[n];
@@ -1663,6 +1669,11 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
         gt_load_rowspec<N>(v, sl, L, r);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
       }
+      if constexpr (!kAhead) {
+        const RView<N, float2, G::NT> O(ob + size_t(s) * a.K * N * N, tid);
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) o[n] = O[r * G::R1 + n];
+      }
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) {
         if (keep) U.store(r * G::R1 + n, v[n]);
@@ -1670,7 +1681,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
       }
       if (last_of_recompute) continue;  // only the stash of this slice is wanted
       const int rn = (r + 1 < G::ROUNDS) ? r + 1 : 0, sn = (r + 1 < G::ROUNDS) ? s : s + 1;
-      if (sn < a.s_hi) {
+      if (kAhead && sn < a.s_hi) {
         const RView<N, float2, G::NT> O(ob + size_t(sn) * a.K * N * N, tid);
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) o[n] = O[rn * G::R1 + n];
@@ -1764,8 +1775,10 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
   // pu holds the stash registers of the current round, read one round ahead
   float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  // at 192 px (768 threads, 168 registers) reading ahead costs spills
+  constexpr bool kAhead = N != 192;
   float2 pu[G::R1];
-  if (a.step_obj) {
+  if (kAhead && a.step_obj) {
     const RView<N, float2, G::NT> U(st + size_t(a.s_hi - 1 - a.s_lo) * N * N, tid);
 #pragma unroll
     for (int n = 0; n < G::R1; ++n) pu[n] = U[n];
@@ -1781,6 +1794,10 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
         gt_load_rowspec<N>(v, sl, L, r);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
       }
+      if (!kAhead && a.step_obj) {
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) pu[n] = U[r * G::R1 + n];
+      }
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) {
         const int q = r * G::R1 + n;
@@ -1788,7 +1805,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
         v[n] = cmulc(v[n], O[q]);
       }
       const int rn = (r + 1 < G::ROUNDS) ? r + 1 : 0, sn = (r + 1 < G::ROUNDS) ? s : s - 1;
-      if (a.step_obj && sn >= a.s_lo) {
+      if (kAhead && a.step_obj && sn >= a.s_lo) {
         const RView<N, float2, G::NT> Un(st + size_t(sn - a.s_lo) * N * N, tid);
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) pu[n] = Un[rn * G::R1 + n];
