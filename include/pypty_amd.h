@@ -260,6 +260,14 @@ int pty_axpby_c64(void* y, const void* x, int64_t n, float alpha, float beta, vo
 int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float phase_weight, void* grad,
                           double* out, double* partial, void* stream);
 
+/* Centre of mass of each of `count` patterns (count, ny, nx) float32 about the
+ * pattern centre, the first step of the DPC start (pypty/dpc.py:139-150,
+ * fft_based_dpc): comx[i] = sum (x - (nx-1)/2) I_i, comy[i] likewise in y
+ * (negated with flip_y, dpc.py:133-134), FP64 sums; comx, comy device arrays
+ * of count doubles. */
+int pty_center_of_mass(const float* patterns, int64_t count, int ny, int nx, int flip_y, double* comx, double* comy,
+                       void* stream);
+
 /* out = 2 Re(-i conj(z) g): Wirtinger gradient -> phase gradient
  * (pypty/utils.py:260-276, complex_grad_to_phase_grad); out float32, n elements. */
 int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream);

@@ -18,6 +18,7 @@
 // inverse column transforms, with each position mapped to its frequency.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstdio>
 
 #include "pty_fft.hpp"
@@ -32,6 +33,7 @@ constexpr int kMaxStages = 16;
 struct LinePlan {
   int L;
   int nst;
+  int direct;  // a stage of a prime radix above kMaxGenericRadix (computed through an LDS copy)
   int r[kMaxStages];
 };
 
@@ -72,6 +74,16 @@ bool make_plan(int L, LinePlan& p) {
   while (m % 2 == 0) {
     p.r[p.nst++] = 2;
     m /= 2;
+  }
+  // larger primes: a direct DFT stage each
+  p.direct = 0;
+  for (int q = kMaxGenericRadix + 2; m > 1 && p.nst < kMaxStages; q += 2) {
+    if (q * q > m) q = m;  // m is prime
+    while (m % q == 0 && p.nst < kMaxStages) {
+      p.r[p.nst++] = q;
+      m /= q;
+      p.direct = 1;
+    }
   }
   return m == 1 && p.nst <= kMaxStages && L >= 2;
 }
@@ -137,16 +149,51 @@ __device__ __noinline__ void stage_bfly_generic(float2* x, int es, int base, int
   for (int k = 0; k < r; ++k) x[(base + s * k) * es] = out[k];
 }
 
-// Transform `nlines` lines in LDS; line l starts at buf + l*lstride and has
-// element stride es.  All threads of the block take part.
+// butterflies of a prime radix r above kMaxGenericRadix: every thread forms one
+// output of one butterfly from the r inputs in LDS (with the stage's twiddles),
+// into tmp (the lines' layout), then the lines are copied back.  All threads
+// of the block take part.
 template <bool INV>
-__device__ void lines_fft(float2* buf, int nlines, int lstride, int es, const LinePlan& pl) {
+__device__ void stage_direct(float2* buf, float2* tmp, int nlines, int lstride, int es, int L, int r, int s, int B) {
+  const int work = nlines * L;
+  for (int w = threadIdx.x; w < work; w += blockDim.x) {
+    const int line = w / L, q = w % L;
+    const int bfly = q / r, k = q % r;
+    const int blk = bfly / s, j = bfly % s;
+    const int base = blk * B + j;
+    const float2* x = buf + line * lstride;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int n = 0; n < r; ++n) {
+      float2 v = x[(base + s * n) * es];
+      if (INV && n > 0) v = cmul(v, twiddle_rt(j * n, B, true));
+      acc = cadd(acc, cmul(v, twiddle_rt((n * k) % r, r, INV)));
+    }
+    if (!INV && k > 0) acc = cmul(acc, twiddle_rt(j * k, B, false));
+    tmp[line * lstride + (base + s * k) * es] = acc;
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < work; w += blockDim.x) {
+    const int line = w / L, q = w % L;
+    buf[line * lstride + q * es] = tmp[line * lstride + q * es];
+  }
+}
+
+// Transform `nlines` lines in LDS; line l starts at buf + l*lstride and has
+// element stride es.  All threads of the block take part.  tmp: LDS of the
+// lines' size, used by the stages of primes above kMaxGenericRadix.
+template <bool INV>
+__device__ void lines_fft(float2* buf, int nlines, int lstride, int es, const LinePlan& pl, float2* tmp) {
   const int L = pl.L;
   for (int tt = 0; tt < pl.nst; ++tt) {
     const int t = INV ? pl.nst - 1 - tt : tt;
     int B = L;
     for (int u = 0; u < t; ++u) B /= pl.r[u];
     const int r = pl.r[t], s = B / r;
+    if (r > kMaxGenericRadix) {
+      stage_direct<INV>(buf, tmp, nlines, lstride, es, L, r, s, B);
+      __syncthreads();
+      continue;
+    }
     const int nb = L / r;  // butterflies per line
     const int work = nb * nlines;
     for (int w = threadIdx.x; w < work; w += blockDim.x) {
@@ -179,7 +226,7 @@ __global__ __launch_bounds__(256) void k_plane_rows(float2* __restrict__ data, i
   float2* g = data + (size_t(plane) * ny + y0) * nx;
   for (int i = threadIdx.x; i < nr * nx; i += blockDim.x) lds[i] = g[i];
   __syncthreads();
-  lines_fft<INV>(lds, nr, nx, 1, pl);
+  lines_fft<INV>(lds, nr, nx, 1, pl, lds + size_t(rows) * nx);
   for (int i = threadIdx.x; i < nr * nx; i += blockDim.x) g[i] = cscale(lds[i], scale);
 }
 
@@ -200,7 +247,7 @@ __global__ __launch_bounds__(256) void k_plane_cols_band(float2* __restrict__ da
     lds[y * cols + c] = g[size_t(y) * nx + c];
   }
   __syncthreads();
-  lines_fft<false>(lds, nc, 1, cols, py);
+  lines_fft<false>(lds, nc, 1, cols, py, lds + size_t(ny) * cols);
   const double max_r = 0.5 * cutoff;
   const double r0 = (0.5 * (cutoff - rolloff)) * (0.5 * (cutoff - rolloff));
   for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
@@ -217,7 +264,7 @@ __global__ __launch_bounds__(256) void k_plane_cols_band(float2* __restrict__ da
     lds[y * cols + c] = cscale(lds[y * cols + c], m);
   }
   __syncthreads();
-  lines_fft<true>(lds, nc, 1, cols, py);
+  lines_fft<true>(lds, nc, 1, cols, py, lds + size_t(ny) * cols);
   for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
     const int y = i / nc, c = i % nc;
     g[size_t(y) * nx + c] = lds[y * cols + c];
@@ -242,7 +289,7 @@ __global__ __launch_bounds__(256) void k_plane_cols(const float2* __restrict__ s
       lds[y * cols + c] = g[size_t(y) * nx + x0 + c];
     }
     __syncthreads();
-    lines_fft<false>(lds, nc, 1, cols, py);
+    lines_fft<false>(lds, nc, 1, cols, py, lds + size_t(ny) * cols);
     for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
       const int y = i / nc, c = i % nc;
       int fy = plan_freq(py, y), fx = plan_freq(px, x0 + c);
@@ -264,7 +311,7 @@ __global__ __launch_bounds__(256) void k_plane_cols(const float2* __restrict__ s
       lds[y * cols + c] = g[size_t(fy) * nx + fx];
     }
     __syncthreads();
-    lines_fft<true>(lds, nc, 1, cols, py);
+    lines_fft<true>(lds, nc, 1, cols, py, lds + size_t(ny) * cols);
     for (int i = threadIdx.x; i < ny * nc; i += blockDim.x) {
       const int y = i / nc, c = i % nc;
       o[size_t(y) * nx + x0 + c] = cscale(lds[y * cols + c], scale);
@@ -381,20 +428,58 @@ inline int plane_launch_check() {
 
 constexpr size_t kPlaneLds = 64 * 1024;
 
-int rows_per_block(int nx) { return int(std::max<size_t>(1, kPlaneLds / (size_t(nx) * 8))); }
-int cols_per_block(int ny) {
-  int c = int(kPlaneLds / (size_t(ny) * 8));
+// lines per workgroup; a plan with a direct stage needs a second copy of the lines in LDS
+int rows_per_block(int nx, const LinePlan& p) {
+  return int(std::max<size_t>(1, kPlaneLds / (size_t(nx) * 8 * (p.direct ? 2 : 1))));
+}
+int cols_per_block(int ny, const LinePlan& p) {
+  int c = int(kPlaneLds / (size_t(ny) * 8 * (p.direct ? 2 : 1)));
   if (c >= 16) return 16;
   if (c >= 8) return 8;
   return std::max(c, 1);
 }
+size_t lds_bytes(int lines, int len, const LinePlan& p) { return size_t(lines) * len * 8 * (p.direct ? 2 : 1); }
 
 int plane_checks(void* data, int64_t planes, int ny, int nx, LinePlan& py, LinePlan& px) {
   if (!data || planes < 1 || ny < 2 || nx < 2) return PTY_ERR_ARG;
   if (!make_plan(ny, py) || !make_plan(nx, px)) return PTY_ERR_SIZE;
-  if (size_t(nx) * 8 > kPlaneLds || size_t(ny) * 8 > kPlaneLds) return PTY_ERR_SIZE;
+  if (size_t(nx) * 8 * (px.direct ? 2 : 1) > kPlaneLds || size_t(ny) * 8 * (py.direct ? 2 : 1) > kPlaneLds)
+    return PTY_ERR_SIZE;
   if (planes * int64_t((ny + 1)) > (int64_t(1) << 31)) return PTY_ERR_ARG;
   return PTY_OK;
+}
+
+// Centre of mass of each pattern about the pattern's centre (pypty/dpc.py:139-150):
+// comx = sum_x,y (x - (nx-1)/2) I(y, x), comy likewise in y (negated when the
+// pattern's y axis is flipped, dpc.py:133-134), in FP64 as numpy forms them.
+// One workgroup per pattern.
+__global__ __launch_bounds__(256) void k_center_of_mass(const float* __restrict__ p, int ny, int nx, int flip_y,
+                                                        double* __restrict__ comx, double* __restrict__ comy) {
+  const float* d = p + size_t(blockIdx.x) * ny * nx;
+  const double cx = 0.5 * double(nx - 1), cy = 0.5 * double(ny - 1);
+  double sx = 0.0, sy = 0.0;
+  const int n = ny * nx;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int y = e / nx, x = e - y * nx;
+    const double v = double(d[e]);
+    sx += (double(x) - cx) * v;
+    sy += (double(y) - cy) * v;
+  }
+  __shared__ double rx[256], ry[256];
+  rx[threadIdx.x] = sx;
+  ry[threadIdx.x] = sy;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) {
+      rx[threadIdx.x] += rx[threadIdx.x + o];
+      ry[threadIdx.x] += ry[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    comx[blockIdx.x] = rx[0];
+    comy[blockIdx.x] = flip_y ? -ry[0] : ry[0];
+  }
 }
 
 }  // namespace
@@ -407,11 +492,11 @@ int pty_fourier_clean_c64(void* data, int64_t planes, int ny, int nx, double cut
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   float2* d = static_cast<float2*>(data);
-  const int rows = rows_per_block(nx), cols = cols_per_block(ny);
+  const int rows = rows_per_block(nx, px), cols = cols_per_block(ny, py);
   const int gr = int(planes) * ((ny + rows - 1) / rows), gc = int(planes) * ((nx + cols - 1) / cols);
-  k_plane_rows<false><<<gr, 256, size_t(rows) * nx * 8, s>>>(d, ny, nx, rows, px, 1.0f);
-  k_plane_cols_band<<<gc, 256, size_t(cols) * ny * 8, s>>>(d, ny, nx, cols, py, px, cutoff, rolloff);
-  k_plane_rows<true><<<gr, 256, size_t(rows) * nx * 8, s>>>(d, ny, nx, rows, px, 1.0f / (float(ny) * float(nx)));
+  k_plane_rows<false><<<gr, 256, lds_bytes(rows, nx, px), s>>>(d, ny, nx, rows, px, 1.0f);
+  k_plane_cols_band<<<gc, 256, lds_bytes(cols, ny, py), s>>>(d, ny, nx, cols, py, px, cutoff, rolloff);
+  k_plane_rows<true><<<gr, 256, lds_bytes(rows, nx, px), s>>>(d, ny, nx, rows, px, 1.0f / (float(ny) * float(nx)));
   return plane_launch_check();
 }
 
@@ -424,18 +509,18 @@ int pty_plane_fft2_c64(void* data, void* scratch, int64_t planes, int ny, int nx
   hipStream_t s = static_cast<hipStream_t>(stream);
   float2* d = static_cast<float2*>(data);
   float2* w = static_cast<float2*>(scratch);
-  const int rows = rows_per_block(nx), cols = cols_per_block(ny);
+  const int rows = rows_per_block(nx, px), cols = cols_per_block(ny, py);
   const int gr = int(planes) * ((ny + rows - 1) / rows), gc = int(planes) * ((nx + cols - 1) / cols);
   const bool inv = (mode == PTY_FFT_INV || mode == PTY_FFT_INV_ISHIFT);
   const int shift = (mode == PTY_FFT_FWD_SHIFT || mode == PTY_FFT_INV_ISHIFT) ? 1 : 0;
   if (!inv) {
     // rows (digit-reversed x) in place, then columns scattered to natural order into scratch, copy back
-    k_plane_rows<false><<<gr, 256, size_t(rows) * nx * 8, s>>>(d, ny, nx, rows, px, 1.0f);
-    k_plane_cols<false><<<gc, 256, size_t(cols) * ny * 8, s>>>(d, w, ny, nx, cols, py, px, shift, 1.0f);
+    k_plane_rows<false><<<gr, 256, lds_bytes(rows, nx, px), s>>>(d, ny, nx, rows, px, 1.0f);
+    k_plane_cols<false><<<gc, 256, lds_bytes(cols, ny, py), s>>>(d, w, ny, nx, cols, py, px, shift, 1.0f);
   } else {
     // gather natural spectrum into digit-reversed columns (scratch), then inverse rows
-    k_plane_cols<true><<<gc, 256, size_t(cols) * ny * 8, s>>>(d, w, ny, nx, cols, py, px, shift, 1.0f);
-    k_plane_rows<true><<<gr, 256, size_t(rows) * nx * 8, s>>>(w, ny, nx, rows, px, 1.0f / (float(ny) * float(nx)));
+    k_plane_cols<true><<<gc, 256, lds_bytes(cols, ny, py), s>>>(d, w, ny, nx, cols, py, px, shift, 1.0f);
+    k_plane_rows<true><<<gr, 256, lds_bytes(rows, nx, px), s>>>(w, ny, nx, rows, px, 1.0f / (float(ny) * float(nx)));
   }
   const hipError_t e = hipMemcpyAsync(d, w, size_t(planes) * ny * nx * 8, hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return PTY_ERR_HIP;
@@ -476,6 +561,15 @@ int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float ph
   k_l1<<<cols, 256, 0, s>>>(static_cast<const float2*>(obj), size_t(n), abs_weight, phase_weight,
                             static_cast<float2*>(grad), partial);
   k_reduce_rows<<<1, 256, 0, s>>>(partial, cols, out);
+  return plane_launch_check();
+}
+
+int pty_center_of_mass(const float* patterns, int64_t count, int ny, int nx, int flip_y, double* comx, double* comy,
+                       void* stream) {
+  if (!patterns || !comx || !comy || count < 0 || count > INT32_MAX || ny < 1 || nx < 1) return PTY_ERR_ARG;
+  if (count == 0) return PTY_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  k_center_of_mass<<<int(count), 256, 0, s>>>(patterns, ny, nx, flip_y ? 1 : 0, comx, comy);
   return plane_launch_check();
 }
 

@@ -359,11 +359,32 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
 
 # ----------------------------------------------------------------- driver
 def _per_epoch(v, epoch):
-    """PyPty's "lambda-type" parameters (pypty/utils.py:2260-2285): callables of the epoch.  Strings are
-    not evaluated (the reference eval()s them); pass a callable instead."""
-    if isinstance(v, str):
-        raise TypeError("pypty_amd.run: string-valued per-epoch parameters are not evaluated; pass a callable")
+    """PyPty's "lambda-type" parameters (pypty/utils.py:2260-2285, 2366-2425): callables of the epoch,
+    or lambda source text (as params.pkl stores them), which ``utils.string_to_lambda`` turns into a
+    callable without eval; other values, text included, are used as they are."""
+    v = utils.string_to_lambda(v)
     return v(epoch) if callable(v) else v
+
+
+def _save_epoch_files(folder, epoch, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None):
+    """checkpoint_{beam_current,obj,probe,positions,stback}_epoch_N.npy of the variables refined in
+    epoch N, as pypty/utils.py:1942-1975 writes them when save_checkpoints_every_epoch selects N."""
+    import os
+
+    if not folder:
+        return
+    os.makedirs(folder, exist_ok=True)
+    name = lambda what: os.path.join(folder, f"checkpoint_{what}_epoch_{epoch}.npy")  # noqa: E731
+    if isinstance(bc, torch.Tensor):
+        np.save(name("beam_current"), bc.cpu().numpy())
+    if save_obj:
+        np.save(name("obj"), o.cpu().numpy())
+    if save_probe:
+        np.save(name("probe"), p.cpu().numpy())
+    if save_pos:
+        np.save(name("positions"), (pos.double() + corr.double()).cpu().numpy())
+    if isinstance(bg, torch.Tensor):
+        np.save(name("stback"), bg.cpu().numpy())
 
 
 def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None):
@@ -433,6 +454,15 @@ def save_params(path, params, strip_dataset=True):
 LOSS_CSV_FIELDS = ["epoch", "time / s", "loss", "sse", "initial step", "matching step", "N linesearch iterations",
                    "dir. derivative", "new dir. derivative", "Constraints contribution", "Free GiB", "Total GiB",
                    "Warnings"]
+
+
+def _start_loss_csv(folder):
+    """A new loss.csv holding the header row (pypty/utils.py:2004-2013)."""
+    import csv
+    import os
+
+    with open(os.path.join(folder, "loss.csv"), "w+", newline="") as fh:
+        csv.DictWriter(fh, fieldnames=LOSS_CSV_FIELDS).writeheader()
 
 
 def _log_loss(folder, row):
@@ -524,9 +554,14 @@ def run(pypty_params):
     seq0 = np.arange(dataset.shape[0]) if seq_param is None else np.asarray(_per_epoch(seq_param, 0))
     # positions: shift to the origin, integer corner + sub-pixel correction, pad the object
     # (pypty/utils.py:1722-1764)
+    # skip_preprocessing (set by initialize.prepare_restart) keeps the positions as given and adds no
+    # margin: the object of a restart carries its margin already (utils.py:1722-1728)
     extra = int(get("extra_space_on_side_px", 0))
-    pos[:, 0] -= pos[:, 0].min()
-    pos[:, 1] -= pos[:, 1].min()
+    if get("skip_preprocessing", False):
+        extra = 0
+    else:
+        pos[:, 0] -= pos[:, 0].min()
+        pos[:, 1] -= pos[:, 1].min()
     pos += extra
     if extra > 0:
         o = np.pad(o, [[extra, 0], [extra, 0], [0, 0], [0, 0]], constant_values=np.mean(o))
@@ -554,12 +589,16 @@ def run(pypty_params):
     folder = get("output_folder", "")
     print_flag = get("print_flag", 3)
     save_inter = get("save_inter_checkpoints", True)
+    save_every = get("save_checkpoints_every_epoch", False)
     save_log = get("save_loss_log", 1)
+    epoch_prev = int(get("epoch_prev", 0))
     if save_log == 2:
         raise NotImplementedError("pypty_amd.run: save_loss_log=2 (regulariser columns) is not supported")
     if folder:
         os.makedirs(folder, exist_ok=True)
         save_params(os.path.join(folder, "params.pkl"), pypty_params, get("strip_dataset_from_params", True))
+        if save_log and epoch_prev == 0:
+            _start_loss_csv(folder)  # a restart appends to the log it continues (utils.py:2004-2013)
     _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
     reset_bfgs_history()
     t0 = time.time()
@@ -570,7 +609,7 @@ def run(pypty_params):
     is_first = True
     step_param = get("update_step_bfgs", 1)  # first line-search step, a value or a callable of the epoch
     epoch_callback = get("epoch_callback", None)
-    for epoch in range(int(get("epoch_prev", 0)), int(get("epoch_max", 200))):
+    for epoch in range(epoch_prev, int(get("epoch_max", 200))):
         warnings = ""
         s_obj = _per_epoch(get("update_obj", 1), epoch)
         s_probe = _per_epoch(get("update_probe", 1), epoch)
@@ -617,7 +656,7 @@ def run(pypty_params):
         steps = sum(int(x != 0) for x in (s_bc, s_obj, s_probe, s_pos, s_bg))
         constraints = sum(int(x != 0) for x in (lw, w_phase, w_abs, *[cw[k] for k in cw if k not in (
             "mixed_variance_sigma", "beta_wedge", "probe_mod_tol")], cw["mixed_variance_sigma"]))
-        if epoch != int(get("epoch_prev", 0)) and (steps != prev_steps or constraints != prev_constraints):
+        if epoch != epoch_prev and (steps != prev_steps or constraints != prev_constraints):
             reset_bfgs_history()
             warnings += "\nResetting the history!"
         prev_steps, prev_constraints = steps, constraints
@@ -670,6 +709,9 @@ def run(pypty_params):
         if save_inter and epoch % int(save_inter) == 0:
             _save_checkpoint(folder, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
                              static_background if s_bg else None, beam_current if s_bc else None)
+        if save_every and epoch % int(save_every) == 0:
+            _save_epoch_files(folder, epoch, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
+                              static_background if s_bg else None, beam_current if s_bc else None)
         if save_log:
             free, total = torch.cuda.mem_get_info(dev)
             _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,

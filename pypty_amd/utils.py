@@ -160,3 +160,87 @@ def try_to_initialize_beam_current(beam_current, measured_data_shape, default_fl
         w = n - bc.shape[0]
         bc = torch.nn.functional.pad(bc, (w, w), value=1.0)
     return bc.contiguous()
+
+
+_SAFE_FUNCS = {"abs": abs, "min": min, "max": max, "int": int, "float": float, "round": round, "bool": bool}
+
+
+def string_to_lambda(lambda_string):
+    """A "lambda-type" parameter stored as text (pypty/utils.py:2366-2382; params.pkl keeps lambdas as
+    source, utils.py:2315-2364) turned back into a callable without eval: the text must be a
+    lambda of numeric arguments whose body uses only numbers, its arguments, arithmetic, comparison,
+    boolean and conditional expressions, tuples and lists, and abs/min/max/int/float/round/bool.
+    Text that is not a lambda is returned unchanged, as the reference returns what eval() rejects; a
+    lambda outside that set raises ValueError (pass a callable instead)."""
+    import ast
+    import operator as op
+
+    if not isinstance(lambda_string, str) or "lambda" not in lambda_string:
+        return lambda_string
+    try:
+        tree = ast.parse(lambda_string.strip(), mode="eval")
+    except SyntaxError:
+        return lambda_string
+    if not isinstance(tree.body, ast.Lambda):
+        return lambda_string
+    lam = tree.body
+    names = [a.arg for a in lam.args.args]
+    if lam.args.vararg or lam.args.kwarg or lam.args.kwonlyargs or lam.args.defaults:
+        raise ValueError(f"pypty_amd: unsupported lambda signature in {lambda_string!r}")
+    bin_ops = {ast.Add: op.add, ast.Sub: op.sub, ast.Mult: op.mul, ast.Div: op.truediv, ast.FloorDiv: op.floordiv,
+               ast.Mod: op.mod, ast.Pow: op.pow}
+    cmp_ops = {ast.Eq: op.eq, ast.NotEq: op.ne, ast.Lt: op.lt, ast.LtE: op.le, ast.Gt: op.gt, ast.GtE: op.ge}
+
+    def ev(n, env):
+        if isinstance(n, ast.Constant) and isinstance(n.value, (int, float, bool, type(None))):
+            return n.value
+        if isinstance(n, ast.Name) and n.id in env:
+            return env[n.id]
+        if isinstance(n, ast.BinOp) and type(n.op) in bin_ops:
+            return bin_ops[type(n.op)](ev(n.left, env), ev(n.right, env))
+        if isinstance(n, ast.UnaryOp) and isinstance(n.op, (ast.USub, ast.UAdd, ast.Not)):
+            v = ev(n.operand, env)
+            return -v if isinstance(n.op, ast.USub) else (+v if isinstance(n.op, ast.UAdd) else not v)
+        if isinstance(n, ast.Compare) and all(type(o) in cmp_ops for o in n.ops):
+            left = ev(n.left, env)
+            for o, c in zip(n.ops, n.comparators):
+                right = ev(c, env)
+                if not cmp_ops[type(o)](left, right):
+                    return False
+                left = right
+            return True
+        if isinstance(n, ast.BoolOp):
+            vals = (ev(v, env) for v in n.values)
+            return all(vals) if isinstance(n.op, ast.And) else any(vals)
+        if isinstance(n, ast.IfExp):
+            return ev(n.body, env) if ev(n.test, env) else ev(n.orelse, env)
+        if isinstance(n, (ast.Tuple, ast.List)):
+            vals = [ev(v, env) for v in n.elts]
+            return tuple(vals) if isinstance(n, ast.Tuple) else vals
+        if (isinstance(n, ast.Call) and isinstance(n.func, ast.Name) and n.func.id in _SAFE_FUNCS
+                and not n.keywords):
+            return _SAFE_FUNCS[n.func.id](*[ev(a, env) for a in n.args])
+        raise ValueError(f"pypty_amd: {lambda_string!r} uses {type(n).__name__}, which is not evaluated; "
+                         "pass a callable")
+
+    ev_check = lam.body
+
+    def f(*args):
+        if len(args) != len(names):
+            raise TypeError(f"{lambda_string!r} takes {len(names)} argument(s)")
+        return ev(ev_check, dict(zip(names, args)))
+
+    # reject unsupported constructs now rather than at the first call
+    allowed = (ast.BinOp, ast.UnaryOp, ast.Compare, ast.BoolOp, ast.IfExp, ast.Tuple, ast.List, ast.Constant,
+               ast.Name, ast.Call, ast.Load, ast.operator, ast.unaryop, ast.cmpop, ast.boolop)
+    for node in ast.walk(lam.body):
+        bad = not isinstance(node, allowed)
+        bad = bad or (isinstance(node, ast.Name) and node.id not in names and node.id not in _SAFE_FUNCS)
+        bad = bad or (isinstance(node, ast.Call) and (not isinstance(node.func, ast.Name) or node.keywords
+                                                      or node.func.id not in _SAFE_FUNCS))
+        bad = bad or (isinstance(node, ast.Constant) and not isinstance(node.value, (int, float, bool, type(None))))
+        if bad:
+            raise ValueError(f"pypty_amd: {lambda_string!r} uses {type(node).__name__}, which is not evaluated; "
+                             "pass a callable")
+    f.__doc__ = lambda_string
+    return f

@@ -89,7 +89,7 @@ def test_loss_and_direction_matches_oracle(case):
 
 
 @pytest.mark.parametrize("shape", [(192, 192), (256, 256), (96, 128), (512, 512), (384, 1536),
-                                   (160, 160), (350, 210), (143, 242), (2000, 35)])
+                                   (160, 160), (350, 210), (143, 242), (2000, 35), (94, 158), (97, 202)])
 @pytest.mark.parametrize("mode", ["fft2", "ifft2", "shift_fft2", "ifft2_ishift"])
 def test_plane_fft_modes_match_numpy(shape, mode):
     torch = _torch()
@@ -109,7 +109,9 @@ def test_plane_fft_modes_match_numpy(shape, mode):
                                                   ((1536, 1536, 1, 1), 2 / 3 - 0.005, 0.0),
                                                   ((192, 192, 1, 1), 0.5, 0.1),
                                                   ((160, 160, 1, 1), 2 / 3 - 0.005, 0.0),
-                                                  ((350, 210, 2, 1), 0.6, 0.05)])
+                                                  ((350, 210, 2, 1), 0.6, 0.05),
+                                                  ((94, 94, 1, 1), 2 / 3 - 0.005, 0.0),
+                                                  ((97, 158, 2, 1), 0.5, 0.0)])
 def test_fourier_clean_matches_oracle(shape, cutoff, rolloff):
     torch = _torch()
     from pypty_amd import utils as putils

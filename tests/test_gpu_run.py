@@ -47,6 +47,9 @@ def test_run_reconstructs_and_writes_reference_files(tmp_path):
     assert co.shape[2:] == (1, 1) and co.dtype == np.complex64
     assert np.load(out + "cg.npy").shape == (144, 2)
     rows = list(csv.reader(open(out + "loss.csv")))
+    # the header row, then one row per epoch (pypty/utils.py:2004-2013, 1877-1940)
+    assert rows[0][:3] == ["epoch", "time / s", "loss"]
+    rows = rows[1:]
     assert len(rows) == 8 and len(rows[0]) == 13
 
 
@@ -137,7 +140,7 @@ def test_run_resets_the_history_when_an_l1_weight_switches_on(tmp_path):
         iterative.run(dict(params, epoch_callback=cb))
     finally:
         iterative.reset_bfgs_history = real_reset
-    rows = list(csv.reader(open(out + "loss.csv")))
+    rows = list(csv.reader(open(out + "loss.csv")))[1:]
     assert len(rows) == 6
     cons = [float(r[9]) for r in rows]
     print(f"  constraints column {cons}; history resets before epochs {resets}", flush=True)
