@@ -63,13 +63,14 @@ def test_argument_errors_need_no_device(lib):
     assert size.value > 0
     b = _lib.PtyBuffers()
     assert lib.pty_loss_and_direction(ctypes.byref(p), ctypes.byref(b), None) == -1
-    # plane transforms: any size whose prime factors are <= 31 plans; 37, 2*41 do not
+    # plane transforms: every length plans (a prime factor above 31 as a direct stage, which needs a
+    # second copy of the line in LDS), up to the LDS: 8192, or 4096 with such a factor
     fake = ctypes.c_void_p(256)
-    # a prime factor above 31 is a direct stage, which needs a second copy of the line: 2 x 4099 is too long
     assert lib.pty_fourier_clean_c64(fake, 1, 2 * 4099, 64, 0.5, 0.0, None) == -2
-    assert lib.pty_fourier_clean_c64(fake, 1, 64, 82, 0.5, 0.0, None) == -2
-    assert lib.pty_plane_fft2_c64(fake, None, 1, 64, 74, 0, None) == -2
+    assert lib.pty_fourier_clean_c64(fake, 1, 64, 2 * 4099, 0.5, 0.0, None) == -2
+    assert lib.pty_plane_fft2_c64(fake, None, 1, 64, 16384, 0, None) == -2
     assert lib.pty_plane_fft2_c64(fake, None, 1, 160, 210, 0, None) == -5  # plans; no scratch given
+    assert lib.pty_plane_fft2_c64(fake, None, 1, 94, 2 * 37, 0, None) == -5
 
 
 def test_workspace_size_formula(lib):
