@@ -9,11 +9,12 @@ including the per-batch data preprocessing (shift, bin, multiplier, pad;
 utils.py:587-653), the static background and its gradient
 (objective.py:359-361, 525-526, 588-589) and, with ``load_one_by_one`` and
 patterns in host memory, streaming of each batch's patterns through pinned
-buffers (objective.py:383-408).  Arguments that select features outside that
-path (near field, tilts, aberrations, beam current, phase plates, the
-regularisers other than the L1 norms of the object's phase and log-amplitude,
-upsampling) raise ``NotImplementedError`` instead of silently computing
-something else.
+buffers (objective.py:383-408), and the per-position beam current and its
+gradient (objective.py:446-450, 688-698).  The constraint terms
+(objective.py:765-921) are added by ``pypty_amd.constraints`` on the device.
+Arguments that select features outside that path (near field, tilts,
+aberrations, phase plates, per-slice distances, upsampling) raise
+``NotImplementedError`` instead of silently computing something else.
 
 Binning follows the evident intent of utils.py:639-644 (sum of the
 data_bin x data_bin blocks); the reference's loop indexes an undefined name

@@ -172,3 +172,63 @@ def test_register_tile_lds_bounds_and_banks(N):
                 check([(L * w + l // T) * LS + T * (CPT * k2 + c) + l % T for l in lanes], 1)
         for n in range(R1):
             check([(l // L + T * n) * LS + L * w + l % L for l in lanes], col_most)
+
+
+def test_register_slab_192_phase_changes():
+    """The 192 px register tile's phase changes (RegSlab in pty_kernels.hip): 768 threads, two rounds
+    of 96 lines, LDS of 96 rows x stride 200.  to_cols must turn every thread's row-spectrum registers
+    (register c*8 + k2 of round r = position 8*(3*k2 + c) + j of row r*96 + ln) into its column
+    registers (register n of round q = row j + 8n of column q*96 + ln), to_rows back; every LDS index
+    stays below 96 * 200; writes put a half-wavefront on distinct bank pairs, reads on at most two."""
+    N, NT, R1, CPT, LINES, LS, H = 192, 768, 24, 3, 96, 200, 12
+    tids = np.arange(NT)
+    ln = 8 * (tids >> 6) + ((tids & 63) >> 3)
+    j = tids & 7
+    tile = np.arange(N * N).reshape(N, N)
+    d = np.zeros((NT, 2, R1), np.int64)
+    for p in range(2):
+        for c in range(CPT):
+            for k2 in range(8):
+                d[:, p, c * 8 + k2] = tile[p * LINES + ln, 8 * (CPT * k2 + c) + j]
+    X = np.full(LINES * LS, -1, np.int64)
+    t = np.zeros((NT, 2, 2, H), np.int64)
+    top = 0
+    for p in range(2):
+        X[:] = -1
+        for c in range(CPT):
+            for k2 in range(8):
+                idx = ln * LS + j + 8 * (CPT * k2 + c)
+                top = max(top, idx.max())
+                X[idx] = d[:, p, c * 8 + k2]
+        for q in range(2):
+            for n in range(H):
+                idx = j * LS + ln + 8 * n * LS + q * LINES
+                top = max(top, idx.max())
+                t[:, p, q, n] = X[idx]
+    assert (t >= 0).all() and top < LINES * LS
+    dc = np.zeros_like(d)
+    for q in range(2):
+        for n in range(R1):
+            dc[:, q, n] = t[:, 0, q, n] if n < H else t[:, 1, q, n - H]
+    for q in range(2):
+        for n in range(R1):
+            assert np.array_equal(dc[:, q, n], tile[j + 8 * n, q * LINES + ln]), (q, n)
+    back = np.zeros_like(d)
+    for p in range(2):
+        X[:] = -1
+        for q in range(2):
+            for n in range(H):
+                X[j * LS + ln + 8 * n * LS + q * LINES] = dc[:, q, n + H * p]
+        for c in range(CPT):
+            for k2 in range(8):
+                back[:, p, c * 8 + k2] = X[ln * LS + j + 8 * (CPT * k2 + c)]
+    assert np.array_equal(back, d)
+
+    def worst(addr):  # most lanes of one half-wavefront on one 8-byte bank pair (64 banks of 4 bytes)
+        w = 0
+        for h in range(NT // 32):
+            a = addr[32 * h:32 * h + 32]
+            w = max(w, np.bincount((a * 2) % 64, minlength=64).max())
+        return w
+
+    assert worst(ln * LS + j) == 1 and worst(j * LS + ln) <= 2
