@@ -5,7 +5,7 @@
 // pypty/iterative.py:56-250).
 //
 // Every layer is one GEMM on the matrix cores with BF16 operands and FP32
-// accumulation (v_mfma_f32_32x32x8_bf16): a convolution as im2col followed by
+// accumulation (v_mfma_f32_32x32x16_bf16): a convolution as im2col followed by
 // cols x W^T, its backward as dY^T x cols (weights) and dY x W then col2im
 // (input); a dense layer is the same with one pixel per pattern.  Activations
 // are pixel-major with channels innermost (NHWC).  Weights are FP32 masters
@@ -21,7 +21,7 @@
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // float -> bf16 bits, round to nearest even (NaN stays NaN)
@@ -44,10 +44,11 @@ int launch_status() {
 // step stages op(A) and op(B)^T as [row][k] tiles in LDS (one 16-byte global
 // load per thread and operand, row pitch GLP) while the previous step's
 // tiles feed the matrix cores; each wavefront owns a 32 x 32 block and
-// accumulates v_mfma_f32_32x32x8_bf16.  Operand lanes: lane l holds
-// A[m0 + l%32][k + 4*(l/32) + v] and B[k + 4*(l/32) + v][n0 + l%32], v = 0..3;
+// accumulates gfx950's v_mfma_f32_32x32x16_bf16 (k 16 per instruction).
+// Operand lanes: lane l holds A[m0 + l%32][k + 8*(l/32) + v] and
+// B[k + 8*(l/32) + v][n0 + l%32], v = 0..7 (one 16-byte LDS read each);
 // accumulator r of lane l is C[m0 + 8*(r/4) + 4*(l/32) + r%4][n0 + l%32].
-constexpr int GBK = 32, GLP = GBK + 4;  // k per step; LDS row pitch (bf16), 72 bytes
+constexpr int GBK = 32, GLP = GBK + 8;  // k per step; LDS row pitch (bf16), 80 bytes: rows 16-byte aligned
 
 // 8 bf16 of the 64 x 32 tile X[row][k] (rows r0.., k k0..), X[row][k] =
 // TRANS ? X[k*ld + row] : X[row*ld + k], zero outside rows < R and k < k_hi.
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
   const int bm = blockIdx.y * 64, bn = blockIdx.x * 64;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
-  const int li = lane & 31, kq = (lane >> 5) * 4;
+  const int li = lane & 31, kq = (lane >> 5) * 8;
   // 16-byte global loads need 8-element aligned rows and base
   const bool avec = (lda % 8 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
   const bool bvec = (ldb % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
@@ -117,10 +118,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
       vb = tile_load<!TB>(B, ldb, N, bn, k0 + GBK, k_hi, bvec);
     }
 #pragma unroll
-    for (int kk = 0; kk < GBK; kk += 8) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(As + (wm + li) * GLP + kk + kq);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(Bs + (wn + li) * GLP + kk + kq);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int kk = 0; kk < GBK; kk += 16) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(As + (wm + li) * GLP + kk + kq);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + (wn + li) * GLP + kk + kq);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
   }
   float* Cz = C + size_t(blockIdx.z) * zstride;
