@@ -36,6 +36,9 @@ extern "C" {
 #define PTY_ERR_BOUNDS -4     /* a window leaves the object */
 #define PTY_ERR_WORKSPACE -5  /* workspace missing or too small */
 
+/* most aberration coefficients per row (pty_problem.n_aberrations) */
+#define PTY_MAX_ABERRATIONS 64
+
 /* fft modes */
 #define PTY_FFT_FWD 0        /* fft2                        fft.py:114 */
 #define PTY_FFT_INV 1        /* ifft2 (1/n^2)               fft.py:136 */
@@ -150,6 +153,15 @@ typedef struct pty_problem {
    * this_beam_current_step); needs pty_buffers.beam_current and
    * beam_current_grad. */
   int step_beam_current;
+  /* Aberrations of the incident beam per position (objective.py:320-326,
+   * 459-465): n_aberrations coefficients per row, in Krivanek order
+   * (utils.py:1272-1308: C10, C12a, C12b, C21a, C21b, C23a, C23b, ...),
+   * n_markers rows; 0 for none.  At most PTY_MAX_ABERRATIONS coefficients. */
+  int n_aberrations;
+  int n_markers;
+  /* 1: also return the gradient of the aberrations (objective.py:673-681,
+   * this_step_aberrations_array); needs pty_buffers.aberrations_grad. */
+  int step_aberrations;
 } pty_problem;
 
 typedef struct pty_buffers {
@@ -191,6 +203,22 @@ typedef struct pty_buffers {
    * the unscaled probe (objective.py:694-696; sign(|bc|) is 1 unless the
    * current is 0); the other rows zero unless accumulate. */
   float* beam_current_grad;
+  /* (n_markers, n_aberrations) coefficients, Angstrom (objective.py:320-326)
+   * and (n_positions) row of each position: the probe spectrum of position
+   * i, after the beam current, is multiplied by
+   * exp(-i sum_j A_j(q) aberrations[aberration_marker[i]][j]) times the band
+   * mask, A_j = 2 pi / lambda (lambda q)^(n+1) / (n+1) times cos(m phi),
+   * sin(m phi) or 1 (utils.py:1333-1367).  As in the reference, a sub-pixel
+   * shift (shift_active) starts again from the spectrum before this phase
+   * plate (objective.py:471-474), so with a shift the plates do not reach
+   * the modelled patterns; the gradient is formed either way. */
+  const float* aberrations;
+  const int32_t* aberration_marker;
+  /* (n_markers, n_aberrations) out: per row the sum over its positions and
+   * the probe modes of 2/n^2 Re(G conj(F) conj(-i A_j)), with G the
+   * spectrum of dL/dP* after the conjugate plate and F the probe spectrum
+   * before it (objective.py:676-679); zero first unless accumulate. */
+  float* aberrations_grad;
 } pty_buffers;
 
 int pty_workspace_bytes(const pty_problem* p, size_t* bytes);

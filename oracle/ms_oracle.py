@@ -163,11 +163,43 @@ def scatteradd(full, my, mx, chop):
 
 
 # ------------------------------------------------------ loss + gradients
+def convert_num_to_nmab(num_abs):
+    """utils.py:1272-1308: (n, m, a/b) of the first num_abs aberrations in Krivanek order."""
+    max_n = int(np.ceil(0.5 * (-5 + np.sqrt(25 + 8 * num_abs))))
+    ns, ms, abs_ = [], [], []
+    for n in range(1, max_n + 1):
+        for m in range((n + 1) % 2, n + 2, 2):
+            if m == 0:
+                ns.append(n), ms.append(m), abs_.append("")
+            else:
+                ns.append(n), ms.append(m), abs_.append("a")
+                ns.append(n), ms.append(m), abs_.append("b")
+    return ns[:num_abs], ms[:num_abs], abs_[:num_abs]
+
+
+def ctf_matrix(kx, ky, num_abs, wavelength):
+    """utils.py:1333-1367: (num_abs, Ny, Nx) phase per unit coefficient, 2 pi / lambda k^(n+1) / (n+1)
+    times cos(m phi), sin(m phi) or 1, with k the beam angle (lambda q)."""
+    ns, ms, abs_ = convert_num_to_nmab(num_abs)
+    kr = np.sqrt(kx ** 2 + ky ** 2)
+    phi = np.angle(kx + 1j * ky)
+    A = np.zeros((num_abs,) + kr.shape)
+    for i in range(num_abs):
+        term = kr ** (ns[i] + 1) / (ns[i] + 1)
+        if abs_[i] == "a":
+            term = term * np.cos(ms[i] * phi)
+        elif abs_[i] == "b":
+            term = term * np.sin(ms[i] * phi)
+        A[i] = term
+    return A * 6.283185307179586 / wavelength
+
+
 def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *, dz, wavelength,
                        pixel_size_x, pixel_size_y, algorithm=LSQ_SQRT, loss_weight=1.0, cutoff=2 / 3,
                        edge=1.0, step_obj=True, step_probe=True, step_pos=False, compute_batch=64,
                        cdtype=np.complex64, return_patterns=False, masks=None, static_background=None,
-                       step_static_background=False, beam_current=None, step_beam_current=False):
+                       step_static_background=False, beam_current=None, step_beam_current=False, aberrations=None,
+                       aberration_marker=None, step_aberrations=False):
     """objective.loss_and_direction restricted to the far-field multislice path.
 
     Follows pypty/objective.py:218-361 (grids, propagator, shift kernels),
@@ -194,6 +226,15 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     reference's one-mode branch conjugates the caller's probe in place
     (objective.py:692) and so feeds the conjugate to the next batch; this
     restatement keeps the probe as given, as its many-mode branch does.
+    aberrations (n_markers, num_abs) with aberration_marker (P,) follow
+    objective.py:320-326, 459-465 (each position's probe spectrum, after the
+    beam current, times exp(-i sum_i A_i a_i) mask of its marker's
+    coefficients) and :673-681 (the probe-side gradient spectrum times the
+    conjugate plate, and per window 2/N^2 sum Re(Fg conj(FP) conj(-i A_i))
+    added to its marker's row).  As in the reference, the sub-pixel shift
+    (:471-474) starts again from the spectrum before the plate, so with a shift
+    the plates leave the loss unchanged while their gradient is still formed.
+    When given, the aberration gradient is appended last.
     """
     fd = _fdt(cdtype)
     obj = np.asarray(obj, dtype=cdtype)
@@ -220,6 +261,13 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     bg_grad = None if bg is None else np.zeros((N, N), dtype=fd)
     bc = None if beam_current is None else np.asarray(beam_current, dtype=fd)
     bc_grad = None if bc is None else np.zeros_like(bc)
+    ab = None if aberrations is None else np.asarray(aberrations, dtype=fd)
+    ab_grad = None if ab is None else np.zeros_like(ab)
+    if ab is not None:
+        marker = np.asarray(aberration_marker)
+        A = ctf_matrix(wavelength * qx, wavelength * qy, ab.shape[1], wavelength)
+        polys = (-1j * A).astype(cdtype)
+        plates = np.exp(np.sum(polys[None] * ab[:, :, None, None], 1)) * mask[None]
     FP = fft2(probe[None], axes=(1, 2))
     loss, sse = fd(0), fd(0)
     seq = np.asarray(sequence)
@@ -236,6 +284,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
             thisbc = np.abs(bc[tcs])
             this_probe = this_probe * thisbc[:, None, None, None]
             FPb = FP * thisbc[:, None, None, None]
+        if ab is not None:
+            this_probe = ifft2(FPb * plates[marker[tcs]][..., None], axes=(1, 2))
         E = None
         if shift_flag:
             corr = corrections[tcs]
@@ -323,6 +373,14 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
                 sh = -2 / (N * N)
                 yx = np.stack((shy, shx), axis=0)  # (2, N, N)
                 pos_grad[tcs] = sh * np.sum(np.real(yx[None, :, :, :, None] * smg[:, None]), axis=(2, 3, 4))
+        if ab is not None:
+            Fg = fft2(interm, axes=(1, 2)) if not shift_flag else Fg
+            Fg = Fg * np.conj(plates[marker[tcs]])[..., None]
+            interm = ifft2(Fg, axes=(1, 2))
+            if step_aberrations:
+                defgr = (2 / (N * N)) * np.sum(np.real((Fg * np.conj(FPb))[:, None] *
+                                                       np.conj(polys)[None, :, :, :, None]), axis=(2, 3, 4))
+                np.add.at(ab_grad, marker[tcs], defgr)
         if bc is not None:
             if step_beam_current:
                 bc_grad[tcs] = 2 * np.sign(thisbc) * np.sum(np.real(interm * np.conj(probe[None])), axis=(1, 2, 3))
@@ -341,6 +399,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
         out = out + (bg_grad,)
     if bc is not None:
         out = out + (bc_grad,)
+    if ab is not None:
+        out = out + (ab_grad,)
     return out
 
 

@@ -20,6 +20,7 @@ PTY_LOSS_LSQ_COMPRESSED, PTY_LOSS_GAUSS_COMPRESSED, PTY_LOSS_POISSON_COMPRESSED 
 PTY_MEASURED_BY_POSITION, PTY_MEASURED_BY_SEQUENCE = 0, 1
 PTY_LAYOUT_REFERENCE, PTY_LAYOUT_PLANAR = 0, 1
 PTY_ERR_WORKSPACE = -5
+PTY_MAX_ABERRATIONS = 64  # coefficients per aberration row
 PTY_STATIC_WORKSPACE_BYTES = 64 << 20  # the workspace array of pty_loss_and_direction_static
 
 EXPORTED = (
@@ -48,6 +49,7 @@ class PtyProblem(ctypes.Structure):
         ("data_pad", ctypes.c_int), ("data_multiplier", ctypes.c_float),
         ("checkpoint_slices", ctypes.c_int), ("lanes", ctypes.c_int),
         ("step_static_background", ctypes.c_int), ("step_beam_current", ctypes.c_int),
+        ("n_aberrations", ctypes.c_int), ("n_markers", ctypes.c_int), ("step_aberrations", ctypes.c_int),
     ]
 
 
@@ -61,6 +63,8 @@ class PtyBuffers(ctypes.Structure):
         ("measured_order", ctypes.c_int), ("accumulate", ctypes.c_int),
         ("static_background", ctypes.c_void_p), ("static_background_grad", ctypes.c_void_p),
         ("beam_current", ctypes.c_void_p), ("beam_current_grad", ctypes.c_void_p),
+        ("aberrations", ctypes.c_void_p), ("aberration_marker", ctypes.c_void_p),
+        ("aberrations_grad", ctypes.c_void_p),
     ]
 
 

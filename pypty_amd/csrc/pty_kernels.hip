@@ -254,6 +254,47 @@ __global__ void k_tables(TableArgs a) {
   }
 }
 
+// (n, m, kind) of aberration j in Krivanek order (utils.py:1272-1308): n =
+// 1, 2, ...; m = (n + 1) % 2, ..., n + 1 in steps of 2; kind 0 for m = 0,
+// else 1 (a, cos) then 2 (b, sin)
+__host__ __device__ inline void nmab_of(int j, int& n, int& m, int& kind) {
+  int k = 0;
+  for (n = 1;; ++n)
+    for (m = (n + 1) % 2; m <= n + 1; m += 2) {
+      if (m == 0) {
+        if (k == j) { kind = 0; return; }
+        k += 1;
+      } else {
+        if (k == j) { kind = 1; return; }
+        if (k + 1 == j) { kind = 2; return; }
+        k += 2;
+      }
+    }
+}
+
+// aberration polynomials A_j (utils.py:1333-1367) in the order of maskc:
+// 2 pi / lambda k^(n+1) / (n+1) times cos(m phi), sin(m phi) or 1, with
+// k = lambda q the beam angle and phi its azimuth; FP64, stored as float
+template <int N, int LAY>
+__global__ void k_aber_table(float* __restrict__ abt, int NA, float px, float py, float lambda) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * N) return;
+  int fy, fx;
+  spec_of<N, LAY>(e, fy, fx);
+  const double kx = double(fmul(lambda, fftfreq_f(fx, N, px))), ky = double(fmul(lambda, fftfreq_f(fy, N, py)));
+  const double kr = sqrt(kx * kx + ky * ky), phi = atan2(ky, kx);
+  for (int j = 0; j < NA; ++j) {
+    int n, m, kind;
+    nmab_of(j, n, m, kind);
+    double t = 1.0;
+    for (int i = 0; i <= n; ++i) t *= kr;
+    t /= double(n + 1);
+    if (kind == 1) t *= cos(double(m) * phi);
+    if (kind == 2) t *= sin(double(m) * phi);
+    abt[size_t(j) * N * N + e] = float(t * 6.283185307179586 / double(lambda));
+  }
+}
+
 // Position-order tables from a caller-supplied propagator (natural order,
 // mask included, as multislice() receives master_propagator_phase_space) and
 // band mask; H^e through the principal branch when edge > 1.
@@ -367,6 +408,23 @@ struct BatchArgs {
   const float* bc;
   double* bc_part;
   float* bc_grad;
+  // aberrations (objective.py:320-326, 459-465, 673-681): NA coefficients per
+  // row (n_markers rows) at ab, the row of each position at marker, and the
+  // polynomials A_j (NA, N, N) in the order of maskc at abt, or NA = 0.
+  // moved: the sub-pixel shift is active (the shifted probe then starts from
+  // the spectrum before the plate, so the plates stay out of the forward).
+  // With ab_w, the probe gradient also writes |bc| Im(G conj(F)) per window
+  // and mode (B, M, N, N, order of maskc), reduced against A_j into ab_part
+  // (B, NA) and added by row into ab_grad (n_markers, NA).  plate: each
+  // window's exp(-i sum_j A_j c_j) (B, N, N, order of maskc).
+  int NA, n_markers, moved;
+  const float* ab;
+  const int* marker;
+  const float* abt;
+  float2* plate;
+  float* ab_w;
+  double* ab_part;
+  float* ab_grad;
   // slice range [s_lo, s_hi) of a forward, adjoint or objgrad launch (a whole
   // pass: 0, S) and the stash blocks of st_slots slots per (window, mode).
   // With r = s - s_lo the forward stores slice s in slot r / st_every when
@@ -407,6 +465,14 @@ __device__ __forceinline__ float bc_scale(const BatchArgs& a, int idx) {
 // gradient sum Re(dL/dP* conj(P)); 0 for a zero current
 __device__ __forceinline__ double bc_factor(const BatchArgs& a, int idx) {
   return (idx >= 0 && a.bc[idx] != 0.f) ? 2.0 : 0.0;
+}
+
+// aberration row of position idx, or null: no aberrations, no position, or a
+// row outside [0, n_markers) (the host checks the rows before a pass)
+__device__ __forceinline__ const float* ab_row(const BatchArgs& a, int idx) {
+  if (a.NA == 0 || idx < 0) return nullptr;
+  const int r = a.marker[idx];
+  return (r >= 0 && r < a.n_markers) ? a.ab + size_t(r) * a.NA : nullptr;
 }
 
 // measured row of batch slot b (sequence entry b0 + b, position idx)
@@ -1188,8 +1254,9 @@ __global__ __launch_bounds__(RT<N>::NT) void k_fprobe_rt(const float2* __restric
   for (int i = 0; i < R::R1; ++i) out.store(i, v[i]);
 }
 
-// shifted probes (objective.py:471-476): ifft2(fprobe * E), natural order
-template <int N>
+// shifted probes (objective.py:471-476): ifft2(fprobe * E), natural order;
+// without a shift, the aberrated probes ifft2(fprobe * plate) (:459-465)
+template <int N, bool AB>
 __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
   using R = RT<N>;
   rt_tw_load<N>();
@@ -1197,7 +1264,8 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
   __shared__ float2 exy[2 * N];
   const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
   const int idx = seq_index(a.seq, a.b0 + b, a.P);
-  const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+  const bool mv = a.moved && idx >= 0;
+  const float dy = mv ? a.corr[2 * idx] : 0.f, dx = mv ? a.corr[2 * idx + 1] : 0.f;
   const unsigned tid = threadIdx.x;
   const RTIds<N> id;
   shift_tables_freq<N>(exy, dy, dx);
@@ -1217,6 +1285,8 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
       R::cspec(i * R::NT + tid, fy, fx);
       const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mask[i]);
       v[i] = cscale(cmul(fp[i], E), s);
+      if constexpr (AB)
+        if (!a.moved) v[i] = cmul(v[i], a.plate[size_t(b) * N * N + i * R::NT + tid]);
     }
     rt_line_inv<N, false>(v, T, id.col, id.cj);
   }
@@ -1232,7 +1302,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
 // accumulated (C-order) into the group's partial, with the position gradient
 // -2/n^2 sum Re(K conj(FP) Q) in FP64; without a shift the natural-order
 // gradient itself is accumulated (R-order)
-template <int N>
+template <int N, bool AB>
 __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
   using R = RT<N>;
   rt_tw_load<N>();
@@ -1282,7 +1352,9 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       }
       continue;
     }
-    const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+    const bool mv = a.moved && idx >= 0;
+    const float dy = mv ? a.corr[2 * idx] : 0.f, dx = mv ? a.corr[2 * idx + 1] : 0.f;
+    float* wrow = AB && a.ab_w ? a.ab_w + (size_t(b) * a.M + m) * N * N : nullptr;
     __syncthreads();  // the previous window's table and transposes are done
     shift_tables_freq<N>(exy, dy, dx);
     rt_fft2_fwd<N>(v, T, id);  // its first barrier also publishes exy
@@ -1294,11 +1366,16 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       const float mk = mask[i];
       const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mk);
       const float2 Q = cmulc(v[i], E);
-      sum[i] = cadd(sum[i], cscale(Q, bcs));
-      if (a.step_pos || a.bc_part) {
+      // after the conjugate aberration plate (objective.py:673-675); the mask is 0 or 1
+      float2 Fg = Q;
+      if constexpr (AB) Fg = cmulc(Q, a.plate[size_t(b) * N * N + i * R::NT + tid]);
+      sum[i] = cadd(sum[i], cscale(Fg, bcs));
+      if (a.step_pos || a.bc_part || (AB && wrow)) {
         const float2 F = fp[i];
-        // Re(Q * conj(FP)): the beam current's sum, Parseval's form of the real-space one
-        sb += double(Q.x) * double(F.x) + double(Q.y) * double(F.y);
+        // Re(G * conj(FP)): the beam current's sum, Parseval's form of the real-space one
+        sb += double(Fg.x) * double(F.x) + double(Fg.y) * double(F.y);
+        if constexpr (AB)
+          if (wrow) wrow[i * R::NT + tid] = float(double(bcs) * (double(Fg.y) * double(F.x) - double(Fg.x) * double(F.y)));
         // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
         const double zy = double(Q.y) * double(F.x) - double(Q.x) * double(F.y);
         const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
@@ -1939,7 +2016,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_fprobe_gt(const float2* __res
 }
 
 // shifted probes (objective.py:471-476): ifft2(fprobe * E), natural order
-template <int N>
+template <int N, bool AB>
 __global__ __launch_bounds__(kGtThreads<N>) void k_probe_shift_gt(BatchArgs a) {
   using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
@@ -1947,7 +2024,8 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probe_shift_gt(BatchArgs a) {
   __shared__ float2 exy[2 * N];
   const int b = blockIdx.x / a.M, m = blockIdx.x % a.M;
   const int idx = seq_index(a.seq, a.b0 + b, a.P);
-  const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+  const bool mv = a.moved && idx >= 0;
+  const float dy = mv ? a.corr[2 * idx] : 0.f, dx = mv ? a.corr[2 * idx + 1] : 0.f;
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
   SlabOf<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
@@ -1973,6 +2051,8 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probe_shift_gt(BatchArgs a) {
       G::cspec(q * G::NT + tid, fy, fx);
       const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mask[q]);
       v[i] = cscale(cmul(fp[q], E), s);
+      if constexpr (AB)
+        if (!a.moved) v[i] = cmul(v[i], a.plate[size_t(b) * N * N + q * G::NT + tid]);
     }
     rt_line_inv<N, true>(v, X, L.ln, L.j);
     gt_store_col<N>(v, sl, L, r);
@@ -1995,7 +2075,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probe_shift_gt(BatchArgs a) {
 // transformed, shifted back and added (C-order) to the group's partial, with
 // the position gradient -2/n^2 sum Re(K conj(FP) Q) in FP64; without a shift
 // the natural-order gradient itself is added (slab R-order)
-template <int N>
+template <int N, bool AB>
 __global__ __launch_bounds__(kGtThreads<N>) void k_probegrad_gt(BatchArgs a) {
   using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
@@ -2041,7 +2121,9 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probegrad_gt(BatchArgs a) {
       }
       continue;
     }
-    const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+    const bool mv = a.moved && idx >= 0;
+    const float dy = mv ? a.corr[2 * idx] : 0.f, dx = mv ? a.corr[2 * idx + 1] : 0.f;
+    float* wrow = AB && a.ab_w ? a.ab_w + (size_t(b) * a.M + m) * N * N : nullptr;
     __syncthreads();  // the previous window's slab and table reads are done
     shift_tables_freq<N>(exy, dy, dx);
     for (int r = 0; r < G::ROUNDS; ++r) {
@@ -2059,7 +2141,13 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probegrad_gt(BatchArgs a) {
     double sy = 0.0, sx = 0.0, sb = 0.0;
     for (int r = 0; r < G::ROUNDS; ++r) {
       round_fence<decltype(sl)>();
-      if ((zm >> r) & 1u) continue;
+      if ((zm >> r) & 1u) {
+        if constexpr (AB)
+          if (wrow)
+#pragma unroll
+            for (int i = 0; i < G::R1; ++i) wrow[(r * G::R1 + i) * G::NT + tid] = 0.f;
+        continue;
+      }
       gt_load_col<N>(v, sl, L, r);
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
@@ -2070,11 +2158,17 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probegrad_gt(BatchArgs a) {
         const float mk = mask[q];
         const float2 E = cscale(cmul(exy[fx], exy[N + fy]), mk);
         const float2 Q = cmulc(v[i], E);
-        acc.store(q, cadd(acc[q], cscale(Q, bcs)));
-        if (a.step_pos || a.bc_part) {
+        // after the conjugate aberration plate (objective.py:673-675); the mask is 0 or 1
+        float2 Fg = Q;
+        if constexpr (AB) Fg = cmulc(Q, a.plate[size_t(b) * N * N + q * G::NT + tid]);
+        acc.store(q, cadd(acc[q], cscale(Fg, bcs)));
+        if (a.step_pos || a.bc_part || (AB && wrow)) {
           const float2 F = fp[q];
-          // Re(Q * conj(FP)): the beam current's sum, Parseval's form of the real-space one
-          sb += double(Q.x) * double(F.x) + double(Q.y) * double(F.y);
+          // Re(G * conj(FP)): the beam current's sum, Parseval's form of the real-space one
+          sb += double(Fg.x) * double(F.x) + double(Fg.y) * double(F.y);
+          if constexpr (AB)
+            if (wrow)
+              wrow[q * G::NT + tid] = float(double(bcs) * (double(Fg.y) * double(F.x) - double(Fg.x) * double(F.y)));
           // Im(conj(FP) * Q) in FP64: the terms of the sum nearly cancel
           const double zy = double(Q.y) * double(F.x) - double(Q.x) * double(F.y);
           const float kx = fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk);
@@ -2281,6 +2375,57 @@ __global__ void k_bc_final(BatchArgs a) {
   a.bc_grad[idx] = float(s);
 }
 
+// each window's phase plate exp(-i chi), chi = sum_j A_j c_j with c its
+// position's row (objective.py:324; the band mask comes with the shift
+// factor), 1 for a window without a row; one thread per element
+template <int N>
+__global__ __launch_bounds__(256) void k_aber_plate(BatchArgs a) {
+  const size_t t = blockIdx.x * size_t(256) + threadIdx.x;
+  if (t >= size_t(a.B) * N * N) return;
+  const int b = int(t / (N * N)), e = int(t % (N * N));
+  const float* c = ab_row(a, seq_index(a.seq, a.b0 + b, a.P));
+  float chi = 0.f;
+  if (c)
+    for (int j = 0; j < a.NA; ++j) chi = fmaf(a.abt[size_t(j) * N * N + e], c[j], chi);
+  float sn, cs;
+  sincosf(chi, &sn, &cs);
+  a.plate[t] = make_float2(cs, -sn);
+}
+
+// per window and coefficient: -2/n^2 sum over modes and frequencies of A_j
+// times |bc| Im(G conj(F)), the window's term of objective.py:676-679 in FP64
+// (Re(Z conj(-i A)) = -A Im(Z))
+template <int N>
+__global__ __launch_bounds__(256) void k_aber_part(BatchArgs a) {
+  __shared__ double red[4];
+  const int b = blockIdx.x / a.NA, j = blockIdx.x % a.NA;
+  const float* A = a.abt + size_t(j) * N * N;
+  double s = 0.0;
+  for (int m = 0; m < a.M; ++m) {
+    const float* w = a.ab_w + (size_t(b) * a.M + m) * N * N;
+    for (int e = threadIdx.x; e < N * N; e += 256) s += double(A[e]) * double(w[e]);
+  }
+  const double S = block_sum(s, red);
+  if (threadIdx.x == 0) a.ab_part[size_t(b) * a.NA + j] = -2.0 / (double(N) * double(N)) * S;
+}
+
+// aberrations_grad[r][j] += the batch's windows of row r in batch order
+// (objective.py:680, scatteradd_abers), one thread per (row, coefficient)
+__global__ void k_aber_final(BatchArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_markers * a.NA) return;
+  const int r = t / a.NA, j = t % a.NA;
+  double s = 0.0;
+  bool any = false;
+  for (int b = 0; b < a.B; ++b) {
+    const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    if (idx < 0 || a.marker[idx] != r) continue;
+    s += a.ab_part[size_t(b) * a.NA + j];
+    any = true;
+  }
+  if (any) a.ab_grad[t] = float(double(a.ab_grad[t]) + s);
+}
+
 // sum over the G group partials of the probe gradient, one thread per
 // element (in group order, so the sum is deterministic)
 __global__ __launch_bounds__(256) void k_probe_reduce(const float2* __restrict__ part, int G, size_t per_group,
@@ -2334,12 +2479,14 @@ struct TimedLaunch {
 enum {
   KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
   KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER,
-  KT_RECOMPUTE, KT_PREPROCESS, KT_BG_GRAD, KT_PROBE_REDUCE, KT_BC_FINAL, KT_COUNT
+  KT_RECOMPUTE, KT_PREPROCESS, KT_BG_GRAD, KT_PROBE_REDUCE, KT_BC_FINAL, KT_AB_TABLE, KT_AB_PLATE, KT_AB_PART,
+  KT_AB_FINAL, KT_COUNT
 };
 const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
                                     "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
                                     "loss_final", "zero", "scatteradd", "gather_grad", "recompute", "preprocess",
-                                    "bg_grad", "probe_reduce", "bc_final"};
+                                    "bg_grad", "probe_reduce", "bc_final", "aber_table", "aber_plate",
+                                    "aber_part", "aber_final"};
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
@@ -2425,13 +2572,14 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 // after lane 0's (a buffer of 0 bytes is not needed by the problem).
 enum {
   WB_H = 0, WB_H0, WB_MASKC, WB_ZCOL, WB_FPROBE, WB_OBJP, WB_GRADP, WB_PROBE_PART, WB_PROBE_SUM, WB_LOSS_PART,
-  WB_OB, WB_PP, WB_STASH, WB_GSTATE, WB_PSI, WB_COEF, WB_GP, WB_MEAS_STAGE, WB_POS_PART, WB_BC_PART, WB_SCRATCH,
-  WB_COUNT
+  WB_ABT, WB_OB, WB_PP, WB_STASH, WB_GSTATE, WB_PSI, WB_COEF, WB_GP, WB_MEAS_STAGE, WB_POS_PART, WB_BC_PART,
+  WB_PLATE, WB_AB_W, WB_AB_PART, WB_SCRATCH, WB_COUNT
 };
 constexpr int kFirstLaneBuffer = WB_OB;
 const char* kBufNames[WB_COUNT] = {"H", "H0", "maskc", "zcol", "fprobe", "objp", "gradp", "probe_part",
-                                   "probe_sum", "loss_part", "Ob", "Pp", "stash", "gstate", "Psi", "coef", "gP",
-                                   "meas_stage", "pos_part", "bc_part", "scratch"};
+                                   "probe_sum", "loss_part", "abt", "Ob", "Pp", "stash", "gstate", "Psi", "coef",
+                                   "gP", "meas_stage", "pos_part", "bc_part", "plate", "ab_w", "ab_part",
+                                   "scratch"};
 struct Layout {
   size_t off[WB_COUNT], bytes[WB_COUNT];  // lane 0's offset for the per-lane buffers
   size_t lane0, lane_bytes, total;
@@ -2515,6 +2663,7 @@ Layout plan_layout(const pty_problem* p) {
   take(WB_PROBE_PART, size_t(G) * M * n2 * 8);
   take(WB_PROBE_SUM, size_t(M) * n2 * 8);
   take(WB_LOSS_PART, size_t(std::max(p->n_positions, 1)) * 2 * 8 * loss_chunks(p->n));
+  take(WB_ABT, size_t(p->n_aberrations) * n2 * 4);
   L.lane0 = o;
   take(WB_OB, size_t(B) * S * K * n2 * 8);
   take(WB_PP, size_t(B) * M * n2 * 8);
@@ -2532,6 +2681,9 @@ Layout plan_layout(const pty_problem* p) {
   take(WB_MEAS_STAGE, preprocessing(p) ? size_t(B) * n2 * 4 : 0);
   take(WB_POS_PART, size_t(B) * M * 2 * 8);
   take(WB_BC_PART, p->step_beam_current ? size_t(B) * M * 8 : 0);
+  take(WB_PLATE, size_t(p->n_aberrations ? B : 0) * n2 * 8);
+  take(WB_AB_W, p->step_aberrations ? size_t(B) * M * n2 * 4 : 0);
+  take(WB_AB_PART, p->step_aberrations ? size_t(B) * p->n_aberrations * 8 : 0);
   // windows larger than 128 px: one tile per workgroup of the largest launch
   L.scratch_tiles = p->n > 128 ? int(std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K,
                                                        size_t(G) * M, size_t(M)}))
@@ -2552,6 +2704,9 @@ int validate(const pty_problem* p) {
   if (p->obj_ny < p->n || p->obj_nx < p->n) return PTY_ERR_ARG;
   if (p->checkpoint_slices < 0) return PTY_ERR_ARG;
   if (p->lanes < 0 || p->lanes > kMaxLanes) return PTY_ERR_ARG;
+  if (p->n_aberrations < 0 || p->n_aberrations > PTY_MAX_ABERRATIONS) return PTY_ERR_ARG;
+  if (p->n_aberrations > 0 && p->n_markers < 1) return PTY_ERR_ARG;
+  if (p->step_aberrations && p->n_aberrations == 0) return PTY_ERR_ARG;
   if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_POISSON_COMPRESSED) return PTY_ERR_ARG;
   if (compressed(p->algorithm) && (p->n_masks < 1 || p->n_masks > kMaxMasks)) return PTY_ERR_ARG;
   if (preprocessing(p)) {
@@ -2574,6 +2729,8 @@ int validate_buffers(const pty_problem* p, const pty_buffers* b) {
   if (b->static_background && compressed(p->algorithm)) return PTY_ERR_ARG;
   if (p->step_static_background && (!b->static_background || !b->static_background_grad)) return PTY_ERR_ARG;
   if (p->step_beam_current && (!b->beam_current || !b->beam_current_grad)) return PTY_ERR_ARG;
+  if (p->n_aberrations > 0 && (!b->aberrations || !b->aberration_marker)) return PTY_ERR_ARG;
+  if (p->step_aberrations && !b->aberrations_grad) return PTY_ERR_ARG;
   if (b->measured_order != PTY_MEASURED_BY_POSITION && b->measured_order != PTY_MEASURED_BY_SEQUENCE)
     return PTY_ERR_ARG;
   return PTY_OK;
@@ -2703,12 +2860,23 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
     else
       k_patch_gt<N><<<a.B * planes, kGtThreads<N>, 0, s>>>(a);
   }));
+  if (a.NA)
+    PTY_TRY(x.launch(KT_AB_PLATE, lane, a, [&](hipStream_t s) {
+      k_aber_plate<N><<<int((size_t(a.B) * N * N + 255) / 256), 256, 0, s>>>(a);
+    }));
   if (a.shift)
     PTY_TRY(x.launch(KT_PROBE_SHIFT, lane, a, [&](hipStream_t s) {
-      if constexpr (LAY == LAY_RT)
-        k_probe_shift_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
-      else
-        k_probe_shift_gt<N><<<a.B * a.M, kGtThreads<N>, 0, s>>>(a);
+      if constexpr (LAY == LAY_RT) {
+        if (a.NA)
+          k_probe_shift_rt<N, true><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
+        else
+          k_probe_shift_rt<N, false><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
+      } else {
+        if (a.NA)
+          k_probe_shift_gt<N, true><<<a.B * a.M, kGtThreads<N>, 0, s>>>(a);
+        else
+          k_probe_shift_gt<N, false><<<a.B * a.M, kGtThreads<N>, 0, s>>>(a);
+      }
     }));
   PTY_TRY(x.launch(KT_FORWARD, lane, a, [&](hipStream_t s) { launch_forward<N>(a, s); }));
   if (a.prep)
@@ -2740,7 +2908,7 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
       PTY_TRY(x.launch(KT_OBJGRAD, lane, g, [&](hipStream_t s) { launch_objgrad<N>(g, s); }));
     }
   } else {
-    if (p->step_obj || p->step_probe || p->step_pos || p->step_beam_current)
+    if (p->step_obj || p->step_probe || p->step_pos || p->step_beam_current || p->step_aberrations)
       PTY_TRY(x.launch(KT_ADJOINT, lane, a, [&](hipStream_t s) { launch_adjoint<N>(a, s); }));
     if (p->step_obj) PTY_TRY(x.launch(KT_OBJGRAD, lane, a, [&](hipStream_t s) { launch_objgrad<N>(a, s); }));
   }
@@ -2750,17 +2918,30 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
       const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
       k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
     }));
-  if (p->step_probe || (p->step_pos && a.shift) || p->step_beam_current)
+  if (p->step_probe || (p->step_pos && a.shift) || p->step_beam_current || p->step_aberrations)
     PTY_TRY(x.launch(KT_PROBEGRAD, lane, a, [&](hipStream_t s) {
-      if constexpr (LAY == LAY_RT)
-        k_probegrad_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
-      else
-        k_probegrad_gt<N><<<a.G * a.M, kGtThreads<N>, 0, s>>>(a);
+      if constexpr (LAY == LAY_RT) {
+        if (a.NA)
+          k_probegrad_rt<N, true><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+        else
+          k_probegrad_rt<N, false><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+      } else {
+        if (a.NA)
+          k_probegrad_gt<N, true><<<a.G * a.M, kGtThreads<N>, 0, s>>>(a);
+        else
+          k_probegrad_gt<N, false><<<a.G * a.M, kGtThreads<N>, 0, s>>>(a);
+      }
     }));
   if (p->step_pos && a.shift)
     PTY_TRY(x.launch(KT_POS_FINAL, lane, a, [&](hipStream_t s) { k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
   if (p->step_beam_current)
     PTY_TRY(x.launch(KT_BC_FINAL, lane, a, [&](hipStream_t s) { k_bc_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
+  if (p->step_aberrations) {
+    PTY_TRY(x.launch(KT_AB_PART, lane, a, [&](hipStream_t s) { k_aber_part<N><<<a.B * a.NA, 256, 0, s>>>(a); }));
+    PTY_TRY(x.launch(KT_AB_FINAL, lane, a, [&](hipStream_t s) {
+      k_aber_final<<<(a.n_markers * a.NA + 255) / 256, 256, 0, s>>>(a);
+    }));
+  }
   if (a.bg_grad)
     PTY_TRY(x.launch(KT_BG_GRAD, lane, a, [&](hipStream_t s) {
       k_bg_grad<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(a);
@@ -2777,7 +2958,9 @@ BatchArgs pass_args(const pty_problem* p, const pty_buffers* bf, const Layout& L
   a.S = p->n_slices; a.K = p->n_obj_modes; a.M = p->n_probe_modes; a.ny = p->obj_ny; a.nx = p->obj_nx;
   a.P = p->n_positions;
   a.algorithm = p->algorithm; a.step_obj = p->step_obj; a.step_probe = p->step_probe; a.step_pos = p->step_pos;
-  a.shift = p->shift_active; a.far_field = 1; a.write_stash = p->step_obj; a.scatter = 1;
+  // aberrations need the per-window probes and their spectral gradient also without a shift
+  a.shift = p->shift_active || p->n_aberrations > 0; a.moved = p->shift_active;
+  a.far_field = 1; a.write_stash = p->step_obj; a.scatter = 1;
   a.loss_weight = p->loss_weight;
   const bool ref = p->obj_layout == PTY_LAYOUT_REFERENCE;
   a.obj = ref ? reinterpret_cast<const float2*>(at(WB_OBJP)) : static_cast<const float2*>(bf->obj);
@@ -2809,6 +2992,13 @@ BatchArgs pass_args(const pty_problem* p, const pty_buffers* bf, const Layout& L
   a.bc = bf->beam_current;
   a.bc_part = p->step_beam_current ? reinterpret_cast<double*>(at(WB_BC_PART)) : nullptr;
   a.bc_grad = p->step_beam_current ? bf->beam_current_grad : nullptr;
+  a.NA = p->n_aberrations; a.n_markers = p->n_markers;
+  a.ab = a.NA ? bf->aberrations : nullptr; a.marker = a.NA ? bf->aberration_marker : nullptr;
+  a.abt = reinterpret_cast<float*>(at(WB_ABT));
+  a.plate = reinterpret_cast<float2*>(at(WB_PLATE));
+  a.ab_w = p->step_aberrations ? reinterpret_cast<float*>(at(WB_AB_W)) : nullptr;
+  a.ab_part = p->step_aberrations ? reinterpret_cast<double*>(at(WB_AB_PART)) : nullptr;
+  a.ab_grad = p->step_aberrations ? bf->aberrations_grad : nullptr;
   whole_pass(a);
   if (L.ck) {
     // the forward keeps the checkpoints and every wave of the last segment
@@ -2840,6 +3030,11 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
                 p->tan_x, p->tan_y, p->damping_cutoff, p->edge_multiplier};
     k_tables<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(t);
   }));
+  if (a.NA)
+    PTY_TRY(x.launch(KT_AB_TABLE, 0, a, [&](hipStream_t s) {
+      k_aber_table<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(const_cast<float*>(a.abt), a.NA, p->pixel_size_x,
+                                                                p->pixel_size_y, p->wavelength);
+    }));
   PTY_TRY(x.launch(KT_FPROBE, 0, a, [&](hipStream_t s) {
     if constexpr (LAY == LAY_RT)
       k_fprobe_rt<N><<<M, RT<N>::NT, 0, s>>>(a.probe, const_cast<float2*>(a.fprobe));
@@ -2855,8 +3050,10 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     PTY_TRY(issue_zero(x, 0, a, bf->static_background_grad, n2 * sizeof(float)));
   if (p->step_beam_current && !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, bf->beam_current_grad, size_t(p->n_positions) * sizeof(float)));
+  if (p->step_aberrations && !bf->accumulate)
+    PTY_TRY(issue_zero(x, 0, a, bf->aberrations_grad, size_t(p->n_markers) * p->n_aberrations * sizeof(float)));
   if (p->step_obj && !bf->accumulate) PTY_TRY(issue_zero(x, 0, a, a.obj_grad, objelems * 8));
-  if ((p->step_probe || p->step_pos || p->step_beam_current) && !bf->accumulate)
+  if ((p->step_probe || p->step_pos || p->step_beam_current || p->step_aberrations) && !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, a.probe_part, size_t(L.G) * M * n2 * 8));
   // lane l's arguments: its own copies of the per-batch buffers
   BatchArgs al[kMaxLanes];
@@ -2867,7 +3064,7 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     auto off = [d](auto* ptr) { return ptr ? reinterpret_cast<decltype(ptr)>(reinterpret_cast<char*>(ptr) + d) : ptr; };
     y.Ob = off(y.Ob); y.Pp = off(y.Pp); y.stash = off(y.stash); y.gstate = off(y.gstate);
     y.Psi = off(y.Psi); y.coef = off(y.coef); y.gP = off(y.gP); y.pos_part = off(y.pos_part);
-    y.bc_part = off(y.bc_part);
+    y.bc_part = off(y.bc_part); y.plate = off(y.plate); y.ab_w = off(y.ab_w); y.ab_part = off(y.ab_part);
     y.meas_stage = off(y.meas_stage); y.scratch = off(y.scratch);
   }
   if (nl > 1) {
@@ -2903,10 +3100,10 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     }));
     PTY_TRY(x.launch(KT_PROBE_FINAL, 0, a, [&](hipStream_t s) {
       if constexpr (LAY == LAY_RT)
-        k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, p->shift_active, a.maskc,
+        k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, a.shift, a.maskc,
                                                      static_cast<float2*>(bf->probe_grad));
       else
-        k_probe_final_gt<N><<<M, kGtThreads<N>, 0, s>>>(psum, p->shift_active, a.maskc,
+        k_probe_final_gt<N><<<M, kGtThreads<N>, 0, s>>>(psum, a.shift, a.maskc,
                                                      static_cast<float2*>(bf->probe_grad), a.scratch);
     }));
   }
@@ -3031,7 +3228,7 @@ struct Access {
   int buf;           // WB_* or an external buffer (kExt + PB_*)
 };
 enum { PB_OBJ = 0, PB_PROBE, PB_MEASURED, PB_LOSS_SSE, PB_OBJ_GRAD, PB_PROBE_GRAD, PB_POS_GRAD, PB_SIM,
-       PB_BG_GRAD, PB_BC, PB_BC_GRAD, PB_COUNT };
+       PB_BG_GRAD, PB_BC, PB_BC_GRAD, PB_AB, PB_MARKER, PB_AB_GRAD, PB_COUNT };
 constexpr int kExt = 100;
 
 // the bytes of every range an op touches; gt: windows larger than 128 px
@@ -3056,9 +3253,32 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
     if (!base) return;
     add(base, ((size_t(B) * MK - 1) * a.st_slots + last_slot + 1) * c8, w, WB_STASH);
   };
+  // each window's aberration plate
+  auto plates = [&]() {
+    if (a.NA) add(a.plate, size_t(B) * c8, false, WB_PLATE);
+  };
   switch (o.kind) {
     case KT_TABLES:
       shared(WB_H, true); shared(WB_H0, true); shared(WB_MASKC, true); shared(WB_ZCOL, true);
+      break;
+    case KT_AB_TABLE:
+      shared(WB_ABT, true);
+      break;
+    case KT_AB_PLATE:
+      shared(WB_ABT, false);
+      add(a.ab, size_t(p->n_markers) * a.NA * 4, false, kExt + PB_AB);
+      add(a.marker, size_t(p->n_positions) * 4, false, kExt + PB_MARKER);
+      add(a.plate, size_t(B) * c8, true, WB_PLATE);
+      break;
+    case KT_AB_PART:
+      shared(WB_ABT, false);
+      add(a.ab_w, size_t(B) * M * c4, false, WB_AB_W);
+      add(a.ab_part, size_t(B) * a.NA * 8, true, WB_AB_PART);
+      break;
+    case KT_AB_FINAL:
+      add(a.ab_part, size_t(B) * a.NA * 8, false, WB_AB_PART);
+      add(a.marker, size_t(p->n_positions) * 4, false, kExt + PB_MARKER);
+      add(a.ab_grad, size_t(p->n_markers) * a.NA * 4, true, kExt + PB_AB_GRAD);
       break;
     case KT_FPROBE:
       add(bf->probe, size_t(M) * c8, false, kExt + PB_PROBE);
@@ -3086,6 +3306,7 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
     case KT_PROBE_SHIFT:
       shared(WB_FPROBE, false); shared(WB_MASKC, false); shared(WB_ZCOL, false);
       add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
+      if (!a.moved) plates();
       add(a.Pp, size_t(B) * M * c8, true, WB_PP);
       tiles(B * M);
       break;
@@ -3154,6 +3375,8 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
       add(a.bc_part, size_t(B) * M * 8, true, WB_BC_PART);
       if (a.bc_part && !a.shift) add(bf->probe, size_t(M) * c8, false, kExt + PB_PROBE);
+      plates();
+      add(a.ab_w, size_t(B) * M * c4, true, WB_AB_W);
       tiles(a.G * M);
       break;
     case KT_BC_FINAL:
@@ -3223,6 +3446,11 @@ int plan_check_n(const pty_problem* p, int n_seq, int nl, int calls, int accumul
     bf.beam_current = static_cast<const float*>(fake(1 + PB_BC));
     bf.beam_current_grad = static_cast<float*>(fake(1 + PB_BC_GRAD));
   }
+  if (p->n_aberrations) {
+    bf.aberrations = static_cast<const float*>(fake(1 + PB_AB));
+    bf.aberration_marker = static_cast<const int*>(fake(1 + PB_MARKER));
+  }
+  if (p->step_aberrations) bf.aberrations_grad = static_cast<float*>(fake(1 + PB_AB_GRAD));
   int rc = validate_buffers(p, &bf);
   if (rc) return rc;
   const int n_batches = (n_seq + L.B - 1) / L.B;

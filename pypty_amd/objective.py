@@ -9,12 +9,14 @@ including the per-batch data preprocessing (shift, bin, multiplier, pad;
 utils.py:587-653), the static background and its gradient
 (objective.py:359-361, 525-526, 588-589) and, with ``load_one_by_one`` and
 patterns in host memory, streaming of each batch's patterns through pinned
-buffers (objective.py:383-408), and the per-position beam current and its
-gradient (objective.py:446-450, 688-698).  The constraint terms
-(objective.py:765-921) are added by ``pypty_amd.constraints`` on the device.
-Arguments that select features outside that path (near field, tilts,
-aberrations, phase plates, per-slice distances, upsampling) raise
-``NotImplementedError`` instead of silently computing something else.
+buffers (objective.py:383-408), the per-position beam current and its
+gradient (objective.py:446-450, 688-698), and the per-position aberration
+phase plates and their gradient (objective.py:320-326, 459-465, 673-681).
+The constraint terms (objective.py:765-921) are added by
+``pypty_amd.constraints`` on the device.  Arguments that select features
+outside that path (near field, tilts, phase plates from a file, probe
+markers, per-slice distances, upsampling) raise ``NotImplementedError``
+instead of silently computing something else.
 
 Binning follows the evident intent of utils.py:639-644 (sum of the
 data_bin x data_bin blocks); the reference's loop indexes an undefined name
@@ -85,6 +87,11 @@ class PassConfig:
     step_static_background: bool = False
     # also return the gradient of the beam current (objective.py:688-698)
     step_beam_current: bool = False
+    # aberration rows of n_aberrations coefficients (objective.py:320-326), 0 for none, and
+    # whether to return their gradient (objective.py:673-681)
+    n_aberrations: int = 0
+    n_markers: int = 0
+    step_aberrations: bool = False
 
     @property
     def compressed(self) -> bool:
@@ -103,7 +110,8 @@ class PassConfig:
             float(self.tan_y), float(self.damping_cutoff), float(self.edge_multiplier), int(self.n_masks),
             int(self.raw_n), int(self.data_shift_y), int(self.data_shift_x), int(self.data_bin), int(self.data_pad),
             float(self.data_multiplier), int(self.checkpoint_slices), int(self.lanes),
-            int(bool(self.step_static_background)), int(bool(self.step_beam_current)))
+            int(bool(self.step_static_background)), int(bool(self.step_beam_current)), int(self.n_aberrations),
+            int(self.n_markers), int(bool(self.step_aberrations)))
 
 
 class PtychoPass:
@@ -136,7 +144,7 @@ class PtychoPass:
     def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
             pos_grad=None, sim_patterns=None, sync=True, masks=None, measured_by_sequence=False, accumulate=False,
             loss_sse=None, static_background=None, static_background_grad=None, beam_current=None,
-            beam_current_grad=None):
+            beam_current_grad=None, aberrations=None, aberration_marker=None, aberrations_grad=None):
         """Launch the pass.  probe_mnn is mode-major (M, n, n).  Returns (loss, sse) as floats
         when ``sync`` else the device tensor [loss, sse].
 
@@ -148,7 +156,11 @@ class PtychoPass:
         float32 receiving its gradient when the config steps it (objective.py:588-589);
         beam_current: (P,) float32, each position's probe taken |beam_current| times before its shift
         (objective.py:446-450); beam_current_grad: (P,) float32 receiving its gradient on the rows of
-        the sequence when the config steps it (objective.py:688-698)."""
+        the sequence when the config steps it (objective.py:688-698); aberrations: (n_markers,
+        n_aberrations) float32 with aberration_marker (P,) int32, the row of each position, whose
+        phase plate multiplies its probe spectrum (objective.py:320-326, 459-465; a row outside
+        [0, n_markers) acts as none); aberrations_grad: (n_markers, n_aberrations) float32 receiving
+        their gradient when the config steps them (objective.py:673-681)."""
         c = self.cfg
         for t, name in ((obj, "obj"), (probe_mnn, "probe"), (positions, "positions"), (measured, "measured"),
                         (sequence, "sequence")):
@@ -195,6 +207,15 @@ class PtychoPass:
                     raise ValueError(f"pypty_amd: {name} must be contiguous float32 ({c.n_positions},)")
         if c.step_beam_current and (beam_current is None or beam_current_grad is None):
             raise ValueError("beam_current and beam_current_grad required")
+        if c.n_aberrations:
+            shapes = ((aberrations, "aberrations", torch.float32, (c.n_markers, c.n_aberrations)),
+                      (aberration_marker, "aberration_marker", torch.int32, (c.n_positions,)))
+            if c.step_aberrations:
+                shapes += ((aberrations_grad, "aberrations_grad", torch.float32, (c.n_markers, c.n_aberrations)),)
+            for t, name, dt, shape in shapes:
+                _lib.require_device_tensor(t, name)
+                if t.dtype != dt or tuple(t.shape) != shape or not t.is_contiguous():
+                    raise ValueError(f"pypty_amd: {name} must be contiguous {dt} {shape}")
         out = self.loss_sse if loss_sse is None else loss_sse
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         bufs = _lib.PtyBuffers(
@@ -204,7 +225,9 @@ class PtychoPass:
             _lib.PTY_MEASURED_BY_SEQUENCE if measured_by_sequence else _lib.PTY_MEASURED_BY_POSITION,
             int(bool(accumulate)), ptr(static_background),
             ptr(static_background_grad) if c.step_static_background else None, ptr(beam_current),
-            ptr(beam_current_grad) if c.step_beam_current else None)
+            ptr(beam_current_grad) if c.step_beam_current else None,
+            ptr(aberrations) if c.n_aberrations else None, ptr(aberration_marker) if c.n_aberrations else None,
+            ptr(aberrations_grad) if c.step_aberrations else None)
         rc = self.lib.pty_loss_and_direction(ctypes.byref(self.c_problem), ctypes.byref(bufs),
                                               _lib.stream_ptr(self.device))
         _lib.check(rc, "loss_and_direction")
@@ -215,7 +238,8 @@ class PtychoPass:
 
     def run_streamed(self, obj, probe_mnn, positions, corrections, measured_host, sequence_host, *, obj_grad=None,
                      probe_grad=None, pos_grad=None, masks=None, static_background=None,
-                     static_background_grad=None, beam_current=None, beam_current_grad=None):
+                     static_background_grad=None, beam_current=None, beam_current_grad=None, aberrations=None,
+                     aberration_marker=None, aberrations_grad=None):
         """The same pass with the measured patterns left in host memory (objective.py:383-408,
         load_one_by_one): each batch's rows are gathered into one of two pinned buffers, copied to the
         device on a side stream while the previous batch computes, and the batch runs as an
@@ -258,7 +282,8 @@ class PtychoPass:
                      probe_grad=probe_grad, pos_grad=pos_grad, sync=False, masks=masks, measured_by_sequence=True,
                      accumulate=k > 0, loss_sse=sums[k], static_background=static_background,
                      static_background_grad=static_background_grad, beam_current=beam_current,
-                     beam_current_grad=beam_current_grad)
+                     beam_current_grad=beam_current_grad, aberrations=aberrations,
+                     aberration_marker=aberration_marker, aberrations_grad=aberrations_grad)
             free[slot].record(main)
             if k + 1 < len(starts):
                 stage(k + 1)  # the host gathers the next batch while this one computes
@@ -361,7 +386,6 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     _unsupported("tilt_mode", bool(tilt_mode))
     _unsupported("static_background with a compressed loss",
                  static_background is not None and "compressed" in str(algorithm_type))
-    _unsupported("aberration_marker", aberration_marker is not None)
     _unsupported("probe_marker", probe_marker is not None)
     _unsupported("phase_plate_in_h5", phase_plate_in_h5 is not None)
     _unsupported("phase_only_obj", bool(phase_only_obj))
@@ -428,6 +452,22 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         bc = bc_full[:P].contiguous()
         if step_bc:
             bc_grad = torch.empty_like(bc)
+    # aberrations per position (objective.py:320-326, 459-465, 673-681): only with a marker, as there
+    ab = ab_marker = ab_grad = None
+    step_ab = aberration_marker is not None and aberrations_array is not None and bool(this_step_aberrations_array)
+    if aberration_marker is not None and aberrations_array is not None:
+        ab = _as_device(aberrations_array, torch.float32, device)
+        if ab.dim() != 2 or not 1 <= ab.shape[1] <= _lib.PTY_MAX_ABERRATIONS:
+            raise ValueError("pypty_amd.loss_and_direction: aberrations_array must be (rows, 1..64 coefficients)")
+        marker_host = _host(aberration_marker, np.int64).ravel()
+        if marker_host.shape[0] != positions.shape[0]:
+            raise ValueError("pypty_amd.loss_and_direction: aberration_marker needs a row per position")
+        if marker_host.size and (marker_host.min() < -ab.shape[0] or marker_host.max() >= ab.shape[0]):
+            raise IndexError("pypty_amd.loss_and_direction: aberration_marker indexes a missing row")
+        marker_host = np.where(marker_host < 0, marker_host + ab.shape[0], marker_host)  # numpy's negative rows
+        ab_marker = torch.as_tensor(marker_host.astype(np.int32)).to(device)
+        if step_ab:
+            ab_grad = torch.empty_like(ab)
     compressed = "compressed" in algorithm_type
     masks_dev = None
     if compressed:
@@ -453,7 +493,9 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         data_shift_y=shift_y, data_shift_x=shift_x, data_bin=1 if compressed else int(data_bin), data_pad=pad,
         data_multiplier=1.0 if compressed else float(data_multiplier), checkpoint_slices=int(checkpoint_slices),
         lanes=1 if stream else 4,  # a streamed pass runs one batch per call
-        step_static_background=step_bg, step_beam_current=step_bc)
+        step_static_background=step_bg, step_beam_current=step_bc,
+        n_aberrations=0 if ab is None else int(ab.shape[1]), n_markers=0 if ab is None else int(ab.shape[0]),
+        step_aberrations=step_ab)
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None
@@ -462,12 +504,14 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         loss, sse = engine.run_streamed(obj, probe_mnn, positions, corrections, measured, seq.cpu().numpy(),
                                         obj_grad=object_grad, probe_grad=probe_grad_mnn, pos_grad=pos_grad,
                                         masks=masks_dev, static_background=bg, static_background_grad=bg_grad,
-                                        beam_current=bc, beam_current_grad=bc_grad)
+                                        beam_current=bc, beam_current_grad=bc_grad, aberrations=ab,
+                                        aberration_marker=ab_marker, aberrations_grad=ab_grad)
     else:
         loss, sse = engine.run(obj, probe_mnn, positions, corrections, measured, seq, obj_grad=object_grad,
                                probe_grad=probe_grad_mnn, pos_grad=pos_grad, masks=masks_dev,
                                static_background=bg, static_background_grad=bg_grad, beam_current=bc,
-                               beam_current_grad=bc_grad)
+                               beam_current_grad=bc_grad, aberrations=ab, aberration_marker=ab_marker,
+                               aberrations_grad=ab_grad)
     if loss != loss:
         raise ValueError("A very specific bad thing. Loss is Nan.")
     probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe
@@ -541,6 +585,11 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         if bc_grad is not None:
             full[:bc.numel()] = bc_grad
         bc_grad = full
+    # objective.py:379: zeros of the aberrations' shape when there are some, else 0
+    if aberrations_array is None:
+        ab_grad = 0
+    elif ab_grad is None:
+        ab_grad = torch.zeros_like(_as_device(aberrations_array, torch.float32, device))
     out = [_arrays.like(t, kind, template)
-           for t in (object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, bc_grad)]
-    return (loss, sse, out[0], out[1], out[2], out[3], out[4], 0, out[5], constraint_contributions)
+           for t in (object_grad, probe_grad, pos_grad, tilts_grad, bg_grad, bc_grad, ab_grad)]
+    return (loss, sse, out[0], out[1], out[2], out[3], out[4], out[6], out[5], constraint_contributions)

@@ -59,10 +59,20 @@ def test_every_tile_family(n, S, K, M, P, batch, ck, lanes):
                                                       data_pad=0, data_multiplier=2.0),
     dict(step_static_background=True), dict(planar=True), dict(algorithm="lsq_sqrt_2"),
     dict(step_beam_current=True), dict(step_beam_current=True, shift_active=False, step_pos=False),
+    dict(n_aberrations=7, n_markers=3, step_aberrations=True),
+    dict(n_aberrations=12, n_markers=10, step_aberrations=True, shift_active=False, step_pos=False,
+         step_beam_current=True),
+    dict(n_aberrations=3, n_markers=1, step_aberrations=False, shift_active=False, step_pos=False),
 ])
 def test_data_format_options(extra):
     n = 32 if extra.get("raw_n") != 64 else 32
     _check(_cfg(n, 3, 1, 2, 10, 3, lanes=3, **extra), calls=2)
+
+
+@pytest.mark.parametrize("n", [128, 192, 256])
+def test_aberrations_on_the_large_tiles(n):
+    _check(_cfg(n, 3, 1, 2, 9, 2, lanes=4, n_aberrations=5, n_markers=4, step_aberrations=True, shift_active=False,
+                step_pos=False), calls=2)
 
 
 def test_streamed_accumulating_calls():
