@@ -279,6 +279,10 @@ int pty_multidot_c64(const void* const* a, const void* const* b, int npairs, int
  * the axpy steps of the two-loop recursion and of the trial point. */
 int pty_axpby_c64(void* y, const void* x, int64_t n, float alpha, float beta, void* stream);
 
+/* The same over n float32 (any n): the steps of the real blocks (positions,
+ * static background, beam current, aberrations), iterative.py:573-681, 770. */
+int pty_axpby_f32(float* y, const float* x, int64_t n, float alpha, float beta, void* stream);
+
 /* L1 norms of the object's phase and log-amplitude (pypty/objective.py:1286-1320,
  * compute_full_l1_constraint, called at :811-830): *out = phase_weight *
  * sum |arg obj| + abs_weight * sum |log |obj|| (FP64 sums), and, when grad is

@@ -27,7 +27,8 @@ EXPORTED = (
     "pty_status_string", "pty_version", "pty_supported_window", "pty_fft2_c64", "pty_bandlimit_c64",
     "pty_scatteradd_c64", "pty_workspace_bytes", "pty_loss_and_direction", "pty_tables_bytes",
     "pty_ms_forward", "pty_ms_adjoint", "pty_timing_enable", "pty_timing_read", "pty_timing_reset",
-    "pty_fourier_clean_c64", "pty_plane_fft2_c64", "pty_multidot_c64", "pty_axpby_c64", "pty_phase_grad_c64",
+    "pty_fourier_clean_c64", "pty_plane_fft2_c64", "pty_multidot_c64", "pty_axpby_c64", "pty_axpby_f32",
+    "pty_phase_grad_c64",
     "pty_l1_constraint_c64", "pty_plan_check", "pty_loss_and_direction_static", "pty_center_of_mass",
     "pty_gemm_bf16", "pty_cnn_im2col", "pty_cnn_col2im", "pty_cnn_bias_act", "pty_cnn_act_grad",
     "pty_cnn_bias_grad", "pty_cnn_to_bf16", "pty_cnn_mse", "pty_cnn_adam",
@@ -116,6 +117,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
     lib.pty_multidot_c64.argtypes = [P(vp), P(vp), i, i64, i, vp, vp, vp]
     lib.pty_axpby_c64.restype = i
     lib.pty_axpby_c64.argtypes = [vp, vp, i64, ctypes.c_float, ctypes.c_float, vp]
+    lib.pty_axpby_f32.restype = i
+    lib.pty_axpby_f32.argtypes = [vp, vp, i64, ctypes.c_float, ctypes.c_float, vp]
     lib.pty_phase_grad_c64.restype = i
     lib.pty_phase_grad_c64.argtypes = [vp, vp, vp, i64, vp]
     lib.pty_center_of_mass.restype = i

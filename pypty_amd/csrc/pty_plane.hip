@@ -553,6 +553,15 @@ int pty_axpby_c64(void* y, const void* x, int64_t n, float alpha, float beta, vo
   return plane_launch_check();
 }
 
+int pty_axpby_f32(float* y, const float* x, int64_t n, float alpha, float beta, void* stream) {
+  if (!y || n < 0) return PTY_ERR_ARG;
+  if (n == 0) return PTY_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int grid = int(std::min<size_t>((size_t(n) + 255) / 256, 8192));
+  k_axpby<<<grid, 256, 0, s>>>(y, x, size_t(n), alpha, beta);
+  return plane_launch_check();
+}
+
 int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float phase_weight, void* grad,
                           double* out, double* partial, void* stream) {
   if (!obj || n < 0 || !out || !partial) return PTY_ERR_ARG;

@@ -7,8 +7,10 @@ tuple and its module-global state (``obj``, ``probe``, ``positions``,
 ``history_bfgs``).  Supported variables: object, probe and sub-pixel
 positions (the ones the benchmark configurations refine), the static
 background held in ``static_background`` (in every loss evaluation; refined
-with ``this_step_static_background``) and the per-position beam current held
-in ``beam_current`` (likewise; refined with ``this_beam_current_step``).
+with ``this_step_static_background``), the per-position beam current held
+in ``beam_current`` (likewise; refined with ``this_beam_current_step``) and
+the aberration rows held in ``aberrations_array`` with ``aberration_marker``
+(likewise; refined with ``this_step_aberrations_array``).
 Every large
 operation runs on the HIP device through this package's kernels:
 loss and gradients (``objective.loss_and_direction``), the band-limits
@@ -101,9 +103,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     Returns (total_loss, sse, constraint_contributions, actual_step, count,
     d_value, new_d_value, warnings, counter_failed_epochs).
     """
-    global obj, probe, positions_correction, static_background, beam_current, history_bfgs
+    global obj, probe, positions_correction, static_background, beam_current, aberrations_array, history_bfgs
     _unsupported("this_step_tilts", bool(this_step_tilts))
-    _unsupported("this_step_aberrations_array", bool(this_step_aberrations_array))
     _unsupported("phase_only_obj", bool(phase_only_obj))
     _unsupported("tune_only_probe_phase", bool(tune_only_probe_phase))
     _unsupported("tune_only_probe_abs", bool(tune_only_probe_abs))
@@ -134,11 +135,21 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         raise ValueError("pypty_amd.bfgs_update: refining the beam current needs a beam_current")
     if beam_current is not None and not isinstance(beam_current, torch.Tensor):
         beam_current = torch.as_tensor(np.asarray(beam_current, np.float32), device=obj.device)
+    # the aberrations, a real (rows, coefficients) block (iterative.py:478, 570, 590-672, 689, 770,
+    # 854-871, 904-909, 1028); in every loss evaluation when there is a marker, as objective.py:320 uses
+    # them only then
+    update_ab = this_step_aberrations_array > 0
+    if update_ab and (aberration_marker is None or aberrations_array is None):
+        raise ValueError("pypty_amd.bfgs_update: refining the aberrations needs aberrations_array and "
+                         "aberration_marker")
+    if aberrations_array is not None and not isinstance(aberrations_array, torch.Tensor):
+        aberrations_array = torch.as_tensor(np.asarray(aberrations_array, np.float32), device=obj.device)
     for key in ("static_background_hist_s", "static_background_hist_y", "beam_current_hist_s",
-                "beam_current_hist_y"):
+                "beam_current_hist_y", "aberrations_hist_s", "aberrations_hist_y"):
         history_bfgs.setdefault(key, [0] * len(history_bfgs["rho_hist"]))
     history_bfgs.setdefault("gamma_static", 1)
     history_bfgs.setdefault("gamma_beam_current", 1)
+    history_bfgs.setdefault("gamma_aberrations", 1)
     cut = damping_cutoff_multislice - update_extra_cut
     clean = not lazy_clean
     # an infinite probe_reg_weight band-limits the probe and its gradient to the aperture instead of
@@ -154,14 +165,14 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         probe_reg_weight = 0
         probe = _clean_aperture(probe)
 
-    def evaluate(o, p, pc, b, c):
+    def evaluate(o, p, pc, b, c, ab):
         out = objective.loss_and_direction(
             o, p, positions, pc, tilts, tilts_correction, this_slice_distances, edge_slice_thickness_multiplier,
             measured_array, algorithm_type, this_wavelength, update_probe, update_obj, update_pos, False, masks,
             pixel_size_x_A, pixel_size_y_A, recon_type, Cs, defocus_array, alpha_near_field, damping_cutoff_multislice,
             0, propmethod, this_chopped_sequence, load_one_by_one, data_multiplier, data_pad, phase_plate_in_h5,
             this_loss_weight, data_bin, data_shift_vector, upsample_pattern, b, update_bg, tilt_mode,
-            aberration_marker, probe_marker, None, compute_batch, phase_only_obj, c, update_bc, False, default_float,
+            aberration_marker, probe_marker, ab, compute_batch, phase_only_obj, c, update_bc, update_ab, default_float,
             default_complex, xp, is_first_epoch, scan_size, scan_step_A, fast_axis_reg_weight_positions,
             slow_axis_reg_weight_positions, slow_axis_reg_weight_tilts, current_deformation_reg_weight_positions,
             current_deformation_reg_weight_tilts, fast_axis_reg_weight_tilts, aperture_mask, probe_reg_weight,
@@ -181,17 +192,20 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         # entry stored in such an epoch serves an epoch that starts refining it without a reset
         bgg = out[6] if update_bg else (torch.zeros_like(b) if isinstance(b, torch.Tensor) else 0)
         bcg = out[8] if update_bc else (torch.zeros_like(c) if isinstance(c, torch.Tensor) else 0)
-        return loss, sse, og, pg, posg, out[9], bgg, bcg
+        abg = out[7] if update_ab else (torch.zeros_like(ab) if isinstance(ab, torch.Tensor) else 0)
+        return loss, sse, og, pg, posg, out[9], bgg, bcg, abg
 
     if update_probe:
         probe = _clean(probe, cut, smooth_rolloff)
     if update_obj and clean:
         obj = _clean(obj, cut, smooth_rolloff)
 
-    on = (update_obj, update_probe, update_pos, update_bg, update_bc)
-    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s", "beam_current_hist_s")
-    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y", "beam_current_hist_y")
-    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static", "gamma_beam_current")
+    on = (update_obj, update_probe, update_pos, update_bg, update_bc, update_ab)
+    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s", "beam_current_hist_s",
+              "aberrations_hist_s")
+    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y", "beam_current_hist_y",
+              "aberrations_hist_y")
+    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static", "gamma_beam_current", "gamma_aberrations")
     NB = len(on)
 
     def direction_dot(u, g):
@@ -203,9 +217,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         return d
 
     if empty_hist:
-        total_loss, this_sse, og, pg, posg, cc, bgg, bcg = evaluate(obj, probe, positions_correction,
-                                                                    static_background, beam_current)
-        g = [og, pg, posg, bgg, bcg]
+        total_loss, this_sse, og, pg, posg, cc, bgg, bcg, abg = evaluate(obj, probe, positions_correction,
+                                                                         static_background, beam_current,
+                                                                         aberrations_array)
+        g = [og, pg, posg, bgg, bcg, abg]
         u_obj = _neg(og) if update_obj else None
         d_val_obj = _dot2(u_obj, u_obj) if update_obj else 0.0
         if d_val_obj == 0:
@@ -216,11 +231,14 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     else:
         total_loss, this_sse = history_bfgs["prev_loss"], history_bfgs["prev_sse"]
         g = [history_bfgs["obj_grad"], history_bfgs["probe_grad"], history_bfgs["pos_grad"],
-             history_bfgs.get("static_background_grad"), history_bfgs.get("beam_current_grad")]
+             history_bfgs.get("static_background_grad"), history_bfgs.get("beam_current_grad"),
+             history_bfgs.get("aberrations_grad")]
         if update_bg and not isinstance(g[3], torch.Tensor):
             g[3] = torch.zeros_like(static_background)  # a background held fixed until now
         if update_bc and not isinstance(g[4], torch.Tensor):
             g[4] = torch.zeros_like(beam_current)  # a current held fixed until now
+        if update_ab and not isinstance(g[5], torch.Tensor):
+            g[5] = torch.zeros_like(aberrations_array)  # aberrations held fixed until now
         cc = history_bfgs["constraint_contributions"]
         if update_probe:
             g[1] = _clean(g[1], cut, smooth_rolloff)
@@ -256,8 +274,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             if on[j] and gm > 0 and gm == gm:
                 lbfgs.axpby(u[j], None, gm, 0.0)
         d_value = direction_dot(u, g)
-    u_obj, u_probe, u_pos, u_bg, u_bc = u
-    og, pg, posg, bgg, bcg = g
+    u_obj, u_probe, u_pos, u_bg, u_bc, u_ab = u
+    og, pg, posg, bgg, bcg, abg = g
     count = -1
     new_d_value = d_value
     this_wolfe_1 = False
@@ -275,9 +293,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         new_bg = lbfgs.axpby(static_background.clone(), u_bg, 1.0, actual_step) if update_bg \
             else static_background
         new_bc = lbfgs.axpby(beam_current.clone(), u_bc, 1.0, actual_step) if update_bc else beam_current
-        new_total_loss, new_sse, nog, npg, nposg, ncc, nbgg, nbcg = evaluate(new_obj, new_probe, new_pc, new_bg,
-                                                                             new_bc)
-        new_d_value = direction_dot(u, (nog, npg, nposg, nbgg, nbcg))
+        new_ab = lbfgs.axpby(aberrations_array.clone(), u_ab, 1.0, actual_step) if update_ab else aberrations_array
+        new_total_loss, new_sse, nog, npg, nposg, ncc, nbgg, nbcg, nabg = evaluate(new_obj, new_probe, new_pc, new_bg,
+                                                                                   new_bc, new_ab)
+        new_d_value = direction_dot(u, (nog, npg, nposg, nbgg, nbcg, nabg))
         this_wolfe_1 = utils.wolfe_1(total_loss, new_total_loss, d_value, actual_step, wolfe_c1_constant)
         this_wolfe_2 = utils.wolfe_2(d_value, new_d_value, wolfe_c2_constant)
         if print_flag >= 3:
@@ -316,26 +335,29 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     s_pos = lbfgs.lincomb(u_pos, actual_step) if update_pos else 0
     s_bg = lbfgs.lincomb(u_bg, actual_step) if update_bg else 0
     s_bc = lbfgs.lincomb(u_bc, actual_step) if update_bc else 0
+    s_ab = lbfgs.lincomb(u_ab, actual_step) if update_ab else 0
     y_obj = lbfgs.axpby(nog.clone(), og, 1.0, -1.0) if update_obj else 0
     y_probe = lbfgs.axpby(npg.clone(), pg, 1.0, -1.0) if update_probe else 0
     y_pos = lbfgs.axpby(nposg.clone(), posg, 1.0, -1.0) if update_pos else 0
     y_bg = lbfgs.axpby(nbgg.clone(), bgg, 1.0, -1.0) if update_bg else 0
     y_bc = lbfgs.axpby(nbcg.clone(), bcg, 1.0, -1.0) if update_bc else 0
+    y_ab = lbfgs.axpby(nabg.clone(), abg, 1.0, -1.0) if update_ab else 0
     for key, v in (("obj_hist_s", s_obj), ("probe_hist_s", s_probe), ("positions_hist_s", s_pos),
-                   ("static_background_hist_s", s_bg), ("beam_current_hist_s", s_bc), ("obj_hist_y", y_obj),
-                   ("probe_hist_y", y_probe), ("positions_hist_y", y_pos), ("static_background_hist_y", y_bg),
-                   ("beam_current_hist_y", y_bc)):
+                   ("static_background_hist_s", s_bg), ("beam_current_hist_s", s_bc), ("aberrations_hist_s", s_ab),
+                   ("obj_hist_y", y_obj), ("probe_hist_y", y_probe), ("positions_hist_y", y_pos),
+                   ("static_background_hist_y", y_bg), ("beam_current_hist_y", y_bc), ("aberrations_hist_y", y_ab)):
         h[key].append(v)
-    for key in ("tilts_hist_s", "aberrations_hist_s", "tilts_hist_y", "aberrations_hist_y"):
+    for key in ("tilts_hist_s", "tilts_hist_y"):
         h[key].append(0)
     h["prev_loss"], h["prev_sse"] = new_total_loss, new_sse
     h["obj_grad"], h["probe_grad"], h["pos_grad"], h["static_background_grad"] = nog, npg, nposg, nbgg
-    h["beam_current_grad"] = nbcg
+    h["beam_current_grad"], h["aberrations_grad"] = nbcg, nabg
     h["constraint_contributions"] = ncc
     yiTsi_tot = 0.0
     for y, s, key, flag in ((y_obj, s_obj, "gamma_obj", update_obj), (y_probe, s_probe, "gamma_probe", update_probe),
                             (y_pos, s_pos, "gamma_pos", update_pos), (y_bg, s_bg, "gamma_static", update_bg),
-                            (y_bc, s_bc, "gamma_beam_current", update_bc)):
+                            (y_bc, s_bc, "gamma_beam_current", update_bc),
+                            (y_ab, s_ab, "gamma_aberrations", update_ab)):
         if flag:
             (yy,) = lbfgs.dots([(y, y)])
             ys = _dot2(y, s)
@@ -354,6 +376,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         static_background = new_bg
     if update_bc:
         beam_current = new_bc
+    if update_ab:
+        aberrations_array = new_ab
     return total_loss, this_sse, cc, actual_step, count, d_value, new_d_value, warnings, counter_failed_epochs
 
 
@@ -366,9 +390,11 @@ def _per_epoch(v, epoch):
     return v(epoch) if callable(v) else v
 
 
-def _save_epoch_files(folder, epoch, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None):
+def _save_epoch_files(folder, epoch, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None, ab=None):
     """checkpoint_{beam_current,obj,probe,positions,stback}_epoch_N.npy of the variables refined in
-    epoch N, as pypty/utils.py:1942-1975 writes them when save_checkpoints_every_epoch selects N."""
+    epoch N, as pypty/utils.py:1942-1975 writes them when save_checkpoints_every_epoch selects N, and
+    checkpoint_aberrations_epcoh_N.npy (the reference's spelling, which its restart reads) for
+    aberrations being refined."""
     import os
 
     if not folder:
@@ -385,11 +411,13 @@ def _save_epoch_files(folder, epoch, o, p, pos, corr, save_obj, save_probe, save
         np.save(name("positions"), (pos.double() + corr.double()).cpu().numpy())
     if isinstance(bg, torch.Tensor):
         np.save(name("stback"), bg.cpu().numpy())
+    if isinstance(ab, torch.Tensor):
+        np.save(os.path.join(folder, f"checkpoint_aberrations_epcoh_{epoch}.npy"), ab.cpu().numpy())
 
 
-def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None):
-    """co.npy / cp.npy / cg.npy, cs.npy for a background and cb.npy for a beam current being refined, as
-    pypty/utils.py:2085-2118 writes them."""
+def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None, ab=None):
+    """co.npy / cp.npy / cg.npy, cs.npy for a background, cb.npy for a beam current and ca.npy for
+    aberrations being refined, as pypty/utils.py:2085-2118 writes them."""
     import os
 
     if not folder:
@@ -405,6 +433,8 @@ def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg
         np.save(os.path.join(folder, "cs.npy"), bg.cpu().numpy())
     if isinstance(bc, torch.Tensor):
         np.save(os.path.join(folder, "cb.npy"), bc.cpu().numpy())
+    if isinstance(ab, torch.Tensor):
+        np.save(os.path.join(folder, "ca.npy"), ab.cpu().numpy())
 
 
 def lambda_source(f):
@@ -506,7 +536,8 @@ def run(pypty_params):
     min_step, hist_length, update_step_bfgs, loss_weight, allow_subPixel_shift,
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
     save_loss_log, print_flag, static_background and update_static_background,
-    beam_current and update_beam_current,
+    beam_current and update_beam_current, aberrations_array, aberration_marker and
+    update_aberrations_array,
     phase_norm_weight and abs_norm_weight (values or callables of the epoch), and two keys
     not in the reference:
     checkpoint_slices (keep every k-th slice's wave and recompute the rest in
@@ -521,7 +552,7 @@ def run(pypty_params):
     import time
 
     global obj, probe, positions, positions_correction, tilts, tilts_correction, static_background, beam_current
-    global history_bfgs
+    global aberrations_array, history_bfgs
     get = pypty_params.get
     dev = torch.device("cuda", torch.cuda.current_device())
     dataset = get("dataset", None)
@@ -585,6 +616,11 @@ def run(pypty_params):
     beam_current = get("beam_current", None)
     if beam_current is not None:
         beam_current = _arrays.to_torch(beam_current, dev, torch.float32).reshape(-1).contiguous()
+    # aberration rows and the row of each position (pypty/iterative.py:176-183); without a marker they
+    # do not enter the model (objective.py:320)
+    aberrations_array = _arrays.to_torch(get("aberrations_array", np.array([[0.0]])), dev, torch.float32)
+    aberrations_array = aberrations_array.reshape(aberrations_array.shape[0], -1).contiguous()
+    aberration_marker = get("aberration_marker", None)
     measured = dataset if isinstance(dataset, torch.Tensor) else torch.from_numpy(dataset).to(dev)
     folder = get("output_folder", "")
     print_flag = get("print_flag", 3)
@@ -603,7 +639,7 @@ def run(pypty_params):
     reset_bfgs_history()
     t0 = time.time()
     counter_failed = 0
-    fail_pos, fail_bg, fail_probe = np.inf, np.inf, np.inf
+    fail_pos, fail_bg, fail_ab, fail_probe = np.inf, np.inf, np.inf, np.inf
     prev_steps = prev_constraints = 0
     losses = []
     is_first = True
@@ -616,15 +652,19 @@ def run(pypty_params):
         s_pos = _per_epoch(get("update_probe_pos", 0), epoch)
         s_bg = _per_epoch(get("update_static_background", 0), epoch) if static_background is not None else 0
         s_bc = _per_epoch(get("update_beam_current", 0), epoch)
+        s_ab = _per_epoch(get("update_aberrations_array", 0), epoch)
         first_step = _per_epoch(step_param, epoch)  # per epoch, as pypty/iterative.py:325-336 reads it
         # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, the static
-        # background after 4, the probe after 6 (tilts and aberrations are not refined here)
+        # background after 4, the aberrations after 5, the probe after 6 (tilts are not refined here)
         if counter_failed >= 2:
             fail_pos = epoch
             warnings += "\nWARNING: Scan grid refinement was manually disabled for the next 10 epochs!"
             if counter_failed >= 4:
                 fail_bg = epoch
                 warnings += "\nWARNING: Static background refinement was manually disabled for the next 10 epochs!"
+            if counter_failed >= 5:
+                fail_ab = epoch
+                warnings += "\nWARNING: Aberrations refinement was manually disabled for the next 10 epochs!"
             if counter_failed >= 6:
                 fail_probe = epoch
                 warnings += "\nWARNING: Probe refinement was manually disabled for the next 10 epochs!"
@@ -632,6 +672,8 @@ def run(pypty_params):
             s_pos = 0
         if abs(fail_bg - epoch) < 10:
             s_bg = 0
+        if abs(fail_ab - epoch) < 10:
+            s_ab = 0
         if abs(fail_probe - epoch) < 10:
             s_probe = 0
         lw = _per_epoch(get("loss_weight", 1), epoch)
@@ -653,7 +695,7 @@ def run(pypty_params):
                            else np.asarray(window, np.float32))
         # the history is reset when the objective changes: a different set of refined variables or of
         # non-zero constraint weights (pypty/iterative.py:383-389; of its weights run() takes these)
-        steps = sum(int(x != 0) for x in (s_bc, s_obj, s_probe, s_pos, s_bg))
+        steps = sum(int(x != 0) for x in (s_bc, s_obj, s_probe, s_pos, s_bg, s_ab))
         constraints = sum(int(x != 0) for x in (lw, w_phase, w_abs, *[cw[k] for k in cw if k not in (
             "mixed_variance_sigma", "beta_wedge", "probe_mod_tol")], cw["mixed_variance_sigma"]))
         if epoch != epoch_prev and (steps != prev_steps or constraints != prev_constraints):
@@ -686,7 +728,8 @@ def run(pypty_params):
             data_shift_vector=[0, 0], smart_memory=False, default_float=np.float32, default_complex=np.complex64,
             default_int=np.int32, upsample_pattern=1,
             this_step_static_background=s_bg, tilt_mode=0,
-            fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=None, this_step_aberrations_array=0,
+            fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=aberration_marker,
+            this_step_aberrations_array=s_ab,
             probe_marker=None, compute_batch=get("compute_batch", 256), current_window=this_window,
             current_window_weight=cw["window_weight"], dynamically_resize_yx_object=False, lazy_clean=False,
             current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=s_bc,
@@ -708,10 +751,12 @@ def run(pypty_params):
                   f"linesearch {count + 1} time {time.time() - t0:.2f} s", flush=True)
         if save_inter and epoch % int(save_inter) == 0:
             _save_checkpoint(folder, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
-                             static_background if s_bg else None, beam_current if s_bc else None)
+                             static_background if s_bg else None, beam_current if s_bc else None,
+                             aberrations_array if s_ab else None)
         if save_every and epoch % int(save_every) == 0:
             _save_epoch_files(folder, epoch, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
-                              static_background if s_bg else None, beam_current if s_bc else None)
+                              static_background if s_bg else None, beam_current if s_bc else None,
+                              aberrations_array if s_ab else None)
         if save_log:
             free, total = torch.cuda.mem_get_info(dev)
             _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,
@@ -724,5 +769,6 @@ def run(pypty_params):
             epoch_callback(epoch, obj, probe)
     _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
     result = {"obj": obj, "probe": probe, "positions": positions.double() + positions_correction.double(),
-              "beam_current": beam_current, "losses": losses, "time_s": time.time() - t0}
+              "beam_current": beam_current, "aberrations_array": aberrations_array, "losses": losses,
+              "time_s": time.time() - t0}
     return result

@@ -2,7 +2,7 @@
 
 ``dots`` evaluates several real inner products Re<a, b> in one launch pair
 with FP64 accumulation (``pty_multidot_c64``); ``axpby`` is y <- alpha y +
-beta x (``pty_axpby_c64``).  Complex64 tensors are reduced as complex,
+beta x (``pty_axpby_c64``, ``pty_axpby_f32`` for the real blocks).  Complex64 tensors are reduced as complex,
 float32 tensors as real.
 """
 from __future__ import annotations
@@ -52,15 +52,13 @@ def axpby(y: torch.Tensor, x, alpha: float, beta: float) -> torch.Tensor:
         if x.dtype != y.dtype or x.numel() != y.numel():
             raise ValueError("pypty_amd.lbfgs.axpby: x and y must match")
     if y.dtype == torch.complex64:
-        n = y.numel()
+        fn = _lib.load().pty_axpby_c64
     elif y.dtype == torch.float32:
-        if y.numel() % 2:
-            raise ValueError("pypty_amd.lbfgs.axpby: float32 vectors of odd length are not supported")
-        n = y.numel() // 2
+        fn = _lib.load().pty_axpby_f32
     else:
         raise TypeError("pypty_amd.lbfgs.axpby: complex64 or float32 required")
-    rc = _lib.load().pty_axpby_c64(y.data_ptr(), None if x is None else x.data_ptr(), n, float(alpha), float(beta),
-                                   _lib.stream_ptr(y.device))
+    rc = fn(y.data_ptr(), None if x is None else x.data_ptr(), y.numel(), float(alpha), float(beta),
+            _lib.stream_ptr(y.device))
     _lib.check(rc, "axpby")
     return y
 

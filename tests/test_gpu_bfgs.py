@@ -203,6 +203,59 @@ def test_epochs_refining_the_beam_current_match_oracle():
         iterative.beam_current = None
 
 
+def test_epochs_refining_the_aberrations_match_oracle():
+    """Aberration rows (3 rows of 5 coefficients, an odd-length real block) refined as a sixth variable
+    with object and probe (iterative.py:478, 570-1028), then held for an epoch (the plates still in the
+    loss): losses, line-search counts and the updated variables follow the oracle."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 4 epochs refining object, probe and aberrations", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=False, seed=37, noise=0.02)
+    rng = np.random.default_rng(11)
+    ab = (rng.uniform(-1, 1, (3, 5)) * np.array([20.0, 10.0, 10.0, 300.0, 300.0])).astype(np.float32)
+    marker = rng.integers(0, 3, 10)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy(),
+             "ab": ab.copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5, "aberration_marker": marker}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.aberrations_array = torch.from_numpy(ab.copy()).to(dev)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    try:
+        for epoch in range(4):
+            ab_on = epoch < 3
+            np.random.seed(400 + epoch)
+            ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, update_ab=ab_on, actual_step=step_o)
+            np.random.seed(400 + epoch)
+            rg = call_loss_and_direction(iterative.bfgs_update,
+                                         **bfgs_kwargs(pr, actual_step=step_g, is_first_epoch=(epoch == 0),
+                                                       aberration_marker=marker,
+                                                       this_step_aberrations_array=int(ab_on)))
+            step_o, step_g = ro[2], rg[3]
+            e_ab = rel(iterative.aberrations_array.cpu().numpy(), state["ab"])
+            print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; step {rg[3]:.3e}/{ro[2]:.3e}; "
+                  f"count {rg[4]}/{ro[3]}; rel err aberrations {e_ab:.2e}", flush=True)
+            assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+            assert rg[4] == ro[3]
+            assert abs(rg[3] - ro[2]) <= 1e-5 * abs(ro[2])
+            assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+            assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
+            assert e_ab < 1e-4
+        assert float(np.abs(state["ab"] - ab).max()) > 0  # the aberrations moved
+    finally:
+        iterative.aberrations_array = None
+
+
 def test_epochs_with_l1_constraints_match_oracle():
     """phase_norm_weight and abs_norm_weight (objective.py:811-830) enter every loss evaluation and
     object gradient of the epoch."""

@@ -195,6 +195,61 @@ def test_run_refines_the_beam_current_and_writes_it(tmp_path):
         iterative.beam_current = None
 
 
+def test_run_refines_the_aberrations_and_writes_them(tmp_path):
+    """Patterns recorded with a probe defocused by a different amount in each half of the scan (rows
+    0 and 1 of aberration_marker), reconstructed from the true object and the in-focus probe (scaled
+    to the dose) with object and per-row defocus C10 refined from 0 (pypty/iterative.py:176-183, 419): the loss falls
+    monotonically, each row's C10 moves towards its true value, and ca.npy and
+    checkpoint_aberrations_epcoh_N.npy hold the last rows (pypty/utils.py:1972-1973, 2115-2117)."""
+    import torch
+
+    from oracle import ms_oracle as orc
+    from pypty_amd import iterative, synth
+
+    print("[run] 8 epochs refining object and per-row defocus", flush=True)
+    dev = torch.device("cuda", 0)
+    px, n = 0.2, 64
+    lam = synth.electron_wavelength(300.0)
+    true_obj = synth.hex_lattice_object(120, 120, px, 2.46, 0.3, 0.5, seed=25)
+    probe = synth.make_probe(n, px, 300.0, 20.0, 0.0, 1, 0.0, seed=26)
+    pos = synth.raster_scan(8, 8, 6.0, origin_px=(0.0, 0.0), jitter_px=0.0, seed=27)
+    ints, corr = synth.split_positions(pos)
+    P = pos.shape[0]
+    marker = (np.arange(P) >= P // 2).astype(np.int64)
+    c10 = np.array([15.0, -10.0])
+    q2, qx, qy, _, mask = orc.create_spatial_frequencies(px, px, n, 2 / 3, 0.0, np.float64)
+    meas = np.zeros((P, n, n), np.float32)
+    for r in range(2):
+        # the probe behind the row's plate: exp(-i pi lambda q^2 C10) times the band mask
+        plate = np.exp(-1j * np.pi * lam * q2 * c10[r]) * mask
+        pr_r = np.fft.ifft2(np.fft.fft2(probe[:, :, 0]) * plate)[:, :, None].astype(np.complex64)
+        m = synth.simulate(true_obj, pr_r, ints, corr, px, 2.0, lam, 1e6, seed=28 + r, device=dev)
+        meas[marker == r] = m[marker == r]
+    # the probe scaled to the dose (the modelled pattern holds n^2 sum |probe|^2), so that the loss
+    # measures the pattern's shape rather than its total
+    probe = (probe * np.sqrt(meas.sum(axis=(1, 2)).mean() / (n * n))).astype(np.complex64)
+    out = str(tmp_path) + os.sep
+    params = dict(dataset=meas, probe=probe, positions=pos, obj=true_obj.astype(np.complex64),
+                  acc_voltage=300.0, pixel_size_x_A=px, pixel_size_y_A=px, slice_distances=np.array([2.0]),
+                  num_slices=1, algorithm="lsq_sqrt", epoch_max=8, update_obj=1, update_probe=0,
+                  update_probe_pos=0, aberrations_array=np.zeros((2, 1), np.float32), aberration_marker=marker,
+                  update_aberrations_array=1, compute_batch=32, output_folder=out, save_inter_checkpoints=1,
+                  save_checkpoints_every_epoch=1, print_flag=0)
+    try:
+        res = iterative.run(params)
+        losses = res["losses"]
+        got = res["aberrations_array"].cpu().numpy()
+        print(f"  loss {losses[0]:.4e} -> {losses[-1]:.4e}; C10 per row {got.ravel()} (true {c10})", flush=True)
+        assert all(b <= a * (1 + 1e-9) for a, b in zip(losses, losses[1:])), losses
+        assert losses[-1] < losses[0]
+        assert np.all(np.abs(got.ravel() - c10) < 0.5 * np.abs(c10))
+        ca = np.load(out + "ca.npy")
+        np.testing.assert_array_equal(ca, got)
+        np.testing.assert_array_equal(np.load(out + "checkpoint_aberrations_epcoh_7.npy"), got)
+    finally:
+        iterative.aberrations_array = None
+
+
 def test_beam_current_initialisation_pads_like_the_reference():
     """utils.try_to_initialize_beam_current (pypty/utils.py:1766-1789): ones for none; a short current
     padded with ones by the shortfall on both sides (numpy's pad with a scalar width)."""
