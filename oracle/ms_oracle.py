@@ -77,11 +77,24 @@ def create_spatial_frequencies(px, py, n, cutoff, rolloff=0.0, fdtype=np.float32
 
 
 def propagator(dz, wavelength, q2, qx, qy, mask, tx=0.0, ty=0.0, cdtype=np.complex64):
-    """Fresnel propagator of one slice, objective.py:337-345 (and multislice_core.py:70-72)."""
+    """Fresnel propagator of one slice, objective.py:337-345 (and multislice_core.py:70-72).
+
+    tx, ty: the in-specimen tilt (tilt columns 3 and 2), scalars or per-window arrays (B, 1, 1),
+    which give one propagator per window (objective.py:482-484)."""
     fd = _fdt(cdtype)
-    dz, wavelength, tx, ty = fd(dz), fd(wavelength), fd(tx), fd(ty)
+    dz, wavelength = fd(dz), fd(wavelength)
+    tx, ty = np.asarray(tx, fd), np.asarray(ty, fd)
     h = np.exp(-3.141592654j * dz * (wavelength * q2 + 2 * (qx * tx + qy * ty))) * mask
     return h.astype(cdtype)
+
+
+def tilt_real_grids(n, px, py, wavelength, cdtype=np.complex64):
+    """objective.py:303-307: the real-space tilt grids i 2 pi fftshift(fftfreq(n)) n p / lambda
+    along y and x, so that exp(xg tan_x + yg tan_y) tilts a wave by (tan_y, tan_x)."""
+    gx, gy = np.meshgrid(sf.fftshift(np.fft.fftfreq(n)), sf.fftshift(np.fft.fftfreq(n)), indexing="xy")
+    xg = (gx * 6.2831855j * n * px / wavelength).astype(cdtype)
+    yg = (gy * 6.2831855j * n * py / wavelength).astype(cdtype)
+    return yg, xg
 
 
 def shift_masks(n, mask, cdtype=np.complex64):
@@ -111,49 +124,90 @@ def fourier_clean(array, cutoff=0.66, rolloff=0.0, fdtype=np.float32):
 
 
 # ------------------------------------------------------------ multislice
-def multislice(probe_b, obj_b, H, edge=1.0):
-    """Split-step forward, pypty/multislice_core.py:9-82 (single distance, no tilts).
+def _props(H, S):
+    """The propagator after each slice 0..S-2: one array for every slice, or a list of them."""
+    return list(H) if isinstance(H, (list, tuple)) else [H] * max(S - 1, 0)
 
-    probe_b (B, N, N, M); obj_b (B, N, N, S, K); H (N, N) propagator.
-    Returns the conjugated stash (B, N, N, S, M, K) and the exit wave (B, N, N, M, K).
+
+def _per_window(h):
+    """(N, N) or (B, N, N) propagator as (1|B, N, N, 1, 1)."""
+    return h[None, :, :, None, None] if h.ndim == 2 else h[:, :, :, None, None]
+
+
+def multislice(probe_b, obj_b, H, edge=1.0, spectral=False):
+    """Split-step forward, pypty/multislice_core.py:9-82.
+
+    probe_b (B, N, N, M); obj_b (B, N, N, S, K); H: the propagator (N, N) or one per window
+    (B, N, N), or a list of S-1 of them, one per slice (this_distances per slice,
+    multislice_core.py:69-71).  The first propagation is raised to edge when edge > 1 (:73-74).
+    Returns the conjugated stash (B, N, N, S, M, K) and the exit wave (B, N, N, M, K), and with
+    spectral also the conjugated spectra fft2(psi_s O_s) that the forward keeps for the
+    in-specimen tilt gradient (:76-77, channel 1 of waves_multislice; slice S-1 stays zero).
     """
     B, N, _, M = probe_b.shape
     S, K = obj_b.shape[3], obj_b.shape[4]
-    Hb = H[None, :, :, None, None]
+    props = _props(H, S)
     waves = np.zeros((B, N, N, S, M, K), dtype=probe_b.dtype)
+    spec = np.zeros_like(waves) if spectral else None
     wave = np.repeat(probe_b[:, :, :, :, None], K, axis=-1)
     for s in range(S):
         waves[:, :, :, s] = wave
         wave = wave * obj_b[:, :, :, s:s + 1, :]
         if s < S - 1:
+            Hb = _per_window(props[s])
             prop = Hb ** edge if (edge > 1 and s == 0) else Hb
-            wave = ifft2(fft2(wave, axes=(1, 2)) * prop, axes=(1, 2))
+            wave = fft2(wave, axes=(1, 2))
+            if spectral:
+                spec[:, :, :, s] = wave
+            wave = ifft2(wave * prop, axes=(1, 2))
+    if spectral:
+        return np.conj(waves), wave, np.conj(spec)
     return np.conj(waves), wave
 
 
-def multislice_grads(g, waves, obj_b, H, mask, edge=1.0, step_obj=True):
-    """Reverse-mode adjoint, pypty/multislice_core.py:85-200 (no tilts).
+def multislice_grads(g, waves, obj_b, H, mask, edge=1.0, step_obj=True, spec=None, qx=None, qy=None,
+                     distances=None):
+    """Reverse-mode adjoint, pypty/multislice_core.py:85-200.
 
-    g (B, N, N, M, K) = dL/d(exit)*; waves = conjugated stash from multislice().
+    g (B, N, N, M, K) = dL/d(exit)*; waves = conjugated stash from multislice(); H as there.
+    With spec (the conjugated spectra from multislice(..., spectral=True)), qx, qy and the slice
+    distances, also the in-specimen tilt gradient (:163-178): per window, for each propagation
+    from slice s-1, -4 pi d / N^2 sum Im(spec[s-1] fft2(g) conj(H) q) over pixels and modes, with
+    d the distance of that propagation (times edge for the first).
     Returns (band-limited window gradients (B, N, N, S, K) or None,
-             probe-side gradient summed over object modes (B, N, N, M)).
+             probe-side gradient summed over object modes (B, N, N, M)[, tilt gradient (B, 2) as (y, x)]).
     """
     waves = waves.copy()
     objc = np.conj(obj_b)
+    B, N = waves.shape[0], waves.shape[1]
     S = waves.shape[3]
-    Hc = np.conj(H)[None, :, :, None, None]
+    props = _props(H, S)
+    tg = None if spec is None else np.zeros((B, 2), dtype=np.float64 if g.dtype == np.complex128 else np.float32)
+    dist = None if distances is None else np.asarray(distances).ravel()
     for s in range(S - 1, -1, -1):
         if step_obj:
             waves[:, :, :, s] *= g
         g = g * objc[:, :, :, s:s + 1, :]
         if s > 0:
+            Hc = np.conj(_per_window(props[s - 1]))
             prop = Hc ** edge if (edge > 1 and s == 1) else Hc
-            g = ifft2(fft2(g, axes=(1, 2)) * prop, axes=(1, 2))
+            g = fft2(g, axes=(1, 2)) * prop
+            if tg is not None:
+                d = dist[0] if dist.shape[0] == 1 else dist[s - 1]
+                if edge > 1 and s == 1:
+                    d = d * edge
+                sh = -12.566370614359172 * d / (N * N)
+                dl = np.sum(spec[:, :, :, s - 1] * g, axis=(3, 4))
+                tg[:, 1] += sh * np.sum(np.imag(dl * qx), axis=(1, 2))
+                tg[:, 0] += sh * np.sum(np.imag(dl * qy), axis=(1, 2))
+            g = ifft2(g, axes=(1, 2))
     grad = None
     if step_obj:
         G = waves[:, :, :, :, 0, :] if waves.shape[4] == 1 else np.sum(waves, axis=4)
         grad = ifft2(fft2(G, axes=(1, 2)) * mask[None, :, :, None, None], axes=(1, 2))
     interm = g[..., 0] if g.shape[-1] == 1 else np.sum(g, axis=-1)
+    if tg is not None:
+        return grad, interm, tg
     return grad, interm
 
 
@@ -199,7 +253,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
                        edge=1.0, step_obj=True, step_probe=True, step_pos=False, compute_batch=64,
                        cdtype=np.complex64, return_patterns=False, masks=None, static_background=None,
                        step_static_background=False, beam_current=None, step_beam_current=False, aberrations=None,
-                       aberration_marker=None, step_aberrations=False):
+                       aberration_marker=None, step_aberrations=False, distances=None, tilts=None, tilt_mode=0,
+                       step_tilts=False):
     """objective.loss_and_direction restricted to the far-field multislice path.
 
     Follows pypty/objective.py:218-361 (grids, propagator, shift kernels),
@@ -235,6 +290,27 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     (:471-474) starts again from the spectrum before the plate, so with a shift
     the plates leave the loss unchanged while their gradient is still formed.
     When given, the aberration gradient is appended last.
+    distances (S-1 or more per-slice distances, or one) replaces dz when given: slice s propagates
+    by distances[s] (multislice_core.py:69-71, 164-168).
+    tilts (T, 6) with T = 1 or one row per position, columns (y, x) before, inside and after the
+    specimen (objective.py:134 of iterative.py, :428-434), and tilt_mode (iterative.py:135):
+      * inside (modes 0, 3, 4): the propagator exp(-i pi d (lambda q^2 + 2 (qx tx + qy ty))) with the
+        window's inside tilt (objective.py:482-484); one tilt row, or modes 1 and 2, use row 0's
+        for every window (:328, :337-345, the master propagator).  The reference keeps that
+        master propagator in a module cache across calls (:337), so a refined single tilt would
+        not reach later calls; this restatement forms it per call.
+      * before (modes 2, 4): the probe times exp(xg tx + yg ty) in real space after the beam
+        current and aberrations, before the sub-pixel shift (:465-470); as with the aberration
+        plates, a shift starts from the probe spectrum taken before the aberrations when there
+        are aberrations, so that then the tilt does not reach the modelled patterns either.
+      * after (modes 1, 3, 4): the exit wave times exp(xg tx + yg ty), band-limited first for one
+        slice (:490-499).
+    step_tilts adds the tilt gradient (T, 6): inside from multislice_grads (:163-178), before
+    and after -2 sum Re(g conj(wave before the tilt) (yg, xg)) over pixels and modes
+    (:620-636, :660-672).  Rows of the windows are assigned, except inside, which adds; with one
+    tilt row every window adds into row 0 (the reference adds a per-window array into that single
+    row there, which only broadcasts for one window per batch).  When tilts are given the tilt
+    gradient is appended last.
     """
     fd = _fdt(cdtype)
     obj = np.asarray(obj, dtype=cdtype)
@@ -244,7 +320,20 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
     N, M = probe.shape[0], probe.shape[2]
     S, K = obj.shape[2], obj.shape[3]
     q2, qx, qy, mask_s, mask = create_spatial_frequencies(pixel_size_x, pixel_size_y, N, cutoff, 0.0, fd)
-    H = propagator(dz, wavelength, q2, qx, qy, mask, cdtype=cdtype)
+    dists = np.asarray([dz] if distances is None else distances, dtype=np.float64).ravel()
+    single_dist = dists.shape[0] == 1
+    tl = None if tilts is None else np.asarray(tilts, dtype=fd)
+    tilt_mode = int(tilt_mode) if tl is not None else 0
+    tilt_before, tilt_after = tilt_mode in (2, 4), tilt_mode in (1, 3, 4)
+    tilt_inside_grad = bool(step_tilts) and tl is not None and tilt_mode in (0, 3, 4)
+    single_tilt = tl is None or tl.shape[0] == 1
+    same_prop = single_tilt or tilt_mode in (1, 2)
+    if tl is not None:
+        yg, xg = tilt_real_grids(N, pixel_size_x, pixel_size_y, wavelength, cdtype)
+        yxg = np.stack((yg, xg), axis=0)  # (2, N, N)
+    tilt_grad = None if tl is None else np.zeros_like(tl)
+    t0 = np.zeros(6, fd) if tl is None else tl[0]
+    H = propagator(dists[0], wavelength, q2, qx, qy, mask, tx=t0[3], ty=t0[2], cdtype=cdtype)
     shy, shx = shift_masks(N, mask, cdtype)
     shift_flag = bool(np.sum(corrections.astype(np.float64) ** 2) != 0.0) or bool(step_pos)
     runx, runy = np.meshgrid(np.arange(N), np.arange(N), indexing="xy")
@@ -284,16 +373,43 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
             thisbc = np.abs(bc[tcs])
             this_probe = this_probe * thisbc[:, None, None, None]
             FPb = FP * thisbc[:, None, None, None]
+        if tl is not None:
+            tb = tl[[0]] if single_tilt else tl[tcs]
+            tcol = [tb[:, c, None, None] for c in range(6)]  # (B|1, 1, 1) each
+        FPs = FPb  # the spectrum the shift starts from
         if ab is not None:
             this_probe = ifft2(FPb * plates[marker[tcs]][..., None], axes=(1, 2))
+        if tilt_before:
+            ramp_b = np.exp(xg[None] * tcol[1] + yg[None] * tcol[0])  # (B|1, N, N)
+            probe_bt = this_probe
+            this_probe = probe_bt * ramp_b[..., None]
+            if ab is None:
+                FPs = fft2(this_probe, axes=(1, 2))
         E = None
         if shift_flag:
             corr = corrections[tcs]
             E = np.exp(shx[None] * corr[:, 1, None, None] + shy[None] * corr[:, 0, None, None]) * mask[None]
-            this_probe = ifft2(FPb * E[..., None], axes=(1, 2))
+            this_probe = ifft2(FPs * E[..., None], axes=(1, 2))
         if this_probe.shape[0] == 1 and len(tcs) > 1:
             this_probe = np.repeat(this_probe, len(tcs), axis=0)
-        waves, exitw = multislice(this_probe, chop, H, edge)
+        if single_dist:
+            props = H if same_prop else propagator(dists[0], wavelength, q2, qx[None], qy[None], mask[None],
+                                                   tx=tcol[3], ty=tcol[2], cdtype=cdtype)
+        else:
+            tx_in, ty_in = (tcol[3], tcol[2]) if tl is not None else (0.0, 0.0)
+            props = [propagator(dists[s], wavelength, q2, qx, qy, mask, tx=tx_in, ty=ty_in, cdtype=cdtype)
+                     for s in range(S - 1)]
+        spec = None
+        if tilt_inside_grad:
+            waves, exitw, spec = multislice(this_probe, chop, props, edge, spectral=True)
+        else:
+            waves, exitw = multislice(this_probe, chop, props, edge)
+        if tilt_after:
+            ramp_a = np.exp(xg[None] * tcol[5] + yg[None] * tcol[4])[:, :, :, None, None]
+            if S == 1:
+                exitw = ifft2(fft2(exitw, axes=(1, 2)) * mask[None, :, :, None, None], axes=(1, 2))
+            exit_bt = exitw
+            exitw = exit_bt * ramp_a
         Psi = shift_fft2(exitw, axes=(1, 2)) * mask_s[None, :, :, None, None]
         I = np.abs(Psi) ** 2
         I = I[:, :, :, 0, 0] if coherent else np.mean(I, axis=(-1, -2))
@@ -362,19 +478,48 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
             bg_grad = bg_grad + np.sum(dI, 0) * 2 * bg * loss_weight
         dWave = dI[:, :, :, None, None] * Psi * (loss_weight * N * N)
         g = ifft2_ishift(dWave, axes=(1, 2))
-        G, interm = multislice_grads(g, waves, chop, H, mask, edge, step_obj)
+        if tilt_after:
+            g = g * np.conj(ramp_a)
+            if step_tilts:
+                gk = -2 * g * np.conj(exit_bt)
+                dl = np.sum(np.real(gk[:, None] * yxg[None, :, :, :, None, None]), axis=(2, 3, 4, 5))
+                if single_tilt:
+                    tilt_grad[0, 4:] += np.sum(dl, 0)
+                else:
+                    tilt_grad[tcs, 4:] = dl
+            if S == 1:
+                g = ifft2(fft2(g, axes=(1, 2)) * mask[None, :, :, None, None], axes=(1, 2))
+        if tilt_inside_grad:
+            G, interm, tg = multislice_grads(g, waves, chop, props, mask, edge, step_obj, spec, qx[None], qy[None],
+                                             dists)
+            if single_tilt:
+                tilt_grad[0, 2:4] += np.sum(tg, 0)
+            else:
+                tilt_grad[tcs, 2:4] += tg
+        else:
+            G, interm = multislice_grads(g, waves, chop, props, mask, edge, step_obj)
         if step_obj:
             scatteradd(obj_grad, my, mx, G)
+        Fg = None
         if shift_flag:
             Fg = fft2(interm, axes=(1, 2)) * np.conj(E)[..., None]
             interm = ifft2(Fg, axes=(1, 2))
             if step_pos:
-                smg = np.conj(FPb) * Fg
+                smg = np.conj(FPs) * Fg
                 sh = -2 / (N * N)
                 yx = np.stack((shy, shx), axis=0)  # (2, N, N)
                 pos_grad[tcs] = sh * np.sum(np.real(yx[None, :, :, :, None] * smg[:, None]), axis=(2, 3, 4))
+        if tilt_before:
+            interm = interm * np.conj(ramp_b)[..., None]
+            if step_tilts:
+                gk = -2 * interm * np.conj(probe_bt)
+                dl = np.sum(np.real(gk[:, None] * yxg[None, :, :, :, None]), axis=(2, 3, 4))
+                if single_tilt:
+                    tilt_grad[0, :2] += np.sum(dl, 0)
+                else:
+                    tilt_grad[tcs, :2] = dl
         if ab is not None:
-            Fg = fft2(interm, axes=(1, 2)) if not shift_flag else Fg
+            Fg = fft2(interm, axes=(1, 2)) if Fg is None else Fg
             Fg = Fg * np.conj(plates[marker[tcs]])[..., None]
             interm = ifft2(Fg, axes=(1, 2))
             if step_aberrations:
@@ -401,6 +546,8 @@ def loss_and_direction(obj, probe, positions, corrections, measured, sequence, *
         out = out + (bc_grad,)
     if ab is not None:
         out = out + (ab_grad,)
+    if tl is not None:
+        out = out + (tilt_grad,)
     return out
 
 
