@@ -162,6 +162,23 @@ typedef struct pty_problem {
   /* 1: also return the gradient of the aberrations (objective.py:673-681,
    * this_step_aberrations_array); needs pty_buffers.aberrations_grad. */
   int step_aberrations;
+  /* Per-slice distances (multislice_core.py:69-71, 164-168): 0 or 1 uses
+   * slice_distance for every propagation; more: pty_buffers.distances holds
+   * n_distances >= n_slices - 1 distances and the propagation after slice s
+   * covers distances[s]. */
+  int n_distances;
+  /* Beam tilts (objective.py:303-307, 328-350, 428-499): tilt_mode as the
+   * reference's (iterative.py:135): 0 inside the specimen, 1 after, 2 before,
+   * 3 inside and after, 4 all three; n_tilts rows (1, or one per position) of
+   * pty_buffers.tilts; 0 for none (tan_x, tan_y then tilt the propagator).
+   * With one row, or modes 1 and 2, every window's propagator takes row 0's
+   * inside tilt; otherwise each window its own.  Tilts before the specimen
+   * (modes 2, 4) are not on this path yet (PTY_ERR_ARG). */
+  int tilt_mode;
+  int n_tilts;
+  /* 1: also return the tilt gradient (objective.py:163-178 of
+   * multislice_core.py, objective.py:620-636); needs pty_buffers.tilts_grad. */
+  int step_tilts;
 } pty_problem;
 
 typedef struct pty_buffers {
@@ -219,6 +236,20 @@ typedef struct pty_buffers {
    * spectrum of dL/dP* after the conjugate plate and F the probe spectrum
    * before it (objective.py:676-679); zero first unless accumulate. */
   float* aberrations_grad;
+  /* (n_distances) per-slice distances, Angstrom, when n_distances > 1 */
+  const float* distances;
+  /* (n_tilts, 6) tilts in radians, columns (y, x) before, inside and after
+   * the specimen (the reference's tilt array plus its correction) */
+  const float* tilts;
+  /* (n_tilts, 6) out: columns 2, 3 (inside) and 4, 5 (after) per the mode;
+   * with one row the sum over the windows of the sequence, else the rows of
+   * `sequence` (the others zero unless accumulate).  Inside: per window
+   * -4 pi d / n^2 sum Im(conj(fft2 u_s) fft2(g_s) q) over the propagations
+   * (into slice s from s-1, d its distance, times the edge multiplier for the
+   * first), the pixels and the modes, u_s the wave entering slice s and g_s
+   * dL/du_s*; after: 2 sum c Im(g conj(e)) with e the exit wave before the
+   * tilt, g its gradient after it, c = 2 pi (j - n/2) p / lambda along y, x. */
+  float* tilts_grad;
 } pty_buffers;
 
 int pty_workspace_bytes(const pty_problem* p, size_t* bytes);
@@ -226,8 +257,9 @@ int pty_workspace_bytes(const pty_problem* p, size_t* bytes);
 /* One full forward + loss + adjoint pass over `sequence`:
  * replaces objective.loss_and_direction (pypty/objective.py:47-967) for
  * far-field multislice, the lsq_sqrt / ml / lsq / lsq_sqrt_2 losses and the
- * three compressed ones, object / probe / position gradients, single slice
- * distance, no tilts, with the reference's per-batch data preprocessing. */
+ * three compressed ones, object / probe / position gradients, one or per-slice
+ * distances, tilts inside and after the specimen, with the reference's
+ * per-batch data preprocessing. */
 int pty_loss_and_direction(const pty_problem* p, const pty_buffers* b, void* stream);
 
 /* Multislice forward on pre-cut windows (pypty/multislice_core.py:9-82).

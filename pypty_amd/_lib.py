@@ -51,6 +51,8 @@ class PtyProblem(ctypes.Structure):
         ("checkpoint_slices", ctypes.c_int), ("lanes", ctypes.c_int),
         ("step_static_background", ctypes.c_int), ("step_beam_current", ctypes.c_int),
         ("n_aberrations", ctypes.c_int), ("n_markers", ctypes.c_int), ("step_aberrations", ctypes.c_int),
+        ("n_distances", ctypes.c_int), ("tilt_mode", ctypes.c_int), ("n_tilts", ctypes.c_int),
+        ("step_tilts", ctypes.c_int),
     ]
 
 
@@ -65,7 +67,8 @@ class PtyBuffers(ctypes.Structure):
         ("static_background", ctypes.c_void_p), ("static_background_grad", ctypes.c_void_p),
         ("beam_current", ctypes.c_void_p), ("beam_current_grad", ctypes.c_void_p),
         ("aberrations", ctypes.c_void_p), ("aberration_marker", ctypes.c_void_p),
-        ("aberrations_grad", ctypes.c_void_p),
+        ("aberrations_grad", ctypes.c_void_p), ("distances", ctypes.c_void_p), ("tilts", ctypes.c_void_p),
+        ("tilts_grad", ctypes.c_void_p),
     ]
 
 

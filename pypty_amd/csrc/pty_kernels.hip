@@ -175,11 +175,13 @@ __global__ __launch_bounds__(Tile<N>::NT) void k_fft2_tiles(float2* __restrict__
 // register tile (N <= 128) or of the slab tile (192, 256).  H and H^e carry
 // the 1/n^2 of the inverse FFT.
 struct TableArgs {
-  float2* H;
+  float2* H;   // one table per propagation (blockIdx.y) with per-slice distances
   float2* H0;
   float* maskc;
   int* zcol;  // (N) 1 where column fx of the band mask is zero for every fy
   float dz, lambda, px, py, tx, ty, cutoff, edge;
+  const float* dists;  // per-slice distances (blockIdx.y indexes them), or null for dz
+  const float* tilt0;  // tilt row 0: its inside tilt (columns 3, 2) replaces tx, ty; or null
 };
 // float32 products/sums without contraction, in numpy's order
 __device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b); }
@@ -224,18 +226,22 @@ template <int N, int LAY>
 __global__ void k_tables(TableArgs a) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N * N) return;
+  const int t = blockIdx.y;  // the propagation after slice t (one table per distance)
   int fy, fx;
   spec_of<N, LAY>(e, fy, fx);
   const float qx = fftfreq_f(fx, N, a.px), qy = fftfreq_f(fy, N, a.py);
   const float q2 = fadd(fmul(qx, qx), fmul(qy, qy));
   const float m = band_mask<N>(a, fy, fx);
-  const float k = fmul(-3.141592654f, a.dz);
-  const float r = fadd(fmul(a.lambda, q2), fmul(2.f, fadd(fmul(qx, a.tx), fmul(qy, a.ty))));
+  const float dz = a.dists ? a.dists[t] : a.dz;
+  const float tx = a.tilt0 ? a.tilt0[3] : a.tx, ty = a.tilt0 ? a.tilt0[2] : a.ty;
+  const float k = fmul(-3.141592654f, dz);
+  const float r = fadd(fmul(a.lambda, q2), fmul(2.f, fadd(fmul(qx, tx), fmul(qy, ty))));
   const float ang = fmul(k, r);
   float sn, cs;
   sincosf(ang, &sn, &cs);
   const float s = 1.0f / float(N * N);
-  a.H[e] = make_float2(cs * m * s, sn * m * s);
+  a.H[size_t(t) * N * N + e] = make_float2(cs * m * s, sn * m * s);
+  if (t > 0) return;
   if (a.edge > 1.f) {
     // complex power through the principal branch, as numpy's z**e
     const float th = atan2f(sn * m, cs * m);
@@ -441,6 +447,27 @@ struct BatchArgs {
   // when s_hi < S, and the wave propagated to slice s_lo - 1 is written there
   // when s_lo > 0 (stash order, (B, M, K, N, N))
   float2* gstate;
+  // per-slice distances (multislice_core.py:69-71): the propagation after
+  // slice s >= 1 uses H + s * Hstride (H0 after slice 0); dists (S - 1) or
+  // null for the single distance dz
+  size_t Hstride;
+  const float* dists;
+  float dz;
+  // tilts (objective.py:428-499): n_tilt rows of 6 at tilt (1 or P rows).
+  // tilt_win: each window multiplies its propagators by its own inside-tilt
+  // factors (the tables hold the untilted propagators); tilt_after: the exit
+  // wave is multiplied by the after-tilt ramp.  Gradients: tg_in from the
+  // spectra Wsp = fft2(u_s) / N^2 (C-order, slot s of S per window and
+  // mode) that the forward keeps; tg_after from the exit wave before the
+  // ramp, Xe (R-order); both summed per window and mode into tilt_part
+  // (B, M*K, 4: inside y, x; after y, x), then into tilt_grad (n_tilt, 6).
+  const float* tilt;
+  int n_tilt, tilt_win, tilt_after, tg_in, tg_after;
+  float lambda, px, py, edge;
+  float2* Wsp;
+  float2* Xe;
+  double* tilt_part;
+  float* tilt_grad;
 };
 
 // a launch over every slice with one stash slot per slice
@@ -473,6 +500,68 @@ __device__ __forceinline__ const float* ab_row(const BatchArgs& a, int idx) {
   if (a.NA == 0 || idx < 0) return nullptr;
   const int r = a.marker[idx];
   return (r >= 0 && r < a.n_markers) ? a.ab + size_t(r) * a.NA : nullptr;
+}
+
+// tilt row of position idx, or null (no tilts, or an invalid window)
+__device__ __forceinline__ const float* tilt_row(const BatchArgs& a, int idx) {
+  if (!a.tilt || idx < 0) return nullptr;
+  return a.tilt + size_t(a.n_tilt == 1 ? 0 : idx) * 6;
+}
+
+// distance of the propagation after slice s (multislice_core.py:70, 166)
+__device__ __forceinline__ float slice_dist(const BatchArgs& a, int s) { return a.dists ? a.dists[s] : a.dz; }
+
+// Inside-tilt factors of one window's propagator after a slice of thickness
+// d (objective.py:482-484): exp(-i pi d 2 (qx tx + qy ty)) = tt[fx] tt[N + fy],
+// the untilted exp(-i pi d lambda q^2) mask being in the tables.  The caller
+// synchronises.
+template <int N>
+__device__ __forceinline__ void tilt_tables_inside(float2* tt, float d, float tx, float ty, float px, float py) {
+  const float k = fmul(-3.141592654f, d);
+  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) {
+    const int f = i % N;
+    const float q = i < N ? fftfreq_f(f, N, px) : fftfreq_f(f, N, py);
+    float sn, cs;
+    sincosf(fmul(k, fmul(2.f, fmul(q, i < N ? tx : ty))), &sn, &cs);
+    tt[i] = make_float2(cs, sn);
+  }
+}
+
+// The real-space tilt grids of objective.py:303-307 (i times these):
+// tt[j] = (cx[j], cy[j]) with c = 2 pi (j - N/2) / N * N * p / lambda in
+// double rounded to float, as the reference forms its complex64 grids.  The
+// caller synchronises.
+template <int N>
+__device__ __forceinline__ void tilt_tables_grid(float2* tt, float px, float py, float lambda) {
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    const double f = double(j - N / 2) / double(N);
+    tt[j] = make_float2(float(f * 6.2831855 * double(N) * double(px) / double(lambda)),
+                        float(f * 6.2831855 * double(N) * double(py) / double(lambda)));
+  }
+}
+
+// exp(xg tx + yg ty) at pixel (y, x) from the grid table (objective.py:491)
+__device__ __forceinline__ float2 tilt_ramp(const float2* tt, int y, int x, float tx, float ty) {
+  float sn, cs;
+  sincosf(fadd(fmul(tt[x].x, tx), fmul(tt[y].y, ty)), &sn, &cs);
+  return make_float2(cs, sn);
+}
+
+// the propagator value h of the window's propagation after slice s with its
+// inside-tilt factors (tt, for frequency (fy, fx)); after slice 0 with an
+// edge multiplier e > 1 the whole per-window propagator is raised to e
+// through the principal branch, as numpy's z**e (multislice_core.py:73-74)
+__device__ __forceinline__ float2 tilted_h(float2 h, const float2* tt, int fy, int fx, int N, bool first, float e) {
+  h = cmul(h, cmul(tt[fx], tt[N + fy]));
+  if (first && e > 1.f) {
+    const float m = (h.x != 0.f || h.y != 0.f) ? 1.f : 0.f;
+    const float nn = float(N) * float(N);
+    const float th = atan2f(h.y * nn, h.x * nn);
+    float s0, c0;
+    sincosf(e * th, &s0, &c0);
+    h = make_float2(c0 * m / nn, s0 * m / nn);
+  }
+  return h;
 }
 
 // measured row of batch slot b (sequence entry b0 + b, position idx)
@@ -1005,11 +1094,12 @@ __global__ __launch_bounds__(RT<N>::NT) void k_patch_rt(BatchArgs a) {
 }
 
 // k_forward over the register tile (multislice_core.py:9-82, objective.py:501-512)
-template <int N>
+template <int N, bool TILT>
 __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
   using R = RT<N>;
   rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
+  __shared__ float2 tt[TILT ? 2 * N : 1];  // tilt factors or grids
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
@@ -1017,6 +1107,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
   const int m = mk / a.K, k = mk % a.K;
   const unsigned tid = threadIdx.x;
   const RTIds<N> id;
+  const float* trow = TILT ? tilt_row(a, seq_index(a.seq, a.b0 + b, a.P)) : nullptr;
   const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
   // o holds the object window of the current slice; the next slice's is read
   // during the column phase so that its latency hides behind the transforms,
@@ -1040,6 +1131,12 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
     for (int n = 0; n < R::R1; ++n) o[n] = O[n];
   }
   float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  float2* wsp = (TILT && a.tg_in) ? a.Wsp + (size_t(b) * MK + mk) * a.S * N * N : nullptr;
+  const float tix = trow ? trow[3] : 0.f, tiy = trow ? trow[2] : 0.f;
+  if ((TILT && a.tilt_win) && !a.dists) {
+    tilt_tables_inside<N>(tt, a.dz, tix, tiy, a.px, a.py);
+    __syncthreads();
+  }
   // wavefronts whose columns the propagators (the band mask) zero skip the
   // column transforms of the slices (of the far field)
   const bool zh = wave_cols_zero<N>(a.zcol, id.col), zm = zh;
@@ -1053,6 +1150,12 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) v[n] = cmul(v[n], o[n]);
     if (s < a.s_hi - 1) {
+      if ((TILT && a.tilt_win) && a.dists) {
+        // per-slice distances: this propagation's own tilt factors
+        __syncthreads();
+        tilt_tables_inside<N>(tt, a.dists[s], tix, tiy, a.px, a.py);
+        __syncthreads();
+      }
       rt_line_fwd<N, true>(v, T, id.row, id.rj);
       rt_rows_to_cols<N>(v, T, id);
       const RView<N, float2> O(ob + size_t(s + 1) * ostride, tid);
@@ -1062,13 +1165,30 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
 #pragma unroll
         for (int i = 0; i < R::R1; ++i) v[i] = make_float2(0.f, 0.f);
       } else {
-        const RView<N, float2> H((s == 0) ? a.H0 : a.H, tid);
+        // with an edge multiplier, the first propagation is H0 = H^e; a window
+        // with its own tilt raises its tilted propagator itself
+        const float2* hs = (s == 0 && !(TILT && a.tilt_win)) ? a.H0 : a.H + size_t(s) * a.Hstride;
+        const RView<N, float2> H(hs, tid);
         float2 h[R::R1];
 #pragma unroll
         for (int i = 0; i < R::R1; ++i) h[i] = H[i];
         rt_line_fwd<N, false>(v, T, id.col, id.cj);
+        if ((TILT && a.tilt_win)) {
+#pragma unroll
+          for (int i = 0; i < R::R1; ++i) {
+            int fy, fx;
+            R::cspec(i * R::NT + tid, fy, fx);
+            h[i] = tilted_h(h[i], tt, fy, fx, N, s == 0, a.edge);
+          }
+        }
 #pragma unroll
         for (int i = 0; i < R::R1; ++i) v[i] = cmul(v[i], h[i]);
+        if (wsp) {
+          // fft2 of the wave entering slice s + 1 (over N^2), for the inside-tilt gradient
+          const RView<N, float2> W(wsp + size_t(s + 1) * N * N, tid);
+#pragma unroll
+          for (int i = 0; i < R::R1; ++i) W.store(i, v[i]);
+        }
 #pragma unroll
         for (int n = 0; n < R::R1; ++n) o[n] = O[n];
         rt_line_inv<N, false>(v, T, id.col, id.cj);
@@ -1078,6 +1198,22 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
     }
   }
   if (a.fin) return;  // segment recompute: the stash is the result
+  if ((TILT && a.tilt_after)) {
+    // the exit wave, band-limited first for one slice, times the after-tilt
+    // ramp (objective.py:490-499); Xe keeps it before the ramp
+    if (a.S == 1) rt_band_limit<N>(v, T, id, a.maskc, a.zcol);
+    if ((TILT && a.tg_after)) {
+      const RView<N, float2> X(a.Xe + (size_t(b) * MK + mk) * N * N, tid);
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) X.store(n, v[n]);
+    }
+    __syncthreads();
+    tilt_tables_grid<N>(tt, a.px, a.py, a.lambda);
+    __syncthreads();
+    const float tax = trow ? trow[5] : 0.f, tay = trow ? trow[4] : 0.f;
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) v[n] = cmul(v[n], tilt_ramp(tt, id.row, id.rj + R::T * n, tax, tay));
+  }
   rt_line_fwd<N, true>(v, T, id.row, id.rj);
   rt_rows_to_cols<N>(v, T, id);
   const RView<N, float2> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
@@ -1093,11 +1229,13 @@ __global__ __launch_bounds__(RT<N>::NT) void k_forward_rt(BatchArgs a) {
 }
 
 // k_adjoint over the register tile (objective.py:595-597, multislice_core.py:85-200)
-template <int N>
+template <int N, bool TILT>
 __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
   using R = RT<N>;
   rt_tw_load<N>();
   __shared__ float2 T[R::ELEMS];
+  __shared__ float2 tt[TILT ? 2 * N : 1];  // tilt factors or grids
+  __shared__ double red[TILT ? 16 : 1];
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
@@ -1106,6 +1244,8 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
   const unsigned tid = threadIdx.x;
   const RTIds<N> id;
   const float sc = 1.0f / float(N * N);
+  const float* trow = TILT ? tilt_row(a, seq_index(a.seq, a.b0 + b, a.P)) : nullptr;
+  double tg[4] = {0.0, 0.0, 0.0, 0.0};  // tilt gradient: inside y, x; after y, x
   float2 v[R::R1];
   // as in the forward: columns zero in the propagators, and in the band mask
   // (where the far field, and so its gradient, is zero)
@@ -1128,13 +1268,39 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
     }
     rt_cols_to_rows<N>(v, T, id);
     rt_line_inv<N, true>(v, T, id.row, id.rj);
+    if ((TILT && a.tilt_after)) {
+      // the gradient through the after-tilt ramp and of its tilt (objective.py:620-636)
+      tilt_tables_grid<N>(tt, a.px, a.py, a.lambda);
+      __syncthreads();
+      const float tax = trow ? trow[5] : 0.f, tay = trow ? trow[4] : 0.f;
+      const RView<N, float2> X(a.Xe + (size_t(b) * MK + mk) * N * N, tid);
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) {
+        const int x = id.rj + R::T * n;
+        v[n] = cmulc(v[n], tilt_ramp(tt, id.row, x, tax, tay));
+        if ((TILT && a.tg_after)) {
+          const float2 e = X[n];
+          const double zi = double(v[n].y) * double(e.x) - double(v[n].x) * double(e.y);  // Im(g conj(e))
+          tg[2] += 2.0 * double(tt[id.row].y) * zi;
+          tg[3] += 2.0 * double(tt[x].x) * zi;
+        }
+      }
+      if (a.S == 1) rt_band_limit<N>(v, T, id, a.maskc, a.zcol);
+      __syncthreads();
+    }
   }
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g (k_objgrad_rt sums the modes); pu holds slot s, read one
   // column phase ahead like the forward's object window
   float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  const float2* wsp = (TILT && a.tg_in) ? a.Wsp + (size_t(b) * MK + mk) * a.S * N * N : nullptr;
   const size_t ostride = size_t(a.K) * N * N;
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  const float tix = trow ? trow[3] : 0.f, tiy = trow ? trow[2] : 0.f;
+  if ((TILT && a.tilt_win) && !a.dists) {
+    tilt_tables_inside<N>(tt, a.dz, tix, tiy, a.px, a.py);
+    __syncthreads();
+  }
   float2 pu[R::R1];
   if (a.step_obj) {
     const RView<N, float2> U(st + size_t(a.s_hi - 1 - a.s_lo) * N * N, tid);
@@ -1151,17 +1317,56 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
 #pragma unroll
     for (int n = 0; n < R::R1; ++n) v[n] = cmulc(v[n], O[n]);
     if (s > 0) {
+      if ((TILT && a.tilt_win) && a.dists) {
+        __syncthreads();
+        tilt_tables_inside<N>(tt, a.dists[s - 1], tix, tiy, a.px, a.py);
+        __syncthreads();
+      }
       rt_line_fwd<N, true>(v, T, id.row, id.rj);
       rt_rows_to_cols<N>(v, T, id);
       if (zh) {
 #pragma unroll
         for (int i = 0; i < R::R1; ++i) v[i] = make_float2(0.f, 0.f);
       } else {
-        const RView<N, float2> H((s == 1) ? a.H0 : a.H, tid);
+        const float2* hs = (s == 1 && !(TILT && a.tilt_win)) ? a.H0 : a.H + size_t(s - 1) * a.Hstride;
+        const RView<N, float2> H(hs, tid);
         float2 h[R::R1];
+        // the propagator is read ahead of the transform, except when the
+        // tilt-gradient sum needs the registers first
+        if (!wsp) {
 #pragma unroll
-        for (int i = 0; i < R::R1; ++i) h[i] = H[i];
+          for (int i = 0; i < R::R1; ++i) h[i] = H[i];
+        }
         rt_line_fwd<N, false>(v, T, id.col, id.cj);
+        if (wsp) {
+          // -4 pi d / N^2 sum Im(conj(fft2 u_s) fft2(g_s) q), with Wsp = fft2(u_s) / N^2
+          // (multislice_core.py:169-178)
+          const RView<N, float2> W(wsp + size_t(s) * N * N, tid);
+          double sy = 0.0, sx = 0.0;
+#pragma unroll
+          for (int i = 0; i < R::R1; ++i) {
+            int fy, fx;
+            R::cspec(i * R::NT + tid, fy, fx);
+            const float2 w = W[i];
+            const double zi = double(w.x) * double(v[i].y) - double(w.y) * double(v[i].x);
+            sx += double(fftfreq_f(fx, N, a.px)) * zi;
+            sy += double(fftfreq_f(fy, N, a.py)) * zi;
+          }
+          double d = double(slice_dist(a, s - 1));
+          if (s == 1 && a.edge > 1.f) d *= double(a.edge);
+          tg[0] += -12.566370614359172 * d * sy;
+          tg[1] += -12.566370614359172 * d * sx;
+#pragma unroll
+          for (int i = 0; i < R::R1; ++i) h[i] = H[i];
+        }
+        if ((TILT && a.tilt_win)) {
+#pragma unroll
+          for (int i = 0; i < R::R1; ++i) {
+            int fy, fx;
+            R::cspec(i * R::NT + tid, fy, fx);
+            h[i] = tilted_h(h[i], tt, fy, fx, N, s == 1, a.edge);
+          }
+        }
 #pragma unroll
         for (int i = 0; i < R::R1; ++i) v[i] = cmulc(v[i], h[i]);
       }
@@ -1180,6 +1385,16 @@ __global__ __launch_bounds__(RT<N>::NT) void k_adjoint_rt(BatchArgs a) {
   const RView<N, float2> out((a.s_lo > 0 ? a.gstate : a.gP) + (size_t(b) * MK + mk) * N * N, tid);
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) out.store(n, v[n]);
+  if ((TILT && a.tg_in) || (TILT && a.tg_after)) {
+    // this segment's part; the segments of a window add in order
+    for (int c = 0; c < 4; ++c) {
+      const double S = block_sum(tg[c], red);
+      if (tid == 0) {
+        double* dst = a.tilt_part + (size_t(b) * MK + mk) * 4 + c;
+        *dst = (a.s_hi < a.S ? *dst : 0.0) + S;
+      }
+    }
+  }
 }
 
 // k_objgrad over the register tile: mode sum from the R-order stash, band
@@ -1693,11 +1908,12 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_patch_gt(BatchArgs a) {
 }
 
 // k_forward for 192/256 px windows (multislice_core.py:9-82, objective.py:501-512)
-template <int N>
+template <int N, bool TILT>
 __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
   using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
+  __shared__ float2 tt[TILT ? 2 * N : 1];  // tilt factors or grids
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
@@ -1706,6 +1922,13 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
   GSlab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  const float* trow = TILT ? tilt_row(a, seq_index(a.seq, a.b0 + b, a.P)) : nullptr;
+  const float tix = trow ? trow[3] : 0.f, tiy = trow ? trow[2] : 0.f;
+  float2* wsp = (TILT && a.tg_in) ? a.Wsp + (size_t(b) * MK + mk) * a.S * N * N : nullptr;
+  if ((TILT && a.tilt_win) && !a.dists) {
+    tilt_tables_inside<N>(tt, a.dz, tix, tiy, a.px, a.py);
+    __syncthreads();
+  }
   const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
   // the shifted probes carry the beam current already
   const float bcs = a.shift ? 1.f : bc_scale(a, seq_index(a.seq, a.b0 + b, a.P));
@@ -1767,11 +1990,46 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
       gt_store_rowspec<N>(v, sl, L, r);
     }
     if (last_of_recompute) break;
+    if ((TILT && a.tilt_after) && s == a.S - 1) {
+      // the exit wave, band-limited first for one slice, times the after-tilt
+      // ramp (objective.py:490-499); Xe keeps it before the ramp
+      sl.to_cols(X, L);
+      if (a.S == 1) gt_band_columns<N>(v, X, sl, L, a.maskc, a.zcol);
+      sl.to_rows(X, L);
+      tilt_tables_grid<N>(tt, a.px, a.py, a.lambda);
+      __syncthreads();
+      const float tax = trow ? trow[5] : 0.f, tay = trow ? trow[4] : 0.f;
+      const RView<N, float2, G::NT> XE(a.Xe ? a.Xe + (size_t(b) * MK + mk) * N * N : a.Psi, tid);
+      // the row spectra back to rows: the band limit's 1/N^2 leaves the wave
+      // itself, else the unnormalised row pair leaves N times it
+      const float rs = a.S == 1 ? 1.f : 1.f / float(N);
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
+        gt_load_rowspec<N>(v, sl, L, r);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) {
+          v[n] = cscale(v[n], rs);
+          if ((TILT && a.tg_after)) XE.store(r * G::R1 + n, v[n]);
+          v[n] = cmul(v[n], tilt_ramp(tt, r * G::LINES + L.ln, L.j + 8 * n, tax, tay));
+        }
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        gt_store_rowspec<N>(v, sl, L, r);
+      }
+    }
     sl.to_cols(X, L);
     // column phase: forward column transform, then the propagator and the
     // inverse column transform, or the far-field band mask after the last slice
     if (s < a.s_hi - 1) {
-      const RView<N, float2, G::NT> H((s == 0) ? a.H0 : a.H, tid);
+      if ((TILT && a.tilt_win) && a.dists) {
+        // per-slice distances: this propagation's own tilt factors
+        __syncthreads();
+        tilt_tables_inside<N>(tt, a.dists[s], tix, tiy, a.px, a.py);
+        __syncthreads();
+      }
+      // with an edge multiplier, the first propagation is H0 = H^e; a window
+      // with its own tilt raises its tilted propagator itself
+      const RView<N, float2, G::NT> H((s == 0 && !(TILT && a.tilt_win)) ? a.H0 : a.H + size_t(s) * a.Hstride, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
         round_fence<decltype(sl)>();
         if ((zh >> r) & 1u) {
@@ -1783,8 +2041,22 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
         for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
         gt_load_col<N>(v, sl, L, r);
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        if ((TILT && a.tilt_win)) {
+#pragma unroll
+          for (int i = 0; i < G::R1; ++i) {
+            int fy, fx;
+            G::cspec((r * G::R1 + i) * G::NT + tid, fy, fx);
+            h[i] = tilted_h(h[i], tt, fy, fx, N, s == 0, a.edge);
+          }
+        }
 #pragma unroll
         for (int i = 0; i < G::R1; ++i) v[i] = cmul(v[i], h[i]);
+        if (wsp) {
+          // fft2 of the wave entering slice s + 1 (over N^2), for the inside-tilt gradient
+          const RView<N, float2, G::NT> W(wsp + size_t(s + 1) * N * N, tid);
+#pragma unroll
+          for (int i = 0; i < G::R1; ++i) W.store(r * G::R1 + i, v[i]);
+        }
         rt_line_inv<N, true>(v, X, L.ln, L.j);
         gt_store_col<N>(v, sl, L, r);
       }
@@ -1809,11 +2081,13 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
 }
 
 // k_adjoint for 192/256 px windows (objective.py:595-597, multislice_core.py:85-200)
-template <int N>
+template <int N, bool TILT>
 __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
   using G = GT<N, kGtThreads<N>>;
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
+  __shared__ float2 tt[TILT ? 2 * N : 1];  // tilt factors or grids
+  __shared__ double red[TILT ? 16 : 1];
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
@@ -1822,6 +2096,10 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
   const unsigned tid = threadIdx.x;
   const GTLane<N> L;
   GSlab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  const float* trow = TILT ? tilt_row(a, seq_index(a.seq, a.b0 + b, a.P)) : nullptr;
+  const float tix = trow ? trow[3] : 0.f, tiy = trow ? trow[2] : 0.f;
+  const float2* wsp = (TILT && a.tg_in) ? a.Wsp + (size_t(b) * MK + mk) * a.S * N * N : nullptr;
+  double tg[4] = {0.0, 0.0, 0.0, 0.0};  // tilt gradient: inside y, x; after y, x
   float2 v[G::R1];
   // a later segment left the wave of slice s_hi - 1 (read in the first row phase)
   const bool from_state = a.s_hi < a.S;
@@ -1846,6 +2124,45 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
       gt_store_col<N>(v, sl, L, r);
     }
     sl.to_rows(X, L);
+    if ((TILT && a.tilt_after)) {
+      // the gradient through the after-tilt ramp and of its tilt
+      // (objective.py:620-636), band-limited for one slice
+      tilt_tables_grid<N>(tt, a.px, a.py, a.lambda);
+      __syncthreads();
+      const float tax = trow ? trow[5] : 0.f, tay = trow ? trow[4] : 0.f;
+      const RView<N, float2, G::NT> XE(a.Xe ? a.Xe + (size_t(b) * MK + mk) * N * N : a.Psi, tid);
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
+        gt_load_rowspec<N>(v, sl, L, r);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+        const int y = r * G::LINES + L.ln;
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) {
+          const int x = L.j + 8 * n;
+          v[n] = cmulc(v[n], tilt_ramp(tt, y, x, tax, tay));
+          if ((TILT && a.tg_after)) {
+            const float2 e = XE[r * G::R1 + n];
+            const double zi = double(v[n].y) * double(e.x) - double(v[n].x) * double(e.y);  // Im(g conj(e))
+            tg[2] += 2.0 * double(tt[y].y) * zi;
+            tg[3] += 2.0 * double(tt[x].x) * zi;
+          }
+          // the slice loop's row inverse follows this row pair (N times the
+          // wave) unless the band limit's 1/N^2 comes between
+          if (a.S != 1) v[n] = cscale(v[n], 1.f / float(N));
+        }
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        gt_store_rowspec<N>(v, sl, L, r);
+      }
+      if (a.S == 1) {
+        sl.to_cols(X, L);
+        gt_band_columns<N>(v, X, sl, L, a.maskc, a.zcol);
+      }
+      sl.to_rows(X, L);
+    }
+  }
+  if ((TILT && a.tilt_win) && !a.dists) {
+    tilt_tables_inside<N>(tt, a.dz, tix, tiy, a.px, a.py);
+    __syncthreads();
   }
   // the stash slot of slice s is overwritten by this mode's window gradient
   // conj(u_s) * g; k_objgrad_gt sums the modes
@@ -1898,7 +2215,13 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
     }
     if (s > 0) {
       sl.to_cols(X, L);
-      const RView<N, float2, G::NT> H((s == 1) ? a.H0 : a.H, tid);
+      if ((TILT && a.tilt_win) && a.dists) {
+        __syncthreads();
+        tilt_tables_inside<N>(tt, a.dists[s - 1], tix, tiy, a.px, a.py);
+        __syncthreads();
+      }
+      const RView<N, float2, G::NT> H((s == 1 && !(TILT && a.tilt_win)) ? a.H0 : a.H + size_t(s - 1) * a.Hstride, tid);
+      double sy = 0.0, sx = 0.0;
       for (int r = 0; r < G::ROUNDS; ++r) {
         round_fence<decltype(sl)>();
         if ((zh >> r) & 1u) {
@@ -1906,14 +2229,46 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
           continue;
         }
         float2 h[G::R1];
+        if (!wsp) {
 #pragma unroll
-        for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
+          for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
+        }
         gt_load_col<N>(v, sl, L, r);
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        if (wsp) {
+          // sum Im(conj(fft2 u_s) fft2(g_s) q) with Wsp = fft2(u_s) / N^2
+          // (multislice_core.py:169-178)
+          const RView<N, float2, G::NT> W(wsp + size_t(s) * N * N, tid);
+#pragma unroll
+          for (int i = 0; i < G::R1; ++i) {
+            int fy, fx;
+            G::cspec((r * G::R1 + i) * G::NT + tid, fy, fx);
+            const float2 w = W[r * G::R1 + i];
+            const double zi = double(w.x) * double(v[i].y) - double(w.y) * double(v[i].x);
+            sx += double(fftfreq_f(fx, N, a.px)) * zi;
+            sy += double(fftfreq_f(fy, N, a.py)) * zi;
+          }
+#pragma unroll
+          for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
+        }
+        if ((TILT && a.tilt_win)) {
+#pragma unroll
+          for (int i = 0; i < G::R1; ++i) {
+            int fy, fx;
+            G::cspec((r * G::R1 + i) * G::NT + tid, fy, fx);
+            h[i] = tilted_h(h[i], tt, fy, fx, N, s == 1, a.edge);
+          }
+        }
 #pragma unroll
         for (int i = 0; i < G::R1; ++i) v[i] = cmulc(v[i], h[i]);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
         gt_store_col<N>(v, sl, L, r);
+      }
+      if (wsp) {
+        double d = double(slice_dist(a, s - 1));
+        if (s == 1 && a.edge > 1.f) d *= double(a.edge);
+        tg[0] += -12.566370614359172 * d * sy;
+        tg[1] += -12.566370614359172 * d * sx;
       }
       sl.to_rows(X, L);
     }
@@ -1926,6 +2281,16 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
       rt_line_inv<N, true>(v, X, L.ln, L.j);
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) GS.store(r * G::R1 + n, v[n]);
+    }
+  }
+  if ((TILT && a.tg_in) || (TILT && a.tg_after)) {
+    // this segment's part; the segments of a window add in order
+    for (int c = 0; c < 4; ++c) {
+      const double S = block_sum(tg[c], red);
+      if (tid == 0) {
+        double* dst = a.tilt_part + (size_t(b) * MK + mk) * 4 + c;
+        *dst = (a.s_hi < a.S ? *dst : 0.0) + S;
+      }
     }
   }
 }
@@ -2375,6 +2740,35 @@ __global__ void k_bc_final(BatchArgs a) {
   a.bc_grad[idx] = float(s);
 }
 
+// tilts_grad (multislice_core.py:176-177, objective.py:618-632): per window the
+// sum over its probe and object modes of the inside (columns 2, 3) and after
+// (4, 5) parts, into the window's row; with one tilt row every window of the
+// batch adds into row 0 in batch order (one thread)
+__global__ void k_tilt_final(BatchArgs a) {
+  const int MK = a.M * a.K;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.n_tilt == 1) {
+    if (b != 0) return;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < a.B; ++w) {
+      if (seq_index(a.seq, a.b0 + w, a.P) < 0) continue;
+      for (int mk = 0; mk < MK; ++mk)
+        for (int c = 0; c < 4; ++c) acc[c] += a.tilt_part[(size_t(w) * MK + mk) * 4 + c];
+    }
+    for (int c = 0; c < 4; ++c)
+      if (c < 2 ? a.tg_in : a.tg_after) a.tilt_grad[2 + c] = float(double(a.tilt_grad[2 + c]) + acc[c]);
+    return;
+  }
+  if (b >= a.B) return;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  if (idx < 0) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int mk = 0; mk < MK; ++mk)
+    for (int c = 0; c < 4; ++c) acc[c] += a.tilt_part[(size_t(b) * MK + mk) * 4 + c];
+  for (int c = 0; c < 4; ++c)
+    if (c < 2 ? a.tg_in : a.tg_after) a.tilt_grad[size_t(idx) * 6 + 2 + c] = float(acc[c]);
+}
+
 // each window's phase plate exp(-i chi), chi = sum_j A_j c_j with c its
 // position's row (objective.py:324; the band mask comes with the shift
 // factor), 1 for a window without a row; one thread per element
@@ -2480,13 +2874,13 @@ enum {
   KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
   KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER,
   KT_RECOMPUTE, KT_PREPROCESS, KT_BG_GRAD, KT_PROBE_REDUCE, KT_BC_FINAL, KT_AB_TABLE, KT_AB_PLATE, KT_AB_PART,
-  KT_AB_FINAL, KT_COUNT
+  KT_AB_FINAL, KT_TILT_FINAL, KT_COUNT
 };
 const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
                                     "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
                                     "loss_final", "zero", "scatteradd", "gather_grad", "recompute", "preprocess",
                                     "bg_grad", "probe_reduce", "bc_final", "aber_table", "aber_plate",
-                                    "aber_part", "aber_final"};
+                                    "aber_part", "aber_final", "tilt_final"};
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
@@ -2573,13 +2967,13 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 enum {
   WB_H = 0, WB_H0, WB_MASKC, WB_ZCOL, WB_FPROBE, WB_OBJP, WB_GRADP, WB_PROBE_PART, WB_PROBE_SUM, WB_LOSS_PART,
   WB_ABT, WB_OB, WB_PP, WB_STASH, WB_GSTATE, WB_PSI, WB_COEF, WB_GP, WB_MEAS_STAGE, WB_POS_PART, WB_BC_PART,
-  WB_PLATE, WB_AB_W, WB_AB_PART, WB_SCRATCH, WB_COUNT
+  WB_PLATE, WB_AB_W, WB_AB_PART, WB_WSP, WB_XE, WB_TILT_PART, WB_SCRATCH, WB_COUNT
 };
 constexpr int kFirstLaneBuffer = WB_OB;
 const char* kBufNames[WB_COUNT] = {"H", "H0", "maskc", "zcol", "fprobe", "objp", "gradp", "probe_part",
                                    "probe_sum", "loss_part", "abt", "Ob", "Pp", "stash", "gstate", "Psi", "coef",
                                    "gP", "meas_stage", "pos_part", "bc_part", "plate", "ab_w", "ab_part",
-                                   "scratch"};
+                                   "Wsp", "Xe", "tilt_part", "scratch"};
 struct Layout {
   size_t off[WB_COUNT], bytes[WB_COUNT];  // lane 0's offset for the per-lane buffers
   size_t lane0, lane_bytes, total;
@@ -2635,6 +3029,29 @@ bool preprocessing(const pty_problem* p) {
          p->data_pad != 0 || (p->data_multiplier != 0.f && p->data_multiplier != 1.f);
 }
 
+// per-slice distances: one propagator table per propagation
+bool per_slice(const pty_problem* p) { return p->n_distances > 1; }
+int n_prop_tables(const pty_problem* p) { return per_slice(p) ? std::max(1, p->n_slices - 1) : 1; }
+// Tilt switches of a pass (objective.py:328-350, 465-499): each window's own
+// inside tilt multiplies its propagators when there is a row per position
+// and either the distances are per slice (multislice_core.py:71 takes the
+// batch's tilts) or the mode tilts inside (0, 3, 4); otherwise the tables
+// take row 0's.  The exit wave is tilted in modes 1, 3, 4.
+struct TiltPlan {
+  bool win, after, g_in, g_after;
+};
+TiltPlan tilt_plan(const pty_problem* p) {
+  TiltPlan t{};
+  if (p->n_tilts < 1) return t;
+  const int md = p->tilt_mode;
+  const bool inside = md == 0 || md == 3 || md == 4;
+  t.win = p->n_tilts > 1 && (per_slice(p) || inside);
+  t.after = md == 1 || md == 3 || md == 4;
+  t.g_in = p->step_tilts && inside;
+  t.g_after = p->step_tilts && t.after;
+  return t;
+}
+
 Layout plan_layout(const pty_problem* p) {
   Layout L{};
   const size_t n2 = size_t(p->n) * p->n;
@@ -2653,7 +3070,7 @@ Layout plan_layout(const pty_problem* p) {
     L.bytes[id] = bytes;
     o += al(bytes);
   };
-  take(WB_H, n2 * 8);
+  take(WB_H, size_t(n_prop_tables(p)) * n2 * 8);
   take(WB_H0, n2 * 8);
   take(WB_MASKC, n2 * 4);
   take(WB_ZCOL, size_t(p->n) * 4);
@@ -2684,6 +3101,10 @@ Layout plan_layout(const pty_problem* p) {
   take(WB_PLATE, size_t(p->n_aberrations ? B : 0) * n2 * 8);
   take(WB_AB_W, p->step_aberrations ? size_t(B) * M * n2 * 4 : 0);
   take(WB_AB_PART, p->step_aberrations ? size_t(B) * p->n_aberrations * 8 : 0);
+  const TiltPlan tp = tilt_plan(p);
+  take(WB_WSP, tp.g_in ? size_t(B) * M * K * S * n2 * 8 : 0);
+  take(WB_XE, tp.g_after ? size_t(B) * M * K * n2 * 8 : 0);
+  take(WB_TILT_PART, (tp.g_in || tp.g_after) ? size_t(B) * M * K * 4 * 8 : 0);
   // windows larger than 128 px: one tile per workgroup of the largest launch
   L.scratch_tiles = p->n > 128 ? int(std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K,
                                                        size_t(G) * M, size_t(M)}))
@@ -2708,6 +3129,12 @@ int validate(const pty_problem* p) {
   if (p->n_aberrations > 0 && p->n_markers < 1) return PTY_ERR_ARG;
   if (p->step_aberrations && p->n_aberrations == 0) return PTY_ERR_ARG;
   if (p->algorithm < PTY_LOSS_LSQ_SQRT || p->algorithm > PTY_LOSS_POISSON_COMPRESSED) return PTY_ERR_ARG;
+  if (p->n_distances < 0 || (p->n_distances > 1 && p->n_distances < p->n_slices - 1)) return PTY_ERR_ARG;
+  if (p->n_tilts < 0 || (p->n_tilts > 1 && p->n_tilts != p->n_positions)) return PTY_ERR_ARG;
+  if (p->tilt_mode < 0 || p->tilt_mode > 4) return PTY_ERR_ARG;
+  // tilts before the specimen (modes 2, 4) are not on this path
+  if (p->n_tilts > 0 && (p->tilt_mode == 2 || p->tilt_mode == 4)) return PTY_ERR_ARG;
+  if (p->step_tilts && p->n_tilts < 1) return PTY_ERR_ARG;
   if (compressed(p->algorithm) && (p->n_masks < 1 || p->n_masks > kMaxMasks)) return PTY_ERR_ARG;
   if (preprocessing(p)) {
     const int raw = p->raw_n ? p->raw_n : p->n, bin = std::max(p->data_bin, 1);
@@ -2731,6 +3158,9 @@ int validate_buffers(const pty_problem* p, const pty_buffers* b) {
   if (p->step_beam_current && (!b->beam_current || !b->beam_current_grad)) return PTY_ERR_ARG;
   if (p->n_aberrations > 0 && (!b->aberrations || !b->aberration_marker)) return PTY_ERR_ARG;
   if (p->step_aberrations && !b->aberrations_grad) return PTY_ERR_ARG;
+  if (per_slice(p) && !b->distances) return PTY_ERR_ARG;
+  if (p->n_tilts > 0 && !b->tilts) return PTY_ERR_ARG;
+  if (p->step_tilts && !b->tilts_grad) return PTY_ERR_ARG;
   if (b->measured_order != PTY_MEASURED_BY_POSITION && b->measured_order != PTY_MEASURED_BY_SEQUENCE)
     return PTY_ERR_ARG;
   return PTY_OK;
@@ -2807,24 +3237,43 @@ int issue_zero(X& x, int lane, const BatchArgs& a, void* ptr, size_t bytes) {
   }, SUB_NONE, ptr, bytes);
 }
 
+// the tilt code paths (per-window propagator factors, after-tilt ramps, tilt
+// gradients) are a separate instantiation, so that the kernels without tilts
+// keep their registers
+inline bool tilted(const BatchArgs& a) { return a.tilt_win || a.tilt_after || a.tg_in || a.tg_after; }
+
 template <int N>
 void launch_forward(const BatchArgs& a, hipStream_t s) {
   constexpr int LAY = kLayout<N>;
   const int grid = xcd_grid(a.B, a.M * a.K);
-  if constexpr (LAY == LAY_RT)
-    k_forward_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
-  else
-    k_forward_gt<N><<<grid, kGtThreads<N>, 0, s>>>(a);
+  if constexpr (LAY == LAY_RT) {
+    if (tilted(a))
+      k_forward_rt<N, true><<<grid, RT<N>::NT, 0, s>>>(a);
+    else
+      k_forward_rt<N, false><<<grid, RT<N>::NT, 0, s>>>(a);
+  } else {
+    if (tilted(a))
+      k_forward_gt<N, true><<<grid, kGtThreads<N>, 0, s>>>(a);
+    else
+      k_forward_gt<N, false><<<grid, kGtThreads<N>, 0, s>>>(a);
+  }
 }
 
 template <int N>
 void launch_adjoint(const BatchArgs& a, hipStream_t s) {
   constexpr int LAY = kLayout<N>;
   const int grid = xcd_grid(a.B, a.M * a.K);
-  if constexpr (LAY == LAY_RT)
-    k_adjoint_rt<N><<<grid, RT<N>::NT, 0, s>>>(a);
-  else
-    k_adjoint_gt<N><<<grid, kGtThreads<N>, 0, s>>>(a);
+  if constexpr (LAY == LAY_RT) {
+    if (tilted(a))
+      k_adjoint_rt<N, true><<<grid, RT<N>::NT, 0, s>>>(a);
+    else
+      k_adjoint_rt<N, false><<<grid, RT<N>::NT, 0, s>>>(a);
+  } else {
+    if (tilted(a))
+      k_adjoint_gt<N, true><<<grid, kGtThreads<N>, 0, s>>>(a);
+    else
+      k_adjoint_gt<N, false><<<grid, kGtThreads<N>, 0, s>>>(a);
+  }
 }
 
 // one workgroup per (window, slice of [s_lo, s_hi), object mode)
@@ -2908,7 +3357,7 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
       PTY_TRY(x.launch(KT_OBJGRAD, lane, g, [&](hipStream_t s) { launch_objgrad<N>(g, s); }));
     }
   } else {
-    if (p->step_obj || p->step_probe || p->step_pos || p->step_beam_current || p->step_aberrations)
+    if (p->step_obj || p->step_probe || p->step_pos || p->step_beam_current || p->step_aberrations || p->step_tilts)
       PTY_TRY(x.launch(KT_ADJOINT, lane, a, [&](hipStream_t s) { launch_adjoint<N>(a, s); }));
     if (p->step_obj) PTY_TRY(x.launch(KT_OBJGRAD, lane, a, [&](hipStream_t s) { launch_objgrad<N>(a, s); }));
   }
@@ -2936,6 +3385,8 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
     PTY_TRY(x.launch(KT_POS_FINAL, lane, a, [&](hipStream_t s) { k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
   if (p->step_beam_current)
     PTY_TRY(x.launch(KT_BC_FINAL, lane, a, [&](hipStream_t s) { k_bc_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
+  if (a.tg_in || a.tg_after)
+    PTY_TRY(x.launch(KT_TILT_FINAL, lane, a, [&](hipStream_t s) { k_tilt_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
   if (p->step_aberrations) {
     PTY_TRY(x.launch(KT_AB_PART, lane, a, [&](hipStream_t s) { k_aber_part<N><<<a.B * a.NA, 256, 0, s>>>(a); }));
     PTY_TRY(x.launch(KT_AB_FINAL, lane, a, [&](hipStream_t s) {
@@ -2999,6 +3450,18 @@ BatchArgs pass_args(const pty_problem* p, const pty_buffers* bf, const Layout& L
   a.ab_w = p->step_aberrations ? reinterpret_cast<float*>(at(WB_AB_W)) : nullptr;
   a.ab_part = p->step_aberrations ? reinterpret_cast<double*>(at(WB_AB_PART)) : nullptr;
   a.ab_grad = p->step_aberrations ? bf->aberrations_grad : nullptr;
+  a.Hstride = per_slice(p) ? size_t(N) * N : 0;
+  a.dists = per_slice(p) ? bf->distances : nullptr;
+  a.dz = p->slice_distance;
+  const TiltPlan tp = tilt_plan(p);
+  a.tilt = p->n_tilts > 0 ? bf->tilts : nullptr;
+  a.n_tilt = p->n_tilts;
+  a.tilt_win = tp.win; a.tilt_after = tp.after; a.tg_in = tp.g_in; a.tg_after = tp.g_after;
+  a.lambda = p->wavelength; a.px = p->pixel_size_x; a.py = p->pixel_size_y; a.edge = p->edge_multiplier;
+  a.Wsp = reinterpret_cast<float2*>(at(WB_WSP));
+  a.Xe = reinterpret_cast<float2*>(at(WB_XE));
+  a.tilt_part = reinterpret_cast<double*>(at(WB_TILT_PART));
+  a.tilt_grad = p->step_tilts ? bf->tilts_grad : nullptr;
   whole_pass(a);
   if (L.ck) {
     // the forward keeps the checkpoints and every wave of the last segment
@@ -3025,10 +3488,14 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
   char* ws = static_cast<char*>(bf->workspace);
   const BatchArgs a = pass_args<N>(p, bf, L, ws);
   PTY_TRY(x.launch(KT_TABLES, 0, a, [&](hipStream_t s) {
+    // tables without tilt when each window brings its own; else row 0's inside
+    // tilt (or the problem's tan_x, tan_y without tilt rows)
+    const bool own = a.tilt_win != 0;
     TableArgs t{const_cast<float2*>(a.H), const_cast<float2*>(a.H0), const_cast<float*>(a.maskc),
                 const_cast<int*>(a.zcol), p->slice_distance, p->wavelength, p->pixel_size_x, p->pixel_size_y,
-                p->tan_x, p->tan_y, p->damping_cutoff, p->edge_multiplier};
-    k_tables<N, LAY><<<(N * N + 255) / 256, 256, 0, s>>>(t);
+                own ? 0.f : p->tan_x, own ? 0.f : p->tan_y, p->damping_cutoff, p->edge_multiplier,
+                a.dists, (a.tilt && !own) ? a.tilt : nullptr};
+    k_tables<N, LAY><<<dim3((N * N + 255) / 256, n_prop_tables(p)), 256, 0, s>>>(t);
   }));
   if (a.NA)
     PTY_TRY(x.launch(KT_AB_TABLE, 0, a, [&](hipStream_t s) {
@@ -3052,6 +3519,8 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     PTY_TRY(issue_zero(x, 0, a, bf->beam_current_grad, size_t(p->n_positions) * sizeof(float)));
   if (p->step_aberrations && !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, bf->aberrations_grad, size_t(p->n_markers) * p->n_aberrations * sizeof(float)));
+  if (p->step_tilts && !bf->accumulate)
+    PTY_TRY(issue_zero(x, 0, a, bf->tilts_grad, size_t(p->n_tilts) * 6 * sizeof(float)));
   if (p->step_obj && !bf->accumulate) PTY_TRY(issue_zero(x, 0, a, a.obj_grad, objelems * 8));
   if ((p->step_probe || p->step_pos || p->step_beam_current || p->step_aberrations) && !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, a.probe_part, size_t(L.G) * M * n2 * 8));
@@ -3065,6 +3534,7 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     y.Ob = off(y.Ob); y.Pp = off(y.Pp); y.stash = off(y.stash); y.gstate = off(y.gstate);
     y.Psi = off(y.Psi); y.coef = off(y.coef); y.gP = off(y.gP); y.pos_part = off(y.pos_part);
     y.bc_part = off(y.bc_part); y.plate = off(y.plate); y.ab_w = off(y.ab_w); y.ab_part = off(y.ab_part);
+    y.Wsp = off(y.Wsp); y.Xe = off(y.Xe); y.tilt_part = off(y.tilt_part);
     y.meas_stage = off(y.meas_stage); y.scratch = off(y.scratch);
   }
   if (nl > 1) {
@@ -3228,7 +3698,7 @@ struct Access {
   int buf;           // WB_* or an external buffer (kExt + PB_*)
 };
 enum { PB_OBJ = 0, PB_PROBE, PB_MEASURED, PB_LOSS_SSE, PB_OBJ_GRAD, PB_PROBE_GRAD, PB_POS_GRAD, PB_SIM,
-       PB_BG_GRAD, PB_BC, PB_BC_GRAD, PB_AB, PB_MARKER, PB_AB_GRAD, PB_COUNT };
+       PB_BG_GRAD, PB_BC, PB_BC_GRAD, PB_AB, PB_MARKER, PB_AB_GRAD, PB_DIST, PB_TILTS, PB_TILTS_GRAD, PB_COUNT };
 constexpr int kExt = 100;
 
 // the bytes of every range an op touches; gt: windows larger than 128 px
@@ -3260,6 +3730,12 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
   switch (o.kind) {
     case KT_TABLES:
       shared(WB_H, true); shared(WB_H0, true); shared(WB_MASKC, true); shared(WB_ZCOL, true);
+      add(a.dists, size_t(p->n_distances) * 4, false, kExt + PB_DIST);
+      add(a.tilt, size_t(p->n_tilts) * 24, false, kExt + PB_TILTS);
+      break;
+    case KT_TILT_FINAL:
+      add(a.tilt_part, size_t(B) * MK * 4 * 8, false, WB_TILT_PART);
+      add(a.tilt_grad, size_t(p->n_tilts) * 24, true, kExt + PB_TILTS_GRAD);
       break;
     case KT_AB_TABLE:
       shared(WB_ABT, true);
@@ -3324,6 +3800,10 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
         last = std::max(last, slot);
       }
       if (a.write_stash && last >= 0) stash_range(a.stash, last, true);
+      add(a.dists, size_t(p->n_distances) * 4, false, kExt + PB_DIST);
+      add(a.tilt, size_t(p->n_tilts) * 24, false, kExt + PB_TILTS);
+      add(a.Wsp, size_t(B) * MK * S * c8, true, WB_WSP);
+      if (!a.fin) add(a.Xe, size_t(B) * MK * c8, true, WB_XE);
       if (a.fin) {
         // the checkpoint of each (window, mode): fin + (b * MK + mk) * fin_stride
         add(a.fin, ((size_t(B) * MK - 1) * a.fin_stride + n2) * 8, false, WB_STASH);
@@ -3355,6 +3835,11 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       shared(WB_H, false); shared(WB_H0, false); shared(WB_ZCOL, false);
       if (a.step_obj) stash_range(a.stash, a.s_hi - 1 - a.s_lo, true);
       add(a.s_lo > 0 ? a.gstate : a.gP, size_t(B) * MK * c8, true, a.s_lo > 0 ? WB_GSTATE : WB_GP);
+      add(a.dists, size_t(p->n_distances) * 4, false, kExt + PB_DIST);
+      add(a.tilt, size_t(p->n_tilts) * 24, false, kExt + PB_TILTS);
+      add(a.Wsp, size_t(B) * MK * S * c8, false, WB_WSP);
+      if (a.s_hi == S) add(a.Xe, size_t(B) * MK * c8, false, WB_XE);
+      add(a.tilt_part, size_t(B) * MK * 4 * 8, true, WB_TILT_PART);
       tiles(xcd_grid(B, MK));
       break;
     case KT_OBJGRAD:
@@ -3451,6 +3936,9 @@ int plan_check_n(const pty_problem* p, int n_seq, int nl, int calls, int accumul
     bf.aberration_marker = static_cast<const int*>(fake(1 + PB_MARKER));
   }
   if (p->step_aberrations) bf.aberrations_grad = static_cast<float*>(fake(1 + PB_AB_GRAD));
+  if (per_slice(p)) bf.distances = static_cast<const float*>(fake(1 + PB_DIST));
+  if (p->n_tilts) bf.tilts = static_cast<const float*>(fake(1 + PB_TILTS));
+  if (p->step_tilts) bf.tilts_grad = static_cast<float*>(fake(1 + PB_TILTS_GRAD));
   int rc = validate_buffers(p, &bf);
   if (rc) return rc;
   const int n_batches = (n_seq + L.B - 1) / L.B;

@@ -103,8 +103,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     Returns (total_loss, sse, constraint_contributions, actual_step, count,
     d_value, new_d_value, warnings, counter_failed_epochs).
     """
-    global obj, probe, positions_correction, static_background, beam_current, aberrations_array, history_bfgs
-    _unsupported("this_step_tilts", bool(this_step_tilts))
+    global obj, probe, positions_correction, tilts, tilts_correction, static_background, beam_current
+    global aberrations_array, history_bfgs
     _unsupported("phase_only_obj", bool(phase_only_obj))
     _unsupported("tune_only_probe_phase", bool(tune_only_probe_phase))
     _unsupported("tune_only_probe_abs", bool(tune_only_probe_abs))
@@ -117,6 +117,15 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     update_obj = this_step_obj > 0
     update_probe = this_step_probe > 0
     update_pos = this_step_pos_correction > 0
+    # the tilts, a real (rows, 6) block refined through tilts_correction (iterative.py:477, 568, 588-670,
+    # 687, 768, 852-868, 892-897, 1027); in every loss evaluation, as there
+    update_tilts = this_step_tilts > 0
+    if tilts is None:
+        tilts = np.zeros((1, 6), np.float32)
+    tilts = _arrays.to_torch(tilts, obj.device, torch.float32).reshape(-1, 6).contiguous()
+    if tilts_correction is None:
+        tilts_correction = torch.zeros_like(tilts)
+    tilts_correction = _arrays.to_torch(tilts_correction, obj.device, torch.float32).reshape(-1, 6).contiguous()
     # the static background, a real (N, N) block refined like the positions (iterative.py:480, 569, 604-671,
     # 688, 769, 855-862, 898-903, 1030)
     update_bg = this_step_static_background > 0
@@ -165,10 +174,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         probe_reg_weight = 0
         probe = _clean_aperture(probe)
 
-    def evaluate(o, p, pc, b, c, ab):
+    def evaluate(o, p, pc, b, c, ab, tc):
         out = objective.loss_and_direction(
-            o, p, positions, pc, tilts, tilts_correction, this_slice_distances, edge_slice_thickness_multiplier,
-            measured_array, algorithm_type, this_wavelength, update_probe, update_obj, update_pos, False, masks,
+            o, p, positions, pc, tilts, tc, this_slice_distances, edge_slice_thickness_multiplier,
+            measured_array, algorithm_type, this_wavelength, update_probe, update_obj, update_pos, update_tilts, masks,
             pixel_size_x_A, pixel_size_y_A, recon_type, Cs, defocus_array, alpha_near_field, damping_cutoff_multislice,
             0, propmethod, this_chopped_sequence, load_one_by_one, data_multiplier, data_pad, phase_plate_in_h5,
             this_loss_weight, data_bin, data_shift_vector, upsample_pattern, b, update_bg, tilt_mode,
@@ -193,19 +202,23 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         bgg = out[6] if update_bg else (torch.zeros_like(b) if isinstance(b, torch.Tensor) else 0)
         bcg = out[8] if update_bc else (torch.zeros_like(c) if isinstance(c, torch.Tensor) else 0)
         abg = out[7] if update_ab else (torch.zeros_like(ab) if isinstance(ab, torch.Tensor) else 0)
-        return loss, sse, og, pg, posg, out[9], bgg, bcg, abg
+        tg = out[5] if update_tilts else torch.zeros_like(tc)
+        return loss, sse, og, pg, posg, out[9], bgg, bcg, abg, tg
 
     if update_probe:
         probe = _clean(probe, cut, smooth_rolloff)
     if update_obj and clean:
         obj = _clean(obj, cut, smooth_rolloff)
 
-    on = (update_obj, update_probe, update_pos, update_bg, update_bc, update_ab)
-    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s", "beam_current_hist_s",
-              "aberrations_hist_s")
-    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y", "beam_current_hist_y",
-              "aberrations_hist_y")
-    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static", "gamma_beam_current", "gamma_aberrations")
+    # the blocks in the reference's order of its dot products (iterative.py:683-690): object, probe,
+    # positions, tilts, then the background, beam current and aberrations
+    on = (update_obj, update_probe, update_pos, update_tilts, update_bg, update_bc, update_ab)
+    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "tilts_hist_s", "static_background_hist_s",
+              "beam_current_hist_s", "aberrations_hist_s")
+    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "tilts_hist_y", "static_background_hist_y",
+              "beam_current_hist_y", "aberrations_hist_y")
+    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_tilts", "gamma_static", "gamma_beam_current",
+              "gamma_aberrations")
     NB = len(on)
 
     def direction_dot(u, g):
@@ -217,10 +230,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         return d
 
     if empty_hist:
-        total_loss, this_sse, og, pg, posg, cc, bgg, bcg, abg = evaluate(obj, probe, positions_correction,
-                                                                         static_background, beam_current,
-                                                                         aberrations_array)
-        g = [og, pg, posg, bgg, bcg, abg]
+        total_loss, this_sse, og, pg, posg, cc, bgg, bcg, abg, tg = evaluate(obj, probe, positions_correction,
+                                                                             static_background, beam_current,
+                                                                             aberrations_array, tilts_correction)
+        g = [og, pg, posg, tg, bgg, bcg, abg]
         u_obj = _neg(og) if update_obj else None
         d_val_obj = _dot2(u_obj, u_obj) if update_obj else 0.0
         if d_val_obj == 0:
@@ -231,14 +244,16 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     else:
         total_loss, this_sse = history_bfgs["prev_loss"], history_bfgs["prev_sse"]
         g = [history_bfgs["obj_grad"], history_bfgs["probe_grad"], history_bfgs["pos_grad"],
-             history_bfgs.get("static_background_grad"), history_bfgs.get("beam_current_grad"),
-             history_bfgs.get("aberrations_grad")]
-        if update_bg and not isinstance(g[3], torch.Tensor):
-            g[3] = torch.zeros_like(static_background)  # a background held fixed until now
-        if update_bc and not isinstance(g[4], torch.Tensor):
-            g[4] = torch.zeros_like(beam_current)  # a current held fixed until now
-        if update_ab and not isinstance(g[5], torch.Tensor):
-            g[5] = torch.zeros_like(aberrations_array)  # aberrations held fixed until now
+             history_bfgs.get("tilt_grad"), history_bfgs.get("static_background_grad"),
+             history_bfgs.get("beam_current_grad"), history_bfgs.get("aberrations_grad")]
+        if update_tilts and not isinstance(g[3], torch.Tensor):
+            g[3] = torch.zeros_like(tilts_correction)  # tilts held fixed until now
+        if update_bg and not isinstance(g[4], torch.Tensor):
+            g[4] = torch.zeros_like(static_background)  # a background held fixed until now
+        if update_bc and not isinstance(g[5], torch.Tensor):
+            g[5] = torch.zeros_like(beam_current)  # a current held fixed until now
+        if update_ab and not isinstance(g[6], torch.Tensor):
+            g[6] = torch.zeros_like(aberrations_array)  # aberrations held fixed until now
         cc = history_bfgs["constraint_contributions"]
         if update_probe:
             g[1] = _clean(g[1], cut, smooth_rolloff)
@@ -274,8 +289,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             if on[j] and gm > 0 and gm == gm:
                 lbfgs.axpby(u[j], None, gm, 0.0)
         d_value = direction_dot(u, g)
-    u_obj, u_probe, u_pos, u_bg, u_bc, u_ab = u
-    og, pg, posg, bgg, bcg, abg = g
+    u_obj, u_probe, u_pos, u_t, u_bg, u_bc, u_ab = u
+    og, pg, posg, tg, bgg, bcg, abg = g
     count = -1
     new_d_value = d_value
     this_wolfe_1 = False
@@ -294,9 +309,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             else static_background
         new_bc = lbfgs.axpby(beam_current.clone(), u_bc, 1.0, actual_step) if update_bc else beam_current
         new_ab = lbfgs.axpby(aberrations_array.clone(), u_ab, 1.0, actual_step) if update_ab else aberrations_array
-        new_total_loss, new_sse, nog, npg, nposg, ncc, nbgg, nbcg, nabg = evaluate(new_obj, new_probe, new_pc, new_bg,
-                                                                                   new_bc, new_ab)
-        new_d_value = direction_dot(u, (nog, npg, nposg, nbgg, nbcg, nabg))
+        new_tc = lbfgs.axpby(tilts_correction.clone(), u_t, 1.0, actual_step) if update_tilts else tilts_correction
+        new_total_loss, new_sse, nog, npg, nposg, ncc, nbgg, nbcg, nabg, ntg = evaluate(
+            new_obj, new_probe, new_pc, new_bg, new_bc, new_ab, new_tc)
+        new_d_value = direction_dot(u, (nog, npg, nposg, ntg, nbgg, nbcg, nabg))
         this_wolfe_1 = utils.wolfe_1(total_loss, new_total_loss, d_value, actual_step, wolfe_c1_constant)
         this_wolfe_2 = utils.wolfe_2(d_value, new_d_value, wolfe_c2_constant)
         if print_flag >= 3:
@@ -333,29 +349,32 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     s_obj = lbfgs.lincomb(u_obj, actual_step) if update_obj else 0
     s_probe = lbfgs.lincomb(u_probe, actual_step) if update_probe else 0
     s_pos = lbfgs.lincomb(u_pos, actual_step) if update_pos else 0
+    s_t = lbfgs.lincomb(u_t, actual_step) if update_tilts else 0
     s_bg = lbfgs.lincomb(u_bg, actual_step) if update_bg else 0
     s_bc = lbfgs.lincomb(u_bc, actual_step) if update_bc else 0
     s_ab = lbfgs.lincomb(u_ab, actual_step) if update_ab else 0
     y_obj = lbfgs.axpby(nog.clone(), og, 1.0, -1.0) if update_obj else 0
     y_probe = lbfgs.axpby(npg.clone(), pg, 1.0, -1.0) if update_probe else 0
     y_pos = lbfgs.axpby(nposg.clone(), posg, 1.0, -1.0) if update_pos else 0
+    y_t = lbfgs.axpby(ntg.clone(), tg, 1.0, -1.0) if update_tilts else 0
     y_bg = lbfgs.axpby(nbgg.clone(), bgg, 1.0, -1.0) if update_bg else 0
     y_bc = lbfgs.axpby(nbcg.clone(), bcg, 1.0, -1.0) if update_bc else 0
     y_ab = lbfgs.axpby(nabg.clone(), abg, 1.0, -1.0) if update_ab else 0
     for key, v in (("obj_hist_s", s_obj), ("probe_hist_s", s_probe), ("positions_hist_s", s_pos),
-                   ("static_background_hist_s", s_bg), ("beam_current_hist_s", s_bc), ("aberrations_hist_s", s_ab),
-                   ("obj_hist_y", y_obj), ("probe_hist_y", y_probe), ("positions_hist_y", y_pos),
-                   ("static_background_hist_y", y_bg), ("beam_current_hist_y", y_bc), ("aberrations_hist_y", y_ab)):
+                   ("tilts_hist_s", s_t), ("static_background_hist_s", s_bg), ("beam_current_hist_s", s_bc),
+                   ("aberrations_hist_s", s_ab), ("obj_hist_y", y_obj), ("probe_hist_y", y_probe),
+                   ("positions_hist_y", y_pos), ("tilts_hist_y", y_t), ("static_background_hist_y", y_bg),
+                   ("beam_current_hist_y", y_bc), ("aberrations_hist_y", y_ab)):
         h[key].append(v)
-    for key in ("tilts_hist_s", "tilts_hist_y"):
-        h[key].append(0)
     h["prev_loss"], h["prev_sse"] = new_total_loss, new_sse
     h["obj_grad"], h["probe_grad"], h["pos_grad"], h["static_background_grad"] = nog, npg, nposg, nbgg
+    h["tilt_grad"] = ntg
     h["beam_current_grad"], h["aberrations_grad"] = nbcg, nabg
     h["constraint_contributions"] = ncc
     yiTsi_tot = 0.0
     for y, s, key, flag in ((y_obj, s_obj, "gamma_obj", update_obj), (y_probe, s_probe, "gamma_probe", update_probe),
-                            (y_pos, s_pos, "gamma_pos", update_pos), (y_bg, s_bg, "gamma_static", update_bg),
+                            (y_pos, s_pos, "gamma_pos", update_pos), (y_t, s_t, "gamma_tilts", update_tilts),
+                            (y_bg, s_bg, "gamma_static", update_bg),
                             (y_bc, s_bc, "gamma_beam_current", update_bc),
                             (y_ab, s_ab, "gamma_aberrations", update_ab)):
         if flag:
@@ -372,6 +391,8 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     obj = new_obj
     if update_pos:
         positions_correction = new_pc
+    if update_tilts:
+        tilts_correction = new_tc
     if update_bg:
         static_background = new_bg
     if update_bc:
@@ -390,8 +411,9 @@ def _per_epoch(v, epoch):
     return v(epoch) if callable(v) else v
 
 
-def _save_epoch_files(folder, epoch, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None, ab=None):
-    """checkpoint_{beam_current,obj,probe,positions,stback}_epoch_N.npy of the variables refined in
+def _save_epoch_files(folder, epoch, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None, ab=None,
+                      tilts_in_use=None):
+    """checkpoint_{beam_current,obj,probe,positions,tilts,stback}_epoch_N.npy of the variables refined in
     epoch N, as pypty/utils.py:1942-1975 writes them when save_checkpoints_every_epoch selects N, and
     checkpoint_aberrations_epcoh_N.npy (the reference's spelling, which its restart reads) for
     aberrations being refined."""
@@ -409,15 +431,19 @@ def _save_epoch_files(folder, epoch, o, p, pos, corr, save_obj, save_probe, save
         np.save(name("probe"), p.cpu().numpy())
     if save_pos:
         np.save(name("positions"), (pos.double() + corr.double()).cpu().numpy())
+    if isinstance(tilts_in_use, torch.Tensor):
+        np.save(name("tilts"), tilts_in_use.cpu().numpy())
     if isinstance(bg, torch.Tensor):
         np.save(name("stback"), bg.cpu().numpy())
     if isinstance(ab, torch.Tensor):
         np.save(os.path.join(folder, f"checkpoint_aberrations_epcoh_{epoch}.npy"), ab.cpu().numpy())
 
 
-def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None, ab=None):
-    """co.npy / cp.npy / cg.npy, cs.npy for a background, cb.npy for a beam current and ca.npy for
-    aberrations being refined, as pypty/utils.py:2085-2118 writes them."""
+def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg=None, bc=None, ab=None,
+                     tilts_in_use=None):
+    """co.npy / cp.npy / cg.npy, cs.npy for a background, cb.npy for a beam current, ca.npy for
+    aberrations and ct.npy for the tilts (array plus correction) being refined, as
+    pypty/utils.py:2085-2118 writes them."""
     import os
 
     if not folder:
@@ -429,6 +455,8 @@ def _save_checkpoint(folder, o, p, pos, corr, save_obj, save_probe, save_pos, bg
         np.save(os.path.join(folder, "cp.npy"), p.cpu().numpy())
     if save_pos:
         np.save(os.path.join(folder, "cg.npy"), (pos.double() + corr.double()).cpu().numpy())
+    if isinstance(tilts_in_use, torch.Tensor):
+        np.save(os.path.join(folder, "ct.npy"), tilts_in_use.cpu().numpy())
     if isinstance(bg, torch.Tensor):
         np.save(os.path.join(folder, "cs.npy"), bg.cpu().numpy())
     if isinstance(bc, torch.Tensor):
@@ -610,8 +638,12 @@ def run(pypty_params):
     probe = utils.fourier_clean(torch.from_numpy(pr).to(dev), cutoff=cutoff, rolloff=rolloff)
     positions = torch.from_numpy(ipos).to(dev)
     positions_correction = torch.from_numpy(corr.astype(np.float32)).to(dev)
-    tilts = np.zeros((1, 6), np.float32)
-    tilts_correction = np.zeros((1, 6), np.float32)
+    # tilts (y, x) before, inside and after the specimen, one row or one per pattern
+    # (pypty/iterative.py:134-135), refined through their correction (utils.py:1741)
+    tilts = _arrays.to_torch(get("tilts", np.zeros((1, 6), np.float32)), dev, torch.float32).reshape(-1, 6)
+    tilts = tilts.contiguous()
+    tilts_correction = torch.zeros_like(tilts)
+    tilt_mode = int(get("tilt_mode", 0))
     static_background = _static_background_from_nothing(get("static_background", None), n, float(cutoff))
     beam_current = get("beam_current", None)
     if beam_current is not None:
@@ -639,7 +671,7 @@ def run(pypty_params):
     reset_bfgs_history()
     t0 = time.time()
     counter_failed = 0
-    fail_pos, fail_bg, fail_ab, fail_probe = np.inf, np.inf, np.inf, np.inf
+    fail_pos, fail_tilts, fail_bg, fail_ab, fail_probe = np.inf, np.inf, np.inf, np.inf, np.inf
     prev_steps = prev_constraints = 0
     losses = []
     is_first = True
@@ -653,12 +685,16 @@ def run(pypty_params):
         s_bg = _per_epoch(get("update_static_background", 0), epoch) if static_background is not None else 0
         s_bc = _per_epoch(get("update_beam_current", 0), epoch)
         s_ab = _per_epoch(get("update_aberrations_array", 0), epoch)
+        s_t = _per_epoch(get("update_tilts", 0), epoch)
         first_step = _per_epoch(step_param, epoch)  # per epoch, as pypty/iterative.py:325-336 reads it
-        # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, the static
-        # background after 4, the aberrations after 5, the probe after 6 (tilts are not refined here)
+        # auto-fixer (pypty/iterative.py:338-379): positions after 2 failed epochs, the tilts after 3,
+        # the static background after 4, the aberrations after 5, the probe after 6
         if counter_failed >= 2:
             fail_pos = epoch
             warnings += "\nWARNING: Scan grid refinement was manually disabled for the next 10 epochs!"
+            if counter_failed >= 3:
+                fail_tilts = epoch
+                warnings += "\nWARNING: Tilt grid refinement was manually disabled for the next 10 epochs!"
             if counter_failed >= 4:
                 fail_bg = epoch
                 warnings += "\nWARNING: Static background refinement was manually disabled for the next 10 epochs!"
@@ -670,6 +706,8 @@ def run(pypty_params):
                 warnings += "\nWARNING: Probe refinement was manually disabled for the next 10 epochs!"
         if abs(fail_pos - epoch) < 10:
             s_pos = 0
+        if abs(fail_tilts - epoch) < 10:
+            s_t = 0
         if abs(fail_bg - epoch) < 10:
             s_bg = 0
         if abs(fail_ab - epoch) < 10:
@@ -686,7 +724,10 @@ def run(pypty_params):
             ("fast_axis_reg_weight_positions", 0), ("slow_axis_reg_weight_positions", 0),
             ("deformation_reg_weight_positions", 0), ("probe_reg_constraint_weight", 0), ("window_weight", 0),
             ("atv_weight", 0), ("mixed_variance_weight", 0), ("mixed_variance_sigma", 0.5), ("wedge_mu", 0),
-            ("beta_wedge", 0), ("probe_mod_reg_weight", 0), ("probe_mod_tol", 0.05))}
+            ("beta_wedge", 0), ("probe_mod_reg_weight", 0), ("probe_mod_tol", 0.05),
+            ("fast_axis_reg_weight_tilts", 0), ("deformation_reg_weight_tilts", 0))}
+        # the reference reads the tilts' slow-axis weight from the positions' key (iterative.py:208)
+        cw["slow_axis_reg_weight_tilts"] = _per_epoch(get("slow_axis_reg_weight_positions", 0), epoch)
         window = get("window", None)
         this_window = None
         if cw["window_weight"] > 0 and window is not None:
@@ -695,7 +736,7 @@ def run(pypty_params):
                            else np.asarray(window, np.float32))
         # the history is reset when the objective changes: a different set of refined variables or of
         # non-zero constraint weights (pypty/iterative.py:383-389; of its weights run() takes these)
-        steps = sum(int(x != 0) for x in (s_bc, s_obj, s_probe, s_pos, s_bg, s_ab))
+        steps = sum(int(x != 0) for x in (s_bc, s_obj, s_probe, s_pos, s_t, s_bg, s_ab))
         constraints = sum(int(x != 0) for x in (lw, w_phase, w_abs, *[cw[k] for k in cw if k not in (
             "mixed_variance_sigma", "beta_wedge", "probe_mod_tol")], cw["mixed_variance_sigma"]))
         if epoch != epoch_prev and (steps != prev_steps or constraints != prev_constraints):
@@ -709,7 +750,7 @@ def run(pypty_params):
         out = bfgs_update(
             algorithm_type=get("algorithm", "lsq_sqrt"), this_slice_distances=slice_distances,
             edge_slice_thickness_multiplier=1.0, this_step_probe=s_probe, this_step_obj=s_obj,
-            this_step_pos_correction=s_pos, this_step_tilts=0, measured_array=measured, this_wavelength=wavelength,
+            this_step_pos_correction=s_pos, this_step_tilts=s_t, measured_array=measured, this_wavelength=wavelength,
             masks=None, pixel_size_x_A=px, pixel_size_y_A=py,
             phase_norm_weight=w_phase, abs_norm_weight=w_abs,
             stepsize_threshold_low=get("min_step", 1e-20), probe_reg_weight=cw["probe_reg_constraint_weight"], aperture_mask=get("aperture_mask", None),
@@ -727,7 +768,7 @@ def run(pypty_params):
             mixed_variance_sigma=cw["mixed_variance_sigma"], data_bin=1,
             data_shift_vector=[0, 0], smart_memory=False, default_float=np.float32, default_complex=np.complex64,
             default_int=np.int32, upsample_pattern=1,
-            this_step_static_background=s_bg, tilt_mode=0,
+            this_step_static_background=s_bg, tilt_mode=tilt_mode,
             fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=aberration_marker,
             this_step_aberrations_array=s_ab,
             probe_marker=None, compute_batch=get("compute_batch", 256), current_window=this_window,
@@ -735,11 +776,13 @@ def run(pypty_params):
             current_gaussian_filter=False, current_apply_gaussian_filter_amplitude=False, this_beam_current_step=s_bc,
             xp=None, remove_fft_cache=False, is_first_epoch=is_first,
             hist_length=_per_epoch(get("hist_length", 10), epoch), actual_step=first_step,
-            fast_axis_reg_weight_positions=cw["fast_axis_reg_weight_positions"], fast_axis_reg_weight_tilts=0,
+            fast_axis_reg_weight_positions=cw["fast_axis_reg_weight_positions"],
+            fast_axis_reg_weight_tilts=cw["fast_axis_reg_weight_tilts"],
             slow_axis_reg_weight_positions=cw["slow_axis_reg_weight_positions"],
-            slow_axis_reg_weight_tilts=0, scan_size=get("scan_size", None), scan_step_A=get("scan_step_A", 1),
+            slow_axis_reg_weight_tilts=cw["slow_axis_reg_weight_tilts"], scan_size=get("scan_size", None),
+            scan_step_A=get("scan_step_A", 1),
             current_deformation_reg_weight_positions=cw["deformation_reg_weight_positions"],
-            current_deformation_reg_weight_tilts=0, warnings=warnings, data_simulation_flag=False,
+            current_deformation_reg_weight_tilts=cw["deformation_reg_weight_tilts"], warnings=warnings, data_simulation_flag=False,
             data_simulation_exitwave=False, pos_puzzling=False, probe_mod_reg_weight=cw["probe_mod_reg_weight"], probe_mod_tol=cw["probe_mod_tol"],
             use_padded_intensity=True, counter_failed_epochs=counter_failed,
             checkpoint_slices=int(get("checkpoint_slices", 0)))
@@ -752,11 +795,11 @@ def run(pypty_params):
         if save_inter and epoch % int(save_inter) == 0:
             _save_checkpoint(folder, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
                              static_background if s_bg else None, beam_current if s_bc else None,
-                             aberrations_array if s_ab else None)
+                             aberrations_array if s_ab else None, (tilts + tilts_correction) if s_t else None)
         if save_every and epoch % int(save_every) == 0:
             _save_epoch_files(folder, epoch, obj, probe, positions, positions_correction, s_obj, s_probe, s_pos,
                               static_background if s_bg else None, beam_current if s_bc else None,
-                              aberrations_array if s_ab else None)
+                              aberrations_array if s_ab else None, (tilts + tilts_correction) if s_t else None)
         if save_log:
             free, total = torch.cuda.mem_get_info(dev)
             _log_loss(folder, {"epoch": epoch, "time / s": time.time() - t0, "loss": total_loss, "sse": sse,
@@ -769,6 +812,7 @@ def run(pypty_params):
             epoch_callback(epoch, obj, probe)
     _save_checkpoint(folder, obj, probe, positions, positions_correction, True, True, True)
     result = {"obj": obj, "probe": probe, "positions": positions.double() + positions_correction.double(),
-              "beam_current": beam_current, "aberrations_array": aberrations_array, "losses": losses,
+              "beam_current": beam_current, "aberrations_array": aberrations_array,
+              "tilts": tilts + tilts_correction, "losses": losses,
               "time_s": time.time() - t0}
     return result

@@ -92,6 +92,13 @@ class PassConfig:
     n_aberrations: int = 0
     n_markers: int = 0
     step_aberrations: bool = False
+    # per-slice distances (multislice_core.py:69-71): more than one means one per propagation
+    n_distances: int = 0
+    # tilt rows (1 or one per position) of the tilt array (objective.py:428-434), the reference's
+    # tilt_mode, and whether to return their gradient
+    tilt_mode: int = 0
+    n_tilts: int = 0
+    step_tilts: bool = False
 
     @property
     def compressed(self) -> bool:
@@ -111,7 +118,8 @@ class PassConfig:
             int(self.raw_n), int(self.data_shift_y), int(self.data_shift_x), int(self.data_bin), int(self.data_pad),
             float(self.data_multiplier), int(self.checkpoint_slices), int(self.lanes),
             int(bool(self.step_static_background)), int(bool(self.step_beam_current)), int(self.n_aberrations),
-            int(self.n_markers), int(bool(self.step_aberrations)))
+            int(self.n_markers), int(bool(self.step_aberrations)), int(self.n_distances), int(self.tilt_mode),
+            int(self.n_tilts), int(bool(self.step_tilts)))
 
 
 class PtychoPass:
@@ -144,7 +152,8 @@ class PtychoPass:
     def run(self, obj, probe_mnn, positions, corrections, measured, sequence, *, obj_grad=None, probe_grad=None,
             pos_grad=None, sim_patterns=None, sync=True, masks=None, measured_by_sequence=False, accumulate=False,
             loss_sse=None, static_background=None, static_background_grad=None, beam_current=None,
-            beam_current_grad=None, aberrations=None, aberration_marker=None, aberrations_grad=None):
+            beam_current_grad=None, aberrations=None, aberration_marker=None, aberrations_grad=None, distances=None,
+            tilts=None, tilts_grad=None):
         """Launch the pass.  probe_mnn is mode-major (M, n, n).  Returns (loss, sse) as floats
         when ``sync`` else the device tensor [loss, sse].
 
@@ -160,7 +169,10 @@ class PtychoPass:
         n_aberrations) float32 with aberration_marker (P,) int32, the row of each position, whose
         phase plate multiplies its probe spectrum (objective.py:320-326, 459-465; a row outside
         [0, n_markers) acts as none); aberrations_grad: (n_markers, n_aberrations) float32 receiving
-        their gradient when the config steps them (objective.py:673-681)."""
+        their gradient when the config steps them (objective.py:673-681); distances: (n_distances,)
+        float32 per-slice distances when the config has more than one (multislice_core.py:69-71);
+        tilts: (n_tilts, 6) float32, the tilt array plus its correction (objective.py:350, 428-434);
+        tilts_grad: (n_tilts, 6) float32 receiving their gradient when the config steps them."""
         c = self.cfg
         for t, name in ((obj, "obj"), (probe_mnn, "probe"), (positions, "positions"), (measured, "measured"),
                         (sequence, "sequence")):
@@ -216,6 +228,17 @@ class PtychoPass:
                 _lib.require_device_tensor(t, name)
                 if t.dtype != dt or tuple(t.shape) != shape or not t.is_contiguous():
                     raise ValueError(f"pypty_amd: {name} must be contiguous {dt} {shape}")
+        checks = []
+        if c.n_distances > 1:
+            checks.append((distances, "distances", (c.n_distances,)))
+        if c.n_tilts:
+            checks.append((tilts, "tilts", (c.n_tilts, 6)))
+            if c.step_tilts:
+                checks.append((tilts_grad, "tilts_grad", (c.n_tilts, 6)))
+        for t, name, shape in checks:
+            _lib.require_device_tensor(t, name)
+            if t.dtype != torch.float32 or tuple(t.shape) != shape or not t.is_contiguous():
+                raise ValueError(f"pypty_amd: {name} must be contiguous float32 {shape}")
         out = self.loss_sse if loss_sse is None else loss_sse
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         bufs = _lib.PtyBuffers(
@@ -227,7 +250,9 @@ class PtychoPass:
             ptr(static_background_grad) if c.step_static_background else None, ptr(beam_current),
             ptr(beam_current_grad) if c.step_beam_current else None,
             ptr(aberrations) if c.n_aberrations else None, ptr(aberration_marker) if c.n_aberrations else None,
-            ptr(aberrations_grad) if c.step_aberrations else None)
+            ptr(aberrations_grad) if c.step_aberrations else None,
+            ptr(distances) if c.n_distances > 1 else None, ptr(tilts) if c.n_tilts else None,
+            ptr(tilts_grad) if c.step_tilts else None)
         rc = self.lib.pty_loss_and_direction(ctypes.byref(self.c_problem), ctypes.byref(bufs),
                                               _lib.stream_ptr(self.device))
         _lib.check(rc, "loss_and_direction")
@@ -239,7 +264,7 @@ class PtychoPass:
     def run_streamed(self, obj, probe_mnn, positions, corrections, measured_host, sequence_host, *, obj_grad=None,
                      probe_grad=None, pos_grad=None, masks=None, static_background=None,
                      static_background_grad=None, beam_current=None, beam_current_grad=None, aberrations=None,
-                     aberration_marker=None, aberrations_grad=None):
+                     aberration_marker=None, aberrations_grad=None, distances=None, tilts=None, tilts_grad=None):
         """The same pass with the measured patterns left in host memory (objective.py:383-408,
         load_one_by_one): each batch's rows are gathered into one of two pinned buffers, copied to the
         device on a side stream while the previous batch computes, and the batch runs as an
@@ -283,7 +308,8 @@ class PtychoPass:
                      accumulate=k > 0, loss_sse=sums[k], static_background=static_background,
                      static_background_grad=static_background_grad, beam_current=beam_current,
                      beam_current_grad=beam_current_grad, aberrations=aberrations,
-                     aberration_marker=aberration_marker, aberrations_grad=aberrations_grad)
+                     aberration_marker=aberration_marker, aberrations_grad=aberrations_grad, distances=distances,
+                     tilts=tilts, tilts_grad=tilts_grad)
             free[slot].record(main)
             if k + 1 < len(starts):
                 stage(k + 1)  # the host gathers the next batch while this one computes
@@ -382,8 +408,7 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     _unsupported("recon_type", recon_type != "far_field")
     _unsupported("propmethod", propmethod != "multislice")
     _unsupported("algorithm_type", algorithm_type not in LOSSES)
-    _unsupported("this_step_tilts", bool(this_step_tilts))
-    _unsupported("tilt_mode", bool(tilt_mode))
+    _unsupported("tilts before the specimen (tilt_mode 2 and 4)", int(tilt_mode) in (2, 4))
     _unsupported("static_background with a compressed loss",
                  static_background is not None and "compressed" in str(algorithm_type))
     _unsupported("probe_marker", probe_marker is not None)
@@ -399,10 +424,11 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     if this_step_pos_correction and any(w != 0 for w in pos_weights) and scan_size is None:
         raise ValueError("pypty_amd.loss_and_direction: the scan-grid constraints on the positions need scan_size")
     dist = _host(this_distances, np.float64).ravel()
-    _unsupported("per-slice distances", dist.shape[0] != 1)
-    tt = _host(this_tilt_array, np.float64) + _host(this_tilts_correction, np.float64)
-    _unsupported("per-position tilts", tt.shape[0] != 1)
-    _unsupported("tilts before/after the specimen", bool(np.any(tt[0, [0, 1, 4, 5]] != 0)))
+    # the tilts in use are the array plus its correction, in the working precision (objective.py:350)
+    tt = _host(this_tilt_array, np.float32).reshape(-1, 6) + _host(this_tilts_correction, np.float32).reshape(-1, 6)
+    tilt_weights = (current_deformation_reg_weight_tilts, fast_axis_reg_weight_tilts, slow_axis_reg_weight_tilts)
+    if this_step_tilts and any(w != 0 for w in tilt_weights) and scan_size is None:
+        raise ValueError("pypty_amd.loss_and_direction: the scan-grid constraints on the tilts need scan_size")
 
     # the caller's array type (numpy, torch, or a device array of another library such as CuPy)
     # is the type of the returned gradients
@@ -477,6 +503,17 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     # per-batch preprocessing (utils.py:587-653; skipped for compressed data as there)
     shift_y, shift_x = (0, 0) if compressed else (int(data_shift_vector[0]), int(data_shift_vector[1]))
     pad = 0 if compressed else int(data_pad) // int(upsample_pattern)
+    # per-slice distances (multislice_core.py:69-71): one per propagation
+    S_dist = dist.shape[0]
+    if S_dist > 1 and S_dist < S - 1:
+        raise ValueError(f"pypty_amd.loss_and_direction: {S_dist} slice distances for {S} slices")
+    dists_dev = torch.as_tensor(dist.astype(np.float32)).to(device) if S_dist > 1 else None
+    # tilts (objective.py:428-499): one row or one per position
+    if tt.shape[0] not in (1, positions.shape[0]):
+        raise ValueError("pypty_amd.loss_and_direction: the tilt array needs one row or one per position")
+    tilts_dev = torch.as_tensor(np.ascontiguousarray(tt)).to(device)
+    step_tilts = bool(this_step_tilts)
+    tilts_grad = torch.empty_like(tilts_dev) if step_tilts else None
     check_windows_inside(positions, seq, n, ny, nx)
     shift_active = bool(torch.any(corrections != 0).item()) or bool(this_step_pos_correction)
     cutoff = float(damping_cutoff_multislice)
@@ -495,7 +532,8 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
         lanes=1 if stream else 4,  # a streamed pass runs one batch per call
         step_static_background=step_bg, step_beam_current=step_bc,
         n_aberrations=0 if ab is None else int(ab.shape[1]), n_markers=0 if ab is None else int(ab.shape[0]),
-        step_aberrations=step_ab)
+        step_aberrations=step_ab, n_distances=S_dist if S_dist > 1 else 0, tilt_mode=int(tilt_mode),
+        n_tilts=int(tt.shape[0]), step_tilts=step_tilts)
     engine = PtychoPass(cfg, device)
     object_grad = torch.empty_like(obj) if this_step_obj else torch.zeros_like(obj)
     probe_grad_mnn = torch.empty((M, n, n), dtype=torch.complex64, device=device) if this_step_probe else None
@@ -505,22 +543,24 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
                                         obj_grad=object_grad, probe_grad=probe_grad_mnn, pos_grad=pos_grad,
                                         masks=masks_dev, static_background=bg, static_background_grad=bg_grad,
                                         beam_current=bc, beam_current_grad=bc_grad, aberrations=ab,
-                                        aberration_marker=ab_marker, aberrations_grad=ab_grad)
+                                        aberration_marker=ab_marker, aberrations_grad=ab_grad, distances=dists_dev,
+                                        tilts=tilts_dev, tilts_grad=tilts_grad)
     else:
         loss, sse = engine.run(obj, probe_mnn, positions, corrections, measured, seq, obj_grad=object_grad,
                                probe_grad=probe_grad_mnn, pos_grad=pos_grad, masks=masks_dev,
                                static_background=bg, static_background_grad=bg_grad, beam_current=bc,
                                beam_current_grad=bc_grad, aberrations=ab, aberration_marker=ab_marker,
-                               aberrations_grad=ab_grad)
+                               aberrations_grad=ab_grad, distances=dists_dev, tilts=tilts_dev, tilts_grad=tilts_grad)
     if loss != loss:
         raise ValueError("A very specific bad thing. Loss is Nan.")
     probe_grad = (probe_grad_mnn.permute(1, 2, 0).contiguous() if this_step_probe
                   else torch.zeros_like(full_probe))
-    tilts_grad = torch.zeros(tt.shape, dtype=torch.float32, device=device)
+    if tilts_grad is None:
+        tilts_grad = torch.zeros_like(tilts_dev)
     # constraints (objective.py:765-921), the reference's fourteen contributions in its order; each
     # adds to the loss (after the loss weight) and to the gradient of what it constrains.  The probe
-    # ones are band-limited with the probe gradient (objective.py:927-933), the tilt ones need tilt
-    # refinement, which is outside this path.
+    # ones are band-limited with the probe gradient (objective.py:927-933); the tilt ones act on the
+    # tilts in use (array plus correction) when the tilts are refined.
     cc = np.zeros(14)
     pos_abs = None
     if this_step_pos_correction and any(w != 0 for w in pos_weights):
@@ -528,10 +568,19 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     pos_terms = ((0, constraints.compute_fast_axis_constraint_on_grid, fast_axis_reg_weight_positions),
                  (1, constraints.compute_deformation_constraint_on_grid, current_deformation_reg_weight_positions),
                  (11, constraints.compute_slow_axis_constraint_on_grid, slow_axis_reg_weight_positions))
+    tilt_abs = tilts_dev.to(torch.float64) if step_tilts and any(w != 0 for w in tilt_weights) else None
+    tilt_terms = {2: (constraints.compute_deformation_constraint_on_grid, current_deformation_reg_weight_tilts),
+                  3: (constraints.compute_fast_axis_constraint_on_grid, fast_axis_reg_weight_tilts),
+                  12: (constraints.compute_slow_axis_constraint_on_grid, slow_axis_reg_weight_tilts)}
     probe_reg = []
     for k in range(14):
         term, grad = 0.0, None
-        if k in (0, 1, 11):
+        if k in tilt_terms:
+            fn, w = tilt_terms[k]
+            if tilt_abs is not None and w != 0:
+                term, grad = fn(tilt_abs, scan_size, w)
+                tilts_grad += grad.to(tilts_grad.dtype)
+        elif k in (0, 1, 11):
             fn, w = next((f, w) for kk, f, w in pos_terms if kk == k)
             if pos_abs is not None and w != 0:
                 term, grad = fn(pos_abs, scan_size, w)

@@ -101,3 +101,19 @@ def test_planted_faults_are_reported(omit, what):
     cfg = _cfg(32, 5, 2, 2, 8, 4, ck=2, lanes=2)
     n, rep = _lib.plan_check(cfg.to_c(), 8, 2, 2, False, omit=omit)
     assert n > 0 and what in rep, rep[-2000:]
+
+
+@pytest.mark.parametrize("n,S,ck", [(32, 4, 0), (64, 5, 2), (192, 3, 0), (256, 2, 0)])
+@pytest.mark.parametrize("tilt", [
+    dict(tilt_mode=0, n_tilts=10, step_tilts=True), dict(tilt_mode=3, n_tilts=10, step_tilts=True),
+    dict(tilt_mode=1, n_tilts=1, step_tilts=True), dict(tilt_mode=0, n_tilts=1, step_tilts=False),
+    dict(tilt_mode=3, n_tilts=10, step_tilts=True, n_distances=4),
+])
+def test_tilts_and_per_slice_distances(n, S, ck, tilt):
+    """The tilt buffers (per-lane spectra, exit waves and partials; the tilt gradient, added into by
+    every batch with one tilt row) and the per-slice propagator tables."""
+    extra = dict(tilt)
+    if extra.get("n_distances"):
+        extra["n_distances"] = max(S - 1, 2)
+    _check(_cfg(n, S, 1, 2, 10, 3, ck=ck, lanes=3, **extra), calls=2)
+
