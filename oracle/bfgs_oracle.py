@@ -21,7 +21,8 @@ def new_history():
             "prev_sse": None, "obj_grad": None, "probe_grad": None, "pos_grad": None, "static_background_grad": None,
             "beam_current_grad": None, "empty_hist": True, "gamma_obj": 1, "gamma_probe": 1, "gamma_pos": 1,
             "gamma_static": 1, "gamma_beam_current": 1, "aberrations_hist_s": [], "aberrations_hist_y": [],
-            "aberrations_grad": None, "gamma_aberrations": 1}
+            "aberrations_grad": None, "gamma_aberrations": 1, "tilts_hist_s": [], "tilts_hist_y": [],
+            "tilt_grad": None, "gamma_tilts": 1}
 
 
 def _d(a, b):
@@ -37,7 +38,7 @@ def _d(a, b):
 def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_pos=False, actual_step=1.0,
                cutoff=2 / 3, update_extra_cut=0.005, wolfe_c1=0.1, wolfe_c2=0.9, max_count=20, reduce_factor=0.1,
                optimism=3, min_step=1e-20, hist_length=10, cdtype=np.complex64, update_bg=False, update_bc=False,
-               update_ab=False):
+               update_ab=False, update_tilts=False):
     """One epoch; mutates state and hist; returns (loss, sse, actual_step, count, d_value, new_d_value, accepted).
 
     A static background is state["bg"] (N, N) when present; it enters every loss evaluation and,
@@ -47,7 +48,10 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
     (iterative.py:481, 571, 591-673, 690, 771, 853-870, 910-915, 1029).  Aberration rows are
     state["ab"] (rows, coefficients) with data["aberration_marker"]; they enter every loss evaluation
     and, with update_ab, are refined as the sixth (iterative.py:478, 570, 590-672, 689, 770, 854-871,
-    904-909, 1028)."""
+    904-909, 1028).  Tilts are data["tilts"] (rows, 6) with data["tilt_mode"] plus state["tc"], their
+    correction; the loss sees their sum (objective.py:350) and, with update_tilts, the correction is
+    refined as the fourth block, after the positions (iterative.py:477, 568, 588-670, 687, 768,
+    852-868, 892-897, 1027)."""
     fd = np.float32 if np.dtype(cdtype) == np.complex64 else np.float64
     cut = float(fd(cutoff) - fd(update_extra_cut))
     bg0 = state.get("bg", data.get("static_background"))
@@ -57,31 +61,38 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
     marker = data.get("aberration_marker")
     ab0 = state.get("ab", data.get("aberrations")) if marker is not None else None
     update_ab = bool(update_ab) and ab0 is not None
-    on = (update_obj, update_probe, update_pos, update_bg, update_bc, update_ab)
+    tilts = data.get("tilts")
+    tc0 = None if tilts is None else state.get("tc", np.zeros_like(tilts, dtype=np.float32))
+    update_tilts = bool(update_tilts) and tilts is not None
+    on = (update_obj, update_probe, update_pos, update_tilts, update_bg, update_bc, update_ab)
     NB = len(on)
-    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "static_background_hist_s", "beam_current_hist_s",
-              "aberrations_hist_s")
-    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "static_background_hist_y", "beam_current_hist_y",
-              "aberrations_hist_y")
-    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_static", "gamma_beam_current", "gamma_aberrations")
+    S_KEYS = ("obj_hist_s", "probe_hist_s", "positions_hist_s", "tilts_hist_s", "static_background_hist_s",
+              "beam_current_hist_s", "aberrations_hist_s")
+    Y_KEYS = ("obj_hist_y", "probe_hist_y", "positions_hist_y", "tilts_hist_y", "static_background_hist_y",
+              "beam_current_hist_y", "aberrations_hist_y")
+    G_KEYS = ("gamma_obj", "gamma_probe", "gamma_pos", "gamma_tilts", "gamma_static", "gamma_beam_current",
+              "gamma_aberrations")
     for k in S_KEYS + Y_KEYS:
         hist.setdefault(k, [0] * len(hist["rho_hist"]))
     hist.setdefault("gamma_static", 1)
     hist.setdefault("gamma_beam_current", 1)
     hist.setdefault("gamma_aberrations", 1)
+    hist.setdefault("gamma_tilts", 1)
 
     def clean(x):
         return orc.fourier_clean(x, cutoff=cut, fdtype=fd).astype(cdtype)
 
     def evaluate(v):
-        o, p, c, b, cur, ab = v
+        o, p, c, tc, b, cur, ab = v
+        tl = None if tilts is None else (np.asarray(tilts, np.float32) + tc).astype(np.float32)
         out = orc.loss_and_direction(
             o, p, state["positions"], c, data["measured"], data["sequence"], dz=data["dz"],
             wavelength=data["wavelength"], pixel_size_x=data["px"], pixel_size_y=data["px"],
             algorithm=data.get("algorithm", "lsq_sqrt"), cutoff=cutoff, step_obj=update_obj,
             step_probe=update_probe, step_pos=update_pos, compute_batch=data.get("compute_batch", 64), cdtype=cdtype,
             static_background=b, step_static_background=update_bg, beam_current=cur, step_beam_current=update_bc,
-            aberrations=ab, aberration_marker=None if ab is None else marker, step_aberrations=update_ab)
+            aberrations=ab, aberration_marker=None if ab is None else marker, step_aberrations=update_ab,
+            distances=data.get("distances"), tilts=tl, tilt_mode=data.get("tilt_mode", 0), step_tilts=update_tilts)
         loss, sse, og, pg, posg = out[:5]
         # L1 constraints of the object (objective.py:811-830), in the loss and object gradient
         for wa, wp in ((0.0, data.get("phase_norm_weight", 0)), (data.get("abs_norm_weight", 0), 0.0)):
@@ -110,7 +121,9 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         bgg = extra.pop(0) if b is not None else None
         bcg = extra.pop(0) if cur is not None else None
         abg = extra.pop(0) if ab is not None else None
-        return loss, sse, [og, pg, posg, bgg if update_bg else (None if b is None else np.zeros_like(b)),
+        tg = extra.pop(0) if tl is not None else None
+        return loss, sse, [og, pg, posg, tg if update_tilts else (None if tl is None else np.zeros_like(tc)),
+                           bgg if update_bg else (None if b is None else np.zeros_like(b)),
                            bcg if update_bc else (None if cur is None else np.zeros_like(cur)),
                            abg if update_ab else (None if ab is None else np.zeros_like(ab))]
 
@@ -121,7 +134,7 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         state["probe"] = clean(state["probe"])
     if update_obj:
         state["obj"] = clean(state["obj"])
-    x = [state["obj"], state["probe"], state["corr"], bg0, bc0, ab0]
+    x = [state["obj"], state["probe"], state["corr"], tc0, bg0, bc0, ab0]
     if hist["empty_hist"]:
         loss, sse, g = evaluate(x)
         uo = -g[0] if update_obj else None
@@ -132,14 +145,16 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         hist["empty_hist"] = False
     else:
         loss, sse = hist["prev_loss"], hist["prev_sse"]
-        g = [hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist.get("static_background_grad"),
-             hist.get("beam_current_grad"), hist.get("aberrations_grad")]
-        if update_bg and g[3] is None:
-            g[3] = np.zeros_like(bg0)
-        if update_bc and g[4] is None:
-            g[4] = np.zeros_like(bc0)
-        if update_ab and g[5] is None:
-            g[5] = np.zeros_like(ab0)
+        g = [hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist.get("tilt_grad"),
+             hist.get("static_background_grad"), hist.get("beam_current_grad"), hist.get("aberrations_grad")]
+        if update_tilts and g[3] is None:
+            g[3] = np.zeros_like(tc0)
+        if update_bg and g[4] is None:
+            g[4] = np.zeros_like(bg0)
+        if update_bc and g[5] is None:
+            g[5] = np.zeros_like(bc0)
+        if update_ab and g[6] is None:
+            g[6] = np.zeros_like(ab0)
         if update_probe:
             g[1] = clean(g[1])
         q = [g[j].copy() if on[j] else None for j in range(NB)]
@@ -178,9 +193,10 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         nx = [clean(x[0] + fd(actual_step) * u[0]) if update_obj else x[0],
               x[1] + fd(actual_step) * u[1] if update_probe else x[1],
               (x[2] + fd(actual_step) * u[2]).astype(fd) if update_pos else x[2],
-              (x[3] + fd(actual_step) * u[3]).astype(fd) if update_bg else x[3],
-              (x[4] + fd(actual_step) * u[4]).astype(fd) if update_bc else x[4],
-              (x[5] + fd(actual_step) * u[5]).astype(fd) if update_ab else x[5]]
+              (x[3] + fd(actual_step) * u[3]).astype(fd) if update_tilts else x[3],
+              (x[4] + fd(actual_step) * u[4]).astype(fd) if update_bg else x[4],
+              (x[5] + fd(actual_step) * u[5]).astype(fd) if update_bc else x[5],
+              (x[6] + fd(actual_step) * u[6]).astype(fd) if update_ab else x[6]]
         nloss, nsse, ng = evaluate(nx)
         nd = ddot(u, ng)
         w1 = orc.wolfe_1(loss, nloss, d_value, actual_step, wolfe_c1)
@@ -207,7 +223,7 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         hist[S_KEYS[j]].append(s[j])
         hist[Y_KEYS[j]].append(y[j])
     hist["prev_loss"], hist["prev_sse"] = nloss, nsse
-    (hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist["static_background_grad"],
+    (hist["obj_grad"], hist["probe_grad"], hist["pos_grad"], hist["tilt_grad"], hist["static_background_grad"],
      hist["beam_current_grad"], hist["aberrations_grad"]) = ng
     tot = 0.0
     for j, key in enumerate(G_KEYS):
@@ -223,10 +239,12 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
     state["obj"], state["probe"] = nx[0], nx[1]
     if update_pos:
         state["corr"] = nx[2]
+    if update_tilts:
+        state["tc"] = nx[3]
     if update_bg:
-        state["bg"] = nx[3]
+        state["bg"] = nx[4]
     if update_bc:
-        state["bc"] = nx[4]
+        state["bc"] = nx[5]
     if update_ab:
-        state["ab"] = nx[5]
+        state["ab"] = nx[6]
     return loss, sse, actual_step, count, d_value, nd, True

@@ -395,3 +395,55 @@ def test_epochs_with_object_and_probe_constraints_match_oracle():
         assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
         assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
         assert float(rg[2][8]) > 0 and float(rg[2][6]) > 0  # the ATV and aperture contributions
+
+
+def test_epochs_refining_the_tilts_match_oracle():
+    """Per-position tilts inside and after the specimen (tilt_mode 3) refined through their correction
+    as a fourth block after the positions (iterative.py:477, 568-1027), with object and probe, then held
+    for an epoch (still in the loss): losses, line-search counts and the updated variables follow the
+    oracle."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 4 epochs refining object, probe and tilts (mode 3)", flush=True)
+    pr = make_problem(n=32, S=3, K=1, M=2, ny=64, nx=64, P=10, shift=False, seed=53, noise=0.02)
+    rng = np.random.default_rng(17)
+    tl = (rng.uniform(-1, 1, (10, 6)) * np.array([0, 0, 2e-2, 2e-2, 1e-3, 1e-3])).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy(),
+             "tc": np.zeros_like(tl)}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5, "tilts": tl, "tilt_mode": 3}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = torch.from_numpy(tl.copy()).to(dev)
+    iterative.tilts_correction = torch.zeros((10, 6), dtype=torch.float32, device=dev)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    try:
+        for epoch in range(4):
+            t_on = epoch < 3
+            np.random.seed(500 + epoch)
+            ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, update_tilts=t_on, actual_step=step_o)
+            np.random.seed(500 + epoch)
+            rg = call_loss_and_direction(iterative.bfgs_update,
+                                         **bfgs_kwargs(pr, actual_step=step_g, is_first_epoch=(epoch == 0),
+                                                       tilt_mode=3, this_step_tilts=int(t_on)))
+            step_o, step_g = ro[2], rg[3]
+            e_t = rel(iterative.tilts_correction.cpu().numpy(), state["tc"]) if np.any(state["tc"]) else 0.0
+            print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; step {rg[3]:.3e}/{ro[2]:.3e}; "
+                  f"count {rg[4]}/{ro[3]}; rel err tilt correction {e_t:.2e}", flush=True)
+            assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+            assert rg[4] == ro[3]
+            assert abs(rg[3] - ro[2]) <= 1e-5 * abs(ro[2])
+            assert rel(iterative.obj.cpu().numpy(), state["obj"]) < 1e-4
+            assert rel(iterative.probe.cpu().numpy(), state["probe"]) < 1e-4
+            assert e_t < 1e-4
+        assert float(np.abs(state["tc"]).max()) > 0  # the tilts moved
+    finally:
+        iterative.tilts = np.zeros((1, 6), np.float32)
+        iterative.tilts_correction = np.zeros((1, 6), np.float32)
