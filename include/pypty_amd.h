@@ -332,6 +332,23 @@ int pty_l1_constraint_c64(const void* obj, int64_t n, float abs_weight, float ph
 int pty_center_of_mass(const float* patterns, int64_t count, int ny, int nx, int flip_y, double* comx, double* comy,
                        void* stream);
 
+/* Charge flipping of the object, in place (pypty/objective.py:1014-1110,
+ * charge_flip without fancy_sigma, applied by iterative.py:503-506): per plane
+ * of obj (n complex64 in `planes` planes, layout PTY_LAYOUT_REFERENCE
+ * (ny, nx, planes) or PTY_LAYOUT_PLANAR) the phase arg O and absorption
+ * -log(|O| + 1e-10) below delta times their plane's largest value are
+ * multiplied by beta, and O becomes exp(-absorption + i phase).
+ * partial: device scratch of 130 * planes floats. */
+int pty_charge_flip_c64(void* obj, int64_t n, int planes, int layout, float delta_phase, float delta_abs,
+                        float beta_phase, float beta_abs, float* partial, void* stream);
+
+/* Gram-Schmidt over the probe modes, in place (pypty/objective.py:1119-1140,
+ * make_states_orthogonal, applied by iterative.py:498-499 when
+ * keep_probe_states_orthogonal): probe (npix, modes) complex64 with the mode
+ * innermost (the reference's (N, N, M)); for i = 1..M-1 and j < i,
+ * p_i -= p_j sum(conj(p_j) p_i) / (1e-10 + sum |p_j|^2), FP64 sums. */
+int pty_orthogonalize_modes_c64(void* probe, int64_t npix, int modes, void* stream);
+
 /* out = 2 Re(-i conj(z) g): Wirtinger gradient -> phase gradient
  * (pypty/utils.py:260-276, complex_grad_to_phase_grad); out float32, n elements. */
 int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream);

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import constraints_oracle as con
 from . import ms_oracle as orc
 
 
@@ -38,7 +39,7 @@ def _d(a, b):
 def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_pos=False, actual_step=1.0,
                cutoff=2 / 3, update_extra_cut=0.005, wolfe_c1=0.1, wolfe_c2=0.9, max_count=20, reduce_factor=0.1,
                optimism=3, min_step=1e-20, hist_length=10, cdtype=np.complex64, update_bg=False, update_bc=False,
-               update_ab=False, update_tilts=False):
+               update_ab=False, update_tilts=False, charge_flip=None, orthogonal=False):
     """One epoch; mutates state and hist; returns (loss, sse, actual_step, count, d_value, new_d_value, accepted).
 
     A static background is state["bg"] (N, N) when present; it enters every loss evaluation and,
@@ -51,7 +52,9 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
     904-909, 1028).  Tilts are data["tilts"] (rows, 6) with data["tilt_mode"] plus state["tc"], their
     correction; the loss sees their sum (objective.py:350) and, with update_tilts, the correction is
     refined as the fourth block, after the positions (iterative.py:477, 568, 588-670, 687, 768,
-    852-868, 892-897, 1027)."""
+    852-868, 892-897, 1027).  orthogonal makes the probe modes orthogonal before the probe is
+    band-limited (iterative.py:498-499); charge_flip = (delta_phase, delta_abs, beta_phase, beta_abs)
+    flips the object and restarts the history before the object is band-limited (:502-505)."""
     fd = np.float32 if np.dtype(cdtype) == np.complex64 else np.float64
     cut = float(fd(cutoff) - fd(update_extra_cut))
     bg0 = state.get("bg", data.get("static_background"))
@@ -131,8 +134,14 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
         return sum(_d(a, b) for a, b, f in zip(u, g, on) if f)
 
     if update_probe:
+        if orthogonal and state["probe"].shape[-1] > 1:
+            state["probe"] = con.make_states_orthogonal(state["probe"])
         state["probe"] = clean(state["probe"])
     if update_obj:
+        if charge_flip is not None:
+            state["obj"] = con.charge_flip(state["obj"], *charge_flip)
+            hist.clear()
+            hist.update(new_history())
         state["obj"] = clean(state["obj"])
     x = [state["obj"], state["probe"], state["corr"], tc0, bg0, bc0, ab0]
     if hist["empty_hist"]:

@@ -145,3 +145,31 @@ def mixed_object_variance_constraint(obj, w, sigma):
     dev = dev * (w / K)
     grad = np.fft.ifft2(np.fft.fft2(dev, axes=(0, 1)) * mask[:, :, None, None], axes=(0, 1))
     return term, grad
+
+
+def charge_flip(a, delta_phase=0.03, delta_abs=0.14, beta_phase=-0.95, beta_abs=-0.95):
+    """objective.py:1014-1110 without fancy_sigma: per slice and mode (axes 2, 3), the phase and the
+    absorption -log(|a| + 1e-10) below delta times their largest value are multiplied by beta, and the
+    object becomes exp(-absorption + i phase).  Works in a's precision, as the reference."""
+    a = np.asarray(a)
+    fd = np.float32 if a.dtype == np.complex64 else np.float64
+    phase = np.angle(a).astype(fd)
+    absorption = (-np.log(np.abs(a) + fd(1e-10))).astype(fd)
+    for i in range(phase.shape[-1]):
+        for j in range(phase.shape[-2]):
+            p = phase[:, :, j, i]
+            q = absorption[:, :, j, i]
+            p[p < fd(delta_phase) * np.max(p)] *= fd(beta_phase)
+            q[q < fd(delta_abs) * np.max(q)] *= fd(beta_abs)
+    return np.exp(-absorption + 1j * phase).astype(a.dtype)
+
+
+def make_states_orthogonal(probe_states):
+    """objective.py:1119-1140: Gram-Schmidt over the last axis, mode i minus its projections on the
+    modes j < i already orthogonalised, sum(conj(p_j) p_i) / (1e-10 + sum |p_j|^2)."""
+    p = np.array(probe_states, copy=True)
+    for i in range(p.shape[-1]):
+        for j in range(i):
+            c = np.sum(np.conj(p[:, :, j]) * p[:, :, i]) / (1e-10 + np.sum(np.abs(p[:, :, j]) ** 2))
+            p[:, :, i] = p[:, :, i] - p[:, :, j] * c
+    return p

@@ -414,6 +414,122 @@ __global__ __launch_bounds__(256) void k_phase_grad(const float2* __restrict__ g
   }
 }
 
+// Charge flipping of the object (objective.py:1014-1110 without fancy_sigma):
+// per plane the largest phase arg O and absorption -log(|O| + 1e-10); values
+// below delta times that largest one are multiplied by beta; the object
+// becomes exp(-absorption + i phase).  Planes: element i of plane i % planes
+// (the reference layout (ny, nx, S, K)) or i / plane_len (planar).
+__device__ __forceinline__ int cf_plane(size_t i, int planes, int layout, size_t plane_len) {
+  return layout == PTY_LAYOUT_REFERENCE ? int(i % size_t(planes)) : int(i / plane_len);
+}
+__device__ __forceinline__ size_t cf_index(size_t k, int plane, int planes, int layout, size_t plane_len) {
+  return layout == PTY_LAYOUT_REFERENCE ? k * size_t(planes) + plane : size_t(plane) * plane_len + k;
+}
+constexpr int kCfBlocks = 64;  // partial maxima per plane
+__global__ __launch_bounds__(256) void k_cf_max(const float2* __restrict__ obj, int planes, int layout,
+                                                size_t plane_len, float* __restrict__ partial) {
+  __shared__ float rp[4], ra[4];
+  const int plane = blockIdx.x / kCfBlocks, blk = blockIdx.x % kCfBlocks;
+  float mp = -INFINITY, ma = -INFINITY;
+  for (size_t k = size_t(blk) * 256 + threadIdx.x; k < plane_len; k += size_t(kCfBlocks) * 256) {
+    const float2 o = obj[cf_index(k, plane, planes, layout, plane_len)];
+    mp = fmaxf(mp, atan2f(o.y, o.x));
+    ma = fmaxf(ma, -logf(hypotf(o.x, o.y) + 1e-10f));
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    mp = fmaxf(mp, __shfl_down(mp, d, 64));
+    ma = fmaxf(ma, __shfl_down(ma, d, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    rp[threadIdx.x >> 6] = mp;
+    ra[threadIdx.x >> 6] = ma;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[2 * size_t(blockIdx.x)] = fmaxf(fmaxf(rp[0], rp[1]), fmaxf(rp[2], rp[3]));
+    partial[2 * size_t(blockIdx.x) + 1] = fmaxf(fmaxf(ra[0], ra[1]), fmaxf(ra[2], ra[3]));
+  }
+}
+// the per-plane maxima after the partials: out[2 * plane + {0, 1}]
+__global__ void k_cf_reduce(const float* __restrict__ partial, int planes, float* __restrict__ out) {
+  const int plane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (plane >= planes) return;
+  float mp = -INFINITY, ma = -INFINITY;
+  for (int b = 0; b < kCfBlocks; ++b) {
+    mp = fmaxf(mp, partial[2 * (size_t(plane) * kCfBlocks + b)]);
+    ma = fmaxf(ma, partial[2 * (size_t(plane) * kCfBlocks + b) + 1]);
+  }
+  out[2 * plane] = mp;
+  out[2 * plane + 1] = ma;
+}
+__global__ __launch_bounds__(256) void k_cf_apply(float2* __restrict__ obj, size_t n, int planes, int layout,
+                                                  size_t plane_len, const float* __restrict__ maxima, float dp, float da,
+                                                  float bp, float ba) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const int plane = cf_plane(i, planes, layout, plane_len);
+    const float2 o = obj[i];
+    float ph = atan2f(o.y, o.x), ab = -logf(hypotf(o.x, o.y) + 1e-10f);
+    if (ph < dp * maxima[2 * plane]) ph *= bp;
+    if (ab < da * maxima[2 * plane + 1]) ab *= ba;
+    const float m = expf(-ab);
+    float sn, cs;
+    sincosf(ph, &sn, &cs);
+    obj[i] = make_float2(m * cs, m * sn);
+  }
+}
+
+// Gram-Schmidt over probe modes (objective.py:1119-1140), one workgroup: for
+// i = 1..M-1 and j < i in the reference's order, p_i -= p_j c with
+// c = sum(conj(p_j) p_i) / (1e-10 + sum |p_j|^2) (FP64 sums in a fixed
+// order); probe (npix, M) with the mode innermost, as the reference's (N, N, M).
+__global__ __launch_bounds__(1024) void k_gram_schmidt(float2* __restrict__ p, int64_t npix, int M) {
+  __shared__ double red[3][16];
+  __shared__ double coef[2];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  for (int i = 1; i < M; ++i) {
+    for (int j = 0; j < i; ++j) {
+      double sr = 0.0, si = 0.0, nn = 0.0;
+      for (int64_t k = tid; k < npix; k += blockDim.x) {
+        const float2 a = p[k * M + j], b = p[k * M + i];
+        sr += double(a.x) * double(b.x) + double(a.y) * double(b.y);  // Re conj(a) b
+        si += double(a.x) * double(b.y) - double(a.y) * double(b.x);  // Im conj(a) b
+        nn += double(a.x) * double(a.x) + double(a.y) * double(a.y);
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_down(sr, o, 64);
+        si += __shfl_down(si, o, 64);
+        nn += __shfl_down(nn, o, 64);
+      }
+      if (lane == 0) {
+        red[0][wid] = sr;
+        red[1][wid] = si;
+        red[2][wid] = nn;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int w = 0; w < nw; ++w) {
+          a += red[0][w];
+          b += red[1][w];
+          c += red[2][w];
+        }
+        coef[0] = a / (1e-10 + c);
+        coef[1] = b / (1e-10 + c);
+      }
+      __syncthreads();
+      const float cr = float(coef[0]), ci = float(coef[1]);
+      for (int64_t k = tid; k < npix; k += blockDim.x) {
+        const float2 a = p[k * M + j];
+        float2 b = p[k * M + i];
+        b.x -= a.x * cr - a.y * ci;
+        b.y -= a.x * ci + a.y * cr;
+        p[k * M + i] = b;
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ================================================================ host
 namespace {
 
@@ -579,6 +695,30 @@ int pty_center_of_mass(const float* patterns, int64_t count, int ny, int nx, int
   if (count == 0) return PTY_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   k_center_of_mass<<<int(count), 256, 0, s>>>(patterns, ny, nx, flip_y ? 1 : 0, comx, comy);
+  return plane_launch_check();
+}
+
+int pty_charge_flip_c64(void* obj, int64_t n, int planes, int layout, float delta_phase, float delta_abs,
+                        float beta_phase, float beta_abs, float* partial, void* stream) {
+  if (!obj || !partial || n < 0 || planes < 1 || n % planes != 0) return PTY_ERR_ARG;
+  if (layout != PTY_LAYOUT_REFERENCE && layout != PTY_LAYOUT_PLANAR) return PTY_ERR_ARG;
+  if (n == 0) return PTY_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t plane_len = size_t(n) / size_t(planes);
+  float2* o = static_cast<float2*>(obj);
+  float* maxima = partial + 2 * size_t(planes) * kCfBlocks;
+  k_cf_max<<<planes * kCfBlocks, 256, 0, s>>>(o, planes, layout, plane_len, partial);
+  k_cf_reduce<<<(planes + 63) / 64, 64, 0, s>>>(partial, planes, maxima);
+  const int grid = int(std::min<size_t>((size_t(n) + 255) / 256, 8192));
+  k_cf_apply<<<grid, 256, 0, s>>>(o, size_t(n), planes, layout, plane_len, maxima, delta_phase, delta_abs, beta_phase,
+                                  beta_abs);
+  return plane_launch_check();
+}
+
+int pty_orthogonalize_modes_c64(void* probe, int64_t npix, int modes, void* stream) {
+  if (!probe || npix < 0 || modes < 1) return PTY_ERR_ARG;
+  if (modes == 1 || npix == 0) return PTY_OK;
+  k_gram_schmidt<<<1, 1024, 0, static_cast<hipStream_t>(stream)>>>(static_cast<float2*>(probe), npix, modes);
   return plane_launch_check();
 }
 

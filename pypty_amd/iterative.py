@@ -108,8 +108,7 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     _unsupported("phase_only_obj", bool(phase_only_obj))
     _unsupported("tune_only_probe_phase", bool(tune_only_probe_phase))
     _unsupported("tune_only_probe_abs", bool(tune_only_probe_abs))
-    _unsupported("do_charge_flip", bool(do_charge_flip))
-    _unsupported("keep_probe_states_orthogonal", bool(keep_probe_states_orthogonal) and probe.shape[-1] > 1)
+    _unsupported("charge flipping with fancy_sigma", bool(do_charge_flip) and fancy_sigma is not None)
     _unsupported("gaussian filters", bool(current_gaussian_filter) or bool(current_apply_gaussian_filter_amplitude))
     _unsupported("dynamically_resize_yx_object", bool(dynamically_resize_yx_object) and bool(this_step_pos_correction))
     _unsupported("pos_puzzling", bool(pos_puzzling) and bool(this_step_pos_correction))
@@ -206,9 +205,17 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
         return loss, sse, og, pg, posg, out[9], bgg, bcg, abg, tg
 
     if update_probe:
+        if keep_probe_states_orthogonal and probe.dim() == 3 and probe.shape[-1] > 1:
+            probe = objective.make_states_orthogonal(probe)  # iterative.py:498-499
         probe = _clean(probe, cut, smooth_rolloff)
-    if update_obj and clean:
-        obj = _clean(obj, cut, smooth_rolloff)
+    if update_obj:
+        if do_charge_flip:
+            # iterative.py:502-505: flip, then start the history afresh
+            obj = objective.charge_flip(obj, cf_delta_phase, cf_delta_abs, cf_beta_phase, cf_beta_abs, fancy_sigma)
+            reset_bfgs_history()
+            empty_hist = True
+        if clean:
+            obj = _clean(obj, cut, smooth_rolloff)
 
     # the blocks in the reference's order of its dot products (iterative.py:683-690): object, probe,
     # positions, tilts, then the background, beam current and aberrations

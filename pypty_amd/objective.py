@@ -342,6 +342,47 @@ def compute_full_l1_constraint(object, abs_norm_weight, phase_norm_weight, grad_
     return float(out.item()), grad
 
 
+def charge_flip(a, delta_phase=0.03, delta_abs=0.14, beta_phase=-0.95, beta_abs=-0.95, fancy_sigma=None, xp=None):
+    """Mirror of pypty/objective.py:1014-1110 on the device (pty_charge_flip_c64): per slice and mode,
+    the phase and the absorption -log(|a| + 1e-10) below delta times their largest value are
+    multiplied by beta, and the object becomes exp(-absorption + i phase).  a: (Ny, Nx, S, K)
+    complex64 (torch tensor, numpy or DLPack array); returns a new array of the caller's type.  The
+    fancy_sigma deconvolution and reconvolution (objective.py:1058-1108) is not on this path."""
+    if fancy_sigma is not None:
+        raise NotImplementedError("pypty_amd.charge_flip: fancy_sigma is outside the accelerated path")
+    kind, template = _arrays.kind_of(a), a
+    t = _arrays.to_torch(a, dtype=torch.complex64)
+    if t.device.type != "cuda":
+        t = t.to(torch.device("cuda", torch.cuda.current_device()))
+    t = t.contiguous().clone()
+    planes = int(np.prod(t.shape[2:])) if t.dim() > 2 else 1
+    partial = torch.empty(130 * planes, dtype=torch.float32, device=t.device)
+    rc = _lib.load().pty_charge_flip_c64(t.data_ptr(), t.numel(), planes, _lib.PTY_LAYOUT_REFERENCE,
+                                         float(delta_phase), float(delta_abs), float(beta_phase), float(beta_abs),
+                                         partial.data_ptr(), _lib.stream_ptr(t.device))
+    _lib.check(rc, "charge_flip")
+    return _arrays.like(t, kind, template)
+
+
+def make_states_orthogonal(probe_states, xp=None):
+    """Mirror of pypty/objective.py:1119-1140 on the device (pty_orthogonalize_modes_c64): Gram-Schmidt
+    over the probe modes (last axis) in the reference's order, mode i minus its projections
+    sum(conj(p_j) p_i) / (1e-10 + sum |p_j|^2) p_j on the modes j < i already made orthogonal, the
+    sums in FP64.  Returns a new array of the caller's type."""
+    kind, template = _arrays.kind_of(probe_states), probe_states
+    t = _arrays.to_torch(probe_states, dtype=torch.complex64)
+    if t.device.type != "cuda":
+        t = t.to(torch.device("cuda", torch.cuda.current_device()))
+    if t.dim() != 3:
+        raise NotImplementedError("pypty_amd.make_states_orthogonal: probe markers (4-d probes) are outside the "
+                                  "accelerated path")
+    t = t.contiguous().clone()
+    rc = _lib.load().pty_orthogonalize_modes_c64(t.data_ptr(), t.shape[0] * t.shape[1], t.shape[2],
+                                                 _lib.stream_ptr(t.device))
+    _lib.check(rc, "orthogonalize_modes")
+    return _arrays.like(t, kind, template)
+
+
 def check_windows_inside(positions: torch.Tensor, sequence: torch.Tensor, n: int, ny: int, nx: int) -> None:
     """Raise if a window [y, y+n) x [x, x+n) of the sequence leaves the object.
 

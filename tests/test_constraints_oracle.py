@@ -99,3 +99,44 @@ def test_missing_wedge_and_mixed_variance_gradients():
     assert co.mixed_object_variance_constraint(O[..., :1], 1.0, 0.3)[0] == 0.0
     same = np.repeat(O[..., :1], 3, axis=-1)
     assert co.mixed_object_variance_constraint(same, 1.0, 0.3)[0] < 1e-20
+
+
+def test_charge_flip_by_hand():
+    """objective.py:1014-1110 on four pixels worked out by hand: the largest phase is 1.0, so phases
+    below 0.3 (0.1 and -0.2) turn into -0.095 and 0.19; the largest absorption is -log 0.5, so only the
+    two unit amplitudes (absorption ~0) fall below 0.14 of it, and stay ~0 when flipped."""
+    from oracle import constraints_oracle as con
+
+    a = np.array([np.exp(0.1j), 0.9 * np.exp(0.5j), 0.5 * np.exp(1.0j), np.exp(-0.2j)]).reshape(2, 2, 1, 1)
+    out = con.charge_flip(a, 0.3, 0.14, -0.95, -0.95)
+    expect = np.array([np.exp(-0.095j), 0.9 * np.exp(0.5j), 0.5 * np.exp(1.0j), np.exp(0.19j)]).reshape(2, 2, 1, 1)
+    assert np.abs(out - expect).max() < 1e-9
+    # beta = 1 leaves the object as it is; every plane has its own maxima
+    b = np.random.default_rng(1).uniform(0.5, 1.0, (8, 8, 3, 2)) * np.exp(1j * np.random.default_rng(2).uniform(
+        -3, 3, (8, 8, 3, 2)))
+    assert np.abs(con.charge_flip(b, 0.3, 0.3, 1.0, 1.0) - b).max() < 1e-9
+    c = b.copy()
+    c[:, :, 1, 1] *= np.exp(0.5j)  # another plane's phases do not move this plane's threshold
+    assert np.abs(con.charge_flip(c, 0.2, 0.2, -0.9, -0.9)[:, :, 0, 0] -
+                  con.charge_flip(b, 0.2, 0.2, -0.9, -0.9)[:, :, 0, 0]).max() == 0
+
+
+def test_gram_schmidt_modes_are_orthogonal_and_span_the_same_space():
+    """objective.py:1119-1140: the modes come out mutually orthogonal, mode 0 unchanged (no
+    normalisation), and mode i differs from the input's by a combination of input modes 0..i-1."""
+    from oracle import constraints_oracle as con
+
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal((16, 16, 4)) + 1j * rng.standard_normal((16, 16, 4))
+    p[:, :, 1] += 0.8 * p[:, :, 0]  # strongly overlapping modes
+    q = con.make_states_orthogonal(p)
+    assert np.array_equal(q[:, :, 0], p[:, :, 0])
+    for i in range(4):
+        for j in range(i):
+            ip = abs(np.vdot(q[:, :, j], q[:, :, i]))
+            assert ip < 1e-10 * np.linalg.norm(q[:, :, j]) * np.linalg.norm(q[:, :, i]), (i, j, ip)
+    for i in range(1, 4):
+        basis = p[:, :, :i].reshape(-1, i)
+        d = (q[:, :, i] - p[:, :, i]).ravel()
+        coef, *_ = np.linalg.lstsq(basis, d, rcond=None)
+        assert np.linalg.norm(basis @ coef - d) < 1e-10 * np.linalg.norm(p[:, :, i])

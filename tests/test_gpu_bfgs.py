@@ -447,3 +447,46 @@ def test_epochs_refining_the_tilts_match_oracle():
     finally:
         iterative.tilts = np.zeros((1, 6), np.float32)
         iterative.tilts_correction = np.zeros((1, 6), np.float32)
+
+
+def test_epochs_with_charge_flip_and_orthogonal_modes_match_oracle():
+    """keep_probe_states_orthogonal and do_charge_flip (iterative.py:497-505): the probe modes made
+    orthogonal before the probe's band limit, the object flipped and the history restarted every
+    epoch; losses, line-search counts and the updated object and probe follow the oracle."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 3 epochs with charge flipping and orthogonal probe modes", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=3, ny=64, nx=64, P=10, shift=False, seed=61, noise=0.02)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.reset_bfgs_history()
+    cf = (0.1, 0.05, -0.5, -0.5)
+    step_o = step_g = 1.0
+    for epoch in range(3):
+        np.random.seed(600 + epoch)
+        ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, actual_step=step_o, charge_flip=cf,
+                                    orthogonal=True)
+        np.random.seed(600 + epoch)
+        rg = call_loss_and_direction(iterative.bfgs_update, **bfgs_kwargs(
+            pr, actual_step=step_g, is_first_epoch=(epoch == 0), do_charge_flip=True, cf_delta_phase=cf[0],
+            cf_delta_abs=cf[1], cf_beta_phase=cf[2], cf_beta_abs=cf[3], keep_probe_states_orthogonal=True))
+        step_o, step_g = ro[2], rg[3]
+        e_obj = rel(iterative.obj.cpu().numpy(), state["obj"])
+        e_probe = rel(iterative.probe.cpu().numpy(), state["probe"])
+        print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; step {rg[3]:.3e}/{ro[2]:.3e}; "
+              f"count {rg[4]}/{ro[3]}; rel err obj {e_obj:.2e} probe {e_probe:.2e}", flush=True)
+        assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+        assert rg[4] == ro[3]
+        assert abs(rg[3] - ro[2]) <= 1e-5 * abs(ro[2])
+        assert e_obj < 1e-4 and e_probe < 1e-4

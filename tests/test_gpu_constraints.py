@@ -116,3 +116,42 @@ def test_loss_and_direction_with_every_constraint():
     print("  vs complex64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v < 1e-4, (k, v)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 1, 1), (96, 80, 3, 2), (512, 512, 2, 1)])
+def test_charge_flip_matches_oracle(shape):
+    """objective.charge_flip (pty_charge_flip_c64) against oracle.constraints_oracle.charge_flip
+    (objective.py:1014-1110): per plane thresholds, phases and absorptions flipped."""
+    import torch
+
+    from oracle import constraints_oracle as con
+    from pypty_amd import objective
+
+    print(f"[constraints] charge flip {shape}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    a = (rng.uniform(0.6, 1.0, shape) * np.exp(1j * rng.uniform(-1.5, 2.5, shape))).astype(np.complex64)
+    ref = con.charge_flip(a.astype(np.complex128), 0.3, 0.14, -0.95, -0.9)
+    got = objective.charge_flip(torch.from_numpy(a).cuda(), 0.3, 0.14, -0.95, -0.9).cpu().numpy()
+    e = rel(got, ref)
+    print(f"  rel err vs FP64 oracle {e:.2e}", flush=True)
+    assert e < 1e-5
+    # numpy in, numpy out
+    assert isinstance(objective.charge_flip(a[:8, :8].copy()), np.ndarray)
+
+
+@pytest.mark.parametrize("n,M", [(64, 2), (128, 5), (256, 10)])
+def test_orthogonal_probe_states_match_oracle(n, M):
+    import torch
+
+    from oracle import constraints_oracle as con
+    from pypty_amd import objective
+
+    print(f"[constraints] Gram-Schmidt of {M} modes at {n} px", flush=True)
+    rng = np.random.default_rng(n + M)
+    p = (rng.standard_normal((n, n, M)) + 1j * rng.standard_normal((n, n, M))).astype(np.complex64)
+    p[:, :, 1] += 0.7 * p[:, :, 0]
+    ref = con.make_states_orthogonal(p.astype(np.complex128))
+    got = objective.make_states_orthogonal(torch.from_numpy(p).cuda()).cpu().numpy()
+    e = rel(got, ref)
+    print(f"  rel err vs FP64 oracle {e:.2e}", flush=True)
+    assert e < 1e-5
