@@ -468,6 +468,14 @@ struct BatchArgs {
   float2* Xe;
   double* tilt_part;
   float* tilt_grad;
+  // tilts before the specimen (objective.py:465-470, 660-672): each window's
+  // probe is |bc| P times the before-tilt ramp (in Pp); with a shift, Fw holds
+  // its spectrum (B, M, N, N, C-order) for the shift and the position
+  // gradient; tg_before: the gradient parts per window and probe mode in
+  // tb_part (B, M, 2: y, x)
+  int tilt_before, tg_before;
+  float2* Fw;
+  double* tb_part;
 };
 
 // a launch over every slice with one stash slot per slice
@@ -1485,9 +1493,10 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
   const RTIds<N> id;
   shift_tables_freq<N>(exy, dy, dx);
   __syncthreads();
-  const RView<N, float2> fp(a.fprobe + size_t(m) * N * N, tid);
+  // with a before-tilt each window brings its own spectrum, the current in it
+  const RView<N, float2> fp(a.Fw ? a.Fw + (size_t(b) * a.M + m) * N * N : a.fprobe + size_t(m) * N * N, tid);
   const RView<N, float> mask(a.maskc, tid);
-  const float s = bc_scale(a, idx) / float(N * N);
+  const float s = (a.Fw ? 1.f : bc_scale(a, idx)) / float(N * N);
   float2 v[R::R1];
   // E, and so the spectrum, is zero on the columns the band mask zeroes
   if (wave_cols_zero<N>(a.zcol, id.col)) {
@@ -1642,6 +1651,131 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_final_rt(const float2* __re
   float2* dst = probe_grad + size_t(blockIdx.x) * N * N;
 #pragma unroll
   for (int n = 0; n < R::R1; ++n) dst[id.row * N + id.rj + R::T * n] = v[n];
+}
+
+// ------------------------------------------ tilts before the specimen
+// the real-space tilt grid value c_j = 2 pi (j - N/2) / N * N * p / lambda (objective.py:304-306)
+__device__ __forceinline__ float tilt_grid(int j, int N, float p, float lambda) {
+  return float(double(j - N / 2) / double(N) * 6.2831855 * double(N) * double(p) / double(lambda));
+}
+
+// each window's probe before its shift: |bc| P times exp(i (cx tx + cy ty))
+// with the window's before-tilt (objective.py:446-450, 465-468), natural
+// order into Pp; one thread per element of (B, M, N, N)
+template <int N>
+__global__ __launch_bounds__(256) void k_tilt_probe(BatchArgs a) {
+  const size_t t = blockIdx.x * size_t(256) + threadIdx.x;
+  if (t >= size_t(a.B) * a.M * N * N) return;
+  const int b = int(t / (size_t(a.M) * N * N)), m = int((t / (N * N)) % a.M), e = int(t % (N * N));
+  const int y = e / N, x = e % N;
+  const int idx = seq_index(a.seq, a.b0 + b, a.P);
+  const float* tr = tilt_row(a, idx);
+  const float ty = tr ? tr[0] : 0.f, tx = tr ? tr[1] : 0.f;
+  float sn, cs;
+  sincosf(fadd(fmul(tilt_grid(x, N, a.px, a.lambda), tx), fmul(tilt_grid(y, N, a.py, a.lambda), ty)), &sn, &cs);
+  a.Pp[t] = cmul(cscale(a.probe[size_t(m) * N * N + e], bc_scale(a, idx)), make_float2(cs, sn));
+}
+
+// probe-side adjoint with a before-tilt (objective.py:641-672, 688-703) over
+// the register tile: per window and mode the gradient summed over object
+// modes, with a shift transformed, shifted back (position gradient from the
+// window's own spectrum Fw, FP64) and transformed back; then times the
+// conjugate ramp, the tilt's gradient 2 sum c Im(g conj(|bc| P)) and the beam
+// current's, and added in natural order (R-order) to the group's partial
+template <int N>
+__global__ __launch_bounds__(RT<N>::NT) void k_probegrad_tilt_rt(BatchArgs a) {
+  using R = RT<N>;
+  rt_tw_load<N>();
+  __shared__ float2 T[R::ELEMS];
+  __shared__ float2 exy[2 * N];
+  __shared__ float2 tt[N];
+  __shared__ double red[16];
+  const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const unsigned tid = threadIdx.x;
+  const RTIds<N> id;
+  tilt_tables_grid<N>(tt, a.px, a.py, a.lambda);
+  const RView<N, float2> acc(a.probe_part + (size_t(g) * a.M + m) * N * N, tid);
+  const RView<N, float> mask(a.maskc, tid);
+  float2 sum[R::R1];
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) sum[i] = acc[i];
+  __syncthreads();
+  for (int b = g; b < a.B; b += a.G) {
+    const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    float2 v[R::R1];
+    {
+      const RView<N, float2> g0(a.gP + (size_t(b) * a.M + m) * a.K * N * N, tid);
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) v[n] = g0[n];
+    }
+    for (int k = 1; k < a.K; ++k) {
+      const RView<N, float2> gk(a.gP + ((size_t(b) * a.M + m) * a.K + k) * N * N, tid);
+#pragma unroll
+      for (int n = 0; n < R::R1; ++n) v[n] = cadd(v[n], gk[n]);
+    }
+    if (a.moved) {
+      const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+      __syncthreads();  // the previous window's table and transposes are done
+      shift_tables_freq<N>(exy, dy, dx);
+      rt_fft2_fwd<N>(v, T, id);  // its first barrier also publishes exy
+      const RView<N, float2> F(a.Fw + (size_t(b) * a.M + m) * N * N, tid);
+      double sy = 0.0, sx = 0.0;
+      const float sc = 1.0f / float(N * N);
+#pragma unroll
+      for (int i = 0; i < R::R1; ++i) {
+        int fy, fx;
+        R::cspec(i * R::NT + tid, fy, fx);
+        const float mk = mask[i];
+        const float2 Q = cmulc(v[i], cscale(cmul(exy[fx], exy[N + fy]), mk));
+        if (a.step_pos) {
+          const float2 f = F[i];
+          const double zy = double(Q.y) * double(f.x) - double(Q.x) * double(f.y);
+          sx += -double(fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk)) * zy;
+          sy += -double(fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk)) * zy;
+        }
+        v[i] = cscale(Q, sc);
+      }
+      if (a.step_pos) {
+        const double SY = block_sum(sy, red);
+        const double SX = block_sum(sx, red);
+        if (tid == 0) {
+          const double sh = -2.0 / (double(N) * double(N));
+          a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
+          a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
+        }
+      }
+      rt_fft2_inv<N>(v, T, id);
+    }
+    const float* tr = tilt_row(a, idx);
+    const float ty = tr ? tr[0] : 0.f, tx = tr ? tr[1] : 0.f;
+    const float bcs = bc_scale(a, idx);
+    double ty_s = 0.0, tx_s = 0.0, sb = 0.0;
+#pragma unroll
+    for (int n = 0; n < R::R1; ++n) {
+      const int y = id.row, x = id.rj + R::T * n;
+      v[n] = cmulc(v[n], tilt_ramp(tt, y, x, tx, ty));
+      const float2 P = a.probe[size_t(m) * N * N + y * N + x];
+      const double zi = double(v[n].y) * double(P.x) - double(v[n].x) * double(P.y);  // Im(g conj(P))
+      ty_s += double(tt[y].y) * zi;
+      tx_s += double(tt[x].x) * zi;
+      sb += double(v[n].x) * double(P.x) + double(v[n].y) * double(P.y);
+      sum[n] = cadd(sum[n], cscale(v[n], bcs));
+    }
+    if (a.tg_before) {
+      const double SY = block_sum(ty_s, red);
+      const double SX = block_sum(tx_s, red);
+      if (tid == 0) {
+        a.tb_part[(size_t(b) * a.M + m) * 2] = 2.0 * double(bcs) * SY;
+        a.tb_part[(size_t(b) * a.M + m) * 2 + 1] = 2.0 * double(bcs) * SX;
+      }
+    }
+    if (a.bc_part) {
+      const double SB = block_sum(sb, red);
+      if (tid == 0) a.bc_part[size_t(b) * a.M + m] = bc_factor(a, idx) * SB;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R::R1; ++i) acc.store(i, sum[i]);
 }
 
 // ------------------------------------------ global-slab pipeline (192, 256)
@@ -2396,9 +2530,11 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probe_shift_gt(BatchArgs a) {
   SlabOf<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
   shift_tables_freq<N>(exy, dy, dx);
   __syncthreads();
-  const RView<N, float2, G::NT> fp(a.fprobe + size_t(m) * N * N, tid);
+  // with a before-tilt each window brings its own spectrum, the current in it
+  const RView<N, float2, G::NT> fp(a.Fw ? a.Fw + (size_t(b) * a.M + m) * N * N : a.fprobe + size_t(m) * N * N,
+                                   tid);
   const RView<N, float, G::NT> mask(a.maskc, tid);
-  const float s = bc_scale(a, idx) / float(N * N);
+  const float s = (a.Fw ? 1.f : bc_scale(a, idx)) / float(N * N);
   float2 v[G::R1];
   // E, and so the spectrum, is zero on the columns the band mask zeroes
   const unsigned zm = gt_cols_zero<N>(a.zcol, L);
@@ -2556,6 +2692,125 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_probegrad_gt(BatchArgs a) {
     if (a.bc_part) {
       const double SB = block_sum(sb, red);
       if (tid == 0) a.bc_part[size_t(b) * a.M + m] = bc_factor(a, idx) / (double(N) * double(N)) * SB;
+    }
+  }
+}
+
+// k_probegrad_tilt_rt for 192/256 px windows: the shift's transforms through
+// the workgroup's slab, the natural-order partial in slab R-order
+template <int N>
+__global__ __launch_bounds__(kGtThreads<N>) void k_probegrad_tilt_gt(BatchArgs a) {
+  using G = GT<N, kGtThreads<N>>;
+  rt_tw_load<N>();
+  __shared__ float2 X[G::LINES * G::LS];
+  __shared__ float2 exy[2 * N];
+  __shared__ float2 tt[N];
+  __shared__ double red[16];
+  const int g = blockIdx.x / a.M, m = blockIdx.x % a.M;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  GSlab<N> sl(a.scratch + size_t(blockIdx.x) * N * N);
+  tilt_tables_grid<N>(tt, a.px, a.py, a.lambda);
+  const RView<N, float2, G::NT> acc(a.probe_part + (size_t(g) * a.M + m) * N * N, tid);
+  const RView<N, float, G::NT> mask(a.maskc, tid);
+  const float sc = 1.0f / float(N * N);
+  float2 v[G::R1];
+  __syncthreads();
+  for (int b = g; b < a.B; b += a.G) {
+    const int idx = seq_index(a.seq, a.b0 + b, a.P);
+    const float2* gp = a.gP + (size_t(b) * a.M + m) * a.K * N * N;
+    if (a.moved) {
+      const float dy = idx >= 0 ? a.corr[2 * idx] : 0.f, dx = idx >= 0 ? a.corr[2 * idx + 1] : 0.f;
+      __syncthreads();  // the previous window's slab and table reads are done
+      shift_tables_freq<N>(exy, dy, dx);
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
+        const size_t row = size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = gp[row + 8 * n];
+        for (int k = 1; k < a.K; ++k)
+#pragma unroll
+          for (int n = 0; n < G::R1; ++n) v[n] = cadd(v[n], gp[size_t(k) * N * N + row + 8 * n]);
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        gt_store_rowspec<N>(v, sl, L, r);
+      }
+      sl.to_cols(X, L);  // the slab holds the row spectra; the table is published
+      const RView<N, float2, G::NT> F(a.Fw + (size_t(b) * a.M + m) * N * N, tid);
+      double sy = 0.0, sx = 0.0;
+      for (int r = 0; r < G::ROUNDS; ++r) {
+        round_fence<decltype(sl)>();
+        gt_load_col<N>(v, sl, L, r);
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+        for (int i = 0; i < G::R1; ++i) {
+          const int q = r * G::R1 + i;
+          int fy, fx;
+          G::cspec(q * G::NT + tid, fy, fx);
+          const float mk = mask[q];
+          const float2 Q = cmulc(v[i], cscale(cmul(exy[fx], exy[N + fy]), mk));
+          if (a.step_pos) {
+            const float2 f = F[q];
+            const double zy = double(Q.y) * double(f.x) - double(Q.x) * double(f.y);
+            sx += -double(fmul(fmul(-6.2831855f, fftfreq_f(fx, N, 1.f)), mk)) * zy;
+            sy += -double(fmul(fmul(-6.2831855f, fftfreq_f(fy, N, 1.f)), mk)) * zy;
+          }
+          v[i] = cscale(Q, sc);
+        }
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+        gt_store_col<N>(v, sl, L, r);
+      }
+      if (a.step_pos) {
+        const double SY = block_sum(sy, red);
+        const double SX = block_sum(sx, red);
+        if (tid == 0) {
+          const double sh = -2.0 / (double(N) * double(N));
+          a.pos_part[(size_t(b) * a.M + m) * 2] = sh * SY;
+          a.pos_part[(size_t(b) * a.M + m) * 2 + 1] = sh * SX;
+        }
+      }
+      sl.to_rows(X, L);
+    }
+    const float* tr = tilt_row(a, idx);
+    const float ty = tr ? tr[0] : 0.f, tx = tr ? tr[1] : 0.f;
+    const float bcs = bc_scale(a, idx);
+    double ty_s = 0.0, tx_s = 0.0, sb = 0.0;
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<decltype(sl)>();
+      const size_t row = size_t(r * G::LINES + L.ln) * N + L.j;
+      if (a.moved) {
+        gt_load_rowspec<N>(v, sl, L, r);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+      } else {
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = gp[row + 8 * n];
+        for (int k = 1; k < a.K; ++k)
+#pragma unroll
+          for (int n = 0; n < G::R1; ++n) v[n] = cadd(v[n], gp[size_t(k) * N * N + row + 8 * n]);
+      }
+      const int y = r * G::LINES + L.ln;
+#pragma unroll
+      for (int n = 0; n < G::R1; ++n) {
+        const int x = L.j + 8 * n;
+        v[n] = cmulc(v[n], tilt_ramp(tt, y, x, tx, ty));
+        const float2 P = a.probe[size_t(m) * N * N + row + 8 * n];
+        const double zi = double(v[n].y) * double(P.x) - double(v[n].x) * double(P.y);  // Im(g conj(P))
+        ty_s += double(tt[y].y) * zi;
+        tx_s += double(tt[x].x) * zi;
+        sb += double(v[n].x) * double(P.x) + double(v[n].y) * double(P.y);
+        acc.store(r * G::R1 + n, cadd(acc[r * G::R1 + n], cscale(v[n], bcs)));
+      }
+    }
+    if (a.tg_before) {
+      const double SY = block_sum(ty_s, red);
+      const double SX = block_sum(tx_s, red);
+      if (tid == 0) {
+        a.tb_part[(size_t(b) * a.M + m) * 2] = 2.0 * double(bcs) * SY;
+        a.tb_part[(size_t(b) * a.M + m) * 2 + 1] = 2.0 * double(bcs) * SX;
+      }
+    }
+    if (a.bc_part) {
+      const double SB = block_sum(sb, red);
+      if (tid == 0) a.bc_part[size_t(b) * a.M + m] = bc_factor(a, idx) * SB;
     }
   }
 }
@@ -2744,29 +2999,42 @@ __global__ void k_bc_final(BatchArgs a) {
 // sum over its probe and object modes of the inside (columns 2, 3) and after
 // (4, 5) parts, into the window's row; with one tilt row every window of the
 // batch adds into row 0 in batch order (one thread)
-__global__ void k_tilt_final(BatchArgs a) {
+// the window's parts in tilt-column order: before (from tb_part, per probe
+// mode), inside and after (from tilt_part, per probe and object mode)
+__device__ __forceinline__ void tilt_window_sums(const BatchArgs& a, int b, double* acc) {
   const int MK = a.M * a.K;
+  for (int c = 0; c < 6; ++c) acc[c] = 0.0;
+  if (a.tg_before)
+    for (int m = 0; m < a.M; ++m)
+      for (int c = 0; c < 2; ++c) acc[c] += a.tb_part[(size_t(b) * a.M + m) * 2 + c];
+  if (a.tg_in || a.tg_after)
+    for (int mk = 0; mk < MK; ++mk)
+      for (int c = 0; c < 4; ++c) acc[2 + c] += a.tilt_part[(size_t(b) * MK + mk) * 4 + c];
+}
+__device__ __forceinline__ bool tilt_col_on(const BatchArgs& a, int c) {
+  return c < 2 ? a.tg_before != 0 : (c < 4 ? a.tg_in != 0 : a.tg_after != 0);
+}
+__global__ void k_tilt_final(BatchArgs a) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6];
   if (a.n_tilt == 1) {
     if (b != 0) return;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double tot[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int w = 0; w < a.B; ++w) {
       if (seq_index(a.seq, a.b0 + w, a.P) < 0) continue;
-      for (int mk = 0; mk < MK; ++mk)
-        for (int c = 0; c < 4; ++c) acc[c] += a.tilt_part[(size_t(w) * MK + mk) * 4 + c];
+      tilt_window_sums(a, w, acc);
+      for (int c = 0; c < 6; ++c) tot[c] += acc[c];
     }
-    for (int c = 0; c < 4; ++c)
-      if (c < 2 ? a.tg_in : a.tg_after) a.tilt_grad[2 + c] = float(double(a.tilt_grad[2 + c]) + acc[c]);
+    for (int c = 0; c < 6; ++c)
+      if (tilt_col_on(a, c)) a.tilt_grad[c] = float(double(a.tilt_grad[c]) + tot[c]);
     return;
   }
   if (b >= a.B) return;
   const int idx = seq_index(a.seq, a.b0 + b, a.P);
   if (idx < 0) return;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int mk = 0; mk < MK; ++mk)
-    for (int c = 0; c < 4; ++c) acc[c] += a.tilt_part[(size_t(b) * MK + mk) * 4 + c];
-  for (int c = 0; c < 4; ++c)
-    if (c < 2 ? a.tg_in : a.tg_after) a.tilt_grad[size_t(idx) * 6 + 2 + c] = float(acc[c]);
+  tilt_window_sums(a, b, acc);
+  for (int c = 0; c < 6; ++c)
+    if (tilt_col_on(a, c)) a.tilt_grad[size_t(idx) * 6 + c] = float(acc[c]);
 }
 
 // each window's phase plate exp(-i chi), chi = sum_j A_j c_j with c its
@@ -2874,13 +3142,13 @@ enum {
   KT_FFT2 = 0, KT_TABLES, KT_LAYOUT, KT_PATCH, KT_FPROBE, KT_PROBE_SHIFT, KT_FORWARD, KT_LOSS, KT_ADJOINT,
   KT_OBJGRAD, KT_PROBEGRAD, KT_POS_FINAL, KT_PROBE_FINAL, KT_LOSS_FINAL, KT_ZERO, KT_SCATTER, KT_GATHER,
   KT_RECOMPUTE, KT_PREPROCESS, KT_BG_GRAD, KT_PROBE_REDUCE, KT_BC_FINAL, KT_AB_TABLE, KT_AB_PLATE, KT_AB_PART,
-  KT_AB_FINAL, KT_TILT_FINAL, KT_COUNT
+  KT_AB_FINAL, KT_TILT_FINAL, KT_TILT_PROBE, KT_COUNT
 };
 const char* kKindNames[KT_COUNT] = {"fft2_tiles", "tables", "layout", "patch", "fprobe", "probe_shift", "forward",
                                     "loss", "adjoint", "objgrad", "probegrad", "pos_final", "probe_final",
                                     "loss_final", "zero", "scatteradd", "gather_grad", "recompute", "preprocess",
                                     "bg_grad", "probe_reduce", "bc_final", "aber_table", "aber_plate",
-                                    "aber_part", "aber_final", "tilt_final"};
+                                    "aber_part", "aber_final", "tilt_final", "tilt_probe"};
 std::mutex g_tmu;
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
@@ -2967,13 +3235,13 @@ int launch_fft2(float2* d, int64_t batch, int mode, const float* mask, hipStream
 enum {
   WB_H = 0, WB_H0, WB_MASKC, WB_ZCOL, WB_FPROBE, WB_OBJP, WB_GRADP, WB_PROBE_PART, WB_PROBE_SUM, WB_LOSS_PART,
   WB_ABT, WB_OB, WB_PP, WB_STASH, WB_GSTATE, WB_PSI, WB_COEF, WB_GP, WB_MEAS_STAGE, WB_POS_PART, WB_BC_PART,
-  WB_PLATE, WB_AB_W, WB_AB_PART, WB_WSP, WB_XE, WB_TILT_PART, WB_SCRATCH, WB_COUNT
+  WB_PLATE, WB_AB_W, WB_AB_PART, WB_WSP, WB_XE, WB_TILT_PART, WB_FW, WB_TB_PART, WB_SCRATCH, WB_COUNT
 };
 constexpr int kFirstLaneBuffer = WB_OB;
 const char* kBufNames[WB_COUNT] = {"H", "H0", "maskc", "zcol", "fprobe", "objp", "gradp", "probe_part",
                                    "probe_sum", "loss_part", "abt", "Ob", "Pp", "stash", "gstate", "Psi", "coef",
                                    "gP", "meas_stage", "pos_part", "bc_part", "plate", "ab_w", "ab_part",
-                                   "Wsp", "Xe", "tilt_part", "scratch"};
+                                   "Wsp", "Xe", "tilt_part", "Fw", "tb_part", "scratch"};
 struct Layout {
   size_t off[WB_COUNT], bytes[WB_COUNT];  // lane 0's offset for the per-lane buffers
   size_t lane0, lane_bytes, total;
@@ -3038,7 +3306,7 @@ int n_prop_tables(const pty_problem* p) { return per_slice(p) ? std::max(1, p->n
 // batch's tilts) or the mode tilts inside (0, 3, 4); otherwise the tables
 // take row 0's.  The exit wave is tilted in modes 1, 3, 4.
 struct TiltPlan {
-  bool win, after, g_in, g_after;
+  bool win, after, before, g_in, g_after, g_before;
 };
 TiltPlan tilt_plan(const pty_problem* p) {
   TiltPlan t{};
@@ -3047,8 +3315,10 @@ TiltPlan tilt_plan(const pty_problem* p) {
   const bool inside = md == 0 || md == 3 || md == 4;
   t.win = p->n_tilts > 1 && (per_slice(p) || inside);
   t.after = md == 1 || md == 3 || md == 4;
+  t.before = md == 2 || md == 4;
   t.g_in = p->step_tilts && inside;
   t.g_after = p->step_tilts && t.after;
+  t.g_before = p->step_tilts && t.before;
   return t;
 }
 
@@ -3105,6 +3375,8 @@ Layout plan_layout(const pty_problem* p) {
   take(WB_WSP, tp.g_in ? size_t(B) * M * K * S * n2 * 8 : 0);
   take(WB_XE, tp.g_after ? size_t(B) * M * K * n2 * 8 : 0);
   take(WB_TILT_PART, (tp.g_in || tp.g_after) ? size_t(B) * M * K * 4 * 8 : 0);
+  take(WB_FW, (tp.before && p->shift_active) ? size_t(B) * M * n2 * 8 : 0);
+  take(WB_TB_PART, tp.g_before ? size_t(B) * M * 2 * 8 : 0);
   // windows larger than 128 px: one tile per workgroup of the largest launch
   L.scratch_tiles = p->n > 128 ? int(std::max<size_t>({size_t(B) * S * K, size_t(((B + 7) / 8) * 8) * M * K,
                                                        size_t(G) * M, size_t(M)}))
@@ -3132,8 +3404,8 @@ int validate(const pty_problem* p) {
   if (p->n_distances < 0 || (p->n_distances > 1 && p->n_distances < p->n_slices - 1)) return PTY_ERR_ARG;
   if (p->n_tilts < 0 || (p->n_tilts > 1 && p->n_tilts != p->n_positions)) return PTY_ERR_ARG;
   if (p->tilt_mode < 0 || p->tilt_mode > 4) return PTY_ERR_ARG;
-  // tilts before the specimen (modes 2, 4) are not on this path
-  if (p->n_tilts > 0 && (p->tilt_mode == 2 || p->tilt_mode == 4)) return PTY_ERR_ARG;
+  // tilts before the specimen (modes 2, 4) with aberration plates are not on this path
+  if (p->n_tilts > 0 && (p->tilt_mode == 2 || p->tilt_mode == 4) && p->n_aberrations > 0) return PTY_ERR_ARG;
   if (p->step_tilts && p->n_tilts < 1) return PTY_ERR_ARG;
   if (compressed(p->algorithm) && (p->n_masks < 1 || p->n_masks > kMaxMasks)) return PTY_ERR_ARG;
   if (preprocessing(p)) {
@@ -3173,7 +3445,7 @@ int validate_buffers(const pty_problem* p, const pty_buffers* b) {
 enum { EV_FORK = 0, EV_JOIN = 1, EV_ACC = 1 + kMaxLanes, EV_COUNT = 1 + 2 * kMaxLanes };
 enum { OP_RECORD = 1000, OP_WAIT = 1001 };
 // sub-kinds of a fill or layout launch
-enum { SUB_NONE = 0, SUB_TO_PLANAR, SUB_TO_REFERENCE };
+enum { SUB_NONE = 0, SUB_TO_PLANAR, SUB_TO_REFERENCE, SUB_PER_WINDOW };
 
 // One step of a pass: a launch of kind KT_* on a lane with its arguments, or
 // an event record / wait (OP_RECORD / OP_WAIT of event ev).  A fill records
@@ -3313,7 +3585,26 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
     PTY_TRY(x.launch(KT_AB_PLATE, lane, a, [&](hipStream_t s) {
       k_aber_plate<N><<<int((size_t(a.B) * N * N + 255) / 256), 256, 0, s>>>(a);
     }));
-  if (a.shift)
+  if (a.tilt_before) {
+    // the tilted probes, then with a shift their spectra and the shifted probes
+    PTY_TRY(x.launch(KT_TILT_PROBE, lane, a, [&](hipStream_t s) {
+      k_tilt_probe<N><<<int((size_t(a.B) * a.M * N * N + 255) / 256), 256, 0, s>>>(a);
+    }));
+    if (a.moved) {
+      PTY_TRY(x.launch(KT_FPROBE, lane, a, [&](hipStream_t s) {
+        if constexpr (LAY == LAY_RT)
+          k_fprobe_rt<N><<<a.B * a.M, RT<N>::NT, 0, s>>>(a.Pp, a.Fw);
+        else
+          k_fprobe_gt<N><<<a.B * a.M, kGtThreads<N>, 0, s>>>(a.Pp, a.Fw, a.scratch);
+      }, SUB_PER_WINDOW));
+      PTY_TRY(x.launch(KT_PROBE_SHIFT, lane, a, [&](hipStream_t s) {
+        if constexpr (LAY == LAY_RT)
+          k_probe_shift_rt<N, false><<<a.B * a.M, RT<N>::NT, 0, s>>>(a);
+        else
+          k_probe_shift_gt<N, false><<<a.B * a.M, kGtThreads<N>, 0, s>>>(a);
+      }));
+    }
+  } else if (a.shift)
     PTY_TRY(x.launch(KT_PROBE_SHIFT, lane, a, [&](hipStream_t s) {
       if constexpr (LAY == LAY_RT) {
         if (a.NA)
@@ -3367,7 +3658,14 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
       const int tiles = ((a.ny + GTILE - 1) / GTILE) * ((a.nx + GTILE - 1) / GTILE);
       k_gather_grad<N><<<planes * tiles, 256, 0, s>>>(a);
     }));
-  if (p->step_probe || (p->step_pos && a.shift) || p->step_beam_current || p->step_aberrations)
+  if (a.tilt_before && (p->step_probe || (p->step_pos && a.moved) || p->step_beam_current || a.tg_before))
+    PTY_TRY(x.launch(KT_PROBEGRAD, lane, a, [&](hipStream_t s) {
+      if constexpr (LAY == LAY_RT)
+        k_probegrad_tilt_rt<N><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+      else
+        k_probegrad_tilt_gt<N><<<a.G * a.M, kGtThreads<N>, 0, s>>>(a);
+    }));
+  else if (p->step_probe || (p->step_pos && a.shift) || p->step_beam_current || p->step_aberrations)
     PTY_TRY(x.launch(KT_PROBEGRAD, lane, a, [&](hipStream_t s) {
       if constexpr (LAY == LAY_RT) {
         if (a.NA)
@@ -3385,7 +3683,7 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
     PTY_TRY(x.launch(KT_POS_FINAL, lane, a, [&](hipStream_t s) { k_pos_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
   if (p->step_beam_current)
     PTY_TRY(x.launch(KT_BC_FINAL, lane, a, [&](hipStream_t s) { k_bc_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
-  if (a.tg_in || a.tg_after)
+  if (a.tg_in || a.tg_after || a.tg_before)
     PTY_TRY(x.launch(KT_TILT_FINAL, lane, a, [&](hipStream_t s) { k_tilt_final<<<(a.B + 63) / 64, 64, 0, s>>>(a); }));
   if (p->step_aberrations) {
     PTY_TRY(x.launch(KT_AB_PART, lane, a, [&](hipStream_t s) { k_aber_part<N><<<a.B * a.NA, 256, 0, s>>>(a); }));
@@ -3462,6 +3760,11 @@ BatchArgs pass_args(const pty_problem* p, const pty_buffers* bf, const Layout& L
   a.Xe = reinterpret_cast<float2*>(at(WB_XE));
   a.tilt_part = reinterpret_cast<double*>(at(WB_TILT_PART));
   a.tilt_grad = p->step_tilts ? bf->tilts_grad : nullptr;
+  a.tilt_before = tp.before; a.tg_before = tp.g_before;
+  a.Fw = reinterpret_cast<float2*>(at(WB_FW));
+  a.tb_part = reinterpret_cast<double*>(at(WB_TB_PART));
+  // a before-tilt gives every window its own probe
+  if (tp.before) a.shift = 1;
   whole_pass(a);
   if (L.ck) {
     // the forward keeps the checkpoints and every wave of the last segment
@@ -3522,7 +3825,8 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
   if (p->step_tilts && !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, bf->tilts_grad, size_t(p->n_tilts) * 6 * sizeof(float)));
   if (p->step_obj && !bf->accumulate) PTY_TRY(issue_zero(x, 0, a, a.obj_grad, objelems * 8));
-  if ((p->step_probe || p->step_pos || p->step_beam_current || p->step_aberrations) && !bf->accumulate)
+  if ((p->step_probe || p->step_pos || p->step_beam_current || p->step_aberrations || a.tg_before) &&
+      !bf->accumulate)
     PTY_TRY(issue_zero(x, 0, a, a.probe_part, size_t(L.G) * M * n2 * 8));
   // lane l's arguments: its own copies of the per-batch buffers
   BatchArgs al[kMaxLanes];
@@ -3534,7 +3838,8 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     y.Ob = off(y.Ob); y.Pp = off(y.Pp); y.stash = off(y.stash); y.gstate = off(y.gstate);
     y.Psi = off(y.Psi); y.coef = off(y.coef); y.gP = off(y.gP); y.pos_part = off(y.pos_part);
     y.bc_part = off(y.bc_part); y.plate = off(y.plate); y.ab_w = off(y.ab_w); y.ab_part = off(y.ab_part);
-    y.Wsp = off(y.Wsp); y.Xe = off(y.Xe); y.tilt_part = off(y.tilt_part);
+    y.Wsp = off(y.Wsp); y.Xe = off(y.Xe); y.tilt_part = off(y.tilt_part); y.Fw = off(y.Fw);
+    y.tb_part = off(y.tb_part);
     y.meas_stage = off(y.meas_stage); y.scratch = off(y.scratch);
   }
   if (nl > 1) {
@@ -3570,10 +3875,10 @@ int issue_pass(const pty_problem* p, const pty_buffers* bf, const Layout& L, int
     }));
     PTY_TRY(x.launch(KT_PROBE_FINAL, 0, a, [&](hipStream_t s) {
       if constexpr (LAY == LAY_RT)
-        k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, a.shift, a.maskc,
+        k_probe_final_rt<N><<<M, RT<N>::NT, 0, s>>>(psum, a.shift && !a.tilt_before, a.maskc,
                                                      static_cast<float2*>(bf->probe_grad));
       else
-        k_probe_final_gt<N><<<M, kGtThreads<N>, 0, s>>>(psum, a.shift, a.maskc,
+        k_probe_final_gt<N><<<M, kGtThreads<N>, 0, s>>>(psum, a.shift && !a.tilt_before, a.maskc,
                                                      static_cast<float2*>(bf->probe_grad), a.scratch);
     }));
   }
@@ -3735,6 +4040,7 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       break;
     case KT_TILT_FINAL:
       add(a.tilt_part, size_t(B) * MK * 4 * 8, false, WB_TILT_PART);
+      add(a.tb_part, size_t(B) * M * 2 * 8, false, WB_TB_PART);
       add(a.tilt_grad, size_t(p->n_tilts) * 24, true, kExt + PB_TILTS_GRAD);
       break;
     case KT_AB_TABLE:
@@ -3757,9 +4063,22 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       add(a.ab_grad, size_t(p->n_markers) * a.NA * 4, true, kExt + PB_AB_GRAD);
       break;
     case KT_FPROBE:
+      if (o.sub == SUB_PER_WINDOW) {
+        // the spectra of the windows' tilted probes
+        add(a.Pp, size_t(B) * M * c8, false, WB_PP);
+        add(a.Fw, size_t(B) * M * c8, true, WB_FW);
+        tiles(B * M);
+        break;
+      }
       add(bf->probe, size_t(M) * c8, false, kExt + PB_PROBE);
       shared(WB_FPROBE, true);
       tiles(M);
+      break;
+    case KT_TILT_PROBE:
+      add(bf->probe, size_t(M) * c8, false, kExt + PB_PROBE);
+      add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
+      add(a.tilt, size_t(p->n_tilts) * 24, false, kExt + PB_TILTS);
+      add(a.Pp, size_t(B) * M * c8, true, WB_PP);
       break;
     case KT_LAYOUT:
       if (o.sub == SUB_TO_PLANAR) {
@@ -3780,7 +4099,11 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       tiles(B * S * K);
       break;
     case KT_PROBE_SHIFT:
-      shared(WB_FPROBE, false); shared(WB_MASKC, false); shared(WB_ZCOL, false);
+      if (a.Fw)
+        add(a.Fw, size_t(B) * M * c8, false, WB_FW);
+      else
+        shared(WB_FPROBE, false);
+      shared(WB_MASKC, false); shared(WB_ZCOL, false);
       add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
       if (!a.moved) plates();
       add(a.Pp, size_t(B) * M * c8, true, WB_PP);
@@ -3853,6 +4176,20 @@ void op_accesses(const Op& o, const pty_problem* p, const pty_buffers* bf, const
       add(a.obj_grad, objbytes, true, p->obj_layout == PTY_LAYOUT_REFERENCE ? WB_GRADP : kExt + PB_OBJ_GRAD);
       break;
     case KT_PROBEGRAD:
+      if (a.tilt_before) {
+        add(a.gP, size_t(B) * MK * c8, false, WB_GP);
+        shared(WB_MASKC, false);
+        add(bf->probe, size_t(M) * c8, false, kExt + PB_PROBE);
+        add(a.bc, size_t(p->n_positions) * 4, false, kExt + PB_BC);
+        add(a.tilt, size_t(p->n_tilts) * 24, false, kExt + PB_TILTS);
+        add(a.Fw, size_t(B) * M * c8, false, WB_FW);
+        add(a.probe_part, size_t(a.G) * M * c8, true, WB_PROBE_PART);
+        if (a.step_pos && a.moved) add(a.pos_part, size_t(B) * M * 2 * 8, true, WB_POS_PART);
+        add(a.bc_part, size_t(B) * M * 8, true, WB_BC_PART);
+        add(a.tb_part, size_t(B) * M * 2 * 8, true, WB_TB_PART);
+        tiles(a.G * M);
+        break;
+      }
       add(a.gP, size_t(B) * MK * c8, false, WB_GP);
       shared(WB_FPROBE, false); shared(WB_MASKC, false);
       add(a.probe_part, size_t(a.G) * M * c8, true, WB_PROBE_PART);

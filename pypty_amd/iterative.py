@@ -10,7 +10,10 @@ background held in ``static_background`` (in every loss evaluation; refined
 with ``this_step_static_background``), the per-position beam current held
 in ``beam_current`` (likewise; refined with ``this_beam_current_step``) and
 the aberration rows held in ``aberrations_array`` with ``aberration_marker``
-(likewise; refined with ``this_step_aberrations_array``).
+(likewise; refined with ``this_step_aberrations_array``) and the tilts held in
+``tilts`` (refined through ``tilts_correction`` with ``this_step_tilts``).
+``keep_probe_states_orthogonal`` and ``do_charge_flip`` (without fancy_sigma)
+act as in iterative.py:497-505.
 Every large
 operation runs on the HIP device through this package's kernels:
 loss and gradients (``objective.loss_and_direction``), the band-limits
@@ -572,7 +575,9 @@ def run(pypty_params):
     extra_space_on_side_px, sequence, output_folder, save_inter_checkpoints,
     save_loss_log, print_flag, static_background and update_static_background,
     beam_current and update_beam_current, aberrations_array, aberration_marker and
-    update_aberrations_array,
+    update_aberrations_array, tilts, tilt_mode and update_tilts with the tilts'
+    fast_axis_reg_weight_tilts, deformation_reg_weight_tilts (and, as there, the slow-axis
+    weight read from slow_axis_reg_weight_positions),
     phase_norm_weight and abs_norm_weight (values or callables of the epoch), and two keys
     not in the reference:
     checkpoint_slices (keep every k-th slice's wave and recompute the rest in

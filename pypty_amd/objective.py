@@ -12,11 +12,14 @@ patterns in host memory, streaming of each batch's patterns through pinned
 buffers (objective.py:383-408), the per-position beam current and its
 gradient (objective.py:446-450, 688-698), and the per-position aberration
 phase plates and their gradient (objective.py:320-326, 459-465, 673-681).
-The constraint terms (objective.py:765-921) are added by
+Beam tilts before, inside and after the specimen with their gradients
+(objective.py:303-307, 328-350, 428-499, 600-672; multislice_core.py:163-178)
+and per-slice distances (multislice_core.py:69-71, 164-168) run in the same
+kernels.  The constraint terms (objective.py:765-921) are added by
 ``pypty_amd.constraints`` on the device.  Arguments that select features
-outside that path (near field, tilts, phase plates from a file, probe
-markers, per-slice distances, upsampling) raise ``NotImplementedError``
-instead of silently computing something else.
+outside that path (near field, phase plates from a file, probe markers,
+upsampling, tilts before the specimen together with aberrations) raise
+``NotImplementedError`` instead of silently computing something else.
 
 Binning follows the evident intent of utils.py:639-644 (sum of the
 data_bin x data_bin blocks); the reference's loop indexes an undefined name
@@ -449,7 +452,8 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
     _unsupported("recon_type", recon_type != "far_field")
     _unsupported("propmethod", propmethod != "multislice")
     _unsupported("algorithm_type", algorithm_type not in LOSSES)
-    _unsupported("tilts before the specimen (tilt_mode 2 and 4)", int(tilt_mode) in (2, 4))
+    _unsupported("tilts before the specimen (tilt_mode 2 and 4) with aberrations",
+                 int(tilt_mode) in (2, 4) and aberration_marker is not None and aberrations_array is not None)
     _unsupported("static_background with a compressed loss",
                  static_background is not None and "compressed" in str(algorithm_type))
     _unsupported("probe_marker", probe_marker is not None)

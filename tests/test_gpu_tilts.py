@@ -1,6 +1,6 @@
-"""Beam tilts and per-slice distances on the GPU (pypty/objective.py:328-350, 428-499, 600-636;
+"""Beam tilts and per-slice distances on the GPU (pypty/objective.py:328-350, 428-499, 600-672;
 pypty/multislice_core.py:69-77, 163-178): the drop-in loss_and_direction with a tilt array (one row or
-one per position), tilt_mode 0, 1 and 3, with and without tilt refinement, and with per-slice
+one per position), tilt_mode 0 to 4, with and without tilt refinement, with a beam current, and with per-slice
 distances, gives the oracle's loss, object, probe, position and tilt gradients; on the register tile
 and both slab tiles; one slice (the exit wave band-limited before the after tilt) and several; an edge
 multiplier with per-window tilts (each window raises its own tilted propagator); several batches over
@@ -32,6 +32,17 @@ CASES = [
     (192, 1, 1, 1, True, True, False, None, 1.0, {}),
     (192, 3, 1, 0, True, True, False, [1.0, 2.5], 1.3, {}),
     (256, 2, 1, 3, True, True, True, None, 1.0, {}),
+    # tilts before the specimen
+    (32, 2, 2, 2, True, True, True, None, 1.0, {}),
+    (64, 3, 1, 2, True, True, False, None, 1.0, {}),
+    (64, 3, 2, 4, True, True, True, None, 1.0, {"bc": True}),
+    (128, 2, 2, 4, False, True, True, None, 1.0, {}),
+    (96, 1, 2, 4, True, True, False, None, 1.0, {"bc": True}),
+    (64, 2, 2, 2, True, False, True, None, 1.0, {}),
+    (64, 3, 2, 4, True, True, True, [1.5, 2.5], 1.0, {"streamed": True}),
+    (192, 2, 2, 4, True, True, True, None, 1.0, {"bc": True}),
+    (192, 2, 1, 2, True, True, False, None, 1.0, {}),
+    (256, 2, 2, 2, False, True, True, None, 1.0, {}),
 ]
 
 
@@ -53,14 +64,17 @@ def test_tilt_pass_matches_oracle(n, S, M, mode, per_pos, step, shift, dists, ed
     pr = make_problem(n=n, S=S, K=1, M=M, ny=n + 40, nx=n + 40, P=P, shift=shift, seed=901 + n + S + mode)
     tl = _tilts(P if per_pos else 1, 13 + mode)
     dist = np.asarray(dists if dists is not None else [pr["dz"]], np.float32)
+    bc = np.random.default_rng(5).uniform(0.7, 1.3, P).astype(np.float32) if extra.get("bc") else None
     args = dict(dz=pr["dz"], distances=dist.astype(np.float64), wavelength=pr["wavelength"],
                 pixel_size_x=pr["px"], pixel_size_y=pr["px"], step_pos=shift, compute_batch=4, loss_weight=0.7,
-                edge=edge, tilts=tl, tilt_mode=mode, step_tilts=step)
+                edge=edge, tilts=tl, tilt_mode=mode, step_tilts=step, beam_current=bc, step_beam_current=bc is not None)
     ref = orc.loss_and_direction(pr["obj"], pr["probe"], pr["pos"], pr["corr"], pr["meas"], pr["seq"],
                                  cdtype=np.complex64, **args)
     ref64 = orc.loss_and_direction(pr["obj"], pr["probe"], pr["pos"], pr["corr"], pr["meas"], pr["seq"],
                                    cdtype=np.complex128, **args)
-    cols = {0: [2, 3], 1: [4, 5], 3: [2, 3, 4, 5]}[mode] if step else []
+    cols = {0: [2, 3], 1: [4, 5], 2: [0, 1], 3: [2, 3, 4, 5], 4: [0, 1, 2, 3, 4, 5]}[mode] if step else []
+    if S == 1:
+        cols = [c for c in cols if c not in (2, 3)]  # one slice: no propagation for the inside tilt to act on
     assert max(rel(ref[k], ref64[k]) for k in (2, 3)) < TOL / 3  # well conditioned
     for c in cols:
         assert rel(ref[-1][:, c], ref64[-1][:, c]) < TOL / 3, c
@@ -75,12 +89,14 @@ def test_tilt_pass_matches_oracle(n, S, M, mode, per_pos, step, shift, dists, ed
         this_step_pos_correction=int(shift), this_step_tilts=int(step), pixel_size_x_A=pr["px"],
         pixel_size_y_A=pr["px"], this_chopped_sequence=pr["seq"], compute_batch=4, this_loss_weight=0.7,
         this_tilt_array=tl, this_tilts_correction=np.zeros_like(tl), tilt_mode=mode,
-        checkpoint_slices=int(extra.get("checkpoint", 0)))
+        checkpoint_slices=int(extra.get("checkpoint", 0)), beam_current=bc, this_beam_current_step=int(bc is not None))
     out = call_loss_and_direction(objective.loss_and_direction, **kw)
     errs = dict(loss=abs(out[0] - ref[0]) / abs(ref[0]), obj=rel(out[2].cpu().numpy(), ref[2]),
                 probe=rel(out[3].cpu().numpy(), ref[3]))
     if shift:
         errs["pos"] = rel(out[4].cpu().numpy(), ref[4])
+    if bc is not None:
+        errs["beam current"] = rel(out[8].cpu().numpy(), ref[5])
     got = out[5].cpu().numpy()
     assert got.shape == tl.shape
     for c in cols:

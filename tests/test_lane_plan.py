@@ -108,6 +108,8 @@ def test_planted_faults_are_reported(omit, what):
     dict(tilt_mode=0, n_tilts=10, step_tilts=True), dict(tilt_mode=3, n_tilts=10, step_tilts=True),
     dict(tilt_mode=1, n_tilts=1, step_tilts=True), dict(tilt_mode=0, n_tilts=1, step_tilts=False),
     dict(tilt_mode=3, n_tilts=10, step_tilts=True, n_distances=4),
+    dict(tilt_mode=4, n_tilts=10, step_tilts=True, step_beam_current=True),
+    dict(tilt_mode=2, n_tilts=1, step_tilts=True, shift_active=False, step_pos=False),
 ])
 def test_tilts_and_per_slice_distances(n, S, ck, tilt):
     """The tilt buffers (per-lane spectra, exit waves and partials; the tilt gradient, added into by
