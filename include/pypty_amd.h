@@ -349,6 +349,10 @@ int pty_charge_flip_c64(void* obj, int64_t n, int planes, int layout, float delt
  * p_i -= p_j sum(conj(p_j) p_i) / (1e-10 + sum |p_j|^2), FP64 sums. */
 int pty_orthogonalize_modes_c64(void* probe, int64_t npix, int modes, void* stream);
 
+/* A line-search step of a phase-only object (pypty/iterative.py:737-738):
+ * out = |obj| exp(i (arg obj + step u)) over n complex64, u float32 */
+int pty_phase_step_c64(const void* obj, const float* u, float step, void* out, int64_t n, void* stream);
+
 /* out = 2 Re(-i conj(z) g): Wirtinger gradient -> phase gradient
  * (pypty/utils.py:260-276, complex_grad_to_phase_grad); out float32, n elements. */
 int pty_phase_grad_c64(const void* g, const void* z, float* out, int64_t n, void* stream);

@@ -39,7 +39,7 @@ def _d(a, b):
 def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_pos=False, actual_step=1.0,
                cutoff=2 / 3, update_extra_cut=0.005, wolfe_c1=0.1, wolfe_c2=0.9, max_count=20, reduce_factor=0.1,
                optimism=3, min_step=1e-20, hist_length=10, cdtype=np.complex64, update_bg=False, update_bc=False,
-               update_ab=False, update_tilts=False, charge_flip=None, orthogonal=False):
+               update_ab=False, update_tilts=False, charge_flip=None, orthogonal=False, phase_only=False):
     """One epoch; mutates state and hist; returns (loss, sse, actual_step, count, d_value, new_d_value, accepted).
 
     A static background is state["bg"] (N, N) when present; it enters every loss evaluation and,
@@ -54,7 +54,10 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
     refined as the fourth block, after the positions (iterative.py:477, 568, 588-670, 687, 768,
     852-868, 892-897, 1027).  orthogonal makes the probe modes orthogonal before the probe is
     band-limited (iterative.py:498-499); charge_flip = (delta_phase, delta_abs, beta_phase, beta_abs)
-    flips the object and restarts the history before the object is band-limited (:502-505)."""
+    flips the object and restarts the history before the object is band-limited (:502-505).
+    phase_only refines the object's phase: its block is the real phase gradient 2 Re(-i conj(O) g)
+    of the band-limited gradient (:549-550, 798-799), the update is not band-limited (:674) and a
+    step makes |O| exp(i (arg O + t u)), band-limited (:737-746)."""
     fd = np.float32 if np.dtype(cdtype) == np.complex64 else np.float64
     cut = float(fd(cutoff) - fd(update_extra_cut))
     bg0 = state.get("bg", data.get("static_background"))
@@ -119,6 +122,8 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
             pg = clean(pg)
         if update_obj:
             og = clean(og)
+            if phase_only:
+                og = orc.complex_grad_to_phase_grad(og, o).astype(fd)
         # objective.py:376-378: zeros of the background's (current's) shape when it is not refined
         extra = list(out[5:])
         bgg = extra.pop(0) if b is not None else None
@@ -186,7 +191,7 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
                 if q[j] is not None:
                     q[j] = q[j] + fd(beta) * hist[key][i]
         u = [None if v is None else -v for v in q]
-        if update_obj:
+        if update_obj and not phase_only:
             u[0] = clean(u[0])
     d_value = ddot(u, g)
     if d_value >= 0:
@@ -199,7 +204,13 @@ def bfgs_epoch(state, hist, data, *, update_obj=True, update_probe=True, update_
     count = -1
     while True:
         count += 1
-        nx = [clean(x[0] + fd(actual_step) * u[0]) if update_obj else x[0],
+        if update_obj and phase_only:
+            o_new = clean((np.abs(x[0]) * np.exp(1j * (np.angle(x[0]) + fd(actual_step) * u[0]))).astype(cdtype))
+        elif update_obj:
+            o_new = clean(x[0] + fd(actual_step) * u[0])
+        else:
+            o_new = x[0]
+        nx = [o_new,
               x[1] + fd(actual_step) * u[1] if update_probe else x[1],
               (x[2] + fd(actual_step) * u[2]).astype(fd) if update_pos else x[2],
               (x[3] + fd(actual_step) * u[3]).astype(fd) if update_tilts else x[3],

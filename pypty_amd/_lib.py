@@ -28,7 +28,7 @@ EXPORTED = (
     "pty_scatteradd_c64", "pty_workspace_bytes", "pty_loss_and_direction", "pty_tables_bytes",
     "pty_ms_forward", "pty_ms_adjoint", "pty_timing_enable", "pty_timing_read", "pty_timing_reset",
     "pty_fourier_clean_c64", "pty_plane_fft2_c64", "pty_multidot_c64", "pty_axpby_c64", "pty_axpby_f32",
-    "pty_phase_grad_c64", "pty_charge_flip_c64", "pty_orthogonalize_modes_c64",
+    "pty_phase_grad_c64", "pty_charge_flip_c64", "pty_orthogonalize_modes_c64", "pty_phase_step_c64",
     "pty_l1_constraint_c64", "pty_plan_check", "pty_loss_and_direction_static", "pty_center_of_mass",
     "pty_gemm_bf16", "pty_cnn_im2col", "pty_cnn_col2im", "pty_cnn_bias_act", "pty_cnn_act_grad",
     "pty_cnn_bias_grad", "pty_cnn_to_bf16", "pty_cnn_mse", "pty_cnn_adam",
@@ -129,6 +129,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
     lib.pty_charge_flip_c64.argtypes = [vp, i64, i, i, f, f, f, f, vp, vp]
     lib.pty_orthogonalize_modes_c64.restype = i
     lib.pty_orthogonalize_modes_c64.argtypes = [vp, i64, i, vp]
+    lib.pty_phase_step_c64.restype = i
+    lib.pty_phase_step_c64.argtypes = [vp, vp, f, vp, i64, vp]
     lib.pty_center_of_mass.restype = i
     lib.pty_center_of_mass.argtypes = [vp, i64, i, i, i, vp, vp, vp]
     lib.pty_l1_constraint_c64.restype = i

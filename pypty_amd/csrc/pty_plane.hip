@@ -530,6 +530,19 @@ __global__ __launch_bounds__(1024) void k_gram_schmidt(float2* __restrict__ p, i
   }
 }
 
+// A step of the phase of a phase-only object (iterative.py:737-738):
+// out = |O| exp(i (arg O + step u)), u the real phase update
+__global__ __launch_bounds__(256) void k_phase_step(const float2* __restrict__ obj, const float* __restrict__ u,
+                                                    float step, float2* __restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const float2 o = obj[i];
+    const float r = hypotf(o.x, o.y), ph = atan2f(o.y, o.x) + step * u[i];
+    float sn, cs;
+    sincosf(ph, &sn, &cs);
+    out[i] = make_float2(r * cs, r * sn);
+  }
+}
+
 // ================================================================ host
 namespace {
 
@@ -719,6 +732,15 @@ int pty_orthogonalize_modes_c64(void* probe, int64_t npix, int modes, void* stre
   if (!probe || npix < 0 || modes < 1) return PTY_ERR_ARG;
   if (modes == 1 || npix == 0) return PTY_OK;
   k_gram_schmidt<<<1, 1024, 0, static_cast<hipStream_t>(stream)>>>(static_cast<float2*>(probe), npix, modes);
+  return plane_launch_check();
+}
+
+int pty_phase_step_c64(const void* obj, const float* u, float step, void* out, int64_t n, void* stream) {
+  if (!obj || !u || !out || n < 0) return PTY_ERR_ARG;
+  if (n == 0) return PTY_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int grid = int(std::min<size_t>((size_t(n) + 255) / 256, 8192));
+  k_phase_step<<<grid, 256, 0, s>>>(static_cast<const float2*>(obj), u, step, static_cast<float2*>(out), size_t(n));
   return plane_launch_check();
 }
 

@@ -29,7 +29,7 @@ import sys
 import numpy as np
 import torch
 
-from . import _arrays, lbfgs, objective, utils
+from . import _arrays, _lib, lbfgs, objective, utils
 
 obj = None
 probe = None
@@ -79,6 +79,15 @@ def _unsupported(name, cond):
         raise NotImplementedError(f"pypty_amd.bfgs_update: '{name}' is outside the accelerated path")
 
 
+def _phase_step(o, u, step):
+    """|o| exp(i (arg o + step u)): a step of a phase-only object (pypty/iterative.py:737-738)."""
+    out = torch.empty_like(o)
+    rc = _lib.load().pty_phase_step_c64(o.data_ptr(), u.contiguous().data_ptr(), float(step), out.data_ptr(),
+                                        o.numel(), _lib.stream_ptr(o.device))
+    _lib.check(rc, "phase_step")
+    return out
+
+
 def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multiplier, this_step_probe, this_step_obj,
                 this_step_pos_correction, this_step_tilts, measured_array, this_wavelength, masks, pixel_size_x_A,
                 pixel_size_y_A, phase_norm_weight, abs_norm_weight, stepsize_threshold_low, probe_reg_weight,
@@ -108,7 +117,7 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     """
     global obj, probe, positions_correction, tilts, tilts_correction, static_background, beam_current
     global aberrations_array, history_bfgs
-    _unsupported("phase_only_obj", bool(phase_only_obj))
+    phase_only = bool(phase_only_obj)
     _unsupported("tune_only_probe_phase", bool(tune_only_probe_phase))
     _unsupported("tune_only_probe_abs", bool(tune_only_probe_abs))
     _unsupported("charge flipping with fancy_sigma", bool(do_charge_flip) and fancy_sigma is not None)
@@ -199,6 +208,9 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
             pg = _clean(pg, cut, smooth_rolloff)
         if clean and update_obj:
             og = _clean(og, cut, smooth_rolloff)
+        if phase_only and update_obj:
+            # a phase-only object is refined in its phase, a real block (iterative.py:549-550, 798-799)
+            og = utils.complex_grad_to_phase_grad(og.contiguous(), o.contiguous())
         # an unrefined background has a zero gradient, as objective.py:376-377 returns it, so a history
         # entry stored in such an epoch serves an epoch that starts refining it without a reset
         bgg = out[6] if update_bg else (torch.zeros_like(b) if isinstance(b, torch.Tensor) else 0)
@@ -287,7 +299,7 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
                 if on[j] and isinstance(history_bfgs[key][i], torch.Tensor):
                     lbfgs.axpby(q[j], history_bfgs[key][i], 1.0, beta)
         u = [lbfgs.axpby(q[j], None, -1.0, 0.0) if on[j] else None for j in range(NB)]
-        if clean and update_obj:
+        if clean and update_obj and not phase_only:
             u[0] = _clean(u[0], cut, smooth_rolloff)
     d_value = direction_dot(u, g)
     if d_value >= 0:
@@ -307,7 +319,10 @@ def bfgs_update(algorithm_type, this_slice_distances, edge_slice_thickness_multi
     while True:
         count += 1
         if update_obj:
-            new_obj = lbfgs.axpby(obj.clone(), u_obj, 1.0, actual_step)
+            if phase_only:
+                new_obj = _phase_step(obj, u_obj, actual_step)  # iterative.py:737-738
+            else:
+                new_obj = lbfgs.axpby(obj.clone(), u_obj, 1.0, actual_step)
             if clean:
                 new_obj = _clean(new_obj, cut, smooth_rolloff)
         else:
@@ -737,7 +752,8 @@ def run(pypty_params):
             ("deformation_reg_weight_positions", 0), ("probe_reg_constraint_weight", 0), ("window_weight", 0),
             ("atv_weight", 0), ("mixed_variance_weight", 0), ("mixed_variance_sigma", 0.5), ("wedge_mu", 0),
             ("beta_wedge", 0), ("probe_mod_reg_weight", 0), ("probe_mod_tol", 0.05),
-            ("fast_axis_reg_weight_tilts", 0), ("deformation_reg_weight_tilts", 0))}
+            ("fast_axis_reg_weight_tilts", 0), ("deformation_reg_weight_tilts", 0), ("phase_only_obj", False),
+            ("keep_probe_states_orthogonal", False), ("do_charge_flip", False))}
         # the reference reads the tilts' slow-axis weight from the positions' key (iterative.py:208)
         cw["slow_axis_reg_weight_tilts"] = _per_epoch(get("slow_axis_reg_weight_positions", 0), epoch)
         window = get("window", None)
@@ -768,9 +784,11 @@ def run(pypty_params):
             stepsize_threshold_low=get("min_step", 1e-20), probe_reg_weight=cw["probe_reg_constraint_weight"], aperture_mask=get("aperture_mask", None),
             recon_type="far_field", defocus_array=np.array([0.0]), Cs=0.0, alpha_near_field=0.0,
             damping_cutoff_multislice=cutoff, smooth_rolloff=rolloff,
-            update_extra_cut=np.float32(get("update_extra_cut", 0.005)), keep_probe_states_orthogonal=False,
-            do_charge_flip=False, cf_delta_phase=0.1, cf_delta_abs=0.01, cf_beta_phase=-0.95, cf_beta_abs=-0.95,
-            phase_only_obj=False, wedge_mu=cw["wedge_mu"], beta_wedge=cw["beta_wedge"],
+            update_extra_cut=np.float32(get("update_extra_cut", 0.005)),
+            keep_probe_states_orthogonal=cw["keep_probe_states_orthogonal"], do_charge_flip=cw["do_charge_flip"],
+            cf_delta_phase=get("cf_delta_phase", 0.1), cf_delta_abs=get("cf_delta_abs", 0.01),
+            cf_beta_phase=get("cf_beta_phase", -0.95), cf_beta_abs=get("cf_beta_abs", -0.95),
+            phase_only_obj=cw["phase_only_obj"], wedge_mu=cw["wedge_mu"], beta_wedge=cw["beta_wedge"],
             wolfe_c1_constant=_per_epoch(get("wolfe_c1_constant", 0.1), epoch),
             wolfe_c2_constant=_per_epoch(get("wolfe_c2_constant", 0.9), epoch), atv_weight=cw["atv_weight"], atv_q=get("atv_q", 1), atv_p=get("atv_p", 2),
             tune_only_probe_phase=False, propmethod="multislice", this_chopped_sequence=seq, load_one_by_one=False,
@@ -781,7 +799,7 @@ def run(pypty_params):
             data_shift_vector=[0, 0], smart_memory=False, default_float=np.float32, default_complex=np.complex64,
             default_int=np.int32, upsample_pattern=1,
             this_step_static_background=s_bg, tilt_mode=tilt_mode,
-            fancy_sigma=None, tune_only_probe_abs=False, aberration_marker=aberration_marker,
+            fancy_sigma=get("fancy_sigma", None), tune_only_probe_abs=False, aberration_marker=aberration_marker,
             this_step_aberrations_array=s_ab,
             probe_marker=None, compute_batch=get("compute_batch", 256), current_window=this_window,
             current_window_weight=cw["window_weight"], dynamically_resize_yx_object=False, lazy_clean=False,

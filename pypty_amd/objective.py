@@ -458,7 +458,6 @@ def loss_and_direction(this_obj, full_probe, this_pos_array, this_pos_correction
                  static_background is not None and "compressed" in str(algorithm_type))
     _unsupported("probe_marker", probe_marker is not None)
     _unsupported("phase_plate_in_h5", phase_plate_in_h5 is not None)
-    _unsupported("phase_only_obj", bool(phase_only_obj))
     _unsupported("upsample_pattern", upsample_pattern != 1)
     _unsupported("data_pad without use_padded_intensity", int(data_pad) != 0 and not use_padded_intensity)
     _unsupported("data_simulation_flag", bool(data_simulation_flag))

@@ -490,3 +490,44 @@ def test_epochs_with_charge_flip_and_orthogonal_modes_match_oracle():
         assert rg[4] == ro[3]
         assert abs(rg[3] - ro[2]) <= 1e-5 * abs(ro[2])
         assert e_obj < 1e-4 and e_probe < 1e-4
+
+
+def test_epochs_with_a_phase_only_object_match_oracle():
+    """phase_only_obj (iterative.py:549-550, 563, 674, 737-738, 798-799): the object is refined in its
+    phase (a real block of 2 Re(-i conj(O) g)), each trial object |O| exp(i (arg O + t u)) then
+    band-limited; losses, line-search counts and the object and probe follow the oracle, and the
+    object's modulus stays that of the start up to the band limit."""
+    import torch
+
+    from pypty_amd import iterative
+
+    print("[bfgs] 3 epochs with a phase-only object", flush=True)
+    pr = make_problem(n=32, S=2, K=1, M=2, ny=64, nx=64, P=10, shift=False, seed=67, noise=0.02)
+    dev = torch.device("cuda", 0)
+    state = {"obj": pr["obj"].copy(), "probe": pr["probe"].copy(), "positions": pr["pos"], "corr": pr["corr"].copy()}
+    hist = bfgs_oracle.new_history()
+    data = {"measured": pr["meas"], "sequence": pr["seq"], "dz": pr["dz"], "wavelength": pr["wavelength"],
+            "px": pr["px"], "algorithm": pr["algorithm"], "compute_batch": 5}
+    iterative.obj = torch.from_numpy(pr["obj"].copy()).to(dev)
+    iterative.probe = torch.from_numpy(pr["probe"].copy()).to(dev)
+    iterative.positions = torch.from_numpy(pr["pos"]).to(dev)
+    iterative.positions_correction = torch.from_numpy(pr["corr"].copy()).to(dev)
+    iterative.tilts = np.zeros((1, 6), np.float32)
+    iterative.tilts_correction = np.zeros((1, 6), np.float32)
+    iterative.reset_bfgs_history()
+    step_o = step_g = 1.0
+    for epoch in range(3):
+        np.random.seed(700 + epoch)
+        ro = bfgs_oracle.bfgs_epoch(state, hist, data, update_pos=False, actual_step=step_o, phase_only=True)
+        np.random.seed(700 + epoch)
+        rg = call_loss_and_direction(iterative.bfgs_update, **bfgs_kwargs(
+            pr, actual_step=step_g, is_first_epoch=(epoch == 0), phase_only_obj=True))
+        step_o, step_g = ro[2], rg[3]
+        e_obj = rel(iterative.obj.cpu().numpy(), state["obj"])
+        e_probe = rel(iterative.probe.cpu().numpy(), state["probe"])
+        print(f"  epoch {epoch}: loss gpu {rg[0]:.6e} oracle {ro[0]:.6e}; step {rg[3]:.3e}/{ro[2]:.3e}; "
+              f"count {rg[4]}/{ro[3]}; rel err obj {e_obj:.2e} probe {e_probe:.2e}", flush=True)
+        assert abs(rg[0] - ro[0]) / abs(ro[0]) < 1e-4
+        assert rg[4] == ro[3]
+        assert abs(rg[3] - ro[2]) <= 1e-5 * abs(ro[2])
+        assert e_obj < 1e-4 and e_probe < 1e-4
