@@ -347,9 +347,11 @@ def build_workload(name, dev, args):
         tune, graph = False, False
     else:
         datasets = [synth.config4(dev, n_scan=args.thick_scan)]
-        kw = dict(batch=512, streams=1, lanes=4, checkpoint=8)
+        kw = dict(batch=512, streams=1, lanes=2, checkpoint=8)
         tune, graph = False, False
     t_gen = time.perf_counter() - t0
+    if getattr(args, "eager", False):
+        graph = False
     step = ragged.RaggedStep(datasets, dev, **kw)
     if args.batches:
         # windows per launch given per dataset (e.g. a tuned choice, to skip the tuner in profiled runs)
@@ -489,6 +491,9 @@ def main(argv=None):
     ap.add_argument("--batches", default="",
                     help="comma-separated windows per launch of each dataset of the headline workload "
                          "(instead of the autotuner)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the headline steps eagerly instead of replaying a captured HIP graph "
+                         "(for counter profiles)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for rocprofv3 counter scaling)")
     ap.add_argument("--recon-epochs", type=int, default=20,

@@ -143,7 +143,9 @@ typedef struct pty_problem {
    * the lanes), so the results are bit-identical to lanes = 1.  The call
    * returns with all work joined to the caller's stream.  A call on a
    * stream that is being captured runs one lane.  0 or 1: one stream.  The
-   * workspace holds one set of per-batch buffers per lane. */
+   * workspace holds one set of per-batch buffers per lane; a problem whose
+   * per-lane buffers would take more than 128 GiB over all lanes runs fewer
+   * lanes (pty_workspace_bytes reports the size for the lanes it runs). */
   int lanes;
   /* 1: also return the gradient of the static background
    * (objective.py:588-589, this_step_static_background); needs

@@ -3269,16 +3269,20 @@ int loss_chunks(int n) {
 
 // windows handled by one launch; windows larger than 128 px keep one tile per
 // workgroup in global scratch, so their batch is capped to bound that buffer
+// (12 GiB per lane: 512 windows of 256 px with 40 slices fit)
 int batch_windows(const pty_problem* p) {
   int B = p->compute_batch;
   if (p->n > 128) {
     const size_t tile = size_t(p->n) * p->n * 8;
     const size_t per_window = size_t(std::max(p->n_slices * p->n_obj_modes, p->n_probe_modes * p->n_obj_modes)) * tile;
-    const int cap = int(std::max<size_t>(8, (size_t(2) << 30) / per_window));
+    const int cap = int(std::max<size_t>(8, (size_t(12) << 30) / per_window));
     B = std::min(B, cap);
   }
   return B;
 }
+// the workspace of all lanes stays within this many bytes: a problem whose
+// per-lane buffers are larger runs fewer lanes (at least one)
+constexpr size_t kLaneBudget = size_t(128) << 30;
 
 // slices per checkpoint segment, or 0 when every slice's wave is kept (the
 // stash feeds only the object gradient)
@@ -3383,7 +3387,7 @@ Layout plan_layout(const pty_problem* p) {
                                : 0;
   take(WB_SCRATCH, size_t(L.scratch_tiles) * n2 * 8);
   L.lane_bytes = o - L.lane0;
-  L.lanes = std::max(1, p->lanes);
+  L.lanes = std::max(1, std::min(p->lanes, int(std::max<size_t>(1, kLaneBudget / std::max<size_t>(L.lane_bytes, 1)))));
   L.total = L.lane0 + size_t(L.lanes) * L.lane_bytes;
   L.B = B;
   L.G = G;

@@ -108,8 +108,9 @@ def test_checkpoint_segments_shrink_the_workspace(lib):
     rc, ck8 = ws(8)
     assert rc == 0
     # one wave slot for every window and mode of a launch; windows larger than 128 px cap the
-    # launch so that one tile per workgroup fits in 2 GiB
-    windows = min(512, max(8, (2 << 30) // (40 * 256 * 256 * 8)))
+    # launch so that one tile per workgroup fits in 12 GiB (all 512 windows here)
+    windows = min(512, max(8, (12 << 30) // (40 * 256 * 256 * 8)))
+    assert windows == 512
     slot = windows * 10 * 256 * 256 * 8
     assert full - ck8 == (40 - 13) * slot
     assert ws(40)[1] == full and ws(0, step_obj=0)[1] == ws(8, step_obj=0)[1]  # inactive cases keep every slice
@@ -171,3 +172,23 @@ def test_struct_layout_matches_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for field, _ in cls._fields_:
             assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
+
+
+def test_lanes_are_limited_by_the_workspace_budget(lib):
+    """Lanes whose per-batch buffers would take more than 128 GiB together are not planned: the thick
+    sample at 512 windows keeping every slice (about 128 GB a lane) runs one lane whatever is asked,
+    and with checkpoints every 8 slices (about 61 GB a lane) two."""
+    from pypty_amd import _lib
+
+    def ws(lanes, ck):
+        p = _lib.PtyProblem()
+        p.n, p.n_slices, p.n_obj_modes, p.n_probe_modes = 256, 40, 1, 10
+        p.obj_ny = p.obj_nx = 1536
+        p.compute_batch, p.n_positions, p.obj_layout, p.lanes = 512, 16384, 1, lanes
+        p.step_obj, p.checkpoint_slices = 1, ck
+        size = ctypes.c_size_t(0)
+        assert lib.pty_workspace_bytes(ctypes.byref(p), ctypes.byref(size)) == 0
+        return size.value
+
+    assert ws(4, 0) == ws(1, 0)
+    assert ws(4, 8) == ws(2, 8) > ws(1, 8)
