@@ -1526,7 +1526,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probe_shift_rt(BatchArgs a) {
 // accumulated (C-order) into the group's partial, with the position gradient
 // -2/n^2 sum Re(K conj(FP) Q) in FP64; without a shift the natural-order
 // gradient itself is accumulated (R-order)
-template <int N, bool AB>
+template <int N, bool AB, bool SHIFT>
 __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
   using R = RT<N>;
   rt_tw_load<N>();
@@ -1558,7 +1558,7 @@ __global__ __launch_bounds__(RT<N>::NT) void k_probegrad_rt(BatchArgs a) {
       for (int n = 0; n < R::R1; ++n) v[n] = cadd(v[n], gk[n]);
     }
     const float bcs = bc_scale(a, idx);
-    if (!a.shift) {
+    if constexpr (!SHIFT) {
       // natural order: register n of the row phase is R-order slot n*NT + tid,
       // element (row, rj + T*n)
       double sb = 0.0;
@@ -2290,8 +2290,10 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
       if (a.S == 1) {
         sl.to_cols(X, L);
         gt_band_columns<N>(v, X, sl, L, a.maskc, a.zcol);
+        sl.to_rows(X, L);
+      } else {
+        __syncthreads();  // the row spectra stay in rows; tt may be rewritten below
       }
-      sl.to_rows(X, L);
     }
   }
   if ((TILT && a.tilt_win) && !a.dists) {
@@ -2362,13 +2364,20 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
           gt_zero_col<N>(v, sl, L, r);
           continue;
         }
+        // the propagator is read before the column transform, except at
+        // 192 px (168 registers) and where the tilt sums come first
+        constexpr bool kHEarly = N != 192;
         float2 h[G::R1];
-        if (!wsp) {
+        if (kHEarly && !wsp) {
 #pragma unroll
           for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
         }
         gt_load_col<N>(v, sl, L, r);
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        if (!kHEarly && !wsp) {
+#pragma unroll
+          for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
+        }
         if (wsp) {
           // sum Im(conj(fft2 u_s) fft2(g_s) q) with Wsp = fft2(u_s) / N^2
           // (multislice_core.py:169-178)
@@ -3672,10 +3681,14 @@ int issue_batch(const BatchArgs& a, const pty_problem* p, int ck, int lane, X& x
   else if (p->step_probe || (p->step_pos && a.shift) || p->step_beam_current || p->step_aberrations)
     PTY_TRY(x.launch(KT_PROBEGRAD, lane, a, [&](hipStream_t s) {
       if constexpr (LAY == LAY_RT) {
-        if (a.NA)
-          k_probegrad_rt<N, true><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+        // the unshifted case (no plates) as its own instantiation: its beam-current
+        // sums beside the shifted case's in one kernel spilled
+        if (!a.shift)
+          k_probegrad_rt<N, false, false><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+        else if (a.NA)
+          k_probegrad_rt<N, true, true><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
         else
-          k_probegrad_rt<N, false><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
+          k_probegrad_rt<N, false, true><<<a.G * a.M, RT<N>::NT, 0, s>>>(a);
       } else {
         if (a.NA)
           k_probegrad_gt<N, true><<<a.G * a.M, kGtThreads<N>, 0, s>>>(a);
