@@ -45,13 +45,15 @@ def shape_key(device, n, slices, obj_modes, probe_modes, positions) -> str:
     return f"{name}|n{n}|s{slices}|k{obj_modes}|m{probe_modes}|p{positions}"
 
 
-def choose_batch(key: str, candidates, make_pass, reps: int = 2):
-    """Return the fastest candidate for ``key``.  ``make_pass(batch)`` returns a callable that
-    launches one full pass on the current stream; the timing is taken with HIP events."""
+def choose_batch(key: str, candidates, make_pass, reps: int = 2, tie: float = 0.02):
+    """Return the fastest candidate for ``key``; a larger candidate within ``tie`` (relative) of
+    the fastest time wins, so that timing noise between near-equal candidates does not change
+    the choice from run to run (fewer, larger launches).  ``make_pass(batch)`` returns a callable
+    that launches one full pass on the current stream; the timing is taken with HIP events."""
     _load()
     if key in _cache:
         return int(_cache[key])
-    best, best_ms = None, float("inf")
+    times = {}
     for b in candidates:
         run = make_pass(int(b))
         run()  # warm-up: workspace and tables
@@ -61,9 +63,9 @@ def choose_batch(key: str, candidates, make_pass, reps: int = 2):
             run()
         stop.record()
         stop.synchronize()
-        ms = start.elapsed_time(stop) / reps
-        if ms < best_ms:
-            best, best_ms = int(b), ms
+        times[int(b)] = start.elapsed_time(stop) / reps
+    fastest = min(times.values())
+    best = max(b for b, ms in times.items() if ms <= fastest * (1.0 + tie))
     _cache[key] = best
     _save()
     return best
