@@ -1844,7 +1844,45 @@ struct GSlab {
   }
   __device__ __forceinline__ void to_cols(float2*, const GTLane<N>&) { __syncthreads(); }
   __device__ __forceinline__ void to_rows(float2*, const GTLane<N>&) { __syncthreads(); }
+  // Columns of the band mask's zero set are exactly zero after a propagation
+  // (H and H0 vanish on them).  The slab's columns fall into groups of eight
+  // (8g .. 8g+7): one column-phase wavefront of a round, one register of the
+  // row phase.  A propagation's column phase does not store a group whose
+  // columns are all zero, and the next row phase does not load it (bit q of
+  // zg, the same for every lane, for register q) but sets it to zero.
+  __device__ __forceinline__ void load_rowspec_z(float2* v, const GTLane<N>& L, int r, unsigned zg) const {
+#pragma unroll
+    for (int c = 0; c < G::CPT; ++c)
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) {
+        const int q = c * 8 + k2;
+        if ((zg >> q) & 1u)
+          v[q] = make_float2(0.f, 0.f);
+        else
+          v[q] = sl.ld(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8);
+      }
+  }
 };
+
+// bit c*8 + k2: every column of the group that row-phase register c*8 + k2
+// reads (8g .. 8g+7, g = CPT*k2 + c) is a zero column of the band mask; the
+// same for every lane and round
+template <int N>
+__device__ __forceinline__ unsigned gt_row_zero_groups(const int* zcol, const GTLane<N>& L) {
+  using G = GT<N, kGtThreads<N>>;
+  static_assert(G::R1 <= 32, "one bit per register");
+  unsigned z = 0;
+  if (zcol) {
+#pragma unroll
+    for (int c = 0; c < G::CPT; ++c)
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) {
+        const bool zero = zcol[RT<N>::col_freq(8 * (G::CPT * k2 + c) + L.j)] != 0;
+        if (__ballot(!zero) == 0ull) z |= 1u << (c * 8 + k2);
+      }
+  }
+  return z;
+}
 
 // The 192 px tile in registers (768 threads, two rounds of 96 lines, 24
 // samples per thread and round: 96 VGPRs).  A phase change moves it through
@@ -2069,8 +2107,10 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
   const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
   float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
   const float2* fin = a.fin ? a.fin + (size_t(b) * MK + mk) * a.fin_stride : nullptr;
-  // rounds whose columns the propagators (the band mask) zero
+  // rounds whose columns the propagators (the band mask) zero, and the row
+  // phase's groups of zero columns
   const unsigned zh = gt_cols_zero<N>(a.zcol, L), zm = zh;
+  const unsigned zrow = gt_row_zero_groups<N>(a.zcol, L);
   // o holds the object registers of the current round; those of the next
   // round (or the next slice's first round) are read while this round's
   // forward row transform runs, and the propagator is read before the column
@@ -2100,7 +2140,8 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) v[n] = cscale(p[8 * n], bcs);
       } else {
-        gt_load_rowspec<N>(v, sl, L, r);
+        // after this launch's propagation into slice s: zero columns unread
+        sl.load_rowspec_z(v, L, r, zrow);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
       }
       if constexpr (!kAhead) {
@@ -2166,10 +2207,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
       const RView<N, float2, G::NT> H((s == 0 && !(TILT && a.tilt_win)) ? a.H0 : a.H + size_t(s) * a.Hstride, tid);
       for (int r = 0; r < G::ROUNDS; ++r) {
         round_fence<decltype(sl)>();
-        if ((zh >> r) & 1u) {
-          gt_zero_col<N>(v, sl, L, r);
-          continue;
-        }
+        if ((zh >> r) & 1u) continue;  // zero columns: neither computed nor stored
         float2 h[G::R1];
 #pragma unroll
         for (int i = 0; i < G::R1; ++i) h[i] = H[r * G::R1 + i];
@@ -2241,6 +2279,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
   // as in the forward: columns zero in the propagators, and in the band mask
   // (where the far field, and so its gradient, is zero)
   const unsigned zh = gt_cols_zero<N>(a.zcol, L);
+  const unsigned zrow = gt_row_zero_groups<N>(a.zcol, L);
   if (!from_state) {
     // far-field adjoint: inverse column transforms of coef * Psi / N^2
     const RView<N, float2, G::NT> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
@@ -2321,7 +2360,12 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
 #pragma unroll
         for (int n = 0; n < G::R1; ++n) v[n] = GS[r * G::R1 + n];
       } else {
-        gt_load_rowspec<N>(v, sl, L, r);
+        // after the far-field adjoint (and an after-tilt) every column was
+        // stored; after a propagation the zero columns were not
+        if (s == a.s_hi - 1)
+          gt_load_rowspec<N>(v, sl, L, r);
+        else
+          sl.load_rowspec_z(v, L, r, zrow);
         rt_line_inv<N, true>(v, X, L.ln, L.j);
       }
       if (!kAhead && a.step_obj) {
@@ -2360,10 +2404,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
       double sy = 0.0, sx = 0.0;
       for (int r = 0; r < G::ROUNDS; ++r) {
         round_fence<decltype(sl)>();
-        if ((zh >> r) & 1u) {
-          gt_zero_col<N>(v, sl, L, r);
-          continue;
-        }
+        if ((zh >> r) & 1u) continue;  // zero columns: neither computed nor stored
         // the propagator is read before the column transform, except at
         // 192 px (168 registers) and where the tilt sums come first
         constexpr bool kHEarly = N != 192;
@@ -2420,7 +2461,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
     // the wave propagated to slice s_lo - 1, as natural rows, for the next segment
     for (int r = 0; r < G::ROUNDS; ++r) {
       round_fence<decltype(sl)>();
-      gt_load_rowspec<N>(v, sl, L, r);
+      sl.load_rowspec_z(v, L, r, zrow);
       rt_line_inv<N, true>(v, X, L.ln, L.j);
 #pragma unroll
       for (int n = 0; n < G::R1; ++n) GS.store(r * G::R1 + n, v[n]);
