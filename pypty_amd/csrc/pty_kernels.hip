@@ -1862,6 +1862,17 @@ struct GSlab {
           v[q] = sl.ld(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8);
       }
   }
+  // row spectra before a column phase that leaves the zero groups alone
+  // (a propagation, the far field): those groups are not stored
+  __device__ __forceinline__ void store_rowspec_z(const float2* v, const GTLane<N>& L, int r, unsigned zg) {
+#pragma unroll
+    for (int c = 0; c < G::CPT; ++c)
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) {
+        const int q = c * 8 + k2;
+        if (!((zg >> q) & 1u)) sl.st(L.vrow, (r * G::LINES * N + 8 * (G::CPT * k2 + c)) * 8, v[q]);
+      }
+  }
 };
 
 // bit c*8 + k2: every column of the group that row-phase register c*8 + k2
@@ -2162,7 +2173,11 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
         for (int n = 0; n < G::R1; ++n) o[n] = O[rn * G::R1 + n];
       }
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
-      gt_store_rowspec<N>(v, sl, L, r);
+      // an after-tilt reads the last slice's row spectra whole
+      if ((TILT && a.tilt_after) && s == a.S - 1)
+        gt_store_rowspec<N>(v, sl, L, r);
+      else
+        sl.store_rowspec_z(v, L, r, zrow);
     }
     if (last_of_recompute) break;
     if ((TILT && a.tilt_after) && s == a.S - 1) {
@@ -2386,7 +2401,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
       }
       if (s > 0) {
         rt_line_fwd<N, true>(v, X, L.ln, L.j);
-        gt_store_rowspec<N>(v, sl, L, r);
+        sl.store_rowspec_z(v, L, r, zrow);  // the propagation skips the zero groups
       } else {
         float2* out = a.gP + (size_t(b) * MK + mk) * N * N + size_t(r * G::LINES + L.ln) * N + L.j;
 #pragma unroll
