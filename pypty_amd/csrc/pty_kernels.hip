@@ -1931,6 +1931,14 @@ struct RegSlab {
 #pragma unroll
     for (int i = 0; i < R1; ++i) d[r][i] = v[i];
   }
+  // A column phase that skips round r (its columns are zero columns of the
+  // band mask) sets it to zero here, so the next row phase reads the zero
+  // groups as zeros straight from the tile (GSlab::load_rowspec_z selects them
+  // instead; a select per register here cost 20 registers).
+  __device__ __forceinline__ void zero_round(int r) {
+#pragma unroll
+    for (int i = 0; i < R1; ++i) d[r][i] = make_float2(0.f, 0.f);
+  }
   __device__ __forceinline__ void to_cols(float2* X, const GTLane<N>& L) {
     float2 t[2][2][H];  // [pass][column round][sample]
 #pragma unroll
@@ -2090,6 +2098,146 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_patch_gt(BatchArgs a) {
   }
 }
 
+// The forward at 192 px with the whole tile in registers (RegSlab, 768
+// threads, 168 registers).  The tile leaves about 70 registers for the
+// transforms and everything else, so the control flow is kept flat: the
+// first slice's row phase (from the probe or a recompute's initial wave, one
+// load path for both) is peeled off the slice loop, the far field follows
+// the loop, a stash store without a slot goes to an empty buffer resource
+// (the hardware drops it), and a recompute segment's last slice runs its
+// row transform like any other.  The object and the propagator are read in
+// chunks of four registers where they are used.  Same results as the slab
+// pipeline (k_forward_gt's general body), which it replaces at 192 px.
+__device__ __forceinline__ void forward_reg192(const BatchArgs& a, float2* X) {
+  constexpr int N = 192, CH = 4;
+  using G = GT<N, kGtThreads<N>>;
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int m = mk / a.K, k = mk % a.K;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  RegSlab sl(nullptr);
+  const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  const unsigned zh = gt_cols_zero<N>(a.zcol, L);
+  float2 v[G::R1];
+  // v holds round r of the wave entering slice s: stash it, times the object
+  auto transmit = [&](int s, int r) {
+    const int slot = a.write_stash ? stash_slot(a, s) : -1;
+    const __amdgpu_buffer_rsrc_t U = __builtin_amdgcn_make_buffer_rsrc(
+        st + size_t(slot >= 0 ? slot : 0) * N * N, 0, slot >= 0 ? N * N * 8 : 0, 0x00020000);
+    const RView<N, float2, G::NT> O(ob + size_t(s) * a.K * N * N, tid);
+    using B = decltype(__builtin_amdgcn_raw_buffer_load_b64(U, 0, 0, 0));
+#pragma unroll
+    for (int n0 = 0; n0 < G::R1; n0 += CH) {
+      __builtin_amdgcn_sched_barrier(0);
+      float2 o[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) o[c] = O[r * G::R1 + n0 + c];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B, v[n0 + c]), U, tid * 8u,
+                                              (r * G::R1 + n0 + c) * G::NT * 8, 0);
+        v[n0 + c] = cmul(v[n0 + c], o[c]);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  // the wave entering slice s_lo: a recompute's initial wave (R-order slots,
+  // element (r*R1 + n)*NT + tid) or the probe (natural rows, element
+  // (r*LINES + ln)*N + j + 8n; the shifted probes carry the beam current)
+  __amdgpu_buffer_rsrc_t W0;
+  unsigned w_voff;
+  int w_round, w_step;
+  float w_scale;
+  if (a.fin) {
+    W0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(a.fin + (size_t(b) * MK + mk) * a.fin_stride), 0,
+                                           N * N * 8, 0x00020000);
+    w_voff = tid * 8u;
+    w_round = G::R1 * G::NT * 8;
+    w_step = G::NT * 8;
+    w_scale = 1.f;
+  } else {
+    const float2* src = a.shift ? a.Pp + (size_t(b) * a.M + m) * N * N : a.probe + size_t(m) * N * N;
+    W0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(src), 0, N * N * 8, 0x00020000);
+    w_voff = unsigned(L.ln * N + L.j) * 8u;
+    w_round = G::LINES * N * 8;
+    w_step = 8 * 8;
+    w_scale = a.shift ? 1.f : bc_scale(a, seq_index(a.seq, a.b0 + b, a.P));
+  }
+#pragma unroll
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<RegSlab>();
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n)
+      v[n] = cscale(__builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(W0, w_voff, r * w_round + n * w_step, 0)),
+                    w_scale);
+    transmit(a.s_lo, r);
+    rt_line_fwd<N, true>(v, X, L.ln, L.j);
+    sl.store_rowspec(v, L, r);
+  }
+  // a recompute segment wants the stash of its slices, not the far field
+  const int s_end = a.fin ? a.s_hi - 1 : a.s_hi;
+  for (int s = a.s_lo; s < a.s_hi - 1; ++s) {
+    sl.to_cols(X, L);
+    // propagation into slice s + 1 (H0 = H^e for the first)
+    const RView<N, float2, G::NT> H(s == 0 ? a.H0 : a.H + size_t(s) * a.Hstride, tid);
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<RegSlab>();
+      if ((zh >> r) & 1u) {
+        sl.zero_round(r);
+        continue;
+      }
+      sl.load_col(v, L, r);
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+      for (int i0 = 0; i0 < G::R1; i0 += CH) {
+        __builtin_amdgcn_sched_barrier(0);
+        float2 h[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) h[c] = H[r * G::R1 + i0 + c];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) v[i0 + c] = cmul(v[i0 + c], h[c]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      rt_line_inv<N, true>(v, X, L.ln, L.j);
+      sl.store_col(v, L, r);
+    }
+    sl.to_rows(X, L);
+    // row phase of slice s + 1
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<RegSlab>();
+      sl.load_rowspec(v, L, r);
+      rt_line_inv<N, true>(v, X, L.ln, L.j);
+      transmit(s + 1, r);
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+      sl.store_rowspec(v, L, r);
+    }
+  }
+  if (s_end < a.s_hi) return;
+  // far field: forward column transform times the band mask
+  sl.to_cols(X, L);
+  const RView<N, float2, G::NT> out(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
+  const RView<N, float, G::NT> mask(a.maskc, tid);
+#pragma unroll
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<RegSlab>();
+    if ((zh >> r) & 1u) {
+#pragma unroll
+      for (int i = 0; i < G::R1; ++i) out.store(r * G::R1 + i, make_float2(0.f, 0.f));
+      continue;
+    }
+    sl.load_col(v, L, r);
+    rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+    for (int i = 0; i < G::R1; ++i) out.store(r * G::R1 + i, cscale(v[i], mask[r * G::R1 + i]));
+  }
+}
+
 // k_forward for 192/256 px windows (multislice_core.py:9-82, objective.py:501-512)
 template <int N, bool TILT>
 __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
@@ -2097,6 +2245,10 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
   rt_tw_load<N>();
   __shared__ float2 X[G::LINES * G::LS];
   __shared__ float2 tt[TILT ? 2 * N : 1];  // tilt factors or grids
+  if constexpr (N == 192 && !TILT) {
+    forward_reg192(a, X);
+    return;
+  }
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
