@@ -2419,6 +2419,156 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
   }
 }
 
+// The adjoint at 192 px with the whole tile in registers (RegSlab), laid out
+// like forward_reg192: flat control flow around the 168 registers; for
+// segments ending above slice 0 (s_hi > 1; k_adjoint_gt runs the others on
+// the slab pipeline).  The row phases come in three compile-time forms (from the tile or from a later
+// segment's wave; ending in a forward row transform or, at slice 0, in the
+// probe-side gradient), chosen outside the round loops; the stash is read
+// and the window gradient written in chunks of four registers, through
+// buffer resources that are empty when the object is not refined (the
+// hardware returns zeros for such reads and drops such writes).  Same
+// results as the slab pipeline (k_adjoint_gt's general body).
+__device__ __forceinline__ void adjoint_reg192(const BatchArgs& a, float2* X) {
+  constexpr int N = 192, CH = 4;
+  using G = GT<N, kGtThreads<N>>;
+  const int MK = a.M * a.K;
+  int b, mk;
+  xcd_decode(blockIdx.x, MK, b, mk);
+  if (b >= a.B) return;
+  const int k = mk % a.K;
+  const unsigned tid = threadIdx.x;
+  const GTLane<N> L;
+  RegSlab sl(nullptr);
+  const unsigned zh = gt_cols_zero<N>(a.zcol, L);
+  // a later segment left the wave of slice s_hi - 1 (R-order)
+  const bool from_state = a.s_hi < a.S;
+  const RView<N, float2, G::NT> GS(a.gstate ? a.gstate + (size_t(b) * MK + mk) * N * N : a.Psi, tid);
+  float2* st = a.stash + (size_t(b) * MK + mk) * a.st_slots * N * N;
+  const float2* ob = a.Ob + (size_t(b) * a.S * a.K + k) * N * N;
+  float2 v[G::R1];
+  // row phase of slice s: v = the gradient wave at slice s (natural rows);
+  // the stash slot becomes this mode's window gradient g conj(u_s), then
+  // g conj(O_s) goes on through the forward row transform into the tile, or
+  // at slice 0 out as the probe-side gradient (natural order)
+  auto row_phase = [&](int s, auto from_gs, auto last) {
+    const __amdgpu_buffer_rsrc_t U = __builtin_amdgcn_make_buffer_rsrc(st + size_t(s - a.s_lo) * N * N, 0,
+                                                                       a.step_obj ? N * N * 8 : 0, 0x00020000);
+    using B64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(U, 0, 0, 0));
+    const RView<N, float2, G::NT> O(ob + size_t(s) * a.K * N * N, tid);
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<RegSlab>();
+      if constexpr (decltype(from_gs)::value) {
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) v[n] = GS[r * G::R1 + n];
+      } else {
+        sl.load_rowspec(v, L, r);
+        rt_line_inv<N, true>(v, X, L.ln, L.j);
+      }
+#pragma unroll
+      for (int n0 = 0; n0 < G::R1; n0 += CH) {
+        __builtin_amdgcn_sched_barrier(0);
+        float2 pu[CH], o[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const int so = (r * G::R1 + n0 + c) * G::NT * 8;
+          pu[c] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(U, tid * 8u, so, 0));
+          o[c] = O[r * G::R1 + n0 + c];
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const int so = (r * G::R1 + n0 + c) * G::NT * 8;
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B64, cmulc(v[n0 + c], pu[c])), U, tid * 8u, so, 0);
+          v[n0 + c] = cmulc(v[n0 + c], o[c]);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (decltype(last)::value) {
+        float2* out = a.gP + (size_t(b) * MK + mk) * N * N + size_t(r * G::LINES + L.ln) * N + L.j;
+#pragma unroll
+        for (int n = 0; n < G::R1; ++n) out[8 * n] = v[n];
+      } else {
+        rt_line_fwd<N, true>(v, X, L.ln, L.j);
+        sl.store_rowspec(v, L, r);
+      }
+    }
+  };
+  // column phase back through the propagation into slice s (s > 0; H0 for s = 1)
+  auto propagate_back = [&](int s) {
+    sl.to_cols(X, L);
+    const RView<N, float2, G::NT> H(s == 1 ? a.H0 : a.H + size_t(s - 1) * a.Hstride, tid);
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<RegSlab>();
+      if ((zh >> r) & 1u) {
+        sl.zero_round(r);
+        continue;
+      }
+      sl.load_col(v, L, r);
+      rt_line_fwd<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+      for (int i0 = 0; i0 < G::R1; i0 += CH) {
+        __builtin_amdgcn_sched_barrier(0);
+        float2 h[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) h[c] = H[r * G::R1 + i0 + c];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) v[i0 + c] = cmulc(v[i0 + c], h[c]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      rt_line_inv<N, true>(v, X, L.ln, L.j);
+      sl.store_col(v, L, r);
+    }
+    sl.to_rows(X, L);
+  };
+  const int s_top = a.s_hi - 1;
+  if (!from_state) {
+    // far-field adjoint: inverse column transforms of coef * Psi / N^2
+    const RView<N, float2, G::NT> psi(a.Psi + (size_t(b) * MK + mk) * N * N, tid);
+    const RView<N, float, G::NT> c(a.coef + size_t(b) * N * N, tid);
+    const float sc = 1.0f / float(N * N);
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+      round_fence<RegSlab>();
+      if ((zh >> r) & 1u) {
+        sl.zero_round(r);
+        continue;
+      }
+#pragma unroll
+      for (int i = 0; i < G::R1; ++i) v[i] = cscale(psi[r * G::R1 + i], c[r * G::R1 + i] * sc);
+      rt_line_inv<N, true>(v, X, L.ln, L.j);
+      sl.store_col(v, L, r);
+    }
+    sl.to_rows(X, L);
+  }
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  if (from_state)
+    row_phase(s_top, T_{}, F_{});
+  else
+    row_phase(s_top, F_{}, F_{});
+  propagate_back(s_top);
+  const int s_stop = a.s_lo > 1 ? a.s_lo : 1;
+  for (int s = s_top - 1; s >= s_stop; --s) {
+    row_phase(s, F_{}, F_{});
+    propagate_back(s);
+  }
+  if (a.s_lo == 0) {
+    row_phase(0, F_{}, T_{});
+    return;
+  }
+  // the wave propagated to slice s_lo - 1 (R-order) for the next segment
+#pragma unroll
+  for (int r = 0; r < G::ROUNDS; ++r) {
+    round_fence<RegSlab>();
+    sl.load_rowspec(v, L, r);
+    rt_line_inv<N, true>(v, X, L.ln, L.j);
+#pragma unroll
+    for (int n = 0; n < G::R1; ++n) GS.store(r * G::R1 + n, v[n]);
+  }
+}
+
 // k_adjoint for 192/256 px windows (objective.py:595-597, multislice_core.py:85-200)
 template <int N, bool TILT>
 __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
@@ -2427,6 +2577,15 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_adjoint_gt(BatchArgs a) {
   __shared__ float2 X[G::LINES * G::LS];
   __shared__ float2 tt[TILT ? 2 * N : 1];  // tilt factors or grids
   __shared__ double red[TILT ? 16 : 1];
+  if constexpr (N == 192 && !TILT) {
+    // a first row phase that is also the last (slice 0 alone) takes the
+    // slab pipeline below: as a further path beside the register tile's it
+    // cost 185-204 spilled registers
+    if (a.s_hi > 1) {
+      adjoint_reg192(a, X);
+      return;
+    }
+  }
   const int MK = a.M * a.K;
   int b, mk;
   xcd_decode(blockIdx.x, MK, b, mk);
