@@ -2,8 +2,9 @@
 the adjoint gives the loss and every gradient of the pass that keeps all slices, bit for bit (the
 recompute runs the same instructions on the same stored values, and the wave crossing a segment
 boundary is stored exactly).  Covers the register tile with four threads per line (32, 96 px) and
-with eight (64, 128 px), the global-slab tile (192, 256 px), uneven last segments, object modes,
-several batches per pass and the degenerate k = 1."""
+with eight (64, 128 px), the slab tiles (192 px in registers, 256 px in workspace memory), uneven
+last segments, object modes, several batches per pass and the degenerate k = 1 (at 192 px its
+one-slice segment at slice 0 runs the slab pipeline, the others the register tile)."""
 import numpy as np
 import pytest
 
@@ -18,6 +19,7 @@ CASES = [
     (96, 6, 1, 2, 6, 4, 4),
     (128, 7, 2, 2, 5, 3, 3),
     (192, 5, 1, 2, 4, 4, 2),
+    (192, 3, 1, 2, 3, 3, 1),
     (256, 4, 1, 1, 3, 2, 3),
 ]
 
