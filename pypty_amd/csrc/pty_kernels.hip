@@ -2109,7 +2109,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_patch_gt(BatchArgs a) {
 // chunks of four registers where they are used.  Same results as the slab
 // pipeline (k_forward_gt's general body), which it replaces at 192 px.
 __device__ __forceinline__ void forward_reg192(const BatchArgs& a, float2* X) {
-  constexpr int N = 192, CH = 4;
+  constexpr int N = 192, CH = 12, CHC = 12;
   using G = GT<N, kGtThreads<N>>;
   const int MK = a.M * a.K;
   int b, mk;
@@ -2194,13 +2194,13 @@ __device__ __forceinline__ void forward_reg192(const BatchArgs& a, float2* X) {
       sl.load_col(v, L, r);
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
-      for (int i0 = 0; i0 < G::R1; i0 += CH) {
+      for (int i0 = 0; i0 < G::R1; i0 += CHC) {
         __builtin_amdgcn_sched_barrier(0);
-        float2 h[CH];
+        float2 h[CHC];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) h[c] = H[r * G::R1 + i0 + c];
+        for (int c = 0; c < CHC; ++c) h[c] = H[r * G::R1 + i0 + c];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) v[i0 + c] = cmul(v[i0 + c], h[c]);
+        for (int c = 0; c < CHC; ++c) v[i0 + c] = cmul(v[i0 + c], h[c]);
       }
       __builtin_amdgcn_sched_barrier(0);
       rt_line_inv<N, true>(v, X, L.ln, L.j);
@@ -2430,7 +2430,7 @@ __global__ __launch_bounds__(kGtThreads<N>) void k_forward_gt(BatchArgs a) {
 // hardware returns zeros for such reads and drops such writes).  Same
 // results as the slab pipeline (k_adjoint_gt's general body).
 __device__ __forceinline__ void adjoint_reg192(const BatchArgs& a, float2* X) {
-  constexpr int N = 192, CH = 4;
+  constexpr int N = 192, CH = 8, CHC = 8;
   using G = GT<N, kGtThreads<N>>;
   const int MK = a.M * a.K;
   int b, mk;
@@ -2508,13 +2508,13 @@ __device__ __forceinline__ void adjoint_reg192(const BatchArgs& a, float2* X) {
       sl.load_col(v, L, r);
       rt_line_fwd<N, true>(v, X, L.ln, L.j);
 #pragma unroll
-      for (int i0 = 0; i0 < G::R1; i0 += CH) {
+      for (int i0 = 0; i0 < G::R1; i0 += CHC) {
         __builtin_amdgcn_sched_barrier(0);
-        float2 h[CH];
+        float2 h[CHC];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) h[c] = H[r * G::R1 + i0 + c];
+        for (int c = 0; c < CHC; ++c) h[c] = H[r * G::R1 + i0 + c];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) v[i0 + c] = cmulc(v[i0 + c], h[c]);
+        for (int c = 0; c < CHC; ++c) v[i0 + c] = cmulc(v[i0 + c], h[c]);
       }
       __builtin_amdgcn_sched_barrier(0);
       rt_line_inv<N, true>(v, X, L.ln, L.j);
