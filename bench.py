@@ -237,6 +237,96 @@ def warm_start_benchmark(dev, pairs, train_epochs, rec_epochs, step0=1e-3):
     return out
 
 
+def thick_recon_benchmark(dev, runs, pairs, train_epochs, rec_epochs, step0=1e-3, n_scan=128):
+    """L-BFGS reconstruction of the thick workload at its full size (1536x1536x40 object, 256 px probe
+    with 10 modes, 128x128 scan, 512 windows per launch, slice checkpoints every 8): `rec_epochs`
+    epochs refining the object, and the probe and positions from epoch 5, with the first line-search
+    step `step0` in epoch 0 and 1 afterwards, from the flat start ('plain') and from the learned start
+    ('warm': a network trained on `pairs` patterns of random 40-slice objects simulated at 256 px, its
+    predictions stitched into the starting object; the build's wall-clock counts).  Each run prints
+    a progress line per epoch; the result holds each run's (seconds, NMSE) curve."""
+    import numpy as np
+    import torch
+
+    from pypty_amd import iterative, metrics, synth, warmstart
+
+    n, patch, batch = 256, 48, 256
+    t0 = time.perf_counter()
+    ds = synth.config4(dev, n_scan=n_scan)
+    t_data = time.perf_counter() - t0
+    ny, nx, S, K = ds.obj.shape
+    pos = ds.positions.astype(np.float64) + ds.corrections
+    sh = np.round(pos.min(axis=0)).astype(int)
+    truth = metrics.projected_phase(ds.obj)
+    tr = truth[sh[0]:, sh[1]:]
+    mask = metrics.fov_mask(tr.shape, pos - pos.min(axis=0), n)
+    base = dict(dataset=ds.measured, probe=ds.probe, positions=pos, acc_voltage=300.0, pixel_size_x_A=ds.pixel_A,
+                pixel_size_y_A=ds.pixel_A, slice_distances=np.array([ds.dz_A]), num_slices=S, algorithm="lsq_sqrt",
+                update_obj=1, update_probe=lambda e: int(e >= 5), update_probe_pos=lambda e: int(e >= 5),
+                compute_batch=512, checkpoint_slices=8, output_folder="", print_flag=0,
+                update_step_bfgs=lambda e: step0 if e == 0 else 1.0)
+    print(f"[thick] data {t_data:.1f} s; untimed one-epoch run", file=sys.stderr, flush=True)
+    iterative.run(dict(base, obj=np.ones_like(ds.obj), epoch_max=1))
+
+    def recon(name, o0, t_extra, data=None):
+        curve, spent = [], [0.0]
+
+        def cb(ep, obj, probe):
+            torch.cuda.synchronize()
+            c0 = time.perf_counter()
+            rec = metrics.projected_phase(obj)[:tr.shape[0], :tr.shape[1]]
+            curve.append((c0 - T0 - spent[0] + t_extra, metrics.nmse(rec, tr, mask)))
+            print(f"[thick] {name} epoch {ep} {curve[-1][0]:.1f} s NMSE {curve[-1][1]:.4f}", file=sys.stderr,
+                  flush=True)
+            spent[0] += time.perf_counter() - c0
+
+        np.random.seed(0)
+        torch.cuda.synchronize()
+        T0 = time.perf_counter()
+        iterative.run(dict(base, obj=o0, epoch_max=rec_epochs, epoch_callback=cb,
+                           **({} if data is None else {"dataset": data})))
+        return [[round(t, 2), round(e, 4)] for t, e in curve]
+
+    out = {"workload": f"thick object 1536x1536x40 dz1A, probe 256px x10 modes, {n_scan}x{n_scan} scan "
+                       f"({n_scan * n_scan} positions), "
+                       "lsq_sqrt, 512 windows per launch, slice checkpoints every 8; object refined from epoch 0, "
+                       f"probe and positions from epoch 5; first L-BFGS step {step0} in epoch 0 then 1",
+           "recon_epochs": rec_epochs,
+           "start_nmse_flat": round(metrics.nmse(np.zeros_like(truth), truth, metrics.fov_mask(truth.shape, pos, n)), 4)}
+    if runs in ("plain", "both"):
+        out["plain"] = recon("plain", np.ones_like(ds.obj), 0.0)
+    if runs in ("warm", "both"):
+        t0 = time.perf_counter()
+        x, t = warmstart.simulate_pairs(pairs, n=n, patch=patch, n_slices=S, pixel_A=ds.pixel_A, dz_A=ds.dz_A,
+                                        device=dev, seed=4500)
+        torch.cuda.synchronize()
+        t_sim = time.perf_counter() - t0
+        net = warmstart.WarmStartNet(n, patch, dev, seed=0)
+        t0 = time.perf_counter()
+        hist = net.train(x, t, epochs=train_epochs, batch=batch, lr=1e-3, seed=1, lr_end=1e-4)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        del x, t
+        print(f"[thick] simulated {pairs} pairs in {t_sim:.1f} s, trained in {t_train:.1f} s", file=sys.stderr,
+              flush=True)
+        warmstart.warm_start_object(net, ds.measured[:256], ds.positions[:256], ds.obj.shape)  # loads kernels
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        meas_dev = torch.from_numpy(ds.measured).to(dev)
+        warm_obj = warmstart.warm_start_object(net, meas_dev, ds.positions, ds.obj.shape)
+        torch.cuda.synchronize()
+        t_build = time.perf_counter() - t0
+        o = np.ones_like(ds.obj)
+        o[:ny - sh[0], :nx - sh[1]] = warm_obj[sh[0]:, sh[1]:]  # run() moves the scan to the origin
+        out.update({"train_pairs": pairs, "train_epochs": train_epochs, "sim_s": round(t_sim, 2),
+                    "train_s": round(t_train, 2), "train_loss": [round(hist[0], 5), round(hist[-1], 5)],
+                    "warm_build_s": round(t_build, 3),
+                    "start_nmse_warm": round(metrics.nmse(metrics.projected_phase(warm_obj), truth,
+                                                          metrics.fov_mask(truth.shape, pos, n)), 4)})
+        out["warm"] = recon("warm", o, t_build, meas_dev)
+    return out
+
+
 def recon_benchmark(dev, epochs, cpu_epochs):
     """Plain reconstruction of the single-slice workload: run() on the GPU, the oracle's bfgs_epoch on the CPU.
 
@@ -502,6 +592,12 @@ def main(argv=None):
                     help="training pairs of the learned warm-start measurement (0: skip)")
     ap.add_argument("--warm-start-epochs", type=int, default=40)
     ap.add_argument("--warm-recon-epochs", type=int, default=20)
+    ap.add_argument("--thick-recon", choices=("plain", "warm", "both"), default=None,
+                    help="instead of the throughput line: the thick workload's L-BFGS reconstruction at full size "
+                         "from the flat start, the learned start or both (one JSON line with their NMSE curves)")
+    ap.add_argument("--thick-recon-epochs", type=int, default=50)
+    ap.add_argument("--thick-warm-pairs", type=int, default=20000)
+    ap.add_argument("--thick-warm-epochs", type=int, default=20)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the pass is measured on one GPU only (the multi-GPU split is a design, not built); "
@@ -511,6 +607,11 @@ def main(argv=None):
     import torch
 
     dev = torch.device("cuda", 0)
+    if args.thick_recon:
+        res = thick_recon_benchmark(dev, args.thick_recon, args.thick_warm_pairs, args.thick_warm_epochs,
+                                    args.thick_recon_epochs, n_scan=args.thick_scan)
+        print(json.dumps({"thick_recon": res}), flush=True)
+        return 0
     datasets, step, head = measure(args.config, dev, args, kernel_timing=not args.no_kernel_timing)
     print(f"[bench] {args.config}: {head['value']:.1f} patterns/s, median {head['ms_per_step']:.2f} ms per step",
           file=sys.stderr, flush=True)
