@@ -241,7 +241,9 @@ def thick_recon_benchmark(dev, runs, pairs, train_epochs, rec_epochs, step0=1e-3
     """L-BFGS reconstruction of the thick workload at its full size (1536x1536x40 object, 256 px probe
     with 10 modes, 128x128 scan, 512 windows per launch, slice checkpoints every 8): `rec_epochs`
     epochs refining the object, and the probe and positions from epoch 5, with the first line-search
-    step `step0` in epoch 0 and 1 afterwards, from the flat start ('plain') and from the learned start
+    step `step0` in the epochs that start a fresh L-BFGS history (0, and 5 where the probe and the
+    positions join; a first step of 1 there took a trial probe to a NaN loss) and 1 in the others,
+    from the flat start ('plain') and from the learned start
     ('warm': a network trained on `pairs` patterns of random 40-slice objects simulated at 256 px, its
     predictions stitched into the starting object; the build's wall-clock counts).  Each run prints
     a progress line per epoch; the result holds each run's (seconds, NMSE) curve."""
@@ -264,7 +266,7 @@ def thick_recon_benchmark(dev, runs, pairs, train_epochs, rec_epochs, step0=1e-3
                 pixel_size_y_A=ds.pixel_A, slice_distances=np.array([ds.dz_A]), num_slices=S, algorithm="lsq_sqrt",
                 update_obj=1, update_probe=lambda e: int(e >= 5), update_probe_pos=lambda e: int(e >= 5),
                 compute_batch=512, checkpoint_slices=8, output_folder="", print_flag=0,
-                update_step_bfgs=lambda e: step0 if e == 0 else 1.0)
+                update_step_bfgs=lambda e: step0 if e in (0, 5) else 1.0)
     print(f"[thick] data {t_data:.1f} s; untimed one-epoch run", file=sys.stderr, flush=True)
     iterative.run(dict(base, obj=np.ones_like(ds.obj), epoch_max=1))
 
@@ -290,7 +292,7 @@ def thick_recon_benchmark(dev, runs, pairs, train_epochs, rec_epochs, step0=1e-3
     out = {"workload": f"thick object 1536x1536x40 dz1A, probe 256px x10 modes, {n_scan}x{n_scan} scan "
                        f"({n_scan * n_scan} positions), "
                        "lsq_sqrt, 512 windows per launch, slice checkpoints every 8; object refined from epoch 0, "
-                       f"probe and positions from epoch 5; first L-BFGS step {step0} in epoch 0 then 1",
+                       f"probe and positions from epoch 5; first L-BFGS step {step0} in epochs 0 and 5, 1 in the others",
            "recon_epochs": rec_epochs,
            "start_nmse_flat": round(metrics.nmse(np.zeros_like(truth), truth, metrics.fov_mask(truth.shape, pos, n)), 4)}
     if runs in ("plain", "both"):
