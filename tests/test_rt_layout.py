@@ -232,3 +232,68 @@ def test_register_slab_192_phase_changes():
         return w
 
     assert worst(ln * LS + j) == 1 and worst(j * LS + ln) <= 2
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_register_slab_192_zero_rounds_are_the_zero_groups(seed):
+    """The 192 px register tile skips the band mask's zero columns (forward_reg192, adjoint_reg192):
+    a column-phase wavefront whose columns are all marked in zcol sets its round to zero in
+    registers (RegSlab::zero_round) instead of transforming it, and the next row phase reads the
+    tile as is.  After to_rows the zeros must sit exactly in the row-phase registers whose group of
+    eight columns gt_row_zero_groups marks, the groups the slab pipeline zeroes on load, for a
+    disc-shaped band (the kernels' case) and for arbitrary marks."""
+    N, NT, R1, CPT, LINES, LS, H, T = 192, 768, 24, 3, 96, 200, 12, 8
+    tids = np.arange(NT)
+    ln = 8 * (tids >> 6) + ((tids & 63) >> 3)
+    j = tids & 7
+    wave = tids >> 6
+
+    def col_freq(col):  # RT<192>::col_freq
+        g = col // T
+        return CPT * (col % T) + (g % CPT) + R1 * (g // CPT)
+
+    rng = np.random.default_rng(seed)
+    f = np.fft.fftfreq(N)
+    zsets = [(np.abs(f) > (1.0 / 3.0)).astype(int), (rng.random(N) < 0.5).astype(int)]
+    # whole groups marked, so that some wavefronts skip
+    g = rng.integers(0, N // 8, 6)
+    z = np.zeros(N, int)
+    for q in g:
+        for c in range(8 * q, 8 * q + 8):
+            z[col_freq(c)] = 1
+    zsets.append(z)
+    for zcol in zsets:
+        # column phase: register n of round q = tile[j + 8n, q*96 + ln]; nonzero values everywhere
+        dc = np.zeros((NT, 2, R1), np.int64)
+        for q in range(2):
+            for n in range(R1):
+                dc[:, q, n] = 1 + (j + 8 * n) * N + q * LINES + ln
+        for q in range(2):
+            marked = zcol[col_freq(q * LINES + ln)] != 0
+            for w in range(NT // 64):
+                lanes = wave == w
+                if marked[lanes].all():  # wave_cols_zero: the ballot of unmarked columns is empty
+                    dc[lanes, q, :] = 0
+        # to_rows (index maps as in the layout test above)
+        X = np.zeros(LINES * LS, np.int64)
+        back = np.zeros_like(dc)
+        for p in range(2):
+            X[:] = -1
+            for q in range(2):
+                for n in range(H):
+                    X[j * LS + ln + 8 * n * LS + q * LINES] = dc[:, q, n + H * p]
+            for c in range(CPT):
+                for k2 in range(8):
+                    back[:, p, c * 8 + k2] = X[ln * LS + j + 8 * (CPT * k2 + c)]
+        assert (back >= 0).all()
+        # gt_row_zero_groups: bit c*8 + k2 when every column 8*(CPT*k2 + c) + j' (j' < 8) is marked
+        for c in range(CPT):
+            for k2 in range(8):
+                grp = CPT * k2 + c
+                zero_group = all(zcol[col_freq(8 * grp + jj)] != 0 for jj in range(8))
+                for p in range(2):
+                    vals = back[:, p, c * 8 + k2]
+                    if zero_group:
+                        assert (vals == 0).all(), (c, k2, p)
+                    else:
+                        assert (vals != 0).all(), (c, k2, p)
