@@ -6,7 +6,7 @@ dataset of the workload, followed by the readback of every dataset's loss and
 sse.  patterns/s = positions / median per-step wall time (after --warmup
 untimed steps).  The default workload is the ragged one: eight seeded datasets
 (windows 96/128/192 px, 4-16 slices, 1-5 probe modes, 39,320 positions), each
-with autotuned windows per launch, spread over two streams and replayed as one
+with autotuned windows per launch, spread over three streams and replayed as one
 captured HIP graph per step.  --also adds further workloads to the same line
 (default: the multislice configuration, object 512x512x10, 128 px probe with
 5 modes, 4096 positions, four batch lanes); --config thick is the
@@ -413,7 +413,7 @@ def recon_benchmark(dev, epochs, cpu_epochs):
 WORKLOADS = {
     "ragged": "8 seeded datasets (seeds 2000-2007), windows {96,128,192} px, 4-16 slices, 1-5 probe modes, "
               "scans 32x32-96x96 (39,320 positions), dose 1e3-1e5, one full pass over every dataset per step, "
-              "autotuned windows per launch, HIP-graph replay over two streams",
+              "autotuned windows per launch, HIP-graph replay over --streams streams (default 3)",
     "multislice": "multislice object 512x512x10 dz2A, probe 128px x5 modes, 64x64 scan (4096 positions), "
                   "batch 256, lsq_sqrt, object+probe+position gradients",
     "thick": "thick object 1536x1536x40 dz1A, probe 256px x10 modes, 128x128 scan (16384 positions), batch 512, "
@@ -431,7 +431,7 @@ def build_workload(name, dev, args):
     t0 = time.perf_counter()
     if name == "ragged":
         datasets = synth.ragged_datasets(dev)
-        kw = dict(batch=256, streams=2, lanes=1, checkpoint=0)
+        kw = dict(batch=256, streams=args.streams, lanes=1, checkpoint=0)
         tune, graph = True, True
     elif name == "multislice":
         datasets = [synth.config1(dev, n_scan=args.scan)]
@@ -583,6 +583,8 @@ def main(argv=None):
     ap.add_argument("--batches", default="",
                     help="comma-separated windows per launch of each dataset of the headline workload "
                          "(instead of the autotuner)")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="streams the ragged workload's datasets are spread over (round robin) inside the graph")
     ap.add_argument("--eager", action="store_true",
                     help="launch the headline steps eagerly instead of replaying a captured HIP graph "
                          "(for counter profiles)")
